@@ -1,0 +1,1195 @@
+// e4ref.cpp -- CPU FP64 restatement of Eilmer4's explicit structured-block
+// gas-dynamic update.  TEST INFRASTRUCTURE ONLY: this file is the oracle that
+// the HIP path is checked against.  Only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg may load it, and only as the checker or as the
+// timed CPU baseline.  The product library never links or calls it.
+//
+// Every function names the reference file:line it restates (paths relative to
+// the reference's src/ directory).  Build: oracle/Makefile
+// (g++ -O2 -ffp-contract=off, so that no FMA contraction changes roundings).
+//
+// Scope: uniform Cartesian structured blocks (dx, dy, dz per block), 2D or 3D,
+// two ghost-cell layers; ideal gas and thermally perfect mixtures (NASA-9 / CEA
+// curves); MUSCL l2r2 reconstruction with van Albada limiting and extrema
+// clipping in the face-local frame; AUSMDV and HLLC fluxes; ghost-cell BCs
+// (flowstate copy, extrapolate copy, internal copy then reflect, periodic
+// full-face copy); time derivatives, the RK stage updates, decode, invalid-cell
+// counting and the CFL signal.
+
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+extern "C" {
+
+enum { NSP_MAX = 8, NSEG_MAX = 4, NQ_MAX = 5 + NSP_MAX };
+enum { GAS_IDEAL = 0, GAS_TP = 1 };
+// FluxCalculator values follow the reference's enum order (eilmer/fvcore.d:184-194);
+// hllc is the new member appended after roe.
+enum { FLUX_AUSMDV = 0, FLUX_HLLC = 9 };
+// GasdynamicUpdate (eilmer/fvcore.d:30-41).
+enum { UPD_EULER = 0, UPD_PC = 1, UPD_MIDPOINT = 2, UPD_CLASSIC_RK3 = 3, UPD_TVD_RK3 = 4, UPD_DENMAN_RK3 = 5 };
+// InterpolateOption (eilmer/fvcore.d:155).
+enum { INTERP_PT = 0, INTERP_RHOU = 1, INTERP_RHOP = 2, INTERP_RHOT = 3 };
+// Boundary kinds (ghost-cell effects, eilmer/bc/ghost_cell_effect/*).
+enum { BC_SLIP_WALL = 0, BC_INFLOW = 1, BC_EXTRAPOLATE = 2, BC_PERIODIC = 3 };
+// Face order of a cell (eilmer/fvcore.d Face enum): north, east, south, west, top, bottom.
+enum { F_NORTH = 0, F_EAST = 1, F_SOUTH = 2, F_WEST = 3, F_TOP = 4, F_BOTTOM = 5 };
+
+static const double R_universal = 8.31451;     // gas/physical_constants.d:13
+static const double T_MIN = 20.0;              // gas/gas_model.d:45
+static const double T_MAX = 100000.0;          // gas/gas_model.d:46
+static const double epsilon_van_albada = 1.0e-12; // eilmer/onedinterp.d:22
+
+struct CEACurve {          // gas/thermo/cea_thermo_curves.d:22-169
+    double R;
+    int nseg;
+    double Tb[NSEG_MAX + 1];
+    double Tbl[NSEG_MAX];
+    double c[NSEG_MAX][9];
+};
+
+struct Gas {
+    int type;
+    int nsp;
+    // ideal gas (gas/ideal_gas.d:31-72)
+    double Rgas, gamma, Cv, Cvinv, Cp;
+    // thermally perfect mix (gas/therm_perf_gas.d:41-110)
+    double R[NSP_MAX];
+    CEACurve curve[NSP_MAX];
+    long newton_f_evals;   // instrumentation: zeroFn evaluations
+};
+
+struct GasState { double rho, p, T, u, a; double massf[NSP_MAX]; };
+struct FlowState { GasState gas; double vel[3]; };
+
+// ---------------------------------------------------------------- gas models
+
+Gas* ref_gas_new_ideal(double mMass, double gamma)
+{
+    Gas* g = (Gas*)calloc(1, sizeof(Gas));
+    g->type = GAS_IDEAL; g->nsp = 1;
+    g->Rgas = R_universal / mMass;                // ideal_gas.d:65
+    g->gamma = gamma;
+    g->Cv = g->Rgas / (gamma - 1.0);              // ideal_gas.d:66
+    g->Cvinv = 1.0 / g->Cv;                       // ideal_gas.d:67
+    g->Cp = g->Rgas * gamma / (gamma - 1.0);      // ideal_gas.d:68
+    return g;
+}
+
+// coeffs: [nsp][nseg][9]; Tbreaks: [nsp][nseg+1]; Tblends: [nsp][nseg-1]
+Gas* ref_gas_new_tp(int nsp, const double* M, int nseg, const double* Tbreaks,
+                    const double* Tblends, const double* coeffs)
+{
+    if (nsp < 1 || nsp > NSP_MAX || nseg < 1 || nseg > NSEG_MAX) return nullptr;
+    Gas* g = (Gas*)calloc(1, sizeof(Gas));
+    g->type = GAS_TP; g->nsp = nsp;
+    for (int s = 0; s < nsp; ++s) {
+        g->R[s] = R_universal / M[s];             // therm_perf_gas.d:68
+        CEACurve& cv = g->curve[s];
+        cv.R = g->R[s]; cv.nseg = nseg;
+        for (int b = 0; b <= nseg; ++b) cv.Tb[b] = Tbreaks[s * (nseg + 1) + b];
+        for (int b = 0; b < nseg - 1; ++b) cv.Tbl[b] = Tblends[s * (nseg - 1) + b];
+        for (int q = 0; q < nseg; ++q)
+            for (int j = 0; j < 9; ++j) cv.c[q][j] = coeffs[(s * nseg + q) * 9 + j];
+    }
+    return g;
+}
+
+void ref_gas_free(Gas* g) { free(g); }
+long ref_gas_newton_evals(Gas* g) { long n = g->newton_f_evals; g->newton_f_evals = 0; return n; }
+
+// cea_thermo_curves.d:136-169 (returns 0 on success)
+static int determineCoefficients(const CEACurve& cv, double T, double a[9])
+{
+    const int nb = cv.nseg + 1;
+    if (T < (cv.Tb[1] - 0.5 * cv.Tbl[0])) { for (int j = 0; j < 9; ++j) a[j] = cv.c[0][j]; return 0; }
+    if (T > (cv.Tb[nb - 2] + 0.5 * cv.Tbl[nb - 3])) { for (int j = 0; j < 9; ++j) a[j] = cv.c[cv.nseg - 1][j]; return 0; }
+    for (int i = 1; i < nb - 1; ++i) {
+        double T_blend_low = cv.Tb[i] - 0.5 * cv.Tbl[i - 1];
+        double T_blend_high = cv.Tb[i] + 0.5 * cv.Tbl[i - 1];
+        if (T >= T_blend_low && T <= T_blend_high) {
+            double wB = (1. / cv.Tbl[i - 1]) * (T - T_blend_low);
+            double wA = 1.0 - wB;
+            for (int j = 0; j < 9; ++j) a[j] = wA * cv.c[i - 1][j] + wB * cv.c[i][j];
+            return 0;
+        }
+        if (T > T_blend_high && T < (cv.Tb[i + 1] - 0.5 * cv.Tbl[i])) {
+            for (int j = 0; j < 9; ++j) a[j] = cv.c[i][j];
+            return 0;
+        }
+    }
+    return -1;
+}
+
+// cea_thermo_curves.d:48-62
+static double eval_Cp(const CEACurve& cv, double T)
+{
+    if (T < cv.Tb[0]) T = cv.Tb[0];
+    if (T > cv.Tb[cv.nseg]) T = cv.Tb[cv.nseg];
+    double a[9];
+    determineCoefficients(cv, T, a);
+    double Cp_on_R = a[0] / (T * T) + a[1] / T + a[2] + a[3] * T;
+    Cp_on_R += a[4] * T * T + a[5] * T * T * T + a[6] * T * T * T * T;
+    return cv.R * Cp_on_R;
+}
+
+// cea_thermo_curves.d:63-91
+static double eval_h(const CEACurve& cv, double T)
+{
+    if (T < cv.Tb[0]) {
+        double T_low = cv.Tb[0];
+        double h_low = eval_h(cv, T_low);
+        double Cp_low = eval_Cp(cv, T_low);
+        return h_low - Cp_low * (T_low - T);
+    }
+    if (T > cv.Tb[cv.nseg]) {
+        double T_high = cv.Tb[cv.nseg];
+        double h_high = eval_h(cv, T_high);
+        double Cp_high = eval_Cp(cv, T_high);
+        return h_high + Cp_high * (T - T_high);
+    }
+    double a[9];
+    determineCoefficients(cv, T, a);
+    double h_on_RT = -a[0] / (T * T) + a[1] / T * log(T) + a[2] + a[3] * T / 2.0;
+    h_on_RT += a[4] * T * T / 3.0 + a[5] * T * T * T / 4.0 + a[6] * T * T * T * T / 5.0 + a[7] / T;
+    return cv.R * T * h_on_RT;
+}
+
+double ref_cea_eval_h(Gas* g, int isp, double T) { return eval_h(g->curve[isp], T); }
+double ref_cea_eval_Cp(Gas* g, int isp, double T) { return eval_Cp(g->curve[isp], T); }
+
+// gas/gas_model.d:330-338
+static double mass_average(const Gas* g, const GasState& Q, const double* phi)
+{
+    double result = 0.0;
+    for (int i = 0; i < g->nsp; ++i) result += Q.massf[i] * phi[i];
+    return result;
+}
+
+// therm_perf_gas_mix_eos.d:63-69
+static void tp_update_energy(const Gas* g, GasState& Q)
+{
+    double vals[NSP_MAX];
+    for (int s = 0; s < g->nsp; ++s) vals[s] = eval_h(g->curve[s], Q.T) - g->R[s] * Q.T;
+    Q.u = mass_average(g, Q, vals);
+}
+
+// nm/newton.d:40-107 specialised to the zeroFn/dzdT closures of
+// therm_perf_gas_mix_eos.d:104-123 (including their side effects on Q.T, Q.u).
+// Returns 0 and the root in *root, or -1 when the solve throws.
+static int tp_newton_solve(Gas* g, GasState& Q, double e_tgt, double x0, double xMin, double xMax,
+                           double tol, double* root)
+{
+    auto f = [&](double T) { Q.T = T; tp_update_energy(g, Q); g->newton_f_evals++; return e_tgt - Q.u; };
+    auto dfdx = [&](double T) {
+        double Cv[NSP_MAX];
+        for (int s = 0; s < g->nsp; ++s) Cv[s] = eval_Cp(g->curve[s], T) - g->R[s];
+        return -1.0 * mass_average(g, Q, Cv);
+    };
+    const int MAXIT = 30;
+    double xL = xMin, xH = xMax;
+    double fL = f(xL);
+    double fH = f(xH);
+    if ((fL > 0.0 && fH > 0.0) || (fL < 0.0 && fH < 0.0)) return -1;
+    if (fL == 0.0) { *root = xMin; return 0; }
+    if (fH == 0.0) { *root = xMax; return 0; }
+    if (fL < 0.0) { xL = xMin; xH = xMax; } else { xH = xMin; xL = xMax; }
+    double rts = x0;
+    double dxold = (xMax - xMin);
+    double dx = dxold;
+    double f0 = f(rts);
+    double df0 = dfdx(rts);
+    for (int j = 0; j < MAXIT; ++j) {
+        if ((((rts - xH) * df0 - f0) * ((rts - xL) * df0 - f0) > 0.0) ||
+            (fabs(2.0 * f0) > fabs(dxold * df0))) {
+            dxold = dx;
+            dx = 0.5 * (xH - xL);
+            rts = xL + dx;
+            if (xL == rts) { *root = rts; return 0; }
+        } else {
+            dxold = dx;
+            dx = f0 / df0;
+            double tmp = rts;
+            rts -= dx;
+            if (tmp == rts) { *root = rts; return 0; }
+        }
+        if (fabs(dx) < tol) { *root = rts; return 0; }
+        f0 = f(rts);
+        df0 = dfdx(rts);
+        if (f0 < 0.0) xL = rts; else xH = rts;
+    }
+    return -1;
+}
+
+// Generic Newton self-test of nm/newton.d:109-141 (cubic and a*x+sin(x)-exp(x)).
+int ref_newton_selftest(int which, double x0, double xMin, double xMax, double* root)
+{
+    auto f1 = [](double x) { return pow(x, 3) + pow(x, 2) - 3 * x - 3; };
+    auto d1 = [](double x) { return 3.0 * pow(x, 2) + 2.0 * x - 3.0; };
+    auto f2 = [](double x) { return 3.0 * x + sin(x) - exp(x); };
+    auto d2 = [](double x) { return 3.0 + cos(x) - exp(x); };
+    double tol = 1.0e-9;
+    const int MAXIT = 30;
+    double xL = xMin, xH = xMax;
+    auto F = [&](double x) { return which == 1 ? f1(x) : f2(x); };
+    auto D = [&](double x) { return which == 1 ? d1(x) : d2(x); };
+    double fL = F(xL), fH = F(xH);
+    if ((fL > 0.0 && fH > 0.0) || (fL < 0.0 && fH < 0.0)) return -1;
+    if (fL == 0.0) { *root = xMin; return 0; }
+    if (fH == 0.0) { *root = xMax; return 0; }
+    if (fL < 0.0) { xL = xMin; xH = xMax; } else { xH = xMin; xL = xMax; }
+    double rts = x0, dxold = xMax - xMin, dx = dxold, f0 = F(rts), df0 = D(rts);
+    for (int j = 0; j < MAXIT; ++j) {
+        if ((((rts - xH) * df0 - f0) * ((rts - xL) * df0 - f0) > 0.0) || (fabs(2.0 * f0) > fabs(dxold * df0))) {
+            dxold = dx; dx = 0.5 * (xH - xL); rts = xL + dx;
+            if (xL == rts) { *root = rts; return 0; }
+        } else {
+            dxold = dx; dx = f0 / df0; double tmp = rts; rts -= dx;
+            if (tmp == rts) { *root = rts; return 0; }
+        }
+        if (fabs(dx) < tol) { *root = rts; return 0; }
+        f0 = F(rts); df0 = D(rts);
+        if (f0 < 0.0) xL = rts; else xH = rts;
+    }
+    return -1;
+}
+
+// therm_perf_gas_mix_eos.d:70-153
+static int tp_update_temperature(Gas* g, GasState& Q)
+{
+    double Tsave = Q.T;
+    double TOL = 1.0e-6;
+    double e_tgt = Q.u;
+    double delT = 1000.0;
+    double T1 = fmax(Q.T - 0.5 * delT, T_MIN);
+    double T2 = T1 + delT;
+    double root;
+    if (tp_newton_solve(g, Q, e_tgt, Tsave, T1, T2, TOL, &root) == 0) { Q.T = root; return 0; }
+    if (tp_newton_solve(g, Q, e_tgt, Tsave, T_MIN, T_MAX, TOL, &root) == 0) { Q.T = root; return 0; }
+    Q.T = Tsave;
+    tp_update_energy(g, Q);
+    return -1;
+}
+
+// thermo/perf_gas_mix_eos.d:43-62, 86-94
+static double heavy_R(const Gas* g, const GasState& Q) { return mass_average(g, Q, g->R); }
+
+// ---- GasModel interface (ideal_gas.d:84-144; therm_perf_gas.d:177-196, 345-385)
+static int update_thermo_from_rhou(Gas* g, GasState& Q)
+{
+    if (g->type == GAS_IDEAL) {
+        if (Q.u <= 0.0 || Q.rho <= 0.0) return -1;
+        Q.T = Q.u * g->Cvinv;
+        Q.p = Q.rho * g->Rgas * Q.T;
+        return 0;
+    }
+    if (tp_update_temperature(g, Q) != 0) return -1;
+    double Rmix = heavy_R(g, Q);
+    Q.p = Q.rho * Rmix * Q.T;
+    return 0;
+}
+
+static int update_thermo_from_pT(Gas* g, GasState& Q)
+{
+    if (g->type == GAS_IDEAL) {
+        if (Q.T <= 0.0 || Q.p <= 0.0) return -1;
+        Q.rho = Q.p / (Q.T * g->Rgas);
+        Q.u = g->Cv * Q.T;
+        return 0;
+    }
+    double Rmix = heavy_R(g, Q);
+    double denom = Rmix * Q.T;
+    Q.rho = Q.p / denom;
+    tp_update_energy(g, Q);
+    return 0;
+}
+
+static int update_thermo_from_rhoT(Gas* g, GasState& Q)
+{
+    if (g->type == GAS_IDEAL) {
+        if (Q.T <= 0.0 || Q.rho <= 0.0) return -1;
+        Q.p = Q.rho * g->Rgas * Q.T;
+        Q.u = g->Cv * Q.T;
+        return 0;
+    }
+    tp_update_energy(g, Q);
+    double Rmix = heavy_R(g, Q);
+    Q.p = Q.rho * Rmix * Q.T;
+    return 0;
+}
+
+static int update_thermo_from_rhop(Gas* g, GasState& Q)
+{
+    if (g->type == GAS_IDEAL) {
+        if (Q.p <= 0.0 || Q.rho <= 0.0) return -1;
+        Q.T = Q.p / (Q.rho * g->Rgas);
+        Q.u = g->Cv * Q.T;
+        return 0;
+    }
+    double Rmix = heavy_R(g, Q);
+    Q.T = Q.p / (Q.rho * Rmix);
+    tp_update_energy(g, Q);
+    return 0;
+}
+
+static int update_sound_speed(Gas* g, GasState& Q)
+{
+    if (g->type == GAS_IDEAL) {
+        if (Q.T <= 0.0) return -1;
+        Q.a = sqrt(g->gamma * g->Rgas * Q.T);
+        return 0;
+    }
+    double Cp[NSP_MAX], Cv[NSP_MAX];
+    for (int i = 0; i < g->nsp; ++i) Cp[i] = eval_Cp(g->curve[i], Q.T);
+    double Cpm = mass_average(g, Q, Cp);
+    for (int i = 0; i < g->nsp; ++i) Cv[i] = eval_Cp(g->curve[i], Q.T) - g->R[i];
+    double Cvm = mass_average(g, Q, Cv);
+    double R = mass_average(g, Q, g->R);
+    Q.a = sqrt(Cpm / Cvm * (R * Q.T));
+    return 0;
+}
+
+// gas/gas_model.d:296-328 (tolerance 0.0, assert_error_tolerance 0.1)
+static int scale_mass_fractions(double* massf, int nsp)
+{
+    if (nsp == 1) {
+        if (fabs(massf[0] - 1.0) > 0.1) return -1;
+        massf[0] = 1.0;
+        return 0;
+    }
+    double massf_sum = 0.0;
+    for (int i = 0; i < nsp; ++i) {
+        massf[i] = massf[i] >= 0.0 ? massf[i] : 0.0;
+        massf_sum += massf[i];
+    }
+    if (fabs(massf_sum - 1.0) > 0.1) return -1;
+    if (fabs(massf_sum - 1.0) > 0.0) {
+        for (int i = 0; i < nsp; ++i) massf[i] /= massf_sum;
+    }
+    return 0;
+}
+
+// Array-of-state entry points for the known-answer tests.  Q layout:
+// [rho, p, T, u, a, massf[0..nsp-1]].
+static void unpackQ(const Gas* g, const double* q, GasState& Q)
+{
+    Q.rho = q[0]; Q.p = q[1]; Q.T = q[2]; Q.u = q[3]; Q.a = q[4];
+    for (int i = 0; i < g->nsp; ++i) Q.massf[i] = q[5 + i];
+}
+static void packQ(const Gas* g, const GasState& Q, double* q)
+{
+    q[0] = Q.rho; q[1] = Q.p; q[2] = Q.T; q[3] = Q.u; q[4] = Q.a;
+    for (int i = 0; i < g->nsp; ++i) q[5 + i] = Q.massf[i];
+}
+int ref_gas_update(Gas* g, int which, double* q)
+{
+    GasState Q; unpackQ(g, q, Q);
+    int rc = -9;
+    switch (which) {
+    case 0: rc = update_thermo_from_pT(g, Q); break;
+    case 1: rc = update_thermo_from_rhou(g, Q); break;
+    case 2: rc = update_thermo_from_rhoT(g, Q); break;
+    case 3: rc = update_thermo_from_rhop(g, Q); break;
+    case 4: rc = update_sound_speed(g, Q); break;
+    case 5: tp_update_energy(g, Q); rc = 0; break;
+    }
+    packQ(g, Q, q);
+    return rc;
+}
+// Batched T(rho,u,Y) decode used to build the temperature tables:
+// q is [n][5+nsp]; returns number of failures.
+int ref_gas_update_many(Gas* g, int which, int n, double* q)
+{
+    int bad = 0;
+    const int w = 5 + g->nsp;
+    #pragma omp parallel for reduction(+:bad) schedule(static)
+    for (int i = 0; i < n; ++i) {
+        GasState Q; unpackQ(g, q + (size_t)i * w, Q);
+        int rc = -9;
+        if (which == 1) rc = update_thermo_from_rhou(g, Q);
+        else if (which == 0) rc = update_thermo_from_pT(g, Q);
+        else if (which == 2) rc = update_thermo_from_rhoT(g, Q);
+        else if (which == 4) rc = update_sound_speed(g, Q);
+        if (rc != 0) bad++;
+        packQ(g, Q, q + (size_t)i * w);
+    }
+    return bad;
+}
+
+// ------------------------------------------------------------ reconstruction
+
+struct L2R2 {   // onedinterp.d:24-42 work variables set by l2r2_prepare
+    double aL0, aR0, lenL0_, lenR0_;
+    double two_over_lenL0_plus_lenL1, two_over_lenR0_plus_lenL0, two_over_lenR1_plus_lenR0;
+    double two_lenL0_plus_lenL1, two_lenR0_plus_lenR1;
+};
+
+// onedinterp.d:118-133
+static void l2r2_prepare(L2R2& w, double lenL1, double lenL0, double lenR0, double lenR1)
+{
+    w.lenL0_ = lenL0;
+    w.lenR0_ = lenR0;
+    w.aL0 = 0.5 * lenL0 / (lenL1 + 2.0 * lenL0 + lenR0);
+    w.aR0 = 0.5 * lenR0 / (lenL0 + 2.0 * lenR0 + lenR1);
+    w.two_over_lenL0_plus_lenL1 = 2.0 / (lenL0 + lenL1);
+    w.two_over_lenR0_plus_lenL0 = 2.0 / (lenR0 + lenL0);
+    w.two_over_lenR1_plus_lenR0 = 2.0 / (lenR1 + lenR0);
+    w.two_lenL0_plus_lenL1 = (2.0 * lenL0 + lenL1);
+    w.two_lenR0_plus_lenR1 = (2.0 * lenR0 + lenR1);
+}
+
+// limiters.d:30-38
+static inline double clip_to_limits(double q, double A, double B)
+{
+    double lower_limit = (A <= B) ? A : B;
+    double upper_limit = (A > B) ? A : B;
+    double qclipped = (q > lower_limit) ? q : lower_limit;
+    return (qclipped <= upper_limit) ? qclipped : upper_limit;
+}
+
+// onedinterp.d:136-162
+static void interp_l2r2_scalar(const L2R2& w, int apply_limiter, int extrema_clipping,
+                               double qL1, double qL0, double qR0, double qR1, double& qL, double& qR)
+{
+    double delLminus = (qL0 - qL1) * w.two_over_lenL0_plus_lenL1;
+    double del = (qR0 - qL0) * w.two_over_lenR0_plus_lenL0;
+    double delRplus = (qR1 - qR0) * w.two_over_lenR1_plus_lenR0;
+    double sL = 1.0;
+    double sR = 1.0;
+    if (apply_limiter) {
+        sL = (delLminus * del + fabs(delLminus * del) + epsilon_van_albada) /
+             (delLminus * delLminus + del * del + epsilon_van_albada);
+        sR = (del * delRplus + fabs(del * delRplus) + epsilon_van_albada) /
+             (del * del + delRplus * delRplus + epsilon_van_albada);
+    }
+    qL = qL0 + sL * w.aL0 * (del * w.two_lenL0_plus_lenL1 + delLminus * w.lenR0_);
+    qR = qR0 - sR * w.aR0 * (delRplus * w.lenL0_ + del * w.two_lenR0_plus_lenR1);
+    if (extrema_clipping) {
+        qL = clip_to_limits(qL, qL0, qR0);
+        qR = clip_to_limits(qR, qL0, qR0);
+    }
+}
+
+void ref_interp_l2r2_scalar(const double* len4, const double* q4, int apply_limiter, int extrema_clipping,
+                            double* qLR)
+{
+    L2R2 w; l2r2_prepare(w, len4[0], len4[1], len4[2], len4[3]);
+    interp_l2r2_scalar(w, apply_limiter, extrema_clipping, q4[0], q4[1], q4[2], q4[3], qLR[0], qLR[1]);
+}
+
+// geom/elements/vector3.d:425-450
+struct Frame { double n[3], t1[3], t2[3]; };
+static inline void to_local(double v[3], const Frame& f)
+{
+    double vx = v[0] * f.n[0] + v[1] * f.n[1] + v[2] * f.n[2];
+    double vy = v[0] * f.t1[0] + v[1] * f.t1[1] + v[2] * f.t1[2];
+    double vz = v[0] * f.t2[0] + v[1] * f.t2[1] + v[2] * f.t2[2];
+    v[0] = vx; v[1] = vy; v[2] = vz;
+}
+static inline void to_global(double v[3], const Frame& f)
+{
+    double vx = v[0] * f.n[0] + v[1] * f.t1[0] + v[2] * f.t2[0];
+    double vy = v[0] * f.n[1] + v[1] * f.t1[1] + v[2] * f.t2[1];
+    double vz = v[0] * f.n[2] + v[1] * f.t1[2] + v[2] * f.t2[2];
+    v[0] = vx; v[1] = vy; v[2] = vz;
+}
+
+struct Config {
+    int dims;
+    int flux_calc;
+    int update_scheme;
+    int interp_order;
+    int apply_limiter;
+    int extrema_clipping;
+    int thermo_interp;
+    int max_invalid_cells;
+};
+
+// onedinterp.d:435-585 together with the EOS mixins of onedinterp.d:1039-1068.
+// cL1..cR1 are copies (the reference's in-place rotate/unrotate of the cells'
+// velocities is an exact round trip).
+static void interp_l2r2(Gas* g, const Config& cfg, const Frame& fr,
+                        FlowState cL1, FlowState cL0, FlowState cR0, FlowState cR1,
+                        double lenL1, double lenL0, double lenR0, double lenR1,
+                        FlowState& Lft, FlowState& Rght)
+{
+    const int nsp = g->nsp;
+    to_local(cL1.vel, fr); to_local(cL0.vel, fr); to_local(cR0.vel, fr); to_local(cR1.vel, fr);
+    L2R2 w; l2r2_prepare(w, lenL1, lenL0, lenR0, lenR1);
+    for (int d = 0; d < 3; ++d)
+        interp_l2r2_scalar(w, cfg.apply_limiter, cfg.extrema_clipping, cL1.vel[d], cL0.vel[d], cR0.vel[d],
+                           cR1.vel[d], Lft.vel[d], Rght.vel[d]);
+    const GasState &gL1 = cL1.gas, &gL0 = cL0.gas, &gR0 = cR0.gas, &gR1 = cR1.gas;
+    if (nsp > 1) {
+        for (int s = 0; s < nsp; ++s)
+            interp_l2r2_scalar(w, cfg.apply_limiter, cfg.extrema_clipping, gL1.massf[s], gL0.massf[s],
+                               gR0.massf[s], gR1.massf[s], Lft.gas.massf[s], Rght.gas.massf[s]);
+        if (scale_mass_fractions(Lft.gas.massf, nsp) != 0)
+            for (int s = 0; s < nsp; ++s) Lft.gas.massf[s] = gL0.massf[s];
+        if (scale_mass_fractions(Rght.gas.massf, nsp) != 0)
+            for (int s = 0; s < nsp; ++s) Rght.gas.massf[s] = gR0.massf[s];
+    } else {
+        Lft.gas.massf[0] = 1.0;
+        Rght.gas.massf[0] = 1.0;
+    }
+    int rcL = 0, rcR = 0;
+    switch (cfg.thermo_interp) {
+    case INTERP_PT:
+        interp_l2r2_scalar(w, cfg.apply_limiter, cfg.extrema_clipping, gL1.p, gL0.p, gR0.p, gR1.p, Lft.gas.p, Rght.gas.p);
+        interp_l2r2_scalar(w, cfg.apply_limiter, cfg.extrema_clipping, gL1.T, gL0.T, gR0.T, gR1.T, Lft.gas.T, Rght.gas.T);
+        rcL = update_thermo_from_pT(g, Lft.gas);
+        if (rcL != 0) { Lft = cL0; }
+        rcR = update_thermo_from_pT(g, Rght.gas);
+        if (rcR != 0) { Rght = cR0; }
+        break;
+    case INTERP_RHOU:
+        interp_l2r2_scalar(w, cfg.apply_limiter, cfg.extrema_clipping, gL1.rho, gL0.rho, gR0.rho, gR1.rho, Lft.gas.rho, Rght.gas.rho);
+        interp_l2r2_scalar(w, cfg.apply_limiter, cfg.extrema_clipping, gL1.u, gL0.u, gR0.u, gR1.u, Lft.gas.u, Rght.gas.u);
+        rcL = update_thermo_from_rhou(g, Lft.gas);
+        if (rcL != 0) { Lft = cL0; }
+        rcR = update_thermo_from_rhou(g, Rght.gas);
+        if (rcR != 0) { Rght = cR0; }
+        break;
+    case INTERP_RHOP:
+        interp_l2r2_scalar(w, cfg.apply_limiter, cfg.extrema_clipping, gL1.rho, gL0.rho, gR0.rho, gR1.rho, Lft.gas.rho, Rght.gas.rho);
+        interp_l2r2_scalar(w, cfg.apply_limiter, cfg.extrema_clipping, gL1.p, gL0.p, gR0.p, gR1.p, Lft.gas.p, Rght.gas.p);
+        rcL = update_thermo_from_rhop(g, Lft.gas);
+        if (rcL != 0) { Lft = cL0; }
+        rcR = update_thermo_from_rhop(g, Rght.gas);
+        if (rcR != 0) { Rght = cR0; }
+        break;
+    case INTERP_RHOT:
+        interp_l2r2_scalar(w, cfg.apply_limiter, cfg.extrema_clipping, gL1.rho, gL0.rho, gR0.rho, gR1.rho, Lft.gas.rho, Rght.gas.rho);
+        interp_l2r2_scalar(w, cfg.apply_limiter, cfg.extrema_clipping, gL1.T, gL0.T, gR0.T, gR1.T, Lft.gas.T, Rght.gas.T);
+        rcL = update_thermo_from_rhoT(g, Lft.gas);
+        if (rcL != 0) { Lft = cL0; }
+        rcR = update_thermo_from_rhoT(g, Rght.gas);
+        if (rcR != 0) { Rght = cR0; }
+        break;
+    }
+    // The fallback copies above are of the rotated cell states (cL0/cR0 are
+    // local-frame copies here), matching the reference, where the fallback
+    // copy happens before the transform back to the global frame.
+    to_global(Lft.vel, fr);
+    to_global(Rght.vel, fr);
+}
+
+// ------------------------------------------------------------------- fluxes
+// Conserved layout: [mass, mom_x, mom_y, mom_z, total_energy, rhoY_0..rhoY_{nsp-1}]
+// (species only when nsp > 1).
+
+// fluxcalc.d:376-529 (velocities already in the face frame)
+static void ausmdv(const Gas* g, const FlowState& Lft, const FlowState& Rght, double* F)
+{
+    double rL = Lft.gas.rho;
+    double pL = Lft.gas.p;
+    double pLrL = pL / rL;
+    double uL = Lft.vel[0];
+    double vL = Lft.vel[1];
+    double wL = Lft.vel[2];
+    double eL = Lft.gas.u;
+    double aL = Lft.gas.a;
+    double keL = 0.5 * (uL * uL + vL * vL + wL * wL);
+    double HL = eL + pLrL + keL;
+    double rR = Rght.gas.rho;
+    double pR = Rght.gas.p;
+    double pRrR = pR / rR;
+    double uR = Rght.vel[0];
+    double vR = Rght.vel[1];
+    double wR = Rght.vel[2];
+    double eR = Rght.gas.u;
+    double aR = Rght.gas.a;
+    double keR = 0.5 * (uR * uR + vR * vR + wR * wR);
+    double HR = eR + pRrR + keR;
+    double alphaL = 2.0 * pLrL / (pLrL + pRrR);
+    double alphaR = 2.0 * pRrR / (pLrL + pRrR);
+    double am = fmax(aL, aR);
+    double ML = uL / am;
+    double MR = uR / am;
+    double pLplus, uLplus;
+    double duL = 0.5 * (uL + fabs(uL));
+    if (fabs(ML) <= 1.0) {
+        pLplus = pL * (ML + 1.0) * (ML + 1.0) * (2.0 - ML) * 0.25;
+        uLplus = alphaL * ((uL + am) * (uL + am) / (4.0 * am) - duL) + duL;
+    } else {
+        pLplus = pL * duL / uL;
+        uLplus = duL;
+    }
+    double pRminus, uRminus;
+    double duR = 0.5 * (uR - fabs(uR));
+    if (fabs(MR) <= 1.0) {
+        pRminus = pR * (MR - 1.0) * (MR - 1.0) * (2.0 + MR) * 0.25;
+        uRminus = alphaR * (-(uR - am) * (uR - am) / (4.0 * am) - duR) + duR;
+    } else {
+        pRminus = pR * duR / uR;
+        uRminus = duR;
+    }
+    double ru_half = uLplus * rL + uRminus * rR;
+    double p_half = pLplus + pRminus;
+    double dp = pL - pR;
+    const double K_SWITCH = 10.0;
+    dp = K_SWITCH * fabs(dp) / fmin(pL, pR);
+    double s = 0.5 * fmin(1.0, dp);
+    double ru2_AUSMV = uLplus * rL * uL + uRminus * rR * uR;
+    double ru2_AUSMD = 0.5 * (ru_half * (uL + uR) - fabs(ru_half) * (uR - uL));
+    double ru2_half = (0.5 + s) * ru2_AUSMV + (0.5 - s) * ru2_AUSMD;
+    const int nsp = g->nsp;
+    F[0] = ru_half;
+    if (ru_half >= 0.0) {
+        F[1] = ru2_half + p_half; F[2] = ru_half * vL; F[3] = ru_half * wL;
+        F[4] = ru_half * HL;
+        if (nsp > 1) for (int i = 0; i < nsp; ++i) F[5 + i] = ru_half * Lft.gas.massf[i];
+    } else {
+        F[1] = ru2_half + p_half; F[2] = ru_half * vR; F[3] = ru_half * wR;
+        F[4] = ru_half * HR;
+        if (nsp > 1) for (int i = 0; i < nsp; ++i) F[5 + i] = ru_half * Rght.gas.massf[i];
+    }
+    const double C_EFIX = 0.125;
+    bool caseA = ((uL - aL) < 0.0) && ((uR - aR) > 0.0);
+    bool caseB = ((uL + aL) < 0.0) && ((uR + aR) > 0.0);
+    double d_ua = 0.0;
+    if (caseA && !caseB) d_ua = C_EFIX * ((uR - aR) - (uL - aL));
+    if (caseB && !caseA) d_ua = C_EFIX * ((uR + aR) - (uL + aL));
+    if (d_ua != 0.0) {
+        F[0] -= d_ua * (rR - rL);
+        F[1] -= d_ua * (rR * uR - rL * uL);
+        F[2] -= d_ua * (rR * vR - rL * vL);
+        F[3] -= d_ua * (rR * wR - rL * wL);
+        F[4] -= d_ua * (rR * HR - rL * HL);
+        if (nsp > 1)
+            for (int i = 0; i < nsp; ++i)
+                F[5 + i] -= d_ua * (rR * Rght.gas.massf[i] - rL * Lft.gas.massf[i]);
+    }
+}
+
+// HLLC (no counterpart in the reference; Toro 2009, "Riemann Solvers and
+// Numerical Methods for Fluid Dynamics", 3rd ed., section 10.4, eqs. 10.37-10.39
+// with the pressure-free contact speed S* of eq. 10.37 and the Davis wave-speed
+// bounds SL = min(uL-aL, uR-aR), SR = max(uL+aL, uR+aR) of eq. 10.48).  The
+// total energy uses the face internal energy (the gas model's u), so the
+// scheme works for any caloric EOS; species are carried by the contact wave.
+static void hllc(const Gas* g, const FlowState& Lft, const FlowState& Rght, double* F)
+{
+    const int nsp = g->nsp;
+    double rL = Lft.gas.rho, pL = Lft.gas.p, uL = Lft.vel[0], vL = Lft.vel[1], wL = Lft.vel[2];
+    double aL = Lft.gas.a;
+    double EL = rL * (Lft.gas.u + 0.5 * (uL * uL + vL * vL + wL * wL));
+    double rR = Rght.gas.rho, pR = Rght.gas.p, uR = Rght.vel[0], vR = Rght.vel[1], wR = Rght.vel[2];
+    double aR = Rght.gas.a;
+    double ER = rR * (Rght.gas.u + 0.5 * (uR * uR + vR * vR + wR * wR));
+    double SL = fmin(uL - aL, uR - aR);
+    double SR = fmax(uL + aL, uR + aR);
+    double mL = rL * (SL - uL);
+    double mR = rR * (SR - uR);
+    double Sstar = (pR - pL + mL * uL - mR * uR) / (mL - mR);
+    double Fl[NQ_MAX], Fr[NQ_MAX];
+    // physical fluxes
+    double mfL = rL * uL, mfR = rR * uR;
+    Fl[0] = mfL; Fl[1] = mfL * uL + pL; Fl[2] = mfL * vL; Fl[3] = mfL * wL; Fl[4] = uL * (EL + pL);
+    Fr[0] = mfR; Fr[1] = mfR * uR + pR; Fr[2] = mfR * vR; Fr[3] = mfR * wR; Fr[4] = uR * (ER + pR);
+    if (nsp > 1) for (int i = 0; i < nsp; ++i) { Fl[5 + i] = mfL * Lft.gas.massf[i]; Fr[5 + i] = mfR * Rght.gas.massf[i]; }
+    const int nq = 5 + (nsp > 1 ? nsp : 0);
+    if (0.0 <= SL) { for (int q = 0; q < nq; ++q) F[q] = Fl[q]; return; }
+    if (SR <= 0.0) { for (int q = 0; q < nq; ++q) F[q] = Fr[q]; return; }
+    const bool left = (0.0 <= Sstar);
+    const double rK = left ? rL : rR, uK = left ? uL : uR, vK = left ? vL : vR, wK = left ? wL : wR;
+    const double pK = left ? pL : pR, EK = left ? EL : ER, SK = left ? SL : SR;
+    const double* FK = left ? Fl : Fr;
+    const FlowState& K = left ? Lft : Rght;
+    double coef = rK * (SK - uK) / (SK - Sstar);
+    double Us[NQ_MAX];
+    Us[0] = coef;
+    Us[1] = coef * Sstar;
+    Us[2] = coef * vK;
+    Us[3] = coef * wK;
+    Us[4] = coef * (EK / rK + (Sstar - uK) * (Sstar + pK / (rK * (SK - uK))));
+    if (nsp > 1) for (int i = 0; i < nsp; ++i) Us[5 + i] = coef * K.gas.massf[i];
+    double UK[NQ_MAX];
+    UK[0] = rK; UK[1] = rK * uK; UK[2] = rK * vK; UK[3] = rK * wK; UK[4] = EK;
+    if (nsp > 1) for (int i = 0; i < nsp; ++i) UK[5 + i] = rK * K.gas.massf[i];
+    for (int q = 0; q < nq; ++q) F[q] = FK[q] + SK * (Us[q] - UK[q]);
+}
+
+// fluxcalc.d:26-128 on a fixed grid (gvel = 0, omegaz = 0).
+static void compute_interface_flux(const Gas* g, const Config& cfg, FlowState Lft, FlowState Rght,
+                                   const Frame& fr, double* F)
+{
+    to_local(Lft.vel, fr);
+    to_local(Rght.vel, fr);
+    if (cfg.flux_calc == FLUX_HLLC) hllc(g, Lft, Rght, F);
+    else ausmdv(g, Lft, Rght, F);
+    // The grid-velocity terms of fluxcalc.d:112-121 add exact zeros on a fixed grid.
+    double m[3] = {F[1], F[2], F[3]};
+    to_global(m, fr);
+    F[1] = m[0]; F[2] = m[1]; F[3] = m[2];
+}
+
+// Stand-alone flux entry for tests.  fs layout: [rho,p,T,u,a,vx,vy,vz,massf..]
+// in the face frame already (n, t1, t2) = (x, y, z).
+int ref_flux(Gas* g, int calc, const double* L, const double* R, double* F)
+{
+    FlowState l, r;
+    auto unpack = [&](const double* s, FlowState& f) {
+        f.gas.rho = s[0]; f.gas.p = s[1]; f.gas.T = s[2]; f.gas.u = s[3]; f.gas.a = s[4];
+        f.vel[0] = s[5]; f.vel[1] = s[6]; f.vel[2] = s[7];
+        for (int i = 0; i < g->nsp; ++i) f.gas.massf[i] = s[8 + i];
+    };
+    unpack(L, l); unpack(R, r);
+    if (calc == FLUX_HLLC) hllc(g, l, r, F); else ausmdv(g, l, r, F);
+    return 0;
+}
+
+// ------------------------------------------------------------------- blocks
+
+struct BC { int type; FlowState fs; };
+
+struct Block {
+    Gas* gas;
+    Config cfg;
+    int ni, nj, nk, nidim, njdim, nkdim;
+    int imin, imax, jmin, jmax, kmin, kmax;
+    int nq;
+    double dx, dy, dz, vol, area[3];
+    Frame frame[3];
+    std::vector<FlowState> fs;
+    std::vector<double> U[4], dUdt[4];
+    std::vector<double> F[3];     // face fluxes, indexed by the cell on the + side
+    std::vector<char> bad;
+    BC bc[6];
+};
+
+static inline size_t cidx(const Block* b, int i, int j, int k) { return ((size_t)k * b->njdim + j) * b->nidim + i; }
+
+// sfluidblock.d:94-126, 205-209 (ghost layers = 2); geometry for a uniform
+// Cartesian grid (the values properties.d:97-139, 568-650 produce for it).
+Block* ref_block_new(Gas* g, int dims, int ni, int nj, int nk, double dx, double dy, double dz)
+{
+    Block* b = new Block();
+    b->gas = g;
+    b->cfg = Config{dims, FLUX_AUSMDV, UPD_PC, 2, 1, 1, INTERP_RHOU, 0};
+    b->ni = ni; b->nj = nj; b->nk = (dims == 3) ? nk : 1;
+    b->nidim = ni + 4; b->njdim = nj + 4; b->nkdim = (dims == 3) ? nk + 4 : 1;
+    b->imin = 2; b->imax = ni + 1; b->jmin = 2; b->jmax = nj + 1;
+    b->kmin = (dims == 3) ? 2 : 0; b->kmax = (dims == 3) ? nk + 1 : 0;
+    b->nq = 5 + (g->nsp > 1 ? g->nsp : 0);
+    b->dx = dx; b->dy = dy; b->dz = dz;
+    if (dims == 3) {
+        b->vol = dx * dy * dz;
+        b->area[0] = dy * dz; b->area[1] = dz * dx; b->area[2] = dx * dy;
+        b->frame[0] = Frame{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        b->frame[1] = Frame{{0, 1, 0}, {0, 0, 1}, {1, 0, 0}};
+        b->frame[2] = Frame{{0, 0, 1}, {1, 0, 0}, {0, 1, 0}};
+    } else {
+        b->vol = dx * dy;
+        b->area[0] = dy; b->area[1] = dx; b->area[2] = 0.0;
+        b->frame[0] = Frame{{1, -0.0, 0}, {-0.0, -1, 0}, {0, 0, 1}};   // fvinterface.d:265-296
+        b->frame[1] = Frame{{0, 1, 0}, {1, 0, 0}, {0, 0, 1}};
+        b->frame[2] = Frame{{0, 0, 1}, {1, 0, 0}, {0, 1, 0}};
+    }
+    size_t n = (size_t)b->nidim * b->njdim * b->nkdim;
+    b->fs.assign(n, FlowState{});
+    for (int t = 0; t < 4; ++t) { b->U[t].assign(n * b->nq, 0.0); b->dUdt[t].assign(n * b->nq, 0.0); }
+    for (int d = 0; d < 3; ++d) b->F[d].assign(n * b->nq, 0.0);
+    b->bad.assign(n, 0);
+    for (int f = 0; f < 6; ++f) b->bc[f].type = BC_SLIP_WALL;   // prep.lua default WallBC_WithSlip
+    return b;
+}
+
+void ref_block_free(Block* b) { delete b; }
+
+void ref_block_set_config(Block* b, int flux_calc, int update_scheme, int interp_order, int apply_limiter,
+                          int extrema_clipping, int thermo_interp, int max_invalid_cells)
+{
+    b->cfg.flux_calc = flux_calc; b->cfg.update_scheme = update_scheme; b->cfg.interp_order = interp_order;
+    b->cfg.apply_limiter = apply_limiter; b->cfg.extrema_clipping = extrema_clipping;
+    b->cfg.thermo_interp = thermo_interp; b->cfg.max_invalid_cells = max_invalid_cells;
+}
+
+// fs vector layout: [rho,p,T,u,a,vx,vy,vz,massf...]
+void ref_block_set_bc(Block* b, int face, int type, const double* fs)
+{
+    b->bc[face].type = type;
+    if (fs) {
+        FlowState& f = b->bc[face].fs;
+        f.gas.rho = fs[0]; f.gas.p = fs[1]; f.gas.T = fs[2]; f.gas.u = fs[3]; f.gas.a = fs[4];
+        f.vel[0] = fs[5]; f.vel[1] = fs[6]; f.vel[2] = fs[7];
+        for (int i = 0; i < b->gas->nsp; ++i) f.gas.massf[i] = fs[8 + i];
+    }
+}
+
+// fvcell.d:494-560 (ftl = 0 unless given)
+static void encode_conserved(const Block* b, const FlowState& fs, double* U)
+{
+    double myrho = fs.gas.rho;
+    U[0] = myrho;
+    U[1] = fs.gas.rho * fs.vel[0]; U[2] = fs.gas.rho * fs.vel[1]; U[3] = fs.gas.rho * fs.vel[2];
+    double u = fs.gas.u;
+    double ke = 0.5 * (fs.vel[0] * fs.vel[0] + fs.vel[1] * fs.vel[1] + fs.vel[2] * fs.vel[2]);
+    U[4] = fs.gas.rho * (u + ke);
+    if (b->gas->nsp > 1)
+        for (int i = 0; i < b->gas->nsp; ++i) U[5 + i] = fs.gas.rho * fs.gas.massf[i];
+}
+
+// fvcell.d:563-732.  Returns 0, -1 (non-positive mass), -2 (bad massf or thermo).
+static int decode_conserved(Block* b, FlowState& fs, double* U)
+{
+    Gas* g = b->gas;
+    if (!(U[0] > 0.0)) return -1;
+    double rho = U[0];
+    fs.gas.rho = rho;
+    double dinv = 1.0 / rho;
+    fs.vel[0] = U[1] * dinv; fs.vel[1] = U[2] * dinv; fs.vel[2] = U[3] * dinv;
+    double rE = U[4];
+    double u = rE * dinv;
+    double ke = 0.5 * (fs.vel[0] * fs.vel[0] + fs.vel[1] * fs.vel[1] + fs.vel[2] * fs.vel[2]);
+    u -= ke;
+    fs.gas.u = u;
+    if (g->nsp > 1) {
+        for (int i = 0; i < g->nsp; ++i) fs.gas.massf[i] = U[5 + i] * dinv;
+        if (scale_mass_fractions(fs.gas.massf, g->nsp) != 0) return -2;
+    } else {
+        fs.gas.massf[0] = U[0] * dinv;   // U.massf[0] tracks U.mass exactly for one species
+    }
+    if (update_thermo_from_rhou(g, fs.gas) != 0) {
+        // ignore_low_T_thermo_update_failure (globalconfig.d:510, default true),
+        // suggested_low_T_value 200 K (globalconfig.d:511)
+        if (rho > 0.0) {
+            fs.gas.T = 200.0;
+            if (update_thermo_from_rhoT(g, fs.gas) != 0) return -2;
+            encode_conserved(b, fs, U);
+        } else {
+            return -2;
+        }
+    }
+    if (update_sound_speed(g, fs.gas) != 0) return -2;
+    return 0;
+}
+
+// flowstate.d:345-388 with gas_state.d:199-241 and FlowStateLimits (fvcore.d:18-27)
+static bool check_data(const Block* b, const FlowState& fs)
+{
+    bool ok = true;
+    const GasState& q = fs.gas;
+    if (!std::isfinite(q.rho) || q.rho < 1.01 * 0.0) ok = false;
+    if (!std::isfinite(q.T) || q.T < 1.01 * 1.0) ok = false;
+    if (!std::isfinite(q.p)) ok = false;
+    if (!std::isfinite(q.a)) ok = false;
+    double f_sum = 0.0;
+    for (int i = 0; i < b->gas->nsp; ++i) f_sum += q.massf[i];
+    if (f_sum < 0.99 || f_sum > 1.01 || !std::isfinite(f_sum)) ok = false;
+    if (fabs(fs.vel[0]) > 30000.0 || fabs(fs.vel[1]) > 30000.0 || fabs(fs.vel[2]) > 30000.0) ok = false;
+    if (q.T < 0.0) ok = false;
+    if (q.T > 50000.0) ok = false;
+    return ok;
+}
+
+// Initial state from (p, T, vel, massf) per active cell, as the initial flow
+// writer does (flowstate.d:177-216 constructor from p,T: update_thermo_from_pT,
+// update_sound_speed), then encode.  Arrays are [ncells] in i-fastest order;
+// massf is [nsp][ncells] (may be null for one species).
+int ref_block_set_from_pT(Block* b, const double* p, const double* T, const double* vx, const double* vy,
+                          const double* vz, const double* massf)
+{
+    const int nsp = b->gas->nsp;
+    const size_t nc = (size_t)b->ni * b->nj * b->nk;
+    size_t c = 0;
+    for (int k = b->kmin; k <= b->kmax; ++k)
+        for (int j = b->jmin; j <= b->jmax; ++j)
+            for (int i = b->imin; i <= b->imax; ++i, ++c) {
+                FlowState& f = b->fs[cidx(b, i, j, k)];
+                f.gas.p = p[c]; f.gas.T = T[c];
+                f.vel[0] = vx[c]; f.vel[1] = vy[c]; f.vel[2] = vz[c];
+                for (int s = 0; s < nsp; ++s) f.gas.massf[s] = massf ? massf[s * nc + c] : 1.0;
+                if (update_thermo_from_pT(b->gas, f.gas) != 0) return -1;
+                if (update_sound_speed(b->gas, f.gas) != 0) return -1;
+                encode_conserved(b, f, &b->U[0][cidx(b, i, j, k) * b->nq]);
+            }
+    return 0;
+}
+
+// Restart path: U0 given ([nq][ncells]); decode into fs.  Tguess (may be null)
+// seeds the thermally perfect Newton as a restart's stored T would.
+int ref_block_set_U(Block* b, const double* U, const double* Tguess)
+{
+    const size_t nc = (size_t)b->ni * b->nj * b->nk;
+    size_t c = 0;
+    for (int k = b->kmin; k <= b->kmax; ++k)
+        for (int j = b->jmin; j <= b->jmax; ++j)
+            for (int i = b->imin; i <= b->imax; ++i, ++c) {
+                size_t ci = cidx(b, i, j, k);
+                double* u = &b->U[0][ci * b->nq];
+                for (int q = 0; q < b->nq; ++q) u[q] = U[q * nc + c];
+                if (Tguess) b->fs[ci].gas.T = Tguess[c];
+                if (decode_conserved(b, b->fs[ci], u) != 0) return -1;
+            }
+    return 0;
+}
+
+void ref_block_get_U(const Block* b, int ftl, double* U)
+{
+    const size_t nc = (size_t)b->ni * b->nj * b->nk;
+    size_t c = 0;
+    for (int k = b->kmin; k <= b->kmax; ++k)
+        for (int j = b->jmin; j <= b->jmax; ++j)
+            for (int i = b->imin; i <= b->imax; ++i, ++c) {
+                const double* u = &b->U[ftl][cidx(b, i, j, k) * b->nq];
+                for (int q = 0; q < b->nq; ++q) U[q * nc + c] = u[q];
+            }
+}
+
+// fs fields: rho,p,T,u,a,vx,vy,vz,massf... as [nfield][ncells]
+void ref_block_get_fs(const Block* b, double* out)
+{
+    const size_t nc = (size_t)b->ni * b->nj * b->nk;
+    size_t c = 0;
+    for (int k = b->kmin; k <= b->kmax; ++k)
+        for (int j = b->jmin; j <= b->jmax; ++j)
+            for (int i = b->imin; i <= b->imax; ++i, ++c) {
+                const FlowState& f = b->fs[cidx(b, i, j, k)];
+                out[0 * nc + c] = f.gas.rho; out[1 * nc + c] = f.gas.p; out[2 * nc + c] = f.gas.T;
+                out[3 * nc + c] = f.gas.u; out[4 * nc + c] = f.gas.a;
+                out[5 * nc + c] = f.vel[0]; out[6 * nc + c] = f.vel[1]; out[7 * nc + c] = f.vel[2];
+                for (int s = 0; s < b->gas->nsp; ++s) out[(8 + s) * nc + c] = f.gas.massf[s];
+            }
+}
+
+// Ghost-cell effects (bc/ghost_cell_effect): flow_state_copy.d:66-169,
+// extrapolate_copy.d:82-… (x_order 0), internal_copy_then_reflect.d:94-310 with
+// ghost_cell.d:32-43, full_face_copy.d:1277-1357 for a block connected to
+// itself across opposite faces (periodic).
+static void apply_bcs(Block* b)
+{
+    const int dims = b->cfg.dims;
+    const int nfaces = (dims == 3) ? 6 : 4;
+    for (int face = 0; face < nfaces; ++face) {
+        const BC& bc = b->bc[face];
+        int dir = (face == F_EAST || face == F_WEST) ? 0 : (face == F_NORTH || face == F_SOUTH) ? 1 : 2;
+        bool plus = (face == F_EAST || face == F_NORTH || face == F_TOP);
+        int lo[3] = {b->imin, b->jmin, b->kmin}, hi[3] = {b->imax, b->jmax, b->kmax};
+        int n[3] = {b->ni, b->nj, b->nk};
+        int a1 = (dir + 1) % 3, a2 = (dir + 2) % 3;
+        for (int q2 = lo[a2]; q2 <= hi[a2]; ++q2)
+            for (int q1 = lo[a1]; q1 <= hi[a1]; ++q1)
+                for (int layer = 1; layer <= 2; ++layer) {
+                    int g[3], s[3];
+                    g[a1] = s[a1] = q1; g[a2] = s[a2] = q2;
+                    g[dir] = plus ? hi[dir] + layer : lo[dir] - layer;
+                    FlowState& dest = b->fs[cidx(b, g[0], g[1], g[2])];
+                    switch (bc.type) {
+                    case BC_INFLOW:
+                        dest = bc.fs;
+                        break;
+                    case BC_EXTRAPOLATE:
+                        s[dir] = plus ? hi[dir] : lo[dir];
+                        dest = b->fs[cidx(b, s[0], s[1], s[2])];
+                        break;
+                    case BC_PERIODIC:
+                        s[dir] = plus ? lo[dir] + layer - 1 : hi[dir] - layer + 1;
+                        dest = b->fs[cidx(b, s[0], s[1], s[2])];
+                        break;
+                    case BC_SLIP_WALL:
+                    default: {
+                        s[dir] = plus ? hi[dir] - layer + 1 : lo[dir] + layer - 1;
+                        dest = b->fs[cidx(b, s[0], s[1], s[2])];
+                        const Frame& fr = b->frame[dir];
+                        to_local(dest.vel, fr);
+                        dest.vel[0] = -(dest.vel[0]);
+                        to_global(dest.vel, fr);
+                        break;
+                    }
+                    }
+                    (void)n;
+                }
+    }
+}
+
+// sfluidblock.d:2016-2346, the classic (l2r2) branch; all BCs here keep ghost
+// data, so the one-sided variants and wall-flux calls are not reached.
+static void convective_flux(Block* b)
+{
+    Gas* g = b->gas;
+    const Config& cfg = b->cfg;
+    const bool do_reconstruction = (cfg.interp_order > 1);
+    const int nq = b->nq;
+    const int ndir = (cfg.dims == 3) ? 3 : 2;
+    const double len[3] = {b->dx, b->dy, b->dz};
+    for (int dir = 0; dir < ndir; ++dir) {
+        int st[3] = {0, 0, 0}; st[dir] = 1;
+        const size_t stride = (size_t)st[0] + (size_t)st[1] * b->nidim + (size_t)st[2] * b->nidim * b->njdim;
+        const int ihi = b->imax + st[0], jhi = b->jmax + st[1], khi = b->kmax + st[2];
+        #pragma omp parallel for collapse(2) schedule(static)
+        for (int k = b->kmin; k <= khi; ++k)
+            for (int j = b->jmin; j <= jhi; ++j)
+                for (int i = b->imin; i <= ihi; ++i) {
+                    size_t cR0 = cidx(b, i, j, k);
+                    size_t cL0 = cR0 - stride, cL1 = cL0 - stride, cR1 = cR0 + stride;
+                    FlowState Lft = b->fs[cL0];
+                    FlowState Rght = b->fs[cR0];
+                    if (do_reconstruction)
+                        interp_l2r2(g, cfg, b->frame[dir], b->fs[cL1], b->fs[cL0], b->fs[cR0], b->fs[cR1],
+                                    len[dir], len[dir], len[dir], len[dir], Lft, Rght);
+                    double Fq[NQ_MAX];
+                    compute_interface_flux(g, cfg, Lft, Rght, b->frame[dir], Fq);
+                    double* dst = &b->F[dir][cR0 * nq];
+                    for (int q = 0; q < nq; ++q) dst[q] = Fq[q];
+                }
+    }
+}
+
+// fvcell.d:735-875 (sources are zero: planar, non-rotating, inviscid)
+static void time_derivatives(Block* b, size_t c, double* dUdt)
+{
+    const int nq = b->nq;
+    const bool three_d = (b->cfg.dims == 3);
+    const size_t sj = b->nidim, sk = (size_t)b->nidim * b->njdim;
+    // face order north, east, south, west, top, bottom (sfluidblock.d:507-518)
+    const double* Fs[6] = {&b->F[1][(c + sj) * nq], &b->F[0][(c + 1) * nq], &b->F[1][c * nq],
+                           &b->F[0][c * nq], three_d ? &b->F[2][(c + sk) * nq] : nullptr,
+                           three_d ? &b->F[2][c * nq] : nullptr};
+    const double outsign[6] = {1.0, 1.0, -1.0, -1.0, 1.0, -1.0};
+    const double A[6] = {b->area[1], b->area[0], b->area[1], b->area[0], b->area[2], b->area[2]};
+    double integral[NQ_MAX];
+    for (int q = 0; q < nq; ++q) integral[q] = 0.0;
+    const int nf = three_d ? 6 : 4;
+    for (int f = 0; f < nf; ++f) {
+        const double* myF = Fs[f];
+        double area = outsign[f] * A[f];
+        integral[0] -= myF[0] * area;
+        integral[1] -= myF[1] * area;
+        integral[2] -= myF[2] * area;
+        if (three_d) integral[3] -= myF[3] * area;
+        integral[4] -= myF[4] * area;
+        for (int q = 5; q < nq; ++q) integral[q] -= myF[q] * area;
+    }
+    double vol_inv = 1.0 / b->vol;
+    for (int q = 0; q < nq; ++q) dUdt[q] = vol_inv * integral[q] + 0.0;
+}
+
+static int nstages(int scheme)
+{
+    switch (scheme) {
+    case UPD_EULER: return 1;
+    case UPD_PC: case UPD_MIDPOINT: return 2;
+    default: return 3;
+    }
+}
+
+// fvcell.d:1153-1393
+static void stage_update(Block* b, int stage, size_t c, double dt)
+{
+    const int nq = b->nq;
+    const int sch = b->cfg.update_scheme;
+    double* d0 = &b->dUdt[0][c * nq];
+    double* d1 = &b->dUdt[1][c * nq];
+    double* d2 = &b->dUdt[2][c * nq];
+    if (stage == 1) {
+        double g1 = 1.0;
+        if (sch == UPD_MIDPOINT || sch == UPD_CLASSIC_RK3) g1 = 0.5;
+        if (sch == UPD_DENMAN_RK3) g1 = 8.0 / 15.0;
+        const double* U0 = &b->U[0][c * nq];
+        double* U1 = &b->U[1][c * nq];
+        for (int q = 0; q < nq; ++q) U1[q] = U0[q] + dt * g1 * d0[q];
+    } else if (stage == 2) {
+        double g1 = 0.5, g2 = 0.5;
+        switch (sch) {
+        case UPD_MIDPOINT: g1 = 0.0; g2 = 1.0; break;
+        case UPD_CLASSIC_RK3: g1 = -1.0; g2 = 2.0; break;
+        case UPD_TVD_RK3: g1 = 0.25; g2 = 0.25; break;
+        case UPD_DENMAN_RK3: g1 = -17.0 / 60.0; g2 = 5.0 / 12.0; break;
+        default: break;
+        }
+        const double* Uo = &b->U[(sch == UPD_DENMAN_RK3) ? 1 : 0][c * nq];
+        double* U2 = &b->U[2][c * nq];
+        for (int q = 0; q < nq; ++q) U2[q] = Uo[q] + dt * (g1 * d0[q] + g2 * d1[q]);
+    } else {
+        double g1 = 1.0 / 6.0, g2 = 1.0 / 6.0, g3 = 4.0 / 6.0;
+        if (sch == UPD_CLASSIC_RK3) { g1 = 1.0 / 6.0; g2 = 4.0 / 6.0; g3 = 1.0 / 6.0; }
+        if (sch == UPD_DENMAN_RK3) { g1 = 0.0; g2 = -5.0 / 12.0; g3 = 3.0 / 4.0; }
+        const double* Uo = &b->U[(sch == UPD_DENMAN_RK3) ? 2 : 0][c * nq];
+        double* U3 = &b->U[3][c * nq];
+        for (int q = 0; q < nq; ++q) U3[q] = Uo[q] + dt * (g1 * d0[q] + g2 * d1[q] + g3 * d2[q]);
+    }
+}
+
+// One gasdynamic_explicit_increment_with_fixed_grid (simcore.d:2303-2952) for
+// one block.  Returns 0; -1 for a decode that throws (non-positive density or
+// failed thermo); -2 when the invalid-cell count exceeds max_invalid_cells.
+// nbad[stage] receives the per-stage invalid-cell count.
+int ref_block_step(Block* b, double dt, int* nbad)
+{
+    const int ns = nstages(b->cfg.update_scheme);
+    const int nq = b->nq;
+    for (int stage = 1; stage <= ns; ++stage) {
+        const int ftl = stage - 1;
+        apply_bcs(b);
+        convective_flux(b);
+        int decode_fail = 0, invalid = 0;
+        #pragma omp parallel for collapse(2) schedule(static) reduction(+:decode_fail, invalid)
+        for (int k = b->kmin; k <= b->kmax; ++k)
+            for (int j = b->jmin; j <= b->jmax; ++j)
+                for (int i = b->imin; i <= b->imax; ++i) {
+                    size_t c = cidx(b, i, j, k);
+                    time_derivatives(b, c, &b->dUdt[ftl][c * nq]);
+                    stage_update(b, stage, c, dt);
+                    if (decode_conserved(b, b->fs[c], &b->U[ftl + 1][c * nq]) != 0) { decode_fail++; continue; }
+                    if (!check_data(b, b->fs[c])) invalid++;
+                }
+        if (nbad) nbad[stage - 1] = invalid + decode_fail;
+        if (decode_fail) return -1;
+        if (invalid > b->cfg.max_invalid_cells) return -2;
+    }
+    std::swap(b->U[0], b->U[ns]);   // simcore.d:2936-2941
+    return 0;
+}
+
+// fvcell.d:1624-1705 (non-stringent CFL) and fluidblock.d:776-862 without
+// super-time-stepping or local time-stepping.  out = {dt_allow, cfl_max, signal_max}.
+void ref_block_dt(Block* b, double dt_current, double cfl_value, int check_cfl, double* out)
+{
+    const bool three_d = (b->cfg.dims == 3);
+    double cfl_allow = 0.9;
+    switch (nstages(b->cfg.update_scheme)) { case 1: cfl_allow = 0.9; break; case 2: cfl_allow = 1.2; break; default: cfl_allow = 1.6; }
+    bool first = true;
+    double dt_allow = 0, cfl_min = 0, cfl_max = 0, signal = 0, smax = 0;
+    const double iLength = b->dx, jLength = b->dy, kLength = b->dz;
+    for (int k = b->kmin; k <= b->kmax; ++k)
+        for (int j = b->jmin; j <= b->jmax; ++j)
+            for (int i = b->imin; i <= b->imax; ++i) {
+                const FlowState& fs = b->fs[cidx(b, i, j, k)];
+                const double* v = fs.vel;
+                const Frame& fN = b->frame[1];
+                const Frame& fE = b->frame[0];
+                const Frame& fT = b->frame[2];
+                double un_N = fabs(v[0] * fN.n[0] + v[1] * fN.n[1] + v[2] * fN.n[2]);
+                double un_E = fabs(v[0] * fE.n[0] + v[1] * fE.n[1] + v[2] * fE.n[2]);
+                double un_T = three_d ? fabs(v[0] * fT.n[0] + v[1] * fT.n[1] + v[2] * fT.n[2]) : 0.0;
+                signal = 0;
+                double signalN = (un_N + fs.gas.a) / jLength;
+                signal = fmax(signal, signalN);
+                double signalE = (un_E + fs.gas.a) / iLength;
+                signal = fmax(signal, signalE);
+                if (three_d) {
+                    double signalT = (un_T + fs.gas.a) / kLength;
+                    signal = fmax(signal, signalT);
+                }
+                smax = fmax(smax, signal);
+                double cfl_local = dt_current * signal;
+                double dt_local = cfl_value / signal;
+                if (first) { cfl_min = cfl_local; cfl_max = cfl_local; dt_allow = dt_local; first = false; }
+                else { cfl_min = fmin(cfl_min, cfl_local); cfl_max = fmax(cfl_max, cfl_local); dt_allow = fmin(dt_allow, dt_local); }
+            }
+    if (check_cfl && (cfl_max < 0.0 || cfl_max > cfl_allow)) {
+        cfl_max = 0.5 * cfl_allow;
+        dt_allow = cfl_max / signal;   // the last cell's signal, as in the reference
+    }
+    (void)cfl_min;
+    out[0] = dt_allow; out[1] = cfl_max; out[2] = smax;
+}
+
+int ref_block_nq(const Block* b) { return b->nq; }
+
+} // extern "C"

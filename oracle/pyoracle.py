@@ -1,0 +1,281 @@
+"""ctypes view of the CPU oracle (oracle/libe4ref.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker or the timed CPU baseline,
+never by the product path (eilmer4_amd/).
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libe4ref.so")
+DATA = os.path.join(os.path.dirname(HERE), "eilmer4_amd", "data")
+
+# enum values shared with the product ABI (reference enums, eilmer/fvcore.d)
+AUSMDV, HLLC = 0, 9
+EULER, PC, MIDPOINT, CLASSIC_RK3, TVD_RK3, DENMAN_RK3 = range(6)
+INTERP_PT, INTERP_RHOU, INTERP_RHOP, INTERP_RHOT = range(4)
+BC_SLIP_WALL, BC_INFLOW, BC_EXTRAPOLATE, BC_PERIODIC = range(4)
+NORTH, EAST, SOUTH, WEST, TOP, BOTTOM = range(6)
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+
+
+def build():
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.ref_gas_new_ideal.restype = vp
+        L.ref_gas_new_ideal.argtypes = [C.c_double, C.c_double]
+        L.ref_gas_new_tp.restype = vp
+        L.ref_gas_new_tp.argtypes = [C.c_int, _dp, C.c_int, _dp, _dp, _dp]
+        L.ref_gas_free.argtypes = [vp]
+        L.ref_gas_newton_evals.restype = C.c_long
+        L.ref_gas_newton_evals.argtypes = [vp]
+        L.ref_cea_eval_h.restype = C.c_double
+        L.ref_cea_eval_h.argtypes = [vp, C.c_int, C.c_double]
+        L.ref_cea_eval_Cp.restype = C.c_double
+        L.ref_cea_eval_Cp.argtypes = [vp, C.c_int, C.c_double]
+        L.ref_newton_selftest.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _dp]
+        L.ref_gas_update.argtypes = [vp, C.c_int, _dp]
+        L.ref_gas_update_many.argtypes = [vp, C.c_int, C.c_int, _dp]
+        L.ref_interp_l2r2_scalar.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp]
+        L.ref_flux.argtypes = [vp, C.c_int, _dp, _dp, _dp]
+        L.ref_block_new.restype = vp
+        L.ref_block_new.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
+        L.ref_block_free.argtypes = [vp]
+        L.ref_block_set_config.argtypes = [vp] + [C.c_int] * 7
+        L.ref_block_set_bc.argtypes = [vp, C.c_int, C.c_int, _dp]
+        L.ref_block_set_from_pT.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.ref_block_set_U.argtypes = [vp, _dp, _dp]
+        L.ref_block_get_U.argtypes = [vp, C.c_int, _dp]
+        L.ref_block_get_fs.argtypes = [vp, _dp]
+        L.ref_block_step.argtypes = [vp, C.c_double, C.POINTER(C.c_int)]
+        L.ref_block_dt.argtypes = [vp, C.c_double, C.c_double, C.c_int, _dp]
+        L.ref_block_nq.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Gas:
+    """Oracle gas model: ideal (mMass, gamma) or thermally perfect (JSON)."""
+
+    def __init__(self, handle, nsp, desc):
+        self.h = handle
+        self.nsp = nsp
+        self.desc = desc
+
+    @staticmethod
+    def ideal(mMass=0.02896, gamma=1.4):
+        return Gas(lib().ref_gas_new_ideal(mMass, gamma), 1, {"model": "IdealGas", "mMass": mMass, "gamma": gamma})
+
+    @staticmethod
+    def from_json(name):
+        path = name if os.path.isabs(name) else os.path.join(DATA, name)
+        d = json.load(open(path))
+        if d["model"] == "IdealGas":
+            return Gas.ideal(d["mMass"], d["gamma"])
+        nsp = len(d["species"])
+        nseg = d["thermo"][0]["nsegments"]
+        M = _f64(d["M"])
+        Tb = _f64([t["T_break_points"] for t in d["thermo"]]).ravel()
+        Tbl = _f64([t["T_blend_ranges"] for t in d["thermo"]]).ravel()
+        co = _f64([t["segments"] for t in d["thermo"]]).ravel()
+        h = lib().ref_gas_new_tp(nsp, _p(M), nseg, _p(Tb), _p(Tbl), _p(co))
+        return Gas(h, nsp, d)
+
+    @staticmethod
+    def single_curve(name, mMass):
+        """A one-species thermally perfect gas holding one CEA curve table."""
+        path = os.path.join(DATA, name)
+        d = json.load(open(path))["curve"]
+        M = _f64([mMass])
+        Tb = _f64(d["T_break_points"])
+        Tbl = _f64(d["T_blend_ranges"])
+        co = _f64(d["segments"]).ravel()
+        h = lib().ref_gas_new_tp(1, _p(M), d["nsegments"], _p(Tb), _p(Tbl), _p(co))
+        return Gas(h, 1, {"model": "curve"})
+
+    def newton_evals(self):
+        return lib().ref_gas_newton_evals(self.h)
+
+    def state(self, rho=0.0, p=0.0, T=0.0, u=0.0, a=0.0, massf=None):
+        q = np.zeros(5 + self.nsp)
+        q[:5] = [rho, p, T, u, a]
+        q[5:] = massf if massf is not None else ([1.0] if self.nsp == 1 else 0.0)
+        return q
+
+    def update(self, which, q):
+        q = _f64(q).copy()
+        which_i = {"pT": 0, "rhou": 1, "rhoT": 2, "rhop": 3, "sound_speed": 4, "energy": 5}[which]
+        rc = lib().ref_gas_update(self.h, which_i, _p(q))
+        return rc, q
+
+    def update_many(self, which, Q):
+        Q = _f64(Q).copy()
+        which_i = {"pT": 0, "rhou": 1, "rhoT": 2, "sound_speed": 4}[which]
+        bad = lib().ref_gas_update_many(self.h, which_i, Q.shape[0], _p(Q))
+        return bad, Q
+
+    def eval_h(self, isp, T):
+        return lib().ref_cea_eval_h(self.h, isp, T)
+
+    def eval_Cp(self, isp, T):
+        return lib().ref_cea_eval_Cp(self.h, isp, T)
+
+    def __del__(self):
+        try:
+            lib().ref_gas_free(self.h)
+        except Exception:
+            pass
+
+
+def interp_l2r2_scalar(lens, q, apply_limiter=True, extrema_clipping=True):
+    out = np.zeros(2)
+    lib().ref_interp_l2r2_scalar(_p(_f64(lens)), _p(_f64(q)), int(apply_limiter), int(extrema_clipping), _p(out))
+    return out[0], out[1]
+
+
+def flux(gas, calc, L, R):
+    """L, R: [rho,p,T,u,a,vn,vt1,vt2,massf...] in the face frame."""
+    nq = 5 + (gas.nsp if gas.nsp > 1 else 0)
+    F = np.zeros(nq)
+    lib().ref_flux(gas.h, calc, _p(_f64(L)), _p(_f64(R)), _p(F))
+    return F
+
+
+class Block:
+    """One structured block of the oracle (uniform Cartesian cells)."""
+
+    FS_FIELDS = ["rho", "p", "T", "u", "a", "vx", "vy", "vz"]
+
+    def __init__(self, gas, dims, ni, nj, nk, dx, dy, dz=1.0):
+        self.gas = gas
+        self.dims = dims
+        self.shape = (nk if dims == 3 else 1, nj, ni)
+        self.ncells = int(np.prod(self.shape))
+        self.h = lib().ref_block_new(gas.h, dims, ni, nj, nk if dims == 3 else 1, dx, dy, dz)
+        self.nq = lib().ref_block_nq(self.h)
+
+    def __del__(self):
+        try:
+            lib().ref_block_free(self.h)
+        except Exception:
+            pass
+
+    def set_config(self, flux_calc=AUSMDV, update_scheme=PC, interp_order=2, apply_limiter=1,
+                   extrema_clipping=1, thermo_interp=INTERP_RHOU, max_invalid_cells=0):
+        lib().ref_block_set_config(self.h, flux_calc, update_scheme, interp_order, apply_limiter,
+                                   extrema_clipping, thermo_interp, max_invalid_cells)
+
+    def set_bc(self, face, kind, fs=None):
+        lib().ref_block_set_bc(self.h, face, kind, _p(_f64(fs)) if fs is not None else None)
+
+    def set_from_pT(self, p, T, vx, vy, vz, massf=None):
+        args = [_f64(np.broadcast_to(a, self.shape)).ravel() for a in (p, T, vx, vy, vz)]
+        mf = _f64(massf).reshape(self.gas.nsp, -1) if massf is not None else None
+        rc = lib().ref_block_set_from_pT(self.h, *[_p(a) for a in args], _p(mf))
+        if rc != 0:
+            raise RuntimeError("set_from_pT failed")
+
+    def set_U(self, U, Tguess=None):
+        U = _f64(U).reshape(self.nq, -1)
+        Tg = _f64(Tguess).ravel() if Tguess is not None else None
+        if lib().ref_block_set_U(self.h, _p(U), _p(Tg)) != 0:
+            raise RuntimeError("decode of U failed")
+
+    def get_U(self, ftl=0):
+        U = np.zeros((self.nq, self.ncells))
+        lib().ref_block_get_U(self.h, ftl, _p(U))
+        return U
+
+    def get_fs(self):
+        out = np.zeros((8 + self.gas.nsp, self.ncells))
+        lib().ref_block_get_fs(self.h, _p(out))
+        return out
+
+    def step(self, dt):
+        nbad = (C.c_int * 3)()
+        rc = lib().ref_block_step(self.h, dt, nbad)
+        return rc, list(nbad)
+
+    def determine_dt(self, dt_current, cfl_value=0.5, check_cfl=True):
+        out = np.zeros(3)
+        lib().ref_block_dt(self.h, dt_current, cfl_value, int(check_cfl), _p(out))
+        return out[0], out[1], out[2]
+
+
+def fs_vector(gas, rho, p, T, u, a, vel, massf=None):
+    v = np.zeros(8 + gas.nsp)
+    v[:8] = [rho, p, T, u, a, vel[0], vel[1], vel[2]]
+    v[8:] = massf if massf is not None else 1.0
+    return v
+
+
+def flowstate_pT(gas, p, T, vel=(0.0, 0.0, 0.0), massf=None):
+    """FlowState from p, T (update_thermo_from_pT + update_sound_speed)."""
+    q = gas.state(p=p, T=T, massf=massf)
+    rc, q = gas.update("pT", q)
+    rc2, q = gas.update("sound_speed", q)
+    assert rc == 0 and rc2 == 0
+    return fs_vector(gas, q[0], q[1], q[2], q[3], q[4], vel, q[5:])
+
+
+class Simulation:
+    """The time loop of simcore.d:791-1131 for one block on the oracle:
+    determine_time_step_size every cfl_count steps (simcore.d:1347-1452),
+    then one explicit increment."""
+
+    def __init__(self, block, cfl_value=0.5, cfl_count=10, dt_init=1.0e-3, dt_max=1.0e-3):
+        self.b = block
+        self.cfl_value, self.cfl_count = cfl_value, cfl_count
+        self.dt_init, self.dt_max = dt_init, dt_max
+        self.dt_global = dt_init
+        self.step_count = 0
+        self.time = 0.0
+
+    def determine_time_step_size(self):
+        if self.step_count % self.cfl_count != 0:
+            return
+        dt_allow, cfl_max, _ = self.b.determine_dt(self.dt_global, self.cfl_value, self.step_count > 0)
+        if self.step_count == 0:
+            dt_allow = min(self.dt_init, dt_allow)
+        if dt_allow <= self.dt_global:
+            self.dt_global = dt_allow
+        else:
+            self.dt_global = min(self.dt_global * 1.5, dt_allow)
+            self.dt_global = min(self.dt_global, self.dt_max)
+
+    def run(self, max_step=None, max_time=None):
+        while True:
+            self.determine_time_step_size()
+            rc, nbad = self.b.step(self.dt_global)
+            if rc != 0:
+                raise RuntimeError("explicit increment failed rc=%d nbad=%s" % (rc, nbad))
+            self.time = self.time + self.dt_global
+            self.step_count += 1
+            if max_time is not None and self.time >= max_time:
+                break
+            if max_step is not None and self.step_count >= max_step:
+                break
+        return self.step_count
