@@ -1,0 +1,581 @@
+// e4hip_api.hip -- C ABI (include/e4hip.h) of the MI355X explicit
+// gas-dynamic update.  Host bookkeeping of gas models and blocks, device
+// buffers, and the launch sequence of gasdynamic_explicit_increment_with_fixed_grid
+// (eilmer/simcore.d:2303-2952) and determine_time_step_size
+// (eilmer/simcore.d:1347-1452).
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/e4hip.h"
+#include "stage_kernels.h"
+
+namespace {
+
+using e4::IdealGas;
+using e4::L2R2W;
+using e4::Prim;
+
+constexpr int TILE_X = 32;
+constexpr int TILE_Y = 8;
+
+struct GasModelH {
+    IdealGas ideal;
+    double mMass;
+};
+
+struct BlockH {
+    bool alive = false;
+    int gm = -1;
+    int dims = 3, ni = 0, nj = 0, nk = 0;
+    double dx = 0, dy = 0, dz = 0;
+    long ncells = 0, qs = 0;
+    int nq = 5;
+    double* U[3] = {nullptr, nullptr, nullptr};   // U[0] is the current state
+    double* d[2] = {nullptr, nullptr};            // dUdt of stages 1 and 2
+    int bc_type[6] = {0, 0, 0, 0, 0, 0};
+    Prim bc_fs[6] = {};
+};
+
+struct ProfEvent { int cls; hipEvent_t a, b; };
+
+struct State {
+    bool init = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    unsigned int* counters = nullptr;          // per-stage counters, 4 uints per block-stage slot
+    unsigned long long* smax = nullptr;
+    double* last_signal = nullptr;
+    int counter_slots = 0;
+    std::vector<GasModelH> gases;
+    std::vector<BlockH> blocks;
+    E4hipStepConfig cfg{E4HIP_FLUX_AUSMDV, E4HIP_UPDATE_PC, 2, 1, 1, 1, E4HIP_THERMO_RHOU, E4HIP_PRECISION_FP64, 0};
+    bool profiling = false;
+    std::vector<ProfEvent> prof_pending;
+    double prof_ms[2] = {0, 0};
+    long prof_n[2] = {0, 0};
+    std::string err;
+};
+
+State g_state;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_state.err = buf;
+    return code;
+}
+
+#define HIP_TRY(call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) return fail(E4HIP_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+BlockH* get_block(int h)
+{
+    if (h < 0 || h >= (int)g_state.blocks.size() || !g_state.blocks[h].alive) return nullptr;
+    return &g_state.blocks[h];
+}
+
+int nstages(int scheme)
+{
+    switch (scheme) {
+    case E4HIP_UPDATE_EULER: return 1;
+    case E4HIP_UPDATE_PC: case E4HIP_UPDATE_MIDPOINT: return 2;
+    default: return 3;
+    }
+}
+
+// l2r2_prepare (eilmer/onedinterp.d:118-133) for a uniform direction of cell length len
+L2R2W l2r2_weights(double len)
+{
+    double lenL1 = len, lenL0 = len, lenR0 = len, lenR1 = len;
+    L2R2W w;
+    w.lenL0 = lenL0;
+    w.lenR0 = lenR0;
+    w.aL0 = 0.5 * lenL0 / (lenL1 + 2.0 * lenL0 + lenR0);
+    w.aR0 = 0.5 * lenR0 / (lenL0 + 2.0 * lenR0 + lenR1);
+    w.two_over_lenL0_plus_lenL1 = 2.0 / (lenL0 + lenL1);
+    w.two_over_lenR0_plus_lenL0 = 2.0 / (lenR0 + lenL0);
+    w.two_over_lenR1_plus_lenR0 = 2.0 / (lenR1 + lenR0);
+    w.two_lenL0_plus_lenL1 = (2.0 * lenL0 + lenL1);
+    w.two_lenR0_plus_lenR1 = (2.0 * lenR0 + lenR1);
+    return w;
+}
+
+int kchunk_for(const BlockH& b)
+{
+    const char* env = getenv("E4HIP_KCHUNK");
+    if (env && atoi(env) > 0) return atoi(env);
+    long nxy = (long)((b.ni + TILE_X - 1) / TILE_X) * ((b.nj + TILE_Y - 1) / TILE_Y);
+    // aim for >= 2048 workgroups (8 per CU) while marching at least 8 planes
+    long chunks = (2048 + nxy - 1) / nxy;
+    long kc = (b.nk + chunks - 1) / chunks;
+    if (kc < 8) kc = 8;
+    if (kc > b.nk) kc = b.nk;
+    return (int)kc;
+}
+
+int ensure_counters(int slots)
+{
+    if (slots <= g_state.counter_slots) return E4HIP_OK;
+    if (g_state.counters) HIP_TRY(hipFree(g_state.counters));
+    HIP_TRY(hipMalloc(&g_state.counters, sizeof(unsigned int) * 4 * slots));
+    g_state.counter_slots = slots;
+    return E4HIP_OK;
+}
+
+void prof_begin(hipEvent_t* a)
+{
+    if (!g_state.profiling) return;
+    hipEventCreate(a);
+    hipEventRecord(*a, g_state.stream);
+}
+
+void prof_end(int cls, hipEvent_t a)
+{
+    if (!g_state.profiling) return;
+    hipEvent_t b;
+    hipEventCreate(&b);
+    hipEventRecord(b, g_state.stream);
+    g_state.prof_pending.push_back(ProfEvent{cls, a, b});
+}
+
+template <int DIMS, int FLUX>
+void launch_stage_t(const e4::StageArgs& A, dim3 grid)
+{
+    hipLaunchKernelGGL((e4::stage_kernel<TILE_X, TILE_Y, DIMS, FLUX>), grid, dim3(TILE_X * TILE_Y), 0,
+                       g_state.stream, A);
+}
+
+int launch_stage(const BlockH& b, int stage, double dt, unsigned int* counters)
+{
+    const E4hipStepConfig& cfg = g_state.cfg;
+    const int sch = cfg.gasdynamic_update_scheme;
+    const int ns = nstages(sch);
+    e4::StageArgs A;
+    memset(&A, 0, sizeof(A));
+    A.qs = b.qs;
+    A.ni = b.ni; A.nj = b.nj; A.nk = (b.dims == 3) ? b.nk : 1;
+    A.kchunk = (b.dims == 3) ? kchunk_for(b) : 1;
+    A.stage = stage;
+    A.high_order = cfg.interpolation_order > 1;
+    A.apply_limiter = cfg.apply_limiter;
+    A.extrema_clipping = cfg.extrema_clipping;
+    A.counters = counters;
+    A.gas = g_state.gases[b.gm].ideal;
+    for (int f = 0; f < 6; ++f) { A.bc_type[f] = b.bc_type[f]; A.bc_fs[f] = b.bc_fs[f]; }
+    // geometry of the uniform Cartesian cell (the oracle uses the same expressions)
+    if (b.dims == 3) {
+        A.vol_inv = 1.0 / (b.dx * b.dy * b.dz);
+        A.area[0] = b.dy * b.dz; A.area[1] = b.dz * b.dx; A.area[2] = b.dx * b.dy;
+    } else {
+        A.vol_inv = 1.0 / (b.dx * b.dy);
+        A.area[0] = b.dy; A.area[1] = b.dx; A.area[2] = 0.0;
+    }
+    A.w[0] = l2r2_weights(b.dx);
+    A.w[1] = l2r2_weights(b.dy);
+    A.w[2] = l2r2_weights(b.dz);
+    // stage coefficients (fvcell.d:1168-1180, 1254-1266, 1333-1342) and buffers
+    double g1 = 1.0, g2 = 0.0, g3 = 0.0;
+    if (stage == 1) {
+        if (sch == E4HIP_UPDATE_MIDPOINT || sch == E4HIP_UPDATE_CLASSIC_RK3) g1 = 0.5;
+        if (sch == E4HIP_UPDATE_DENMAN_RK3) g1 = 8.0 / 15.0;
+        A.Us = b.U[0]; A.Uold = b.U[0];
+        A.Unew = (ns == 1) ? b.U[1] : b.U[1];
+        A.dout = (ns > 1) ? b.d[0] : nullptr;
+    } else if (stage == 2) {
+        g1 = 0.5; g2 = 0.5;
+        switch (sch) {
+        case E4HIP_UPDATE_MIDPOINT: g1 = 0.0; g2 = 1.0; break;
+        case E4HIP_UPDATE_CLASSIC_RK3: g1 = -1.0; g2 = 2.0; break;
+        case E4HIP_UPDATE_TVD_RK3: g1 = 0.25; g2 = 0.25; break;
+        case E4HIP_UPDATE_DENMAN_RK3: g1 = -17.0 / 60.0; g2 = 5.0 / 12.0; break;
+        default: break;
+        }
+        A.Us = b.U[1];
+        A.d0 = b.d[0];
+        if (ns == 2) {            // pc, midpoint: new state written over U[0] in place
+            A.Uold = b.U[0]; A.Unew = b.U[0]; A.dout = nullptr;
+        } else {
+            A.Uold = (sch == E4HIP_UPDATE_DENMAN_RK3) ? b.U[1] : b.U[0];
+            A.Unew = b.U[2]; A.dout = b.d[1];
+        }
+    } else {
+        g1 = 1.0 / 6.0; g2 = 1.0 / 6.0; g3 = 4.0 / 6.0;
+        if (sch == E4HIP_UPDATE_CLASSIC_RK3) { g1 = 1.0 / 6.0; g2 = 4.0 / 6.0; g3 = 1.0 / 6.0; }
+        if (sch == E4HIP_UPDATE_DENMAN_RK3) { g1 = 0.0; g2 = -5.0 / 12.0; g3 = 3.0 / 4.0; }
+        A.Us = b.U[2];
+        A.d0 = b.d[0]; A.d1 = b.d[1];
+        A.Uold = (sch == E4HIP_UPDATE_DENMAN_RK3) ? b.U[2] : b.U[0];
+        A.Unew = b.U[0];          // U[0] is not read as a stencil in stage 3
+        A.dout = nullptr;
+        if (sch == E4HIP_UPDATE_DENMAN_RK3) A.Unew = b.U[0];
+    }
+    A.dt = dt; A.g1 = g1; A.g2 = g2; A.g3 = g3;
+    A.c1 = dt * g1;
+    const int nk = A.nk;
+    dim3 grid((b.ni + TILE_X - 1) / TILE_X, (b.nj + TILE_Y - 1) / TILE_Y, (nk + A.kchunk - 1) / A.kchunk);
+    hipEvent_t ev = nullptr;
+    prof_begin(&ev);
+    const int flux = cfg.flux_calculator;
+    if (b.dims == 3) {
+        if (flux == E4HIP_FLUX_HLLC) launch_stage_t<3, e4::FLUX_HLLC>(A, grid);
+        else launch_stage_t<3, e4::FLUX_AUSMDV>(A, grid);
+    } else {
+        if (flux == E4HIP_FLUX_HLLC) launch_stage_t<2, e4::FLUX_HLLC>(A, grid);
+        else launch_stage_t<2, e4::FLUX_AUSMDV>(A, grid);
+    }
+    prof_end(0, ev);
+    HIP_TRY(hipGetLastError());
+    return E4HIP_OK;
+}
+
+int alloc_block_buffers(BlockH& b)
+{
+    const size_t bytes = sizeof(double) * (size_t)b.qs * b.nq;
+    const int ns = nstages(g_state.cfg.gasdynamic_update_scheme);
+    if (!b.U[0]) HIP_TRY(hipMalloc(&b.U[0], bytes));
+    if (!b.U[1]) HIP_TRY(hipMalloc(&b.U[1], bytes));
+    if (ns >= 2 && !b.d[0]) HIP_TRY(hipMalloc(&b.d[0], bytes));
+    if (ns >= 3) {
+        if (!b.U[2]) HIP_TRY(hipMalloc(&b.U[2], bytes));
+        if (!b.d[1]) HIP_TRY(hipMalloc(&b.d[1], bytes));
+    }
+    return E4HIP_OK;
+}
+
+int free_block_buffers(BlockH& b)
+{
+    for (auto& p : b.U) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
+    for (auto& p : b.d) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
+    return E4HIP_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int e4hip_init(int device)
+{
+    if (g_state.init) return E4HIP_OK;
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreateWithFlags(&g_state.stream, hipStreamNonBlocking));
+    HIP_TRY(hipMalloc(&g_state.smax, sizeof(unsigned long long)));
+    HIP_TRY(hipMalloc(&g_state.last_signal, sizeof(double)));
+    g_state.device = device;
+    g_state.init = true;
+    int rc = ensure_counters(16);
+    if (rc) return rc;
+    return E4HIP_OK;
+}
+
+int e4hip_finalize(void)
+{
+    if (!g_state.init) return E4HIP_OK;
+    HIP_TRY(hipStreamSynchronize(g_state.stream));
+    for (auto& b : g_state.blocks) if (b.alive) { free_block_buffers(b); b.alive = false; }
+    g_state.blocks.clear();
+    g_state.gases.clear();
+    for (auto& pe : g_state.prof_pending) { hipEventDestroy(pe.a); hipEventDestroy(pe.b); }
+    g_state.prof_pending.clear();
+    if (g_state.counters) HIP_TRY(hipFree(g_state.counters));
+    if (g_state.smax) HIP_TRY(hipFree(g_state.smax));
+    if (g_state.last_signal) HIP_TRY(hipFree(g_state.last_signal));
+    g_state.counters = nullptr; g_state.smax = nullptr; g_state.last_signal = nullptr;
+    g_state.counter_slots = 0;
+    HIP_TRY(hipStreamDestroy(g_state.stream));
+    g_state.stream = nullptr;
+    g_state.init = false;
+    return E4HIP_OK;
+}
+
+int e4hip_last_error(char* buf, int len)
+{
+    if (!buf || len <= 0) return E4HIP_ERR_ARG;
+    snprintf(buf, (size_t)len, "%s", g_state.err.c_str());
+    return E4HIP_OK;
+}
+
+void* e4hip_stream(void) { return (void*)g_state.stream; }
+
+int e4hip_synchronize(void)
+{
+    if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
+    HIP_TRY(hipStreamSynchronize(g_state.stream));
+    return E4HIP_OK;
+}
+
+int e4hip_gas_ideal(double mMass, double gamma, int* gm)
+{
+    if (!(mMass > 0.0) || !(gamma > 1.0) || !gm) return fail(E4HIP_ERR_ARG, "bad ideal-gas parameters");
+    GasModelH g;
+    g.mMass = mMass;
+    g.ideal.Rgas = 8.31451 / mMass;                     // physical_constants.d:13, ideal_gas.d:65
+    g.ideal.gamma = gamma;
+    g.ideal.Cv = g.ideal.Rgas / (gamma - 1.0);
+    g.ideal.Cvinv = 1.0 / g.ideal.Cv;
+    g.ideal.gammaR = gamma * g.ideal.Rgas;
+    g_state.gases.push_back(g);
+    *gm = (int)g_state.gases.size() - 1;
+    return E4HIP_OK;
+}
+
+int e4hip_block_create(int gm, int dims, int ni, int nj, int nk, double dx, double dy, double dz, int* blk)
+{
+    if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
+    if (gm < 0 || gm >= (int)g_state.gases.size()) return fail(E4HIP_ERR_HANDLE, "unknown gas model %d", gm);
+    if (!blk || (dims != 2 && dims != 3)) return fail(E4HIP_ERR_ARG, "dims must be 2 or 3");
+    if (dims == 2) nk = 1;
+    if (ni < 2 || nj < 2 || (dims == 3 && nk < 2))
+        return fail(E4HIP_ERR_ARG, "each active direction needs at least 2 cells");
+    if (!(dx > 0) || !(dy > 0) || (dims == 3 && !(dz > 0))) return fail(E4HIP_ERR_ARG, "cell sizes must be positive");
+    BlockH b;
+    b.alive = true;
+    b.gm = gm; b.dims = dims; b.ni = ni; b.nj = nj; b.nk = nk;
+    b.dx = dx; b.dy = dy; b.dz = (dims == 3) ? dz : 1.0;
+    b.ncells = (long)ni * nj * nk;
+    b.qs = (b.ncells + 63) / 64 * 64;
+    b.nq = 5;
+    int rc = alloc_block_buffers(b);
+    if (rc) { free_block_buffers(b); return rc; }
+    g_state.blocks.push_back(b);
+    *blk = (int)g_state.blocks.size() - 1;
+    return E4HIP_OK;
+}
+
+int e4hip_block_destroy(int blk)
+{
+    BlockH* b = get_block(blk);
+    if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
+    HIP_TRY(hipStreamSynchronize(g_state.stream));
+    int rc = free_block_buffers(*b);
+    b->alive = false;
+    return rc;
+}
+
+int e4hip_block_set_bc(int blk, int face, int type, const double* fs)
+{
+    BlockH* b = get_block(blk);
+    if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
+    if (face < 0 || face > 5 || (b->dims == 2 && face > 3)) return fail(E4HIP_ERR_ARG, "bad face %d", face);
+    if (type < E4HIP_BC_SLIP_WALL || type > E4HIP_BC_PERIODIC) return fail(E4HIP_ERR_ARG, "bad bc type %d", type);
+    if (type == E4HIP_BC_INFLOW) {
+        if (!fs) return fail(E4HIP_ERR_ARG, "inflow needs a flow state");
+        // fs = rho, p, T, u, a, vx, vy, vz
+        Prim P;
+        P.rho = fs[0]; P.u = fs[3]; P.a = fs[4]; P.v0 = fs[5]; P.v1 = fs[6]; P.v2 = fs[7];
+        if (!(P.rho > 0) || !(P.u > 0) || !(P.a > 0)) return fail(E4HIP_ERR_ARG, "invalid inflow state");
+        b->bc_fs[face] = P;
+    }
+    b->bc_type[face] = type;
+    return E4HIP_OK;
+}
+
+int e4hip_block_ncells(int blk, long* ncells, int* nq)
+{
+    BlockH* b = get_block(blk);
+    if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
+    if (ncells) *ncells = b->ncells;
+    if (nq) *nq = b->nq;
+    return E4HIP_OK;
+}
+
+static int run_decode(const BlockH& b, double* fs_dev, unsigned int* nbad)
+{
+    hipLaunchKernelGGL(e4::reset_counters_kernel, dim3(1), dim3(64), 0, g_state.stream, g_state.counters, 4, nullptr);
+    e4::DecodeArgs D;
+    D.U = b.U[0]; D.fs = fs_dev; D.qs = b.qs; D.ncells = b.ncells;
+    D.gas = g_state.gases[b.gm].ideal; D.counters = g_state.counters;
+    hipLaunchKernelGGL(e4::decode_kernel, dim3((unsigned)((b.ncells + 255) / 256)), dim3(256), 0, g_state.stream, D);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(nbad, g_state.counters, sizeof(unsigned int), hipMemcpyDeviceToHost, g_state.stream));
+    HIP_TRY(hipStreamSynchronize(g_state.stream));
+    return E4HIP_OK;
+}
+
+int e4hip_block_upload(int blk, const double* U, const double* Tguess)
+{
+    (void)Tguess;
+    BlockH* b = get_block(blk);
+    if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
+    if (!U) return fail(E4HIP_ERR_ARG, "null U");
+    for (int q = 0; q < b->nq; ++q)
+        HIP_TRY(hipMemcpyAsync(b->U[0] + (size_t)q * b->qs, U + (size_t)q * b->ncells, sizeof(double) * b->ncells,
+                               hipMemcpyHostToDevice, g_state.stream));
+    unsigned int nbad = 0;
+    int rc = run_decode(*b, nullptr, &nbad);
+    if (rc) return rc;
+    if (nbad) return fail(E4HIP_ERR_DECODE, "%u cells of block %d do not decode to a valid state", nbad, blk);
+    return E4HIP_OK;
+}
+
+int e4hip_block_download(int blk, double* U, double* fs)
+{
+    BlockH* b = get_block(blk);
+    if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
+    if (U) {
+        for (int q = 0; q < b->nq; ++q)
+            HIP_TRY(hipMemcpyAsync(U + (size_t)q * b->ncells, b->U[0] + (size_t)q * b->qs, sizeof(double) * b->ncells,
+                                   hipMemcpyDeviceToHost, g_state.stream));
+    }
+    if (fs) {
+        double* fs_dev = nullptr;
+        HIP_TRY(hipMalloc(&fs_dev, sizeof(double) * 8 * b->ncells));
+        unsigned int nbad = 0;
+        int rc = run_decode(*b, fs_dev, &nbad);
+        if (rc) { hipFree(fs_dev); return rc; }
+        HIP_TRY(hipMemcpyAsync(fs, fs_dev, sizeof(double) * 8 * b->ncells, hipMemcpyDeviceToHost, g_state.stream));
+        HIP_TRY(hipStreamSynchronize(g_state.stream));
+        HIP_TRY(hipFree(fs_dev));
+    }
+    HIP_TRY(hipStreamSynchronize(g_state.stream));
+    return E4HIP_OK;
+}
+
+int e4hip_config(const E4hipStepConfig* c)
+{
+    if (!c) return fail(E4HIP_ERR_ARG, "null config");
+    if (c->flux_calculator != E4HIP_FLUX_AUSMDV && c->flux_calculator != E4HIP_FLUX_HLLC)
+        return fail(E4HIP_ERR_ARG, "flux_calculator %d is not available on the device path", c->flux_calculator);
+    if (c->gasdynamic_update_scheme < E4HIP_UPDATE_EULER || c->gasdynamic_update_scheme > E4HIP_UPDATE_DENMAN_RK3)
+        return fail(E4HIP_ERR_ARG, "gasdynamic_update_scheme %d is not available", c->gasdynamic_update_scheme);
+    if (c->interpolation_order != 1 && c->interpolation_order != 2)
+        return fail(E4HIP_ERR_ARG, "interpolation_order must be 1 or 2");
+    if (c->interpolate_in_local_frame != 1) return fail(E4HIP_ERR_ARG, "interpolate_in_local_frame must be true");
+    if (c->thermo_interpolator != E4HIP_THERMO_RHOU) return fail(E4HIP_ERR_ARG, "thermo_interpolator must be rhou");
+    if (c->precision_mode != E4HIP_PRECISION_FP64) return fail(E4HIP_ERR_ARG, "precision_mode %d not available", c->precision_mode);
+    g_state.cfg = *c;
+    for (auto& b : g_state.blocks)
+        if (b.alive) { int rc = alloc_block_buffers(b); if (rc) return rc; }
+    return E4HIP_OK;
+}
+
+int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_cells)
+{
+    if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
+    if (!blks || nblk <= 0) return fail(E4HIP_ERR_ARG, "no blocks");
+    if (!(dt > 0.0)) return fail(E4HIP_ERR_ARG, "dt must be positive");
+    for (int n = 0; n < nblk; ++n)
+        if (!get_block(blks[n])) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blks[n]);
+    const int ns = nstages(g_state.cfg.gasdynamic_update_scheme);
+    int rc = ensure_counters(nblk * 3);
+    if (rc) return rc;
+    const int nslots = nblk * 3;
+    hipLaunchKernelGGL(e4::reset_counters_kernel, dim3(1), dim3(1024), 0, g_state.stream, g_state.counters,
+                       4 * nslots, nullptr);
+    for (int stage = 1; stage <= ns; ++stage) {
+        for (int n = 0; n < nblk; ++n) {
+            BlockH& b = *get_block(blks[n]);
+            rc = launch_stage(b, stage, dt, g_state.counters + 4 * (n * 3 + (stage - 1)));
+            if (rc) return rc;
+        }
+    }
+    for (int n = 0; n < nblk; ++n) {
+        BlockH& b = *get_block(blks[n]);
+        if (ns == 1) std::swap(b.U[0], b.U[1]);   // Euler: the new state is in U[1]
+    }
+    std::vector<unsigned int> h(4 * nslots);
+    HIP_TRY(hipMemcpyAsync(h.data(), g_state.counters, sizeof(unsigned int) * 4 * nslots, hipMemcpyDeviceToHost,
+                           g_state.stream));
+    HIP_TRY(hipStreamSynchronize(g_state.stream));
+    int stage_bad[3] = {0, 0, 0};
+    bool too_many = false, decode_fail = false;
+    for (int n = 0; n < nblk; ++n)
+        for (int s = 0; s < 3; ++s) {
+            unsigned int inv = h[4 * (n * 3 + s)], dec = h[4 * (n * 3 + s) + 1];
+            stage_bad[s] += (int)(inv + dec);
+            if ((int)inv > g_state.cfg.max_invalid_cells) too_many = true;
+            if (dec) decode_fail = true;
+        }
+    if (n_bad_cells) for (int s = 0; s < 3; ++s) n_bad_cells[s] = stage_bad[s];
+    if (decode_fail) return fail(E4HIP_ERR_DECODE, "Bad cell with negative mass.");
+    if (too_many) return fail(E4HIP_ERR_BAD_CELLS, "Too many bad cells; go home.");
+    return E4HIP_OK;
+}
+
+int e4hip_determine_dt(const int* blks, int nblk, double dt_current, double cfl_value, int check_cfl,
+                       double* dt_allow_out, double* cfl_max_out)
+{
+    if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
+    if (!blks || nblk <= 0 || !dt_allow_out || !cfl_max_out) return fail(E4HIP_ERR_ARG, "bad arguments");
+    double cfl_allow = 0.9;   // fluidblock.d:798-803
+    switch (nstages(g_state.cfg.gasdynamic_update_scheme)) { case 1: cfl_allow = 0.9; break; case 2: cfl_allow = 1.2; break; default: cfl_allow = 1.6; }
+    double dt_allow_all = 1.7976931348623157e308, cfl_max_all = 0.0;
+    for (int n = 0; n < nblk; ++n) {
+        BlockH* b = get_block(blks[n]);
+        if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blks[n]);
+        hipLaunchKernelGGL(e4::reset_counters_kernel, dim3(1), dim3(64), 0, g_state.stream, g_state.counters, 0,
+                           g_state.smax);
+        e4::CflArgs C;
+        C.U = b->U[0]; C.qs = b->qs; C.ni = b->ni; C.nj = b->nj; C.nk = b->nk; C.dims = b->dims;
+        C.iLength = b->dx; C.jLength = b->dy; C.kLength = b->dz;
+        C.gas = g_state.gases[b->gm].ideal;
+        C.smax_bits = g_state.smax; C.last_signal = g_state.last_signal;
+        hipEvent_t ev = nullptr;
+        prof_begin(&ev);
+        hipLaunchKernelGGL(e4::cfl_kernel, dim3((unsigned)((b->ncells + 255) / 256)), dim3(256), 0, g_state.stream, C);
+        prof_end(1, ev);
+        HIP_TRY(hipGetLastError());
+        unsigned long long bits = 0;
+        double last = 0.0;
+        HIP_TRY(hipMemcpyAsync(&bits, g_state.smax, sizeof(bits), hipMemcpyDeviceToHost, g_state.stream));
+        HIP_TRY(hipMemcpyAsync(&last, g_state.last_signal, sizeof(last), hipMemcpyDeviceToHost, g_state.stream));
+        HIP_TRY(hipStreamSynchronize(g_state.stream));
+        double smax;
+        memcpy(&smax, &bits, sizeof(smax));
+        // min over cells of cfl/signal == cfl/max(signal) and max of dt*signal ==
+        // dt*max(signal), since rounding is monotone
+        double dt_allow = cfl_value / smax;
+        double cfl_max = dt_current * smax;
+        if (check_cfl && (cfl_max < 0.0 || cfl_max > cfl_allow)) {
+            cfl_max = 0.5 * cfl_allow;
+            dt_allow = cfl_max / last;   // the last cell's signal, as fluidblock.d:856 uses it
+        }
+        dt_allow_all = fmin(dt_allow_all, dt_allow);
+        cfl_max_all = fmax(cfl_max_all, cfl_max);
+    }
+    *dt_allow_out = dt_allow_all;
+    *cfl_max_out = cfl_max_all;
+    return E4HIP_OK;
+}
+
+int e4hip_profile_enable(int on)
+{
+    g_state.profiling = on != 0;
+    return E4HIP_OK;
+}
+
+int e4hip_profile_read(int k, double* total_ms, long* launches)
+{
+    if (k < 0 || k > 1 || !total_ms || !launches) return fail(E4HIP_ERR_ARG, "bad arguments");
+    if (!g_state.prof_pending.empty()) {
+        HIP_TRY(hipStreamSynchronize(g_state.stream));
+        for (auto& pe : g_state.prof_pending) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, pe.a, pe.b);
+            g_state.prof_ms[pe.cls] += ms;
+            g_state.prof_n[pe.cls] += 1;
+            hipEventDestroy(pe.a);
+            hipEventDestroy(pe.b);
+        }
+        g_state.prof_pending.clear();
+    }
+    *total_ms = g_state.prof_ms[k];
+    *launches = g_state.prof_n[k];
+    g_state.prof_ms[k] = 0.0;
+    g_state.prof_n[k] = 0;
+    return E4HIP_OK;
+}
+
+} // extern "C"

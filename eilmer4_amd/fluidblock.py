@@ -1,0 +1,68 @@
+"""Structured fluid block on the device (the SFluidBlock role,
+eilmer/sfluidblock.d:48-126), restricted to uniform Cartesian cells."""
+import ctypes as C
+
+import numpy as np
+
+from . import lib as _lib
+from .fvcore import BCType, Face
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class SFluidBlock:
+    """A block of ni x nj (x nk) cells of size dx x dy (x dz) whose conserved
+    state lives in device memory; every face starts as a slip wall."""
+
+    def __init__(self, gas, dims, ni, nj, nk=1, dx=1.0, dy=1.0, dz=1.0, label=""):
+        self.gas = gas
+        self.dims = dims
+        self.ni, self.nj, self.nk = ni, nj, (nk if dims == 3 else 1)
+        self.dx, self.dy, self.dz = dx, dy, (dz if dims == 3 else 1.0)
+        self.label = label
+        h = C.c_int(-1)
+        _lib.check(_lib.load().e4hip_block_create(gas.handle(), dims, ni, nj, self.nk, dx, dy, self.dz,
+                                                  C.byref(h)))
+        self.id = h.value
+        n = C.c_long(0)
+        nq = C.c_int(0)
+        _lib.check(_lib.load().e4hip_block_ncells(self.id, C.byref(n), C.byref(nq)))
+        self.ncells, self.nq = n.value, nq.value
+        self.bc = {f: BCType.slip_wall for f in Face}
+
+    @property
+    def shape(self):
+        return (self.nk, self.nj, self.ni)
+
+    def set_bc(self, face, kind, fs=None):
+        fsv = None if fs is None else np.ascontiguousarray(fs, dtype=np.float64)
+        _lib.check(_lib.load().e4hip_block_set_bc(self.id, int(face), int(kind),
+                                                  _p(fsv) if fsv is not None else None))
+        self.bc[Face(face)] = BCType(kind)
+
+    def upload(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.nq, self.ncells)
+        _lib.check(_lib.load().e4hip_block_upload(self.id, _p(U), None))
+
+    def set_from_pT(self, p, T, vx, vy, vz):
+        """Initial state from pressure, temperature and velocity per cell
+        (the initial-flow writer's FlowState(p, T), then encode_conserved)."""
+        shp = self.shape
+        p, T, vx, vy, vz = (np.broadcast_to(np.asarray(a, dtype=np.float64), shp).ravel() for a in (p, T, vx, vy, vz))
+        rho, u, _ = self.gas.rho_u_a_from_pT(p, T)
+        U = self.gas.encode(rho, u, vx, vy, vz)
+        self.upload(U)
+        return U
+
+    def download(self, with_fs=False):
+        U = np.empty((self.nq, self.ncells))
+        fs = np.empty((8, self.ncells)) if with_fs else None
+        _lib.check(_lib.load().e4hip_block_download(self.id, _p(U), _p(fs) if fs is not None else None))
+        return (U, fs) if with_fs else U
+
+    def destroy(self):
+        if self.id >= 0:
+            _lib.check(_lib.load().e4hip_block_destroy(self.id))
+            self.id = -1

@@ -1,0 +1,88 @@
+"""ctypes binding of libe4hip.so (the C ABI declared in include/e4hip.h).
+
+The library is built in-tree by ``eilmer4_amd.build`` (hipcc, gfx950).  There
+is no CPU fallback: if the library is missing or cannot be loaded, every entry
+point raises.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libe4hip.so")
+
+E4HIP_OK = 0
+ERRORS = {-1: "E4HIP_ERR_ARG", -2: "E4HIP_ERR_HIP", -3: "E4HIP_ERR_HANDLE", -4: "E4HIP_ERR_BAD_CELLS",
+          -5: "E4HIP_ERR_DECODE", -6: "E4HIP_ERR_NOT_INIT"}
+
+# every symbol include/e4hip.h declares
+EXPORTS = [
+    "e4hip_init", "e4hip_finalize", "e4hip_last_error", "e4hip_stream", "e4hip_synchronize",
+    "e4hip_gas_ideal", "e4hip_block_create", "e4hip_block_destroy", "e4hip_block_set_bc",
+    "e4hip_block_upload", "e4hip_block_download", "e4hip_block_ncells", "e4hip_config",
+    "e4hip_explicit_increment", "e4hip_determine_dt", "e4hip_profile_enable", "e4hip_profile_read",
+]
+
+
+class E4hipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code
+
+
+class StepConfig(C.Structure):
+    """E4hipStepConfig (include/e4hip.h)."""
+    _fields_ = [("flux_calculator", C.c_int), ("gasdynamic_update_scheme", C.c_int),
+                ("interpolation_order", C.c_int), ("apply_limiter", C.c_int),
+                ("extrema_clipping", C.c_int), ("interpolate_in_local_frame", C.c_int),
+                ("thermo_interpolator", C.c_int), ("precision_mode", C.c_int),
+                ("max_invalid_cells", C.c_int)]
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def load(path=LIB_PATH):
+    """Load libe4hip.so (no build, no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libe4hip.so is not built (%s); run eilmer4_amd.build.build()" % path)
+    L = C.CDLL(path)
+    L.e4hip_init.argtypes = [C.c_int]
+    L.e4hip_last_error.argtypes = [C.c_char_p, C.c_int]
+    L.e4hip_stream.restype = C.c_void_p
+    L.e4hip_gas_ideal.argtypes = [C.c_double, C.c_double, _ip]
+    L.e4hip_block_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                     C.c_double, _ip]
+    L.e4hip_block_destroy.argtypes = [C.c_int]
+    L.e4hip_block_set_bc.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
+    L.e4hip_block_upload.argtypes = [C.c_int, _dp, _dp]
+    L.e4hip_block_download.argtypes = [C.c_int, _dp, _dp]
+    L.e4hip_block_ncells.argtypes = [C.c_int, C.POINTER(C.c_long), _ip]
+    L.e4hip_config.argtypes = [C.POINTER(StepConfig)]
+    L.e4hip_explicit_increment.argtypes = [_ip, C.c_int, C.c_double, _ip]
+    L.e4hip_determine_dt.argtypes = [_ip, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
+    L.e4hip_profile_enable.argtypes = [C.c_int]
+    L.e4hip_profile_read.argtypes = [C.c_int, _dp, C.POINTER(C.c_long)]
+    _lib = L
+    return L
+
+
+def init(device=0):
+    """e4hip_init on first use (idempotent in the library)."""
+    check(load().e4hip_init(device))
+
+
+def last_error():
+    buf = C.create_string_buffer(512)
+    load().e4hip_last_error(buf, 512)
+    return buf.value.decode()
+
+
+def check(rc):
+    if rc != E4HIP_OK:
+        raise E4hipError(rc, last_error())
+    return rc

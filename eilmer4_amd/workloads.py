@@ -1,0 +1,108 @@
+"""Synthetic initial fields of the benchmark workloads (primitive p, T, vel per
+cell) and their boundary conditions.  Seeds are fixed so that every run,
+CPU or GPU, starts from identical inputs.
+
+planar_bump_2d   2D 128x64 cells on 1.0 m x 0.5 m, ideal air, uniform
+                 p=100 kPa, T=300 K, Mach 2 in +x, plus a Gaussian bump
+                 (+10 % density and pressure, radius 0.05 m) centred in the
+                 left third; supersonic inflow west, extrapolated outflow
+                 east, slip walls north/south.
+shocked_box_3d   3D unit cube, n^3 cells, ideal air; smooth field from random
+                 Fourier modes with 1 <= |k| <= 8 (amplitudes ~ |k|^-2) mapped
+                 into density factor [0.5, 2], velocity perturbation
+                 [-300, 300] m/s and pressure factor [0.5, 2], multiplied onto
+                 a planar Mach-3 normal shock at x = 0.3 (Rankine-Hugoniot
+                 jump from rho=1 kg/m^3, p=100 kPa); periodic y/z, supersonic
+                 inflow west, extrapolated outflow east.
+"""
+import numpy as np
+
+from .fvcore import BCType, Face
+
+SEED_BUMP_2D = 20261201
+SEED_BOX_3D = 20261202
+
+
+def planar_bump_2d(gas, ni=128, nj=64, Lx=1.0, Ly=0.5, seed=SEED_BUMP_2D):
+    rng = np.random.default_rng(seed)
+    cx = rng.uniform(0.0, Lx / 3.0)
+    cy = rng.uniform(0.0, Ly)
+    dx, dy = Lx / ni, Ly / nj
+    x = (np.arange(ni) + 0.5) * dx
+    y = (np.arange(nj) + 0.5) * dy
+    X, Y = np.meshgrid(x, y, indexing="xy")          # [nj][ni]
+    p0, T0 = 100.0e3, 300.0
+    a0 = float(np.sqrt(gas.gamma * gas.Rgas * T0))
+    u0 = 2.0 * a0
+    r2 = (X - cx) ** 2 + (Y - cy) ** 2
+    bump = 0.1 * np.exp(-r2 / 0.05 ** 2)
+    rho0 = p0 / (T0 * gas.Rgas)
+    rho = rho0 * (1.0 + bump)
+    p = p0 * (1.0 + bump)
+    T = p / (rho * gas.Rgas)
+    shape = (1, nj, ni)
+    fields = dict(p=p.reshape(shape), T=T.reshape(shape), vx=np.full(shape, u0), vy=np.zeros(shape),
+                  vz=np.zeros(shape))
+    inflow = gas.flowstate(p0, T0, (u0, 0.0, 0.0))
+    bcs = {Face.west: (BCType.inflow, inflow), Face.east: (BCType.extrapolate, None),
+           Face.north: (BCType.slip_wall, None), Face.south: (BCType.slip_wall, None)}
+    geom = dict(dims=2, ni=ni, nj=nj, nk=1, dx=dx, dy=dy, dz=1.0)
+    return geom, fields, bcs, dict(centre=(cx, cy), a0=a0, u0=u0)
+
+
+def _smooth_field(n, rng, kmax=8):
+    """Real field on an n^3 periodic grid: sum of modes 1 <= |k| <= kmax with
+    amplitude |k|^-2 and uniform random phase, scaled to [-1, 1]."""
+    k = np.fft.fftfreq(n, d=1.0 / n)
+    kz = np.fft.rfftfreq(n, d=1.0 / n)
+    KX, KY, KZ = np.meshgrid(k, k, kz, indexing="ij")
+    kk = np.sqrt(KX ** 2 + KY ** 2 + KZ ** 2)
+    mask = (kk >= 1.0) & (kk <= kmax)
+    spec = np.zeros(KX.shape, dtype=np.complex128)
+    phase = rng.uniform(0.0, 2.0 * np.pi, size=int(mask.sum()))
+    spec[mask] = (kk[mask] ** -2.0) * np.exp(1j * phase)
+    f = np.fft.irfftn(spec, s=(n, n, n), axes=(0, 1, 2))   # indexed [x][y][z]
+    f = f - f.min()
+    f = 2.0 * f / f.max() - 1.0
+    return np.ascontiguousarray(np.transpose(f, (2, 1, 0)))   # -> [k][j][i]
+
+
+def shocked_box_3d(gas, n=256, seed=SEED_BOX_3D, x_shock=0.3, mach=3.0):
+    rng = np.random.default_rng(seed)
+    s_rho, s_p, s_u, s_v, s_w = (_smooth_field(n, rng) for _ in range(5))
+    g = gas.gamma
+    rho1, p1 = 1.0, 100.0e3
+    a1 = float(np.sqrt(g * p1 / rho1))
+    u1 = mach * a1
+    # Rankine-Hugoniot ratios across a normal shock (the relations of gasdyn/idealgasflow.d:115-186)
+    M2 = mach * mach
+    r_rho = (g + 1.0) * M2 / ((g - 1.0) * M2 + 2.0)
+    r_p = 1.0 + 2.0 * g / (g + 1.0) * (M2 - 1.0)
+    rho2, p2, u2 = rho1 * r_rho, p1 * r_p, u1 / r_rho
+    dx = 1.0 / n
+    x = (np.arange(n) + 0.5) * dx
+    up = (x < x_shock)[None, None, :]
+    f_rho = 1.25 + 0.75 * s_rho          # [0.5, 2.0]
+    f_p = 1.25 + 0.75 * s_p              # [0.5, 2.0]
+    rho = np.where(up, rho1, rho2) * f_rho
+    p = np.where(up, p1, p2) * f_p
+    T = p / (rho * gas.Rgas)
+    vx = np.where(up, u1, u2) + 300.0 * s_u
+    vy = 300.0 * s_v
+    vz = 300.0 * s_w
+    fields = dict(p=p, T=T, vx=vx, vy=vy, vz=vz)
+    T1 = p1 / (rho1 * gas.Rgas)
+    inflow = gas.flowstate(p1, T1, (u1, 0.0, 0.0))
+    bcs = {Face.west: (BCType.inflow, inflow), Face.east: (BCType.extrapolate, None),
+           Face.north: (BCType.periodic, None), Face.south: (BCType.periodic, None),
+           Face.top: (BCType.periodic, None), Face.bottom: (BCType.periodic, None)}
+    geom = dict(dims=3, ni=n, nj=n, nk=n, dx=dx, dy=dx, dz=dx)
+    return geom, fields, bcs, dict(u1=u1, u2=u2, rho2=rho2, p2=p2)
+
+
+def initial_U(gas, fields):
+    """Conserved state [5][ncells] of the primitive fields (FlowState from p, T;
+    encode_conserved)."""
+    p, T = fields["p"].ravel(), fields["T"].ravel()
+    rho, u, _ = gas.rho_u_a_from_pT(p, T)
+    return gas.encode(rho, u, fields["vx"], fields["vy"], fields["vz"])

@@ -1,0 +1,160 @@
+/* e4hip.h -- C ABI of the MI355X explicit gas-dynamic update for structured
+ * fluid blocks.  All entry points are extern "C", take plain pointers and
+ * sizes, return an int status (0 = E4HIP_OK, < 0 = error; text via
+ * e4hip_last_error) and never throw across the boundary.
+ *
+ * This is the drop-in for Eilmer4's
+ *   gasdynamic_explicit_increment_with_fixed_grid()  (eilmer/simcore.d:2303-2952)
+ * on one GPU, together with the pieces of its callers that touch block data:
+ *   determine_time_step_size()                       (eilmer/simcore.d:1347-1452,
+ *                                                      eilmer/fluidblock.d:776-862)
+ *   block upload after init_simulation's decode      (eilmer/simcore.d:313-331)
+ *   block download before write_solution_files       (eilmer/simcore.d:637-684)
+ * The FFI precedent it follows is the reference's own GPU chemistry binding
+ * (eilmer/cuda_gpu_chem.d:23-27: extern(C) launcher, plain double* arrays) and
+ * the handle/int-status convention of gas/gas_cwrap.d:35-140.
+ *
+ * Array layouts: a block has ni*nj*nk active cells (nk = 1 in 2D), index
+ * c = (k*nj + j)*ni + i (the reference's ijk order without the ghost layers,
+ * eilmer/sfluidblock.d:205-209).  Conserved arrays are structure-of-arrays
+ * [nq][ncells] with q = mass, x-momentum, y-momentum, z-momentum, total
+ * energy, then rho*Y_s for each species of a multi-species gas
+ * (eilmer/conservedquantities.d:18-50).  Flow-state arrays are
+ * [8 + nsp][ncells]: rho, p, T, u, a, vel.x, vel.y, vel.z, massf[...]
+ * (eilmer/flowstate.d:30-44, gas/gas_state.d:18-47).
+ */
+#ifndef E4HIP_H
+#define E4HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes */
+#define E4HIP_OK               0
+#define E4HIP_ERR_ARG         -1  /* bad argument or unsupported combination */
+#define E4HIP_ERR_HIP         -2  /* a HIP runtime call failed */
+#define E4HIP_ERR_HANDLE      -3  /* unknown gas-model or block handle */
+#define E4HIP_ERR_BAD_CELLS   -4  /* invalid cells above max_invalid_cells
+                                     ("Too many bad cells", simcore.d:2480-2493) */
+#define E4HIP_ERR_DECODE      -5  /* decode_conserved would throw: non-positive
+                                     density or failed thermo (fvcell.d:570-589, 705-719) */
+#define E4HIP_ERR_NOT_INIT    -6  /* e4hip_init not called */
+
+/* ---- FluxCalculator (eilmer/fvcore.d:184-194; values kept, hllc appended) */
+#define E4HIP_FLUX_AUSMDV      0
+#define E4HIP_FLUX_HLLC        9
+
+/* ---- GasdynamicUpdate (eilmer/fvcore.d:30-41) */
+#define E4HIP_UPDATE_EULER       0
+#define E4HIP_UPDATE_PC          1
+#define E4HIP_UPDATE_MIDPOINT    2
+#define E4HIP_UPDATE_CLASSIC_RK3 3
+#define E4HIP_UPDATE_TVD_RK3     4
+#define E4HIP_UPDATE_DENMAN_RK3  5
+
+/* ---- InterpolateOption (eilmer/fvcore.d:155); the device path implements rhou,
+ *      the reference default (globalconfig.d:565) */
+#define E4HIP_THERMO_RHOU      1
+
+/* ---- ghost-cell boundary effects (eilmer/bc/ghost_cell_effect/) */
+#define E4HIP_BC_SLIP_WALL     0  /* WallBC_WithSlip: internal_copy_then_reflect.d:94-310 */
+#define E4HIP_BC_INFLOW        1  /* InFlowBC_Supersonic: flow_state_copy.d:66-169 */
+#define E4HIP_BC_EXTRAPOLATE   2  /* OutFlowBC_SimpleExtrapolate x_order 0: extrapolate_copy.d:82- */
+#define E4HIP_BC_PERIODIC      3  /* ExchangeBC_FullFace of a block with itself across
+                                     opposite faces: full_face_copy.d:1277-1357 */
+
+/* ---- face names (eilmer/fvcore.d Face enum order) */
+#define E4HIP_FACE_NORTH  0
+#define E4HIP_FACE_EAST   1
+#define E4HIP_FACE_SOUTH  2
+#define E4HIP_FACE_WEST   3
+#define E4HIP_FACE_TOP    4
+#define E4HIP_FACE_BOTTOM 5
+
+/* ---- arithmetic of the flux path */
+#define E4HIP_PRECISION_FP64   0  /* everything in FP64 (the plain form) */
+
+/* Step configuration: the LocalConfig fields the path reads
+ * (eilmer/globalconfig.d:440-569, 875, 1008). */
+typedef struct E4hipStepConfig {
+    int flux_calculator;             /* E4HIP_FLUX_*; reference default ausmdv here */
+    int gasdynamic_update_scheme;    /* E4HIP_UPDATE_*; default pc */
+    int interpolation_order;         /* 1 or 2; default 2 */
+    int apply_limiter;               /* van Albada on/off; default 1 */
+    int extrema_clipping;            /* default 1 */
+    int interpolate_in_local_frame;  /* must be 1 (default) */
+    int thermo_interpolator;         /* must be E4HIP_THERMO_RHOU (default) */
+    int precision_mode;              /* E4HIP_PRECISION_* */
+    int max_invalid_cells;           /* default 0 */
+} E4hipStepConfig;
+
+/* Library set-up on one GPU; creates the library's HIP stream.
+ * Replaces nothing in the reference (the D runtime has no device set-up). */
+int e4hip_init(int device);
+int e4hip_finalize(void);
+/* Copies the last error text (NUL-terminated, truncated to len). */
+int e4hip_last_error(char* buf, int len);
+/* The hipStream_t all work is queued on (for event timing by callers). */
+void* e4hip_stream(void);
+int e4hip_synchronize(void);
+
+/* Gas models.  Ideal gas from molecular mass and gamma
+ * (gas/ideal_gas.d:31-72, R = 8.31451/mMass). */
+int e4hip_gas_ideal(double mMass, double gamma, int* gm);
+
+/* Blocks.  Uniform Cartesian cells of size dx*dy*dz (dz ignored for dims 2);
+ * replaces SFluidBlock construction + compute_primary_cell_geometric_data
+ * (eilmer/sfluidblock.d:48-126, 650-842) for axis-aligned grids.  Every face
+ * starts as a slip wall (prep.lua:535, 551 default). Each active direction
+ * needs at least 2 cells. */
+int e4hip_block_create(int gm, int dims, int ni, int nj, int nk,
+                       double dx, double dy, double dz, int* blk);
+int e4hip_block_destroy(int blk);
+/* Boundary condition of one face (E4HIP_FACE_*, E4HIP_BC_*).  For
+ * E4HIP_BC_INFLOW, fs gives the ghost flow state [8 + nsp] as above; else
+ * fs may be NULL.  Periodic faces must be set on both opposite faces.
+ * Replaces BoundaryCondition.applyPreReconAction for these effects
+ * (eilmer/bc/boundary_condition.d:274-292) and the exchange of
+ * simcore.d:1599-1668 for a block connected to itself. */
+int e4hip_block_set_bc(int blk, int face, int type, const double* fs);
+/* Host -> device copy of U[0] ([nq][ncells]); the flow state is the decode of
+ * U (fvcell.d:563-732).  Tguess is reserved for multi-species gases and may be
+ * NULL. Returns E4HIP_ERR_DECODE if some cell does not decode to a valid state. */
+int e4hip_block_upload(int blk, const double* U, const double* Tguess);
+/* Device -> host copy of U[0] and (if fs != NULL) the decoded flow state. */
+int e4hip_block_download(int blk, double* U, double* fs);
+int e4hip_block_ncells(int blk, long* ncells, int* nq);
+
+/* Configuration shared by all blocks (GlobalConfig/LocalConfig fields). */
+int e4hip_config(const E4hipStepConfig* c);
+
+/* gasdynamic_explicit_increment_with_fixed_grid() (simcore.d:2303-2952) for
+ * the listed blocks and time step dt: every stage of the configured scheme,
+ * ending with U[0] holding the new state (the U swap of simcore.d:2936-2941).
+ * n_bad_cells (may be NULL) receives, per stage (3 ints), the invalid-cell
+ * count summed over the blocks (count_invalid_cells, fluidblock.d:433-499).
+ * Returns E4HIP_ERR_BAD_CELLS when a stage's count of any block exceeds
+ * max_invalid_cells and E4HIP_ERR_DECODE when a decode would throw; as in the
+ * reference, the caller then ends the run. */
+int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_cells);
+
+/* determine_time_step_size for the listed blocks (non-stringent CFL,
+ * fvcell.d:1624-1705; fluidblock.d:776-862): dt_allow = min over cells of
+ * cfl_value/signal and cfl_max = max over cells of dt_current*signal,
+ * reduced over blocks (simcore.d:1385-1394), with the reference's rescaling
+ * when check_cfl and cfl_max exceeds cfl_allow. */
+int e4hip_determine_dt(const int* blks, int nblk, double dt_current, double cfl_value,
+                       int check_cfl, double* dt_allow, double* cfl_max);
+
+/* Per-kernel timing with HIP events on the library stream (off by default).
+ * e4hip_profile_read returns, for kernel class k (0 = stage kernels,
+ * 1 = CFL kernel), the summed milliseconds and launch count since the last
+ * read, and resets them. */
+int e4hip_profile_enable(int on);
+int e4hip_profile_read(int k, double* total_ms, long* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* E4HIP_H */

@@ -1,0 +1,76 @@
+"""Shared set-up for oracle-vs-device comparisons: the same geometry, boundary
+conditions and initial conserved state go to the CPU oracle and to the device
+library."""
+import numpy as np
+
+import pyoracle as O
+import eilmer4_amd as e4
+from eilmer4_amd import workloads
+
+
+def oracle_block(geom, bcs, U0, flux=O.AUSMDV, scheme=O.PC, order=2, limiter=1, clipping=1,
+                 gas=None, max_invalid_cells=0):
+    g = gas or O.Gas.ideal()
+    b = O.Block(g, geom["dims"], geom["ni"], geom["nj"], geom["nk"], geom["dx"], geom["dy"], geom["dz"])
+    b.set_config(flux_calc=int(flux), update_scheme=int(scheme), interp_order=order, apply_limiter=limiter,
+                 extrema_clipping=clipping, max_invalid_cells=max_invalid_cells)
+    for face, (kind, fs) in bcs.items():
+        b.set_bc(int(face), int(kind), fs)
+    b.set_U(U0)
+    return b
+
+
+def device_block(gas, geom, bcs, U0):
+    b = e4.SFluidBlock(gas, geom["dims"], geom["ni"], geom["nj"], geom["nk"], geom["dx"], geom["dy"], geom["dz"])
+    for face, (kind, fs) in bcs.items():
+        b.set_bc(face, kind, fs)
+    b.upload(U0)
+    return b
+
+
+def device_sim(blocks, flux=e4.FluxCalculator.ausmdv, scheme=e4.GasdynamicUpdate.pc, order=2, limiter=True,
+               clipping=True, cfl=0.5, max_invalid_cells=0):
+    cfg = e4.GlobalConfig(flux_calculator=flux, gasdynamic_update_scheme=scheme, interpolation_order=order,
+                          apply_limiter=limiter, extrema_clipping=clipping, cfl_value=cfl,
+                          max_invalid_cells=max_invalid_cells)
+    return e4.Simulation(blocks, cfg)
+
+
+def rel_err(A, B):
+    """max over conserved quantities of max|A_q - B_q| / max|B_q|."""
+    A = np.asarray(A).reshape(A.shape[0], -1)
+    B = np.asarray(B).reshape(B.shape[0], -1)
+    out = []
+    for q in range(A.shape[0]):
+        scale = np.max(np.abs(B[q]))
+        if scale == 0.0:
+            scale = 1.0
+        out.append(np.max(np.abs(A[q] - B[q])) / scale)
+    return np.array(out)
+
+
+def run_pair(geom, fields, bcs, nsteps, flux="ausmdv", scheme="pc", order=2, limiter=True, clipping=True,
+             cfl=0.5, check_every=None):
+    """Advance the oracle and the device from the same U0 with their own dt
+    policies; returns (U_oracle, U_device, dts_oracle, dts_device, checkpoints)."""
+    gas = e4.IdealGas()
+    U0 = workloads.initial_U(gas, fields)
+    fc = e4.flux_calculator_from_name(flux)
+    us = e4.update_scheme_from_name(scheme)
+    ob = oracle_block(geom, bcs, U0, flux=int(fc), scheme=int(us), order=order, limiter=int(limiter),
+                      clipping=int(clipping))
+    osim = O.Simulation(ob, cfl_value=cfl)
+    db = device_block(gas, geom, bcs, U0)
+    dsim = device_sim([db], flux=fc, scheme=us, order=order, limiter=limiter, clipping=clipping, cfl=cfl)
+    dts_o, dts_d, checkpoints = [], [], []
+    for s in range(nsteps):
+        osim.run(max_step=s + 1)
+        dts_o.append(osim.dt_global)
+        dsim.integrate_in_time(target_time=1e30, max_step=s + 1)
+        dts_d.append(dsim.state.dt_global)
+        if check_every and (s + 1) % check_every == 0:
+            checkpoints.append((s + 1, ob.get_U(), db.download()))
+    Uo = ob.get_U()
+    Ud = db.download()
+    db.destroy()
+    return Uo, Ud, np.array(dts_o), np.array(dts_d), checkpoints
