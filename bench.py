@@ -45,10 +45,10 @@ def read_traffic(name):
     return None
 
 
-def cpu_baseline(n_sample=64, steps=2):
+def cpu_baseline(n_sample=128, min_seconds=10.0, max_steps=100):
     """The CPU oracle (a port of the reference algorithm) on a bounded sample of
-    the same workload: the shocked-box field at n_sample^3 cells, `steps` RK2
-    steps, all host threads OpenMP gives it."""
+    the same workload: the shocked-box field at n_sample^3 cells, RK2 steps
+    until min_seconds of CPU work (at least 2), all host threads OpenMP gives it."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     from eilmer4_amd import IdealGas, workloads
@@ -63,9 +63,11 @@ def cpu_baseline(n_sample=64, steps=2):
     dt_allow, _, _ = b.determine_dt(1.0e-3, 0.5, False)
     b.step(dt_allow)                      # warm-up
     t0 = time.perf_counter()
-    for _ in range(steps):
+    steps = 0
+    while steps < 2 or (time.perf_counter() - t0 < min_seconds and steps < max_steps):
         rc, _ = b.step(dt_allow)
         assert rc == 0
+        steps += 1
     el = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return dict(value=n_sample ** 3 * steps / el / 1e6, unit="Mcell-updates/s", cores=cores, kind="port",
@@ -79,7 +81,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--flux", default="ausmdv,hllc", help="comma list of flux calculators to run")
+    ap.add_argument("--precision", default="fp64,mixed,fp64_exact",
+                    help="comma list of precision modes; the first is the headline")
+    ap.add_argument("--acc-steps", type=int, default=20, help="steps of the mixed-vs-fp64 accuracy check")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (HBM-counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": n * 8, "calibrate_copy_bytes_written": n * 8}))
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (requested --gpus %d)" % args.gpus)
         sys.exit(2)
@@ -98,15 +116,20 @@ def main():
     ncells = geom["ni"] * geom["nj"] * geom["nk"]
     stream = torch.cuda.ExternalStream(L.e4hip_stream())
 
-    results = {}
-    for flux in ("ausmdv", "hllc"):
+    def make_sim(flux, precision, U):
         blk = e4.SFluidBlock(gas, 3, geom["ni"], geom["nj"], geom["nk"], geom["dx"], geom["dy"], geom["dz"])
         for face, (kind, fs) in bcs.items():
             blk.set_bc(face, kind, fs)
-        blk.upload(U0)
-        cfg = e4.GlobalConfig(flux_calculator=e4.flux_calculator_from_name(flux), cfl_value=0.5)
-        sim = e4.Simulation([blk], cfg)
-        log("%s: warmup %d steps" % (flux, args.warmup))
+        blk.upload(U)
+        cfg = e4.GlobalConfig(flux_calculator=e4.flux_calculator_from_name(flux), cfl_value=0.5,
+                              precision_mode=e4.PrecisionMode[precision])
+        return blk, e4.Simulation([blk], cfg)
+
+    results = {}
+    runs = [(f, p) for p in args.precision.split(",") for f in args.flux.split(",")]
+    for flux, precision in runs:
+        blk, sim = make_sim(flux, precision, U0)
+        log("%s/%s: warmup %d steps" % (flux, precision, args.warmup))
         sim.integrate_in_time(target_time=1e30, max_step=args.warmup)
         torch.cuda.synchronize()
         ev0 = torch.cuda.Event(enable_timing=True)
@@ -127,17 +150,39 @@ def main():
         e4.lib.check(L.e4hip_profile_read(1, C.byref(tot), C.byref(cnt)))
         cfl_ms = tot.value / max(cnt.value, 1)
         L.e4hip_profile_enable(0)
-        results[flux] = dict(wall=wall, ev_ms=ev_ms, stage_ms=stage_ms, cfl_ms=cfl_ms,
-                             steps_done=sim.state.step, dt=sim.state.dt_global)
-        log("%s: %d steps in %.3f s (events %.3f ms), stage launch %.3f ms, cfl %.3f ms" %
-            (flux, args.steps, wall, ev_ms, stage_ms, cfl_ms))
+        results[(flux, precision)] = dict(wall=wall, ev_ms=ev_ms, stage_ms=stage_ms, cfl_ms=cfl_ms,
+                                          steps_done=sim.state.step, dt=sim.state.dt_global)
+        log("%s/%s: %d steps in %.3f s (events %.3f ms), stage launch %.3f ms, cfl %.3f ms" %
+            (flux, precision, args.steps, wall, ev_ms, stage_ms, cfl_ms))
         blk.destroy()
 
-    r = results["ausmdv"]
+    # accuracy of the mixed form against the plain FP64 form: same U0, same
+    # fixed dt, `acc_steps` steps each
+    accuracy = {}
+    if "mixed" in args.precision.split(",") and args.acc_steps > 0:
+        dt_fixed = results[runs[0]]["dt"]
+        finals = {}
+        for precision in ("fp64", "mixed"):
+            blk, sim = make_sim("ausmdv", precision, U0)
+            sim.cfg.fixed_time_step = True
+            sim.state.dt_global = dt_fixed
+            sim.integrate_in_time(target_time=1e30, max_step=args.acc_steps)
+            finals[precision] = blk.download()
+            blk.destroy()
+        a, b = finals["mixed"], finals["fp64"]
+        rel = [float(np.max(np.abs(a[q] - b[q])) / np.max(np.abs(b[q]))) for q in range(5)]
+        accuracy = {"steps": args.acc_steps, "flux": "ausmdv", "vs": "plain FP64 device run",
+                    "rho_linf_rel": rel[0], "momentum_linf_rel": max(rel[1:4]), "energy_linf_rel": rel[4]}
+        log("mixed vs fp64 after %d steps: %s" % (args.acc_steps, accuracy))
+        del finals, a, b
+
+    head = (args.flux.split(",")[0], args.precision.split(",")[0])
+    r = results[head]
     ms_per_step = r["wall"] * 1e3 / args.steps
     value = ncells * args.steps / r["wall"] / 1e6
-    # dominant kernel: the fused stage kernel; its algorithmic bytes per launch
-    # are half of a cell-step's 200 B (two stage launches per step)
+    # dominant kernel: the fused stage kernel of the headline variant; its
+    # algorithmic bytes per launch are half of a cell-step's 200 B (two stage
+    # launches per step)
     alg_bytes_launch = ALG_BYTES_PER_CELL_STEP / 2.0 * ncells
     achieved = alg_bytes_launch / (r["stage_ms"] * 1e-3) / 1e9
     traffic = read_traffic("stage_kernel_256cube_traffic.txt")
@@ -157,12 +202,16 @@ def main():
         "config": {"workload": "3D single block %d^3, ideal air, FP64 plain form, MUSCL van Albada + ausmdv, "
                                "pc (RK2), shocked random-mode field" % args.n,
                    "cells": ncells, "flux_calculator": "ausmdv", "update": "pc", "precision": "fp64"},
-        "hllc": {"value": round(ncells * args.steps / results["hllc"]["wall"] / 1e6, 3),
-                 "ms_per_step": round(results["hllc"]["wall"] * 1e3 / args.steps, 4),
-                 "stage_launch_ms": round(results["hllc"]["stage_ms"], 4)},
+        "variants": {"%s/%s" % k: {"value": round(ncells * args.steps / v["wall"] / 1e6, 3),
+                                  "ms_per_step": round(v["wall"] * 1e3 / args.steps, 4),
+                                  "stage_launch_ms": round(v["stage_ms"], 4),
+                                  "stage_hbm_frac": round(ALG_BYTES_PER_CELL_STEP / 2.0 * ncells /
+                                                          (v["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                     for k, v in results.items()},
+        "mixed_accuracy": accuracy,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "stage_kernel<32,8,3,ausmdv>", "stage_launch_ms": round(r["stage_ms"], 4),
+                     "kernel": "stage_kernel<%s,32,8,3,%s>" % (head[1], head[0]), "stage_launch_ms": round(r["stage_ms"], 4),
                      "cfl_launch_ms": round(r["cfl_ms"], 4),
                      "alg_bytes_per_launch": alg_bytes_launch},
     }

@@ -1,31 +1,51 @@
 """In-tree build of libe4hip.so for gfx950 with hipcc.
 
-FP64 parity build: -ffp-contract=off keeps every multiply and add separately
-rounded so that the device arithmetic reproduces the reference's roundings.
-Packed FP32 instructions are disabled for all code (-packed-fp32-ops).
+Three translation units:
+  stage_exact.hip  -ffp-contract=off: the FP64 stage kernel with the
+                   reference's roundings (bit-identical to the CPU oracle)
+  stage_fast.hip   -ffp-contract=fast: the plain FP64 and mixed FP32/FP64
+                   stage kernels
+  e4hip_api.hip    -ffp-contract=off: C ABI, host bookkeeping, CFL/decode kernels
+Packed FP32 instructions are disabled in all of them (-packed-fp32-ops).
 """
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "e4hip_api.hip")
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_obj")
 OUT = os.path.join(HERE, "libe4hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
-         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result",
-         "-Wno-unused-value"]
+COMMON = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Xclang", "-target-feature", "-Xclang",
+          "-packed-fp32-ops", "-Wno-unused-result", "-Wno-unused-value"]
+UNITS = [("stage_exact.hip", ["-ffp-contract=off"]),
+         ("stage_fast.hip", ["-ffp-contract=fast"]),
+         ("e4hip_api.hip", ["-ffp-contract=off"])]
 
 
-def build(verbose=False):
-    cmd = [HIPCC] + FLAGS + ["-o", OUT + ".tmp", SRC]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout)
-        raise RuntimeError("hipcc failed building libe4hip.so")
+        raise RuntimeError("hipcc failed: %s" % " ".join(cmd))
+    return r.stdout
+
+
+def build(verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    objs, cmds = [], []
+    for src, flags in UNITS:
+        obj = os.path.join(OBJ, src.replace(".hip", ".o"))
+        objs.append(obj)
+        cmds.append([HIPCC] + COMMON + flags + ["-c", "-o", obj, os.path.join(CSRC, src)])
+    with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        list(ex.map(lambda c: _run(c, verbose), cmds))
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT + ".tmp"] + objs, verbose)
     os.replace(OUT + ".tmp", OUT)
     return OUT
 

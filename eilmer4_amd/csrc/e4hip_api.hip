@@ -13,7 +13,7 @@
 #include <vector>
 
 #include "../../include/e4hip.h"
-#include "stage_kernels.h"
+#include "aux_kernels.h"
 
 namespace {
 
@@ -21,8 +21,8 @@ using e4::IdealGas;
 using e4::L2R2W;
 using e4::Prim;
 
-constexpr int TILE_X = 32;
-constexpr int TILE_Y = 8;
+using e4::TILE_X;
+using e4::TILE_Y;
 
 struct GasModelH {
     IdealGas ideal;
@@ -151,13 +151,6 @@ void prof_end(int cls, hipEvent_t a)
     g_state.prof_pending.push_back(ProfEvent{cls, a, b});
 }
 
-template <int DIMS, int FLUX>
-void launch_stage_t(const e4::StageArgs& A, dim3 grid)
-{
-    hipLaunchKernelGGL((e4::stage_kernel<TILE_X, TILE_Y, DIMS, FLUX>), grid, dim3(TILE_X * TILE_Y), 0,
-                       g_state.stream, A);
-}
-
 int launch_stage(const BlockH& b, int stage, double dt, unsigned int* counters)
 {
     const E4hipStepConfig& cfg = g_state.cfg;
@@ -228,13 +221,11 @@ int launch_stage(const BlockH& b, int stage, double dt, unsigned int* counters)
     dim3 grid((b.ni + TILE_X - 1) / TILE_X, (b.nj + TILE_Y - 1) / TILE_Y, (nk + A.kchunk - 1) / A.kchunk);
     hipEvent_t ev = nullptr;
     prof_begin(&ev);
-    const int flux = cfg.flux_calculator;
-    if (b.dims == 3) {
-        if (flux == E4HIP_FLUX_HLLC) launch_stage_t<3, e4::FLUX_HLLC>(A, grid);
-        else launch_stage_t<3, e4::FLUX_AUSMDV>(A, grid);
-    } else {
-        if (flux == E4HIP_FLUX_HLLC) launch_stage_t<2, e4::FLUX_HLLC>(A, grid);
-        else launch_stage_t<2, e4::FLUX_AUSMDV>(A, grid);
+    const int flux = (cfg.flux_calculator == E4HIP_FLUX_HLLC) ? e4::FLUX_HLLC : e4::FLUX_AUSMDV;
+    switch (cfg.precision_mode) {
+    case E4HIP_PRECISION_FP64_EXACT: e4::launch_stage_exact(A, grid, g_state.stream, b.dims, flux); break;
+    case E4HIP_PRECISION_MIXED: e4::launch_stage_mixed(A, grid, g_state.stream, b.dims, flux); break;
+    default: e4::launch_stage_fast64(A, grid, g_state.stream, b.dims, flux); break;
     }
     prof_end(0, ev);
     HIP_TRY(hipGetLastError());
@@ -454,7 +445,8 @@ int e4hip_config(const E4hipStepConfig* c)
         return fail(E4HIP_ERR_ARG, "interpolation_order must be 1 or 2");
     if (c->interpolate_in_local_frame != 1) return fail(E4HIP_ERR_ARG, "interpolate_in_local_frame must be true");
     if (c->thermo_interpolator != E4HIP_THERMO_RHOU) return fail(E4HIP_ERR_ARG, "thermo_interpolator must be rhou");
-    if (c->precision_mode != E4HIP_PRECISION_FP64) return fail(E4HIP_ERR_ARG, "precision_mode %d not available", c->precision_mode);
+    if (c->precision_mode < E4HIP_PRECISION_FP64 || c->precision_mode > E4HIP_PRECISION_MIXED)
+        return fail(E4HIP_ERR_ARG, "precision_mode %d not available", c->precision_mode);
     g_state.cfg = *c;
     for (auto& b : g_state.blocks)
         if (b.alive) { int rc = alloc_block_buffers(b); if (rc) return rc; }
@@ -524,7 +516,9 @@ int e4hip_determine_dt(const int* blks, int nblk, double dt_current, double cfl_
         C.smax_bits = g_state.smax; C.last_signal = g_state.last_signal;
         hipEvent_t ev = nullptr;
         prof_begin(&ev);
-        hipLaunchKernelGGL(e4::cfl_kernel, dim3((unsigned)((b->ncells + 255) / 256)), dim3(256), 0, g_state.stream, C);
+        long nwg = (b->ncells + 255) / 256;
+        if (nwg > 4096) nwg = 4096;
+        hipLaunchKernelGGL(e4::cfl_kernel<e4::ModeExact>, dim3((unsigned)nwg), dim3(256), 0, g_state.stream, C);
         prof_end(1, ev);
         HIP_TRY(hipGetLastError());
         unsigned long long bits = 0;

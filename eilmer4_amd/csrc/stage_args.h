@@ -1,0 +1,42 @@
+// stage_args.h -- launch parameters of the stage kernel, shared by the host
+// API and the kernel translation units.
+#pragma once
+#include "gasdyn_device.h"
+
+namespace e4 {
+
+enum { BC_SLIP_WALL = 0, BC_INFLOW = 1, BC_EXTRAPOLATE = 2, BC_PERIODIC = 3 };
+enum { FACE_NORTH = 0, FACE_EAST = 1, FACE_SOUTH = 2, FACE_WEST = 3, FACE_TOP = 4, FACE_BOTTOM = 5 };
+
+struct StageArgs {
+    const double* __restrict__ Us;    // state the fluxes are evaluated from (stencil)
+    const double* __restrict__ Uold;  // base state of the update (cell-local)
+    const double* __restrict__ d0;    // dUdt of stage 1 (stages 2, 3)
+    const double* __restrict__ d1;    // dUdt of stage 2 (stage 3)
+    double* __restrict__ dout;        // where this stage's dUdt is kept (or null)
+    double* __restrict__ Unew;        // new state (may alias Uold)
+    unsigned int* counters;           // [0] invalid cells, [1] decode failures
+    long qs;                          // stride between conserved quantities
+    int ni, nj, nk, kchunk;
+    int stage;                        // 1, 2 or 3
+    int high_order, apply_limiter, extrema_clipping;
+    double c1;                        // dt*gamma_1 for stage 1
+    double dt, g1, g2, g3;
+    double vol_inv;
+    double area[3];                   // i-, j-, k-face areas
+    L2R2W w[3];
+    IdealGas gas;
+    int bc_type[6];
+    Prim bc_fs[6];                    // inflow ghost states (global frame)
+};
+
+// Launchers, one per arithmetic mode; each lives in its own translation unit
+// (stage_exact.hip is compiled without FMA contraction, stage_fast.hip with it).
+void launch_stage_exact(const StageArgs& A, dim3 grid, hipStream_t stream, int dims, int flux);
+void launch_stage_fast64(const StageArgs& A, dim3 grid, hipStream_t stream, int dims, int flux);
+void launch_stage_mixed(const StageArgs& A, dim3 grid, hipStream_t stream, int dims, int flux);
+
+constexpr int TILE_X = 32;
+constexpr int TILE_Y = 8;
+
+} // namespace e4

@@ -1,5 +1,5 @@
-// stage_kernels.h -- the fused stage kernel of the explicit update and the CFL
-// signal kernel for ideal-gas structured blocks on gfx950.
+// stage_kernels.h -- the fused stage kernel of the explicit update for
+// ideal-gas structured blocks on gfx950, templated on the arithmetic mode.
 //
 // One launch performs, for every active cell of a block, what
 // gasdynamic_explicit_increment_with_fixed_grid does per stage
@@ -18,41 +18,18 @@
 // the bottom of the next.  The flux divergence, stage update, decode check and
 // the write of the new state all happen in registers.
 #pragma once
-#include "gasdyn_device.h"
+#include "stage_args.h"
 
 namespace e4 {
-
-enum { BC_SLIP_WALL = 0, BC_INFLOW = 1, BC_EXTRAPOLATE = 2, BC_PERIODIC = 3 };
-enum { FACE_NORTH = 0, FACE_EAST = 1, FACE_SOUTH = 2, FACE_WEST = 3, FACE_TOP = 4, FACE_BOTTOM = 5 };
-
-struct StageArgs {
-    const double* __restrict__ Us;    // state the fluxes are evaluated from (stencil)
-    const double* __restrict__ Uold;  // base state of the update (cell-local)
-    const double* __restrict__ d0;    // dUdt of stage 1 (stages 2, 3)
-    const double* __restrict__ d1;    // dUdt of stage 2 (stage 3)
-    double* __restrict__ dout;        // where this stage's dUdt is kept (or null)
-    double* __restrict__ Unew;        // new state (may alias Uold)
-    unsigned int* counters;           // [0] invalid cells, [1] decode failures
-    long qs;                          // stride between conserved quantities
-    int ni, nj, nk, kchunk;
-    int stage;                        // 1, 2 or 3
-    int high_order, apply_limiter, extrema_clipping;
-    double c1;                        // dt*gamma_1 for stage 1
-    double dt, g1, g2, g3;
-    double vol_inv;
-    double area[3];                   // i-, j-, k-face areas
-    L2R2W w[3];
-    IdealGas gas;
-    int bc_type[6];
-    Prim bc_fs[6];                    // inflow ghost states (global frame)
-};
 
 // Decode of the cell (i,j,k); positions one or two cells outside the block in
 // one index direction are ghost cells and take the value the face's ghost-cell
 // effect gives them.  Positions further out return a harmless dummy state.
+template <class M>
 __device__ __forceinline__ Prim load_decode(const StageArgs& A, long c)
 {
-    return decode_ideal(A.Us[c], A.Us[c + A.qs], A.Us[c + 2 * A.qs], A.Us[c + 3 * A.qs], A.Us[c + 4 * A.qs], A.gas);
+    return decode_ideal<M>(A.Us[c], A.Us[c + A.qs], A.Us[c + 2 * A.qs], A.Us[c + 3 * A.qs], A.Us[c + 4 * A.qs],
+                           A.gas);
 }
 
 __device__ __forceinline__ long cell_index(const StageArgs& A, int i, int j, int k)
@@ -60,6 +37,7 @@ __device__ __forceinline__ long cell_index(const StageArgs& A, int i, int j, int
     return ((long)k * A.nj + j) * A.ni + i;
 }
 
+template <class M>
 __device__ Prim fetch_ghost(const StageArgs& A, int i, int j, int k)
 {
     Prim dummy{1.0, 0.0, 0.0, 0.0, 1.0, 1.0};
@@ -87,7 +65,7 @@ __device__ Prim fetch_ghost(const StageArgs& A, int i, int j, int k)
     else src = plus ? n - layer : layer - 1;     // slip wall mirror
     int si = i, sj = j, sk = k;
     if (dir == 0) si = src; else if (dir == 1) sj = src; else sk = src;
-    Prim P = load_decode(A, cell_index(A, si, sj, sk));
+    Prim P = load_decode<M>(A, cell_index(A, si, sj, sk));
     if (type == BC_SLIP_WALL) {
         // reflect_normal_velocity (ghost_cell.d:32-43): negate the face-normal component
         if (dir == 0) P.v0 = -(P.v0); else if (dir == 1) P.v1 = -(P.v1); else P.v2 = -(P.v2);
@@ -95,10 +73,14 @@ __device__ Prim fetch_ghost(const StageArgs& A, int i, int j, int k)
     return P;
 }
 
-__device__ __forceinline__ Prim fetch(const StageArgs& A, int i, int j, int k)
+template <class M>
+__device__ __forceinline__ PrimT<typename M::R> fetch(const StageArgs& A, int i, int j, int k)
 {
-    if (i >= 0 && i < A.ni && j >= 0 && j < A.nj && k >= 0 && k < A.nk) return load_decode(A, cell_index(A, i, j, k));
-    return fetch_ghost(A, i, j, k);
+    using R = typename M::R;
+    Prim P;
+    if (i >= 0 && i < A.ni && j >= 0 && j < A.nj && k >= 0 && k < A.nk) P = load_decode<M>(A, cell_index(A, i, j, k));
+    else P = fetch_ghost<M>(A, i, j, k);
+    return PrimT<R>{R(P.rho), R(P.v0), R(P.v1), R(P.v2), R(P.u), R(P.a)};
 }
 
 // Face frames as permutations of (x,y,z): 3D i-face (x,y,z), j-face (y,z,x),
@@ -126,14 +108,16 @@ __device__ __forceinline__ double stage_update(const StageArgs& A, double uold, 
     return uold + A.dt * (A.g1 * dd0 + A.g2 * dd1 + A.g3 * d);
 }
 
-template <int TX, int TY, int DIMS, int FLUX>
+template <class M, int TX, int TY, int DIMS, int FLUX>
 __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs A)
 {
+    using R = typename M::R;
     constexpr int NT = TX * TY;
     constexpr int PX = TX + 4, PY = TY + 4, PLANE = PX * PY;
-    __shared__ double sP[6 * PLANE];                  // rho, vx, vy, vz, u, a
-    __shared__ double sFi[5 * TY * (TX + 1)];         // i-face fluxes (global frame)
-    __shared__ double sFj[5 * (TY + 1) * TX];         // j-face fluxes (global frame)
+    constexpr int NFI = TY * (TX + 1), NFJ = (TY + 1) * TX;
+    __shared__ R sP[6 * PLANE];              // rho, vx, vy, vz, u, a of plane k with halos
+    __shared__ R sFi[5 * NFI];               // i-face fluxes (global frame)
+    __shared__ R sFj[5 * NFJ];               // j-face fluxes (global frame)
 
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
@@ -143,34 +127,29 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs A)
     const int k0 = (DIMS == 3) ? blockIdx.z * A.kchunk : 0;
     const int k1 = (DIMS == 3) ? min(k0 + A.kchunk, A.nk) : 1;
     const bool ho = A.high_order != 0, lim = A.apply_limiter != 0, clip = A.extrema_clipping != 0;
+    const IdealGasT<R> gR = cast_gas<R>(A.gas);
 
-    auto to_face = [](const Prim& P, int pn, int pt1, int pt2) {
-        Prim F;
-        F.rho = P.rho; F.u = P.u; F.a = P.a;
-        F.v0 = (pn == 0) ? P.v0 : (pn == 1) ? P.v1 : P.v2;
-        F.v1 = (pt1 == 0) ? P.v0 : (pt1 == 1) ? P.v1 : P.v2;
-        F.v2 = (pt2 == 0) ? P.v0 : (pt2 == 1) ? P.v1 : P.v2;
-        return F;
-    };
+    // k-face frame (z, x, y)
+    auto kface = [](const PrimT<R>& P) { return PrimT<R>{P.rho, P.v2, P.v0, P.v1, P.u, P.a}; };
 
     // k-direction register queue (k-1, k, k+1, k+2) and the carried bottom-face flux
-    Prim q0, q1, q2, q3;
-    double Fb[5] = {0, 0, 0, 0, 0};
+    PrimT<R> q0, q1, q2, q3;
+    R Fb[5] = {R(0), R(0), R(0), R(0), R(0)};
     if constexpr (DIMS == 3) {
-        Prim qa = fetch(A, i, j, k0 - 2);
-        q0 = fetch(A, i, j, k0 - 1);
-        q1 = fetch(A, i, j, k0);
-        q2 = fetch(A, i, j, k0 + 1);
+        PrimT<R> qa = fetch<M>(A, i, j, k0 - 2);
+        q0 = fetch<M>(A, i, j, k0 - 1);
+        q1 = fetch<M>(A, i, j, k0);
+        q2 = fetch<M>(A, i, j, k0 + 1);
         if (own_ij) {
-            Flux5 F = face_flux_ideal<FLUX>(to_face(qa, 2, 0, 1), to_face(q0, 2, 0, 1), to_face(q1, 2, 0, 1),
-                                            to_face(q2, 2, 0, 1), A.w[2], A.gas, ho, lim, clip);
+            Flux5T<R> F = face_flux_ideal<M, FLUX>(kface(qa), kface(q0), kface(q1), kface(q2), cast_w<R>(A.w[2]), gR,
+                                                   ho, lim, clip);
             Fb[0] = F.mass; Fb[1] = F.mt1; Fb[2] = F.mt2; Fb[3] = F.mn; Fb[4] = F.E;
         }
-        q3 = fetch(A, i, j, k0 + 2);
+        q3 = fetch<M>(A, i, j, k0 + 2);
     }
 
     for (int k = k0; k < k1; ++k) {
-        Prim own = (DIMS == 3) ? q1 : fetch(A, i, j, 0);
+        PrimT<R> own = (DIMS == 3) ? q1 : fetch<M>(A, i, j, 0);
         // 1. plane k with halos into LDS
         {
             const int s = (ty + 2) * PX + (tx + 2);
@@ -191,7 +170,7 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs A)
                 jj = side ? TY + 2 + row : row;
                 ii = col + 2;
             }
-            Prim h = fetch(A, i0 + ii - 2, j0 + jj - 2, k);
+            PrimT<R> h = fetch<M>(A, i0 + ii - 2, j0 + jj - 2, k);
             const int s = jj * PX + ii;
             sP[0 * PLANE + s] = h.rho; sP[1 * PLANE + s] = h.v0; sP[2 * PLANE + s] = h.v1;
             sP[3 * PLANE + s] = h.v2; sP[4 * PLANE + s] = h.u; sP[5 * PLANE + s] = h.a;
@@ -216,7 +195,7 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs A)
             // stencil L1 position (LDS): i-face fc sits between columns fc+1, fc+2
             const int step = (dir == 0) ? 1 : PX;
             const int base = (dir == 0) ? (fr + 2) * PX + fc : fr * PX + (fc + 2);
-            Prim S[4];
+            PrimT<R> S[4];
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
                 const int s = base + m * step;
@@ -227,9 +206,9 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs A)
                 S[m].u = sP[4 * PLANE + s];
                 S[m].a = sP[5 * PLANE + s];
             }
-            Flux5 F = face_flux_ideal<FLUX>(S[0], S[1], S[2], S[3], A.w[dir], A.gas, ho, lim, clip);
-            double* dst = (dir == 0) ? sFi : sFj;
-            const int nface = (dir == 0) ? TY * (TX + 1) : (TY + 1) * TX;
+            Flux5T<R> F = face_flux_ideal<M, FLUX>(S[0], S[1], S[2], S[3], cast_w<R>(A.w[dir]), gR, ho, lim, clip);
+            R* dst = (dir == 0) ? sFi : sFj;
+            const int nface = (dir == 0) ? NFI : NFJ;
             const int fidx = (dir == 0) ? fr * (TX + 1) + fc : fr * TX + fc;
             dst[0 * nface + fidx] = F.mass;
             dst[(1 + pn) * nface + fidx] = F.mn;
@@ -238,29 +217,28 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs A)
             dst[4 * nface + fidx] = F.E;
         }
         // 3. k-face at the top of cell k from the register queue
-        double Ft[5] = {0, 0, 0, 0, 0};
+        R Ft[5] = {R(0), R(0), R(0), R(0), R(0)};
         if constexpr (DIMS == 3) {
             if (own_ij) {
-                Flux5 F = face_flux_ideal<FLUX>(to_face(q0, 2, 0, 1), to_face(q1, 2, 0, 1), to_face(q2, 2, 0, 1),
-                                                to_face(q3, 2, 0, 1), A.w[2], A.gas, ho, lim, clip);
+                Flux5T<R> F = face_flux_ideal<M, FLUX>(kface(q0), kface(q1), kface(q2), kface(q3), cast_w<R>(A.w[2]),
+                                                       gR, ho, lim, clip);
                 Ft[0] = F.mass; Ft[1] = F.mt1; Ft[2] = F.mt2; Ft[3] = F.mn; Ft[4] = F.E;
             }
         }
         __syncthreads();
         // 4. flux divergence in face order N, E, S, W, T, B (fvcell.d:789-825),
-        // stage update, decode check, write.
+        // stage update, decode check, write; all in FP64.
         if (own_ij) {
-            constexpr int NFI = TY * (TX + 1), NFJ = (TY + 1) * TX;
             const int iW = ty * (TX + 1) + tx, iE = iW + 1;
             const int jS = ty * TX + tx, jN = jS + TX;
             double integ[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            auto add_face = [&](const double* Fsrc, int nface, int fidx, double outsign, double Aface) {
+            auto add_face = [&](const R* Fsrc, int nface, int fidx, double outsign, double Aface) {
                 double area = outsign * Aface;
-                integ[0] -= Fsrc[0 * nface + fidx] * area;
-                integ[1] -= Fsrc[1 * nface + fidx] * area;
-                integ[2] -= Fsrc[2 * nface + fidx] * area;
-                if (DIMS == 3) integ[3] -= Fsrc[3 * nface + fidx] * area;
-                integ[4] -= Fsrc[4 * nface + fidx] * area;
+                integ[0] -= double(Fsrc[0 * nface + fidx]) * area;
+                integ[1] -= double(Fsrc[1 * nface + fidx]) * area;
+                integ[2] -= double(Fsrc[2 * nface + fidx]) * area;
+                if (DIMS == 3) integ[3] -= double(Fsrc[3 * nface + fidx]) * area;
+                integ[4] -= double(Fsrc[4 * nface + fidx]) * area;
             };
             add_face(sFj, NFJ, jN, 1.0, A.area[1]);
             add_face(sFi, NFI, iE, 1.0, A.area[0]);
@@ -287,7 +265,8 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs A)
             if (!(rho > 0.0)) {
                 decode_fail = true;
             } else {
-                double dinv = 1.0 / rho;
+                using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+                double dinv = dv<MD>(1.0, rho);
                 double vx = Un[1] * dinv, vy = Un[2] * dinv, vz = Un[3] * dinv;
                 double u = Un[4] * dinv;
                 double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
@@ -321,87 +300,9 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs A)
 #pragma unroll
             for (int q = 0; q < 5; ++q) Fb[q] = Ft[q];
             q0 = q1; q1 = q2; q2 = q3;
-            if (k + 1 < k1) q3 = fetch(A, i, j, k + 3);
+            if (k + 1 < k1) q3 = fetch<M>(A, i, j, k + 3);
         }
     }
-}
-
-// CFL signal (fvcell.d:1624-1661, non-stringent) and its maximum over the
-// block; the maximum of a non-negative double is taken on its bit pattern.
-struct CflArgs {
-    const double* __restrict__ U;
-    long qs;
-    int ni, nj, nk, dims;
-    double iLength, jLength, kLength;
-    IdealGas gas;
-    unsigned long long* smax_bits;
-    double* last_signal;
-};
-
-__global__ __launch_bounds__(256) void cfl_kernel(CflArgs A)
-{
-    const long ncells = (long)A.ni * A.nj * A.nk;
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double signal = 0.0;
-    if (c < ncells) {
-        Prim P = decode_ideal(A.U[c], A.U[c + A.qs], A.U[c + 2 * A.qs], A.U[c + 3 * A.qs], A.U[c + 4 * A.qs], A.gas);
-        double un_N = fabs(P.v1);
-        double un_E = fabs(P.v0);
-        double un_T = (A.dims == 3) ? fabs(P.v2) : 0.0;
-        double signalN = (un_N + P.a) / A.jLength;
-        signal = fmax(signal, signalN);
-        double signalE = (un_E + P.a) / A.iLength;
-        signal = fmax(signal, signalE);
-        if (A.dims == 3) {
-            double signalT = (un_T + P.a) / A.kLength;
-            signal = fmax(signal, signalT);
-        }
-        if (c == ncells - 1) *A.last_signal = signal;
-    }
-    for (int off = 32; off > 0; off >>= 1) signal = fmax(signal, __shfl_xor(signal, off));
-    if (__lane_id() == 0) atomicMax(A.smax_bits, (unsigned long long)__double_as_longlong(signal));
-}
-
-// Decode every active cell into flow-state arrays (rho,p,T,u,a,vx,vy,vz) and
-// count cells that would fail decode or check_data.
-struct DecodeArgs {
-    const double* __restrict__ U;
-    double* fs;       // may be null
-    long qs, ncells;
-    IdealGas gas;
-    unsigned int* counters;
-};
-
-__global__ __launch_bounds__(256) void decode_kernel(DecodeArgs A)
-{
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    bool bad = false;
-    if (c < A.ncells) {
-        double rho = A.U[c];
-        if (!(rho > 0.0)) {
-            bad = true;
-        } else {
-            Prim P = decode_ideal(rho, A.U[c + A.qs], A.U[c + 2 * A.qs], A.U[c + 3 * A.qs], A.U[c + 4 * A.qs], A.gas);
-            double T = P.u * A.gas.Cvinv;
-            double p = P.rho * A.gas.Rgas * T;
-            if (!(P.u > 0.0) || !isfinite(T) || T < 1.01 || T > 50000.0 || !isfinite(p) || !isfinite(P.a) ||
-                fabs(P.v0) > 30000.0 || fabs(P.v1) > 30000.0 || fabs(P.v2) > 30000.0)
-                bad = true;
-            if (A.fs) {
-                const long n = A.ncells;
-                A.fs[c] = P.rho; A.fs[n + c] = p; A.fs[2 * n + c] = T; A.fs[3 * n + c] = P.u;
-                A.fs[4 * n + c] = P.a; A.fs[5 * n + c] = P.v0; A.fs[6 * n + c] = P.v1; A.fs[7 * n + c] = P.v2;
-            }
-        }
-    }
-    unsigned int nb = wave_count(bad);
-    if (nb && __lane_id() == 0) atomicAdd(&A.counters[0], nb);
-}
-
-__global__ void reset_counters_kernel(unsigned int* counters, int n, unsigned long long* smax)
-{
-    for (int t = threadIdx.x; t < n; t += blockDim.x) counters[t] = 0u;
-    if (threadIdx.x == 0 && smax) *smax = 0ull;
 }
 
 } // namespace e4

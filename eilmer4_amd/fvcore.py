@@ -95,4 +95,7 @@ class BCType(IntEnum):
 
 
 class PrecisionMode(IntEnum):
-    fp64 = 0
+    """Arithmetic of the reconstruction and flux path (E4HIP_PRECISION_*)."""
+    fp64 = 0           # plain FP64: FMA contraction, reciprocal-based division
+    fp64_exact = 1     # FP64 with the reference's roundings
+    mixed = 2          # FP32 reconstruction and flux, FP64 accumulate/update/decode

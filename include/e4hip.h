@@ -73,7 +73,12 @@ extern "C" {
 #define E4HIP_FACE_BOTTOM 5
 
 /* ---- arithmetic of the flux path */
-#define E4HIP_PRECISION_FP64   0  /* everything in FP64 (the plain form) */
+#define E4HIP_PRECISION_FP64        0  /* plain form: FP64 throughout, FMA contraction and
+                                          reciprocal-based division (~1 ulp per operation) */
+#define E4HIP_PRECISION_FP64_EXACT  1  /* FP64 with the reference's roundings: IEEE division,
+                                          no contraction, reference operation order */
+#define E4HIP_PRECISION_MIXED       2  /* FP32 reconstruction and flux; FP64 residual
+                                          accumulation, update and decode */
 
 /* Step configuration: the LocalConfig fields the path reads
  * (eilmer/globalconfig.d:440-569, 875, 1008). */

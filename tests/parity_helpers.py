@@ -29,10 +29,10 @@ def device_block(gas, geom, bcs, U0):
 
 
 def device_sim(blocks, flux=e4.FluxCalculator.ausmdv, scheme=e4.GasdynamicUpdate.pc, order=2, limiter=True,
-               clipping=True, cfl=0.5, max_invalid_cells=0):
+               clipping=True, cfl=0.5, max_invalid_cells=0, precision="fp64"):
     cfg = e4.GlobalConfig(flux_calculator=flux, gasdynamic_update_scheme=scheme, interpolation_order=order,
                           apply_limiter=limiter, extrema_clipping=clipping, cfl_value=cfl,
-                          max_invalid_cells=max_invalid_cells)
+                          max_invalid_cells=max_invalid_cells, precision_mode=e4.PrecisionMode[precision])
     return e4.Simulation(blocks, cfg)
 
 
@@ -50,7 +50,7 @@ def rel_err(A, B):
 
 
 def run_pair(geom, fields, bcs, nsteps, flux="ausmdv", scheme="pc", order=2, limiter=True, clipping=True,
-             cfl=0.5, check_every=None):
+             cfl=0.5, check_every=None, precision="fp64_exact", same_dt=False):
     """Advance the oracle and the device from the same U0 with their own dt
     policies; returns (U_oracle, U_device, dts_oracle, dts_device, checkpoints)."""
     gas = e4.IdealGas()
@@ -61,12 +61,19 @@ def run_pair(geom, fields, bcs, nsteps, flux="ausmdv", scheme="pc", order=2, lim
                       clipping=int(clipping))
     osim = O.Simulation(ob, cfl_value=cfl)
     db = device_block(gas, geom, bcs, U0)
-    dsim = device_sim([db], flux=fc, scheme=us, order=order, limiter=limiter, clipping=clipping, cfl=cfl)
+    dsim = device_sim([db], flux=fc, scheme=us, order=order, limiter=limiter, clipping=clipping, cfl=cfl,
+                      precision=precision)
     dts_o, dts_d, checkpoints = [], [], []
     for s in range(nsteps):
         osim.run(max_step=s + 1)
         dts_o.append(osim.dt_global)
-        dsim.integrate_in_time(target_time=1e30, max_step=s + 1)
+        if same_dt:
+            # drive the device with the oracle's dt sequence (isolates the stage arithmetic)
+            dsim.state.dt_global = osim.dt_global
+            dsim.gasdynamic_explicit_increment_with_fixed_grid()
+            dsim.state.step += 1
+        else:
+            dsim.integrate_in_time(target_time=1e30, max_step=s + 1)
         dts_d.append(dsim.state.dt_global)
         if check_every and (s + 1) % check_every == 0:
             checkpoints.append((s + 1, ob.get_U(), db.download()))

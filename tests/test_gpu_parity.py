@@ -1,87 +1,96 @@
 """Device path vs the CPU oracle on identical inputs (through the C ABI).
 
-Tolerances are those the FP64 plain form is held to: per conserved quantity,
-max|dU|/max|U| <= 1e-12 after one step and <= 1e-9 after 100 steps.  The
-library is built without FMA contraction and with the reference's operation
-order, so the two normally agree bit for bit; the test prints the observed
-difference.
+Bars, per conserved quantity q, as max|dU_q| / max|U_q|:
+  fp64_exact  bit-identical to the oracle (reference roundings)
+  fp64        <= 1e-12 after one step, <= 1e-9 after 100 steps
+  mixed       density L-inf relative error <= 1e-5 (FP32 reconstruction and flux)
 """
 import numpy as np
 import pytest
 
-import pyoracle as O
 import eilmer4_amd as e4
 from eilmer4_amd import workloads
-from parity_helpers import run_pair, rel_err, device_block, device_sim, oracle_block
+from parity_helpers import run_pair, rel_err, device_block, device_sim
 
 pytestmark = pytest.mark.gpu
 
 TOL_1 = 1.0e-12
 TOL_100 = 1.0e-9
+TOL_MIXED_RHO = 1.0e-5
 
 
 def small_box(n=24, seed=workloads.SEED_BOX_3D):
     return workloads.shocked_box_3d(e4.IdealGas(), n=n, seed=seed)
 
 
-@pytest.mark.parametrize("flux", ["ausmdv", "hllc"])
-def test_box3d_one_step(gpu, flux):
-    geom, fields, bcs, _ = small_box(24)
-    print("case box3d n=24 flux=%s steps=1" % flux, flush=True)
-    Uo, Ud, dto, dtd, _ = run_pair(geom, fields, bcs, 1, flux=flux)
+def check(precision, Ud, Uo, tol):
     err = rel_err(Ud, Uo)
-    print("rel err per quantity", err, "dt", dto, dtd, "bitwise", np.array_equal(Ud, Uo))
+    bitwise = bool(np.array_equal(Ud, Uo))
+    print("  %s: rel err per quantity %s bitwise %s" % (precision, err, bitwise), flush=True)
     assert np.all(np.isfinite(Ud))
-    assert dto[0] == dtd[0]
-    assert err.max() <= TOL_1
+    if precision == "fp64_exact":
+        assert bitwise
+    elif precision == "mixed":
+        assert err[0] <= TOL_MIXED_RHO
+    else:
+        assert err.max() <= tol
+    return err
 
 
+@pytest.mark.parametrize("precision", ["fp64_exact", "fp64", "mixed"])
 @pytest.mark.parametrize("flux", ["ausmdv", "hllc"])
-def test_box3d_100_steps(gpu, flux):
+def test_box3d_one_step(gpu, flux, precision):
+    geom, fields, bcs, _ = small_box(24)
+    print("case box3d n=24 flux=%s precision=%s steps=1" % (flux, precision), flush=True)
+    Uo, Ud, dto, dtd, _ = run_pair(geom, fields, bcs, 1, flux=flux, precision=precision)
+    assert dto[0] == dtd[0]
+    check(precision, Ud, Uo, TOL_1)
+
+
+@pytest.mark.parametrize("precision", ["fp64_exact", "fp64", "mixed"])
+@pytest.mark.parametrize("flux", ["ausmdv", "hllc"])
+def test_box3d_100_steps(gpu, flux, precision):
     geom, fields, bcs, _ = small_box(20)
-    print("case box3d n=20 flux=%s steps=100" % flux, flush=True)
-    Uo, Ud, dto, dtd, cps = run_pair(geom, fields, bcs, 100, flux=flux, check_every=50)
-    err = rel_err(Ud, Uo)
-    print("rel err per quantity", err, "bitwise", np.array_equal(Ud, Uo))
-    assert np.max(np.abs(dto - dtd) / dto) <= TOL_100
-    assert err.max() <= TOL_100
+    print("case box3d n=20 flux=%s precision=%s steps=100" % (flux, precision), flush=True)
+    Uo, Ud, dto, dtd, cps = run_pair(geom, fields, bcs, 100, flux=flux, precision=precision, same_dt=True)
+    check(precision, Ud, Uo, TOL_100)
 
 
-def test_planar_bump_2d_200_steps(gpu):
-    """The 2D 128x64 Mach-2 bump workload, 200 steps, ausmdv, pc, CFL 0.5."""
+@pytest.mark.parametrize("precision", ["fp64_exact", "fp64"])
+def test_planar_bump_2d_200_steps(gpu, precision):
+    """The 2D 128x64 Mach-2 bump workload, 200 steps, ausmdv, pc, CFL 0.5,
+    each side with its own dt policy."""
     gas = e4.IdealGas()
     geom, fields, bcs, _ = workloads.planar_bump_2d(gas)
-    print("case bump2d 128x64 steps=200", flush=True)
-    Uo, Ud, dto, dtd, cps = run_pair(geom, fields, bcs, 200, check_every=1)
-    e1 = rel_err(cps[0][2], cps[0][1])
-    err = rel_err(Ud, Uo)
-    print("step1", e1, "step200", err, "bitwise", np.array_equal(Ud, Uo))
-    assert e1.max() <= TOL_1
-    assert err.max() <= TOL_100
+    print("case bump2d 128x64 steps=200 precision=%s" % precision, flush=True)
+    Uo, Ud, dto, dtd, cps = run_pair(geom, fields, bcs, 200, check_every=1, precision=precision)
+    check(precision, cps[0][2], cps[0][1], TOL_1)
+    check(precision, Ud, Uo, TOL_100)
     assert np.max(np.abs(dto - dtd) / dto) <= TOL_100
 
 
+@pytest.mark.parametrize("precision", ["fp64_exact", "fp64"])
 @pytest.mark.parametrize("scheme", ["euler", "midpoint", "classic_rk3", "tvd_rk3", "denman_rk3"])
-def test_update_schemes(gpu, scheme):
+def test_update_schemes(gpu, scheme, precision):
     geom, fields, bcs, _ = small_box(16, seed=7)
-    print("case box3d n=16 scheme=%s steps=3" % scheme, flush=True)
-    Uo, Ud, dto, dtd, _ = run_pair(geom, fields, bcs, 3, scheme=scheme, cfl=0.3)
-    err = rel_err(Ud, Uo)
-    print(scheme, err)
-    assert err.max() <= TOL_1 * 10
+    print("case box3d n=16 scheme=%s precision=%s steps=3" % (scheme, precision), flush=True)
+    Uo, Ud, dto, dtd, _ = run_pair(geom, fields, bcs, 3, scheme=scheme, cfl=0.3, precision=precision)
+    check(precision, Ud, Uo, TOL_1 * 10)
 
 
+@pytest.mark.parametrize("precision", ["fp64_exact", "fp64"])
 @pytest.mark.parametrize("order,limiter,clipping", [(1, True, True), (2, False, True), (2, True, False)])
-def test_reconstruction_options(gpu, order, limiter, clipping):
+def test_reconstruction_options(gpu, order, limiter, clipping, precision):
     geom, fields, bcs, _ = small_box(16, seed=11)
-    print("case options order=%d limiter=%d clipping=%d" % (order, limiter, clipping), flush=True)
-    Uo, Ud, _, _, _ = run_pair(geom, fields, bcs, 2, order=order, limiter=limiter, clipping=clipping, cfl=0.3)
-    err = rel_err(Ud, Uo)
-    print(err)
-    assert err.max() <= TOL_1 * 10
+    print("case options order=%d limiter=%d clipping=%d precision=%s" % (order, limiter, clipping, precision),
+          flush=True)
+    Uo, Ud, _, _, _ = run_pair(geom, fields, bcs, 2, order=order, limiter=limiter, clipping=clipping, cfl=0.3,
+                               precision=precision)
+    check(precision, Ud, Uo, TOL_1 * 10)
 
 
-def test_ragged_sizes_and_all_bcs(gpu):
+@pytest.mark.parametrize("precision", ["fp64_exact", "fp64", "mixed"])
+def test_ragged_sizes_and_all_bcs(gpu, precision):
     """Block sizes that are not multiples of the tile, and every face kind on
     one block: inflow W, extrapolate E, slip walls N/S, periodic T/B."""
     gas = e4.IdealGas()
@@ -99,12 +108,10 @@ def test_ragged_sizes_and_all_bcs(gpu):
     bcs = {e4.Face.west: (e4.BCType.inflow, inflow), e4.Face.east: (e4.BCType.extrapolate, None),
            e4.Face.north: (e4.BCType.slip_wall, None), e4.Face.south: (e4.BCType.slip_wall, None),
            e4.Face.top: (e4.BCType.periodic, None), e4.Face.bottom: (e4.BCType.periodic, None)}
-    print("case ragged 37x13x11 all-bcs", flush=True)
+    print("case ragged 37x13x11 all-bcs precision=%s" % precision, flush=True)
     for flux in ("ausmdv", "hllc"):
-        Uo, Ud, dto, dtd, _ = run_pair(geom, fields, bcs, 5, flux=flux)
-        err = rel_err(Ud, Uo)
-        print(flux, err, "bitwise", np.array_equal(Ud, Uo))
-        assert err.max() <= TOL_1 * 10
+        Uo, Ud, dto, dtd, _ = run_pair(geom, fields, bcs, 5, flux=flux, precision=precision, same_dt=True)
+        check(precision, Ud, Uo, TOL_1 * 10)
 
 
 def test_sod_shock_tube_on_device(gpu):
