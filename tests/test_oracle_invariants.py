@@ -1,0 +1,272 @@
+"""Properties the mathematics fixes, checked on the oracle (CPU only).  Each is
+chosen so that a plausible slip in the restatement (dropped term, wrong sign or
+index, transposed operand) fails at least one of them."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from exact_riemann import sample
+
+G = 1.4
+RGAS = 8.31451 / 0.02896
+CV = RGAS / (G - 1.0)
+
+
+def fs(rho, p, vel, gas=None):
+    gas = gas or O.Gas.ideal()
+    T = p / (rho * RGAS)
+    return O.flowstate_pT(gas, p, T, vel)
+
+
+def physical_flux(s):
+    rho, p, u, a, vn, vt1, vt2 = s[0], s[1], s[3], s[4], s[5], s[6], s[7]
+    E = rho * (u + 0.5 * (vn * vn + vt1 * vt1 + vt2 * vt2))
+    return np.array([rho * vn, rho * vn * vn + p, rho * vn * vt1, rho * vn * vt2, vn * (E + p)])
+
+
+@pytest.mark.parametrize("calc", [O.AUSMDV, O.HLLC])
+def test_flux_consistency(calc):
+    """F(U, U) is the physical flux (AUSMDV: Wada & Liou 1994 eq. 29-37 reduce
+    to it for equal states; HLLC: Toro 10.4)."""
+    gas = O.Gas.ideal()
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        rho = rng.uniform(0.1, 5.0)
+        p = rng.uniform(1e3, 1e6)
+        vel = rng.uniform(-1500, 1500, 3)
+        s = fs(rho, p, vel, gas)
+        F = O.flux(gas, calc, s, s)
+        Fp = physical_flux(s)
+        scale = np.abs(Fp).max() + 1e-300
+        assert np.max(np.abs(F - Fp)) / scale < 1e-12
+
+
+@pytest.mark.parametrize("calc", [O.AUSMDV, O.HLLC])
+def test_supersonic_upwinding(calc):
+    """Both sides supersonic to the right: the flux is the left state's."""
+    gas = O.Gas.ideal()
+    rng = np.random.default_rng(2)
+    for _ in range(100):
+        L = fs(rng.uniform(0.5, 2), rng.uniform(5e4, 2e5), (rng.uniform(900, 1500), rng.uniform(-50, 50), 0.0), gas)
+        R = fs(rng.uniform(0.5, 2), rng.uniform(5e4, 2e5), (rng.uniform(900, 1500), rng.uniform(-50, 50), 0.0), gas)
+        if L[5] < 1.01 * L[4] or R[5] < 1.01 * R[4]:
+            continue
+        F = O.flux(gas, calc, L, R)
+        Fp = physical_flux(L)
+        assert np.max(np.abs(F - Fp)) / np.abs(Fp).max() < 1e-12
+
+
+@pytest.mark.parametrize("calc", [O.AUSMDV, O.HLLC])
+def test_flux_mirror_symmetry(calc):
+    """Swapping the states and negating normal velocities negates the mass
+    and energy flux and keeps the normal momentum flux."""
+    gas = O.Gas.ideal()
+    rng = np.random.default_rng(3)
+    for _ in range(100):
+        L = fs(rng.uniform(0.5, 2), rng.uniform(5e4, 2e5), rng.uniform(-400, 400, 3), gas)
+        R = fs(rng.uniform(0.5, 2), rng.uniform(5e4, 2e5), rng.uniform(-400, 400, 3), gas)
+        F = O.flux(gas, calc, L, R)
+        Lm, Rm = R.copy(), L.copy()
+        Lm[5] = -Lm[5]
+        Rm[5] = -Rm[5]
+        Fm = O.flux(gas, calc, Lm, Rm)
+        s = np.abs(F).max()
+        assert abs(F[0] + Fm[0]) / s < 1e-12
+        assert abs(F[1] - Fm[1]) / s < 1e-12
+        assert abs(F[4] + Fm[4]) / s < 1e-12
+
+
+def test_van_albada_properties():
+    """s = 1 on linear data (face values exact), limiting to first order at an
+    extremum, and clipping keeps values inside [qL0, qR0]."""
+    lens = [0.1, 0.1, 0.1, 0.1]
+    qL, qR = O.interp_l2r2_scalar(lens, [1.0, 2.0, 3.0, 4.0])
+    assert abs(qL - 2.5) < 1e-14 and abs(qR - 2.5) < 1e-14
+    qL, qR = O.interp_l2r2_scalar(lens, [1.0, 3.0, 2.0, 0.5])      # extremum at L0
+    assert abs(qL - 3.0) < 1e-9
+    rng = np.random.default_rng(4)
+    for _ in range(500):
+        q = rng.uniform(-1, 1, 4)
+        qL, qR = O.interp_l2r2_scalar(lens, q)
+        lo, hi = min(q[1], q[2]), max(q[1], q[2])
+        assert lo <= qL <= hi and lo <= qR <= hi
+    # a step between flat regions: the limiter (eps = 1e-12 in numerator and
+    # denominator) leaves only an eps-sized slope, even without clipping
+    qL, qR = O.interp_l2r2_scalar(lens, [0.0, 0.0, 1.0, 1.0], True, False)
+    assert abs(qL) < 1e-12 and abs(qR - 1.0) < 1e-12 and qL != 0.0
+
+
+def periodic_box(n, rng, calc=O.AUSMDV, amp=0.1):
+    g = O.Gas.ideal()
+    b = O.Block(g, 3, n, n, n, 1.0 / n, 1.0 / n, 1.0 / n)
+    b.set_config(flux_calc=calc)
+    for f in range(6):
+        b.set_bc(f, O.BC_PERIODIC)
+    shape = (n, n, n)
+    p = 1e5 * (1 + amp * rng.standard_normal(shape))
+    T = 300 * (1 + amp * rng.standard_normal(shape))
+    v = [100 * amp * rng.standard_normal(shape) for _ in range(3)]
+    b.set_from_pT(p, T, *v)
+    return b
+
+
+@pytest.mark.parametrize("calc", [O.AUSMDV, O.HLLC])
+def test_free_stream_preservation(calc):
+    """A uniform state in a periodic box does not change (every flux pair cancels)."""
+    g = O.Gas.ideal()
+    n = 8
+    b = O.Block(g, 3, n, n, n, 0.1, 0.1, 0.1)
+    b.set_config(flux_calc=calc)
+    for f in range(6):
+        b.set_bc(f, O.BC_PERIODIC)
+    b.set_from_pT(1e5, 300.0, 200.0, -50.0, 30.0)
+    U0 = b.get_U()
+    for _ in range(5):
+        assert b.step(1e-5)[0] == 0
+    assert np.array_equal(b.get_U(), U0)
+
+
+@pytest.mark.parametrize("calc", [O.AUSMDV, O.HLLC])
+def test_conservation_periodic(calc):
+    """Sum over cells of mass, momentum and energy is conserved in a periodic
+    box (flux differencing); only summation rounding remains."""
+    b = periodic_box(12, np.random.default_rng(5), calc)
+    U0 = b.get_U().sum(axis=1)
+    dt, _, _ = b.determine_dt(1e-3, 0.5, False)
+    for _ in range(10):
+        assert b.step(dt)[0] == 0
+    U1 = b.get_U().sum(axis=1)
+    scale = np.array([U0[0], np.abs(b.get_U()[1:4]).sum() / 3] * 1 + [0] * 0)
+    assert abs(U1[0] - U0[0]) / U0[0] < 1e-13
+    assert abs(U1[4] - U0[4]) / U0[4] < 1e-13
+    mom_scale = np.abs(b.get_U()[1:4]).sum()
+    assert np.max(np.abs(U1[1:4] - U0[1:4])) / mom_scale < 1e-13
+
+
+def test_axis_permutation_symmetry():
+    """Cycling the axes (x->y->z->x) of field, velocity and box permutes the
+    result: i-, j- and k-faces use the cyclic frames (x,y,z), (y,z,x), (z,x,y)."""
+    n = 10
+    rng = np.random.default_rng(6)
+    g = O.Gas.ideal()
+
+    def make(perm):
+        b = O.Block(g, 3, n, n, n, 0.1, 0.1, 0.1)
+        b.set_config()
+        for f in range(6):
+            b.set_bc(f, O.BC_PERIODIC)
+        return b
+
+    shape = (n, n, n)   # [k][j][i]
+    p = 1e5 * (1 + 0.1 * rng.standard_normal(shape))
+    T = 300 * (1 + 0.1 * rng.standard_normal(shape))
+    vx, vy, vz = (20 * rng.standard_normal(shape) for _ in range(3))
+    a = make(0)
+    a.set_from_pT(p, T, vx, vy, vz)
+    # new x = old z, new y = old x, new z = old y: array [k'][j'][i'] = old[j'][i'][k']
+    tr = lambda f: np.ascontiguousarray(np.transpose(f, (1, 2, 0)))
+    b = make(1)
+    b.set_from_pT(tr(p), tr(T), tr(vz), tr(vx), tr(vy))
+    for _ in range(3):
+        a.step(1e-5)
+        b.step(1e-5)
+    Ua = a.get_U().reshape(5, n, n, n)
+    Ub = b.get_U().reshape(5, n, n, n)
+    for qa, qb in ((0, 0), (3, 1), (1, 2), (2, 3), (4, 4)):
+        ref = tr(Ua[qa])
+        assert np.max(np.abs(Ub[qb] - ref)) / np.abs(ref).max() < 1e-12
+
+
+def test_stationary_normal_shock():
+    """A Mach-3 normal shock with Rankine-Hugoniot states (the relations of
+    gasdyn/idealgasflow.d:115-186) stays within one cell over 200 steps."""
+    g = O.Gas.ideal()
+    ni = 64
+    b = O.Block(g, 3, ni, 2, 2, 1.0 / ni, 0.05, 0.05)
+    b.set_config()
+    M = 3.0
+    rho1, p1 = 1.0, 1e5
+    a1 = np.sqrt(G * p1 / rho1)
+    u1 = M * a1
+    r = (G + 1) * M * M / ((G - 1) * M * M + 2)
+    pr = 1 + 2 * G / (G + 1) * (M * M - 1)
+    rho2, p2, u2 = rho1 * r, p1 * pr, u1 / r
+    x = (np.arange(ni) + 0.5) / ni
+    up = x < 0.5
+    rho = np.where(up, rho1, rho2)
+    p = np.where(up, p1, p2)
+    u = np.where(up, u1, u2)
+    shape = (2, 2, ni)
+    b.set_from_pT(np.broadcast_to(p, shape), np.broadcast_to(p / (rho * RGAS), shape), np.broadcast_to(u, shape),
+                  0.0, 0.0)
+    b.set_bc(O.WEST, O.BC_INFLOW, O.flowstate_pT(g, p1, p1 / (rho1 * RGAS), (u1, 0, 0)))
+    b.set_bc(O.EAST, O.BC_EXTRAPOLATE)
+    sim = O.Simulation(b)
+    sim.run(max_step=200)
+    r_ = b.get_fs()[0].reshape(shape)[0, 0]
+    pos = np.argmax(np.abs(np.diff(r_)))
+    assert abs(pos + 1 - ni // 2) <= 1
+
+
+@pytest.mark.parametrize("calc", [O.AUSMDV, O.HLLC])
+@pytest.mark.parametrize("case", [
+    # Toro 2009, table 4.1 tests 1, 2 and 4 (left/right rho, u, p; end time;
+    # diaphragm; pressure scale to air-like units)
+    ((1.0, 0.75, 1.0), (0.125, 0.0, 0.1), 0.2, 0.3, 1.0e5),
+    ((1.0, -2.0, 0.4), (1.0, 2.0, 0.4), 0.15, 0.5, 1.0e5),
+    ((5.99924, 19.5975, 460.894), (5.99242, -6.19633, 46.0950), 0.035, 0.4, 1.0e3),
+])
+def test_riemann_problems_vs_exact(calc, case):
+    """1D Riemann problems (gamma 1.4) against the exact solution: L1 density
+    error of a captured solution on 200 cells.  Toro's non-dimensional data are
+    scaled to air-like units (p x s Pa, velocity x sqrt(s) m/s, time /
+    sqrt(s)) so that temperatures stay above the check_data floor."""
+    Ln, Rn, tn, x0, sp = case
+    sv = np.sqrt(sp)
+    L = (Ln[0], Ln[1] * sv, Ln[2] * sp)
+    R = (Rn[0], Rn[1] * sv, Rn[2] * sp)
+    tend = tn / sv
+    g = O.Gas.ideal()
+    ni = 200
+    b = O.Block(g, 3, ni, 2, 2, 1.0 / ni, 0.01, 0.01)
+    b.set_config(flux_calc=calc)
+    x = (np.arange(ni) + 0.5) / ni
+    rho = np.where(x < x0, L[0], R[0])
+    u = np.where(x < x0, L[1], R[1])
+    p = np.where(x < x0, L[2], R[2])
+    shape = (2, 2, ni)
+    b.set_from_pT(np.broadcast_to(p, shape), np.broadcast_to(p / (rho * RGAS), shape), np.broadcast_to(u, shape),
+                  0.0, 0.0)
+    b.set_bc(O.WEST, O.BC_EXTRAPOLATE)
+    b.set_bc(O.EAST, O.BC_EXTRAPOLATE)
+    sim = O.Simulation(b, cfl_value=0.4, dt_init=1e-6, dt_max=1.0)
+    sim.run(max_time=tend, max_step=100000)
+    r_ = b.get_fs()[0].reshape(shape)[0, 0]
+    ex = sample(L, R, 1.4, (x - x0) / sim.time)
+    l1 = np.mean(np.abs(r_ - ex[:, 0])) / np.mean(np.abs(ex[:, 0]))
+    assert l1 < 0.04, l1
+
+
+def test_tp_newton_inverts_energy():
+    """u(T(u)) = u to the solver tolerance, T independent of density, and
+    Cv = du/dT (central difference of the energy)."""
+    tp = O.Gas.from_json("therm-perf-5-species-air.json")
+    rng = np.random.default_rng(8)
+    # outside the coefficient blend ranges ([800,1200] K and [5500,6500] K),
+    # where the linear blend makes eval_Cp differ from d(eval_h)/dT
+    Ts = [T for T in rng.uniform(300, 9000, 200) if not (750 < T < 1250 or 5450 < T < 6550)][:50]
+    for T in Ts:
+        Y = rng.dirichlet(np.ones(5) * 2)
+        rc, q = tp.update("energy", tp.state(T=T, massf=Y))
+        utgt = q[3]
+        rc, q1 = tp.update("rhou", tp.state(rho=0.5, u=utgt, T=T * 1.2, massf=Y))
+        rc2, q2 = tp.update("rhou", tp.state(rho=5.0, u=utgt, T=T * 1.2, massf=Y))
+        assert rc == 0 and rc2 == 0
+        assert q1[2] == q2[2]
+        assert abs(q1[2] - T) / T < 1e-8
+        h = 1e-4 * T
+        _, qa = tp.update("energy", tp.state(T=T + h, massf=Y))
+        _, qb = tp.update("energy", tp.state(T=T - h, massf=Y))
+        Cv_fd = (qa[3] - qb[3]) / (2 * h)
+        Cv = sum(Y[s] * (tp.eval_Cp(s, T) - 8.31451 / tp.desc["M"][s]) for s in range(5))
+        assert abs(Cv_fd - Cv) / Cv < 1e-6
