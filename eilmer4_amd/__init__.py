@@ -5,11 +5,11 @@ gfx950); this package is the host side that drives it.
 """
 from .fvcore import (BCType, Face, FlowSolverException, FluxCalculator, GasdynamicUpdate, InterpolateOption,
                      PrecisionMode, flux_calculator_from_name, flux_calculator_name, update_scheme_from_name)
-from .gas import IdealGas
+from .gas import IdealGas, ThermallyPerfectGas
 from .lib import E4hipError, load
 from .fluidblock import SFluidBlock
 from .simcore import GlobalConfig, Simulation
 
 __all__ = ["BCType", "Face", "FlowSolverException", "FluxCalculator", "GasdynamicUpdate", "InterpolateOption",
            "PrecisionMode", "flux_calculator_from_name", "flux_calculator_name", "update_scheme_from_name",
-           "IdealGas", "E4hipError", "load", "SFluidBlock", "GlobalConfig", "Simulation"]
+           "IdealGas", "ThermallyPerfectGas", "E4hipError", "load", "SFluidBlock", "GlobalConfig", "Simulation"]

@@ -98,4 +98,30 @@ __global__ void reset_counters_kernel(unsigned int* counters, int n, unsigned lo
 }
 
 
+// CFL signal from the primitive array (fvcell.d:1624-1661).
+__global__ __launch_bounds__(256) void tp_cfl_kernel(const double* __restrict__ P, long qs, int ni, int nj, int nk,
+                                                     int dims, double iL, double jL, double kL,
+                                                     unsigned long long* smax_bits, double* last_signal)
+{
+    __shared__ double wave_max[4];
+    const long ncells = (long)ni * nj * nk;
+    double sig = 0.0;
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += (long)gridDim.x * blockDim.x) {
+        double a = P[P_A * qs + c];
+        double s = 0.0;
+        s = fmax(s, (fabs(P[P_VY * qs + c]) + a) / jL);
+        s = fmax(s, (fabs(P[P_VX * qs + c]) + a) / iL);
+        if (dims == 3) s = fmax(s, (fabs(P[P_VZ * qs + c]) + a) / kL);
+        if (c == ncells - 1) *last_signal = s;
+        sig = fmax(sig, s);
+    }
+    for (int off = 32; off > 0; off >>= 1) sig = fmax(sig, __shfl_xor(sig, off));
+    if (__lane_id() == 0) wave_max[threadIdx.x / 64] = sig;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = fmax(fmax(wave_max[0], wave_max[1]), fmax(wave_max[2], wave_max[3]));
+        atomicMax(smax_bits, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
 } // namespace e4

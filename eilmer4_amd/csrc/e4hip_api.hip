@@ -24,9 +24,15 @@ using e4::Prim;
 using e4::TILE_X;
 using e4::TILE_Y;
 
+enum { GAS_IDEAL = 0, GAS_TP = 1 };
+
 struct GasModelH {
-    IdealGas ideal;
-    double mMass;
+    int type = GAS_IDEAL;
+    IdealGas ideal{};
+    double mMass = 0.0;
+    int nsp = 1;
+    e4::TPGas tp{};
+    e4::TPGas* tp_dev = nullptr;     // device copy read by the kernels
 };
 
 struct BlockH {
@@ -36,10 +42,17 @@ struct BlockH {
     double dx = 0, dy = 0, dz = 0;
     long ncells = 0, qs = 0;
     int nq = 5;
+    int nsp = 1;
     double* U[3] = {nullptr, nullptr, nullptr};   // U[0] is the current state
     double* d[2] = {nullptr, nullptr};            // dUdt of stages 1 and 2
     int bc_type[6] = {0, 0, 0, 0, 0, 0};
     Prim bc_fs[6] = {};
+    // thermally perfect gas only: primitive arrays (ping-pong) and face fluxes
+    double* P[2] = {nullptr, nullptr};
+    double* F[3] = {nullptr, nullptr, nullptr};
+    long fqs[3] = {0, 0, 0};
+    double bc_fs_tp[6][8 + e4::TP_NSP_MAX] = {};
+    bool is_tp() const;
 };
 
 struct ProfEvent { int cls; hipEvent_t a, b; };
@@ -63,6 +76,8 @@ struct State {
 };
 
 State g_state;
+
+bool BlockH::is_tp() const { return g_state.gases[gm].type == GAS_TP; }
 
 int fail(int code, const char* fmt, ...)
 {
@@ -232,10 +247,97 @@ int launch_stage(const BlockH& b, int stage, double dt, unsigned int* counters)
     return E4HIP_OK;
 }
 
+// Stage sequencing of the thermally perfect path: the same U/dUdt buffer
+// use as launch_stage; the primitive state goes P[0] -> P[1] -> P[0] ...
+int launch_tp_stage(BlockH& b, int stage, double dt, unsigned int* counters)
+{
+    const E4hipStepConfig& cfg = g_state.cfg;
+    const int sch = cfg.gasdynamic_update_scheme;
+    const int ns = nstages(sch);
+    e4::TPArgs A;
+    memset(&A, 0, sizeof(A));
+    A.qs = b.qs;
+    for (int dd = 0; dd < 3; ++dd) { A.F[dd] = b.F[dd]; A.fqs[dd] = b.fqs[dd]; }
+    A.ni = b.ni; A.nj = b.nj; A.nk = (b.dims == 3) ? b.nk : 1; A.dims = b.dims;
+    A.stage = stage;
+    A.high_order = cfg.interpolation_order > 1;
+    A.apply_limiter = cfg.apply_limiter;
+    A.extrema_clipping = cfg.extrema_clipping;
+    A.counters = counters;
+    A.gas = g_state.gases[b.gm].tp_dev;
+    for (int f = 0; f < 6; ++f) {
+        A.bc_type[f] = b.bc_type[f];
+        memcpy(A.bc_fs[f], b.bc_fs_tp[f], sizeof(A.bc_fs[f]));
+    }
+    if (b.dims == 3) {
+        A.vol_inv = 1.0 / (b.dx * b.dy * b.dz);
+        A.area[0] = b.dy * b.dz; A.area[1] = b.dz * b.dx; A.area[2] = b.dx * b.dy;
+    } else {
+        A.vol_inv = 1.0 / (b.dx * b.dy);
+        A.area[0] = b.dy; A.area[1] = b.dx; A.area[2] = 0.0;
+    }
+    A.w[0] = l2r2_weights(b.dx);
+    A.w[1] = l2r2_weights(b.dy);
+    A.w[2] = l2r2_weights(b.dz);
+    double g1 = 1.0, g2 = 0.0, g3 = 0.0;
+    if (stage == 1) {
+        if (sch == E4HIP_UPDATE_MIDPOINT || sch == E4HIP_UPDATE_CLASSIC_RK3) g1 = 0.5;
+        if (sch == E4HIP_UPDATE_DENMAN_RK3) g1 = 8.0 / 15.0;
+        A.Uold = b.U[0]; A.Unew = b.U[1];
+        A.dout = (ns > 1) ? b.d[0] : nullptr;
+    } else if (stage == 2) {
+        g1 = 0.5; g2 = 0.5;
+        switch (sch) {
+        case E4HIP_UPDATE_MIDPOINT: g1 = 0.0; g2 = 1.0; break;
+        case E4HIP_UPDATE_CLASSIC_RK3: g1 = -1.0; g2 = 2.0; break;
+        case E4HIP_UPDATE_TVD_RK3: g1 = 0.25; g2 = 0.25; break;
+        case E4HIP_UPDATE_DENMAN_RK3: g1 = -17.0 / 60.0; g2 = 5.0 / 12.0; break;
+        default: break;
+        }
+        A.d0 = b.d[0];
+        if (ns == 2) { A.Uold = b.U[0]; A.Unew = b.U[0]; A.dout = nullptr; }
+        else {
+            A.Uold = (sch == E4HIP_UPDATE_DENMAN_RK3) ? b.U[1] : b.U[0];
+            A.Unew = b.U[2]; A.dout = b.d[1];
+        }
+    } else {
+        g1 = 1.0 / 6.0; g2 = 1.0 / 6.0; g3 = 4.0 / 6.0;
+        if (sch == E4HIP_UPDATE_CLASSIC_RK3) { g1 = 1.0 / 6.0; g2 = 4.0 / 6.0; g3 = 1.0 / 6.0; }
+        if (sch == E4HIP_UPDATE_DENMAN_RK3) { g1 = 0.0; g2 = -5.0 / 12.0; g3 = 3.0 / 4.0; }
+        A.d0 = b.d[0]; A.d1 = b.d[1];
+        A.Uold = (sch == E4HIP_UPDATE_DENMAN_RK3) ? b.U[2] : b.U[0];
+        A.Unew = b.U[0];
+    }
+    A.dt = dt; A.g1 = g1; A.g2 = g2; A.g3 = g3;
+    A.c1 = dt * g1;
+    A.Pin = b.P[0];
+    A.Pout = b.P[1];
+    const int flux = (cfg.flux_calculator == E4HIP_FLUX_HLLC) ? e4::FLUX_HLLC : e4::FLUX_AUSMDV;
+    hipEvent_t ev = nullptr;
+    prof_begin(&ev);
+    if (cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT) e4::launch_tp_stage_exact(A, g_state.stream, flux);
+    else e4::launch_tp_stage_fast64(A, g_state.stream, flux);
+    prof_end(0, ev);
+    HIP_TRY(hipGetLastError());
+    std::swap(b.P[0], b.P[1]);
+    return E4HIP_OK;
+}
+
 int alloc_block_buffers(BlockH& b)
 {
     const size_t bytes = sizeof(double) * (size_t)b.qs * b.nq;
     const int ns = nstages(g_state.cfg.gasdynamic_update_scheme);
+    if (b.is_tp()) {
+        const size_t pbytes = sizeof(double) * (size_t)b.qs * (8 + b.nsp);
+        for (int t = 0; t < 2; ++t) if (!b.P[t]) HIP_TRY(hipMalloc(&b.P[t], pbytes));
+        const long nk = (b.dims == 3) ? b.nk : 1;
+        const long nf[3] = {(long)(b.ni + 1) * b.nj * nk, (long)b.ni * (b.nj + 1) * nk,
+                            (b.dims == 3) ? (long)b.ni * b.nj * (b.nk + 1) : 1};
+        for (int dd = 0; dd < 3; ++dd) {
+            b.fqs[dd] = (nf[dd] + 63) / 64 * 64;
+            if (!b.F[dd]) HIP_TRY(hipMalloc(&b.F[dd], sizeof(double) * b.fqs[dd] * b.nq));
+        }
+    }
     if (!b.U[0]) HIP_TRY(hipMalloc(&b.U[0], bytes));
     if (!b.U[1]) HIP_TRY(hipMalloc(&b.U[1], bytes));
     if (ns >= 2 && !b.d[0]) HIP_TRY(hipMalloc(&b.d[0], bytes));
@@ -250,6 +352,8 @@ int free_block_buffers(BlockH& b)
 {
     for (auto& p : b.U) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     for (auto& p : b.d) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
+    for (auto& p : b.P) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
+    for (auto& p : b.F) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     return E4HIP_OK;
 }
 
@@ -322,6 +426,35 @@ int e4hip_gas_ideal(double mMass, double gamma, int* gm)
     return E4HIP_OK;
 }
 
+int e4hip_gas_tp(int nsp, const double* M, int nseg, const double* Tbreaks, const double* Tblends,
+                 const double* coeffs, int* gm)
+{
+    if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
+    if (nsp != 5) return fail(E4HIP_ERR_ARG, "the device path is built for 5 species (got %d)", nsp);
+    if (nseg < 1 || nseg > e4::TP_NSEG_MAX || !M || !Tbreaks || !Tblends || !coeffs || !gm)
+        return fail(E4HIP_ERR_ARG, "bad thermally perfect gas parameters");
+    GasModelH g;
+    g.type = GAS_TP;
+    g.nsp = nsp;
+    g.tp.nsp = nsp;
+    for (int s = 0; s < nsp; ++s) {
+        if (!(M[s] > 0.0)) return fail(E4HIP_ERR_ARG, "molecular mass must be positive");
+        g.tp.R[s] = 8.31451 / M[s];                   // therm_perf_gas.d:68
+        e4::CeaCurve& cv = g.tp.curve[s];
+        cv.R = g.tp.R[s];
+        cv.nseg = nseg;
+        for (int q = 0; q <= nseg; ++q) cv.Tb[q] = Tbreaks[s * (nseg + 1) + q];
+        for (int q = 0; q < nseg - 1; ++q) cv.Tbl[q] = Tblends[s * (nseg - 1) + q];
+        for (int q = 0; q < nseg; ++q)
+            for (int j = 0; j < 9; ++j) cv.c[q][j] = coeffs[(s * nseg + q) * 9 + j];
+    }
+    HIP_TRY(hipMalloc(&g.tp_dev, sizeof(e4::TPGas)));
+    HIP_TRY(hipMemcpy(g.tp_dev, &g.tp, sizeof(e4::TPGas), hipMemcpyHostToDevice));
+    g_state.gases.push_back(g);
+    *gm = (int)g_state.gases.size() - 1;
+    return E4HIP_OK;
+}
+
 int e4hip_block_create(int gm, int dims, int ni, int nj, int nk, double dx, double dy, double dz, int* blk)
 {
     if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
@@ -337,7 +470,8 @@ int e4hip_block_create(int gm, int dims, int ni, int nj, int nk, double dx, doub
     b.dx = dx; b.dy = dy; b.dz = (dims == 3) ? dz : 1.0;
     b.ncells = (long)ni * nj * nk;
     b.qs = (b.ncells + 63) / 64 * 64;
-    b.nq = 5;
+    b.nsp = g_state.gases[gm].nsp;
+    b.nq = (b.nsp > 1) ? 5 + b.nsp : 5;
     int rc = alloc_block_buffers(b);
     if (rc) { free_block_buffers(b); return rc; }
     g_state.blocks.push_back(b);
@@ -361,7 +495,11 @@ int e4hip_block_set_bc(int blk, int face, int type, const double* fs)
     if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
     if (face < 0 || face > 5 || (b->dims == 2 && face > 3)) return fail(E4HIP_ERR_ARG, "bad face %d", face);
     if (type < E4HIP_BC_SLIP_WALL || type > E4HIP_BC_PERIODIC) return fail(E4HIP_ERR_ARG, "bad bc type %d", type);
-    if (type == E4HIP_BC_INFLOW) {
+    if (type == E4HIP_BC_INFLOW && b->is_tp()) {
+        if (!fs) return fail(E4HIP_ERR_ARG, "inflow needs a flow state");
+        if (!(fs[0] > 0) || !(fs[2] > 0) || !(fs[4] > 0)) return fail(E4HIP_ERR_ARG, "invalid inflow state");
+        for (int q = 0; q < 8 + b->nsp; ++q) b->bc_fs_tp[face][q] = fs[q];
+    } else if (type == E4HIP_BC_INFLOW) {
         if (!fs) return fail(E4HIP_ERR_ARG, "inflow needs a flow state");
         // fs = rho, p, T, u, a, vx, vy, vz
         Prim P;
@@ -395,15 +533,42 @@ static int run_decode(const BlockH& b, double* fs_dev, unsigned int* nbad)
     return E4HIP_OK;
 }
 
+static e4::TPArgs tp_args_for(const BlockH& b)
+{
+    e4::TPArgs A;
+    memset(&A, 0, sizeof(A));
+    A.qs = b.qs; A.ni = b.ni; A.nj = b.nj; A.nk = (b.dims == 3) ? b.nk : 1; A.dims = b.dims;
+    A.gas = g_state.gases[b.gm].tp_dev;
+    A.counters = g_state.counters;
+    return A;
+}
+
 int e4hip_block_upload(int blk, const double* U, const double* Tguess)
 {
-    (void)Tguess;
     BlockH* b = get_block(blk);
     if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
     if (!U) return fail(E4HIP_ERR_ARG, "null U");
+    if (b->is_tp() && !Tguess) return fail(E4HIP_ERR_ARG, "a thermally perfect gas needs the temperature guess");
     for (int q = 0; q < b->nq; ++q)
         HIP_TRY(hipMemcpyAsync(b->U[0] + (size_t)q * b->qs, U + (size_t)q * b->ncells, sizeof(double) * b->ncells,
                                hipMemcpyHostToDevice, g_state.stream));
+    if (b->is_tp()) {
+        double* Tg = nullptr;
+        HIP_TRY(hipMalloc(&Tg, sizeof(double) * b->ncells));
+        HIP_TRY(hipMemcpyAsync(Tg, Tguess, sizeof(double) * b->ncells, hipMemcpyHostToDevice, g_state.stream));
+        hipLaunchKernelGGL(e4::reset_counters_kernel, dim3(1), dim3(64), 0, g_state.stream, g_state.counters, 4, nullptr);
+        e4::TPArgs A = tp_args_for(*b);
+        A.Pout = b->P[0];
+        if (g_state.cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT) e4::launch_tp_decode_exact(A, b->U[0], Tg, g_state.stream);
+        else e4::launch_tp_decode_fast64(A, b->U[0], Tg, g_state.stream);
+        HIP_TRY(hipGetLastError());
+        unsigned int nbad = 0;
+        HIP_TRY(hipMemcpyAsync(&nbad, g_state.counters, sizeof(unsigned int), hipMemcpyDeviceToHost, g_state.stream));
+        HIP_TRY(hipStreamSynchronize(g_state.stream));
+        HIP_TRY(hipFree(Tg));
+        if (nbad) return fail(E4HIP_ERR_DECODE, "%u cells of block %d do not decode to a valid state", nbad, blk);
+        return E4HIP_OK;
+    }
     unsigned int nbad = 0;
     int rc = run_decode(*b, nullptr, &nbad);
     if (rc) return rc;
@@ -420,7 +585,15 @@ int e4hip_block_download(int blk, double* U, double* fs)
             HIP_TRY(hipMemcpyAsync(U + (size_t)q * b->ncells, b->U[0] + (size_t)q * b->qs, sizeof(double) * b->ncells,
                                    hipMemcpyDeviceToHost, g_state.stream));
     }
-    if (fs) {
+    if (fs && b->is_tp()) {
+        // primitive layout rho,vx,vy,vz,u,T,p,a,Y -> flow-state layout rho,p,T,u,a,vx,vy,vz,Y
+        const int map[8] = {e4::P_RHO, e4::P_P, e4::P_T, e4::P_U, e4::P_A, e4::P_VX, e4::P_VY, e4::P_VZ};
+        for (int q = 0; q < 8 + b->nsp; ++q) {
+            int src = (q < 8) ? map[q] : q;
+            HIP_TRY(hipMemcpyAsync(fs + (size_t)q * b->ncells, b->P[0] + (size_t)src * b->qs,
+                                   sizeof(double) * b->ncells, hipMemcpyDeviceToHost, g_state.stream));
+        }
+    } else if (fs) {
         double* fs_dev = nullptr;
         HIP_TRY(hipMalloc(&fs_dev, sizeof(double) * 8 * b->ncells));
         unsigned int nbad = 0;
@@ -469,7 +642,8 @@ int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_ce
     for (int stage = 1; stage <= ns; ++stage) {
         for (int n = 0; n < nblk; ++n) {
             BlockH& b = *get_block(blks[n]);
-            rc = launch_stage(b, stage, dt, g_state.counters + 4 * (n * 3 + (stage - 1)));
+            rc = b.is_tp() ? launch_tp_stage(b, stage, dt, g_state.counters + 4 * (n * 3 + (stage - 1)))
+                           : launch_stage(b, stage, dt, g_state.counters + 4 * (n * 3 + (stage - 1)));
             if (rc) return rc;
         }
     }
@@ -518,7 +692,11 @@ int e4hip_determine_dt(const int* blks, int nblk, double dt_current, double cfl_
         prof_begin(&ev);
         long nwg = (b->ncells + 255) / 256;
         if (nwg > 4096) nwg = 4096;
-        hipLaunchKernelGGL(e4::cfl_kernel<e4::ModeExact>, dim3((unsigned)nwg), dim3(256), 0, g_state.stream, C);
+        if (b->is_tp())
+            hipLaunchKernelGGL(e4::tp_cfl_kernel, dim3((unsigned)nwg), dim3(256), 0, g_state.stream, b->P[0], b->qs,
+                               b->ni, b->nj, b->nk, b->dims, b->dx, b->dy, b->dz, g_state.smax, g_state.last_signal);
+        else
+            hipLaunchKernelGGL(e4::cfl_kernel<e4::ModeExact>, dim3((unsigned)nwg), dim3(256), 0, g_state.stream, C);
         prof_end(1, ev);
         HIP_TRY(hipGetLastError());
         unsigned long long bits = 0;

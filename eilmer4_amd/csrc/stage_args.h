@@ -2,6 +2,7 @@
 // API and the kernel translation units.
 #pragma once
 #include "gasdyn_device.h"
+#include "tp_gas_device.h"
 
 namespace e4 {
 
@@ -30,11 +31,43 @@ struct StageArgs {
     Prim bc_fs[6];                    // inflow ghost states (global frame)
 };
 
+enum { P_RHO = 0, P_VX = 1, P_VY = 2, P_VZ = 3, P_U = 4, P_T = 5, P_P = 6, P_A = 7, P_Y = 8 };
+
+struct TPArgs {
+    const double* __restrict__ Pin;   // primitive state the stage starts from [8+nsp][qs]
+    double* __restrict__ Pout;        // primitive state after the update
+    const double* __restrict__ Uold;
+    const double* __restrict__ d0;
+    const double* __restrict__ d1;
+    double* __restrict__ dout;
+    double* __restrict__ Unew;
+    double* F[3];                     // face fluxes [5+nsp][nfaces(dir)]
+    unsigned int* counters;           // [0] invalid, [1] decode failures, [2] Newton f-evals (cells)
+    long qs;
+    long fqs[3];
+    int ni, nj, nk, dims;
+    int stage;
+    int high_order, apply_limiter, extrema_clipping;
+    double c1, dt, g1, g2, g3;
+    double vol_inv;
+    double area[3];
+    L2R2W w[3];
+    const TPGas* gas;                 // device copy of the gas model (read-only)
+    int bc_type[6];
+    double bc_fs[6][8 + TP_NSP_MAX];  // ghost flow state of inflow faces: rho,p,T,u,a,vx,vy,vz,Y
+};
+
 // Launchers, one per arithmetic mode; each lives in its own translation unit
 // (stage_exact.hip is compiled without FMA contraction, stage_fast.hip with it).
 void launch_stage_exact(const StageArgs& A, dim3 grid, hipStream_t stream, int dims, int flux);
 void launch_stage_fast64(const StageArgs& A, dim3 grid, hipStream_t stream, int dims, int flux);
 void launch_stage_mixed(const StageArgs& A, dim3 grid, hipStream_t stream, int dims, int flux);
+// thermally perfect mixture (five species): faces of every direction, then the
+// update; decode of an uploaded state
+void launch_tp_stage_exact(const TPArgs& A, hipStream_t stream, int flux);
+void launch_tp_stage_fast64(const TPArgs& A, hipStream_t stream, int flux);
+void launch_tp_decode_exact(const TPArgs& A, const double* U, const double* Tguess, hipStream_t stream);
+void launch_tp_decode_fast64(const TPArgs& A, const double* U, const double* Tguess, hipStream_t stream);
 
 constexpr int TILE_X = 32;
 constexpr int TILE_Y = 8;

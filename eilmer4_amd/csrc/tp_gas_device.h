@@ -1,0 +1,260 @@
+// tp_gas_device.h -- thermally perfect gas mixture on the device: NASA-9 (CEA)
+// curves with linear blending, the bracketed Newton temperature solve with the
+// reference's side effect on u, pressure and frozen sound speed.
+//
+// Follows gas/thermo/cea_thermo_curves.d:48-169, therm_perf_gas_mix_eos.d:63-153,
+// nm/newton.d:40-107, perf_gas_mix_eos.d:43-94 and therm_perf_gas.d:177-196,
+// 345-385.  Templated on the arithmetic mode M of gasdyn_device.h (division
+// rule); these functions are always evaluated in FP64.
+#pragma once
+#include "gasdyn_device.h"
+
+namespace e4 {
+
+constexpr int TP_NSP_MAX = 8;
+constexpr int TP_NSEG_MAX = 4;
+
+struct CeaCurve {
+    double R;
+    int nseg;
+    double Tb[TP_NSEG_MAX + 1];
+    double Tbl[TP_NSEG_MAX];
+    double c[TP_NSEG_MAX][9];
+};
+
+struct TPGas {
+    int nsp;
+    double R[TP_NSP_MAX];
+    CeaCurve curve[TP_NSP_MAX];
+};
+
+template <int NSP>
+struct TPState {
+    double rho, p, T, u, a;
+    double Y[NSP];
+};
+
+// cea_thermo_curves.d:136-169
+__device__ __forceinline__ void cea_coefficients(const CeaCurve& cv, double T, double a[9])
+{
+    const int nb = cv.nseg + 1;
+    int seg = -1;
+    double wB = 0.0;
+    if (T < (cv.Tb[1] - 0.5 * cv.Tbl[0])) {
+        seg = 0;
+    } else if (T > (cv.Tb[nb - 2] + 0.5 * cv.Tbl[nb - 3])) {
+        seg = cv.nseg - 1;
+    } else {
+        for (int i = 1; i < nb - 1; ++i) {
+            double T_blend_low = cv.Tb[i] - 0.5 * cv.Tbl[i - 1];
+            double T_blend_high = cv.Tb[i] + 0.5 * cv.Tbl[i - 1];
+            if (T >= T_blend_low && T <= T_blend_high) {
+                wB = (1. / cv.Tbl[i - 1]) * (T - T_blend_low);
+                double wA = 1.0 - wB;
+#pragma unroll
+                for (int j = 0; j < 9; ++j) a[j] = wA * cv.c[i - 1][j] + wB * cv.c[i][j];
+                return;
+            }
+            if (T > T_blend_high && T < (cv.Tb[i + 1] - 0.5 * cv.Tbl[i])) { seg = i; break; }
+        }
+        if (seg < 0) seg = 0;   // unreachable for T inside the curve range
+    }
+#pragma unroll
+    for (int j = 0; j < 9; ++j) a[j] = cv.c[seg][j];
+}
+
+// cea_thermo_curves.d:48-62
+template <class M>
+__device__ __forceinline__ double cea_Cp(const CeaCurve& cv, double T)
+{
+    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    if (T < cv.Tb[0]) T = cv.Tb[0];
+    if (T > cv.Tb[cv.nseg]) T = cv.Tb[cv.nseg];
+    double a[9];
+    cea_coefficients(cv, T, a);
+    double Cp_on_R = dv<MD>(a[0], T * T) + dv<MD>(a[1], T) + a[2] + a[3] * T;
+    Cp_on_R += a[4] * T * T + a[5] * T * T * T + a[6] * T * T * T * T;
+    return cv.R * Cp_on_R;
+}
+
+// cea_thermo_curves.d:69-90 inside the curve range
+template <class M>
+__device__ __forceinline__ double cea_h_inside(const CeaCurve& cv, double T)
+{
+    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    double a[9];
+    cea_coefficients(cv, T, a);
+    double h_on_RT = dv<MD>(-a[0], T * T) + dv<MD>(a[1], T) * log(T) + a[2] + dv<MD>(a[3] * T, 2.0);
+    h_on_RT += dv<MD>(a[4] * T * T, 3.0) + dv<MD>(a[5] * T * T * T, 4.0) + dv<MD>(a[6] * T * T * T * T, 5.0) +
+               dv<MD>(a[7], T);
+    return cv.R * T * h_on_RT;
+}
+
+// cea_thermo_curves.d:63-91 (constant-Cp extension outside the range)
+template <class M>
+__device__ __forceinline__ double cea_h(const CeaCurve& cv, double T)
+{
+    if (T < cv.Tb[0]) {
+        double T_low = cv.Tb[0];
+        return cea_h_inside<M>(cv, T_low) - cea_Cp<M>(cv, T_low) * (T_low - T);
+    }
+    if (T > cv.Tb[cv.nseg]) {
+        double T_high = cv.Tb[cv.nseg];
+        return cea_h_inside<M>(cv, T_high) + cea_Cp<M>(cv, T_high) * (T - T_high);
+    }
+    return cea_h_inside<M>(cv, T);
+}
+
+// therm_perf_gas_mix_eos.d:63-69 (mass_average, gas_model.d:330-338)
+template <class M, int NSP>
+__device__ __forceinline__ double tp_energy(const TPGas& g, const double* Y, double T)
+{
+    double result = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) {
+        double e = cea_h<M>(g.curve[s], T) - g.R[s] * T;
+        result += Y[s] * e;
+    }
+    return result;
+}
+
+// -(sum Y_s (Cp_s - R_s)), the dzdT closure of therm_perf_gas_mix_eos.d:116-123
+template <class M, int NSP>
+__device__ __forceinline__ double tp_dzdT(const TPGas& g, const double* Y, double T)
+{
+    double result = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) {
+        double Cv = cea_Cp<M>(g.curve[s], T) - g.R[s];
+        result += Y[s] * Cv;
+    }
+    return -1.0 * result;
+}
+
+// nm/newton.d:40-107 with the zeroFn/dzdT closures; Q.T and Q.u receive the
+// closures' side effects.  Returns 0 and the root, or -1 when it would throw.
+template <class M, int NSP>
+__device__ int tp_newton(const TPGas& g, TPState<NSP>& Q, double e_tgt, double x0, double xMin, double xMax,
+                         double tol, double* root, int* fevals)
+{
+    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    auto f = [&](double T) {
+        Q.T = T;
+        Q.u = tp_energy<M, NSP>(g, Q.Y, T);
+        (*fevals)++;
+        return e_tgt - Q.u;
+    };
+    const int MAXIT = 30;
+    double xL = xMin, xH = xMax;
+    double fL = f(xL);
+    double fH = f(xH);
+    if ((fL > 0.0 && fH > 0.0) || (fL < 0.0 && fH < 0.0)) return -1;
+    if (fL == 0.0) { *root = xMin; return 0; }
+    if (fH == 0.0) { *root = xMax; return 0; }
+    if (fL < 0.0) { xL = xMin; xH = xMax; } else { xH = xMin; xL = xMax; }
+    double rts = x0;
+    double dxold = (xMax - xMin);
+    double dx = dxold;
+    double f0 = f(rts);
+    double df0 = tp_dzdT<M, NSP>(g, Q.Y, rts);
+    for (int j = 0; j < MAXIT; ++j) {
+        if ((((rts - xH) * df0 - f0) * ((rts - xL) * df0 - f0) > 0.0) || (fabs(2.0 * f0) > fabs(dxold * df0))) {
+            dxold = dx;
+            dx = 0.5 * (xH - xL);
+            rts = xL + dx;
+            if (xL == rts) { *root = rts; return 0; }
+        } else {
+            dxold = dx;
+            dx = dv<MD>(f0, df0);
+            double tmp = rts;
+            rts -= dx;
+            if (tmp == rts) { *root = rts; return 0; }
+        }
+        if (fabs(dx) < tol) { *root = rts; return 0; }
+        f0 = f(rts);
+        df0 = tp_dzdT<M, NSP>(g, Q.Y, rts);
+        if (f0 < 0.0) xL = rts; else xH = rts;
+    }
+    return -1;
+}
+
+// therm_perf_gas_mix_eos.d:70-153: guess = Q.T, bracket [max(T-500, 20), +1000],
+// then [20, 1e5]; tolerance 1e-6 K.
+template <class M, int NSP>
+__device__ int tp_update_temperature(const TPGas& g, TPState<NSP>& Q, int* fevals)
+{
+    double Tsave = Q.T;
+    double e_tgt = Q.u;
+    double delT = 1000.0;
+    double T1 = fmax(Q.T - 0.5 * delT, 20.0);
+    double T2 = T1 + delT;
+    double root;
+    if (tp_newton<M, NSP>(g, Q, e_tgt, Tsave, T1, T2, 1.0e-6, &root, fevals) == 0) { Q.T = root; return 0; }
+    if (tp_newton<M, NSP>(g, Q, e_tgt, Tsave, 20.0, 100000.0, 1.0e-6, &root, fevals) == 0) { Q.T = root; return 0; }
+    Q.T = Tsave;
+    Q.u = tp_energy<M, NSP>(g, Q.Y, Tsave);
+    return -1;
+}
+
+template <int NSP>
+__device__ __forceinline__ double tp_Rmix(const TPGas& g, const double* Y)
+{
+    double Rmix = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) Rmix += Y[s] * g.R[s];
+    return Rmix;
+}
+
+// therm_perf_gas.d:187-191 with perf_gas_mix_eos.d:43-49
+template <class M, int NSP>
+__device__ __forceinline__ int tp_update_thermo_from_rhou(const TPGas& g, TPState<NSP>& Q, int* fevals)
+{
+    if (tp_update_temperature<M, NSP>(g, Q, fevals) != 0) return -1;
+    double Rmix = tp_Rmix<NSP>(g, Q.Y);
+    Q.p = Q.rho * Rmix * Q.T;
+    return 0;
+}
+
+// therm_perf_gas.d:177-181 (low-T fallback path of decode)
+template <class M, int NSP>
+__device__ __forceinline__ void tp_update_thermo_from_rhoT(const TPGas& g, TPState<NSP>& Q)
+{
+    Q.u = tp_energy<M, NSP>(g, Q.Y, Q.T);
+    double Rmix = tp_Rmix<NSP>(g, Q.Y);
+    Q.p = Q.rho * Rmix * Q.T;
+}
+
+// therm_perf_gas.d:345-385: a = sqrt(Cp/Cv * R T), frozen
+template <class M, int NSP>
+__device__ __forceinline__ double tp_sound_speed(const TPGas& g, const TPState<NSP>& Q)
+{
+    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    double Cp = 0.0, Cv = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) Cp += Q.Y[s] * cea_Cp<M>(g.curve[s], Q.T);
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) Cv += Q.Y[s] * (cea_Cp<M>(g.curve[s], Q.T) - g.R[s]);
+    double R = tp_Rmix<NSP>(g, Q.Y);
+    return sqrt(dv<MD>(Cp, Cv) * (R * Q.T));
+}
+
+// gas/gas_model.d:296-328 (tolerance 0, assert_error_tolerance 0.1)
+template <class M, int NSP>
+__device__ __forceinline__ int scale_mass_fractions(double* Y)
+{
+    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    double sum = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) {
+        Y[s] = Y[s] >= 0.0 ? Y[s] : 0.0;
+        sum += Y[s];
+    }
+    if (fabs(sum - 1.0) > 0.1) return -1;
+    if (fabs(sum - 1.0) > 0.0) {
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) Y[s] = dv<MD>(Y[s], sum);
+    }
+    return 0;
+}
+
+} // namespace e4

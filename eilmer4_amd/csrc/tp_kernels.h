@@ -1,0 +1,412 @@
+// tp_kernels.h -- the explicit update for a thermally perfect gas mixture.
+//
+// The face EOS is a Newton solve on each side of every face (the rhou thermo
+// interpolator, onedinterp.d:534-551 with the mixin of 1039-1068), so this path
+// is compute-bound; it keeps a per-cell primitive array (rho, vel, u, T, p, a,
+// Y[NSP]; SoA, ping-ponged between stages) because u and T are not functions
+// of U alone here: T is the previous stage's Newton result (the next solve's
+// guess) and u is the energy of the last Newton iterate
+// (therm_perf_gas_mix_eos.d:104-114).
+//
+//   tp_face_kernel<M,NSP,FLUX,DIR>  one thread per face of direction DIR:
+//       l2r2 reconstruction of velocity (face frame), mass fractions (then
+//       scale_mass_fractions), rho and u; face Newton; flux; global frame.
+//   tp_update_kernel<M,NSP>         one thread per cell: time_derivatives,
+//       stage update, decode_conserved with Newton (guess: the cell's T),
+//       low-T fallback, check_data; writes U and the primitive state.
+#pragma once
+#include "stage_args.h"
+#include "tp_gas_device.h"
+
+namespace e4 {
+
+template <int NSP>
+struct TPCell {
+    double rho, p, T, u, a;
+    double v[3];
+    double Y[NSP];
+};
+
+template <int NSP>
+__device__ __forceinline__ TPCell<NSP> tp_load(const TPArgs& A, long c)
+{
+    TPCell<NSP> s;
+    const double* P = A.Pin;
+    const long q = A.qs;
+    s.rho = P[P_RHO * q + c]; s.v[0] = P[P_VX * q + c]; s.v[1] = P[P_VY * q + c]; s.v[2] = P[P_VZ * q + c];
+    s.u = P[P_U * q + c]; s.T = P[P_T * q + c]; s.p = P[P_P * q + c]; s.a = P[P_A * q + c];
+#pragma unroll
+    for (int sp = 0; sp < NSP; ++sp) s.Y[sp] = P[(P_Y + sp) * q + c];
+    return s;
+}
+
+// The cell at (i,j,k) or the ghost value its boundary effect gives it (the same
+// mapping as the ideal-gas stage kernel's fetch_ghost).
+template <int NSP>
+__device__ TPCell<NSP> tp_fetch(const TPArgs& A, int i, int j, int k)
+{
+    if (i >= 0 && i < A.ni && j >= 0 && j < A.nj && k >= 0 && k < A.nk)
+        return tp_load<NSP>(A, ((long)k * A.nj + j) * A.ni + i);
+    int dir, idx, n;
+    if (i < 0 || i >= A.ni) { dir = 0; idx = i; n = A.ni; }
+    else if (j < 0 || j >= A.nj) { dir = 1; idx = j; n = A.nj; }
+    else { dir = 2; idx = k; n = A.nk; }
+    const bool plus = idx >= n;
+    const int layer = plus ? idx - n + 1 : -idx;
+    const int face = (dir == 0) ? (plus ? FACE_EAST : FACE_WEST)
+                   : (dir == 1) ? (plus ? FACE_NORTH : FACE_SOUTH)
+                                : (plus ? FACE_TOP : FACE_BOTTOM);
+    const int type = A.bc_type[face];
+    if (type == BC_INFLOW) {
+        const double* f = A.bc_fs[face];
+        TPCell<NSP> s;
+        s.rho = f[0]; s.p = f[1]; s.T = f[2]; s.u = f[3]; s.a = f[4];
+        s.v[0] = f[5]; s.v[1] = f[6]; s.v[2] = f[7];
+#pragma unroll
+        for (int sp = 0; sp < NSP; ++sp) s.Y[sp] = f[8 + sp];
+        return s;
+    }
+    int src;
+    if (type == BC_EXTRAPOLATE) src = plus ? n - 1 : 0;
+    else if (type == BC_PERIODIC) src = plus ? layer - 1 : n - layer;
+    else src = plus ? n - layer : layer - 1;
+    int si = i, sj = j, sk = k;
+    if (dir == 0) si = src; else if (dir == 1) sj = src; else sk = src;
+    TPCell<NSP> s = tp_load<NSP>(A, ((long)sk * A.nj + sj) * A.ni + si);
+    if (type == BC_SLIP_WALL) s.v[dir] = -(s.v[dir]);
+    return s;
+}
+
+// fluxcalc.d:376-529 with species; velocities in the face frame (v[0] normal).
+template <class M, int NSP>
+__device__ __forceinline__ void tp_ausmdv(const TPCell<NSP>& L, const TPCell<NSP>& Rt, double* F)
+{
+    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    double rL = L.rho, pL = L.p, pLrL = dv<MD>(pL, rL), uL = L.v[0], vL = L.v[1], wL = L.v[2];
+    double eL = L.u, aL = L.a;
+    double keL = 0.5 * (uL * uL + vL * vL + wL * wL);
+    double HL = eL + pLrL + keL;
+    double rR = Rt.rho, pR = Rt.p, pRrR = dv<MD>(pR, rR), uR = Rt.v[0], vR = Rt.v[1], wR = Rt.v[2];
+    double eR = Rt.u, aR = Rt.a;
+    double keR = 0.5 * (uR * uR + vR * vR + wR * wR);
+    double HR = eR + pRrR + keR;
+    double alphaL = dv<MD>(2.0 * pLrL, pLrL + pRrR);
+    double alphaR = dv<MD>(2.0 * pRrR, pLrL + pRrR);
+    double am = fmax(aL, aR);
+    double ML = dv<MD>(uL, am);
+    double MR = dv<MD>(uR, am);
+    double pLplus, uLplus;
+    double duL = 0.5 * (uL + fabs(uL));
+    if (fabs(ML) <= 1.0) {
+        pLplus = pL * (ML + 1.0) * (ML + 1.0) * (2.0 - ML) * 0.25;
+        uLplus = alphaL * (dv<MD>((uL + am) * (uL + am), 4.0 * am) - duL) + duL;
+    } else {
+        pLplus = dv<MD>(pL * duL, uL);
+        uLplus = duL;
+    }
+    double pRminus, uRminus;
+    double duR = 0.5 * (uR - fabs(uR));
+    if (fabs(MR) <= 1.0) {
+        pRminus = pR * (MR - 1.0) * (MR - 1.0) * (2.0 + MR) * 0.25;
+        uRminus = alphaR * (dv<MD>(-(uR - am) * (uR - am), 4.0 * am) - duR) + duR;
+    } else {
+        pRminus = dv<MD>(pR * duR, uR);
+        uRminus = duR;
+    }
+    double ru_half = uLplus * rL + uRminus * rR;
+    double p_half = pLplus + pRminus;
+    double dp = pL - pR;
+    dp = dv<MD>(10.0 * fabs(dp), fmin(pL, pR));
+    double s = 0.5 * fmin(1.0, dp);
+    double ru2_AUSMV = uLplus * rL * uL + uRminus * rR * uR;
+    double ru2_AUSMD = 0.5 * (ru_half * (uL + uR) - fabs(ru_half) * (uR - uL));
+    double ru2_half = (0.5 + s) * ru2_AUSMV + (0.5 - s) * ru2_AUSMD;
+    F[0] = ru_half;
+    F[1] = ru2_half + p_half;
+    if (ru_half >= 0.0) {
+        F[2] = ru_half * vL; F[3] = ru_half * wL; F[4] = ru_half * HL;
+#pragma unroll
+        for (int i = 0; i < NSP; ++i) F[5 + i] = ru_half * L.Y[i];
+    } else {
+        F[2] = ru_half * vR; F[3] = ru_half * wR; F[4] = ru_half * HR;
+#pragma unroll
+        for (int i = 0; i < NSP; ++i) F[5 + i] = ru_half * Rt.Y[i];
+    }
+    bool caseA = ((uL - aL) < 0.0) && ((uR - aR) > 0.0);
+    bool caseB = ((uL + aL) < 0.0) && ((uR + aR) > 0.0);
+    double d_ua = 0.0;
+    if (caseA && !caseB) d_ua = 0.125 * ((uR - aR) - (uL - aL));
+    if (caseB && !caseA) d_ua = 0.125 * ((uR + aR) - (uL + aL));
+    if (d_ua != 0.0) {
+        F[0] -= d_ua * (rR - rL);
+        F[1] -= d_ua * (rR * uR - rL * uL);
+        F[2] -= d_ua * (rR * vR - rL * vL);
+        F[3] -= d_ua * (rR * wR - rL * wL);
+        F[4] -= d_ua * (rR * HR - rL * HL);
+#pragma unroll
+        for (int i = 0; i < NSP; ++i) F[5 + i] -= d_ua * (rR * Rt.Y[i] - rL * L.Y[i]);
+    }
+}
+
+// HLLC with species carried by the contact wave (the oracle's hllc).
+template <class M, int NSP>
+__device__ __forceinline__ void tp_hllc(const TPCell<NSP>& L, const TPCell<NSP>& Rt, double* F)
+{
+    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    double rL = L.rho, pL = L.p, uL = L.v[0], vL = L.v[1], wL = L.v[2], aL = L.a;
+    double EL = rL * (L.u + 0.5 * (uL * uL + vL * vL + wL * wL));
+    double rR = Rt.rho, pR = Rt.p, uR = Rt.v[0], vR = Rt.v[1], wR = Rt.v[2], aR = Rt.a;
+    double ER = rR * (Rt.u + 0.5 * (uR * uR + vR * vR + wR * wR));
+    double SL = fmin(uL - aL, uR - aR);
+    double SR = fmax(uL + aL, uR + aR);
+    double mL = rL * (SL - uL);
+    double mR = rR * (SR - uR);
+    double Sstar = dv<MD>(pR - pL + mL * uL - mR * uR, mL - mR);
+    const bool useL = (0.0 <= SL) || (!(SR <= 0.0) && (0.0 <= Sstar));
+    const TPCell<NSP>& K = useL ? L : Rt;
+    const double rK = K.rho, uK = K.v[0], vK = K.v[1], wK = K.v[2], pK = K.p, EK = useL ? EL : ER;
+    double mf = rK * uK;
+    F[0] = mf; F[1] = mf * uK + pK; F[2] = mf * vK; F[3] = mf * wK; F[4] = uK * (EK + pK);
+#pragma unroll
+    for (int i = 0; i < NSP; ++i) F[5 + i] = mf * K.Y[i];
+    if ((0.0 <= SL) || (SR <= 0.0)) return;
+    const double SK = useL ? SL : SR;
+    double coef = dv<MD>(rK * (SK - uK), SK - Sstar);
+    double Us[5];
+    Us[0] = coef;
+    Us[1] = coef * Sstar;
+    Us[2] = coef * vK;
+    Us[3] = coef * wK;
+    Us[4] = coef * (dv<MD>(EK, rK) + (Sstar - uK) * (Sstar + dv<MD>(pK, rK * (SK - uK))));
+    F[0] = F[0] + SK * (Us[0] - rK);
+    F[1] = F[1] + SK * (Us[1] - rK * uK);
+    F[2] = F[2] + SK * (Us[2] - rK * vK);
+    F[3] = F[3] + SK * (Us[3] - rK * wK);
+    F[4] = F[4] + SK * (Us[4] - EK);
+#pragma unroll
+    for (int i = 0; i < NSP; ++i) F[5 + i] = F[5 + i] + SK * (coef * K.Y[i] - rK * K.Y[i]);
+}
+
+template <class M, int NSP, int FLUX, int DIR>
+__global__ __launch_bounds__(128) void tp_face_kernel(TPArgs A)
+{
+    const int nfi = A.ni + (DIR == 0), nfj = A.nj + (DIR == 1), nfk = (A.dims == 3) ? A.nk + (DIR == 2) : 1;
+    const long nf = (long)nfi * nfj * nfk;
+    const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const int i = (int)(f % nfi);
+    const int j = (int)((f / nfi) % nfj);
+    const int k = (int)(f / ((long)nfi * nfj));
+    const int di = (DIR == 0), dj = (DIR == 1), dk = (DIR == 2);
+    // face frame permutation: i (x,y,z); j 3D (y,z,x), 2D (y,x,z); k (z,x,y)
+    const int pn = DIR, pt1 = (DIR == 0) ? 1 : (DIR == 1) ? ((A.dims == 3) ? 2 : 0) : 0;
+    const int pt2 = (DIR == 0) ? 2 : (DIR == 1) ? ((A.dims == 3) ? 0 : 2) : 1;
+    TPCell<NSP> c[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        TPCell<NSP> s = tp_fetch<NSP>(A, i + (m - 2) * di, j + (m - 2) * dj, k + (m - 2) * dk);
+        double v0 = s.v[pn], v1 = s.v[pt1], v2 = s.v[pt2];
+        s.v[0] = v0; s.v[1] = v1; s.v[2] = v2;
+        c[m] = s;
+    }
+    TPCell<NSP> Lf = c[1], Rf = c[2];
+    int fe = 0;
+    if (A.high_order) {
+        const L2R2W& w = A.w[DIR];
+        const bool lim = A.apply_limiter, clip = A.extrema_clipping;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            interp_l2r2_scalar<M>(w, lim, clip, c[0].v[d], c[1].v[d], c[2].v[d], c[3].v[d], Lf.v[d], Rf.v[d]);
+#pragma unroll
+        for (int s = 0; s < NSP; ++s)
+            interp_l2r2_scalar<M>(w, lim, clip, c[0].Y[s], c[1].Y[s], c[2].Y[s], c[3].Y[s], Lf.Y[s], Rf.Y[s]);
+        if (scale_mass_fractions<M, NSP>(Lf.Y) != 0) {
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) Lf.Y[s] = c[1].Y[s];
+        }
+        if (scale_mass_fractions<M, NSP>(Rf.Y) != 0) {
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) Rf.Y[s] = c[2].Y[s];
+        }
+        interp_l2r2_scalar<M>(w, lim, clip, c[0].rho, c[1].rho, c[2].rho, c[3].rho, Lf.rho, Rf.rho);
+        interp_l2r2_scalar<M>(w, lim, clip, c[0].u, c[1].u, c[2].u, c[3].u, Lf.u, Rf.u);
+        // update_thermo_from_rhou on each side; on failure the side is the cell's state
+        TPState<NSP> Q;
+        Q.rho = Lf.rho; Q.T = Lf.T; Q.u = Lf.u;
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) Q.Y[s] = Lf.Y[s];
+        if (tp_update_thermo_from_rhou<M, NSP>(*A.gas, Q, &fe) == 0) { Lf.T = Q.T; Lf.u = Q.u; Lf.p = Q.p; }
+        else Lf = c[1];
+        Q.rho = Rf.rho; Q.T = Rf.T; Q.u = Rf.u;
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) Q.Y[s] = Rf.Y[s];
+        if (tp_update_thermo_from_rhou<M, NSP>(*A.gas, Q, &fe) == 0) { Rf.T = Q.T; Rf.u = Q.u; Rf.p = Q.p; }
+        else Rf = c[2];
+    }
+    double F[5 + NSP];
+    if constexpr (FLUX == FLUX_HLLC) tp_hllc<M, NSP>(Lf, Rf, F);
+    else tp_ausmdv<M, NSP>(Lf, Rf, F);
+    double* dst = A.F[DIR];
+    const long fq = A.fqs[DIR];
+    dst[f] = F[0];
+    dst[(1 + pn) * fq + f] = F[1];
+    dst[(1 + pt1) * fq + f] = F[2];
+    dst[(1 + pt2) * fq + f] = F[3];
+    dst[4 * fq + f] = F[4];
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) dst[(5 + s) * fq + f] = F[5 + s];
+}
+
+template <class M, int NSP>
+__global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
+{
+    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    constexpr int NQ = 5 + NSP;
+    const long ncells = (long)A.ni * A.nj * A.nk;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool invalid = false, decode_fail = false;
+    int fe = 0;
+    if (c < ncells) {
+        const int i = (int)(c % A.ni), j = (int)((c / A.ni) % A.nj), k = (int)(c / ((long)A.ni * A.nj));
+        const long iW = ((long)k * A.nj + j) * (A.ni + 1) + i;
+        const long jS = ((long)k * (A.nj + 1) + j) * A.ni + i;
+        const long kB = c;
+        const double* Fi = A.F[0];
+        const double* Fj = A.F[1];
+        const double* Fk = A.F[2];
+        double integ[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) integ[q] = 0.0;
+        auto add_face = [&](const double* Fsrc, long fq, long fidx, double outsign, double Aface) {
+            double area = outsign * Aface;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                if (q == 3 && A.dims != 3) continue;
+                integ[q] -= Fsrc[q * fq + fidx] * area;
+            }
+        };
+        add_face(Fj, A.fqs[1], jS + A.ni, 1.0, A.area[1]);
+        add_face(Fi, A.fqs[0], iW + 1, 1.0, A.area[0]);
+        add_face(Fj, A.fqs[1], jS, -1.0, A.area[1]);
+        add_face(Fi, A.fqs[0], iW, -1.0, A.area[0]);
+        if (A.dims == 3) {
+            add_face(Fk, A.fqs[2], kB + (long)A.ni * A.nj, 1.0, A.area[2]);
+            add_face(Fk, A.fqs[2], kB, -1.0, A.area[2]);
+        }
+        const long qs = A.qs;
+        double Un[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double d = A.vol_inv * integ[q] + 0.0;
+            double dd0 = (A.stage >= 2) ? A.d0[c + q * qs] : 0.0;
+            double dd1 = (A.stage >= 3) ? A.d1[c + q * qs] : 0.0;
+            if (A.dout) A.dout[c + q * qs] = d;
+            double uold = A.Uold[c + q * qs];
+            if (A.stage == 1) Un[q] = uold + A.c1 * d;
+            else if (A.stage == 2) Un[q] = uold + A.dt * (A.g1 * dd0 + A.g2 * d);
+            else Un[q] = uold + A.dt * (A.g1 * dd0 + A.g2 * dd1 + A.g3 * d);
+        }
+        // decode_conserved (fvcell.d:563-732)
+        TPState<NSP> Q;
+        double v[3] = {0.0, 0.0, 0.0};
+        double rho = Un[0];
+        if (!(rho > 0.0)) {
+            decode_fail = true;
+        } else {
+            Q.rho = rho;
+            double dinv = dv<MD>(1.0, rho);
+            v[0] = Un[1] * dinv; v[1] = Un[2] * dinv; v[2] = Un[3] * dinv;
+            double u = Un[4] * dinv;
+            double ke = 0.5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            u -= ke;
+            Q.u = u;
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) Q.Y[s] = Un[5 + s] * dinv;
+            if (scale_mass_fractions<M, NSP>(Q.Y) != 0) {
+                decode_fail = true;
+            } else {
+                Q.T = A.Pin[P_T * qs + c];     // the previous temperature is the Newton guess
+                if (tp_update_thermo_from_rhou<M, NSP>(*A.gas, Q, &fe) != 0) {
+                    // ignore_low_T_thermo_update_failure: T = 200 K and re-encode U
+                    Q.T = 200.0;
+                    tp_update_thermo_from_rhoT<M, NSP>(*A.gas, Q);
+                    Un[1] = rho * v[0]; Un[2] = rho * v[1]; Un[3] = rho * v[2];
+                    double ke2 = 0.5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                    Un[4] = rho * (Q.u + ke2);
+#pragma unroll
+                    for (int s = 0; s < NSP; ++s) Un[5 + s] = rho * Q.Y[s];
+                }
+                Q.a = tp_sound_speed<M, NSP>(*A.gas, Q);
+                // check_data (flowstate.d:345-388)
+                double fsum = 0.0;
+#pragma unroll
+                for (int s = 0; s < NSP; ++s) fsum += Q.Y[s];
+                if (!isfinite(Q.rho) || Q.rho < 0.0) invalid = true;
+                if (!isfinite(Q.T) || Q.T < 1.01) invalid = true;
+                if (!isfinite(Q.p) || !isfinite(Q.a)) invalid = true;
+                if (fsum < 0.99 || fsum > 1.01 || !isfinite(fsum)) invalid = true;
+                if (fabs(v[0]) > 30000.0 || fabs(v[1]) > 30000.0 || fabs(v[2]) > 30000.0) invalid = true;
+                if (Q.T > 50000.0) invalid = true;
+                double* P = A.Pout;
+                P[P_RHO * qs + c] = Q.rho; P[P_VX * qs + c] = v[0]; P[P_VY * qs + c] = v[1]; P[P_VZ * qs + c] = v[2];
+                P[P_U * qs + c] = Q.u; P[P_T * qs + c] = Q.T; P[P_P * qs + c] = Q.p; P[P_A * qs + c] = Q.a;
+#pragma unroll
+                for (int s = 0; s < NSP; ++s) P[(P_Y + s) * qs + c] = Q.Y[s];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) A.Unew[c + q * qs] = Un[q];
+    }
+    unsigned int nbad = (unsigned int)__popcll(__ballot(invalid));
+    unsigned int nfail = (unsigned int)__popcll(__ballot(decode_fail));
+    if (__lane_id() == 0) {
+        if (nbad) atomicAdd(&A.counters[0], nbad);
+        if (nfail) atomicAdd(&A.counters[1], nfail);
+    }
+    (void)fe;
+}
+
+// Decode of an uploaded U with a temperature guess into the primitive array
+// (the restart path: read_solution then decode_conserved).
+template <class M, int NSP>
+__global__ __launch_bounds__(128) void tp_decode_kernel(TPArgs A, const double* __restrict__ U,
+                                                        const double* __restrict__ Tguess)
+{
+    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    const long ncells = (long)A.ni * A.nj * A.nk;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool bad = false;
+    int fe = 0;
+    if (c < ncells) {
+        const long qs = A.qs;
+        double rho = U[c];
+        if (!(rho > 0.0)) {
+            bad = true;
+        } else {
+            TPState<NSP> Q;
+            Q.rho = rho;
+            double dinv = dv<MD>(1.0, rho);
+            double v[3] = {U[qs + c] * dinv, U[2 * qs + c] * dinv, U[3 * qs + c] * dinv};
+            double u = U[4 * qs + c] * dinv;
+            u -= 0.5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            Q.u = u;
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) Q.Y[s] = U[(5 + s) * qs + c] * dinv;
+            Q.T = Tguess[c];
+            if (scale_mass_fractions<M, NSP>(Q.Y) != 0 || tp_update_thermo_from_rhou<M, NSP>(*A.gas, Q, &fe) != 0) {
+                bad = true;
+            } else {
+                Q.a = tp_sound_speed<M, NSP>(*A.gas, Q);
+                double* P = A.Pout;
+                P[P_RHO * qs + c] = Q.rho; P[P_VX * qs + c] = v[0]; P[P_VY * qs + c] = v[1]; P[P_VZ * qs + c] = v[2];
+                P[P_U * qs + c] = Q.u; P[P_T * qs + c] = Q.T; P[P_P * qs + c] = Q.p; P[P_A * qs + c] = Q.a;
+#pragma unroll
+                for (int s = 0; s < NSP; ++s) P[(P_Y + s) * qs + c] = Q.Y[s];
+            }
+        }
+    }
+    unsigned int nb = (unsigned int)__popcll(__ballot(bad));
+    if (nb && __lane_id() == 0) atomicAdd(&A.counters[0], nb);
+}
+
+} // namespace e4

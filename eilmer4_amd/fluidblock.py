@@ -42,9 +42,10 @@ class SFluidBlock:
                                                   _p(fsv) if fsv is not None else None))
         self.bc[Face(face)] = BCType(kind)
 
-    def upload(self, U):
+    def upload(self, U, Tguess=None):
         U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.nq, self.ncells)
-        _lib.check(_lib.load().e4hip_block_upload(self.id, _p(U), None))
+        Tg = None if Tguess is None else np.ascontiguousarray(Tguess, dtype=np.float64).ravel()
+        _lib.check(_lib.load().e4hip_block_upload(self.id, _p(U), _p(Tg) if Tg is not None else None))
 
     def set_from_pT(self, p, T, vx, vy, vz):
         """Initial state from pressure, temperature and velocity per cell
@@ -58,7 +59,8 @@ class SFluidBlock:
 
     def download(self, with_fs=False):
         U = np.empty((self.nq, self.ncells))
-        fs = np.empty((8, self.ncells)) if with_fs else None
+        nsp = getattr(self.gas, "n_species", 1)
+        fs = np.empty((8 + (nsp if nsp > 1 else 0), self.ncells)) if with_fs else None
         _lib.check(_lib.load().e4hip_block_download(self.id, _p(U), _p(fs) if fs is not None else None))
         return (U, fs) if with_fs else U
 

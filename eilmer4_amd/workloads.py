@@ -21,6 +21,7 @@ from .fvcore import BCType, Face
 
 SEED_BUMP_2D = 20261201
 SEED_BOX_3D = 20261202
+SEED_TP_BOX_3D = 20261204
 
 
 def planar_bump_2d(gas, ni=128, nj=64, Lx=1.0, Ly=0.5, seed=SEED_BUMP_2D):
@@ -106,3 +107,43 @@ def initial_U(gas, fields):
     p, T = fields["p"].ravel(), fields["T"].ravel()
     rho, u, _ = gas.rho_u_a_from_pT(p, T)
     return gas.encode(rho, u, fields["vx"], fields["vy"], fields["vz"])
+
+
+def _smooth_unit(n, rng, kmax=8):
+    return 0.5 * (_smooth_field(n, rng, kmax) + 1.0)          # [0, 1]
+
+
+def thermally_perfect_box_3d(gas, n=128, seed=SEED_TP_BOX_3D):
+    """3D n^3 unit cube of 5-species air (N2, O2, NO, N, O): temperature
+    300-8000 K and density 0.01-1.0 kg/m^3 from smooth random modes, velocity
+    +-300 m/s; mass fractions drawn from a Dirichlet distribution of
+    concentration 200 about a composition that moves from N2 0.767 / O2 0.233
+    toward dissociated air as (T - 300)/7700 rises.  Periodic in every
+    direction.  Returns geometry, primitive fields (incl. Y [nsp][...]), BCs."""
+    rng = np.random.default_rng(seed)
+    T = 300.0 + 7700.0 * _smooth_unit(n, rng)
+    rho = 0.01 + 0.99 * _smooth_unit(n, rng)
+    vx, vy, vz = (300.0 * _smooth_field(n, rng) for _ in range(3))
+    cold = np.array([0.767, 0.233, 0.0, 0.0, 0.0])
+    hot = np.array([0.60, 0.0, 0.02, 0.148, 0.232])
+    w = (T - 300.0) / 7700.0
+    base = (1.0 - w)[None] * cold[:, None, None, None] + w[None] * hot[:, None, None, None]
+    alpha = 200.0 * (base + 1.0e-3)
+    g = rng.standard_gamma(alpha)
+    Y = g / g.sum(axis=0, keepdims=True)
+    Rm = gas.Rmix(Y)
+    p = rho * Rm * T
+    fields = dict(p=p, T=T, rho=rho, vx=vx, vy=vy, vz=vz, Y=Y)
+    bcs = {f: (BCType.periodic, None) for f in (Face.west, Face.east, Face.north, Face.south, Face.top, Face.bottom)}
+    geom = dict(dims=3, ni=n, nj=n, nk=n, dx=1.0 / n, dy=1.0 / n, dz=1.0 / n)
+    return geom, fields, bcs, {}
+
+
+def initial_U_tp(gas, fields):
+    """Conserved state [5+nsp][ncells] and temperature guess of a thermally
+    perfect field (FlowState from rho, T, Y; encode_conserved)."""
+    T = fields["T"].ravel()
+    Y = fields["Y"].reshape(gas.n_species, -1)
+    u = gas.energy(T, Y)
+    U = gas.encode(fields["rho"].ravel(), u, fields["vx"], fields["vy"], fields["vz"], Y)
+    return U, T.copy()

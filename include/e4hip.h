@@ -108,6 +108,14 @@ int e4hip_synchronize(void);
  * (gas/ideal_gas.d:31-72, R = 8.31451/mMass). */
 int e4hip_gas_ideal(double mMass, double gamma, int* gm);
 
+/* Thermally perfect mixture (gas/therm_perf_gas.d:41-110): nsp species of
+ * molecular masses M[nsp], each with nseg NASA-9 (CEA) coefficient sets
+ * coeffs[nsp][nseg][9], break points Tbreaks[nsp][nseg+1] and blend ranges
+ * Tblends[nsp][nseg-1] (gas/thermo/cea_thermo_curves.d:22-45).  The device path
+ * is built for nsp = 5 (N2, O2, NO, N, O air). */
+int e4hip_gas_tp(int nsp, const double* M, int nseg, const double* Tbreaks, const double* Tblends,
+                 const double* coeffs, int* gm);
+
 /* Blocks.  Uniform Cartesian cells of size dx*dy*dz (dz ignored for dims 2);
  * replaces SFluidBlock construction + compute_primary_cell_geometric_data
  * (eilmer/sfluidblock.d:48-126, 650-842) for axis-aligned grids.  Every face
@@ -124,8 +132,10 @@ int e4hip_block_destroy(int blk);
  * simcore.d:1599-1668 for a block connected to itself. */
 int e4hip_block_set_bc(int blk, int face, int type, const double* fs);
 /* Host -> device copy of U[0] ([nq][ncells]); the flow state is the decode of
- * U (fvcell.d:563-732).  Tguess is reserved for multi-species gases and may be
- * NULL. Returns E4HIP_ERR_DECODE if some cell does not decode to a valid state. */
+ * U (fvcell.d:563-732).  For a thermally perfect gas, Tguess[ncells] is the
+ * temperature the Newton solve starts from (the flow state's T on restart);
+ * for an ideal gas it may be NULL.  Returns E4HIP_ERR_DECODE if some cell
+ * does not decode to a valid state. */
 int e4hip_block_upload(int blk, const double* U, const double* Tguess);
 /* Device -> host copy of U[0] and (if fs != NULL) the decoded flow state. */
 int e4hip_block_download(int blk, double* U, double* fs);

@@ -1,0 +1,105 @@
+"""Thermally perfect 5-species air on the device vs the CPU oracle.
+
+The device and the oracle run the same algorithm (NASA-9 curves, bracketed
+Newton from the stored temperature, the u side effect, species reconstruction
+and the face EOS).  The logarithm in the enthalpy comes from different
+libraries (device math library vs glibc), so bitwise equality is not expected;
+the bar is 1e-10 relative on T per decode and on each conserved quantity.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+import eilmer4_amd as e4
+from eilmer4_amd import workloads
+from parity_helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1.0e-10
+
+
+def setup_pair(n=12, flux="ausmdv", precision="fp64_exact", bcs=None, seed=workloads.SEED_TP_BOX_3D):
+    gas = e4.ThermallyPerfectGas.from_json()
+    geom, fields, bcs0, _ = workloads.thermally_perfect_box_3d(gas, n=n, seed=seed)
+    bcs = bcs or bcs0
+    U0, Tg = workloads.initial_U_tp(gas, fields)
+    og = O.Gas.from_json("therm-perf-5-species-air.json")
+    ob = O.Block(og, 3, n, n, n, geom["dx"], geom["dy"], geom["dz"])
+    fc = e4.flux_calculator_from_name(flux)
+    ob.set_config(flux_calc=int(fc))
+    for face, (kind, fs) in bcs.items():
+        ob.set_bc(int(face), int(kind), fs)
+    ob.set_U(U0, Tg)
+    db = e4.SFluidBlock(gas, 3, n, n, n, geom["dx"], geom["dy"], geom["dz"])
+    for face, (kind, fs) in bcs.items():
+        db.set_bc(face, kind, fs)
+    sim = e4.Simulation([db], e4.GlobalConfig(flux_calculator=fc, precision_mode=e4.PrecisionMode[precision]))
+    db.upload(U0, Tg)
+    return ob, db, sim
+
+
+@pytest.mark.parametrize("precision", ["fp64_exact", "fp64"])
+def test_tp_decode_matches_oracle(gpu, precision):
+    print("case tp decode n=12 precision=%s" % precision, flush=True)
+    ob, db, sim = setup_pair(12, precision=precision)
+    U, fs = db.download(with_fs=True)
+    fo = ob.get_fs()
+    err = {name: float(np.max(np.abs(fs[q] - fo[q]) / np.maximum(np.abs(fo[q]), 1e-300)))
+           for q, name in enumerate(["rho", "p", "T", "u", "a"])}
+    print("  decode rel err", err, flush=True)
+    assert err["T"] <= TOL and err["p"] <= TOL and err["a"] <= TOL
+    assert err["u"] <= 1e-8          # u carries the Newton side effect (last iterate's energy)
+    db.destroy()
+
+
+@pytest.mark.parametrize("precision", ["fp64_exact", "fp64"])
+@pytest.mark.parametrize("flux", ["ausmdv", "hllc"])
+def test_tp_steps_match_oracle(gpu, flux, precision):
+    print("case tp n=12 flux=%s precision=%s steps=5" % (flux, precision), flush=True)
+    ob, db, sim = setup_pair(12, flux=flux, precision=precision)
+    dt, _, _ = ob.determine_dt(1.0e-3, 0.5, False)
+    sim.cfg.fixed_time_step = True
+    for s in range(5):
+        assert ob.step(dt)[0] == 0
+        sim.state.dt_global = dt
+        sim.gasdynamic_explicit_increment_with_fixed_grid()
+    U, fs = db.download(with_fs=True)
+    Uo = ob.get_U()
+    fo = ob.get_fs()
+    err = rel_err(U, Uo)
+    errT = float(np.max(np.abs(fs[2] - fo[2]) / fo[2]))
+    print("  U rel err per quantity", err, "T", errT, "bitwise", np.array_equal(U, Uo), flush=True)
+    assert err.max() <= TOL
+    assert errT <= TOL
+    db.destroy()
+
+
+def test_tp_dt_matches_oracle(gpu):
+    ob, db, sim = setup_pair(12)
+    dto, cflo, _ = ob.determine_dt(1.0e-3, 0.5, False)
+    dtd, cfld = sim.block_dt(1.0e-3, False)
+    print("dt oracle %.17g device %.17g" % (dto, dtd))
+    assert abs(dtd - dto) / dto <= TOL
+    db.destroy()
+
+
+def test_tp_all_bcs(gpu):
+    """Slip walls, extrapolate and an inflow of 5-species air on a ragged block."""
+    gas = e4.ThermallyPerfectGas.from_json()
+    Yin = np.array([0.767, 0.233, 0.0, 0.0, 0.0])
+    inflow = gas.flowstate(1.0e4, 2000.0, (1500.0, 0.0, 0.0), Yin)
+    bcs = {e4.Face.west: (e4.BCType.inflow, inflow), e4.Face.east: (e4.BCType.extrapolate, None),
+           e4.Face.north: (e4.BCType.slip_wall, None), e4.Face.south: (e4.BCType.slip_wall, None),
+           e4.Face.top: (e4.BCType.periodic, None), e4.Face.bottom: (e4.BCType.periodic, None)}
+    print("case tp bcs n=10", flush=True)
+    ob, db, sim = setup_pair(10, bcs=bcs, seed=99)
+    dt, _, _ = ob.determine_dt(1.0e-3, 0.5, False)
+    for s in range(3):
+        assert ob.step(dt)[0] == 0
+        sim.state.dt_global = dt
+        sim.gasdynamic_explicit_increment_with_fixed_grid()
+    err = rel_err(db.download(), ob.get_U())
+    print("  U rel err per quantity", err, flush=True)
+    assert err.max() <= TOL
+    db.destroy()
