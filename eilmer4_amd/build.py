@@ -6,6 +6,7 @@ Three translation units:
   stage_fast.hip   -ffp-contract=fast: the plain FP64 and mixed FP32/FP64
                    stage kernels
   e4hip_api.hip    -ffp-contract=off: C ABI, host bookkeeping, CFL/decode kernels
+  mlp_train.hip    -ffp-contract=fast: training kernels of the temperature-guess network
 Packed FP32 instructions are disabled in all of them (-packed-fp32-ops).
 """
 import os
@@ -23,7 +24,8 @@ COMMON = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Xclang", "-ta
           "-packed-fp32-ops", "-Wno-unused-result", "-Wno-unused-value"]
 UNITS = [("stage_exact.hip", ["-ffp-contract=off"]),
          ("stage_fast.hip", ["-ffp-contract=fast"]),
-         ("e4hip_api.hip", ["-ffp-contract=off"])]
+         ("e4hip_api.hip", ["-ffp-contract=off"]),
+         ("mlp_train.hip", ["-ffp-contract=fast"])]
 
 
 def _run(cmd, verbose):

@@ -33,6 +33,8 @@ struct GasModelH {
     int nsp = 1;
     e4::TPGas tp{};
     e4::TPGas* tp_dev = nullptr;     // device copy read by the kernels
+    e4::GuessModel* guess_dev = nullptr;
+    __bf16* guess_w = nullptr;       // W1 [64][32], W2 [64][64], W3 [64][64] in BF16
 };
 
 struct BlockH {
@@ -67,11 +69,12 @@ struct State {
     int counter_slots = 0;
     std::vector<GasModelH> gases;
     std::vector<BlockH> blocks;
-    E4hipStepConfig cfg{E4HIP_FLUX_AUSMDV, E4HIP_UPDATE_PC, 2, 1, 1, 1, E4HIP_THERMO_RHOU, E4HIP_PRECISION_FP64, 0};
+    E4hipStepConfig cfg{E4HIP_FLUX_AUSMDV, E4HIP_UPDATE_PC, 2, 1, 1, 1, E4HIP_THERMO_RHOU, E4HIP_PRECISION_FP64, 0, 0};
     bool profiling = false;
     std::vector<ProfEvent> prof_pending;
     double prof_ms[2] = {0, 0};
     long prof_n[2] = {0, 0};
+    long long decode_evals = 0, decode_fallbacks = 0;
     std::string err;
 };
 
@@ -265,6 +268,7 @@ int launch_tp_stage(BlockH& b, int stage, double dt, unsigned int* counters)
     A.extrema_clipping = cfg.extrema_clipping;
     A.counters = counters;
     A.gas = g_state.gases[b.gm].tp_dev;
+    A.guess = cfg.learned_eos_guess ? g_state.gases[b.gm].guess_dev : nullptr;
     for (int f = 0; f < 6; ++f) {
         A.bc_type[f] = b.bc_type[f];
         memcpy(A.bc_fs[f], b.bc_fs_tp[f], sizeof(A.bc_fs[f]));
@@ -455,6 +459,70 @@ int e4hip_gas_tp(int nsp, const double* M, int nseg, const double* Tbreaks, cons
     return E4HIP_OK;
 }
 
+int e4hip_gas_tp_set_guess_model(int gm, const float* params, const float* in_mean, const float* in_std,
+                                 float T_mean, float T_std)
+{
+    if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
+    if (gm < 0 || gm >= (int)g_state.gases.size() || g_state.gases[gm].type != GAS_TP)
+        return fail(E4HIP_ERR_HANDLE, "not a thermally perfect gas model: %d", gm);
+    if (!params || !in_mean || !in_std) return fail(E4HIP_ERR_ARG, "null model arrays");
+    GasModelH& g = g_state.gases[gm];
+    // flat FP32 layout of the training kernels: W1[64][8], b1, W2[64][64], b2, W3[64][64], b3, w4[64], b4
+    const float* W1 = params;
+    const float* b1 = W1 + 64 * 8;
+    const float* W2 = b1 + 64;
+    const float* b2 = W2 + 64 * 64;
+    const float* W3 = b2 + 64;
+    const float* b3 = W3 + 64 * 64;
+    const float* w4 = b3 + 64;
+    const float* b4 = w4 + 64;
+    std::vector<__bf16> w(64 * 32 + 2 * 64 * 64);
+    for (int n = 0; n < 64; ++n)
+        for (int k = 0; k < 32; ++k) w[n * 32 + k] = (__bf16)((k < 7) ? W1[n * 8 + k] : 0.0f);
+    for (int q = 0; q < 64 * 64; ++q) { w[64 * 32 + q] = (__bf16)W2[q]; w[64 * 32 + 64 * 64 + q] = (__bf16)W3[q]; }
+    if (!g.guess_w) HIP_TRY(hipMalloc(&g.guess_w, sizeof(__bf16) * w.size()));
+    HIP_TRY(hipMemcpy(g.guess_w, w.data(), sizeof(__bf16) * w.size(), hipMemcpyHostToDevice));
+    e4::GuessModel M;
+    memset(&M, 0, sizeof(M));
+    for (int f = 0; f < 7; ++f) { M.in_mean[f] = in_mean[f]; M.in_std[f] = in_std[f]; }
+    M.T_mean = T_mean; M.T_std = T_std;
+    for (int n = 0; n < 64; ++n) { M.b1[n] = b1[n]; M.b2[n] = b2[n]; M.b3[n] = b3[n]; M.w4[n] = w4[n]; }
+    M.b4 = b4[0];
+    M.W1 = g.guess_w;
+    M.W2 = g.guess_w + 64 * 32;
+    M.W3 = g.guess_w + 64 * 32 + 64 * 64;
+    if (!g.guess_dev) HIP_TRY(hipMalloc(&g.guess_dev, sizeof(e4::GuessModel)));
+    HIP_TRY(hipMemcpy(g.guess_dev, &M, sizeof(M), hipMemcpyHostToDevice));
+    return E4HIP_OK;
+}
+
+int e4hip_tp_guess(int gm, int n, const double* x, double* T)
+{
+    if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
+    if (gm < 0 || gm >= (int)g_state.gases.size() || !g_state.gases[gm].guess_dev)
+        return fail(E4HIP_ERR_HANDLE, "gas model %d has no guess model", gm);
+    if (n <= 0 || !x || !T) return fail(E4HIP_ERR_ARG, "bad arguments");
+    double *dx = nullptr, *dT = nullptr;
+    HIP_TRY(hipMalloc(&dx, sizeof(double) * 7 * n));
+    HIP_TRY(hipMalloc(&dT, sizeof(double) * n));
+    HIP_TRY(hipMemcpyAsync(dx, x, sizeof(double) * 7 * n, hipMemcpyHostToDevice, g_state.stream));
+    e4::launch_mlp_guess(g_state.gases[gm].guess_dev, dx, n, dT, g_state.stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(T, dT, sizeof(double) * n, hipMemcpyDeviceToHost, g_state.stream));
+    HIP_TRY(hipStreamSynchronize(g_state.stream));
+    HIP_TRY(hipFree(dx));
+    HIP_TRY(hipFree(dT));
+    return E4HIP_OK;
+}
+
+int e4hip_tp_decode_stats(long long* energy_evals, long long* fallbacks, int reset)
+{
+    if (energy_evals) *energy_evals = g_state.decode_evals;
+    if (fallbacks) *fallbacks = g_state.decode_fallbacks;
+    if (reset) { g_state.decode_evals = 0; g_state.decode_fallbacks = 0; }
+    return E4HIP_OK;
+}
+
 int e4hip_block_create(int gm, int dims, int ni, int nj, int nk, double dx, double dy, double dz, int* blk)
 {
     if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
@@ -559,13 +627,17 @@ int e4hip_block_upload(int blk, const double* U, const double* Tguess)
         hipLaunchKernelGGL(e4::reset_counters_kernel, dim3(1), dim3(64), 0, g_state.stream, g_state.counters, 4, nullptr);
         e4::TPArgs A = tp_args_for(*b);
         A.Pout = b->P[0];
+        A.guess = g_state.cfg.learned_eos_guess ? g_state.gases[b->gm].guess_dev : nullptr;
         if (g_state.cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT) e4::launch_tp_decode_exact(A, b->U[0], Tg, g_state.stream);
         else e4::launch_tp_decode_fast64(A, b->U[0], Tg, g_state.stream);
         HIP_TRY(hipGetLastError());
-        unsigned int nbad = 0;
-        HIP_TRY(hipMemcpyAsync(&nbad, g_state.counters, sizeof(unsigned int), hipMemcpyDeviceToHost, g_state.stream));
+        unsigned int cnt[4] = {0, 0, 0, 0};
+        HIP_TRY(hipMemcpyAsync(cnt, g_state.counters, sizeof(cnt), hipMemcpyDeviceToHost, g_state.stream));
         HIP_TRY(hipStreamSynchronize(g_state.stream));
         HIP_TRY(hipFree(Tg));
+        unsigned int nbad = cnt[0];
+        g_state.decode_evals += cnt[2];
+        g_state.decode_fallbacks += cnt[3];
         if (nbad) return fail(E4HIP_ERR_DECODE, "%u cells of block %d do not decode to a valid state", nbad, blk);
         return E4HIP_OK;
     }
@@ -660,6 +732,8 @@ int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_ce
     for (int n = 0; n < nblk; ++n)
         for (int s = 0; s < 3; ++s) {
             unsigned int inv = h[4 * (n * 3 + s)], dec = h[4 * (n * 3 + s) + 1];
+            g_state.decode_evals += h[4 * (n * 3 + s) + 2];
+            g_state.decode_fallbacks += h[4 * (n * 3 + s) + 3];
             stage_bad[s] += (int)(inv + dec);
             if ((int)inv > g_state.cfg.max_invalid_cells) too_many = true;
             if (dec) decode_fail = true;

@@ -3,6 +3,7 @@
 #pragma once
 #include "gasdyn_device.h"
 #include "tp_gas_device.h"
+#include "learned_eos.h"
 
 namespace e4 {
 
@@ -42,7 +43,8 @@ struct TPArgs {
     double* __restrict__ dout;
     double* __restrict__ Unew;
     double* F[3];                     // face fluxes [5+nsp][nfaces(dir)]
-    unsigned int* counters;           // [0] invalid, [1] decode failures, [2] Newton f-evals (cells)
+    unsigned int* counters;           // [0] invalid, [1] decode failures, [2] decode energy evaluations,
+                                      // [3] cells whose learned guess fell back to the reference solve
     long qs;
     long fqs[3];
     int ni, nj, nk, dims;
@@ -53,6 +55,7 @@ struct TPArgs {
     double area[3];
     L2R2W w[3];
     const TPGas* gas;                 // device copy of the gas model (read-only)
+    const GuessModel* guess;          // learned temperature guess of the decode, or null
     int bc_type[6];
     double bc_fs[6][8 + TP_NSP_MAX];  // ghost flow state of inflow faces: rho,p,T,u,a,vx,vy,vz,Y
 };
@@ -68,6 +71,8 @@ void launch_tp_stage_exact(const TPArgs& A, hipStream_t stream, int flux);
 void launch_tp_stage_fast64(const TPArgs& A, hipStream_t stream, int flux);
 void launch_tp_decode_exact(const TPArgs& A, const double* U, const double* Tguess, hipStream_t stream);
 void launch_tp_decode_fast64(const TPArgs& A, const double* U, const double* Tguess, hipStream_t stream);
+
+void launch_mlp_guess(const GuessModel* G, const double* x, int n, double* T, hipStream_t stream);
 
 constexpr int TILE_X = 32;
 constexpr int TILE_Y = 8;

@@ -262,18 +262,23 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
 {
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
     constexpr int NQ = 5 + NSP;
+    __shared__ __bf16 gm_lds[2 * GM_LDS_PER_WAVE];
     const long ncells = (long)A.ni * A.nj * A.nk;
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    bool invalid = false, decode_fail = false;
+    const bool active = c < ncells;
+    const long qs = A.qs;
+    bool invalid = false, decode_fail = false, fell_back = false;
     int fe = 0;
-    if (c < ncells) {
+    double Un[NQ];
+    TPState<NSP> Q;
+    double v[3] = {0.0, 0.0, 0.0};
+    double rho = 1.0;
+    bool have_state = false;
+    if (active) {
         const int i = (int)(c % A.ni), j = (int)((c / A.ni) % A.nj), k = (int)(c / ((long)A.ni * A.nj));
         const long iW = ((long)k * A.nj + j) * (A.ni + 1) + i;
         const long jS = ((long)k * (A.nj + 1) + j) * A.ni + i;
         const long kB = c;
-        const double* Fi = A.F[0];
-        const double* Fj = A.F[1];
-        const double* Fk = A.F[2];
         double integ[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) integ[q] = 0.0;
@@ -285,16 +290,14 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
                 integ[q] -= Fsrc[q * fq + fidx] * area;
             }
         };
-        add_face(Fj, A.fqs[1], jS + A.ni, 1.0, A.area[1]);
-        add_face(Fi, A.fqs[0], iW + 1, 1.0, A.area[0]);
-        add_face(Fj, A.fqs[1], jS, -1.0, A.area[1]);
-        add_face(Fi, A.fqs[0], iW, -1.0, A.area[0]);
+        add_face(A.F[1], A.fqs[1], jS + A.ni, 1.0, A.area[1]);
+        add_face(A.F[0], A.fqs[0], iW + 1, 1.0, A.area[0]);
+        add_face(A.F[1], A.fqs[1], jS, -1.0, A.area[1]);
+        add_face(A.F[0], A.fqs[0], iW, -1.0, A.area[0]);
         if (A.dims == 3) {
-            add_face(Fk, A.fqs[2], kB + (long)A.ni * A.nj, 1.0, A.area[2]);
-            add_face(Fk, A.fqs[2], kB, -1.0, A.area[2]);
+            add_face(A.F[2], A.fqs[2], kB + (long)A.ni * A.nj, 1.0, A.area[2]);
+            add_face(A.F[2], A.fqs[2], kB, -1.0, A.area[2]);
         }
-        const long qs = A.qs;
-        double Un[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             double d = A.vol_inv * integ[q] + 0.0;
@@ -306,10 +309,8 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
             else if (A.stage == 2) Un[q] = uold + A.dt * (A.g1 * dd0 + A.g2 * d);
             else Un[q] = uold + A.dt * (A.g1 * dd0 + A.g2 * dd1 + A.g3 * d);
         }
-        // decode_conserved (fvcell.d:563-732)
-        TPState<NSP> Q;
-        double v[3] = {0.0, 0.0, 0.0};
-        double rho = Un[0];
+        // decode_conserved (fvcell.d:563-732): density, velocity, u, mass fractions
+        rho = Un[0];
         if (!(rho > 0.0)) {
             decode_fail = true;
         } else {
@@ -322,48 +323,78 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
             Q.u = u;
 #pragma unroll
             for (int s = 0; s < NSP; ++s) Q.Y[s] = Un[5 + s] * dinv;
-            if (scale_mass_fractions<M, NSP>(Q.Y) != 0) {
-                decode_fail = true;
-            } else {
-                Q.T = A.Pin[P_T * qs + c];     // the previous temperature is the Newton guess
-                if (tp_update_thermo_from_rhou<M, NSP>(*A.gas, Q, &fe) != 0) {
-                    // ignore_low_T_thermo_update_failure: T = 200 K and re-encode U
-                    Q.T = 200.0;
-                    tp_update_thermo_from_rhoT<M, NSP>(*A.gas, Q);
-                    Un[1] = rho * v[0]; Un[2] = rho * v[1]; Un[3] = rho * v[2];
-                    double ke2 = 0.5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-                    Un[4] = rho * (Q.u + ke2);
-#pragma unroll
-                    for (int s = 0; s < NSP; ++s) Un[5 + s] = rho * Q.Y[s];
-                }
-                Q.a = tp_sound_speed<M, NSP>(*A.gas, Q);
-                // check_data (flowstate.d:345-388)
-                double fsum = 0.0;
-#pragma unroll
-                for (int s = 0; s < NSP; ++s) fsum += Q.Y[s];
-                if (!isfinite(Q.rho) || Q.rho < 0.0) invalid = true;
-                if (!isfinite(Q.T) || Q.T < 1.01) invalid = true;
-                if (!isfinite(Q.p) || !isfinite(Q.a)) invalid = true;
-                if (fsum < 0.99 || fsum > 1.01 || !isfinite(fsum)) invalid = true;
-                if (fabs(v[0]) > 30000.0 || fabs(v[1]) > 30000.0 || fabs(v[2]) > 30000.0) invalid = true;
-                if (Q.T > 50000.0) invalid = true;
-                double* P = A.Pout;
-                P[P_RHO * qs + c] = Q.rho; P[P_VX * qs + c] = v[0]; P[P_VY * qs + c] = v[1]; P[P_VZ * qs + c] = v[2];
-                P[P_U * qs + c] = Q.u; P[P_T * qs + c] = Q.T; P[P_P * qs + c] = Q.p; P[P_A * qs + c] = Q.a;
-#pragma unroll
-                for (int s = 0; s < NSP; ++s) P[(P_Y + s) * qs + c] = Q.Y[s];
-            }
+            if (scale_mass_fractions<M, NSP>(Q.Y) != 0) decode_fail = true;
+            else have_state = true;
         }
+    }
+    // learned initial guess for the whole wave (all lanes take part)
+    double Tguess = 0.0;
+    if (A.guess) {
+        double x[7] = {have_state ? Q.rho : 1.0, have_state ? Q.u : 1.0e6, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < 5 && s < NSP; ++s) x[2 + s] = have_state ? Q.Y[s] : 0.2;
+        Tguess = (double)mlp_guess_wave(*A.guess, x, gm_lds + (threadIdx.x / 64) * GM_LDS_PER_WAVE);
+    }
+    if (have_state) {
+        const double e_tgt = Q.u;
+        int rc;
+        if (A.guess) {
+            rc = tp_polish<M, NSP>(*A.gas, Q, e_tgt, Tguess, &fe);
+            if (rc == 0) {
+                double Rmix = tp_Rmix<NSP>(*A.gas, Q.Y);
+                Q.p = Q.rho * Rmix * Q.T;
+            } else {
+                fell_back = true;
+                Q.u = e_tgt;
+                Q.T = A.Pin[P_T * qs + c];
+                rc = tp_update_thermo_from_rhou<M, NSP>(*A.gas, Q, &fe);
+            }
+        } else {
+            Q.T = A.Pin[P_T * qs + c];     // the previous temperature is the Newton guess
+            rc = tp_update_thermo_from_rhou<M, NSP>(*A.gas, Q, &fe);
+        }
+        if (rc != 0) {
+            // ignore_low_T_thermo_update_failure: T = 200 K and re-encode U
+            Q.T = 200.0;
+            tp_update_thermo_from_rhoT<M, NSP>(*A.gas, Q);
+            Un[1] = rho * v[0]; Un[2] = rho * v[1]; Un[3] = rho * v[2];
+            double ke2 = 0.5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            Un[4] = rho * (Q.u + ke2);
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) Un[5 + s] = rho * Q.Y[s];
+        }
+        Q.a = tp_sound_speed<M, NSP>(*A.gas, Q);
+        // check_data (flowstate.d:345-388)
+        double fsum = 0.0;
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) fsum += Q.Y[s];
+        if (!isfinite(Q.rho) || Q.rho < 0.0) invalid = true;
+        if (!isfinite(Q.T) || Q.T < 1.01) invalid = true;
+        if (!isfinite(Q.p) || !isfinite(Q.a)) invalid = true;
+        if (fsum < 0.99 || fsum > 1.01 || !isfinite(fsum)) invalid = true;
+        if (fabs(v[0]) > 30000.0 || fabs(v[1]) > 30000.0 || fabs(v[2]) > 30000.0) invalid = true;
+        if (Q.T > 50000.0) invalid = true;
+        double* P = A.Pout;
+        P[P_RHO * qs + c] = Q.rho; P[P_VX * qs + c] = v[0]; P[P_VY * qs + c] = v[1]; P[P_VZ * qs + c] = v[2];
+        P[P_U * qs + c] = Q.u; P[P_T * qs + c] = Q.T; P[P_P * qs + c] = Q.p; P[P_A * qs + c] = Q.a;
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) P[(P_Y + s) * qs + c] = Q.Y[s];
+    }
+    if (active) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) A.Unew[c + q * qs] = Un[q];
     }
     unsigned int nbad = (unsigned int)__popcll(__ballot(invalid));
     unsigned int nfail = (unsigned int)__popcll(__ballot(decode_fail));
+    unsigned int nfb = (unsigned int)__popcll(__ballot(fell_back));
+    int fsum_w = fe;
+    for (int off = 32; off > 0; off >>= 1) fsum_w += __shfl_xor(fsum_w, off);
     if (__lane_id() == 0) {
         if (nbad) atomicAdd(&A.counters[0], nbad);
         if (nfail) atomicAdd(&A.counters[1], nfail);
+        if (fsum_w) atomicAdd(&A.counters[2], (unsigned int)fsum_w);
+        if (nfb) atomicAdd(&A.counters[3], nfb);
     }
-    (void)fe;
 }
 
 // Decode of an uploaded U with a temperature guess into the primitive array
@@ -373,40 +404,71 @@ __global__ __launch_bounds__(128) void tp_decode_kernel(TPArgs A, const double* 
                                                         const double* __restrict__ Tguess)
 {
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    __shared__ __bf16 gm_lds[2 * GM_LDS_PER_WAVE];
     const long ncells = (long)A.ni * A.nj * A.nk;
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    bool bad = false;
+    const long qs = A.qs;
+    bool bad = false, have_state = false, fell_back = false;
     int fe = 0;
+    TPState<NSP> Q;
+    double v[3] = {0.0, 0.0, 0.0};
     if (c < ncells) {
-        const long qs = A.qs;
         double rho = U[c];
         if (!(rho > 0.0)) {
             bad = true;
         } else {
-            TPState<NSP> Q;
             Q.rho = rho;
             double dinv = dv<MD>(1.0, rho);
-            double v[3] = {U[qs + c] * dinv, U[2 * qs + c] * dinv, U[3 * qs + c] * dinv};
+            v[0] = U[qs + c] * dinv; v[1] = U[2 * qs + c] * dinv; v[2] = U[3 * qs + c] * dinv;
             double u = U[4 * qs + c] * dinv;
             u -= 0.5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
             Q.u = u;
 #pragma unroll
             for (int s = 0; s < NSP; ++s) Q.Y[s] = U[(5 + s) * qs + c] * dinv;
-            Q.T = Tguess[c];
-            if (scale_mass_fractions<M, NSP>(Q.Y) != 0 || tp_update_thermo_from_rhou<M, NSP>(*A.gas, Q, &fe) != 0) {
-                bad = true;
-            } else {
-                Q.a = tp_sound_speed<M, NSP>(*A.gas, Q);
-                double* P = A.Pout;
-                P[P_RHO * qs + c] = Q.rho; P[P_VX * qs + c] = v[0]; P[P_VY * qs + c] = v[1]; P[P_VZ * qs + c] = v[2];
-                P[P_U * qs + c] = Q.u; P[P_T * qs + c] = Q.T; P[P_P * qs + c] = Q.p; P[P_A * qs + c] = Q.a;
+            if (scale_mass_fractions<M, NSP>(Q.Y) != 0) bad = true;
+            else have_state = true;
+        }
+    }
+    double Tg = 0.0;
+    if (A.guess) {
+        double x[7] = {have_state ? Q.rho : 1.0, have_state ? Q.u : 1.0e6, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int s = 0; s < NSP; ++s) P[(P_Y + s) * qs + c] = Q.Y[s];
-            }
+        for (int s = 0; s < 5 && s < NSP; ++s) x[2 + s] = have_state ? Q.Y[s] : 0.2;
+        Tg = (double)mlp_guess_wave(*A.guess, x, gm_lds + (threadIdx.x / 64) * GM_LDS_PER_WAVE);
+    }
+    if (have_state) {
+        const double e_tgt = Q.u;
+        int rc = -1;
+        if (A.guess) {
+            rc = tp_polish<M, NSP>(*A.gas, Q, e_tgt, Tg, &fe);
+            if (rc == 0) Q.p = Q.rho * tp_Rmix<NSP>(*A.gas, Q.Y) * Q.T;
+            else fell_back = true;
+        }
+        if (rc != 0) {
+            Q.u = e_tgt;
+            Q.T = Tguess[c];
+            rc = tp_update_thermo_from_rhou<M, NSP>(*A.gas, Q, &fe);
+        }
+        if (rc != 0) {
+            bad = true;
+        } else {
+            Q.a = tp_sound_speed<M, NSP>(*A.gas, Q);
+            double* P = A.Pout;
+            P[P_RHO * qs + c] = Q.rho; P[P_VX * qs + c] = v[0]; P[P_VY * qs + c] = v[1]; P[P_VZ * qs + c] = v[2];
+            P[P_U * qs + c] = Q.u; P[P_T * qs + c] = Q.T; P[P_P * qs + c] = Q.p; P[P_A * qs + c] = Q.a;
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) P[(P_Y + s) * qs + c] = Q.Y[s];
         }
     }
     unsigned int nb = (unsigned int)__popcll(__ballot(bad));
-    if (nb && __lane_id() == 0) atomicAdd(&A.counters[0], nb);
+    unsigned int nfb = (unsigned int)__popcll(__ballot(fell_back));
+    int fsum_w = fe;
+    for (int off = 32; off > 0; off >>= 1) fsum_w += __shfl_xor(fsum_w, off);
+    if (__lane_id() == 0) {
+        if (nb) atomicAdd(&A.counters[0], nb);
+        if (fsum_w) atomicAdd(&A.counters[2], (unsigned int)fsum_w);
+        if (nfb) atomicAdd(&A.counters[3], nfb);
+    }
 }
 
 } // namespace e4

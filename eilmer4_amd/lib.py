@@ -20,6 +20,7 @@ EXPORTS = [
     "e4hip_gas_ideal", "e4hip_gas_tp", "e4hip_block_create", "e4hip_block_destroy", "e4hip_block_set_bc",
     "e4hip_block_upload", "e4hip_block_download", "e4hip_block_ncells", "e4hip_config",
     "e4hip_explicit_increment", "e4hip_determine_dt", "e4hip_profile_enable", "e4hip_profile_read",
+    "e4hip_gas_tp_set_guess_model", "e4hip_tp_guess", "e4hip_tp_decode_stats", "e4hip_mlp_grad", "e4hip_mlp_train",
 ]
 
 
@@ -35,7 +36,7 @@ class StepConfig(C.Structure):
                 ("interpolation_order", C.c_int), ("apply_limiter", C.c_int),
                 ("extrema_clipping", C.c_int), ("interpolate_in_local_frame", C.c_int),
                 ("thermo_interpolator", C.c_int), ("precision_mode", C.c_int),
-                ("max_invalid_cells", C.c_int)]
+                ("max_invalid_cells", C.c_int), ("learned_eos_guess", C.c_int)]
 
 
 _lib = None
@@ -68,6 +69,12 @@ def load(path=LIB_PATH):
     L.e4hip_determine_dt.argtypes = [_ip, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
     L.e4hip_profile_enable.argtypes = [C.c_int]
     L.e4hip_profile_read.argtypes = [C.c_int, _dp, C.POINTER(C.c_long)]
+    _fp = C.POINTER(C.c_float)
+    L.e4hip_gas_tp_set_guess_model.argtypes = [C.c_int, _fp, _fp, _fp, C.c_float, C.c_float]
+    L.e4hip_tp_guess.argtypes = [C.c_int, C.c_int, _dp, _dp]
+    L.e4hip_tp_decode_stats.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.c_int]
+    L.e4hip_mlp_grad.argtypes = [_fp, _fp, _fp, C.c_int, _fp, _fp]
+    L.e4hip_mlp_train.argtypes = [_fp, _fp, _fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _fp]
     _lib = L
     return L
 

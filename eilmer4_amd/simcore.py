@@ -32,6 +32,7 @@ class GlobalConfig:
     thermo_interpolator: int = InterpolateOption.rhou
     precision_mode: int = PrecisionMode.fp64
     max_invalid_cells: int = 0
+    learned_eos_guess: bool = False
     cfl_value: float = 0.5
     cfl_count: int = 10
     dt_init: float = 1.0e-3
@@ -47,7 +48,8 @@ class GlobalConfig:
         return _lib.StepConfig(int(self.flux_calculator), int(self.gasdynamic_update_scheme),
                                int(self.interpolation_order), int(bool(self.apply_limiter)),
                                int(bool(self.extrema_clipping)), int(bool(self.interpolate_in_local_frame)),
-                               int(self.thermo_interpolator), int(self.precision_mode), int(self.max_invalid_cells))
+                               int(self.thermo_interpolator), int(self.precision_mode), int(self.max_invalid_cells),
+                               int(bool(self.learned_eos_guess)))
 
 
 class SimState:

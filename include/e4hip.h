@@ -92,6 +92,10 @@ typedef struct E4hipStepConfig {
     int thermo_interpolator;         /* must be E4HIP_THERMO_RHOU (default) */
     int precision_mode;              /* E4HIP_PRECISION_* */
     int max_invalid_cells;           /* default 0 */
+    int learned_eos_guess;           /* thermally perfect gas: start the decode's temperature
+                                        solve from the learned guess (e4hip_gas_tp_set_guess_model)
+                                        with an FP64 Newton polish; 0 = the reference's solve
+                                        from the stored temperature */
 } E4hipStepConfig;
 
 /* Library set-up on one GPU; creates the library's HIP stream.
@@ -115,6 +119,33 @@ int e4hip_gas_ideal(double mMass, double gamma, int* gm);
  * is built for nsp = 5 (N2, O2, NO, N, O air). */
 int e4hip_gas_tp(int nsp, const double* M, int nseg, const double* Tbreaks, const double* Tblends,
                  const double* coeffs, int* gm);
+
+/* Learned initial guess of the thermally perfect temperature solve (no
+ * counterpart in the reference; it replaces the starting point of
+ * therm_perf_gas_mix_eos.d:70-153 in the decode).  A 7 -> 64 -> 64 -> 64 -> 1
+ * SiLU network on (rho, u, Y0..Y4), evaluated in BF16 MFMA with FP32
+ * accumulation, then two FP64 Newton steps judged by the energy residual at
+ * the result (accepted when |residual / (dz/dT)| <= 1e-11 T); a cell that fails
+ * the judge falls back to the reference's bracketed solve from its stored
+ * temperature.  params: E4HIP_MLP_NPARAM floats in the training layout
+ * (W1[64][8], b1[64], W2[64][64], b2, W3[64][64], b3, w4[64], b4); features are
+ * normalised as (x - in_mean) * in_std, and T = T_mean + T_std * output. */
+#define E4HIP_MLP_NPARAM 8961
+int e4hip_gas_tp_set_guess_model(int gm, const float* params, const float* in_mean, const float* in_std,
+                                 float T_mean, float T_std);
+/* Guesses for n states x[n][7] (rho, u, Y0..Y4) -> T[n] (host arrays). */
+int e4hip_tp_guess(int gm, int n, const double* x, double* T);
+/* Energy evaluations and learned-guess fallbacks of the decode temperature
+ * solves since the last reset (summed over blocks and stages). */
+int e4hip_tp_decode_stats(long long* energy_evals, long long* fallbacks, int reset);
+/* Training of the guess network with this library's own kernels (FP32).
+ * X[n][8] normalised features (8th = 0), T[n] normalised targets.
+ * e4hip_mlp_grad: mean squared error and its gradient for one batch.
+ * e4hip_mlp_train: Adam with cosine learning-rate decay, params updated in place,
+ * loss_per_epoch (may be NULL) receives [epochs] values. */
+int e4hip_mlp_grad(const float* params, const float* X, const float* T, int n, float* loss, float* grads);
+int e4hip_mlp_train(float* params, const float* X, const float* T, int n, int epochs, int batch, float lr,
+                    unsigned long long seed, float* loss_per_epoch);
 
 /* Blocks.  Uniform Cartesian cells of size dx*dy*dz (dz ignored for dims 2);
  * replaces SFluidBlock construction + compute_primary_cell_geometric_data
