@@ -74,6 +74,55 @@ def cpu_baseline(n_sample=128, min_seconds=10.0, max_steps=100):
                 sample="%d^3 cells of the shocked-box workload, %d RK2 steps (ausmdv), %.1f s" % (n_sample, steps, el))
 
 
+def tp_workload(e4, workloads, L, n=128, steps=5, warmup=1):
+    """The 5-species thermally perfect workload (n^3, AUSMDV, pc, plain FP64):
+    the reference decode (bracketed Newton from the stored temperature) and the
+    learned-guess decode, each timed over `steps` steps, and their difference."""
+    import torch
+    from eilmer4_amd import learned_eos
+    gas = e4.ThermallyPerfectGas.from_json()
+    model = learned_eos.GuessModel.load()
+    model.install(gas)
+    geom, fields, bcs, _ = workloads.thermally_perfect_box_3d(gas, n=n)
+    U0, Tg = workloads.initial_U_tp(gas, fields)
+    del fields
+    out, finals = {}, {}
+    dt_fixed = None
+    for learned in (False, True):
+        blk = e4.SFluidBlock(gas, 3, n, n, n, 1.0 / n, 1.0 / n, 1.0 / n)
+        for face, (kind, fs) in bcs.items():
+            blk.set_bc(face, kind, fs)
+        sim = e4.Simulation([blk], e4.GlobalConfig(learned_eos_guess=learned))
+        blk.upload(U0, Tg)
+        if dt_fixed is None:
+            dt_fixed = sim.block_dt(1e-3, False)[0]
+        sim.cfg.fixed_time_step = True
+        sim.state.dt_global = dt_fixed
+        sim.integrate_in_time(target_time=1e30, max_step=warmup)
+        torch.cuda.synchronize()
+        learned_eos.decode_stats(reset=True)
+        t0 = time.perf_counter()
+        sim.integrate_in_time(target_time=1e30, max_step=warmup + steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ev, fb = learned_eos.decode_stats(reset=True)
+        U, fs = blk.download(with_fs=True)
+        finals[learned] = (U, fs)
+        out["learned" if learned else "reference"] = {
+            "value": round(n ** 3 * steps / wall / 1e6, 3), "ms_per_step": round(wall * 1e3 / steps, 3),
+            "decode_energy_evals_per_cell_stage": round(ev / (n ** 3 * steps * 2), 3),
+            "decode_fallback_fraction": round(fb / (n ** 3 * steps * 2), 5)}
+        blk.destroy()
+    (Ur, fr), (Ul, fl) = finals[False], finals[True]
+    out["learned_vs_reference"] = {
+        "steps": warmup + steps,
+        "rho_linf_rel": float(np.max(np.abs(Ul[0] - Ur[0])) / np.max(np.abs(Ur[0]))),
+        "T_linf_rel": float(np.max(np.abs(fl[2] - fr[2]) / fr[2]))}
+    out["workload"] = "3D %d^3 5-species thermally perfect air (N2,O2,NO,N,O), T 300-8000 K, ausmdv, pc, FP64" % n
+    out["guess_model"] = "7-64-64-64-1 SiLU, BF16 MFMA inference, trained on 2.16 M device Newton states"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +134,7 @@ def main():
     ap.add_argument("--precision", default="fp64,mixed,fp64_exact",
                     help="comma list of precision modes; the first is the headline")
     ap.add_argument("--acc-steps", type=int, default=20, help="steps of the mixed-vs-fp64 accuracy check")
+    ap.add_argument("--tp-n", type=int, default=128, help="cells per side of the 5-species workload (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM-counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -215,6 +265,10 @@ def main():
                      "cfl_launch_ms": round(r["cfl_ms"], 4),
                      "alg_bytes_per_launch": alg_bytes_launch},
     }
+    if args.tp_n > 0:
+        log("thermally perfect %d^3 workload ..." % args.tp_n)
+        out["thermally_perfect"] = tp_workload(e4, workloads, L, n=args.tp_n)
+        log("thermally perfect: %s" % out["thermally_perfect"])
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline()
