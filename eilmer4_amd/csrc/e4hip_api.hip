@@ -49,6 +49,7 @@ struct BlockH {
     double* d[2] = {nullptr, nullptr};            // dUdt of stages 1 and 2
     int bc_type[6] = {0, 0, 0, 0, 0, 0};
     Prim bc_fs[6] = {};
+    e4::BCTable* bc_dev = nullptr;                // device copy of bc_type, bc_fs
     // thermally perfect gas only: primitive arrays (ping-pong) and face fluxes
     double* P[2] = {nullptr, nullptr};
     double* F[3] = {nullptr, nullptr, nullptr};
@@ -131,6 +132,19 @@ L2R2W l2r2_weights(double len)
     return w;
 }
 
+// The four distinct numbers of l2r2_weights(len) that the stage kernel's
+// per-cell reconstruction uses; the equalities hold because every length is len.
+bool l2r2_uniform(double len, e4::L2R2U& u)
+{
+    L2R2W w = l2r2_weights(len);
+    u.c = w.two_over_lenL0_plus_lenL1;
+    u.a = w.aL0;
+    u.len = w.lenR0;
+    u.len3 = w.two_lenL0_plus_lenL1;
+    return w.two_over_lenR0_plus_lenL0 == u.c && w.two_over_lenR1_plus_lenR0 == u.c && w.aR0 == u.a &&
+           w.lenL0 == u.len && w.two_lenR0_plus_lenR1 == u.len3;
+}
+
 int kchunk_for(const BlockH& b)
 {
     const char* env = getenv("E4HIP_KCHUNK");
@@ -185,7 +199,7 @@ int launch_stage(const BlockH& b, int stage, double dt, unsigned int* counters)
     A.extrema_clipping = cfg.extrema_clipping;
     A.counters = counters;
     A.gas = g_state.gases[b.gm].ideal;
-    for (int f = 0; f < 6; ++f) { A.bc_type[f] = b.bc_type[f]; A.bc_fs[f] = b.bc_fs[f]; }
+    A.bc = b.bc_dev;
     // geometry of the uniform Cartesian cell (the oracle uses the same expressions)
     if (b.dims == 3) {
         A.vol_inv = 1.0 / (b.dx * b.dy * b.dz);
@@ -194,9 +208,8 @@ int launch_stage(const BlockH& b, int stage, double dt, unsigned int* counters)
         A.vol_inv = 1.0 / (b.dx * b.dy);
         A.area[0] = b.dy; A.area[1] = b.dx; A.area[2] = 0.0;
     }
-    A.w[0] = l2r2_weights(b.dx);
-    A.w[1] = l2r2_weights(b.dy);
-    A.w[2] = l2r2_weights(b.dz);
+    if (!l2r2_uniform(b.dx, A.wu[0]) || !l2r2_uniform(b.dy, A.wu[1]) || !l2r2_uniform(b.dz, A.wu[2]))
+        return fail(E4HIP_ERR_ARG, "l2r2 weights of a uniform direction are not uniform");
     // stage coefficients (fvcell.d:1168-1180, 1254-1266, 1333-1342) and buffers
     double g1 = 1.0, g2 = 0.0, g3 = 0.0;
     if (stage == 1) {
@@ -342,6 +355,7 @@ int alloc_block_buffers(BlockH& b)
             if (!b.F[dd]) HIP_TRY(hipMalloc(&b.F[dd], sizeof(double) * b.fqs[dd] * b.nq));
         }
     }
+    if (!b.bc_dev) HIP_TRY(hipMalloc(&b.bc_dev, sizeof(e4::BCTable)));
     if (!b.U[0]) HIP_TRY(hipMalloc(&b.U[0], bytes));
     if (!b.U[1]) HIP_TRY(hipMalloc(&b.U[1], bytes));
     if (ns >= 2 && !b.d[0]) HIP_TRY(hipMalloc(&b.d[0], bytes));
@@ -358,6 +372,19 @@ int free_block_buffers(BlockH& b)
     for (auto& p : b.d) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     for (auto& p : b.P) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     for (auto& p : b.F) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
+    if (b.bc_dev) { HIP_TRY(hipFree(b.bc_dev)); b.bc_dev = nullptr; }
+    return E4HIP_OK;
+}
+
+// Copy the block's ghost-cell effects to its device table, after the work
+// already queued on the library stream (which may read the old table).
+int upload_bc(const BlockH& b)
+{
+    e4::BCTable t;
+    memset(&t, 0, sizeof(t));
+    for (int f = 0; f < 6; ++f) { t.type[f] = b.bc_type[f]; t.fs[f] = b.bc_fs[f]; }
+    HIP_TRY(hipStreamSynchronize(g_state.stream));
+    HIP_TRY(hipMemcpy(b.bc_dev, &t, sizeof(t), hipMemcpyHostToDevice));
     return E4HIP_OK;
 }
 
@@ -541,6 +568,7 @@ int e4hip_block_create(int gm, int dims, int ni, int nj, int nk, double dx, doub
     b.nsp = g_state.gases[gm].nsp;
     b.nq = (b.nsp > 1) ? 5 + b.nsp : 5;
     int rc = alloc_block_buffers(b);
+    if (!rc) rc = upload_bc(b);
     if (rc) { free_block_buffers(b); return rc; }
     g_state.blocks.push_back(b);
     *blk = (int)g_state.blocks.size() - 1;
@@ -576,7 +604,7 @@ int e4hip_block_set_bc(int blk, int face, int type, const double* fs)
         b->bc_fs[face] = P;
     }
     b->bc_type[face] = type;
-    return E4HIP_OK;
+    return upload_bc(*b);
 }
 
 int e4hip_block_ncells(int blk, long* ncells, int* nq)

@@ -97,6 +97,21 @@ __device__ __forceinline__ L2R2WT<R> cast_w(const L2R2W& w)
                      R(w.two_lenR0_plus_lenR1)};
 }
 
+// The weights of l2r2_prepare on a uniform grid: with every cell length equal
+// to len, the three 2/(len+len) are one number c, aL0 == aR0 == a,
+// lenL0 == lenR0 == len and 2 lenL0 + lenL1 == 2 lenR0 + lenR1 == len3.
+template <class R>
+struct L2R2UT {
+    R c, a, len, len3;
+};
+using L2R2U = L2R2UT<double>;
+
+template <class R>
+__device__ __forceinline__ L2R2UT<R> cast_wu(const L2R2U& w)
+{
+    return L2R2UT<R>{R(w.c), R(w.a), R(w.len), R(w.len3)};
+}
+
 template <class R>
 __device__ __forceinline__ IdealGasT<R> cast_gas(const IdealGas& g)
 {
@@ -111,6 +126,16 @@ __device__ __forceinline__ R clip_to_limits(R q, R A, R B)
     R upper_limit = (A > B) ? A : B;
     R qclipped = (q > lower_limit) ? q : lower_limit;
     return (qclipped <= upper_limit) ? qclipped : upper_limit;
+}
+
+// clip_to_limits in the mode's form: the reference's comparisons for
+// ModeExact; max/min for the fast modes (the same value for ordered,
+// non-NaN operands up to the sign of a zero).
+template <class M>
+__device__ __forceinline__ typename M::R clip_limits(typename M::R q, typename M::R A, typename M::R B)
+{
+    if constexpr (!M::kFast) return clip_to_limits(q, A, B);
+    else return mn(mx(q, mn(A, B)), mx(A, B));
 }
 
 // onedinterp.d:136-162
@@ -168,39 +193,6 @@ template <class R>
 struct FaceStateT {
     R rho, p, u, a, vn, vt1, vt2;
 };
-
-// interp_l2r2 (onedinterp.d:435-585) for an ideal gas with thermo_interpolator
-// rhou, the EOS mixins of onedinterp.d:1039-1068 (fall back to the cell state
-// when update_thermo_from_rhou rejects the face values) and the copy of the
-// cell-centre sound speed (sfluidblock.d:2199-2208).  Inputs are in the face
-// frame.  With order 1 the face states are the adjacent cells'.
-template <class M>
-__device__ __forceinline__ void reconstruct_ideal(const PrimT<typename M::R>& L1, const PrimT<typename M::R>& L0,
-                                                  const PrimT<typename M::R>& R0, const PrimT<typename M::R>& R1,
-                                                  const L2R2WT<typename M::R>& w, const IdealGasT<typename M::R>& g,
-                                                  bool high_order, bool apply_limiter, bool extrema_clipping,
-                                                  FaceStateT<typename M::R>& Lf, FaceStateT<typename M::R>& Rf)
-{
-    using R = typename M::R;
-    Lf.a = L0.a;
-    Rf.a = R0.a;
-    bool okL = false, okR = false;
-    if (high_order) {
-        interp_l2r2_scalar<M>(w, apply_limiter, extrema_clipping, L1.v0, L0.v0, R0.v0, R1.v0, Lf.vn, Rf.vn);
-        interp_l2r2_scalar<M>(w, apply_limiter, extrema_clipping, L1.v1, L0.v1, R0.v1, R1.v1, Lf.vt1, Rf.vt1);
-        interp_l2r2_scalar<M>(w, apply_limiter, extrema_clipping, L1.v2, L0.v2, R0.v2, R1.v2, Lf.vt2, Rf.vt2);
-        interp_l2r2_scalar<M>(w, apply_limiter, extrema_clipping, L1.rho, L0.rho, R0.rho, R1.rho, Lf.rho, Rf.rho);
-        interp_l2r2_scalar<M>(w, apply_limiter, extrema_clipping, L1.u, L0.u, R0.u, R1.u, Lf.u, Rf.u);
-        okL = !(Lf.u <= R(0) || Lf.rho <= R(0));
-        okR = !(Rf.u <= R(0) || Rf.rho <= R(0));
-    }
-    if (!okL) { Lf.rho = L0.rho; Lf.u = L0.u; Lf.vn = L0.v0; Lf.vt1 = L0.v1; Lf.vt2 = L0.v2; }
-    if (!okR) { Rf.rho = R0.rho; Rf.u = R0.u; Rf.vn = R0.v0; Rf.vt1 = R0.v1; Rf.vt2 = R0.v2; }
-    R TL = Lf.u * g.Cvinv;
-    Lf.p = Lf.rho * g.Rgas * TL;
-    R TR = Rf.u * g.Cvinv;
-    Rf.p = Rf.rho * g.Rgas * TR;
-}
 
 // Flux in the face frame: mass, normal momentum, two tangential momenta, energy.
 template <class R>
@@ -347,18 +339,6 @@ __device__ __forceinline__ Flux5T<typename M::R> flux_calc(const FaceStateT<type
 {
     if constexpr (FLUX == FLUX_HLLC) return hllc<M>(L, R);
     else return ausmdv<M>(L, R);
-}
-
-// Reconstruct and evaluate the flux at one face; stencil states in the face frame.
-template <class M, int FLUX>
-__device__ __forceinline__ Flux5T<typename M::R> face_flux_ideal(
-    const PrimT<typename M::R>& L1, const PrimT<typename M::R>& L0, const PrimT<typename M::R>& R0,
-    const PrimT<typename M::R>& R1, const L2R2WT<typename M::R>& w, const IdealGasT<typename M::R>& g,
-    bool high_order, bool apply_limiter, bool extrema_clipping)
-{
-    FaceStateT<typename M::R> Lf, Rf;
-    reconstruct_ideal<M>(L1, L0, R0, R1, w, g, high_order, apply_limiter, extrema_clipping, Lf, Rf);
-    return flux_calc<M, FLUX>(Lf, Rf);
 }
 
 } // namespace e4

@@ -10,6 +10,14 @@ namespace e4 {
 enum { BC_SLIP_WALL = 0, BC_INFLOW = 1, BC_EXTRAPOLATE = 2, BC_PERIODIC = 3 };
 enum { FACE_NORTH = 0, FACE_EAST = 1, FACE_SOUTH = 2, FACE_WEST = 3, FACE_TOP = 4, FACE_BOTTOM = 5 };
 
+// Ghost-cell effects of the six faces, kept in device memory so that the
+// kernel reads them only where a stencil leaves the block.
+struct BCTable {
+    int type[6];
+    int pad[2];
+    Prim fs[6];                       // inflow ghost states (global frame)
+};
+
 struct StageArgs {
     const double* __restrict__ Us;    // state the fluxes are evaluated from (stencil)
     const double* __restrict__ Uold;  // base state of the update (cell-local)
@@ -26,10 +34,9 @@ struct StageArgs {
     double dt, g1, g2, g3;
     double vol_inv;
     double area[3];                   // i-, j-, k-face areas
-    L2R2W w[3];
+    L2R2U wu[3];                      // l2r2 weights per direction (uniform grid)
     IdealGas gas;
-    int bc_type[6];
-    Prim bc_fs[6];                    // inflow ghost states (global frame)
+    const BCTable* bc;
 };
 
 enum { P_RHO = 0, P_VX = 1, P_VY = 2, P_VZ = 3, P_U = 4, P_T = 5, P_P = 6, P_A = 7, P_Y = 8 };

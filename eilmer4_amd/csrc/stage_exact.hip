@@ -7,14 +7,7 @@ namespace e4 {
 
 void launch_stage_exact(const StageArgs& A, dim3 grid, hipStream_t stream, int dims, int flux)
 {
-    const dim3 block(TILE_X * TILE_Y);
-    if (dims == 3) {
-        if (flux == FLUX_HLLC) hipLaunchKernelGGL((stage_kernel<ModeExact, TILE_X, TILE_Y, 3, FLUX_HLLC>), grid, block, 0, stream, A);
-        else hipLaunchKernelGGL((stage_kernel<ModeExact, TILE_X, TILE_Y, 3, FLUX_AUSMDV>), grid, block, 0, stream, A);
-    } else {
-        if (flux == FLUX_HLLC) hipLaunchKernelGGL((stage_kernel<ModeExact, TILE_X, TILE_Y, 2, FLUX_HLLC>), grid, block, 0, stream, A);
-        else hipLaunchKernelGGL((stage_kernel<ModeExact, TILE_X, TILE_Y, 2, FLUX_AUSMDV>), grid, block, 0, stream, A);
-    }
+    launch_stage_mode<ModeExact>(A, grid, stream, dims, flux);
 }
 
 void launch_tp_stage_exact(const TPArgs& A, hipStream_t stream, int flux)

@@ -10,17 +10,34 @@
 // low-T fallback, and count_invalid_cells (fluidblock.d:433-499).
 //
 // Layout: conserved state is SoA [5][ncells] in HBM, i fastest.  A workgroup
-// owns a TX x TY column of cells and marches up a chunk of k-planes.  Each
-// plane's primitive state (rho, vel, u, a), with two halo cells in i and j, is
-// decoded once into LDS; i- and j-faces are reconstructed from LDS, each face
-// once per plane; the k-direction stencil of a column lives in a four-plane
-// register queue, and the k-face flux of the top of one plane is carried as
-// the bottom of the next.  The flux divergence, stage update, decode check and
-// the write of the new state all happen in registers.
+// owns a TX x TY column of cells and marches up a chunk of k-planes.  Per
+// plane:
+//   1. the plane's primitive state (rho, vel, u, a), with two halo cells in i
+//      and j, is decoded once into LDS;
+//   2. every cell reconstructs its two face states per direction (the left
+//      state of its plus face and the right state of its minus face) from one
+//      limiter evaluation per scalar: on the uniform grid the van Albada
+//      factor of both is the same expression of the same two differences, so
+//      the results are those of the per-face evaluation of
+//      onedinterp.d:136-162, bit for bit.  Right states go to LDS; the
+//      k-direction stencil lives in a three-plane register queue;
+//   3. every cell evaluates the flux of its plus faces (i, j from LDS, k from
+//      registers); fluxes of the tile's west and south faces are extra items;
+//   4. flux divergence, stage update, decode check and the write of the new
+//      state happen in registers.
 #pragma once
 #include "stage_args.h"
 
 namespace e4 {
+
+// The stride between conserved quantities, opaque to the optimiser: without
+// it the address bases A.X + q*qs of every array and quantity are hoisted out
+// of the k-loop into scalar registers, more than the register file holds.
+__device__ __forceinline__ long opaque_qs(long qs)
+{
+    asm volatile("" : "+s"(qs));
+    return qs;
+}
 
 // Decode of the cell (i,j,k); positions one or two cells outside the block in
 // one index direction are ghost cells and take the value the face's ghost-cell
@@ -28,8 +45,9 @@ namespace e4 {
 template <class M>
 __device__ __forceinline__ Prim load_decode(const StageArgs& A, long c)
 {
-    return decode_ideal<M>(A.Us[c], A.Us[c + A.qs], A.Us[c + 2 * A.qs], A.Us[c + 3 * A.qs], A.Us[c + 4 * A.qs],
-                           A.gas);
+    const long qs = opaque_qs(A.qs);
+    const double* p = A.Us + c;
+    return decode_ideal<M>(p[0], p[qs], p[2 * qs], p[3 * qs], p[4 * qs], A.gas);
 }
 
 __device__ __forceinline__ long cell_index(const StageArgs& A, int i, int j, int k)
@@ -57,8 +75,8 @@ __device__ Prim fetch_ghost(const StageArgs& A, int i, int j, int k)
     const int face = (dir == 0) ? (plus ? FACE_EAST : FACE_WEST)
                    : (dir == 1) ? (plus ? FACE_NORTH : FACE_SOUTH)
                                 : (plus ? FACE_TOP : FACE_BOTTOM);
-    const int type = A.bc_type[face];
-    if (type == BC_INFLOW) return A.bc_fs[face];
+    const int type = A.bc->type[face];
+    if (type == BC_INFLOW) return A.bc->fs[face];
     int src;
     if (type == BC_EXTRAPOLATE) src = plus ? n - 1 : 0;
     else if (type == BC_PERIODIC) src = plus ? layer - 1 : n - layer;
@@ -83,23 +101,6 @@ __device__ __forceinline__ PrimT<typename M::R> fetch(const StageArgs& A, int i,
     return PrimT<R>{R(P.rho), R(P.v0), R(P.v1), R(P.v2), R(P.u), R(P.a)};
 }
 
-// Face frames as permutations of (x,y,z): 3D i-face (x,y,z), j-face (y,z,x),
-// k-face (z,x,y); 2D i-face (x,y,z), j-face (y,x,z) (sfluidblock.d:527-588,
-// fvinterface.d:265-296).  The 2D i-face's t1 = -y is an exact sign flip that
-// every operation of the face path commutes with, so it is not applied.
-template <int DIMS>
-__device__ __forceinline__ void face_perm(int dir, int& pn, int& pt1, int& pt2)
-{
-    if (dir == 0) { pn = 0; pt1 = 1; pt2 = 2; }
-    else if (dir == 1) { if (DIMS == 3) { pn = 1; pt1 = 2; pt2 = 0; } else { pn = 1; pt1 = 0; pt2 = 2; } }
-    else { pn = 2; pt1 = 0; pt2 = 1; }
-}
-
-__device__ __forceinline__ unsigned int wave_count(bool pred)
-{
-    return (unsigned int)__popcll(__ballot(pred));
-}
-
 // The update formulas of fvcell.d:1153-1393 for one conserved quantity.
 __device__ __forceinline__ double stage_update(const StageArgs& A, double uold, double dd0, double dd1, double d)
 {
@@ -108,16 +109,152 @@ __device__ __forceinline__ double stage_update(const StageArgs& A, double uold, 
     return uold + A.dt * (A.g1 * dd0 + A.g2 * dd1 + A.g3 * d);
 }
 
-template <class M, int TX, int TY, int DIMS, int FLUX>
+__device__ __forceinline__ unsigned int wave_count(bool pred)
+{
+    return (unsigned int)__popcll(__ballot(pred));
+}
+
+// A reconstructed face state of a cell: density, velocity in the global frame,
+// internal energy and the cell-centre sound speed (sfluidblock.d:2199-2208).
+template <class R>
+struct SideT {
+    R rho, v0, v1, v2, u, a;
+};
+
+// onedinterp.d:136-162 for the two faces of one cell on a uniform grid.  For
+// the plus face (qL1, qL0, qR0) = (qm, q0, qp) and for the minus face
+// (qL0, qR0, qR1) = (qm, q0, qp); with equal cell lengths the three scalings
+// are the same number, so delLminus of the plus face and del of the minus face
+// are both dm, del of the plus face and delRplus of the minus face are both
+// dp, and sL of the plus face equals sR of the minus face.
+template <class M>
+__device__ __forceinline__ void interp_l2r2_cell(const L2R2UT<typename M::R>& w, bool apply_limiter,
+                                                 bool extrema_clipping, typename M::R qm, typename M::R q0,
+                                                 typename M::R qp, typename M::R& qLp, typename M::R& qRm)
+{
+    using R = typename M::R;
+    const R eps = R(1.0e-12);   // onedinterp.d:22
+    R dm = (q0 - qm) * w.c;
+    R dp = (qp - q0) * w.c;
+    R s = R(1.0);
+    if (apply_limiter) s = dv<M>(dm * dp + ab(dm * dp) + eps, dm * dm + dp * dp + eps);
+    qLp = q0 + s * w.a * (dp * w.len3 + dm * w.len);
+    qRm = q0 - s * w.a * (dp * w.len + dm * w.len3);
+    if (extrema_clipping) {
+        qLp = clip_limits<M>(qLp, q0, qp);
+        qRm = clip_limits<M>(qRm, qm, q0);
+    }
+}
+
+// Both face states of the cell Q0 in one direction (neighbours Qm, Qp), with
+// the EOS fall-back of onedinterp.d:1039-1068: a face state with u <= 0 or
+// rho <= 0 is replaced by the cell state.  With order 1 both are the cell state.
+template <class M>
+__device__ __forceinline__ void recon_cell(const L2R2UT<typename M::R>& w, bool high_order, bool apply_limiter,
+                                           bool extrema_clipping, const PrimT<typename M::R>& Qm,
+                                           const PrimT<typename M::R>& Q0, const PrimT<typename M::R>& Qp,
+                                           SideT<typename M::R>& Lp, SideT<typename M::R>& Rm)
+{
+    using R = typename M::R;
+    Lp = SideT<R>{Q0.rho, Q0.v0, Q0.v1, Q0.v2, Q0.u, Q0.a};
+    Rm = Lp;
+    if (high_order) {
+        SideT<R> l, r;
+        interp_l2r2_cell<M>(w, apply_limiter, extrema_clipping, Qm.v0, Q0.v0, Qp.v0, l.v0, r.v0);
+        interp_l2r2_cell<M>(w, apply_limiter, extrema_clipping, Qm.v1, Q0.v1, Qp.v1, l.v1, r.v1);
+        interp_l2r2_cell<M>(w, apply_limiter, extrema_clipping, Qm.v2, Q0.v2, Qp.v2, l.v2, r.v2);
+        interp_l2r2_cell<M>(w, apply_limiter, extrema_clipping, Qm.rho, Q0.rho, Qp.rho, l.rho, r.rho);
+        interp_l2r2_cell<M>(w, apply_limiter, extrema_clipping, Qm.u, Q0.u, Qp.u, l.u, r.u);
+        if (!(l.u <= R(0) || l.rho <= R(0))) { Lp.rho = l.rho; Lp.v0 = l.v0; Lp.v1 = l.v1; Lp.v2 = l.v2; Lp.u = l.u; }
+        if (!(r.u <= R(0) || r.rho <= R(0))) { Rm.rho = r.rho; Rm.v0 = r.v0; Rm.v1 = r.v1; Rm.v2 = r.v2; Rm.u = r.u; }
+    }
+}
+
+template <int C, class R>
+__device__ __forceinline__ R vcomp(const SideT<R>& S)
+{
+    if constexpr (C == 0) return S.v0;
+    else if constexpr (C == 1) return S.v1;
+    else return S.v2;
+}
+
+// Flux through a face from its left and right states; the face frame is the
+// permutation (PN, PT1, PT2) of the global axes: 3D i-face (x,y,z), j-face
+// (y,z,x), k-face (z,x,y); 2D i-face (x,y,z), j-face (y,x,z)
+// (sfluidblock.d:527-588, fvinterface.d:265-296).  The 2D i-face's t1 = -y is
+// an exact sign flip that every operation of the face path commutes with, so
+// it is not applied.  The face EOS is p = rho Rgas (u / Cv) (ideal_gas.d:98-106).
+// Fg receives the flux in the global frame (mass, x-, y-, z-momentum, energy).
+template <class M, int FLUX, int PN, int PT1, int PT2>
+__device__ __forceinline__ void side_flux(const SideT<typename M::R>& L, const SideT<typename M::R>& Rs,
+                                          const IdealGasT<typename M::R>& g, typename M::R Fg[5])
+{
+    using R = typename M::R;
+    FaceStateT<R> Lf, Rf;
+    Lf.rho = L.rho; Lf.u = L.u; Lf.a = L.a;
+    Lf.vn = vcomp<PN>(L); Lf.vt1 = vcomp<PT1>(L); Lf.vt2 = vcomp<PT2>(L);
+    Rf.rho = Rs.rho; Rf.u = Rs.u; Rf.a = Rs.a;
+    Rf.vn = vcomp<PN>(Rs); Rf.vt1 = vcomp<PT1>(Rs); Rf.vt2 = vcomp<PT2>(Rs);
+    R TL = Lf.u * g.Cvinv;
+    Lf.p = Lf.rho * g.Rgas * TL;
+    R TR = Rf.u * g.Cvinv;
+    Rf.p = Rf.rho * g.Rgas * TR;
+    Flux5T<R> F = flux_calc<M, FLUX>(Lf, Rf);
+    Fg[0] = F.mass;
+    Fg[1 + PN] = F.mn;
+    Fg[1 + PT1] = F.mt1;
+    Fg[1 + PT2] = F.mt2;
+    Fg[4] = F.E;
+}
+
+template <class R>
+__device__ __forceinline__ PrimT<R> lds_prim(const R* sP, int plane, int s)
+{
+    return PrimT<R>{sP[s], sP[plane + s], sP[2 * plane + s], sP[3 * plane + s], sP[4 * plane + s],
+                    sP[5 * plane + s]};
+}
+
+template <class R>
+__device__ __forceinline__ void lds_put_side(R* dst, int nslot, int s, const SideT<R>& S)
+{
+    dst[s] = S.rho; dst[nslot + s] = S.v0; dst[2 * nslot + s] = S.v1;
+    dst[3 * nslot + s] = S.v2; dst[4 * nslot + s] = S.u; dst[5 * nslot + s] = S.a;
+}
+
+template <class R>
+__device__ __forceinline__ SideT<R> lds_side(const R* src, int nslot, int s)
+{
+    return SideT<R>{src[s], src[nslot + s], src[2 * nslot + s], src[3 * nslot + s], src[4 * nslot + s],
+                    src[5 * nslot + s]};
+}
+
+template <class R>
+__device__ __forceinline__ void lds_put_flux(R* dst, int nslot, int s, const R F[5])
+{
+#pragma unroll
+    for (int q = 0; q < 5; ++q) dst[q * nslot + s] = F[q];
+}
+
+// DEF: the default reconstruction (order 2, limiter, extrema clipping) fixed
+// at compile time; otherwise the three options are read from the arguments.
+template <class M, int TX, int TY, int DIMS, int FLUX, bool DEF>
 __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs A)
 {
     using R = typename M::R;
     constexpr int NT = TX * TY;
     constexpr int PX = TX + 4, PY = TY + 4, PLANE = PX * PY;
-    constexpr int NFI = TY * (TX + 1), NFJ = (TY + 1) * TX;
+    constexpr int NSI = (TX + 1) * TY;       // i-face slots: face fc (0..TX) of row r is r*(TX+1)+fc
+    constexpr int NSJ = TX * (TY + 1);       // j-face slots: face fr (0..TY) of column c is fr*TX+c
+    constexpr int NHALO = 2 * (TX + TY);     // face states of halo cells
+    constexpr int NBND = TX + TY;            // west and south faces of the tile
+    static_assert(NHALO <= NT && NBND <= NT, "tile too narrow for its halo work");
+    // j-face frame
+    constexpr int JN = 1, JT1 = (DIMS == 3) ? 2 : 0, JT2 = (DIMS == 3) ? 0 : 2;
     __shared__ R sP[6 * PLANE];              // rho, vx, vy, vz, u, a of plane k with halos
-    __shared__ R sFi[5 * NFI];               // i-face fluxes (global frame)
-    __shared__ R sFj[5 * NFJ];               // j-face fluxes (global frame)
+    __shared__ R sI[6 * NSI];                // right states of i-faces, then i-face fluxes (global frame)
+    __shared__ R sJ[6 * NSJ];                // right states of j-faces, then j-face fluxes
+    __shared__ R sLw[6 * TY];                // left states of the tile's west faces
+    __shared__ R sLs[6 * TX];                // left states of the tile's south faces
 
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
@@ -126,36 +263,35 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs A)
     const bool own_ij = (i < A.ni) && (j < A.nj);
     const int k0 = (DIMS == 3) ? blockIdx.z * A.kchunk : 0;
     const int k1 = (DIMS == 3) ? min(k0 + A.kchunk, A.nk) : 1;
-    const bool ho = A.high_order != 0, lim = A.apply_limiter != 0, clip = A.extrema_clipping != 0;
+    const bool ho = DEF || A.high_order != 0, lim = DEF || A.apply_limiter != 0;
+    const bool clip = DEF || A.extrema_clipping != 0;
     const IdealGasT<R> gR = cast_gas<R>(A.gas);
+    const L2R2UT<R> wi = cast_wu<R>(A.wu[0]), wj = cast_wu<R>(A.wu[1]);
 
-    // k-face frame (z, x, y)
-    auto kface = [](const PrimT<R>& P) { return PrimT<R>{P.rho, P.v2, P.v0, P.v1, P.u, P.a}; };
-
-    // k-direction register queue (k-1, k, k+1, k+2) and the carried bottom-face flux
-    PrimT<R> q0, q1, q2, q3;
+    // k-direction register queue: own = plane k, qa = k+1, qb = k+2; LpK is the
+    // left state of the top face of the own cell, Fb the flux of its bottom face.
+    PrimT<R> own, qa, qb;
+    SideT<R> LpK;
     R Fb[5] = {R(0), R(0), R(0), R(0), R(0)};
     if constexpr (DIMS == 3) {
-        PrimT<R> qa = fetch<M>(A, i, j, k0 - 2);
-        q0 = fetch<M>(A, i, j, k0 - 1);
-        q1 = fetch<M>(A, i, j, k0);
-        q2 = fetch<M>(A, i, j, k0 + 1);
-        if (own_ij) {
-            Flux5T<R> F = face_flux_ideal<M, FLUX>(kface(qa), kface(q0), kface(q1), kface(q2), cast_w<R>(A.w[2]), gR,
-                                                   ho, lim, clip);
-            Fb[0] = F.mass; Fb[1] = F.mt1; Fb[2] = F.mt2; Fb[3] = F.mn; Fb[4] = F.E;
-        }
-        q3 = fetch<M>(A, i, j, k0 + 2);
+        const L2R2UT<R> wk = cast_wu<R>(A.wu[2]);
+        PrimT<R> qm2 = fetch<M>(A, i, j, k0 - 2);
+        PrimT<R> qm1 = fetch<M>(A, i, j, k0 - 1);
+        own = fetch<M>(A, i, j, k0);
+        qa = fetch<M>(A, i, j, k0 + 1);
+        SideT<R> Lbelow, Rdummy, Rown;
+        recon_cell<M>(wk, ho, lim, clip, qm2, qm1, own, Lbelow, Rdummy);
+        recon_cell<M>(wk, ho, lim, clip, qm1, own, qa, LpK, Rown);
+        if (own_ij) side_flux<M, FLUX, 2, 0, 1>(Lbelow, Rown, gR, Fb);
+        qb = fetch<M>(A, i, j, k0 + 2);
     }
 
     for (int k = k0; k < k1; ++k) {
-        PrimT<R> own = (DIMS == 3) ? q1 : fetch<M>(A, i, j, 0);
+        if constexpr (DIMS == 2) own = fetch<M>(A, i, j, 0);
         // 1. plane k with halos into LDS
-        {
-            const int s = (ty + 2) * PX + (tx + 2);
-            sP[0 * PLANE + s] = own.rho; sP[1 * PLANE + s] = own.v0; sP[2 * PLANE + s] = own.v1;
-            sP[3 * PLANE + s] = own.v2; sP[4 * PLANE + s] = own.u; sP[5 * PLANE + s] = own.a;
-        }
+        const int s0 = (ty + 2) * PX + (tx + 2);
+        sP[0 * PLANE + s0] = own.rho; sP[1 * PLANE + s0] = own.v0; sP[2 * PLANE + s0] = own.v1;
+        sP[3 * PLANE + s0] = own.v2; sP[4 * PLANE + s0] = own.u; sP[5 * PLANE + s0] = own.a;
         for (int t = tid; t < 4 * (TX + TY); t += NT) {
             int ii, jj;
             if (t < 4 * TY) {
@@ -176,61 +312,82 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs A)
             sP[3 * PLANE + s] = h.v2; sP[4 * PLANE + s] = h.u; sP[5 * PLANE + s] = h.a;
         }
         __syncthreads();
-        // 2. i- and j-faces of plane k: work item w < NT is the west face of cell
-        // (tx,ty), NT <= w < 2NT the south face, and the rest the east faces of
-        // the last column and the north faces of the top row.
-        for (int wI = tid; wI < 2 * NT + TX + TY; wI += NT) {
-            int dir, fr, fc;   // face direction, face row, face column (tile-local)
-            if (wI < NT) { dir = 0; fr = ty; fc = tx; }
-            else if (wI < 2 * NT) { dir = 1; fr = ty; fc = tx; }
-            else {
-                int e = wI - 2 * NT;
-                if (e < TY) { dir = 0; fr = e; fc = TX; } else { dir = 1; fr = TY; fc = e - TY; }
-            }
-            const int gi = i0 + fc, gj = j0 + fr;
-            const bool valid = (dir == 0) ? (gi <= A.ni && gj < A.nj) : (gj <= A.nj && gi < A.ni);
-            if (!valid) continue;
-            int pn, pt1, pt2;
-            face_perm<DIMS>(dir, pn, pt1, pt2);
-            // stencil L1 position (LDS): i-face fc sits between columns fc+1, fc+2
-            const int step = (dir == 0) ? 1 : PX;
-            const int base = (dir == 0) ? (fr + 2) * PX + fc : fr * PX + (fc + 2);
-            PrimT<R> S[4];
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const int s = base + m * step;
-                S[m].rho = sP[0 * PLANE + s];
-                S[m].v0 = sP[(1 + pn) * PLANE + s];
-                S[m].v1 = sP[(1 + pt1) * PLANE + s];
-                S[m].v2 = sP[(1 + pt2) * PLANE + s];
-                S[m].u = sP[4 * PLANE + s];
-                S[m].a = sP[5 * PLANE + s];
-            }
-            Flux5T<R> F = face_flux_ideal<M, FLUX>(S[0], S[1], S[2], S[3], cast_w<R>(A.w[dir]), gR, ho, lim, clip);
-            R* dst = (dir == 0) ? sFi : sFj;
-            const int nface = (dir == 0) ? NFI : NFJ;
-            const int fidx = (dir == 0) ? fr * (TX + 1) + fc : fr * TX + fc;
-            dst[0 * nface + fidx] = F.mass;
-            dst[(1 + pn) * nface + fidx] = F.mn;
-            dst[(1 + pt1) * nface + fidx] = F.mt1;
-            dst[(1 + pt2) * nface + fidx] = F.mt2;
-            dst[4 * nface + fidx] = F.E;
+        // 2. face states: own cell in i and j (right states to LDS, left states
+        // kept), halo cells' states the tile's boundary faces need, and the
+        // k-direction states of cell k+1.
+        SideT<R> LpI, LpJ;
+        {
+            SideT<R> Rm;
+            recon_cell<M>(wi, ho, lim, clip, lds_prim(sP, PLANE, s0 - 1), own, lds_prim(sP, PLANE, s0 + 1), LpI, Rm);
+            lds_put_side(sI, NSI, ty * (TX + 1) + tx, Rm);
+            recon_cell<M>(wj, ho, lim, clip, lds_prim(sP, PLANE, s0 - PX), own, lds_prim(sP, PLANE, s0 + PX), LpJ, Rm);
+            lds_put_side(sJ, NSJ, ty * TX + tx, Rm);
         }
-        // 3. k-face at the top of cell k from the register queue
+        {
+            const int e = tid - (NT - NHALO);
+            if (e >= 0) {
+                // e < TY: west neighbour column (left state), e < 2TY: east neighbour
+                // column (right state), then south row (left), north row (right)
+                int s, step, dst_kind, slot;
+                if (e < TY) { s = (e + 2) * PX + 1; step = 1; dst_kind = 0; slot = e; }
+                else if (e < 2 * TY) { s = (e - TY + 2) * PX + TX + 2; step = 1; dst_kind = 1; slot = (e - TY) * (TX + 1) + TX; }
+                else if (e < 2 * TY + TX) { s = PX + (e - 2 * TY) + 2; step = PX; dst_kind = 2; slot = e - 2 * TY; }
+                else { s = (TY + 2) * PX + (e - 2 * TY - TX) + 2; step = PX; dst_kind = 3; slot = TY * TX + (e - 2 * TY - TX); }
+                SideT<R> L, Rr;
+                if (step == 1)
+                    recon_cell<M>(wi, ho, lim, clip, lds_prim(sP, PLANE, s - 1), lds_prim(sP, PLANE, s),
+                                  lds_prim(sP, PLANE, s + 1), L, Rr);
+                else
+                    recon_cell<M>(wj, ho, lim, clip, lds_prim(sP, PLANE, s - PX), lds_prim(sP, PLANE, s),
+                                  lds_prim(sP, PLANE, s + PX), L, Rr);
+                if (dst_kind == 0) lds_put_side(sLw, TY, slot, L);
+                else if (dst_kind == 1) lds_put_side(sI, NSI, slot, Rr);
+                else if (dst_kind == 2) lds_put_side(sLs, TX, slot, L);
+                else lds_put_side(sJ, NSJ, slot, Rr);
+            }
+        }
+        // k-direction: face states of cell k+1, the flux of the own cell's top face
         R Ft[5] = {R(0), R(0), R(0), R(0), R(0)};
         if constexpr (DIMS == 3) {
-            if (own_ij) {
-                Flux5T<R> F = face_flux_ideal<M, FLUX>(kface(q0), kface(q1), kface(q2), kface(q3), cast_w<R>(A.w[2]),
-                                                       gR, ho, lim, clip);
-                Ft[0] = F.mass; Ft[1] = F.mt1; Ft[2] = F.mt2; Ft[3] = F.mn; Ft[4] = F.E;
+            SideT<R> LpK1, RmK1;
+            recon_cell<M>(cast_wu<R>(A.wu[2]), ho, lim, clip, own, qa, qb, LpK1, RmK1);
+            if (own_ij) side_flux<M, FLUX, 2, 0, 1>(LpK, RmK1, gR, Ft);
+            LpK = LpK1;
+        }
+        __syncthreads();
+        // 3. fluxes of the own cell's plus faces and of the tile's west and
+        // south faces; each face's flux overwrites its right state in LDS
+        // (the same thread read it).
+        if (own_ij) {
+            R Fx[5];
+            const int si = ty * (TX + 1) + tx + 1, sj = (ty + 1) * TX + tx;
+            side_flux<M, FLUX, 0, 1, 2>(LpI, lds_side(sI, NSI, si), gR, Fx);
+            lds_put_flux(sI, NSI, si, Fx);
+            side_flux<M, FLUX, JN, JT1, JT2>(LpJ, lds_side(sJ, NSJ, sj), gR, Fx);
+            lds_put_flux(sJ, NSJ, sj, Fx);
+        }
+        if (tid < NBND) {
+            R Fx[5];
+            if (tid < TY) {
+                if (j0 + tid < A.nj) {
+                    const int si = tid * (TX + 1);
+                    side_flux<M, FLUX, 0, 1, 2>(lds_side(sLw, TY, tid), lds_side(sI, NSI, si), gR, Fx);
+                    lds_put_flux(sI, NSI, si, Fx);
+                }
+            } else {
+                const int c = tid - TY;
+                if (i0 + c < A.ni) {
+                    side_flux<M, FLUX, JN, JT1, JT2>(lds_side(sLs, TX, c), lds_side(sJ, NSJ, c), gR, Fx);
+                    lds_put_flux(sJ, NSJ, c, Fx);
+                }
             }
         }
         __syncthreads();
         // 4. flux divergence in face order N, E, S, W, T, B (fvcell.d:789-825),
         // stage update, decode check, write; all in FP64.
         if (own_ij) {
-            const int iW = ty * (TX + 1) + tx, iE = iW + 1;
-            const int jS = ty * TX + tx, jN = jS + TX;
+            const int iW = ty * (TX + 1) + tx, jS = ty * TX + tx;
+            const int iE = iW + 1, jN = jS + TX;
             double integ[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
             auto add_face = [&](const R* Fsrc, int nface, int fidx, double outsign, double Aface) {
                 double area = outsign * Aface;
@@ -240,16 +397,16 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs A)
                 if (DIMS == 3) integ[3] -= double(Fsrc[3 * nface + fidx]) * area;
                 integ[4] -= double(Fsrc[4 * nface + fidx]) * area;
             };
-            add_face(sFj, NFJ, jN, 1.0, A.area[1]);
-            add_face(sFi, NFI, iE, 1.0, A.area[0]);
-            add_face(sFj, NFJ, jS, -1.0, A.area[1]);
-            add_face(sFi, NFI, iW, -1.0, A.area[0]);
+            add_face(sJ, NSJ, jN, 1.0, A.area[1]);
+            add_face(sI, NSI, iE, 1.0, A.area[0]);
+            add_face(sJ, NSJ, jS, -1.0, A.area[1]);
+            add_face(sI, NSI, iW, -1.0, A.area[0]);
             if constexpr (DIMS == 3) {
                 add_face(Ft, 1, 0, 1.0, A.area[2]);
                 add_face(Fb, 1, 0, -1.0, A.area[2]);
             }
             const long c = cell_index(A, i, j, k);
-            const long qs = A.qs;
+            const long qs = opaque_qs(A.qs);
             double Un[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q) {
@@ -299,9 +456,32 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs A)
         if constexpr (DIMS == 3) {
 #pragma unroll
             for (int q = 0; q < 5; ++q) Fb[q] = Ft[q];
-            q0 = q1; q1 = q2; q2 = q3;
-            if (k + 1 < k1) q3 = fetch<M>(A, i, j, k + 3);
+            own = qa; qa = qb;
+            if (k + 1 < k1) qb = fetch<M>(A, i, j, k + 3);
         }
+    }
+}
+
+template <class M, int DIMS, int FLUX>
+void launch_stage_df(const StageArgs& A, dim3 grid, hipStream_t stream)
+{
+    const dim3 block(TILE_X * TILE_Y);
+    if (A.high_order && A.apply_limiter && A.extrema_clipping)
+        hipLaunchKernelGGL((stage_kernel<M, TILE_X, TILE_Y, DIMS, FLUX, true>), grid, block, 0, stream, A);
+    else
+        hipLaunchKernelGGL((stage_kernel<M, TILE_X, TILE_Y, DIMS, FLUX, false>), grid, block, 0, stream, A);
+}
+
+// Launch of the stage kernel of mode M for a block of dims dimensions.
+template <class M>
+void launch_stage_mode(const StageArgs& A, dim3 grid, hipStream_t stream, int dims, int flux)
+{
+    if (dims == 3) {
+        if (flux == FLUX_HLLC) launch_stage_df<M, 3, FLUX_HLLC>(A, grid, stream);
+        else launch_stage_df<M, 3, FLUX_AUSMDV>(A, grid, stream);
+    } else {
+        if (flux == FLUX_HLLC) launch_stage_df<M, 2, FLUX_HLLC>(A, grid, stream);
+        else launch_stage_df<M, 2, FLUX_AUSMDV>(A, grid, stream);
     }
 }
 
