@@ -8,10 +8,9 @@
 //              operand order, so results are the reference arithmetic's
 //              roundings (bit-identical to the CPU oracle).
 //   ModeFast64 FP64; FMA contraction; a/b evaluated as a*rcp(b) with the
-//              reciprocal refined by two Newton steps and one residual
-//              correction of the quotient (within ~1 ulp); reciprocals of
-//              repeated denominators are shared by common-subexpression
-//              elimination.
+//              reciprocal refined by one Newton step and the quotient by one
+//              residual correction (within ~1 ulp); reciprocals of repeated
+//              denominators are shared; p/rho of a face state is Rgas T.
 //   ModeMixed  FP32 reconstruction and flux (same division scheme in FP32);
 //              decode, accumulation and update stay FP64 in the stage kernel.
 #pragma once
@@ -27,34 +26,33 @@ struct ModeExact { using R = double; static constexpr bool kFast = false; };
 struct ModeFast64 { using R = double; static constexpr bool kFast = true; };
 struct ModeMixed { using R = float; static constexpr bool kFast = true; };
 
-__device__ __forceinline__ double rcp_refined(double b)
+// Reciprocal seeds for the fast division: v_rcp_f64 is good to about 2^-23
+// relative, one Newton step takes it to about 2^-46; v_rcp_f32 is good to
+// about one ulp and is used as is.
+__device__ __forceinline__ double rcp_seed(double b)
 {
     double r = __builtin_amdgcn_rcp(b);
     double e = fma(-b, r, 1.0);
-    r = fma(e, r, r);
-    e = fma(-b, r, 1.0);
-    r = fma(e, r, r);
-    return r;
+    return fma(e, r, r);
 }
 
-__device__ __forceinline__ float rcp_refined(float b)
+__device__ __forceinline__ float rcp_seed(float b) { return __builtin_amdgcn_rcpf(b); }
+
+// a/b from a reciprocal seed rb of b: q = a rb, then one correction with the
+// exact residual a - b q; the result is within about one ulp of a/b.
+template <class R>
+__device__ __forceinline__ R dv_seed(R a, R b, R rb)
 {
-    float r = __builtin_amdgcn_rcpf(b);
-    float e = fmaf(-b, r, 1.0f);
-    return fmaf(e, r, r);
+    R q = a * rb;
+    R res = fma(-b, q, a);
+    return fma(res, rb, q);
 }
 
 template <class M>
 __device__ __forceinline__ typename M::R dv(typename M::R a, typename M::R b)
 {
-    if constexpr (!M::kFast) {
-        return a / b;
-    } else {
-        typename M::R r = rcp_refined(b);
-        typename M::R q = a * r;
-        typename M::R res = fma(-b, q, a);
-        return fma(res, r, q);
-    }
+    if constexpr (!M::kFast) return a / b;
+    else return dv_seed(a, b, rcp_seed(b));
 }
 
 template <class R> __device__ __forceinline__ R mx(R a, R b) { return fmax(a, b); }
@@ -129,12 +127,13 @@ __device__ __forceinline__ R clip_to_limits(R q, R A, R B)
 }
 
 // clip_to_limits in the mode's form: the reference's comparisons for
-// ModeExact; max/min for the fast modes (the same value for ordered,
-// non-NaN operands up to the sign of a zero).
+// ModeExact; max/min (FP64) or the median of three (FP32) for the fast modes
+// (the same value for ordered, non-NaN operands up to the sign of a zero).
 template <class M>
 __device__ __forceinline__ typename M::R clip_limits(typename M::R q, typename M::R A, typename M::R B)
 {
     if constexpr (!M::kFast) return clip_to_limits(q, A, B);
+    else if constexpr (std::is_same<typename M::R, float>::value) return __builtin_amdgcn_fmed3f(q, A, B);
     else return mn(mx(q, mn(A, B)), mx(A, B));
 }
 
@@ -188,10 +187,11 @@ __device__ __forceinline__ Prim decode_ideal(double U0, double U1, double U2, do
     return P;
 }
 
-// A face-side state after reconstruction and the face EOS.
+// A face-side state after reconstruction and the face EOS; RT = Rgas T, the
+// ratio p/rho of the ideal gas, is what the fast modes use for p/rho.
 template <class R>
 struct FaceStateT {
-    R rho, p, u, a, vn, vt1, vt2;
+    R rho, p, u, a, vn, vt1, vt2, RT;
 };
 
 // Flux in the face frame: mass, normal momentum, two tangential momenta, energy.
@@ -208,7 +208,7 @@ __device__ __forceinline__ Flux5T<typename M::R> ausmdv(const FaceStateT<typenam
     using R = typename M::R;
     R rL = L.rho;
     R pL = L.p;
-    R pLrL = dv<M>(pL, rL);
+    R pLrL = M::kFast ? L.RT : dv<M>(pL, rL);
     R uL = L.vn;
     R vL = L.vt1;
     R wL = L.vt2;
@@ -218,7 +218,7 @@ __device__ __forceinline__ Flux5T<typename M::R> ausmdv(const FaceStateT<typenam
     R HL = eL + pLrL + keL;
     R rR = Rs.rho;
     R pR = Rs.p;
-    R pRrR = dv<M>(pR, rR);
+    R pRrR = M::kFast ? Rs.RT : dv<M>(pR, rR);
     R uR = Rs.vn;
     R vR = Rs.vt1;
     R wR = Rs.vt2;
@@ -229,13 +229,23 @@ __device__ __forceinline__ Flux5T<typename M::R> ausmdv(const FaceStateT<typenam
     R alphaL = dv<M>(R(2.0) * pLrL, pLrL + pRrR);
     R alphaR = dv<M>(R(2.0) * pRrR, pLrL + pRrR);
     R am = mx(aL, aR);
-    R ML = dv<M>(uL, am);
-    R MR = dv<M>(uR, am);
+    R ML, MR, r4am = R(0);
+    if constexpr (M::kFast) {
+        // one reciprocal seed serves uL/am, uR/am and x/(4 am) (4 is exact)
+        R ram = rcp_seed(am);
+        ML = dv_seed(uL, am, ram);
+        MR = dv_seed(uR, am, ram);
+        r4am = R(0.25) * ram;
+    } else {
+        ML = dv<M>(uL, am);
+        MR = dv<M>(uR, am);
+    }
+    auto over4am = [&](R x) { return M::kFast ? dv_seed(x, R(4.0) * am, r4am) : dv<M>(x, R(4.0) * am); };
     R pLplus, uLplus;
     R duL = R(0.5) * (uL + ab(uL));
     if (ab(ML) <= R(1.0)) {
         pLplus = pL * (ML + R(1.0)) * (ML + R(1.0)) * (R(2.0) - ML) * R(0.25);
-        uLplus = alphaL * (dv<M>((uL + am) * (uL + am), R(4.0) * am) - duL) + duL;
+        uLplus = alphaL * (over4am((uL + am) * (uL + am)) - duL) + duL;
     } else {
         pLplus = dv<M>(pL * duL, uL);
         uLplus = duL;
@@ -244,7 +254,7 @@ __device__ __forceinline__ Flux5T<typename M::R> ausmdv(const FaceStateT<typenam
     R duR = R(0.5) * (uR - ab(uR));
     if (ab(MR) <= R(1.0)) {
         pRminus = pR * (MR - R(1.0)) * (MR - R(1.0)) * (R(2.0) + MR) * R(0.25);
-        uRminus = alphaR * (dv<M>(-(uR - am) * (uR - am), R(4.0) * am) - duR) + duR;
+        uRminus = alphaR * (over4am(-(uR - am) * (uR - am)) - duR) + duR;
     } else {
         pRminus = dv<M>(pR * duR, uR);
         uRminus = duR;

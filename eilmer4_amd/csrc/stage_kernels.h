@@ -165,8 +165,12 @@ __device__ __forceinline__ void recon_cell(const L2R2UT<typename M::R>& w, bool 
         interp_l2r2_cell<M>(w, apply_limiter, extrema_clipping, Qm.v2, Q0.v2, Qp.v2, l.v2, r.v2);
         interp_l2r2_cell<M>(w, apply_limiter, extrema_clipping, Qm.rho, Q0.rho, Qp.rho, l.rho, r.rho);
         interp_l2r2_cell<M>(w, apply_limiter, extrema_clipping, Qm.u, Q0.u, Qp.u, l.u, r.u);
-        if (!(l.u <= R(0) || l.rho <= R(0))) { Lp.rho = l.rho; Lp.v0 = l.v0; Lp.v1 = l.v1; Lp.v2 = l.v2; Lp.u = l.u; }
-        if (!(r.u <= R(0) || r.rho <= R(0))) { Rm.rho = r.rho; Rm.v0 = r.v0; Rm.v1 = r.v1; Rm.v2 = r.v2; Rm.u = r.u; }
+        // the fall-back is rare: a branch around it, not selects
+        const SideT<R> cell = Lp;
+        Lp = SideT<R>{l.rho, l.v0, l.v1, l.v2, l.u, Q0.a};
+        Rm = SideT<R>{r.rho, r.v0, r.v1, r.v2, r.u, Q0.a};
+        if (__builtin_expect(l.u <= R(0) || l.rho <= R(0), 0)) Lp = cell;
+        if (__builtin_expect(r.u <= R(0) || r.rho <= R(0), 0)) Rm = cell;
     }
 }
 
@@ -196,9 +200,17 @@ __device__ __forceinline__ void side_flux(const SideT<typename M::R>& L, const S
     Rf.rho = Rs.rho; Rf.u = Rs.u; Rf.a = Rs.a;
     Rf.vn = vcomp<PN>(Rs); Rf.vt1 = vcomp<PT1>(Rs); Rf.vt2 = vcomp<PT2>(Rs);
     R TL = Lf.u * g.Cvinv;
-    Lf.p = Lf.rho * g.Rgas * TL;
     R TR = Rf.u * g.Cvinv;
-    Rf.p = Rf.rho * g.Rgas * TR;
+    if constexpr (M::kFast) {
+        Lf.RT = g.Rgas * TL;
+        Rf.RT = g.Rgas * TR;
+        Lf.p = Lf.rho * Lf.RT;
+        Rf.p = Rf.rho * Rf.RT;
+    } else {
+        Lf.p = Lf.rho * g.Rgas * TL;
+        Rf.p = Rf.rho * g.Rgas * TR;
+        Lf.RT = Rf.RT = R(0);
+    }
     Flux5T<R> F = flux_calc<M, FLUX>(Lf, Rf);
     Fg[0] = F.mass;
     Fg[1 + PN] = F.mn;
@@ -235,10 +247,28 @@ __device__ __forceinline__ void lds_put_flux(R* dst, int nslot, int s, const R F
     for (int q = 0; q < 5; ++q) dst[q * nslot + s] = F[q];
 }
 
+// Waves per SIMD the register allocation of the stage kernel aims for
+// (measured at 256^3: three for the exact FP64 form, whose few spills cost
+// less than the occupancy gains; four for the mixed form, which then fits in
+// 128 registers; the plain FP64 form spills heavily below 256 and keeps two).
+#ifndef E4_STAGE_WAVES_EXACT
+#define E4_STAGE_WAVES_EXACT 3
+#endif
+#ifndef E4_STAGE_WAVES_FAST64
+#define E4_STAGE_WAVES_FAST64 1
+#endif
+#ifndef E4_STAGE_WAVES_MIXED
+#define E4_STAGE_WAVES_MIXED 4
+#endif
+template <class M> struct StageOccupancy { static constexpr int waves = E4_STAGE_WAVES_FAST64; };
+template <> struct StageOccupancy<ModeExact> { static constexpr int waves = E4_STAGE_WAVES_EXACT; };
+template <> struct StageOccupancy<ModeMixed> { static constexpr int waves = E4_STAGE_WAVES_MIXED; };
+
 // DEF: the default reconstruction (order 2, limiter, extrema clipping) fixed
 // at compile time; otherwise the three options are read from the arguments.
 template <class M, int TX, int TY, int DIMS, int FLUX, bool DEF>
-__global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs A)
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(StageOccupancy<M>::waves)))
+void stage_kernel(StageArgs A)
 {
     using R = typename M::R;
     constexpr int NT = TX * TY;
