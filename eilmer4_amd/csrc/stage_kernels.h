@@ -39,6 +39,12 @@ __device__ __forceinline__ long opaque_qs(long qs)
     return qs;
 }
 
+__device__ __forceinline__ int opaque_i(int x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 // Decode of the cell (i,j,k); positions one or two cells outside the block in
 // one index direction are ghost cells and take the value the face's ghost-cell
 // effect gives them.  Positions further out return a harmless dummy state.
@@ -169,8 +175,11 @@ __device__ __forceinline__ void recon_cell(const L2R2UT<typename M::R>& w, bool 
         const SideT<R> cell = Lp;
         Lp = SideT<R>{l.rho, l.v0, l.v1, l.v2, l.u, Q0.a};
         Rm = SideT<R>{r.rho, r.v0, r.v1, r.v2, r.u, Q0.a};
-        if (__builtin_expect(l.u <= R(0) || l.rho <= R(0), 0)) Lp = cell;
-        if (__builtin_expect(r.u <= R(0) || r.rho <= R(0), 0)) Rm = cell;
+        const bool badL = l.u <= R(0) || l.rho <= R(0), badR = r.u <= R(0) || r.rho <= R(0);
+        if (__builtin_expect(badL || badR, 0)) {
+            if (badL) Lp = cell;
+            if (badR) Rm = cell;
+        }
     }
 }
 
@@ -248,14 +257,14 @@ __device__ __forceinline__ void lds_put_flux(R* dst, int nslot, int s, const R F
 }
 
 // Waves per SIMD the register allocation of the stage kernel aims for
-// (measured at 256^3: three for the exact FP64 form, whose few spills cost
-// less than the occupancy gains; four for the mixed form, which then fits in
-// 128 registers; the plain FP64 form spills heavily below 256 and keeps two).
+// (measured at 256^3: three for the FP64 forms, whose few spills cost less
+// than the occupancy gains, and which the 48 KiB of LDS per workgroup allows;
+// four for the mixed form, which then fits in 128 registers).
 #ifndef E4_STAGE_WAVES_EXACT
 #define E4_STAGE_WAVES_EXACT 3
 #endif
 #ifndef E4_STAGE_WAVES_FAST64
-#define E4_STAGE_WAVES_FAST64 1
+#define E4_STAGE_WAVES_FAST64 3
 #endif
 #ifndef E4_STAGE_WAVES_MIXED
 #define E4_STAGE_WAVES_MIXED 4
@@ -298,31 +307,51 @@ void stage_kernel(StageArgs A)
     const IdealGasT<R> gR = cast_gas<R>(A.gas);
     const L2R2UT<R> wi = cast_wu<R>(A.wu[0]), wj = cast_wu<R>(A.wu[1]);
 
-    // k-direction register queue: own = plane k, qa = k+1, qb = k+2; LpK is the
-    // left state of the top face of the own cell, Fb the flux of its bottom face.
+    // k-direction register queue at the top of iteration k: own = plane k,
+    // qa = k+1, qb = k+2; LpC is the left state of the top face of cell k+1,
+    // Ft and Fb the fluxes of the own cell's top and bottom faces.  The
+    // k-direction work of the next plane is done at the end of an iteration,
+    // where the i/j face states are no longer live.
     PrimT<R> own, qa, qb;
-    SideT<R> LpK;
-    R Fb[5] = {R(0), R(0), R(0), R(0), R(0)};
+    SideT<R> LpC;
+    R Ft[5] = {R(0), R(0), R(0), R(0), R(0)}, Fb[5] = {R(0), R(0), R(0), R(0), R(0)};
     if constexpr (DIMS == 3) {
         const L2R2UT<R> wk = cast_wu<R>(A.wu[2]);
         PrimT<R> qm2 = fetch<M>(A, i, j, k0 - 2);
         PrimT<R> qm1 = fetch<M>(A, i, j, k0 - 1);
         own = fetch<M>(A, i, j, k0);
         qa = fetch<M>(A, i, j, k0 + 1);
-        SideT<R> Lbelow, Rdummy, Rown;
-        recon_cell<M>(wk, ho, lim, clip, qm2, qm1, own, Lbelow, Rdummy);
-        recon_cell<M>(wk, ho, lim, clip, qm1, own, qa, LpK, Rown);
-        if (own_ij) side_flux<M, FLUX, 2, 0, 1>(Lbelow, Rown, gR, Fb);
         qb = fetch<M>(A, i, j, k0 + 2);
+        SideT<R> Lbelow, Rdummy, Lown, Rown, Rabove;
+        recon_cell<M>(wk, ho, lim, clip, qm2, qm1, own, Lbelow, Rdummy);
+        recon_cell<M>(wk, ho, lim, clip, qm1, own, qa, Lown, Rown);
+        if (own_ij) side_flux<M, FLUX, 2, 0, 1>(Lbelow, Rown, gR, Fb);
+        recon_cell<M>(wk, ho, lim, clip, own, qa, qb, LpC, Rabove);
+        if (own_ij) side_flux<M, FLUX, 2, 0, 1>(Lown, Rabove, gR, Ft);
     }
 
     for (int k = k0; k < k1; ++k) {
-        if constexpr (DIMS == 2) own = fetch<M>(A, i, j, 0);
+        // per-thread offsets are recomputed in every iteration rather than
+        // held in registers across the loop
+        const int tidk = opaque_i(tid);
+        const int txk = tidk % TX, tyk = tidk / TX;
+        const int ik = i0 + txk, jk = j0 + tyk;
+        const bool own_k = (ik < A.ni) && (jk < A.nj);
+        if constexpr (DIMS == 2) own = fetch<M>(A, ik, jk, 0);
+        // conserved state of plane k+3, loaded now and decoded at the end
+        double raw[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        const bool raw_ok = (DIMS == 3) && own_k && (k + 1 < k1) && (k + 3 < A.nk);
+        if (raw_ok) {
+            const long qs = opaque_qs(A.qs);
+            const double* p = A.Us + cell_index(A, ik, jk, k + 3);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) raw[q] = p[q * qs];
+        }
         // 1. plane k with halos into LDS
-        const int s0 = (ty + 2) * PX + (tx + 2);
+        const int s0 = (tyk + 2) * PX + (txk + 2);
         sP[0 * PLANE + s0] = own.rho; sP[1 * PLANE + s0] = own.v0; sP[2 * PLANE + s0] = own.v1;
         sP[3 * PLANE + s0] = own.v2; sP[4 * PLANE + s0] = own.u; sP[5 * PLANE + s0] = own.a;
-        for (int t = tid; t < 4 * (TX + TY); t += NT) {
+        for (int t = tidk; t < 4 * (TX + TY); t += NT) {
             int ii, jj;
             if (t < 4 * TY) {
                 int side = t / (2 * TY), rem = t % (2 * TY);
@@ -343,18 +372,9 @@ void stage_kernel(StageArgs A)
         }
         __syncthreads();
         // 2. face states: own cell in i and j (right states to LDS, left states
-        // kept), halo cells' states the tile's boundary faces need, and the
-        // k-direction states of cell k+1.
-        SideT<R> LpI, LpJ;
+        // kept) and the halo cells' states the tile's boundary faces need.
         {
-            SideT<R> Rm;
-            recon_cell<M>(wi, ho, lim, clip, lds_prim(sP, PLANE, s0 - 1), own, lds_prim(sP, PLANE, s0 + 1), LpI, Rm);
-            lds_put_side(sI, NSI, ty * (TX + 1) + tx, Rm);
-            recon_cell<M>(wj, ho, lim, clip, lds_prim(sP, PLANE, s0 - PX), own, lds_prim(sP, PLANE, s0 + PX), LpJ, Rm);
-            lds_put_side(sJ, NSJ, ty * TX + tx, Rm);
-        }
-        {
-            const int e = tid - (NT - NHALO);
+            const int e = tidk - (NT - NHALO);
             if (e >= 0) {
                 // e < TY: west neighbour column (left state), e < 2TY: east neighbour
                 // column (right state), then south row (left), north row (right)
@@ -376,36 +396,36 @@ void stage_kernel(StageArgs A)
                 else lds_put_side(sJ, NSJ, slot, Rr);
             }
         }
-        // k-direction: face states of cell k+1, the flux of the own cell's top face
-        R Ft[5] = {R(0), R(0), R(0), R(0), R(0)};
-        if constexpr (DIMS == 3) {
-            SideT<R> LpK1, RmK1;
-            recon_cell<M>(cast_wu<R>(A.wu[2]), ho, lim, clip, own, qa, qb, LpK1, RmK1);
-            if (own_ij) side_flux<M, FLUX, 2, 0, 1>(LpK, RmK1, gR, Ft);
-            LpK = LpK1;
+        SideT<R> LpI, LpJ;
+        {
+            SideT<R> Rm;
+            recon_cell<M>(wi, ho, lim, clip, lds_prim(sP, PLANE, s0 - 1), own, lds_prim(sP, PLANE, s0 + 1), LpI, Rm);
+            lds_put_side(sI, NSI, tyk * (TX + 1) + txk, Rm);
+            recon_cell<M>(wj, ho, lim, clip, lds_prim(sP, PLANE, s0 - PX), own, lds_prim(sP, PLANE, s0 + PX), LpJ, Rm);
+            lds_put_side(sJ, NSJ, tyk * TX + txk, Rm);
         }
         __syncthreads();
         // 3. fluxes of the own cell's plus faces and of the tile's west and
         // south faces; each face's flux overwrites its right state in LDS
         // (the same thread read it).
-        if (own_ij) {
+        if (own_k) {
             R Fx[5];
-            const int si = ty * (TX + 1) + tx + 1, sj = (ty + 1) * TX + tx;
+            const int si = tyk * (TX + 1) + txk + 1, sj = (tyk + 1) * TX + txk;
             side_flux<M, FLUX, 0, 1, 2>(LpI, lds_side(sI, NSI, si), gR, Fx);
             lds_put_flux(sI, NSI, si, Fx);
             side_flux<M, FLUX, JN, JT1, JT2>(LpJ, lds_side(sJ, NSJ, sj), gR, Fx);
             lds_put_flux(sJ, NSJ, sj, Fx);
         }
-        if (tid < NBND) {
+        if (tidk < NBND) {
             R Fx[5];
-            if (tid < TY) {
-                if (j0 + tid < A.nj) {
-                    const int si = tid * (TX + 1);
-                    side_flux<M, FLUX, 0, 1, 2>(lds_side(sLw, TY, tid), lds_side(sI, NSI, si), gR, Fx);
+            if (tidk < TY) {
+                if (j0 + tidk < A.nj) {
+                    const int si = tidk * (TX + 1);
+                    side_flux<M, FLUX, 0, 1, 2>(lds_side(sLw, TY, tidk), lds_side(sI, NSI, si), gR, Fx);
                     lds_put_flux(sI, NSI, si, Fx);
                 }
             } else {
-                const int c = tid - TY;
+                const int c = tidk - TY;
                 if (i0 + c < A.ni) {
                     side_flux<M, FLUX, JN, JT1, JT2>(lds_side(sLs, TX, c), lds_side(sJ, NSJ, c), gR, Fx);
                     lds_put_flux(sJ, NSJ, c, Fx);
@@ -415,8 +435,8 @@ void stage_kernel(StageArgs A)
         __syncthreads();
         // 4. flux divergence in face order N, E, S, W, T, B (fvcell.d:789-825),
         // stage update, decode check, write; all in FP64.
-        if (own_ij) {
-            const int iW = ty * (TX + 1) + tx, jS = ty * TX + tx;
+        if (own_k) {
+            const int iW = tyk * (TX + 1) + txk, jS = tyk * TX + txk;
             const int iE = iW + 1, jN = jS + TX;
             double integ[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
             auto add_face = [&](const R* Fsrc, int nface, int fidx, double outsign, double Aface) {
@@ -435,7 +455,7 @@ void stage_kernel(StageArgs A)
                 add_face(Ft, 1, 0, 1.0, A.area[2]);
                 add_face(Fb, 1, 0, -1.0, A.area[2]);
             }
-            const long c = cell_index(A, i, j, k);
+            const long c = cell_index(A, ik, jk, k);
             const long qs = opaque_qs(A.qs);
             double Un[5];
 #pragma unroll
@@ -483,11 +503,25 @@ void stage_kernel(StageArgs A)
                 if (nfail) atomicAdd(&A.counters[1], nfail);
             }
         }
+        // 5. k-direction work for plane k+1: face states of cell k+2 and the
+        // flux of the top face of cell k+1
         if constexpr (DIMS == 3) {
+            if (k + 1 < k1) {
+                PrimT<R> q3;
+                if (raw_ok) {
+                    Prim P = decode_ideal<M>(raw[0], raw[1], raw[2], raw[3], raw[4], A.gas);
+                    q3 = PrimT<R>{R(P.rho), R(P.v0), R(P.v1), R(P.v2), R(P.u), R(P.a)};
+                } else {
+                    q3 = fetch<M>(A, ik, jk, k + 3);
+                }
+                SideT<R> Lnext, Rnext;
+                recon_cell<M>(cast_wu<R>(A.wu[2]), ho, lim, clip, qa, qb, q3, Lnext, Rnext);
 #pragma unroll
-            for (int q = 0; q < 5; ++q) Fb[q] = Ft[q];
-            own = qa; qa = qb;
-            if (k + 1 < k1) qb = fetch<M>(A, i, j, k + 3);
+                for (int q = 0; q < 5; ++q) Fb[q] = Ft[q];
+                if (own_k) side_flux<M, FLUX, 2, 0, 1>(LpC, Rnext, gR, Ft);
+                LpC = Lnext;
+                own = qa; qa = qb; qb = q3;
+            }
         }
     }
 }
