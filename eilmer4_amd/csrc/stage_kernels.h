@@ -45,35 +45,56 @@ __device__ __forceinline__ int opaque_i(int x)
     return x;
 }
 
+// The kernel's arguments read through a pointer to the kernarg segment that
+// the optimiser cannot see through: each phase of the plane loop re-reads what
+// it uses (scalar loads) instead of every argument being held in scalar
+// registers across the loop, which overflows the scalar register file.
+// (The host pass of the compiler only parses the kernel; it sees a plain
+// pointer.)
+#if defined(__HIP_DEVICE_COMPILE__)
+using KArgs = const __attribute__((address_space(4))) StageArgs*;
+
+__device__ __forceinline__ KArgs kargs()
+{
+    KArgs p = (KArgs)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return p;
+}
+#else
+using KArgs = const StageArgs*;
+__device__ __forceinline__ KArgs kargs() { return nullptr; }
+#endif
+
 // Decode of the cell (i,j,k); positions one or two cells outside the block in
 // one index direction are ghost cells and take the value the face's ghost-cell
 // effect gives them.  Positions further out return a harmless dummy state.
 template <class M>
-__device__ __forceinline__ Prim load_decode(const StageArgs& A, long c)
+__device__ __forceinline__ Prim load_decode(KArgs A, long c)
 {
-    const long qs = opaque_qs(A.qs);
-    const double* p = A.Us + c;
-    return decode_ideal<M>(p[0], p[qs], p[2 * qs], p[3 * qs], p[4 * qs], A.gas);
+    const long qs = opaque_qs(A->qs);
+    const double* p = A->Us + c;
+    const IdealGas g = A->gas;
+    return decode_ideal<M>(p[0], p[qs], p[2 * qs], p[3 * qs], p[4 * qs], g);
 }
 
-__device__ __forceinline__ long cell_index(const StageArgs& A, int i, int j, int k)
+__device__ __forceinline__ long cell_index(KArgs A, int i, int j, int k)
 {
-    return ((long)k * A.nj + j) * A.ni + i;
+    return ((long)k * A->nj + j) * A->ni + i;
 }
 
 template <class M>
-__device__ Prim fetch_ghost(const StageArgs& A, int i, int j, int k)
+__device__ Prim fetch_ghost(KArgs A, int i, int j, int k)
 {
     Prim dummy{1.0, 0.0, 0.0, 0.0, 1.0, 1.0};
     int dir, idx, n;
-    if (i < 0 || i >= A.ni) {
-        if (j < 0 || j >= A.nj || k < 0 || k >= A.nk) return dummy;
-        dir = 0; idx = i; n = A.ni;
-    } else if (j < 0 || j >= A.nj) {
-        if (k < 0 || k >= A.nk) return dummy;
-        dir = 1; idx = j; n = A.nj;
+    if (i < 0 || i >= A->ni) {
+        if (j < 0 || j >= A->nj || k < 0 || k >= A->nk) return dummy;
+        dir = 0; idx = i; n = A->ni;
+    } else if (j < 0 || j >= A->nj) {
+        if (k < 0 || k >= A->nk) return dummy;
+        dir = 1; idx = j; n = A->nj;
     } else {
-        dir = 2; idx = k; n = A.nk;
+        dir = 2; idx = k; n = A->nk;
     }
     const bool plus = idx >= n;
     const int layer = plus ? idx - n + 1 : -idx;
@@ -81,8 +102,8 @@ __device__ Prim fetch_ghost(const StageArgs& A, int i, int j, int k)
     const int face = (dir == 0) ? (plus ? FACE_EAST : FACE_WEST)
                    : (dir == 1) ? (plus ? FACE_NORTH : FACE_SOUTH)
                                 : (plus ? FACE_TOP : FACE_BOTTOM);
-    const int type = A.bc->type[face];
-    if (type == BC_INFLOW) return A.bc->fs[face];
+    const int type = A->bc->type[face];
+    if (type == BC_INFLOW) return A->bc->fs[face];
     int src;
     if (type == BC_EXTRAPOLATE) src = plus ? n - 1 : 0;
     else if (type == BC_PERIODIC) src = plus ? layer - 1 : n - layer;
@@ -98,21 +119,21 @@ __device__ Prim fetch_ghost(const StageArgs& A, int i, int j, int k)
 }
 
 template <class M>
-__device__ __forceinline__ PrimT<typename M::R> fetch(const StageArgs& A, int i, int j, int k)
+__device__ __forceinline__ PrimT<typename M::R> fetch(KArgs A, int i, int j, int k)
 {
     using R = typename M::R;
     Prim P;
-    if (i >= 0 && i < A.ni && j >= 0 && j < A.nj && k >= 0 && k < A.nk) P = load_decode<M>(A, cell_index(A, i, j, k));
+    if (i >= 0 && i < A->ni && j >= 0 && j < A->nj && k >= 0 && k < A->nk) P = load_decode<M>(A, cell_index(A, i, j, k));
     else P = fetch_ghost<M>(A, i, j, k);
     return PrimT<R>{R(P.rho), R(P.v0), R(P.v1), R(P.v2), R(P.u), R(P.a)};
 }
 
 // The update formulas of fvcell.d:1153-1393 for one conserved quantity.
-__device__ __forceinline__ double stage_update(const StageArgs& A, double uold, double dd0, double dd1, double d)
+__device__ __forceinline__ double stage_update(KArgs A, double uold, double dd0, double dd1, double d)
 {
-    if (A.stage == 1) return uold + A.c1 * d;
-    if (A.stage == 2) return uold + A.dt * (A.g1 * dd0 + A.g2 * d);
-    return uold + A.dt * (A.g1 * dd0 + A.g2 * dd1 + A.g3 * d);
+    if (A->stage == 1) return uold + A->c1 * d;
+    if (A->stage == 2) return uold + A->dt * (A->g1 * dd0 + A->g2 * d);
+    return uold + A->dt * (A->g1 * dd0 + A->g2 * dd1 + A->g3 * d);
 }
 
 __device__ __forceinline__ unsigned int wave_count(bool pred)
@@ -295,17 +316,16 @@ void stage_kernel(StageArgs A)
     __shared__ R sLw[6 * TY];                // left states of the tile's west faces
     __shared__ R sLs[6 * TX];                // left states of the tile's south faces
 
+    KArgs ap = kargs();
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
     const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
     const int i = i0 + tx, j = j0 + ty;
-    const bool own_ij = (i < A.ni) && (j < A.nj);
-    const int k0 = (DIMS == 3) ? blockIdx.z * A.kchunk : 0;
-    const int k1 = (DIMS == 3) ? min(k0 + A.kchunk, A.nk) : 1;
-    const bool ho = DEF || A.high_order != 0, lim = DEF || A.apply_limiter != 0;
-    const bool clip = DEF || A.extrema_clipping != 0;
-    const IdealGasT<R> gR = cast_gas<R>(A.gas);
-    const L2R2UT<R> wi = cast_wu<R>(A.wu[0]), wj = cast_wu<R>(A.wu[1]);
+    const bool own_ij = (i < ap->ni) && (j < ap->nj);
+    const int k0 = (DIMS == 3) ? blockIdx.z * ap->kchunk : 0;
+    const int k1 = (DIMS == 3) ? min(k0 + ap->kchunk, ap->nk) : 1;
+    const bool ho = DEF || ap->high_order != 0, lim = DEF || ap->apply_limiter != 0;
+    const bool clip = DEF || ap->extrema_clipping != 0;
 
     // k-direction register queue at the top of iteration k: own = plane k,
     // qa = k+1, qb = k+2; LpC is the left state of the top face of cell k+1,
@@ -316,12 +336,13 @@ void stage_kernel(StageArgs A)
     SideT<R> LpC;
     R Ft[5] = {R(0), R(0), R(0), R(0), R(0)}, Fb[5] = {R(0), R(0), R(0), R(0), R(0)};
     if constexpr (DIMS == 3) {
-        const L2R2UT<R> wk = cast_wu<R>(A.wu[2]);
-        PrimT<R> qm2 = fetch<M>(A, i, j, k0 - 2);
-        PrimT<R> qm1 = fetch<M>(A, i, j, k0 - 1);
-        own = fetch<M>(A, i, j, k0);
-        qa = fetch<M>(A, i, j, k0 + 1);
-        qb = fetch<M>(A, i, j, k0 + 2);
+        const L2R2UT<R> wk = cast_wu<R>(L2R2U(ap->wu[2]));
+        const IdealGasT<R> gR = cast_gas<R>(IdealGas(ap->gas));
+        PrimT<R> qm2 = fetch<M>(ap, i, j, k0 - 2);
+        PrimT<R> qm1 = fetch<M>(ap, i, j, k0 - 1);
+        own = fetch<M>(ap, i, j, k0);
+        qa = fetch<M>(ap, i, j, k0 + 1);
+        qb = fetch<M>(ap, i, j, k0 + 2);
         SideT<R> Lbelow, Rdummy, Lown, Rown, Rabove;
         recon_cell<M>(wk, ho, lim, clip, qm2, qm1, own, Lbelow, Rdummy);
         recon_cell<M>(wk, ho, lim, clip, qm1, own, qa, Lown, Rown);
@@ -333,17 +354,18 @@ void stage_kernel(StageArgs A)
     for (int k = k0; k < k1; ++k) {
         // per-thread offsets are recomputed in every iteration rather than
         // held in registers across the loop
+        ap = kargs();
         const int tidk = opaque_i(tid);
         const int txk = tidk % TX, tyk = tidk / TX;
         const int ik = i0 + txk, jk = j0 + tyk;
-        const bool own_k = (ik < A.ni) && (jk < A.nj);
-        if constexpr (DIMS == 2) own = fetch<M>(A, ik, jk, 0);
+        const bool own_k = (ik < ap->ni) && (jk < ap->nj);
+        if constexpr (DIMS == 2) own = fetch<M>(ap, ik, jk, 0);
         // conserved state of plane k+3, loaded now and decoded at the end
         double raw[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        const bool raw_ok = (DIMS == 3) && own_k && (k + 1 < k1) && (k + 3 < A.nk);
+        const bool raw_ok = (DIMS == 3) && own_k && (k + 1 < k1) && (k + 3 < ap->nk);
         if (raw_ok) {
-            const long qs = opaque_qs(A.qs);
-            const double* p = A.Us + cell_index(A, ik, jk, k + 3);
+            const long qs = opaque_qs(ap->qs);
+            const double* p = ap->Us + cell_index(ap, ik, jk, k + 3);
 #pragma unroll
             for (int q = 0; q < 5; ++q) raw[q] = p[q * qs];
         }
@@ -365,7 +387,7 @@ void stage_kernel(StageArgs A)
                 jj = side ? TY + 2 + row : row;
                 ii = col + 2;
             }
-            PrimT<R> h = fetch<M>(A, i0 + ii - 2, j0 + jj - 2, k);
+            PrimT<R> h = fetch<M>(ap, i0 + ii - 2, j0 + jj - 2, k);
             const int s = jj * PX + ii;
             sP[0 * PLANE + s] = h.rho; sP[1 * PLANE + s] = h.v0; sP[2 * PLANE + s] = h.v1;
             sP[3 * PLANE + s] = h.v2; sP[4 * PLANE + s] = h.u; sP[5 * PLANE + s] = h.a;
@@ -373,6 +395,8 @@ void stage_kernel(StageArgs A)
         __syncthreads();
         // 2. face states: own cell in i and j (right states to LDS, left states
         // kept) and the halo cells' states the tile's boundary faces need.
+        ap = kargs();
+        const L2R2UT<R> wi = cast_wu<R>(L2R2U(ap->wu[0])), wj = cast_wu<R>(L2R2U(ap->wu[1]));
         {
             const int e = tidk - (NT - NHALO);
             if (e >= 0) {
@@ -408,6 +432,8 @@ void stage_kernel(StageArgs A)
         // 3. fluxes of the own cell's plus faces and of the tile's west and
         // south faces; each face's flux overwrites its right state in LDS
         // (the same thread read it).
+        ap = kargs();
+        const IdealGasT<R> gR = cast_gas<R>(IdealGas(ap->gas));
         if (own_k) {
             R Fx[5];
             const int si = tyk * (TX + 1) + txk + 1, sj = (tyk + 1) * TX + txk;
@@ -419,14 +445,14 @@ void stage_kernel(StageArgs A)
         if (tidk < NBND) {
             R Fx[5];
             if (tidk < TY) {
-                if (j0 + tidk < A.nj) {
+                if (j0 + tidk < ap->nj) {
                     const int si = tidk * (TX + 1);
                     side_flux<M, FLUX, 0, 1, 2>(lds_side(sLw, TY, tidk), lds_side(sI, NSI, si), gR, Fx);
                     lds_put_flux(sI, NSI, si, Fx);
                 }
             } else {
                 const int c = tidk - TY;
-                if (i0 + c < A.ni) {
+                if (i0 + c < ap->ni) {
                     side_flux<M, FLUX, JN, JT1, JT2>(lds_side(sLs, TX, c), lds_side(sJ, NSJ, c), gR, Fx);
                     lds_put_flux(sJ, NSJ, c, Fx);
                 }
@@ -435,6 +461,7 @@ void stage_kernel(StageArgs A)
         __syncthreads();
         // 4. flux divergence in face order N, E, S, W, T, B (fvcell.d:789-825),
         // stage update, decode check, write; all in FP64.
+        ap = kargs();
         if (own_k) {
             const int iW = tyk * (TX + 1) + txk, jS = tyk * TX + txk;
             const int iE = iW + 1, jN = jS + TX;
@@ -447,24 +474,24 @@ void stage_kernel(StageArgs A)
                 if (DIMS == 3) integ[3] -= double(Fsrc[3 * nface + fidx]) * area;
                 integ[4] -= double(Fsrc[4 * nface + fidx]) * area;
             };
-            add_face(sJ, NSJ, jN, 1.0, A.area[1]);
-            add_face(sI, NSI, iE, 1.0, A.area[0]);
-            add_face(sJ, NSJ, jS, -1.0, A.area[1]);
-            add_face(sI, NSI, iW, -1.0, A.area[0]);
+            add_face(sJ, NSJ, jN, 1.0, ap->area[1]);
+            add_face(sI, NSI, iE, 1.0, ap->area[0]);
+            add_face(sJ, NSJ, jS, -1.0, ap->area[1]);
+            add_face(sI, NSI, iW, -1.0, ap->area[0]);
             if constexpr (DIMS == 3) {
-                add_face(Ft, 1, 0, 1.0, A.area[2]);
-                add_face(Fb, 1, 0, -1.0, A.area[2]);
+                add_face(Ft, 1, 0, 1.0, ap->area[2]);
+                add_face(Fb, 1, 0, -1.0, ap->area[2]);
             }
-            const long c = cell_index(A, ik, jk, k);
-            const long qs = opaque_qs(A.qs);
+            const long c = cell_index(ap, ik, jk, k);
+            const long qs = opaque_qs(ap->qs);
             double Un[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q) {
-                double d = A.vol_inv * integ[q] + 0.0;
-                double dd0 = (A.stage >= 2) ? A.d0[c + q * qs] : 0.0;
-                double dd1 = (A.stage >= 3) ? A.d1[c + q * qs] : 0.0;
-                if (A.dout) A.dout[c + q * qs] = d;
-                Un[q] = stage_update(A, A.Uold[c + q * qs], dd0, dd1, d);
+                double d = ap->vol_inv * integ[q] + 0.0;
+                double dd0 = (ap->stage >= 2) ? ap->d0[c + q * qs] : 0.0;
+                double dd1 = (ap->stage >= 3) ? ap->d1[c + q * qs] : 0.0;
+                if (ap->dout) ap->dout[c + q * qs] = d;
+                Un[q] = stage_update(ap, ap->Uold[c + q * qs], dd0, dd1, d);
             }
             // decode_conserved checks and the low-T fallback (fvcell.d:570-589, 680-697)
             bool decode_fail = false, invalid = false;
@@ -481,13 +508,13 @@ void stage_kernel(StageArgs A)
                 double T;
                 if (u <= 0.0) {
                     T = 200.0;
-                    u = A.gas.Cv * T;
+                    u = ap->gas.Cv * T;
                     Un[1] = rho * vx; Un[2] = rho * vy; Un[3] = rho * vz;
                     Un[4] = rho * (u + ke);
                 } else {
-                    T = u * A.gas.Cvinv;
+                    T = u * ap->gas.Cvinv;
                 }
-                double p = rho * A.gas.Rgas * T;
+                double p = rho * ap->gas.Rgas * T;
                 // check_data (flowstate.d:345-388, gas_state.d:199-241)
                 if (!isfinite(rho)) invalid = true;
                 if (!isfinite(T) || T < 1.01 * 1.0) invalid = true;
@@ -496,26 +523,28 @@ void stage_kernel(StageArgs A)
                 if (T > 50000.0) invalid = true;
             }
 #pragma unroll
-            for (int q = 0; q < 5; ++q) A.Unew[c + q * qs] = Un[q];
+            for (int q = 0; q < 5; ++q) ap->Unew[c + q * qs] = Un[q];
             unsigned int nbad = wave_count(invalid), nfail = wave_count(decode_fail);
             if ((nbad | nfail) && (__lane_id() == __ffsll((unsigned long long)__ballot(true)) - 1)) {
-                if (nbad) atomicAdd(&A.counters[0], nbad);
-                if (nfail) atomicAdd(&A.counters[1], nfail);
+                if (nbad) atomicAdd(&ap->counters[0], nbad);
+                if (nfail) atomicAdd(&ap->counters[1], nfail);
             }
         }
         // 5. k-direction work for plane k+1: face states of cell k+2 and the
         // flux of the top face of cell k+1
         if constexpr (DIMS == 3) {
             if (k + 1 < k1) {
+                ap = kargs();
+                const IdealGasT<R> gR = cast_gas<R>(IdealGas(ap->gas));
                 PrimT<R> q3;
                 if (raw_ok) {
-                    Prim P = decode_ideal<M>(raw[0], raw[1], raw[2], raw[3], raw[4], A.gas);
+                    Prim P = decode_ideal<M>(raw[0], raw[1], raw[2], raw[3], raw[4], IdealGas(ap->gas));
                     q3 = PrimT<R>{R(P.rho), R(P.v0), R(P.v1), R(P.v2), R(P.u), R(P.a)};
                 } else {
-                    q3 = fetch<M>(A, ik, jk, k + 3);
+                    q3 = fetch<M>(ap, ik, jk, k + 3);
                 }
                 SideT<R> Lnext, Rnext;
-                recon_cell<M>(cast_wu<R>(A.wu[2]), ho, lim, clip, qa, qb, q3, Lnext, Rnext);
+                recon_cell<M>(cast_wu<R>(L2R2U(ap->wu[2])), ho, lim, clip, qa, qb, q3, Lnext, Rnext);
 #pragma unroll
                 for (int q = 0; q < 5; ++q) Fb[q] = Ft[q];
                 if (own_k) side_flux<M, FLUX, 2, 0, 1>(LpC, Rnext, gR, Ft);
