@@ -7,9 +7,10 @@ from .fvcore import (BCType, Face, FlowSolverException, FluxCalculator, Gasdynam
                      PrecisionMode, flux_calculator_from_name, flux_calculator_name, update_scheme_from_name)
 from .gas import IdealGas, ThermallyPerfectGas
 from .lib import E4hipError, load
-from .fluidblock import SFluidBlock
+from .fluidblock import SFluidBlock, connect_blocks
 from .simcore import GlobalConfig, Simulation
 
 __all__ = ["BCType", "Face", "FlowSolverException", "FluxCalculator", "GasdynamicUpdate", "InterpolateOption",
            "PrecisionMode", "flux_calculator_from_name", "flux_calculator_name", "update_scheme_from_name",
-           "IdealGas", "ThermallyPerfectGas", "E4hipError", "load", "SFluidBlock", "GlobalConfig", "Simulation"]
+           "IdealGas", "ThermallyPerfectGas", "E4hipError", "load", "SFluidBlock", "connect_blocks", "GlobalConfig",
+           "Simulation"]

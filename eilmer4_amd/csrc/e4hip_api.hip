@@ -49,7 +49,10 @@ struct BlockH {
     double* d[2] = {nullptr, nullptr};            // dUdt of stages 1 and 2
     int bc_type[6] = {0, 0, 0, 0, 0, 0};
     Prim bc_fs[6] = {};
-    e4::BCTable* bc_dev = nullptr;                // device copy of bc_type, bc_fs
+    e4::BCTable* bc_dev = nullptr;                // device copy of bc_type, bc_fs and connections
+    int nbr_blk[6] = {-1, -1, -1, -1, -1, -1};    // connected block per face (E4HIP_BC_CONNECTED)
+    int nbr_face[6] = {0, 0, 0, 0, 0, 0};
+    int nbr_orient[6] = {0, 0, 0, 0, 0, 0};
     // thermally perfect gas only: primitive arrays (ping-pong) and face fluxes
     double* P[2] = {nullptr, nullptr};
     double* F[3] = {nullptr, nullptr, nullptr};
@@ -69,6 +72,7 @@ struct State {
     double* last_signal = nullptr;
     int counter_slots = 0;
     std::vector<GasModelH> gases;
+    std::vector<const double*> stage_pin;      // P[0] of every block at the start of the current stage
     std::vector<BlockH> blocks;
     E4hipStepConfig cfg{E4HIP_FLUX_AUSMDV, E4HIP_UPDATE_PC, 2, 1, 1, 1, E4HIP_THERMO_RHOU, E4HIP_PRECISION_FP64, 0, 0};
     bool profiling = false;
@@ -183,6 +187,17 @@ void prof_end(int cls, hipEvent_t a)
     g_state.prof_pending.push_back(ProfEvent{cls, a, b});
 }
 
+e4::Connection connection_of(const BlockH& b, int f)
+{
+    e4::Connection c;
+    memset(&c, 0, sizeof(c));
+    const BlockH* o = get_block(b.nbr_blk[f]);
+    if (b.bc_type[f] != E4HIP_BC_CONNECTED || !o) return c;
+    c.n[0] = o->ni; c.n[1] = o->nj; c.n[2] = (o->dims == 3) ? o->nk : 1;
+    c.face = b.nbr_face[f]; c.orient = b.nbr_orient[f]; c.dims = b.dims; c.qs = o->qs;
+    return c;
+}
+
 int launch_stage(const BlockH& b, int stage, double dt, unsigned int* counters)
 {
     const E4hipStepConfig& cfg = g_state.cfg;
@@ -200,6 +215,12 @@ int launch_stage(const BlockH& b, int stage, double dt, unsigned int* counters)
     A.counters = counters;
     A.gas = g_state.gases[b.gm].ideal;
     A.bc = b.bc_dev;
+    for (int f = 0; f < 6; ++f) {
+        if (b.bc_type[f] != E4HIP_BC_CONNECTED) continue;
+        const BlockH* o = get_block(b.nbr_blk[f]);
+        if (!o) return fail(E4HIP_ERR_HANDLE, "face %d is connected to a destroyed block", f);
+        A.nbrU[f] = o->U[stage - 1];   // the stencil state of stage s is U[s-1] in every block
+    }
     // geometry of the uniform Cartesian cell (the oracle uses the same expressions)
     if (b.dims == 3) {
         A.vol_inv = 1.0 / (b.dx * b.dy * b.dz);
@@ -285,6 +306,13 @@ int launch_tp_stage(BlockH& b, int stage, double dt, unsigned int* counters)
     for (int f = 0; f < 6; ++f) {
         A.bc_type[f] = b.bc_type[f];
         memcpy(A.bc_fs[f], b.bc_fs_tp[f], sizeof(A.bc_fs[f]));
+        if (b.bc_type[f] != E4HIP_BC_CONNECTED) continue;
+        const BlockH* o = get_block(b.nbr_blk[f]);
+        if (!o) return fail(E4HIP_ERR_HANDLE, "face %d is connected to a destroyed block", f);
+        A.nbr[f] = connection_of(b, f);
+        // the other block's primitive state at the start of this stage (its
+        // own stage launch, earlier in the same stage, swaps P[0] and P[1])
+        A.nbrP[f] = g_state.stage_pin[b.nbr_blk[f]];
     }
     if (b.dims == 3) {
         A.vol_inv = 1.0 / (b.dx * b.dy * b.dz);
@@ -382,7 +410,7 @@ int upload_bc(const BlockH& b)
 {
     e4::BCTable t;
     memset(&t, 0, sizeof(t));
-    for (int f = 0; f < 6; ++f) { t.type[f] = b.bc_type[f]; t.fs[f] = b.bc_fs[f]; }
+    for (int f = 0; f < 6; ++f) { t.type[f] = b.bc_type[f]; t.fs[f] = b.bc_fs[f]; t.nbr[f] = connection_of(b, f); }
     HIP_TRY(hipStreamSynchronize(g_state.stream));
     HIP_TRY(hipMemcpy(b.bc_dev, &t, sizeof(t), hipMemcpyHostToDevice));
     return E4HIP_OK;
@@ -604,6 +632,46 @@ int e4hip_block_set_bc(int blk, int face, int type, const double* fs)
         b->bc_fs[face] = P;
     }
     b->bc_type[face] = type;
+    b->nbr_blk[face] = -1;
+    return upload_bc(*b);
+}
+
+int e4hip_block_connect(int blk, int face, int other, int other_face, int orientation)
+{
+    BlockH* b = get_block(blk);
+    BlockH* o = get_block(other);
+    if (!b || !o) return fail(E4HIP_ERR_HANDLE, "unknown block %d or %d", blk, other);
+    const int nf = (b->dims == 3) ? 6 : 4;
+    if (face < 0 || face >= nf || other_face < 0 || other_face >= nf) return fail(E4HIP_ERR_ARG, "bad face");
+    if (b->dims != o->dims) return fail(E4HIP_ERR_ARG, "connected blocks must have the same dimensionality");
+    if (b->gm != o->gm) return fail(E4HIP_ERR_ARG, "connected blocks must share the gas model");
+    if (orientation < 0 || orientation > 3 || (b->dims == 2 && orientation != 0))
+        return fail(E4HIP_ERR_ARG, "bad orientation %d", orientation);
+    // every ghost cell must map into the other block: check the corners of the face
+    const int n[3] = {b->ni, b->nj, (b->dims == 3) ? b->nk : 1};
+    const int on[3] = {o->ni, o->nj, (o->dims == 3) ? o->nk : 1};
+    const int ax0 = (face == E4HIP_FACE_NORTH || face == E4HIP_FACE_SOUTH || face >= E4HIP_FACE_TOP) ? 0 : 1;
+    const int ax1 = (face >= E4HIP_FACE_TOP) ? 1 : 2;
+    const int na = n[ax0], nb = (b->dims == 3) ? n[ax1] : 1;
+    // the face sizes must agree: the mapped face of the other block has as many cells
+    const int oax0 = (other_face == E4HIP_FACE_NORTH || other_face == E4HIP_FACE_SOUTH || other_face >= E4HIP_FACE_TOP) ? 0 : 1;
+    const int oax1 = (other_face >= E4HIP_FACE_TOP) ? 1 : 2;
+    const long cells = (long)na * nb, ocells = (long)on[oax0] * ((b->dims == 3) ? on[oax1] : 1);
+    if (cells != ocells) return fail(E4HIP_ERR_ARG, "connected faces have different cell counts");
+    for (int layer = 1; layer <= 2; ++layer)
+        for (int ca = 0; ca < 2; ++ca)
+            for (int cb = 0; cb < 2; ++cb) {
+                int src[3];
+                e4::connection_source(b->dims, face, other_face, orientation, on, layer, ca ? na - 1 : 0,
+                                      cb ? nb - 1 : 0, src);
+                for (int d = 0; d < 3; ++d)
+                    if (src[d] < 0 || src[d] >= on[d])
+                        return fail(E4HIP_ERR_ARG, "face sizes do not match under orientation %d", orientation);
+            }
+    b->bc_type[face] = E4HIP_BC_CONNECTED;
+    b->nbr_blk[face] = other;
+    b->nbr_face[face] = other_face;
+    b->nbr_orient[face] = orientation;
     return upload_bc(*b);
 }
 
@@ -740,6 +808,9 @@ int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_ce
     hipLaunchKernelGGL(e4::reset_counters_kernel, dim3(1), dim3(1024), 0, g_state.stream, g_state.counters,
                        4 * nslots, nullptr);
     for (int stage = 1; stage <= ns; ++stage) {
+        g_state.stage_pin.assign(g_state.blocks.size(), nullptr);
+        for (size_t h = 0; h < g_state.blocks.size(); ++h)
+            if (g_state.blocks[h].alive) g_state.stage_pin[h] = g_state.blocks[h].P[0];
         for (int n = 0; n < nblk; ++n) {
             BlockH& b = *get_block(blks[n]);
             rc = b.is_tp() ? launch_tp_stage(b, stage, dt, g_state.counters + 4 * (n * 3 + (stage - 1)))

@@ -7,8 +7,58 @@
 
 namespace e4 {
 
-enum { BC_SLIP_WALL = 0, BC_INFLOW = 1, BC_EXTRAPOLATE = 2, BC_PERIODIC = 3 };
+enum { BC_SLIP_WALL = 0, BC_INFLOW = 1, BC_EXTRAPOLATE = 2, BC_PERIODIC = 3, BC_CONNECTED = 4 };
 enum { FACE_NORTH = 0, FACE_EAST = 1, FACE_SOUTH = 2, FACE_WEST = 3, FACE_TOP = 4, FACE_BOTTOM = 5 };
+
+// A face connected to a face of another block (full-face copy): the other
+// block's cell counts, the face, the orientation, the dimensionality and the
+// other block's quantity stride.
+struct Connection {
+    int n[3];
+    int face, orient, dims;
+    long qs;
+};
+
+// The cell of the other block that the ghost at (layer, a, b) of face f copies
+// (full_face_copy.d:142-962).  In-face coordinates (a, b) are the index axes
+// north/south (i, k), east/west (j, k), top/bottom (i, j); 2D faces use a.
+// 2D: the coordinate runs reversed when f and g are both in {south, east} or
+// both in {north, west}.  3D: with x~ = n - 1 - x, orientation o gives
+// (a', b') = (a, b), (b~, a), (a~, b~), (b, a~) for o = 0..3, where o = 1 and 3
+// trade places when f is right-handed (east, south, top: (a x b) along the
+// outward normal), and the first output is reversed when f and g have the same
+// handedness.  Layer L copies the cell at depth L-1 inside face g.
+__host__ __device__ inline void connection_source(int dims, int f, int g, int o, const int* n, int layer, int a, int b,
+                                                  int* src)
+{
+    // in-face axes and normal of face g, without indexed local arrays
+    const bool ns_g = (g == FACE_NORTH || g == FACE_SOUTH), tb_g = (g == FACE_TOP || g == FACE_BOTTOM);
+    const int A0 = (ns_g || tb_g) ? 0 : 1, A1 = tb_g ? 1 : 2, dn = ns_g ? 1 : (tb_g ? 2 : 0);
+    const int nA0 = (A0 == 0) ? n[0] : n[1], nA1 = (A1 == 1) ? n[1] : n[2];
+    const int nD = (dn == 0) ? n[0] : ((dn == 1) ? n[1] : n[2]);
+    const bool plus_g = (g == FACE_NORTH || g == FACE_EAST || g == FACE_TOP);
+    const bool right_f = (f == FACE_EAST || f == FACE_SOUTH || f == FACE_TOP);
+    const bool right_g = (g == FACE_EAST || g == FACE_SOUTH || g == FACE_TOP);
+    const int depth = plus_g ? nD - layer : layer - 1;
+    int p, q = 0;
+    if (dims == 2) {
+        const bool se_f = (f == FACE_SOUTH || f == FACE_EAST), se_g = (g == FACE_SOUTH || g == FACE_EAST);
+        p = (se_f == se_g) ? nA0 - 1 - a : a;
+    } else {
+        if (right_f && (o & 1)) o = (o + 2) & 3;
+        switch (o) {
+        case 0: p = a; q = b; break;
+        case 1: p = nA0 - 1 - b; q = a; break;
+        case 2: p = nA0 - 1 - a; q = nA1 - 1 - b; break;
+        default: p = b; q = nA1 - 1 - a; break;
+        }
+        if (right_f == right_g) p = nA0 - 1 - p;
+    }
+    // (p, q) along (A0, A1), depth along dn; in 2D A1 is k and q = 0
+    src[0] = (dn == 0) ? depth : p;
+    src[1] = (dn == 1) ? depth : ((A0 == 1) ? p : q);
+    src[2] = (dn == 2) ? depth : ((A1 == 2) ? q : 0);
+}
 
 // Ghost-cell effects of the six faces, kept in device memory so that the
 // kernel reads them only where a stencil leaves the block.
@@ -16,6 +66,7 @@ struct BCTable {
     int type[6];
     int pad[2];
     Prim fs[6];                       // inflow ghost states (global frame)
+    Connection nbr[6];                // connected faces
 };
 
 struct StageArgs {
@@ -37,6 +88,7 @@ struct StageArgs {
     L2R2U wu[3];                      // l2r2 weights per direction (uniform grid)
     IdealGas gas;
     const BCTable* bc;
+    const double* nbrU[6];            // stencil-state arrays of the blocks connected to each face
 };
 
 enum { P_RHO = 0, P_VX = 1, P_VY = 2, P_VZ = 3, P_U = 4, P_T = 5, P_P = 6, P_A = 7, P_Y = 8 };
@@ -65,6 +117,8 @@ struct TPArgs {
     const GuessModel* guess;          // learned temperature guess of the decode, or null
     int bc_type[6];
     double bc_fs[6][8 + TP_NSP_MAX];  // ghost flow state of inflow faces: rho,p,T,u,a,vx,vy,vz,Y
+    Connection nbr[6];                // connected faces
+    const double* nbrP[6];            // primitive arrays of the connected blocks at the stage start
 };
 
 // Launchers, one per arithmetic mode; each lives in its own translation unit

@@ -104,6 +104,19 @@ __device__ Prim fetch_ghost(KArgs A, int i, int j, int k)
                                 : (plus ? FACE_TOP : FACE_BOTTOM);
     const int type = A->bc->type[face];
     if (type == BC_INFLOW) return A->bc->fs[face];
+    if (type == BC_CONNECTED) {
+        // full_face_copy.d:1277-1357: the mapped interior cell of the other block
+        const Connection nb = A->bc->nbr[face];
+        const int fa = (face == FACE_NORTH || face == FACE_SOUTH || face >= FACE_TOP) ? i : j;
+        const int fb = (face >= FACE_TOP) ? j : k;
+        int src[3];
+        connection_source(nb.dims, face, nb.face, nb.orient, nb.n, layer, fa, fb, src);
+        const long c = ((long)src[2] * nb.n[1] + src[1]) * nb.n[0] + src[0];
+        const double* p = A->nbrU[face] + c;
+        const long qs = nb.qs;
+        const IdealGas g = A->gas;
+        return decode_ideal<M>(p[0], p[qs], p[2 * qs], p[3 * qs], p[4 * qs], g);
+    }
     int src;
     if (type == BC_EXTRAPOLATE) src = plus ? n - 1 : 0;
     else if (type == BC_PERIODIC) src = plus ? layer - 1 : n - layer;

@@ -66,6 +66,23 @@ __device__ TPCell<NSP> tp_fetch(const TPArgs& A, int i, int j, int k)
         for (int sp = 0; sp < NSP; ++sp) s.Y[sp] = f[8 + sp];
         return s;
     }
+    if (type == BC_CONNECTED) {
+        // full_face_copy.d:1277-1357: the mapped interior cell of the other block
+        const Connection& nb = A.nbr[face];
+        const int fa = (face == FACE_NORTH || face == FACE_SOUTH || face >= FACE_TOP) ? i : j;
+        const int fb = (face >= FACE_TOP) ? j : k;
+        int sc[3];
+        connection_source(nb.dims, face, nb.face, nb.orient, nb.n, layer, fa, fb, sc);
+        const long c = ((long)sc[2] * nb.n[1] + sc[1]) * nb.n[0] + sc[0];
+        const double* P = A.nbrP[face];
+        const long q = nb.qs;
+        TPCell<NSP> s;
+        s.rho = P[P_RHO * q + c]; s.v[0] = P[P_VX * q + c]; s.v[1] = P[P_VY * q + c]; s.v[2] = P[P_VZ * q + c];
+        s.u = P[P_U * q + c]; s.T = P[P_T * q + c]; s.p = P[P_P * q + c]; s.a = P[P_A * q + c];
+#pragma unroll
+        for (int sp = 0; sp < NSP; ++sp) s.Y[sp] = P[(P_Y + sp) * q + c];
+        return s;
+    }
     int src;
     if (type == BC_EXTRAPOLATE) src = plus ? n - 1 : 0;
     else if (type == BC_PERIODIC) src = plus ? layer - 1 : n - layer;

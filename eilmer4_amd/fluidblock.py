@@ -42,6 +42,14 @@ class SFluidBlock:
                                                   _p(fsv) if fsv is not None else None))
         self.bc[Face(face)] = BCType(kind)
 
+    def connect(self, face, other, other_face, orientation=0):
+        """One side of a full-face connection: the ghost cells of `face` copy
+        the mapped interior cells of `other` next to `other_face` at every
+        stage (ExchangeBC_FullFace{otherBlock, otherFace, orientation},
+        prep.lua:681-684; mapping full_face_copy.d:142-962)."""
+        _lib.check(_lib.load().e4hip_block_connect(self.id, int(face), other.id, int(other_face), int(orientation)))
+        self.bc[Face(face)] = BCType.connected
+
     def upload(self, U, Tguess=None):
         U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.nq, self.ncells)
         Tg = None if Tguess is None else np.ascontiguousarray(Tguess, dtype=np.float64).ravel()
@@ -68,3 +76,10 @@ class SFluidBlock:
         if self.id >= 0:
             _lib.check(_lib.load().e4hip_block_destroy(self.id))
             self.id = -1
+
+
+def connect_blocks(blkA, faceA, blkB, faceB, orientation=0):
+    """Full-face connection of two blocks, both sides with the same
+    orientation (connectBlocks, prep.lua:667-684)."""
+    blkA.connect(faceA, blkB, faceB, orientation)
+    blkB.connect(faceB, blkA, faceA, orientation)

@@ -92,6 +92,7 @@ class BCType(IntEnum):
     inflow = 1             # InFlowBC_Supersonic (flow_state_copy)
     extrapolate = 2        # OutFlowBC_SimpleExtrapolate, x_order 0
     periodic = 3           # ExchangeBC_FullFace of a block with itself
+    connected = 4          # ExchangeBC_FullFace to another block (SFluidBlock.connect)
 
 
 class PrecisionMode(IntEnum):

@@ -63,6 +63,8 @@ extern "C" {
 #define E4HIP_BC_EXTRAPOLATE   2  /* OutFlowBC_SimpleExtrapolate x_order 0: extrapolate_copy.d:82- */
 #define E4HIP_BC_PERIODIC      3  /* ExchangeBC_FullFace of a block with itself across
                                      opposite faces: full_face_copy.d:1277-1357 */
+#define E4HIP_BC_CONNECTED     4  /* ExchangeBC_FullFace to another block (set by
+                                     e4hip_block_connect): full_face_copy.d:142-962, 1277-1357 */
 
 /* ---- face names (eilmer/fvcore.d Face enum order) */
 #define E4HIP_FACE_NORTH  0
@@ -162,6 +164,15 @@ int e4hip_block_destroy(int blk);
  * (eilmer/bc/boundary_condition.d:274-292) and the exchange of
  * simcore.d:1599-1668 for a block connected to itself. */
 int e4hip_block_set_bc(int blk, int face, int type, const double* fs);
+/* Full-face connection of face `face` of blk to face `other_face` of block
+ * `other` with orientation 0..3 (0 in 2D): the ghost cells of the face copy
+ * the mapped interior cells of the other block at the start of every stage.
+ * One side of the pair, as one ExchangeBC_FullFace{otherBlock, otherFace,
+ * orientation} (prep.lua:681-684, full_face_copy.d:142-962, exchange
+ * simcore.d:1599-1668); connectBlocks sets both sides with the same
+ * orientation.  The blocks must share the gas model and dimensionality and the
+ * face sizes must match under the orientation.  blk == other is allowed. */
+int e4hip_block_connect(int blk, int face, int other, int other_face, int orientation);
 /* Host -> device copy of U[0] ([nq][ncells]); the flow state is the decode of
  * U (fvcell.d:563-732).  For a thermally perfect gas, Tguess[ncells] is the
  * temperature the Newton solve starts from (the flow state's T on restart);

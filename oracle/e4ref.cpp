@@ -12,9 +12,11 @@
 // two ghost-cell layers; ideal gas and thermally perfect mixtures (NASA-9 / CEA
 // curves); MUSCL l2r2 reconstruction with van Albada limiting and extrema
 // clipping in the face-local frame; AUSMDV and HLLC fluxes; ghost-cell BCs
-// (flowstate copy, extrapolate copy, internal copy then reflect, periodic
-// full-face copy); time derivatives, the RK stage updates, decode, invalid-cell
-// counting and the CFL signal.
+// (flowstate copy, extrapolate copy, internal copy then reflect, full-face
+// copy between blocks in any of the reference's face/orientation pairings, and
+// periodic as a block connected to itself); time derivatives, the RK stage
+// updates, decode, invalid-cell counting and the CFL signal; several connected
+// blocks stepped together.
 
 #include <cmath>
 #include <cstdint>
@@ -35,7 +37,7 @@ enum { UPD_EULER = 0, UPD_PC = 1, UPD_MIDPOINT = 2, UPD_CLASSIC_RK3 = 3, UPD_TVD
 // InterpolateOption (eilmer/fvcore.d:155).
 enum { INTERP_PT = 0, INTERP_RHOU = 1, INTERP_RHOP = 2, INTERP_RHOT = 3 };
 // Boundary kinds (ghost-cell effects, eilmer/bc/ghost_cell_effect/*).
-enum { BC_SLIP_WALL = 0, BC_INFLOW = 1, BC_EXTRAPOLATE = 2, BC_PERIODIC = 3 };
+enum { BC_SLIP_WALL = 0, BC_INFLOW = 1, BC_EXTRAPOLATE = 2, BC_PERIODIC = 3, BC_CONNECTED = 4 };
 // Face order of a cell (eilmer/fvcore.d Face enum): north, east, south, west, top, bottom.
 enum { F_NORTH = 0, F_EAST = 1, F_SOUTH = 2, F_WEST = 3, F_TOP = 4, F_BOTTOM = 5 };
 
@@ -745,7 +747,8 @@ int ref_flux(Gas* g, int calc, const double* L, const double* R, double* F)
 
 // ------------------------------------------------------------------- blocks
 
-struct BC { int type; FlowState fs; };
+struct Block;
+struct BC { int type; FlowState fs; Block* other; int other_face, orientation; };
 
 struct Block {
     Gas* gas;
@@ -957,6 +960,67 @@ void ref_block_get_fs(const Block* b, double* out)
             }
 }
 
+// ------------------------------------------------------- block connections
+//
+// full_face_copy.d:142-962 (set_up_cell_mapping_phase0): the ghost cell at
+// layer L (1, 2) outside face f, at in-face position (a, b), copies the
+// interior cell of the other block at depth L-1 inside face g.  The in-face
+// coordinates of a face are the index axes the reference loops over:
+// north/south (i, k), east/west (j, k), top/bottom (i, j); 2D faces use the
+// first one only.  The reference lists the mapping case by case; it is
+// restated here by the rule the cases follow:
+//   2D: traversing both faces counter-clockwise around their own blocks, the
+//       positions run in opposite senses, so the coordinate is reversed when
+//       f and g are both in {south, east} or both in {north, west};
+//   3D: with x~ = n - 1 - x, orientation o gives (a', b') = (a, b), (b~, a),
+//       (a~, b~), (b, a~) for o = 0..3; the first output is reversed when the
+//       two faces have the same handedness (sign of (a x b) . outward normal:
+//       +1 for east, south, top, -1 for north, west, bottom); and o = 1, 3
+//       trade places when f is right-handed.
+static const int FACE_AX[6][2] = {{0, 2}, {1, 2}, {0, 2}, {1, 2}, {0, 1}, {0, 1}};
+static const int FACE_NORMAL[6] = {1, 0, 1, 0, 2, 2};
+static const int FACE_PLUS[6] = {1, 1, 0, 0, 1, 0};
+static const int FACE_HAND[6] = {-1, 1, 1, -1, 1, -1};
+
+// Source cell (0-based active indices src[3]) in a block of sizes n[3] for the
+// ghost at (layer, a, b) of face f connected to face g with orientation o.
+void ref_connection_source(int dims, int f, int g, int o, const int* n, int layer, int a, int b, int* src)
+{
+    int out0, out1 = 0;
+    if (dims == 2) {
+        const bool grp_f = (f == F_SOUTH || f == F_EAST), grp_g = (g == F_SOUTH || g == F_EAST);
+        const int ax = (g == F_NORTH || g == F_SOUTH) ? 0 : 1;
+        out0 = (grp_f == grp_g) ? n[ax] - 1 - a : a;
+        src[ax] = out0;
+        src[1 - ax] = FACE_PLUS[g] ? n[1 - ax] - layer : layer - 1;
+        src[2] = 0;
+        return;
+    }
+    const int A0 = FACE_AX[g][0], A1 = FACE_AX[g][1];
+    if (FACE_HAND[f] == 1 && (o & 1)) o = (o + 2) % 4;
+    switch (o) {
+    case 0: out0 = a; out1 = b; break;
+    case 1: out0 = n[A0] - 1 - b; out1 = a; break;
+    case 2: out0 = n[A0] - 1 - a; out1 = n[A1] - 1 - b; break;
+    default: out0 = b; out1 = n[A1] - 1 - a; break;
+    }
+    if (FACE_HAND[f] == FACE_HAND[g]) out0 = n[A0] - 1 - out0;
+    src[A0] = out0;
+    src[A1] = out1;
+    const int dn = FACE_NORMAL[g];
+    src[dn] = FACE_PLUS[g] ? n[dn] - layer : layer - 1;
+}
+
+// Connect face `face` of b to face `other_face` of other (one direction; the
+// caller connects both sides, as the reference's connectBlocks does).
+void ref_block_connect(Block* b, int face, Block* other, int other_face, int orientation)
+{
+    b->bc[face].type = BC_CONNECTED;
+    b->bc[face].other = other;
+    b->bc[face].other_face = other_face;
+    b->bc[face].orientation = orientation;
+}
+
 // Ghost-cell effects (bc/ghost_cell_effect): flow_state_copy.d:66-169,
 // extrapolate_copy.d:82-… (x_order 0), internal_copy_then_reflect.d:94-310 with
 // ghost_cell.d:32-43, full_face_copy.d:1277-1357 for a block connected to
@@ -991,6 +1055,19 @@ static void apply_bcs(Block* b)
                         s[dir] = plus ? lo[dir] + layer - 1 : hi[dir] - layer + 1;
                         dest = b->fs[cidx(b, s[0], s[1], s[2])];
                         break;
+                    case BC_CONNECTED: {
+                        // full_face_copy.d:1277-1357, 1633-1646: copy the mapped
+                        // interior flow state of the other block
+                        const Block* o = bc.other;
+                        const int on[3] = {o->ni, o->nj, o->nk};
+                        const int gidx[3] = {g[0] - b->imin, g[1] - b->jmin, g[2] - b->kmin};
+                        const int fa = gidx[FACE_AX[face][0]];
+                        const int fb = (dims == 3) ? gidx[FACE_AX[face][1]] : 0;
+                        int src[3];
+                        ref_connection_source(dims, face, bc.other_face, bc.orientation, on, layer, fa, fb, src);
+                        dest = o->fs[cidx(o, src[0] + o->imin, src[1] + o->jmin, src[2] + o->kmin)];
+                        break;
+                    }
                     case BC_SLIP_WALL:
                     default: {
                         s[dir] = plus ? hi[dir] - layer + 1 : lo[dir] + layer - 1;
@@ -1119,15 +1196,15 @@ static void stage_update(Block* b, int stage, size_t c, double dt)
 // one block.  Returns 0; -1 for a decode that throws (non-positive density or
 // failed thermo); -2 when the invalid-cell count exceeds max_invalid_cells.
 // nbad[stage] receives the per-stage invalid-cell count.
-int ref_block_step(Block* b, double dt, int* nbad)
+// Flux, derivatives, update and decode of one stage of one block whose ghost
+// cells are filled.
+static void block_stage(Block* b, int stage, double dt, int& decode_fail_out, int& invalid_out)
 {
-    const int ns = nstages(b->cfg.update_scheme);
     const int nq = b->nq;
-    for (int stage = 1; stage <= ns; ++stage) {
-        const int ftl = stage - 1;
-        apply_bcs(b);
-        convective_flux(b);
-        int decode_fail = 0, invalid = 0;
+    const int ftl = stage - 1;
+    convective_flux(b);
+    int decode_fail = 0, invalid = 0;
+    {
         #pragma omp parallel for collapse(2) schedule(static) reduction(+:decode_fail, invalid)
         for (int k = b->kmin; k <= b->kmax; ++k)
             for (int j = b->jmin; j <= b->jmax; ++j)
@@ -1138,11 +1215,41 @@ int ref_block_step(Block* b, double dt, int* nbad)
                     if (decode_conserved(b, b->fs[c], &b->U[ftl + 1][c * nq]) != 0) { decode_fail++; continue; }
                     if (!check_data(b, b->fs[c])) invalid++;
                 }
-        if (nbad) nbad[stage - 1] = invalid + decode_fail;
-        if (decode_fail) return -1;
-        if (invalid > b->cfg.max_invalid_cells) return -2;
     }
-    std::swap(b->U[0], b->U[ns]);   // simcore.d:2936-2941
+    decode_fail_out = decode_fail;
+    invalid_out = invalid;
+}
+
+int ref_multi_step(Block** blks, int nblk, double dt, int* nbad);
+
+int ref_block_step(Block* b, double dt, int* nbad)
+{
+    return ref_multi_step(&b, 1, dt, nbad);
+}
+
+// gasdynamic_explicit_increment_with_fixed_grid over several blocks: per stage,
+// every block's ghost cells are filled first (exchange_ghost_cell_boundary_data,
+// simcore.d:1599-1668, then the pre-reconstruction BCs) and then every block is
+// updated.  nbad[stage] receives the invalid-cell count summed over blocks;
+// the invalid-cell limit applies per block.
+int ref_multi_step(Block** blks, int nblk, double dt, int* nbad)
+{
+    const int ns = nstages(blks[0]->cfg.update_scheme);
+    for (int stage = 1; stage <= ns; ++stage) {
+        for (int n = 0; n < nblk; ++n) apply_bcs(blks[n]);
+        int total = 0, any_fail = 0, too_many = 0;
+        for (int n = 0; n < nblk; ++n) {
+            int decode_fail = 0, invalid = 0;
+            block_stage(blks[n], stage, dt, decode_fail, invalid);
+            total += invalid + decode_fail;
+            if (decode_fail) any_fail = 1;
+            if (invalid > blks[n]->cfg.max_invalid_cells) too_many = 1;
+        }
+        if (nbad) nbad[stage - 1] = total;
+        if (any_fail) return -1;
+        if (too_many) return -2;
+    }
+    for (int n = 0; n < nblk; ++n) std::swap(blks[n]->U[0], blks[n]->U[ns]);   // simcore.d:2936-2941
     return 0;
 }
 

@@ -18,7 +18,7 @@ DATA = os.path.join(os.path.dirname(HERE), "eilmer4_amd", "data")
 AUSMDV, HLLC = 0, 9
 EULER, PC, MIDPOINT, CLASSIC_RK3, TVD_RK3, DENMAN_RK3 = range(6)
 INTERP_PT, INTERP_RHOU, INTERP_RHOP, INTERP_RHOT = range(4)
-BC_SLIP_WALL, BC_INFLOW, BC_EXTRAPOLATE, BC_PERIODIC = range(4)
+BC_SLIP_WALL, BC_INFLOW, BC_EXTRAPOLATE, BC_PERIODIC, BC_CONNECTED = range(5)
 NORTH, EAST, SOUTH, WEST, TOP, BOTTOM = range(6)
 
 _lib = None
@@ -63,6 +63,10 @@ def lib():
         L.ref_block_get_U.argtypes = [vp, C.c_int, _dp]
         L.ref_block_get_fs.argtypes = [vp, _dp]
         L.ref_block_step.argtypes = [vp, C.c_double, C.POINTER(C.c_int)]
+        L.ref_multi_step.argtypes = [C.POINTER(vp), C.c_int, C.c_double, C.POINTER(C.c_int)]
+        L.ref_block_connect.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int]
+        _ip = C.POINTER(C.c_int)
+        L.ref_connection_source.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _ip, C.c_int, C.c_int, C.c_int, _ip]
         L.ref_block_dt.argtypes = [vp, C.c_double, C.c_double, C.c_int, _dp]
         L.ref_block_nq.argtypes = [vp]
         _lib = L
@@ -219,6 +223,11 @@ class Block:
         rc = lib().ref_block_step(self.h, dt, nbad)
         return rc, list(nbad)
 
+    def connect(self, face, other, other_face, orientation=0):
+        """One side of a full-face connection (ExchangeBC_FullFace)."""
+        self._peers = getattr(self, "_peers", []) + [other]   # keep the other block alive
+        lib().ref_block_connect(self.h, face, other.h, other_face, orientation)
+
     def determine_dt(self, dt_current, cfl_value=0.5, check_cfl=True):
         out = np.zeros(3)
         lib().ref_block_dt(self.h, dt_current, cfl_value, int(check_cfl), _p(out))
@@ -279,3 +288,26 @@ class Simulation:
             if max_step is not None and self.step_count >= max_step:
                 break
         return self.step_count
+
+
+def connect_blocks(blkA, faceA, blkB, faceB, orientation=0):
+    """connectBlocks (prep.lua:667-684): both sides, same orientation."""
+    blkA.connect(faceA, blkB, faceB, orientation)
+    blkB.connect(faceB, blkA, faceA, orientation)
+
+
+def multi_step(blocks, dt):
+    """One explicit increment of several connected blocks (ghost exchange of
+    all blocks before each stage).  Returns (rc, nbad per stage)."""
+    arr = (C.c_void_p * len(blocks))(*[b.h for b in blocks])
+    nbad = (C.c_int * 3)()
+    rc = lib().ref_multi_step(arr, len(blocks), dt, nbad)
+    return rc, list(nbad)
+
+
+def connection_source(dims, face, other_face, orientation, n, layer, a, b):
+    """Source cell (i, j, k) of the ghost at (layer, a, b) of a connected face."""
+    nn = (C.c_int * 3)(*n)
+    src = (C.c_int * 3)()
+    lib().ref_connection_source(dims, face, other_face, orientation, nn, layer, a, b, src)
+    return tuple(src)
