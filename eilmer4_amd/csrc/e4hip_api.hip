@@ -507,6 +507,16 @@ int e4hip_gas_tp(int nsp, const double* M, int nseg, const double* Tbreaks, cons
         for (int q = 0; q < nseg; ++q)
             for (int j = 0; j < 9; ++j) cv.c[q][j] = coeffs[(s * nseg + q) * 9 + j];
     }
+    // the mixture form of the fast modes needs every species on the same breakpoints
+    g.tp.shared = 1;
+    for (int s = 1; s < nsp; ++s) {
+        const e4::CeaCurve &a = g.tp.curve[0], &b = g.tp.curve[s];
+        for (int q = 0; q <= nseg; ++q) if (a.Tb[q] != b.Tb[q]) g.tp.shared = 0;
+        for (int q = 0; q < nseg - 1; ++q) if (a.Tbl[q] != b.Tbl[q]) g.tp.shared = 0;
+    }
+    for (int q = 0; q < nseg; ++q)
+        for (int k = 0; k < 8; ++k)
+            for (int s = 0; s < nsp; ++s) g.tp.RC[q][k][s] = g.tp.R[s] * g.tp.curve[s].c[q][k];
     HIP_TRY(hipMalloc(&g.tp_dev, sizeof(e4::TPGas)));
     HIP_TRY(hipMemcpy(g.tp_dev, &g.tp, sizeof(e4::TPGas), hipMemcpyHostToDevice));
     g_state.gases.push_back(g);

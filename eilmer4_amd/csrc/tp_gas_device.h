@@ -5,7 +5,12 @@
 // Follows gas/thermo/cea_thermo_curves.d:48-169, therm_perf_gas_mix_eos.d:63-153,
 // nm/newton.d:40-107, perf_gas_mix_eos.d:43-94 and therm_perf_gas.d:177-196,
 // 345-385.  Templated on the arithmetic mode M of gasdyn_device.h (division
-// rule); these functions are always evaluated in FP64.
+// rule); these functions are always evaluated in FP64.  ModeExact evaluates
+// every species' curve as the reference does; the fast modes, when all species
+// share their breakpoints and blend ranges, evaluate the mixture as one curve
+// whose coefficients are the Y_s R_s weighted sums of the species' (the energy
+// and heat capacity are linear in the coefficients), with the reciprocal of T
+// shared by all terms.
 #pragma once
 #include "gasdyn_device.h"
 
@@ -24,8 +29,10 @@ struct CeaCurve {
 
 struct TPGas {
     int nsp;
+    int shared;                                   // all species share nseg, Tb, Tbl
     double R[TP_NSP_MAX];
     CeaCurve curve[TP_NSP_MAX];
+    double RC[TP_NSEG_MAX][8][TP_NSP_MAX];        // R_s a_{s,k} of segment q: RC[q][k][s]
 };
 
 template <int NSP>
@@ -105,10 +112,91 @@ __device__ __forceinline__ double cea_h(const CeaCurve& cv, double T)
     return cea_h_inside<M>(cv, T);
 }
 
+template <int NSP>
+__device__ __forceinline__ double tp_Rmix(const TPGas& g, const double* Y)
+{
+    double Rmix = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) Rmix += Y[s] * g.R[s];
+    return Rmix;
+}
+
+// Mixture coefficients M_k = sum_s Y_s R_s a_{s,k} at T (fast modes, shared
+// breakpoints): the segment choice and the linear blend of cea_coefficients
+// (cea_thermo_curves.d:136-169), applied to the mixture.
+template <int NSP>
+__device__ __forceinline__ void tp_mix_coeffs(const TPGas& g, const double* Y, double T, double Mk[8])
+{
+    const CeaCurve& cv = g.curve[0];
+    const int nb = cv.nseg + 1;
+    int s0 = -1, s1 = -1;
+    double wB = 0.0;
+    if (T < (cv.Tb[1] - 0.5 * cv.Tbl[0])) {
+        s0 = 0;
+    } else if (T > (cv.Tb[nb - 2] + 0.5 * cv.Tbl[nb - 3])) {
+        s0 = cv.nseg - 1;
+    } else {
+        for (int i = 1; i < nb - 1; ++i) {
+            const double lo = cv.Tb[i] - 0.5 * cv.Tbl[i - 1], hi = cv.Tb[i] + 0.5 * cv.Tbl[i - 1];
+            if (T >= lo && T <= hi) { s0 = i - 1; s1 = i; wB = dv_seed(1.0, cv.Tbl[i - 1], rcp_seed(cv.Tbl[i - 1])) * (T - lo); break; }
+            if (T > hi && T < (cv.Tb[i + 1] - 0.5 * cv.Tbl[i])) { s0 = i; break; }
+        }
+        if (s0 < 0) s0 = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        double m = 0.0;
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) m += Y[s] * g.RC[s0][k][s];
+        Mk[k] = m;
+    }
+    if (s1 >= 0) {
+        const double wA = 1.0 - wB;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            double m = 0.0;
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) m += Y[s] * g.RC[s1][k][s];
+            Mk[k] = wA * Mk[k] + wB * m;
+        }
+    }
+}
+
+// Mixture energy u = sum_s Y_s (h_s - R_s T) and Cv = sum_s Y_s (Cp_s - R_s)
+// at T (fast modes, shared breakpoints); outside the curve range the
+// constant-Cp extension of cea_thermo_curves.d:63-91.
+template <int NSP>
+__device__ __forceinline__ void tp_mix_u_cv(const TPGas& g, const double* Y, double Rmix, double T, double* u,
+                                            double* cvmix)
+{
+    const CeaCurve& c0 = g.curve[0];
+    const double Tlo = c0.Tb[0], Thi = c0.Tb[c0.nseg];
+    const double Tc = fmin(fmax(T, Tlo), Thi);
+    double Mk[8];
+    tp_mix_coeffs<NSP>(g, Y, Tc, Mk);
+    const double r = dv_seed(1.0, Tc, rcp_seed(Tc));
+    const double lnT = log(Tc);
+    const double T2 = Tc * Tc, T3 = T2 * Tc, T4 = T2 * T2;
+    double h = -Mk[0] * r + Mk[1] * lnT + Mk[7] +
+               Tc * (Mk[2] + Tc * (Mk[3] * 0.5 + Tc * (Mk[4] * (1.0 / 3.0) + Tc * (Mk[5] * 0.25 + Tc * (Mk[6] * 0.2)))));
+    const double cp = Mk[0] * r * r + Mk[1] * r + Mk[2] + Mk[3] * Tc + Mk[4] * T2 + Mk[5] * T3 + Mk[6] * T4;
+    if (T < Tlo) h -= cp * (Tlo - T);
+    else if (T > Thi) h += cp * (T - Thi);
+    *u = h - Rmix * T;
+    *cvmix = cp - Rmix;
+}
+
 // therm_perf_gas_mix_eos.d:63-69 (mass_average, gas_model.d:330-338)
 template <class M, int NSP>
 __device__ __forceinline__ double tp_energy(const TPGas& g, const double* Y, double T)
 {
+    if constexpr (M::kFast) {
+        if (g.shared) {
+            double u, cv;
+            tp_mix_u_cv<NSP>(g, Y, tp_Rmix<NSP>(g, Y), T, &u, &cv);
+            return u;
+        }
+    }
     double result = 0.0;
 #pragma unroll
     for (int s = 0; s < NSP; ++s) {
@@ -122,6 +210,13 @@ __device__ __forceinline__ double tp_energy(const TPGas& g, const double* Y, dou
 template <class M, int NSP>
 __device__ __forceinline__ double tp_dzdT(const TPGas& g, const double* Y, double T)
 {
+    if constexpr (M::kFast) {
+        if (g.shared) {
+            double u, cv;
+            tp_mix_u_cv<NSP>(g, Y, tp_Rmix<NSP>(g, Y), T, &u, &cv);
+            return -1.0 * cv;
+        }
+    }
     double result = 0.0;
 #pragma unroll
     for (int s = 0; s < NSP; ++s) {
@@ -152,11 +247,29 @@ __device__ int tp_newton(const TPGas& g, TPState<NSP>& Q, double e_tgt, double x
     if (fL == 0.0) { *root = xMin; return 0; }
     if (fH == 0.0) { *root = xMax; return 0; }
     if (fL < 0.0) { xL = xMin; xH = xMax; } else { xH = xMin; xL = xMax; }
+    // f and its derivative at one T (the fast modes evaluate both from one
+    // set of mixture coefficients)
+    auto fdf = [&](double T, double& fv, double& dfv) {
+        if constexpr (M::kFast) {
+            if (g.shared) {
+                double u, cv;
+                tp_mix_u_cv<NSP>(g, Q.Y, tp_Rmix<NSP>(g, Q.Y), T, &u, &cv);
+                Q.T = T;
+                Q.u = u;
+                (*fevals)++;
+                fv = e_tgt - u;
+                dfv = -1.0 * cv;
+                return;
+            }
+        }
+        fv = f(T);
+        dfv = tp_dzdT<M, NSP>(g, Q.Y, T);
+    };
     double rts = x0;
     double dxold = (xMax - xMin);
     double dx = dxold;
-    double f0 = f(rts);
-    double df0 = tp_dzdT<M, NSP>(g, Q.Y, rts);
+    double f0, df0;
+    fdf(rts, f0, df0);
     for (int j = 0; j < MAXIT; ++j) {
         if ((((rts - xH) * df0 - f0) * ((rts - xL) * df0 - f0) > 0.0) || (fabs(2.0 * f0) > fabs(dxold * df0))) {
             dxold = dx;
@@ -171,8 +284,7 @@ __device__ int tp_newton(const TPGas& g, TPState<NSP>& Q, double e_tgt, double x
             if (tmp == rts) { *root = rts; return 0; }
         }
         if (fabs(dx) < tol) { *root = rts; return 0; }
-        f0 = f(rts);
-        df0 = tp_dzdT<M, NSP>(g, Q.Y, rts);
+        fdf(rts, f0, df0);
         if (f0 < 0.0) xL = rts; else xH = rts;
     }
     return -1;
@@ -196,14 +308,7 @@ __device__ int tp_update_temperature(const TPGas& g, TPState<NSP>& Q, int* feval
     return -1;
 }
 
-template <int NSP>
-__device__ __forceinline__ double tp_Rmix(const TPGas& g, const double* Y)
-{
-    double Rmix = 0.0;
-#pragma unroll
-    for (int s = 0; s < NSP; ++s) Rmix += Y[s] * g.R[s];
-    return Rmix;
-}
+
 
 // therm_perf_gas.d:187-191 with perf_gas_mix_eos.d:43-49
 template <class M, int NSP>
@@ -229,6 +334,14 @@ template <class M, int NSP>
 __device__ __forceinline__ double tp_sound_speed(const TPGas& g, const TPState<NSP>& Q)
 {
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    if constexpr (M::kFast) {
+        if (g.shared) {
+            const double R = tp_Rmix<NSP>(g, Q.Y);
+            double u, cv;
+            tp_mix_u_cv<NSP>(g, Q.Y, R, Q.T, &u, &cv);
+            return sqrt(dv<MD>(cv + R, cv) * (R * Q.T));
+        }
+    }
     double Cp = 0.0, Cv = 0.0;
 #pragma unroll
     for (int s = 0; s < NSP; ++s) Cp += Q.Y[s] * cea_Cp<M>(g.curve[s], Q.T);
