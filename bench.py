@@ -75,9 +75,13 @@ def cpu_baseline(n_sample=128, min_seconds=10.0, max_steps=100):
 
 
 def tp_workload(e4, workloads, L, n=128, steps=5, warmup=1):
-    """The 5-species thermally perfect workload (n^3, AUSMDV, pc, plain FP64):
-    the reference decode (bracketed Newton from the stored temperature) and the
-    learned-guess decode, each timed over `steps` steps, and their difference."""
+    """The 5-species thermally perfect workload (n^3, AUSMDV, pc), each variant
+    timed over `steps` steps at one fixed dt from the same state:
+      plain          FP64, the reference's bracketed Newton from the stored T
+      fp64+learned   FP64, decode from the learned guess (polish, fallback)
+      mixed          Newton polish from the cell/stored T at faces and decode
+      mixed+learned  as mixed, decode from the learned guess
+    with each variant's difference from plain after the run."""
     import torch
     from eilmer4_amd import learned_eos
     gas = e4.ThermallyPerfectGas.from_json()
@@ -88,11 +92,14 @@ def tp_workload(e4, workloads, L, n=128, steps=5, warmup=1):
     del fields
     out, finals = {}, {}
     dt_fixed = None
-    for learned in (False, True):
+    variants = [("plain", "fp64", False), ("fp64+learned", "fp64", True), ("mixed", "mixed", False),
+                ("mixed+learned", "mixed", True)]
+    for name, precision, learned in variants:
         blk = e4.SFluidBlock(gas, 3, n, n, n, 1.0 / n, 1.0 / n, 1.0 / n)
         for face, (kind, fs) in bcs.items():
             blk.set_bc(face, kind, fs)
-        sim = e4.Simulation([blk], e4.GlobalConfig(learned_eos_guess=learned))
+        sim = e4.Simulation([blk], e4.GlobalConfig(learned_eos_guess=learned,
+                                                   precision_mode=e4.PrecisionMode[precision]))
         blk.upload(U0, Tg)
         if dt_fixed is None:
             dt_fixed = sim.block_dt(1e-3, False)[0]
@@ -106,20 +113,145 @@ def tp_workload(e4, workloads, L, n=128, steps=5, warmup=1):
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         ev, fb = learned_eos.decode_stats(reset=True)
-        U, fs = blk.download(with_fs=True)
-        finals[learned] = (U, fs)
-        out["learned" if learned else "reference"] = {
+        finals[name] = blk.download(with_fs=True)
+        out[name] = {
             "value": round(n ** 3 * steps / wall / 1e6, 3), "ms_per_step": round(wall * 1e3 / steps, 3),
             "decode_energy_evals_per_cell_stage": round(ev / (n ** 3 * steps * 2), 3),
             "decode_fallback_fraction": round(fb / (n ** 3 * steps * 2), 5)}
         blk.destroy()
-    (Ur, fr), (Ul, fl) = finals[False], finals[True]
-    out["learned_vs_reference"] = {
-        "steps": warmup + steps,
-        "rho_linf_rel": float(np.max(np.abs(Ul[0] - Ur[0])) / np.max(np.abs(Ur[0]))),
-        "T_linf_rel": float(np.max(np.abs(fl[2] - fr[2]) / fr[2]))}
-    out["workload"] = "3D %d^3 5-species thermally perfect air (N2,O2,NO,N,O), T 300-8000 K, ausmdv, pc, FP64" % n
+    Ur, fr = finals["plain"]
+    for name, _, _ in variants[1:]:
+        U, f = finals[name]
+        out[name]["vs_plain"] = {
+            "steps": warmup + steps, "speedup": round(out[name]["value"] / out["plain"]["value"], 3),
+            "rho_linf_rel": float(np.max(np.abs(U[0] - Ur[0])) / np.max(np.abs(Ur[0]))),
+            "T_linf_rel": float(np.max(np.abs(f[2] - fr[2]) / fr[2]))}
+    out["workload"] = "3D %d^3 5-species thermally perfect air (N2,O2,NO,N,O), T 300-8000 K, ausmdv, pc" % n
     out["guess_model"] = "7-64-64-64-1 SiLU, BF16 MFMA inference, trained on 2.16 M device Newton states"
+    return out
+
+
+def c5_workload(e4, workloads, steps=3, warmup=1):
+    """The batched multi-block workload (48 blocks: 32 ideal air, 16
+    five-species air, random face-matched connections in every orientation
+    within each gas class).  Variants, each from the same state at one fixed dt:
+      plain                  FP64, reference Newton, one launch per block
+      fp64+batched           FP64, the ideal blocks of a stage in one launch
+      mixed+learned+batched  mixed precision, learned decode guess, batched
+    with each variant's speed-up over plain and its difference from plain."""
+    import torch
+    from eilmer4_amd import learned_eos
+    ig, tg = e4.IdealGas(), e4.ThermallyPerfectGas.from_json()
+    learned_eos.GuessModel.load().install(tg)
+    t0 = time.perf_counter()
+    descs, conns = workloads.multiblock_c5(ig, tg)
+    ncells = sum(d["ni"] * d["nj"] * d["nk"] for d in descs)
+    log("C5 workload: %d blocks, %.2f M cells, generated in %.1f s" % (len(descs), ncells / 1e6,
+                                                                     time.perf_counter() - t0))
+    variants = [("plain", "fp64", False, False), ("fp64+batched", "fp64", True, False),
+                ("mixed+learned+batched", "mixed", True, True)]
+    out, finals, dt_fixed = {}, {}, None
+    for name, prec, batch, learned in variants:
+        blks = []
+        for d in descs:
+            g = ig if d["cls"] == "ideal" else tg
+            b = e4.SFluidBlock(g, 3, d["ni"], d["nj"], d["nk"], d["dx"], d["dy"], d["dz"])
+            for face, (kind, fs) in d["bcs"].items():
+                b.set_bc(face, kind, fs)
+            blks.append(b)
+        for (a, fa, b, fb, o) in conns:
+            e4.connect_blocks(blks[a], fa, blks[b], fb, o)
+        sim = e4.Simulation(blks, e4.GlobalConfig(precision_mode=e4.PrecisionMode[prec], batch_blocks=batch,
+                                                  learned_eos_guess=learned))
+        for b, d in zip(blks, descs):
+            b.upload(d["U0"], d["Tguess"])
+        if dt_fixed is None:
+            dt_fixed = 0.5 * sim.block_dt(1e-3, False)[0]
+        sim.cfg.fixed_time_step = True
+        sim.state.dt_global = dt_fixed
+        sim.integrate_in_time(target_time=1e30, max_step=warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.integrate_in_time(target_time=1e30, max_step=warmup + steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        finals[name] = [b.download()[0] for b in blks]
+        out[name] = {"value": round(ncells * steps / wall / 1e6, 3), "ms_per_step": round(wall * 1e3 / steps, 3)}
+        for b in blks:
+            b.destroy()
+        log("  C5 %s: %s" % (name, out[name]))
+    for name, _, _, _ in variants[1:]:
+        err = max(float(np.max(np.abs(a - r)) / np.max(np.abs(r))) for a, r in zip(finals[name], finals["plain"]))
+        out[name]["vs_plain"] = {"speedup": round(out[name]["value"] / out["plain"]["value"], 3),
+                                 "rho_linf_rel": err, "steps": warmup + steps}
+    out["workload"] = ("48 blocks (32 ideal air, 16 five-species air), %.2f M cells, sizes %s..%s, %d face-matched "
+                       "connections (orientations 0-3), ausmdv, pc, fixed dt" %
+                       (ncells / 1e6, min((d["ni"], d["nj"], d["nk"]) for d in descs),
+                        max((d["ni"], d["nj"], d["nk"]) for d in descs), len(conns)))
+    return out
+
+
+def shock_position(U, n):
+    """Mean over (j,k) lines of the i-position (cells) of the largest density jump."""
+    rho = U[0].reshape(n, n, n)
+    jump = np.abs(np.diff(rho, axis=2))
+    return float(np.mean(np.argmax(jump, axis=2)) + 0.5)
+
+
+def accuracy_study(e4, workloads, sizes=(256, 384), checkpoints=(1, 100, 1000), seed=20261203):
+    """Mixed precision against plain FP64 on the shocked box (seed 20261203) at
+    each size after 1, 100, 1000 steps, both driven with the plain run's dt
+    sequence: L-infinity differences of density, momentum and total energy
+    relative to each field's maximum, and the shock positions."""
+    import torch
+    gas = e4.IdealGas()
+    out = {}
+    for n in sizes:
+        log("accuracy study %d^3" % n)
+        geom, fields, bcs, _ = workloads.shocked_box_3d(gas, n=n, seed=seed)
+        U0 = workloads.initial_U(gas, fields)
+        del fields
+        runs = {}
+        for prec in ("fp64", "mixed"):
+            blk = e4.SFluidBlock(gas, 3, n, n, n, geom["dx"], geom["dy"], geom["dz"])
+            for face, (kind, fs) in bcs.items():
+                blk.set_bc(face, kind, fs)
+            sim = e4.Simulation([blk], e4.GlobalConfig(precision_mode=e4.PrecisionMode[prec]))
+            blk.upload(U0)
+            runs[prec] = (blk, sim)
+        del U0
+        res = []
+        step = 0
+        t0 = time.perf_counter()
+        for cp in checkpoints:
+            while step < cp:
+                blk64, sim64 = runs["fp64"]
+                sim64.apply_config()
+                sim64.determine_time_step_size()
+                sim64.gasdynamic_explicit_increment_with_fixed_grid()
+                sim64.state.step += 1
+                blkm, simm = runs["mixed"]
+                simm.apply_config()
+                simm.state.dt_global = sim64.state.dt_global
+                simm.gasdynamic_explicit_increment_with_fixed_grid()
+                simm.state.step += 1
+                step += 1
+            torch.cuda.synchronize()
+            Ua = runs["fp64"][0].download()
+            Ub = runs["mixed"][0].download()
+            def rel(q):
+                return float(np.max(np.abs(Ub[q] - Ua[q])) / np.max(np.abs(Ua[q])))
+            xa, xb = shock_position(Ua, n), shock_position(Ub, n)
+            res.append({"steps": step, "rho_linf_rel": rel(0), "momentum_linf_rel": max(rel(1), rel(2), rel(3)),
+                        "energy_linf_rel": rel(4), "shock_x_cells_fp64": round(xa, 4),
+                        "shock_x_cells_mixed": round(xb, 4), "shock_shift_cells": round(xb - xa, 6),
+                        "time": runs["fp64"][1].state.time})
+            log("  %d^3 after %d steps: %s" % (n, step, res[-1]))
+            del Ua, Ub
+        for blk, _ in runs.values():
+            blk.destroy()
+        out["%d^3" % n] = res
+        log("  %d^3 study took %.1f s" % (n, time.perf_counter() - t0))
     return out
 
 
@@ -135,6 +267,9 @@ def main():
                     help="comma list of precision modes; the first is the headline")
     ap.add_argument("--acc-steps", type=int, default=20, help="steps of the mixed-vs-fp64 accuracy check")
     ap.add_argument("--tp-n", type=int, default=128, help="cells per side of the 5-species workload (0: skip)")
+    ap.add_argument("--c5", type=int, default=1, help="run the batched multi-block workload (0: skip)")
+    ap.add_argument("--accuracy-study", action="store_true",
+                    help="only run the mixed-vs-FP64 study (256^3 and 384^3, 1/100/1000 steps)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM-counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -158,6 +293,10 @@ def main():
 
     L = e4.load()
     e4.lib.check(L.e4hip_init(0))
+    if args.accuracy_study:
+        print(json.dumps({"accuracy_study": accuracy_study(e4, workloads), "workload": "shocked box seed 20261203",
+                          "vs": "plain FP64 device run, same dt sequence"}), flush=True)
+        return
     gas = e4.IdealGas()
     log("generating %d^3 shocked-box field" % args.n)
     geom, fields, bcs, _ = workloads.shocked_box_3d(gas, n=args.n)
@@ -269,6 +408,9 @@ def main():
         log("thermally perfect %d^3 workload ..." % args.tp_n)
         out["thermally_perfect"] = tp_workload(e4, workloads, L, n=args.tp_n)
         log("thermally perfect: %s" % out["thermally_perfect"])
+    if args.c5:
+        out["multiblock_c5"] = c5_workload(e4, workloads)
+        log("multi-block: %s" % out["multiblock_c5"])
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline()

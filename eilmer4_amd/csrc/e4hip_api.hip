@@ -63,6 +63,15 @@ struct BlockH {
 
 struct ProfEvent { int cls; hipEvent_t a, b; };
 
+// Staging of the batched stage launches: three slots (one per stage of a step).
+struct BatchBuffers {
+    void* args_dev = nullptr;
+    void* args_host = nullptr;
+    void* map_dev = nullptr;
+    void* map_host = nullptr;
+    size_t args_slot = 0, map_slot = 0;
+};
+
 struct State {
     bool init = false;
     int device = 0;
@@ -73,6 +82,8 @@ struct State {
     int counter_slots = 0;
     std::vector<GasModelH> gases;
     std::vector<const double*> stage_pin;      // P[0] of every block at the start of the current stage
+    BatchBuffers batch;
+    int batch_min_blocks = 2;                  // batch the ideal blocks of a stage from this many on
     std::vector<BlockH> blocks;
     E4hipStepConfig cfg{E4HIP_FLUX_AUSMDV, E4HIP_UPDATE_PC, 2, 1, 1, 1, E4HIP_THERMO_RHOU, E4HIP_PRECISION_FP64, 0, 0};
     bool profiling = false;
@@ -198,16 +209,18 @@ e4::Connection connection_of(const BlockH& b, int f)
     return c;
 }
 
-int launch_stage(const BlockH& b, int stage, double dt, unsigned int* counters)
+// The stage kernel's arguments for one block; kchunk <= 0 picks the
+// single-launch chunk.
+int stage_args_for(const BlockH& b, int stage, double dt, unsigned int* counters, int kchunk, e4::StageArgs& A,
+                   dim3& grid)
 {
     const E4hipStepConfig& cfg = g_state.cfg;
     const int sch = cfg.gasdynamic_update_scheme;
     const int ns = nstages(sch);
-    e4::StageArgs A;
     memset(&A, 0, sizeof(A));
     A.qs = b.qs;
     A.ni = b.ni; A.nj = b.nj; A.nk = (b.dims == 3) ? b.nk : 1;
-    A.kchunk = (b.dims == 3) ? kchunk_for(b) : 1;
+    A.kchunk = (b.dims == 3) ? (kchunk > 0 ? kchunk : kchunk_for(b)) : 1;
     A.stage = stage;
     A.high_order = cfg.interpolation_order > 1;
     A.apply_limiter = cfg.apply_limiter;
@@ -270,7 +283,19 @@ int launch_stage(const BlockH& b, int stage, double dt, unsigned int* counters)
     A.dt = dt; A.g1 = g1; A.g2 = g2; A.g3 = g3;
     A.c1 = dt * g1;
     const int nk = A.nk;
-    dim3 grid((b.ni + TILE_X - 1) / TILE_X, (b.nj + TILE_Y - 1) / TILE_Y, (nk + A.kchunk - 1) / A.kchunk);
+    A.tiles_x = (b.ni + TILE_X - 1) / TILE_X;
+    A.tiles_y = (b.nj + TILE_Y - 1) / TILE_Y;
+    grid = dim3(A.tiles_x, A.tiles_y, (nk + A.kchunk - 1) / A.kchunk);
+    return E4HIP_OK;
+}
+
+int launch_stage(const BlockH& b, int stage, double dt, unsigned int* counters)
+{
+    const E4hipStepConfig& cfg = g_state.cfg;
+    e4::StageArgs A;
+    dim3 grid;
+    int rc = stage_args_for(b, stage, dt, counters, 0, A, grid);
+    if (rc) return rc;
     hipEvent_t ev = nullptr;
     prof_begin(&ev);
     const int flux = (cfg.flux_calculator == E4HIP_FLUX_HLLC) ? e4::FLUX_HLLC : e4::FLUX_AUSMDV;
@@ -278,6 +303,78 @@ int launch_stage(const BlockH& b, int stage, double dt, unsigned int* counters)
     case E4HIP_PRECISION_FP64_EXACT: e4::launch_stage_exact(A, grid, g_state.stream, b.dims, flux); break;
     case E4HIP_PRECISION_MIXED: e4::launch_stage_mixed(A, grid, g_state.stream, b.dims, flux); break;
     default: e4::launch_stage_fast64(A, grid, g_state.stream, b.dims, flux); break;
+    }
+    prof_end(0, ev);
+    HIP_TRY(hipGetLastError());
+    return E4HIP_OK;
+}
+
+// One launch for one stage of several ideal-gas blocks of the same
+// dimensionality: their StageArgs go to a device table (one slot per stage of
+// the step, staged through pinned host memory; the host rewrites a slot only
+// after the previous step's synchronisation), and each workgroup finds its
+// block in a workgroup-to-entry map.  The k-chunk of every block is chosen so
+// that the whole launch has >= 2048 workgroups.
+int launch_stage_batch(const std::vector<const BlockH*>& blks, const std::vector<unsigned int*>& counters,
+                       int stage, double dt)
+{
+    const E4hipStepConfig& cfg = g_state.cfg;
+    const int n = (int)blks.size();
+    long txy = 0;
+    for (const BlockH* b : blks)
+        txy += (long)((b->ni + TILE_X - 1) / TILE_X) * ((b->nj + TILE_Y - 1) / TILE_Y);
+    const long chunks = (2048 + txy - 1) / txy;
+    std::vector<e4::StageArgs> args(n);
+    std::vector<int> map;
+    for (int e = 0; e < n; ++e) {
+        const BlockH& b = *blks[e];
+        int kc = 1;
+        if (b.dims == 3) {
+            kc = (int)((b.nk + chunks - 1) / chunks);
+            if (kc < 8) kc = 8;
+            if (kc > b.nk) kc = b.nk;
+        }
+        dim3 grid;
+        int rc = stage_args_for(b, stage, dt, counters[e], kc, args[e], grid);
+        if (rc) return rc;
+        args[e].wg0 = (int)map.size();
+        const long nwg = (long)grid.x * grid.y * grid.z;
+        map.insert(map.end(), nwg, e);
+    }
+    // grow the staging buffers (no work in flight may read them then)
+    const size_t slot_args = sizeof(e4::StageArgs) * n, slot_map = sizeof(int) * map.size();
+    BatchBuffers& bb = g_state.batch;
+    if (slot_args > bb.args_slot || slot_map > bb.map_slot) {
+        HIP_TRY(hipStreamSynchronize(g_state.stream));
+        if (bb.args_dev) { HIP_TRY(hipFree(bb.args_dev)); HIP_TRY(hipHostFree(bb.args_host)); }
+        if (bb.map_dev) { HIP_TRY(hipFree(bb.map_dev)); HIP_TRY(hipHostFree(bb.map_host)); }
+        bb.args_slot = std::max(slot_args, bb.args_slot) * 2;
+        bb.map_slot = std::max(slot_map, bb.map_slot) * 2;
+        HIP_TRY(hipMalloc(&bb.args_dev, bb.args_slot * 3));
+        HIP_TRY(hipHostMalloc(&bb.args_host, bb.args_slot * 3));
+        HIP_TRY(hipMalloc(&bb.map_dev, bb.map_slot * 3));
+        HIP_TRY(hipHostMalloc(&bb.map_host, bb.map_slot * 3));
+    }
+    char* ah = (char*)bb.args_host + bb.args_slot * (stage - 1);
+    char* ad = (char*)bb.args_dev + bb.args_slot * (stage - 1);
+    char* mh = (char*)bb.map_host + bb.map_slot * (stage - 1);
+    char* md = (char*)bb.map_dev + bb.map_slot * (stage - 1);
+    memcpy(ah, args.data(), slot_args);
+    memcpy(mh, map.data(), slot_map);
+    HIP_TRY(hipMemcpyAsync(ad, ah, slot_args, hipMemcpyHostToDevice, g_state.stream));
+    HIP_TRY(hipMemcpyAsync(md, mh, slot_map, hipMemcpyHostToDevice, g_state.stream));
+    e4::BatchArgs B;
+    B.table = (unsigned long long)(uintptr_t)ad;
+    B.wg_entry = (const int*)md;
+    const bool def = args[0].high_order && args[0].apply_limiter && args[0].extrema_clipping;
+    const int flux = (cfg.flux_calculator == E4HIP_FLUX_HLLC) ? e4::FLUX_HLLC : e4::FLUX_AUSMDV;
+    hipEvent_t ev = nullptr;
+    prof_begin(&ev);
+    const int nwg = (int)map.size();
+    switch (cfg.precision_mode) {
+    case E4HIP_PRECISION_FP64_EXACT: e4::launch_stage_batch_exact(B, nwg, def, g_state.stream, blks[0]->dims, flux); break;
+    case E4HIP_PRECISION_MIXED: e4::launch_stage_batch_mixed(B, nwg, def, g_state.stream, blks[0]->dims, flux); break;
+    default: e4::launch_stage_batch_fast64(B, nwg, def, g_state.stream, blks[0]->dims, flux); break;
     }
     prof_end(0, ev);
     HIP_TRY(hipGetLastError());
@@ -361,6 +458,7 @@ int launch_tp_stage(BlockH& b, int stage, double dt, unsigned int* counters)
     hipEvent_t ev = nullptr;
     prof_begin(&ev);
     if (cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT) e4::launch_tp_stage_exact(A, g_state.stream, flux);
+    else if (cfg.precision_mode == E4HIP_PRECISION_MIXED) e4::launch_tp_stage_mixed(A, g_state.stream, flux);
     else e4::launch_tp_stage_fast64(A, g_state.stream, flux);
     prof_end(0, ev);
     HIP_TRY(hipGetLastError());
@@ -444,6 +542,9 @@ int e4hip_finalize(void)
     for (auto& pe : g_state.prof_pending) { hipEventDestroy(pe.a); hipEventDestroy(pe.b); }
     g_state.prof_pending.clear();
     if (g_state.counters) HIP_TRY(hipFree(g_state.counters));
+    if (g_state.batch.args_dev) { HIP_TRY(hipFree(g_state.batch.args_dev)); HIP_TRY(hipHostFree(g_state.batch.args_host)); }
+    if (g_state.batch.map_dev) { HIP_TRY(hipFree(g_state.batch.map_dev)); HIP_TRY(hipHostFree(g_state.batch.map_host)); }
+    g_state.batch = BatchBuffers();
     if (g_state.smax) HIP_TRY(hipFree(g_state.smax));
     if (g_state.last_signal) HIP_TRY(hipFree(g_state.last_signal));
     g_state.counters = nullptr; g_state.smax = nullptr; g_state.last_signal = nullptr;
@@ -821,11 +922,31 @@ int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_ce
         g_state.stage_pin.assign(g_state.blocks.size(), nullptr);
         for (size_t h = 0; h < g_state.blocks.size(); ++h)
             if (g_state.blocks[h].alive) g_state.stage_pin[h] = g_state.blocks[h].P[0];
+        // ideal-gas blocks: one batched launch per dimensionality when there
+        // are several; thermally perfect blocks: their kernel sequence each
+        std::vector<const BlockH*> batch[2];
+        std::vector<unsigned int*> bcount[2];
         for (int n = 0; n < nblk; ++n) {
             BlockH& b = *get_block(blks[n]);
-            rc = b.is_tp() ? launch_tp_stage(b, stage, dt, g_state.counters + 4 * (n * 3 + (stage - 1)))
-                           : launch_stage(b, stage, dt, g_state.counters + 4 * (n * 3 + (stage - 1)));
-            if (rc) return rc;
+            unsigned int* cnt = g_state.counters + 4 * (n * 3 + (stage - 1));
+            if (b.is_tp()) {
+                rc = launch_tp_stage(b, stage, dt, cnt);
+                if (rc) return rc;
+            } else {
+                batch[b.dims - 2].push_back(&b);
+                bcount[b.dims - 2].push_back(cnt);
+            }
+        }
+        for (int d = 0; d < 2; ++d) {
+            if (g_state.cfg.batch_blocks && (int)batch[d].size() >= g_state.batch_min_blocks) {
+                rc = launch_stage_batch(batch[d], bcount[d], stage, dt);
+                if (rc) return rc;
+            } else {
+                for (size_t e = 0; e < batch[d].size(); ++e) {
+                    rc = launch_stage(*batch[d][e], stage, dt, bcount[d][e]);
+                    if (rc) return rc;
+                }
+            }
         }
     }
     for (int n = 0; n < nblk; ++n) {

@@ -103,12 +103,26 @@ __device__ int tp_polish(const TPGas& g, TPState<NSP>& Q, double e_tgt, double T
 {
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
     if (!(T0 >= 20.0 && T0 <= 100000.0)) return -1;
+    // energy and its T derivative at one temperature (one mixture evaluation
+    // in the fast modes)
+    auto u_dzdT = [&](double T, double& u, double& dz) {
+        if constexpr (M::kFast) {
+            if (g.shared) {
+                double cv;
+                tp_mix_u_cv<NSP>(g, Q.Y, tp_Rmix<NSP>(g, Q.Y), T, &u, &cv);
+                dz = -1.0 * cv;
+                return;
+            }
+        }
+        u = tp_energy<M, NSP>(g, Q.Y, T);
+        dz = tp_dzdT<M, NSP>(g, Q.Y, T);
+    };
     double T = T0, df = 1.0;
     for (int it = 0; it < 2; ++it) {
-        double u = tp_energy<M, NSP>(g, Q.Y, T);
+        double u;
+        u_dzdT(T, u, df);
         (*fevals)++;
         double f = e_tgt - u;
-        df = tp_dzdT<M, NSP>(g, Q.Y, T);
         T -= dv<MD>(f, df);
     }
     if (!(T >= 20.0 && T <= 100000.0)) return -1;

@@ -89,6 +89,16 @@ struct StageArgs {
     IdealGas gas;
     const BCTable* bc;
     const double* nbrU[6];            // stencil-state arrays of the blocks connected to each face
+    int tiles_x, tiles_y;             // workgroup tiles along i and j
+    int wg0;                          // first workgroup of this block in a batched launch
+};
+
+// A batched stage launch: one workgroup per (block, tile, k-chunk) over all
+// blocks of the batch; workgroup w belongs to table entry wg_entry[w].  The
+// table address is an integer so the struct is the same on host and device.
+struct BatchArgs {
+    unsigned long long table;         // const StageArgs[nentries] in device memory
+    const int* wg_entry;
 };
 
 enum { P_RHO = 0, P_VX = 1, P_VY = 2, P_VZ = 3, P_U = 4, P_T = 5, P_P = 6, P_A = 7, P_Y = 8 };
@@ -126,10 +136,16 @@ struct TPArgs {
 void launch_stage_exact(const StageArgs& A, dim3 grid, hipStream_t stream, int dims, int flux);
 void launch_stage_fast64(const StageArgs& A, dim3 grid, hipStream_t stream, int dims, int flux);
 void launch_stage_mixed(const StageArgs& A, dim3 grid, hipStream_t stream, int dims, int flux);
+// batched: nwg workgroups over the blocks of one table (same dims; default
+// reconstruction options or not, for all of them)
+void launch_stage_batch_exact(const BatchArgs& B, int nwg, bool def, hipStream_t stream, int dims, int flux);
+void launch_stage_batch_fast64(const BatchArgs& B, int nwg, bool def, hipStream_t stream, int dims, int flux);
+void launch_stage_batch_mixed(const BatchArgs& B, int nwg, bool def, hipStream_t stream, int dims, int flux);
 // thermally perfect mixture (five species): faces of every direction, then the
 // update; decode of an uploaded state
 void launch_tp_stage_exact(const TPArgs& A, hipStream_t stream, int flux);
 void launch_tp_stage_fast64(const TPArgs& A, hipStream_t stream, int flux);
+void launch_tp_stage_mixed(const TPArgs& A, hipStream_t stream, int flux);
 void launch_tp_decode_exact(const TPArgs& A, const double* U, const double* Tguess, hipStream_t stream);
 void launch_tp_decode_fast64(const TPArgs& A, const double* U, const double* Tguess, hipStream_t stream);
 

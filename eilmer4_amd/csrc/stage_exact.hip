@@ -10,6 +10,11 @@ void launch_stage_exact(const StageArgs& A, dim3 grid, hipStream_t stream, int d
     launch_stage_mode<ModeExact>(A, grid, stream, dims, flux);
 }
 
+void launch_stage_batch_exact(const BatchArgs& B, int nwg, bool def, hipStream_t stream, int dims, int flux)
+{
+    launch_stage_batch_mode<ModeExact>(B, nwg, def, stream, dims, flux);
+}
+
 void launch_tp_stage_exact(const TPArgs& A, hipStream_t stream, int flux)
 {
     const int nfk = (A.dims == 3) ? A.nk : 1;

@@ -15,7 +15,18 @@ void launch_stage_mixed(const StageArgs& A, dim3 grid, hipStream_t stream, int d
     launch_stage_mode<ModeMixed>(A, grid, stream, dims, flux);
 }
 
-void launch_tp_stage_fast64(const TPArgs& A, hipStream_t stream, int flux)
+void launch_stage_batch_fast64(const BatchArgs& B, int nwg, bool def, hipStream_t stream, int dims, int flux)
+{
+    launch_stage_batch_mode<ModeFast64>(B, nwg, def, stream, dims, flux);
+}
+
+void launch_stage_batch_mixed(const BatchArgs& B, int nwg, bool def, hipStream_t stream, int dims, int flux)
+{
+    launch_stage_batch_mode<ModeMixed>(B, nwg, def, stream, dims, flux);
+}
+
+template <class M>
+static void launch_tp_stage_mode(const TPArgs& A, hipStream_t stream, int flux)
 {
     const int nfk = (A.dims == 3) ? A.nk : 1;
     const long nf0 = (long)(A.ni + 1) * A.nj * nfk, nf1 = (long)A.ni * (A.nj + 1) * nfk;
@@ -23,15 +34,25 @@ void launch_tp_stage_fast64(const TPArgs& A, hipStream_t stream, int flux)
     const dim3 b(128);
     auto g = [](long n) { return dim3((unsigned)((n + 127) / 128)); };
     if (flux == FLUX_HLLC) {
-        hipLaunchKernelGGL((tp_face_kernel<ModeFast64, 5, FLUX_HLLC, 0>), g(nf0), b, 0, stream, A);
-        hipLaunchKernelGGL((tp_face_kernel<ModeFast64, 5, FLUX_HLLC, 1>), g(nf1), b, 0, stream, A);
-        if (A.dims == 3) hipLaunchKernelGGL((tp_face_kernel<ModeFast64, 5, FLUX_HLLC, 2>), g(nf2), b, 0, stream, A);
+        hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX_HLLC, 0>), g(nf0), b, 0, stream, A);
+        hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX_HLLC, 1>), g(nf1), b, 0, stream, A);
+        if (A.dims == 3) hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX_HLLC, 2>), g(nf2), b, 0, stream, A);
     } else {
-        hipLaunchKernelGGL((tp_face_kernel<ModeFast64, 5, FLUX_AUSMDV, 0>), g(nf0), b, 0, stream, A);
-        hipLaunchKernelGGL((tp_face_kernel<ModeFast64, 5, FLUX_AUSMDV, 1>), g(nf1), b, 0, stream, A);
-        if (A.dims == 3) hipLaunchKernelGGL((tp_face_kernel<ModeFast64, 5, FLUX_AUSMDV, 2>), g(nf2), b, 0, stream, A);
+        hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX_AUSMDV, 0>), g(nf0), b, 0, stream, A);
+        hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX_AUSMDV, 1>), g(nf1), b, 0, stream, A);
+        if (A.dims == 3) hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX_AUSMDV, 2>), g(nf2), b, 0, stream, A);
     }
-    hipLaunchKernelGGL((tp_update_kernel<ModeFast64, 5>), g(nc), b, 0, stream, A);
+    hipLaunchKernelGGL((tp_update_kernel<M, 5>), g(nc), b, 0, stream, A);
+}
+
+void launch_tp_stage_fast64(const TPArgs& A, hipStream_t stream, int flux)
+{
+    launch_tp_stage_mode<ModeFast64>(A, stream, flux);
+}
+
+void launch_tp_stage_mixed(const TPArgs& A, hipStream_t stream, int flux)
+{
+    launch_tp_stage_mode<ModeMixed>(A, stream, flux);
 }
 
 void launch_tp_decode_fast64(const TPArgs& A, const double* U, const double* Tguess, hipStream_t stream)

@@ -51,8 +51,12 @@ __device__ __forceinline__ int opaque_i(int x)
 // registers across the loop, which overflows the scalar register file.
 // (The host pass of the compiler only parses the kernel; it sees a plain
 // pointer.)
+// A batched launch's kernel argument is a BatchArgs; each workgroup reads its
+// block's StageArgs from the table through the same kind of pointer (a
+// read-only, uniform address, so scalar loads).
 #if defined(__HIP_DEVICE_COMPILE__)
 using KArgs = const __attribute__((address_space(4))) StageArgs*;
+using KBatch = const __attribute__((address_space(4))) BatchArgs*;
 
 __device__ __forceinline__ KArgs kargs()
 {
@@ -60,9 +64,26 @@ __device__ __forceinline__ KArgs kargs()
     asm volatile("" : "+s"(p));
     return p;
 }
+
+__device__ __forceinline__ KArgs kargs_entry(int e)
+{
+    KBatch b = (KBatch)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(b));
+    KArgs p = (KArgs)(b->table + (unsigned long long)e * sizeof(StageArgs));
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
+__device__ __forceinline__ int batch_entry()
+{
+    KBatch b = (KBatch)__builtin_amdgcn_kernarg_segment_ptr();
+    return __builtin_amdgcn_readfirstlane(b->wg_entry[blockIdx.x]);
+}
 #else
 using KArgs = const StageArgs*;
 __device__ __forceinline__ KArgs kargs() { return nullptr; }
+__device__ __forceinline__ KArgs kargs_entry(int) { return nullptr; }
+__device__ __forceinline__ int batch_entry() { return 0; }
 #endif
 
 // Decode of the cell (i,j,k); positions one or two cells outside the block in
@@ -309,10 +330,13 @@ template <> struct StageOccupancy<ModeMixed> { static constexpr int waves = E4_S
 
 // DEF: the default reconstruction (order 2, limiter, extrema clipping) fixed
 // at compile time; otherwise the three options are read from the arguments.
-template <class M, int TX, int TY, int DIMS, int FLUX, bool DEF>
+template <class M, int TX, int TY, int DIMS, int FLUX, bool DEF, bool BATCH>
 __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(StageOccupancy<M>::waves)))
-void stage_kernel(StageArgs A)
+void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type A)
 {
+    // the arguments of this workgroup's block, re-read per phase
+    const int entry = BATCH ? batch_entry() : 0;
+    auto args = [&]() { return BATCH ? kargs_entry(entry) : kargs(); };
     using R = typename M::R;
     constexpr int NT = TX * TY;
     constexpr int PX = TX + 4, PY = TY + 4, PLANE = PX * PY;
@@ -329,13 +353,20 @@ void stage_kernel(StageArgs A)
     __shared__ R sLw[6 * TY];                // left states of the tile's west faces
     __shared__ R sLs[6 * TX];                // left states of the tile's south faces
 
-    KArgs ap = kargs();
+    KArgs ap = args();
+    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    if constexpr (BATCH) {
+        const int w = (int)blockIdx.x - ap->wg0;
+        bx = w % ap->tiles_x;
+        by = (w / ap->tiles_x) % ap->tiles_y;
+        bz = w / (ap->tiles_x * ap->tiles_y);
+    }
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
-    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const int i0 = bx * TX, j0 = by * TY;
     const int i = i0 + tx, j = j0 + ty;
     const bool own_ij = (i < ap->ni) && (j < ap->nj);
-    const int k0 = (DIMS == 3) ? blockIdx.z * ap->kchunk : 0;
+    const int k0 = (DIMS == 3) ? bz * ap->kchunk : 0;
     const int k1 = (DIMS == 3) ? min(k0 + ap->kchunk, ap->nk) : 1;
     const bool ho = DEF || ap->high_order != 0, lim = DEF || ap->apply_limiter != 0;
     const bool clip = DEF || ap->extrema_clipping != 0;
@@ -367,7 +398,7 @@ void stage_kernel(StageArgs A)
     for (int k = k0; k < k1; ++k) {
         // per-thread offsets are recomputed in every iteration rather than
         // held in registers across the loop
-        ap = kargs();
+        ap = args();
         const int tidk = opaque_i(tid);
         const int txk = tidk % TX, tyk = tidk / TX;
         const int ik = i0 + txk, jk = j0 + tyk;
@@ -408,7 +439,7 @@ void stage_kernel(StageArgs A)
         __syncthreads();
         // 2. face states: own cell in i and j (right states to LDS, left states
         // kept) and the halo cells' states the tile's boundary faces need.
-        ap = kargs();
+        ap = args();
         const L2R2UT<R> wi = cast_wu<R>(L2R2U(ap->wu[0])), wj = cast_wu<R>(L2R2U(ap->wu[1]));
         {
             const int e = tidk - (NT - NHALO);
@@ -445,7 +476,7 @@ void stage_kernel(StageArgs A)
         // 3. fluxes of the own cell's plus faces and of the tile's west and
         // south faces; each face's flux overwrites its right state in LDS
         // (the same thread read it).
-        ap = kargs();
+        ap = args();
         const IdealGasT<R> gR = cast_gas<R>(IdealGas(ap->gas));
         if (own_k) {
             R Fx[5];
@@ -474,7 +505,7 @@ void stage_kernel(StageArgs A)
         __syncthreads();
         // 4. flux divergence in face order N, E, S, W, T, B (fvcell.d:789-825),
         // stage update, decode check, write; all in FP64.
-        ap = kargs();
+        ap = args();
         if (own_k) {
             const int iW = tyk * (TX + 1) + txk, jS = tyk * TX + txk;
             const int iE = iW + 1, jN = jS + TX;
@@ -547,7 +578,7 @@ void stage_kernel(StageArgs A)
         // flux of the top face of cell k+1
         if constexpr (DIMS == 3) {
             if (k + 1 < k1) {
-                ap = kargs();
+                ap = args();
                 const IdealGasT<R> gR = cast_gas<R>(IdealGas(ap->gas));
                 PrimT<R> q3;
                 if (raw_ok) {
@@ -573,9 +604,29 @@ void launch_stage_df(const StageArgs& A, dim3 grid, hipStream_t stream)
 {
     const dim3 block(TILE_X * TILE_Y);
     if (A.high_order && A.apply_limiter && A.extrema_clipping)
-        hipLaunchKernelGGL((stage_kernel<M, TILE_X, TILE_Y, DIMS, FLUX, true>), grid, block, 0, stream, A);
+        hipLaunchKernelGGL((stage_kernel<M, TILE_X, TILE_Y, DIMS, FLUX, true, false>), grid, block, 0, stream, A);
     else
-        hipLaunchKernelGGL((stage_kernel<M, TILE_X, TILE_Y, DIMS, FLUX, false>), grid, block, 0, stream, A);
+        hipLaunchKernelGGL((stage_kernel<M, TILE_X, TILE_Y, DIMS, FLUX, false, false>), grid, block, 0, stream, A);
+}
+
+template <class M, int DIMS, int FLUX>
+void launch_stage_batch_df(const BatchArgs& B, int nwg, bool def, hipStream_t stream)
+{
+    const dim3 block(TILE_X * TILE_Y), grid(nwg);
+    if (def) hipLaunchKernelGGL((stage_kernel<M, TILE_X, TILE_Y, DIMS, FLUX, true, true>), grid, block, 0, stream, B);
+    else hipLaunchKernelGGL((stage_kernel<M, TILE_X, TILE_Y, DIMS, FLUX, false, true>), grid, block, 0, stream, B);
+}
+
+template <class M>
+void launch_stage_batch_mode(const BatchArgs& B, int nwg, bool def, hipStream_t stream, int dims, int flux)
+{
+    if (dims == 3) {
+        if (flux == FLUX_HLLC) launch_stage_batch_df<M, 3, FLUX_HLLC>(B, nwg, def, stream);
+        else launch_stage_batch_df<M, 3, FLUX_AUSMDV>(B, nwg, def, stream);
+    } else {
+        if (flux == FLUX_HLLC) launch_stage_batch_df<M, 2, FLUX_HLLC>(B, nwg, def, stream);
+        else launch_stage_batch_df<M, 2, FLUX_AUSMDV>(B, nwg, def, stream);
+    }
 }
 
 // Launch of the stage kernel of mode M for a block of dims dimensions.

@@ -20,6 +20,29 @@
 
 namespace e4 {
 
+// Modes of the thermally perfect path: ModeExact and ModeFast64 as for the
+// ideal gas; ModeMixed keeps the FP64 arithmetic of ModeFast64 and replaces
+// the bracketed Newton solves by the polish of learned_eos.h (Newton from the
+// cell's temperature, accepted by its energy residual, else the reference's
+// bracketed solve): the face EOS starts from the face's cell, the decode from
+// the stored temperature (or the learned guess when one is installed).
+template <class M> struct TPArith { using type = M; static constexpr bool polish = false; };
+template <> struct TPArith<ModeMixed> { using type = ModeFast64; static constexpr bool polish = true; };
+
+// update_thermo_from_rhou of a face state (therm_perf_gas.d:187-191); with
+// polish, Newton from Q.T first.
+template <class M, int NSP, bool POLISH>
+__device__ __forceinline__ int tp_face_eos(const TPGas& g, TPState<NSP>& Q, int* fevals)
+{
+    if constexpr (POLISH) {
+        if (tp_polish<M, NSP>(g, Q, Q.u, Q.T, fevals) == 0) {
+            Q.p = Q.rho * tp_Rmix<NSP>(g, Q.Y) * Q.T;
+            return 0;
+        }
+    }
+    return tp_update_thermo_from_rhou<M, NSP>(g, Q, fevals);
+}
+
 template <int NSP>
 struct TPCell {
     double rho, p, T, u, a;
@@ -207,6 +230,8 @@ __device__ __forceinline__ void tp_hllc(const TPCell<NSP>& L, const TPCell<NSP>&
 template <class M, int NSP, int FLUX, int DIR>
 __global__ __launch_bounds__(128) void tp_face_kernel(TPArgs A)
 {
+    using MA = typename TPArith<M>::type;
+    constexpr bool kPolish = TPArith<M>::polish;
     const int nfi = A.ni + (DIR == 0), nfj = A.nj + (DIR == 1), nfk = (A.dims == 3) ? A.nk + (DIR == 2) : 1;
     const long nf = (long)nfi * nfj * nfk;
     const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -233,36 +258,36 @@ __global__ __launch_bounds__(128) void tp_face_kernel(TPArgs A)
         const bool lim = A.apply_limiter, clip = A.extrema_clipping;
 #pragma unroll
         for (int d = 0; d < 3; ++d)
-            interp_l2r2_scalar<M>(w, lim, clip, c[0].v[d], c[1].v[d], c[2].v[d], c[3].v[d], Lf.v[d], Rf.v[d]);
+            interp_l2r2_scalar<MA>(w, lim, clip, c[0].v[d], c[1].v[d], c[2].v[d], c[3].v[d], Lf.v[d], Rf.v[d]);
 #pragma unroll
         for (int s = 0; s < NSP; ++s)
-            interp_l2r2_scalar<M>(w, lim, clip, c[0].Y[s], c[1].Y[s], c[2].Y[s], c[3].Y[s], Lf.Y[s], Rf.Y[s]);
-        if (scale_mass_fractions<M, NSP>(Lf.Y) != 0) {
+            interp_l2r2_scalar<MA>(w, lim, clip, c[0].Y[s], c[1].Y[s], c[2].Y[s], c[3].Y[s], Lf.Y[s], Rf.Y[s]);
+        if (scale_mass_fractions<MA, NSP>(Lf.Y) != 0) {
 #pragma unroll
             for (int s = 0; s < NSP; ++s) Lf.Y[s] = c[1].Y[s];
         }
-        if (scale_mass_fractions<M, NSP>(Rf.Y) != 0) {
+        if (scale_mass_fractions<MA, NSP>(Rf.Y) != 0) {
 #pragma unroll
             for (int s = 0; s < NSP; ++s) Rf.Y[s] = c[2].Y[s];
         }
-        interp_l2r2_scalar<M>(w, lim, clip, c[0].rho, c[1].rho, c[2].rho, c[3].rho, Lf.rho, Rf.rho);
-        interp_l2r2_scalar<M>(w, lim, clip, c[0].u, c[1].u, c[2].u, c[3].u, Lf.u, Rf.u);
+        interp_l2r2_scalar<MA>(w, lim, clip, c[0].rho, c[1].rho, c[2].rho, c[3].rho, Lf.rho, Rf.rho);
+        interp_l2r2_scalar<MA>(w, lim, clip, c[0].u, c[1].u, c[2].u, c[3].u, Lf.u, Rf.u);
         // update_thermo_from_rhou on each side; on failure the side is the cell's state
         TPState<NSP> Q;
         Q.rho = Lf.rho; Q.T = Lf.T; Q.u = Lf.u;
 #pragma unroll
         for (int s = 0; s < NSP; ++s) Q.Y[s] = Lf.Y[s];
-        if (tp_update_thermo_from_rhou<M, NSP>(*A.gas, Q, &fe) == 0) { Lf.T = Q.T; Lf.u = Q.u; Lf.p = Q.p; }
+        if (tp_face_eos<MA, NSP, kPolish>(*A.gas, Q, &fe) == 0) { Lf.T = Q.T; Lf.u = Q.u; Lf.p = Q.p; }
         else Lf = c[1];
         Q.rho = Rf.rho; Q.T = Rf.T; Q.u = Rf.u;
 #pragma unroll
         for (int s = 0; s < NSP; ++s) Q.Y[s] = Rf.Y[s];
-        if (tp_update_thermo_from_rhou<M, NSP>(*A.gas, Q, &fe) == 0) { Rf.T = Q.T; Rf.u = Q.u; Rf.p = Q.p; }
+        if (tp_face_eos<MA, NSP, kPolish>(*A.gas, Q, &fe) == 0) { Rf.T = Q.T; Rf.u = Q.u; Rf.p = Q.p; }
         else Rf = c[2];
     }
     double F[5 + NSP];
-    if constexpr (FLUX == FLUX_HLLC) tp_hllc<M, NSP>(Lf, Rf, F);
-    else tp_ausmdv<M, NSP>(Lf, Rf, F);
+    if constexpr (FLUX == FLUX_HLLC) tp_hllc<MA, NSP>(Lf, Rf, F);
+    else tp_ausmdv<MA, NSP>(Lf, Rf, F);
     double* dst = A.F[DIR];
     const long fq = A.fqs[DIR];
     dst[f] = F[0];
@@ -277,6 +302,8 @@ __global__ __launch_bounds__(128) void tp_face_kernel(TPArgs A)
 template <class M, int NSP>
 __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
 {
+    using MA = typename TPArith<M>::type;
+    constexpr bool kPolish = TPArith<M>::polish;
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
     constexpr int NQ = 5 + NSP;
     __shared__ __bf16 gm_lds[2 * GM_LDS_PER_WAVE];
@@ -340,7 +367,7 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
             Q.u = u;
 #pragma unroll
             for (int s = 0; s < NSP; ++s) Q.Y[s] = Un[5 + s] * dinv;
-            if (scale_mass_fractions<M, NSP>(Q.Y) != 0) decode_fail = true;
+            if (scale_mass_fractions<MA, NSP>(Q.Y) != 0) decode_fail = true;
             else have_state = true;
         }
     }
@@ -356,7 +383,7 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
         const double e_tgt = Q.u;
         int rc;
         if (A.guess) {
-            rc = tp_polish<M, NSP>(*A.gas, Q, e_tgt, Tguess, &fe);
+            rc = tp_polish<MA, NSP>(*A.gas, Q, e_tgt, Tguess, &fe);
             if (rc == 0) {
                 double Rmix = tp_Rmix<NSP>(*A.gas, Q.Y);
                 Q.p = Q.rho * Rmix * Q.T;
@@ -364,23 +391,23 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
                 fell_back = true;
                 Q.u = e_tgt;
                 Q.T = A.Pin[P_T * qs + c];
-                rc = tp_update_thermo_from_rhou<M, NSP>(*A.gas, Q, &fe);
+                rc = tp_update_thermo_from_rhou<MA, NSP>(*A.gas, Q, &fe);
             }
         } else {
             Q.T = A.Pin[P_T * qs + c];     // the previous temperature is the Newton guess
-            rc = tp_update_thermo_from_rhou<M, NSP>(*A.gas, Q, &fe);
+            rc = tp_face_eos<MA, NSP, kPolish>(*A.gas, Q, &fe);
         }
         if (rc != 0) {
             // ignore_low_T_thermo_update_failure: T = 200 K and re-encode U
             Q.T = 200.0;
-            tp_update_thermo_from_rhoT<M, NSP>(*A.gas, Q);
+            tp_update_thermo_from_rhoT<MA, NSP>(*A.gas, Q);
             Un[1] = rho * v[0]; Un[2] = rho * v[1]; Un[3] = rho * v[2];
             double ke2 = 0.5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
             Un[4] = rho * (Q.u + ke2);
 #pragma unroll
             for (int s = 0; s < NSP; ++s) Un[5 + s] = rho * Q.Y[s];
         }
-        Q.a = tp_sound_speed<M, NSP>(*A.gas, Q);
+        Q.a = tp_sound_speed<MA, NSP>(*A.gas, Q);
         // check_data (flowstate.d:345-388)
         double fsum = 0.0;
 #pragma unroll

@@ -37,7 +37,7 @@ class StepConfig(C.Structure):
                 ("interpolation_order", C.c_int), ("apply_limiter", C.c_int),
                 ("extrema_clipping", C.c_int), ("interpolate_in_local_frame", C.c_int),
                 ("thermo_interpolator", C.c_int), ("precision_mode", C.c_int),
-                ("max_invalid_cells", C.c_int), ("learned_eos_guess", C.c_int)]
+                ("max_invalid_cells", C.c_int), ("learned_eos_guess", C.c_int), ("batch_blocks", C.c_int)]
 
 
 _lib = None

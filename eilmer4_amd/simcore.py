@@ -33,6 +33,7 @@ class GlobalConfig:
     precision_mode: int = PrecisionMode.fp64
     max_invalid_cells: int = 0
     learned_eos_guess: bool = False
+    batch_blocks: bool = True
     cfl_value: float = 0.5
     cfl_count: int = 10
     dt_init: float = 1.0e-3
@@ -49,7 +50,7 @@ class GlobalConfig:
                                int(self.interpolation_order), int(bool(self.apply_limiter)),
                                int(bool(self.extrema_clipping)), int(bool(self.interpolate_in_local_frame)),
                                int(self.thermo_interpolator), int(self.precision_mode), int(self.max_invalid_cells),
-                               int(bool(self.learned_eos_guess)))
+                               int(bool(self.learned_eos_guess)), int(bool(self.batch_blocks)))
 
 
 class SimState:

@@ -147,3 +147,116 @@ def initial_U_tp(gas, fields):
     u = gas.energy(T, Y)
     U = gas.encode(fields["rho"].ravel(), u, fields["vx"], fields["vy"], fields["vz"], Y)
     return U, T.copy()
+
+
+SEED_MULTIBLOCK = 20261205
+_IN_FACE = {0: (0, 2), 1: (1, 2), 2: (0, 2), 3: (1, 2), 4: (0, 1), 5: (0, 1)}   # face -> in-face axes (i,j,k)
+_NORMAL = {0: 1, 1: 0, 2: 1, 3: 0, 4: 2, 5: 2}
+
+
+def _smooth_block(shape, rng, nmodes=3, kmax=3):
+    """Smooth field on a (nk, nj, ni) block scaled to [-1, 1]: a sum of
+    separable products of short random Fourier series in x, y, z."""
+    nk, nj, ni = shape
+    f = np.zeros(shape)
+    for _ in range(nmodes):
+        parts = []
+        for n in (nk, nj, ni):
+            x = (np.arange(n) + 0.5) / n
+            g = np.zeros(n)
+            for k in range(1, kmax + 1):
+                g += rng.normal() / k * np.cos(2.0 * np.pi * k * x + rng.uniform(0, 2.0 * np.pi))
+            parts.append(g)
+        f += parts[0][:, None, None] * parts[1][None, :, None] * parts[2][None, None, :]
+    f -= f.min()
+    return 2.0 * f / max(f.max(), 1e-300) - 1.0
+
+
+def _block_tree(rng, nblocks, base_lo=(1, 1, 1), base_hi=(10, 6, 4)):
+    """Random tree of face-matched connections: block sizes in units of 8
+    cells, each new block joined by a random free face and orientation to a
+    random free face of an earlier block, its in-face sizes taken from that
+    face (swapped for odd orientations), its normal size drawn log-uniform."""
+    def logu(lo, hi):
+        return int(round(np.exp(rng.uniform(np.log(lo), np.log(hi + 0.999)) - 0.0)))
+    dims = [[max(base_lo[a], min(base_hi[a], logu(base_lo[a], base_hi[a]))) for a in range(3)]]
+    free = [set(range(6))]
+    conns = []
+    while len(dims) < nblocks:
+        A = int(rng.choice([b for b in range(len(dims)) if free[b]]))
+        fa = int(rng.choice(sorted(free[A])))
+        fb = int(rng.integers(6))
+        o = int(rng.integers(4))
+        na = [dims[A][_IN_FACE[fa][0]], dims[A][_IN_FACE[fa][1]]]
+        if o % 2 == 1:
+            na = na[::-1]
+        nb = [0, 0, 0]
+        nb[_IN_FACE[fb][0]], nb[_IN_FACE[fb][1]] = na
+        ax = _NORMAL[fb]
+        nb[ax] = max(base_lo[ax], min(base_hi[ax], logu(base_lo[ax], base_hi[ax])))
+        dims.append(nb)
+        free.append(set(range(6)) - {fb})
+        free[A].discard(fa)
+        conns.append((A, fa, len(dims) - 1, fb, o))
+    return dims, conns, free
+
+
+def multiblock_c5(ideal_gas, tp_gas, seed=SEED_MULTIBLOCK, n_ideal=32, n_tp=16, target_cells=40.0e6):
+    """48 uniform Cartesian blocks (32 ideal air, 16 five-species air), each
+    class a random tree of face-matched full-face connections in all
+    orientations (no connection crosses gas classes).  Base sizes are
+    log-uniform in 8..80 x 8..48 x 8..32 cells (units of 8), scaled by one
+    integer factor so that the total is near target_cells (factor 2 gives the
+    16^3 .. 160x96x64 range).  Free faces: one supersonic inflow in the ideal
+    class, otherwise extrapolated outflow or slip walls.  Fields are smooth per
+    block (ideal: p 70-130 kPa, T 210-390 K, |v| <= 100 m/s; mixture: T
+    300-8000 K, rho 0.01-1 kg/m^3, Dirichlet composition as in
+    thermally_perfect_box_3d).  Returns a list of block descriptions
+    (gas class, sizes, cell size, BCs, U0, Tguess) and the connections
+    (block a, face a, block b, face b, orientation) over the global list."""
+    rng = np.random.default_rng(seed)
+    di, ci, fi = _block_tree(rng, n_ideal)
+    dt_, ct, ft = _block_tree(rng, n_tp)
+    base = sum(int(np.prod(d)) for d in di + dt_) * 512
+    m = max(1, int(round((target_cells / base) ** (1.0 / 3.0))))
+    blocks, conns = [], []
+    for cls, dims, cn, free in (("ideal", di, ci, fi), ("tp", dt_, ct, ft)):
+        off = len(blocks)
+        for b, (ui, uj, uk) in enumerate(dims):
+            ni, nj, nk = 8 * m * ui, 8 * m * uj, 8 * m * uk
+            shape = (nk, nj, ni)
+            bcs = {}
+            for f in sorted(free[b]):
+                bcs[Face(f)] = (BCType.extrapolate if rng.random() < 0.5 else BCType.slip_wall, None)
+            if cls == "ideal":
+                p = 1.0e5 * (1.0 + 0.3 * _smooth_block(shape, rng))
+                T = 300.0 * (1.0 + 0.3 * _smooth_block(shape, rng))
+                v = [100.0 * _smooth_block(shape, rng) for _ in range(3)]
+                rho, u, _ = ideal_gas.rho_u_a_from_pT(p.ravel(), T.ravel())
+                U0 = ideal_gas.encode(rho, u, *[x.ravel() for x in v])
+                Tg = None
+            else:
+                T = 300.0 + 7700.0 * 0.5 * (_smooth_block(shape, rng) + 1.0)
+                rho = 0.01 + 0.99 * 0.5 * (_smooth_block(shape, rng) + 1.0)
+                v = [300.0 * _smooth_block(shape, rng) for _ in range(3)]
+                cold = np.array([0.767, 0.233, 0.0, 0.0, 0.0])
+                hot = np.array([0.60, 0.0, 0.02, 0.148, 0.232])
+                w = (T - 300.0) / 7700.0
+                basec = (1.0 - w)[None] * cold[:, None, None, None] + w[None] * hot[:, None, None, None]
+                g = rng.standard_gamma(200.0 * (basec + 1.0e-3))
+                Y = (g / g.sum(axis=0, keepdims=True)).reshape(tp_gas.n_species, -1)
+                u = tp_gas.energy(T.ravel(), Y)
+                U0 = tp_gas.encode(rho.ravel(), u, *[x.ravel() for x in v], Y)
+                Tg = T.ravel().copy()
+            blocks.append(dict(cls=cls, ni=ni, nj=nj, nk=nk, dx=1.0 / ni, dy=1.0 / nj, dz=1.0 / nk, bcs=bcs,
+                               U0=U0, Tguess=Tg))
+        for (a, fa, b, fb, o) in cn:
+            conns.append((off + a, fa, off + b, fb, o))
+    # one supersonic inflow face in the ideal class (Mach 2 air at 100 kPa, 300 K)
+    for blk in blocks[:n_ideal]:
+        free_w = [f for f, (k, _) in blk["bcs"].items() if f == Face.west]
+        if free_w:
+            a0 = float(np.sqrt(ideal_gas.gamma * ideal_gas.Rgas * 300.0))
+            blk["bcs"][Face.west] = (BCType.inflow, ideal_gas.flowstate(1.0e5, 300.0, (2.0 * a0, 0.0, 0.0)))
+            break
+    return blocks, conns

@@ -98,6 +98,9 @@ typedef struct E4hipStepConfig {
                                         solve from the learned guess (e4hip_gas_tp_set_guess_model)
                                         with an FP64 Newton polish; 0 = the reference's solve
                                         from the stored temperature */
+    int batch_blocks;                /* 1: the ideal-gas blocks of one stage (of one
+                                        dimensionality) go into one kernel launch when there
+                                        are two or more; 0: one launch per block */
 } E4hipStepConfig;
 
 /* Library set-up on one GPU; creates the library's HIP stream.

@@ -103,3 +103,23 @@ def test_tp_all_bcs(gpu):
     print("  U rel err per quantity", err, flush=True)
     assert err.max() <= TOL
     db.destroy()
+
+
+def test_tp_mixed_mode_close_to_plain(gpu):
+    """Mixed mode of the mixture path (Newton polish from the cell and stored
+    temperatures, bracketed solve as fall-back) against the plain FP64 path
+    over five steps: density within 1e-5, temperature within 1e-8."""
+    print("case tp mixed vs fp64 n=16", flush=True)
+    runs = {}
+    for precision in ("fp64", "mixed"):
+        ob, db, sim = setup_pair(16, precision=precision)
+        for _ in range(5):
+            sim.state.dt_global = 2.0e-7
+            sim.gasdynamic_explicit_increment_with_fixed_grid()
+        runs[precision] = db.download(with_fs=True)
+        db.destroy()
+    (Ua, fa), (Ub, fb) = runs["fp64"], runs["mixed"]
+    e_rho = float(np.max(np.abs(Ub[0] - Ua[0])) / np.max(np.abs(Ua[0])))
+    e_T = float(np.max(np.abs(fb[2] - fa[2]) / fa[2]))
+    print("  rho %.3e T %.3e" % (e_rho, e_T), flush=True)
+    assert e_rho <= 1e-5 and e_T <= 1e-8
