@@ -108,16 +108,21 @@ def tp_workload(e4, workloads, L, n=128, steps=5, warmup=1):
         sim.integrate_in_time(target_time=1e30, max_step=warmup)
         torch.cuda.synchronize()
         learned_eos.decode_stats(reset=True)
+        learned_eos.face_stats(reset=True)
         t0 = time.perf_counter()
         sim.integrate_in_time(target_time=1e30, max_step=warmup + steps)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         ev, fb = learned_eos.decode_stats(reset=True)
+        fev, ffb = learned_eos.face_stats(reset=True)
         finals[name] = blk.download(with_fs=True)
+        nside = 2 * 3 * n * n * (n + 1) * steps * 2    # face-state solves: two per face, two stages per step
         out[name] = {
             "value": round(n ** 3 * steps / wall / 1e6, 3), "ms_per_step": round(wall * 1e3 / steps, 3),
             "decode_energy_evals_per_cell_stage": round(ev / (n ** 3 * steps * 2), 3),
-            "decode_fallback_fraction": round(fb / (n ** 3 * steps * 2), 5)}
+            "decode_fallback_fraction": round(fb / (n ** 3 * steps * 2), 5),
+            "face_energy_evals_per_solve": round(fev / nside, 3),
+            "face_fallback_fraction": round(ffb / nside, 5)}
         blk.destroy()
     Ur, fr = finals["plain"]
     for name, _, _ in variants[1:]:

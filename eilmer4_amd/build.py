@@ -1,6 +1,6 @@
 """In-tree build of libe4hip.so for gfx950 with hipcc.
 
-Three translation units:
+Four translation units:
   stage_exact.hip  -ffp-contract=off: the FP64 stage kernel with the
                    reference's roundings (bit-identical to the CPU oracle)
   stage_fast.hip   -ffp-contract=fast: the plain FP64 and mixed FP32/FP64

@@ -76,7 +76,7 @@ struct State {
     bool init = false;
     int device = 0;
     hipStream_t stream = nullptr;
-    unsigned int* counters = nullptr;          // per-stage counters, 4 uints per block-stage slot
+    unsigned int* counters = nullptr;          // per-stage counters, e4::COUNTER_SLOT uints per block-stage slot
     unsigned long long* smax = nullptr;
     double* last_signal = nullptr;
     int counter_slots = 0;
@@ -91,6 +91,7 @@ struct State {
     double prof_ms[2] = {0, 0};
     long prof_n[2] = {0, 0};
     long long decode_evals = 0, decode_fallbacks = 0;
+    long long face_evals = 0, face_fallbacks = 0;      // face temperature solves of the thermally perfect path
     std::string err;
 };
 
@@ -177,7 +178,7 @@ int ensure_counters(int slots)
 {
     if (slots <= g_state.counter_slots) return E4HIP_OK;
     if (g_state.counters) HIP_TRY(hipFree(g_state.counters));
-    HIP_TRY(hipMalloc(&g_state.counters, sizeof(unsigned int) * 4 * slots));
+    HIP_TRY(hipMalloc(&g_state.counters, sizeof(unsigned int) * e4::COUNTER_SLOT * slots));
     g_state.counter_slots = slots;
     return E4HIP_OK;
 }
@@ -400,13 +401,11 @@ int launch_tp_stage(BlockH& b, int stage, double dt, unsigned int* counters)
     A.counters = counters;
     A.gas = g_state.gases[b.gm].tp_dev;
     A.guess = cfg.learned_eos_guess ? g_state.gases[b.gm].guess_dev : nullptr;
+    A.bc = b.bc_dev;
     for (int f = 0; f < 6; ++f) {
-        A.bc_type[f] = b.bc_type[f];
-        memcpy(A.bc_fs[f], b.bc_fs_tp[f], sizeof(A.bc_fs[f]));
         if (b.bc_type[f] != E4HIP_BC_CONNECTED) continue;
         const BlockH* o = get_block(b.nbr_blk[f]);
         if (!o) return fail(E4HIP_ERR_HANDLE, "face %d is connected to a destroyed block", f);
-        A.nbr[f] = connection_of(b, f);
         // the other block's primitive state at the start of this stage (its
         // own stage launch, earlier in the same stage, swaps P[0] and P[1])
         A.nbrP[f] = g_state.stage_pin[b.nbr_blk[f]];
@@ -508,7 +507,10 @@ int upload_bc(const BlockH& b)
 {
     e4::BCTable t;
     memset(&t, 0, sizeof(t));
-    for (int f = 0; f < 6; ++f) { t.type[f] = b.bc_type[f]; t.fs[f] = b.bc_fs[f]; t.nbr[f] = connection_of(b, f); }
+    for (int f = 0; f < 6; ++f) {
+        t.type[f] = b.bc_type[f]; t.fs[f] = b.bc_fs[f]; t.nbr[f] = connection_of(b, f);
+        memcpy(t.fs_tp[f], b.bc_fs_tp[f], sizeof(t.fs_tp[f]));
+    }
     HIP_TRY(hipStreamSynchronize(g_state.stream));
     HIP_TRY(hipMemcpy(b.bc_dev, &t, sizeof(t), hipMemcpyHostToDevice));
     return E4HIP_OK;
@@ -689,6 +691,14 @@ int e4hip_tp_decode_stats(long long* energy_evals, long long* fallbacks, int res
     return E4HIP_OK;
 }
 
+int e4hip_tp_face_stats(long long* energy_evals, long long* fallbacks, int reset)
+{
+    if (energy_evals) *energy_evals = g_state.face_evals;
+    if (fallbacks) *fallbacks = g_state.face_fallbacks;
+    if (reset) { g_state.face_evals = 0; g_state.face_fallbacks = 0; }
+    return E4HIP_OK;
+}
+
 int e4hip_block_create(int gm, int dims, int ni, int nj, int nk, double dx, double dy, double dz, int* blk)
 {
     if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
@@ -733,7 +743,11 @@ int e4hip_block_set_bc(int blk, int face, int type, const double* fs)
     if (type == E4HIP_BC_INFLOW && b->is_tp()) {
         if (!fs) return fail(E4HIP_ERR_ARG, "inflow needs a flow state");
         if (!(fs[0] > 0) || !(fs[2] > 0) || !(fs[4] > 0)) return fail(E4HIP_ERR_ARG, "invalid inflow state");
-        for (int q = 0; q < 8 + b->nsp; ++q) b->bc_fs_tp[face][q] = fs[q];
+        // fs = rho, p, T, u, a, vx, vy, vz, Y; stored in the primitive-array order
+        double* t = b->bc_fs_tp[face];
+        t[e4::P_RHO] = fs[0]; t[e4::P_P] = fs[1]; t[e4::P_T] = fs[2]; t[e4::P_U] = fs[3]; t[e4::P_A] = fs[4];
+        t[e4::P_VX] = fs[5]; t[e4::P_VY] = fs[6]; t[e4::P_VZ] = fs[7];
+        for (int q = 0; q < b->nsp; ++q) t[e4::P_Y + q] = fs[8 + q];
     } else if (type == E4HIP_BC_INFLOW) {
         if (!fs) return fail(E4HIP_ERR_ARG, "inflow needs a flow state");
         // fs = rho, p, T, u, a, vx, vy, vz
@@ -797,7 +811,8 @@ int e4hip_block_ncells(int blk, long* ncells, int* nq)
 
 static int run_decode(const BlockH& b, double* fs_dev, unsigned int* nbad)
 {
-    hipLaunchKernelGGL(e4::reset_counters_kernel, dim3(1), dim3(64), 0, g_state.stream, g_state.counters, 4, nullptr);
+    hipLaunchKernelGGL(e4::reset_counters_kernel, dim3(1), dim3(64), 0, g_state.stream, g_state.counters,
+                       e4::COUNTER_SLOT, nullptr);
     e4::DecodeArgs D;
     D.U = b.U[0]; D.fs = fs_dev; D.qs = b.qs; D.ncells = b.ncells;
     D.gas = g_state.gases[b.gm].ideal; D.counters = g_state.counters;
@@ -831,14 +846,15 @@ int e4hip_block_upload(int blk, const double* U, const double* Tguess)
         double* Tg = nullptr;
         HIP_TRY(hipMalloc(&Tg, sizeof(double) * b->ncells));
         HIP_TRY(hipMemcpyAsync(Tg, Tguess, sizeof(double) * b->ncells, hipMemcpyHostToDevice, g_state.stream));
-        hipLaunchKernelGGL(e4::reset_counters_kernel, dim3(1), dim3(64), 0, g_state.stream, g_state.counters, 4, nullptr);
+        hipLaunchKernelGGL(e4::reset_counters_kernel, dim3(1), dim3(64), 0, g_state.stream, g_state.counters,
+                       e4::COUNTER_SLOT, nullptr);
         e4::TPArgs A = tp_args_for(*b);
         A.Pout = b->P[0];
         A.guess = g_state.cfg.learned_eos_guess ? g_state.gases[b->gm].guess_dev : nullptr;
         if (g_state.cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT) e4::launch_tp_decode_exact(A, b->U[0], Tg, g_state.stream);
         else e4::launch_tp_decode_fast64(A, b->U[0], Tg, g_state.stream);
         HIP_TRY(hipGetLastError());
-        unsigned int cnt[4] = {0, 0, 0, 0};
+        unsigned int cnt[e4::COUNTER_SLOT] = {};
         HIP_TRY(hipMemcpyAsync(cnt, g_state.counters, sizeof(cnt), hipMemcpyDeviceToHost, g_state.stream));
         HIP_TRY(hipStreamSynchronize(g_state.stream));
         HIP_TRY(hipFree(Tg));
@@ -917,7 +933,7 @@ int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_ce
     if (rc) return rc;
     const int nslots = nblk * 3;
     hipLaunchKernelGGL(e4::reset_counters_kernel, dim3(1), dim3(1024), 0, g_state.stream, g_state.counters,
-                       4 * nslots, nullptr);
+                       e4::COUNTER_SLOT * nslots, nullptr);
     for (int stage = 1; stage <= ns; ++stage) {
         g_state.stage_pin.assign(g_state.blocks.size(), nullptr);
         for (size_t h = 0; h < g_state.blocks.size(); ++h)
@@ -928,7 +944,7 @@ int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_ce
         std::vector<unsigned int*> bcount[2];
         for (int n = 0; n < nblk; ++n) {
             BlockH& b = *get_block(blks[n]);
-            unsigned int* cnt = g_state.counters + 4 * (n * 3 + (stage - 1));
+            unsigned int* cnt = g_state.counters + e4::COUNTER_SLOT * (n * 3 + (stage - 1));
             if (b.is_tp()) {
                 rc = launch_tp_stage(b, stage, dt, cnt);
                 if (rc) return rc;
@@ -953,17 +969,21 @@ int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_ce
         BlockH& b = *get_block(blks[n]);
         if (ns == 1) std::swap(b.U[0], b.U[1]);   // Euler: the new state is in U[1]
     }
-    std::vector<unsigned int> h(4 * nslots);
-    HIP_TRY(hipMemcpyAsync(h.data(), g_state.counters, sizeof(unsigned int) * 4 * nslots, hipMemcpyDeviceToHost,
+    const int CS = e4::COUNTER_SLOT;
+    std::vector<unsigned int> h(CS * nslots);
+    HIP_TRY(hipMemcpyAsync(h.data(), g_state.counters, sizeof(unsigned int) * CS * nslots, hipMemcpyDeviceToHost,
                            g_state.stream));
     HIP_TRY(hipStreamSynchronize(g_state.stream));
     int stage_bad[3] = {0, 0, 0};
     bool too_many = false, decode_fail = false;
     for (int n = 0; n < nblk; ++n)
         for (int s = 0; s < 3; ++s) {
-            unsigned int inv = h[4 * (n * 3 + s)], dec = h[4 * (n * 3 + s) + 1];
-            g_state.decode_evals += h[4 * (n * 3 + s) + 2];
-            g_state.decode_fallbacks += h[4 * (n * 3 + s) + 3];
+            const unsigned int* c = &h[CS * (n * 3 + s)];
+            unsigned int inv = c[0], dec = c[1];
+            g_state.decode_evals += c[2];
+            g_state.decode_fallbacks += c[3];
+            g_state.face_evals += c[4];
+            g_state.face_fallbacks += c[5];
             stage_bad[s] += (int)(inv + dec);
             if ((int)inv > g_state.cfg.max_invalid_cells) too_many = true;
             if (dec) decode_fail = true;

@@ -88,51 +88,73 @@ __device__ __forceinline__ float mlp_guess_wave(const GuessModel& G, const doubl
     return G.T_mean + G.T_std * y;
 }
 
-// FP64 Newton polish from the learned guess: two Newton steps on
-// f(T) = e_tgt - u(T) without bracketing, then the judge: the energy residual
-// at the result, divided by the last derivative, estimates the remaining
-// temperature error; the polish is accepted when that estimate is at most
-// 1e-11 T.  (Inside the curves' blend ranges the derivative sum_s Y_s (Cp_s -
-// R_s) is not the exact derivative of the blended energy, so convergence there
-// is only linear and the size of the last step alone would not bound the
-// error.)  Q.u is left at the energy of the last evaluated temperature, which
-// here is the result.  Returns 0 when accepted, -1 to fall back to the
-// reference's bracketed solve.
+// FP64 polish from a temperature guess (the learned guess, or the stored or
+// the face's cell temperature): two iterations without bracketing, then the
+// judge: the energy residual at the result, divided by the derivative there,
+// estimates the remaining temperature error, and the polish is accepted when
+// that estimate is at most 1e-11 T.  Q.u is left at the energy of the accepted
+// temperature.  Returns 0 when accepted, -1 to fall back to the reference's
+// bracketed solve.
+//
+// Fast modes with shared breakpoints: Halley steps (cubic convergence) on the
+// exact u, du/dT and d2u/dT2 of the blended curves, the species sums computed
+// once per curve piece.  From a face's cell temperature (typically tens to a
+// few hundred kelvin from the face root) two Halley steps reach the judge's
+// bound; Newton would need three.  Otherwise: Newton steps with the
+// reference's dzdT, sum_s Y_s (Cp_s - R_s), which inside a blend is not the
+// derivative of the blended energy (convergence there is only linear, hence a
+// judge on the residual rather than on the size of the last step).
 template <class M, int NSP>
-__device__ int tp_polish(const TPGas& g, TPState<NSP>& Q, double e_tgt, double T0, int* fevals)
+__device__ __forceinline__ int tp_polish(const TPGas& g, TPState<NSP>& Q, double e_tgt, double T0, int* fevals)
 {
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
     if (!(T0 >= 20.0 && T0 <= 100000.0)) return -1;
-    // energy and its T derivative at one temperature (one mixture evaluation
-    // in the fast modes)
-    auto u_dzdT = [&](double T, double& u, double& dz) {
-        if constexpr (M::kFast) {
-            if (g.shared) {
-                double cv;
-                tp_mix_u_cv<NSP>(g, Q.Y, tp_Rmix<NSP>(g, Q.Y), T, &u, &cv);
-                dz = -1.0 * cv;
-                return;
+    if constexpr (M::kFast) {
+        if (g.shared) {
+            const CeaCurve& c0 = g.curve[0];
+            const double Tlo = c0.Tb[0], Thi = c0.Tb[c0.nseg];
+            const double Rmix = tp_Rmix<NSP>(g, Q.Y);
+            TPPiece P;
+            P.lo = 1.0; P.hi = 0.0;   // empty: the first evaluation finds its piece
+            double T = T0;
+#pragma nounroll
+            for (int it = 0;; ++it) {
+                if (it == 2 && !(T >= 20.0 && T <= 100000.0)) return -1;
+                const double Tc = fmin(fmax(T, Tlo), Thi);
+                if (!(Tc >= P.lo && Tc <= P.hi)) tp_mix_piece<NSP>(g, Q.Y, Tc, P);
+                double u, du, d2u;
+                tp_piece_eval(P, Rmix, Tlo, Thi, T, &u, &du, &d2u);
+                (*fevals)++;
+                const double f = u - e_tgt;
+                if (it == 2) {
+                    const double err = fabs(dv<MD>(f, du));
+                    if (!(err <= 1.0e-11 * T)) return -1;
+                    Q.T = T;
+                    Q.u = u;
+                    return 0;
+                }
+                T -= dv<MD>(2.0 * f * du, 2.0 * du * du - f * d2u);
             }
         }
-        u = tp_energy<M, NSP>(g, Q.Y, T);
-        dz = tp_dzdT<M, NSP>(g, Q.Y, T);
-    };
-    double T = T0, df = 1.0;
-    for (int it = 0; it < 2; ++it) {
-        double u;
-        u_dzdT(T, u, df);
-        (*fevals)++;
-        double f = e_tgt - u;
-        T -= dv<MD>(f, df);
     }
-    if (!(T >= 20.0 && T <= 100000.0)) return -1;
-    double u = tp_energy<M, NSP>(g, Q.Y, T);
-    (*fevals)++;
-    double err = fabs(dv<MD>(e_tgt - u, df));
-    if (!(err <= 1.0e-11 * T)) return -1;
-    Q.T = T;
-    Q.u = u;
-    return 0;
+    // two Newton steps, then the energy at the result (one evaluation site)
+    double T = T0, df = 1.0;
+#pragma nounroll
+    for (int it = 0;; ++it) {
+        if (it == 2 && !(T >= 20.0 && T <= 100000.0)) return -1;
+        double u, dz;
+        tp_u_dzdT<M, NSP>(g, Q.Y, T, it < 2, &u, &dz);
+        (*fevals)++;
+        if (it == 2) {
+            const double err = fabs(dv<MD>(e_tgt - u, df));
+            if (!(err <= 1.0e-11 * T)) return -1;
+            Q.T = T;
+            Q.u = u;
+            return 0;
+        }
+        df = dz;
+        T -= dv<MD>(e_tgt - u, df);
+    }
 }
 
 } // namespace e4

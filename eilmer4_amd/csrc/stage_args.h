@@ -67,6 +67,7 @@ struct BCTable {
     int pad[2];
     Prim fs[6];                       // inflow ghost states (global frame)
     Connection nbr[6];                // connected faces
+    double fs_tp[6][8 + TP_NSP_MAX];  // mixture inflow states in the primitive-array order (P_*)
 };
 
 struct StageArgs {
@@ -101,6 +102,13 @@ struct BatchArgs {
     const int* wg_entry;
 };
 
+// Counters of one block and stage: [0] invalid cells, [1] decode failures,
+// [2] decode energy evaluations, [3] decodes whose polish (from the learned
+// guess or the stored temperature) fell back to the reference solve,
+// [4] face energy evaluations, [5] face-state solves whose polish fell back
+// (thermally perfect path).
+constexpr int COUNTER_SLOT = 8;
+
 enum { P_RHO = 0, P_VX = 1, P_VY = 2, P_VZ = 3, P_U = 4, P_T = 5, P_P = 6, P_A = 7, P_Y = 8 };
 
 struct TPArgs {
@@ -112,8 +120,7 @@ struct TPArgs {
     double* __restrict__ dout;
     double* __restrict__ Unew;
     double* F[3];                     // face fluxes [5+nsp][nfaces(dir)]
-    unsigned int* counters;           // [0] invalid, [1] decode failures, [2] decode energy evaluations,
-                                      // [3] cells whose learned guess fell back to the reference solve
+    unsigned int* counters;           // one slot of COUNTER_SLOT
     long qs;
     long fqs[3];
     int ni, nj, nk, dims;
@@ -125,9 +132,7 @@ struct TPArgs {
     L2R2W w[3];
     const TPGas* gas;                 // device copy of the gas model (read-only)
     const GuessModel* guess;          // learned temperature guess of the decode, or null
-    int bc_type[6];
-    double bc_fs[6][8 + TP_NSP_MAX];  // ghost flow state of inflow faces: rho,p,T,u,a,vx,vy,vz,Y
-    Connection nbr[6];                // connected faces
+    const BCTable* bc;                // boundary effects (device memory)
     const double* nbrP[6];            // primitive arrays of the connected blocks at the stage start
 };
 

@@ -31,7 +31,8 @@ void launch_tp_stage_exact(const TPArgs& A, hipStream_t stream, int flux)
         hipLaunchKernelGGL((tp_face_kernel<ModeExact, 5, FLUX_AUSMDV, 1>), g(nf1), b, 0, stream, A);
         if (A.dims == 3) hipLaunchKernelGGL((tp_face_kernel<ModeExact, 5, FLUX_AUSMDV, 2>), g(nf2), b, 0, stream, A);
     }
-    hipLaunchKernelGGL((tp_update_kernel<ModeExact, 5>), g(nc), b, 0, stream, A);
+    if (A.guess) hipLaunchKernelGGL((tp_update_kernel<ModeExact, 5, true>), g(nc), b, 0, stream, A);
+    else hipLaunchKernelGGL((tp_update_kernel<ModeExact, 5, false>), g(nc), b, 0, stream, A);
 }
 
 void launch_tp_decode_exact(const TPArgs& A, const double* U, const double* Tguess, hipStream_t stream)

@@ -162,6 +162,13 @@ __device__ __forceinline__ void tp_mix_coeffs(const TPGas& g, const double* Y, d
     }
 }
 
+// Mixture enthalpy of the mixture coefficients M at T (r = 1/T, lnT = log T)
+__device__ __forceinline__ double tp_mix_h(const double* M, double T, double r, double lnT)
+{
+    return -M[0] * r + M[1] * lnT + M[7] +
+           T * (M[2] + T * (M[3] * 0.5 + T * (M[4] * (1.0 / 3.0) + T * (M[5] * 0.25 + T * (M[6] * 0.2)))));
+}
+
 // Mixture energy u = sum_s Y_s (h_s - R_s T) and Cv = sum_s Y_s (Cp_s - R_s)
 // at T (fast modes, shared breakpoints); outside the curve range the
 // constant-Cp extension of cea_thermo_curves.d:63-91.
@@ -177,13 +184,92 @@ __device__ __forceinline__ void tp_mix_u_cv(const TPGas& g, const double* Y, dou
     const double r = dv_seed(1.0, Tc, rcp_seed(Tc));
     const double lnT = log(Tc);
     const double T2 = Tc * Tc, T3 = T2 * Tc, T4 = T2 * T2;
-    double h = -Mk[0] * r + Mk[1] * lnT + Mk[7] +
-               Tc * (Mk[2] + Tc * (Mk[3] * 0.5 + Tc * (Mk[4] * (1.0 / 3.0) + Tc * (Mk[5] * 0.25 + Tc * (Mk[6] * 0.2)))));
+    double h = tp_mix_h(Mk, Tc, r, lnT);
     const double cp = Mk[0] * r * r + Mk[1] * r + Mk[2] + Mk[3] * Tc + Mk[4] * T2 + Mk[5] * T3 + Mk[6] * T4;
     if (T < Tlo) h -= cp * (Tlo - T);
     else if (T > Thi) h += cp * (T - Thi);
     *u = h - Rmix * T;
     *cvmix = cp - Rmix;
+}
+
+// One piece of the mixture's blended curves (fast modes, shared breakpoints):
+// for T in [lo, hi] the mixture coefficients are M + (T - t0) dM, with dM = 0
+// outside the blend ranges (cea_thermo_curves.d:136-169 applied to the
+// mixture).  A solve that stays on one piece computes the species sums once.
+struct TPPiece {
+    double lo, hi, t0;
+    double M[8], dM[8];
+    int blend;
+};
+
+template <int NSP>
+__device__ __forceinline__ void tp_mix_piece(const TPGas& g, const double* Y, double Tc, TPPiece& P)
+{
+    const CeaCurve& cv = g.curve[0];
+    const int nb = cv.nseg + 1;
+    int s0 = -1, s1 = -1;
+    double lo = -1.0e300, hi = 1.0e300;
+    if (Tc < (cv.Tb[1] - 0.5 * cv.Tbl[0])) {
+        s0 = 0;
+        hi = cv.Tb[1] - 0.5 * cv.Tbl[0];
+    } else if (Tc > (cv.Tb[nb - 2] + 0.5 * cv.Tbl[nb - 3])) {
+        s0 = cv.nseg - 1;
+        lo = cv.Tb[nb - 2] + 0.5 * cv.Tbl[nb - 3];
+    } else {
+        for (int i = 1; i < nb - 1; ++i) {
+            const double blo = cv.Tb[i] - 0.5 * cv.Tbl[i - 1], bhi = cv.Tb[i] + 0.5 * cv.Tbl[i - 1];
+            if (Tc >= blo && Tc <= bhi) { s0 = i - 1; s1 = i; lo = blo; hi = bhi; break; }
+            const double next = cv.Tb[i + 1] - 0.5 * cv.Tbl[i];
+            if (Tc > bhi && Tc < next) { s0 = i; lo = bhi; hi = next; break; }
+        }
+        if (s0 < 0) { s0 = 0; lo = Tc; hi = Tc; }   // unreachable for Tc inside the curve range
+    }
+    P.lo = lo; P.hi = hi; P.t0 = lo;
+    P.blend = s1 >= 0;
+    const double rbl = (s1 >= 0) ? dv_seed(1.0, cv.Tbl[s1 - 1], rcp_seed(cv.Tbl[s1 - 1])) : 0.0;
+    const int sb = (s1 >= 0) ? s1 : s0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) {
+            a += Y[s] * g.RC[s0][k][s];
+            b += Y[s] * g.RC[sb][k][s];
+        }
+        P.M[k] = a;
+        P.dM[k] = (b - a) * rbl;
+    }
+}
+
+// u, du/dT and d2u/dT2 of the mixture at T from its piece (exact derivatives
+// of the blended curves; constant-Cp extension outside the curve range,
+// cea_thermo_curves.d:63-91).
+__device__ __forceinline__ void tp_piece_eval(const TPPiece& P, double Rmix, double Tlo, double Thi, double T,
+                                              double* u, double* du, double* d2u)
+{
+    const double Tc = fmin(fmax(T, Tlo), Thi);
+    const double w = Tc - P.t0;
+    double Mk[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) Mk[k] = P.M[k] + w * P.dM[k];
+    const double r = dv_seed(1.0, Tc, rcp_seed(Tc));
+    const double lnT = log(Tc);
+    const double T2 = Tc * Tc, T3 = T2 * Tc, T4 = T2 * T2;
+    double h = tp_mix_h(Mk, Tc, r, lnT);
+    const double cp = Mk[0] * r * r + Mk[1] * r + Mk[2] + Mk[3] * Tc + Mk[4] * T2 + Mk[5] * T3 + Mk[6] * T4;
+    double dh = cp;
+    double d2h = -2.0 * Mk[0] * r * r * r - Mk[1] * r * r + Mk[3] + 2.0 * Mk[4] * Tc + 3.0 * Mk[5] * T2 +
+                 4.0 * Mk[6] * T3;
+    if (P.blend) {
+        dh += tp_mix_h(P.dM, Tc, r, lnT);
+        d2h += 2.0 * (P.dM[0] * r * r + P.dM[1] * r + P.dM[2] + P.dM[3] * Tc + P.dM[4] * T2 + P.dM[5] * T3 +
+                      P.dM[6] * T4);
+    }
+    if (T < Tlo) { h -= cp * (Tlo - T); dh = cp; d2h = 0.0; }
+    else if (T > Thi) { h += cp * (T - Thi); dh = cp; d2h = 0.0; }
+    *u = h - Rmix * T;
+    *du = dh - Rmix;
+    *d2u = d2h;
 }
 
 // therm_perf_gas_mix_eos.d:63-69 (mass_average, gas_model.d:330-338)
@@ -226,51 +312,62 @@ __device__ __forceinline__ double tp_dzdT(const TPGas& g, const double* Y, doubl
     return -1.0 * result;
 }
 
+// Mixture energy at T and, when want_d, the dzdT closure's value
+// -(sum Y_s Cv_s) (therm_perf_gas_mix_eos.d:104-123); the fast modes take
+// both from one mixture evaluation.
+template <class M, int NSP>
+__device__ __forceinline__ void tp_u_dzdT(const TPGas& g, const double* Y, double T, bool want_d, double* u,
+                                          double* dz)
+{
+    if constexpr (M::kFast) {
+        if (g.shared) {
+            double cv;
+            tp_mix_u_cv<NSP>(g, Y, tp_Rmix<NSP>(g, Y), T, u, &cv);
+            *dz = -1.0 * cv;
+            return;
+        }
+    }
+    *u = tp_energy<M, NSP>(g, Y, T);
+    *dz = want_d ? tp_dzdT<M, NSP>(g, Y, T) : 0.0;
+}
+
 // nm/newton.d:40-107 with the zeroFn/dzdT closures; Q.T and Q.u receive the
 // closures' side effects.  Returns 0 and the root, or -1 when it would throw.
+// All energy evaluations (the two bracket ends, then the iterates from x0)
+// go through one call site in one loop, which keeps the inlined code to one
+// copy of the energy evaluation.
 template <class M, int NSP>
-__device__ int tp_newton(const TPGas& g, TPState<NSP>& Q, double e_tgt, double x0, double xMin, double xMax,
-                         double tol, double* root, int* fevals)
+__device__ __forceinline__ int tp_newton(const TPGas& g, TPState<NSP>& Q, double e_tgt, double x0, double xMin,
+                                         double xMax, double tol, double* root, int* fevals)
 {
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
-    auto f = [&](double T) {
-        Q.T = T;
-        Q.u = tp_energy<M, NSP>(g, Q.Y, T);
-        (*fevals)++;
-        return e_tgt - Q.u;
-    };
     const int MAXIT = 30;
-    double xL = xMin, xH = xMax;
-    double fL = f(xL);
-    double fH = f(xH);
-    if ((fL > 0.0 && fH > 0.0) || (fL < 0.0 && fH < 0.0)) return -1;
-    if (fL == 0.0) { *root = xMin; return 0; }
-    if (fH == 0.0) { *root = xMax; return 0; }
-    if (fL < 0.0) { xL = xMin; xH = xMax; } else { xH = xMin; xL = xMax; }
-    // f and its derivative at one T (the fast modes evaluate both from one
-    // set of mixture coefficients)
-    auto fdf = [&](double T, double& fv, double& dfv) {
-        if constexpr (M::kFast) {
-            if (g.shared) {
-                double u, cv;
-                tp_mix_u_cv<NSP>(g, Q.Y, tp_Rmix<NSP>(g, Q.Y), T, &u, &cv);
-                Q.T = T;
-                Q.u = u;
-                (*fevals)++;
-                fv = e_tgt - u;
-                dfv = -1.0 * cv;
-                return;
-            }
-        }
-        fv = f(T);
-        dfv = tp_dzdT<M, NSP>(g, Q.Y, T);
-    };
+    double xL = xMin, xH = xMax, fL = 0.0;
     double rts = x0;
     double dxold = (xMax - xMin);
     double dx = dxold;
-    double f0, df0;
-    fdf(rts, f0, df0);
-    for (int j = 0; j < MAXIT; ++j) {
+#pragma nounroll
+    for (int n = 0;; ++n) {
+        // evaluation n: 0 the lower bracket end, 1 the upper, 2+ the iterates
+        const double T = (n == 0) ? xMin : (n == 1) ? xMax : rts;
+        double u, df0;
+        tp_u_dzdT<M, NSP>(g, Q.Y, T, n >= 2, &u, &df0);
+        Q.T = T;
+        Q.u = u;
+        (*fevals)++;
+        const double f0 = e_tgt - u;
+        if (n == 0) { fL = f0; continue; }
+        if (n == 1) {
+            const double fH = f0;
+            if ((fL > 0.0 && fH > 0.0) || (fL < 0.0 && fH < 0.0)) return -1;
+            if (fL == 0.0) { *root = xMin; return 0; }
+            if (fH == 0.0) { *root = xMax; return 0; }
+            if (fL < 0.0) { xL = xMin; xH = xMax; } else { xH = xMin; xL = xMax; }
+            continue;
+        }
+        const int j = n - 2;   // Newton steps taken so far
+        if (j > 0) { if (f0 < 0.0) xL = rts; else xH = rts; }
+        if (j == MAXIT) return -1;
         if ((((rts - xH) * df0 - f0) * ((rts - xL) * df0 - f0) > 0.0) || (fabs(2.0 * f0) > fabs(dxold * df0))) {
             dxold = dx;
             dx = 0.5 * (xH - xL);
@@ -284,10 +381,7 @@ __device__ int tp_newton(const TPGas& g, TPState<NSP>& Q, double e_tgt, double x
             if (tmp == rts) { *root = rts; return 0; }
         }
         if (fabs(dx) < tol) { *root = rts; return 0; }
-        fdf(rts, f0, df0);
-        if (f0 < 0.0) xL = rts; else xH = rts;
     }
-    return -1;
 }
 
 // therm_perf_gas_mix_eos.d:70-153: guess = Q.T, bracket [max(T-500, 20), +1000],
@@ -295,20 +389,20 @@ __device__ int tp_newton(const TPGas& g, TPState<NSP>& Q, double e_tgt, double x
 template <class M, int NSP>
 __device__ int tp_update_temperature(const TPGas& g, TPState<NSP>& Q, int* fevals)
 {
-    double Tsave = Q.T;
-    double e_tgt = Q.u;
-    double delT = 1000.0;
-    double T1 = fmax(Q.T - 0.5 * delT, 20.0);
-    double T2 = T1 + delT;
-    double root;
-    if (tp_newton<M, NSP>(g, Q, e_tgt, Tsave, T1, T2, 1.0e-6, &root, fevals) == 0) { Q.T = root; return 0; }
-    if (tp_newton<M, NSP>(g, Q, e_tgt, Tsave, 20.0, 100000.0, 1.0e-6, &root, fevals) == 0) { Q.T = root; return 0; }
+    const double Tsave = Q.T;
+    const double e_tgt = Q.u;
+    const double delT = 1000.0;
+    const double T1 = fmax(Q.T - 0.5 * delT, 20.0);
+#pragma nounroll
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        const double lo = attempt ? 20.0 : T1, hi = attempt ? 100000.0 : T1 + delT;
+        double root;
+        if (tp_newton<M, NSP>(g, Q, e_tgt, Tsave, lo, hi, 1.0e-6, &root, fevals) == 0) { Q.T = root; return 0; }
+    }
     Q.T = Tsave;
     Q.u = tp_energy<M, NSP>(g, Q.Y, Tsave);
     return -1;
 }
-
-
 
 // therm_perf_gas.d:187-191 with perf_gas_mix_eos.d:43-49
 template <class M, int NSP>

@@ -11,7 +11,7 @@
 //   tp_face_kernel<M,NSP,FLUX,DIR>  one thread per face of direction DIR:
 //       l2r2 reconstruction of velocity (face frame), mass fractions (then
 //       scale_mass_fractions), rho and u; face Newton; flux; global frame.
-//   tp_update_kernel<M,NSP>         one thread per cell: time_derivatives,
+//   tp_update_kernel<M,NSP,GUESS>   one thread per cell: time_derivatives,
 //       stage update, decode_conserved with Newton (guess: the cell's T),
 //       low-T fallback, check_data; writes U and the primitive state.
 #pragma once
@@ -30,15 +30,16 @@ template <class M> struct TPArith { using type = M; static constexpr bool polish
 template <> struct TPArith<ModeMixed> { using type = ModeFast64; static constexpr bool polish = true; };
 
 // update_thermo_from_rhou of a face state (therm_perf_gas.d:187-191); with
-// polish, Newton from Q.T first.
+// polish, Newton from Q.T first (*fallbacks counts the solves it hands on).
 template <class M, int NSP, bool POLISH>
-__device__ __forceinline__ int tp_face_eos(const TPGas& g, TPState<NSP>& Q, int* fevals)
+__device__ __forceinline__ int tp_face_eos(const TPGas& g, TPState<NSP>& Q, int* fevals, int* fallbacks)
 {
     if constexpr (POLISH) {
         if (tp_polish<M, NSP>(g, Q, Q.u, Q.T, fevals) == 0) {
             Q.p = Q.rho * tp_Rmix<NSP>(g, Q.Y) * Q.T;
             return 0;
         }
+        (*fallbacks)++;
     }
     return tp_update_thermo_from_rhou<M, NSP>(g, Q, fevals);
 }
@@ -50,26 +51,21 @@ struct TPCell {
     double Y[NSP];
 };
 
-template <int NSP>
-__device__ __forceinline__ TPCell<NSP> tp_load(const TPArgs& A, long c)
-{
-    TPCell<NSP> s;
-    const double* P = A.Pin;
-    const long q = A.qs;
-    s.rho = P[P_RHO * q + c]; s.v[0] = P[P_VX * q + c]; s.v[1] = P[P_VY * q + c]; s.v[2] = P[P_VZ * q + c];
-    s.u = P[P_U * q + c]; s.T = P[P_T * q + c]; s.p = P[P_P * q + c]; s.a = P[P_A * q + c];
-#pragma unroll
-    for (int sp = 0; sp < NSP; ++sp) s.Y[sp] = P[(P_Y + sp) * q + c];
-    return s;
-}
+// Where the values of one stencil position come from: field f (P_* order)
+// is base[f * stride] -- the primitive array of this block or of a connected
+// block (stride qs), or a fixed inflow state (stride 1); a slip wall negates
+// the velocity component `flip` (-1: none).  The same mapping as the
+// ideal-gas stage kernel's fetch_ghost.
+struct TPRef {
+    const double* base;
+    long stride;
+    int flip;
+};
 
-// The cell at (i,j,k) or the ghost value its boundary effect gives it (the same
-// mapping as the ideal-gas stage kernel's fetch_ghost).
-template <int NSP>
-__device__ TPCell<NSP> tp_fetch(const TPArgs& A, int i, int j, int k)
+__device__ __forceinline__ TPRef tp_ref(const TPArgs& A, int i, int j, int k)
 {
     if (i >= 0 && i < A.ni && j >= 0 && j < A.nj && k >= 0 && k < A.nk)
-        return tp_load<NSP>(A, ((long)k * A.nj + j) * A.ni + i);
+        return TPRef{A.Pin + (((long)k * A.nj + j) * A.ni + i), A.qs, -1};
     int dir, idx, n;
     if (i < 0 || i >= A.ni) { dir = 0; idx = i; n = A.ni; }
     else if (j < 0 || j >= A.nj) { dir = 1; idx = j; n = A.nj; }
@@ -79,32 +75,17 @@ __device__ TPCell<NSP> tp_fetch(const TPArgs& A, int i, int j, int k)
     const int face = (dir == 0) ? (plus ? FACE_EAST : FACE_WEST)
                    : (dir == 1) ? (plus ? FACE_NORTH : FACE_SOUTH)
                                 : (plus ? FACE_TOP : FACE_BOTTOM);
-    const int type = A.bc_type[face];
-    if (type == BC_INFLOW) {
-        const double* f = A.bc_fs[face];
-        TPCell<NSP> s;
-        s.rho = f[0]; s.p = f[1]; s.T = f[2]; s.u = f[3]; s.a = f[4];
-        s.v[0] = f[5]; s.v[1] = f[6]; s.v[2] = f[7];
-#pragma unroll
-        for (int sp = 0; sp < NSP; ++sp) s.Y[sp] = f[8 + sp];
-        return s;
-    }
+    const BCTable* bc = A.bc;
+    const int type = bc->type[face];
+    if (type == BC_INFLOW) return TPRef{bc->fs_tp[face], 1, -1};
     if (type == BC_CONNECTED) {
         // full_face_copy.d:1277-1357: the mapped interior cell of the other block
-        const Connection& nb = A.nbr[face];
+        const Connection nb = bc->nbr[face];
         const int fa = (face == FACE_NORTH || face == FACE_SOUTH || face >= FACE_TOP) ? i : j;
         const int fb = (face >= FACE_TOP) ? j : k;
         int sc[3];
         connection_source(nb.dims, face, nb.face, nb.orient, nb.n, layer, fa, fb, sc);
-        const long c = ((long)sc[2] * nb.n[1] + sc[1]) * nb.n[0] + sc[0];
-        const double* P = A.nbrP[face];
-        const long q = nb.qs;
-        TPCell<NSP> s;
-        s.rho = P[P_RHO * q + c]; s.v[0] = P[P_VX * q + c]; s.v[1] = P[P_VY * q + c]; s.v[2] = P[P_VZ * q + c];
-        s.u = P[P_U * q + c]; s.T = P[P_T * q + c]; s.p = P[P_P * q + c]; s.a = P[P_A * q + c];
-#pragma unroll
-        for (int sp = 0; sp < NSP; ++sp) s.Y[sp] = P[(P_Y + sp) * q + c];
-        return s;
+        return TPRef{A.nbrP[face] + (((long)sc[2] * nb.n[1] + sc[1]) * nb.n[0] + sc[0]), nb.qs, -1};
     }
     int src;
     if (type == BC_EXTRAPOLATE) src = plus ? n - 1 : 0;
@@ -112,8 +93,27 @@ __device__ TPCell<NSP> tp_fetch(const TPArgs& A, int i, int j, int k)
     else src = plus ? n - layer : layer - 1;
     int si = i, sj = j, sk = k;
     if (dir == 0) si = src; else if (dir == 1) sj = src; else sk = src;
-    TPCell<NSP> s = tp_load<NSP>(A, ((long)sk * A.nj + sj) * A.ni + si);
-    if (type == BC_SLIP_WALL) s.v[dir] = -(s.v[dir]);
+    return TPRef{A.Pin + (((long)sk * A.nj + sj) * A.ni + si), A.qs, (type == BC_SLIP_WALL) ? dir : -1};
+}
+
+__device__ __forceinline__ double tp_val(const TPRef& r, int field) { return r.base[field * r.stride]; }
+
+// velocity component comp (global frame) with the slip-wall reflection
+__device__ __forceinline__ double tp_vel(const TPRef& r, int comp)
+{
+    const double v = r.base[(P_VX + comp) * r.stride];
+    return (r.flip == comp) ? -v : v;
+}
+
+// The whole state of a stencil position, velocity in the face frame (pn, pt1, pt2).
+template <int NSP>
+__device__ __forceinline__ TPCell<NSP> tp_cell(const TPRef& r, int pn, int pt1, int pt2)
+{
+    TPCell<NSP> s;
+    s.rho = tp_val(r, P_RHO); s.p = tp_val(r, P_P); s.T = tp_val(r, P_T); s.u = tp_val(r, P_U); s.a = tp_val(r, P_A);
+    s.v[0] = tp_vel(r, pn); s.v[1] = tp_vel(r, pt1); s.v[2] = tp_vel(r, pt2);
+#pragma unroll
+    for (int sp = 0; sp < NSP; ++sp) s.Y[sp] = tp_val(r, P_Y + sp);
     return s;
 }
 
@@ -234,79 +234,111 @@ __global__ __launch_bounds__(128) void tp_face_kernel(TPArgs A)
     constexpr bool kPolish = TPArith<M>::polish;
     const int nfi = A.ni + (DIR == 0), nfj = A.nj + (DIR == 1), nfk = (A.dims == 3) ? A.nk + (DIR == 2) : 1;
     const long nf = (long)nfi * nfj * nfk;
-    const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nf) return;
+    const long f0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = f0 < nf;
+    const long f = active ? f0 : nf - 1;   // idle lanes compute the last face and store nothing
     const int i = (int)(f % nfi);
     const int j = (int)((f / nfi) % nfj);
     const int k = (int)(f / ((long)nfi * nfj));
     const int di = (DIR == 0), dj = (DIR == 1), dk = (DIR == 2);
     // face frame permutation: i (x,y,z); j 3D (y,z,x), 2D (y,x,z); k (z,x,y)
-    const int pn = DIR, pt1 = (DIR == 0) ? 1 : (DIR == 1) ? ((A.dims == 3) ? 2 : 0) : 0;
-    const int pt2 = (DIR == 0) ? 2 : (DIR == 1) ? ((A.dims == 3) ? 0 : 2) : 1;
-    TPCell<NSP> c[4];
+    const bool d3 = (A.dims == 3);
+    const int pn = DIR, pt1 = (DIR == 0) ? 1 : (DIR == 1) ? (d3 ? 2 : 0) : 0;
+    const int pt2 = (DIR == 0) ? 2 : (DIR == 1) ? (d3 ? 0 : 2) : 1;
+    // the stencil's four positions; values are loaded one scalar at a time
+    TPRef r[4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        TPCell<NSP> s = tp_fetch<NSP>(A, i + (m - 2) * di, j + (m - 2) * dj, k + (m - 2) * dk);
-        double v0 = s.v[pn], v1 = s.v[pt1], v2 = s.v[pt2];
-        s.v[0] = v0; s.v[1] = v1; s.v[2] = v2;
-        c[m] = s;
-    }
-    TPCell<NSP> Lf = c[1], Rf = c[2];
-    int fe = 0;
+    for (int m = 0; m < 4; ++m) r[m] = tp_ref(A, i + (m - 2) * di, j + (m - 2) * dj, k + (m - 2) * dk);
+    TPCell<NSP> Lf, Rf;
+    int fe = 0, fb = 0;
+    bool okL = false, okR = false;
     if (A.high_order) {
         const L2R2W& w = A.w[DIR];
         const bool lim = A.apply_limiter, clip = A.extrema_clipping;
+        const int comp[3] = {pn, pt1, pt2};
 #pragma unroll
         for (int d = 0; d < 3; ++d)
-            interp_l2r2_scalar<MA>(w, lim, clip, c[0].v[d], c[1].v[d], c[2].v[d], c[3].v[d], Lf.v[d], Rf.v[d]);
+            interp_l2r2_scalar<MA>(w, lim, clip, tp_vel(r[0], comp[d]), tp_vel(r[1], comp[d]), tp_vel(r[2], comp[d]),
+                                   tp_vel(r[3], comp[d]), Lf.v[d], Rf.v[d]);
 #pragma unroll
         for (int s = 0; s < NSP; ++s)
-            interp_l2r2_scalar<MA>(w, lim, clip, c[0].Y[s], c[1].Y[s], c[2].Y[s], c[3].Y[s], Lf.Y[s], Rf.Y[s]);
+            interp_l2r2_scalar<MA>(w, lim, clip, tp_val(r[0], P_Y + s), tp_val(r[1], P_Y + s), tp_val(r[2], P_Y + s),
+                                   tp_val(r[3], P_Y + s), Lf.Y[s], Rf.Y[s]);
         if (scale_mass_fractions<MA, NSP>(Lf.Y) != 0) {
 #pragma unroll
-            for (int s = 0; s < NSP; ++s) Lf.Y[s] = c[1].Y[s];
+            for (int s = 0; s < NSP; ++s) Lf.Y[s] = tp_val(r[1], P_Y + s);
         }
         if (scale_mass_fractions<MA, NSP>(Rf.Y) != 0) {
 #pragma unroll
-            for (int s = 0; s < NSP; ++s) Rf.Y[s] = c[2].Y[s];
+            for (int s = 0; s < NSP; ++s) Rf.Y[s] = tp_val(r[2], P_Y + s);
         }
-        interp_l2r2_scalar<MA>(w, lim, clip, c[0].rho, c[1].rho, c[2].rho, c[3].rho, Lf.rho, Rf.rho);
-        interp_l2r2_scalar<MA>(w, lim, clip, c[0].u, c[1].u, c[2].u, c[3].u, Lf.u, Rf.u);
-        // update_thermo_from_rhou on each side; on failure the side is the cell's state
+        interp_l2r2_scalar<MA>(w, lim, clip, tp_val(r[0], P_RHO), tp_val(r[1], P_RHO), tp_val(r[2], P_RHO),
+                               tp_val(r[3], P_RHO), Lf.rho, Rf.rho);
+        interp_l2r2_scalar<MA>(w, lim, clip, tp_val(r[0], P_U), tp_val(r[1], P_U), tp_val(r[2], P_U),
+                               tp_val(r[3], P_U), Lf.u, Rf.u);
+        // update_thermo_from_rhou on each side (guess: the side's cell T); on
+        // failure the side is the cell's state (onedinterp.d:1039-1068); the
+        // sound speed is the cell's (sfluidblock.d:2199-2208).  The polish of
+        // the mixed mode starts from T reconstructed like rho and u instead,
+        // which is within a few kelvin of the face's root where the field is
+        // smooth (the cell T can be hundreds of kelvin away).
+        double TgL = tp_val(r[1], P_T), TgR = tp_val(r[2], P_T);
+        if constexpr (kPolish) {
+            double tl, tr;
+            interp_l2r2_scalar<MA>(w, lim, clip, tp_val(r[0], P_T), TgL, TgR, tp_val(r[3], P_T), tl, tr);
+            TgL = (tl > 20.0 && tl < 100000.0) ? tl : TgL;
+            TgR = (tr > 20.0 && tr < 100000.0) ? tr : TgR;
+        }
         TPState<NSP> Q;
-        Q.rho = Lf.rho; Q.T = Lf.T; Q.u = Lf.u;
+        Q.rho = Lf.rho; Q.T = TgL; Q.u = Lf.u;
 #pragma unroll
         for (int s = 0; s < NSP; ++s) Q.Y[s] = Lf.Y[s];
-        if (tp_face_eos<MA, NSP, kPolish>(*A.gas, Q, &fe) == 0) { Lf.T = Q.T; Lf.u = Q.u; Lf.p = Q.p; }
-        else Lf = c[1];
-        Q.rho = Rf.rho; Q.T = Rf.T; Q.u = Rf.u;
+        if (tp_face_eos<MA, NSP, kPolish>(*A.gas, Q, &fe, &fb) == 0) {
+            Lf.T = Q.T; Lf.u = Q.u; Lf.p = Q.p; Lf.a = tp_val(r[1], P_A); okL = true;
+        }
+        Q.rho = Rf.rho; Q.T = TgR; Q.u = Rf.u;
 #pragma unroll
         for (int s = 0; s < NSP; ++s) Q.Y[s] = Rf.Y[s];
-        if (tp_face_eos<MA, NSP, kPolish>(*A.gas, Q, &fe) == 0) { Rf.T = Q.T; Rf.u = Q.u; Rf.p = Q.p; }
-        else Rf = c[2];
+        if (tp_face_eos<MA, NSP, kPolish>(*A.gas, Q, &fe, &fb) == 0) {
+            Rf.T = Q.T; Rf.u = Q.u; Rf.p = Q.p; Rf.a = tp_val(r[2], P_A); okR = true;
+        }
     }
+    if (!okL) Lf = tp_cell<NSP>(r[1], pn, pt1, pt2);
+    if (!okR) Rf = tp_cell<NSP>(r[2], pn, pt1, pt2);
     double F[5 + NSP];
     if constexpr (FLUX == FLUX_HLLC) tp_hllc<MA, NSP>(Lf, Rf, F);
     else tp_ausmdv<MA, NSP>(Lf, Rf, F);
-    double* dst = A.F[DIR];
-    const long fq = A.fqs[DIR];
-    dst[f] = F[0];
-    dst[(1 + pn) * fq + f] = F[1];
-    dst[(1 + pt1) * fq + f] = F[2];
-    dst[(1 + pt2) * fq + f] = F[3];
-    dst[4 * fq + f] = F[4];
+    if (active) {
+        double* dst = A.F[DIR];
+        const long fq = A.fqs[DIR];
+        dst[f] = F[0];
+        dst[(1 + pn) * fq + f] = F[1];
+        dst[(1 + pt1) * fq + f] = F[2];
+        dst[(1 + pt2) * fq + f] = F[3];
+        dst[4 * fq + f] = F[4];
 #pragma unroll
-    for (int s = 0; s < NSP; ++s) dst[(5 + s) * fq + f] = F[5 + s];
+        for (int s = 0; s < NSP; ++s) dst[(5 + s) * fq + f] = F[5 + s];
+    }
+    if (!active) { fe = 0; fb = 0; }
+    for (int off = 32; off > 0; off >>= 1) {
+        fe += __shfl_xor(fe, off);
+        fb += __shfl_xor(fb, off);
+    }
+    if (__lane_id() == 0) {
+        if (fe) atomicAdd(&A.counters[4], (unsigned int)fe);
+        if (fb) atomicAdd(&A.counters[5], (unsigned int)fb);
+    }
 }
 
-template <class M, int NSP>
+// GUESS: A.guess is installed (the learned temperature guess; its LDS staging
+// and MLP code exist only in that instantiation).
+template <class M, int NSP, bool GUESS>
 __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
 {
     using MA = typename TPArith<M>::type;
     constexpr bool kPolish = TPArith<M>::polish;
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
     constexpr int NQ = 5 + NSP;
-    __shared__ __bf16 gm_lds[2 * GM_LDS_PER_WAVE];
     const long ncells = (long)A.ni * A.nj * A.nk;
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = c < ncells;
@@ -352,6 +384,7 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
             if (A.stage == 1) Un[q] = uold + A.c1 * d;
             else if (A.stage == 2) Un[q] = uold + A.dt * (A.g1 * dd0 + A.g2 * d);
             else Un[q] = uold + A.dt * (A.g1 * dd0 + A.g2 * dd1 + A.g3 * d);
+            A.Unew[c + q * qs] = Un[q];   // stored now; the low-T fallback below re-stores what it changes
         }
         // decode_conserved (fvcell.d:563-732): density, velocity, u, mass fractions
         rho = Un[0];
@@ -373,7 +406,8 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
     }
     // learned initial guess for the whole wave (all lanes take part)
     double Tguess = 0.0;
-    if (A.guess) {
+    if constexpr (GUESS) {
+        __shared__ __bf16 gm_lds[2 * GM_LDS_PER_WAVE];
         double x[7] = {have_state ? Q.rho : 1.0, have_state ? Q.u : 1.0e6, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int s = 0; s < 5 && s < NSP; ++s) x[2 + s] = have_state ? Q.Y[s] : 0.2;
@@ -382,7 +416,7 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
     if (have_state) {
         const double e_tgt = Q.u;
         int rc;
-        if (A.guess) {
+        if constexpr (GUESS) {
             rc = tp_polish<MA, NSP>(*A.gas, Q, e_tgt, Tguess, &fe);
             if (rc == 0) {
                 double Rmix = tp_Rmix<NSP>(*A.gas, Q.Y);
@@ -395,17 +429,20 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
             }
         } else {
             Q.T = A.Pin[P_T * qs + c];     // the previous temperature is the Newton guess
-            rc = tp_face_eos<MA, NSP, kPolish>(*A.gas, Q, &fe);
+            int nfb_cell = 0;
+            rc = tp_face_eos<MA, NSP, kPolish>(*A.gas, Q, &fe, &nfb_cell);
+            fell_back = nfb_cell != 0;
         }
         if (rc != 0) {
             // ignore_low_T_thermo_update_failure: T = 200 K and re-encode U
             Q.T = 200.0;
             tp_update_thermo_from_rhoT<MA, NSP>(*A.gas, Q);
-            Un[1] = rho * v[0]; Un[2] = rho * v[1]; Un[3] = rho * v[2];
+            double* Un_out = A.Unew + c;
+            Un_out[qs] = rho * v[0]; Un_out[2 * qs] = rho * v[1]; Un_out[3 * qs] = rho * v[2];
             double ke2 = 0.5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-            Un[4] = rho * (Q.u + ke2);
+            Un_out[4 * qs] = rho * (Q.u + ke2);
 #pragma unroll
-            for (int s = 0; s < NSP; ++s) Un[5 + s] = rho * Q.Y[s];
+            for (int s = 0; s < NSP; ++s) Un_out[(5 + s) * qs] = rho * Q.Y[s];
         }
         Q.a = tp_sound_speed<MA, NSP>(*A.gas, Q);
         // check_data (flowstate.d:345-388)
@@ -423,10 +460,6 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
         P[P_U * qs + c] = Q.u; P[P_T * qs + c] = Q.T; P[P_P * qs + c] = Q.p; P[P_A * qs + c] = Q.a;
 #pragma unroll
         for (int s = 0; s < NSP; ++s) P[(P_Y + s) * qs + c] = Q.Y[s];
-    }
-    if (active) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) A.Unew[c + q * qs] = Un[q];
     }
     unsigned int nbad = (unsigned int)__popcll(__ballot(invalid));
     unsigned int nfail = (unsigned int)__popcll(__ballot(decode_fail));

@@ -144,8 +144,16 @@ def device_guess(gas, X):
 
 
 def decode_stats(reset=True):
+    """(energy evaluations, polish fall-backs) of the decode temperature solves."""
     ev, fb = C.c_longlong(0), C.c_longlong(0)
     _lib.check(_lib.load().e4hip_tp_decode_stats(C.byref(ev), C.byref(fb), int(reset)))
+    return ev.value, fb.value
+
+
+def face_stats(reset=True):
+    """(energy evaluations, polish fall-backs) of the face-state temperature solves."""
+    ev, fb = C.c_longlong(0), C.c_longlong(0)
+    _lib.check(_lib.load().e4hip_tp_face_stats(C.byref(ev), C.byref(fb), int(reset)))
     return ev.value, fb.value
 
 

@@ -21,7 +21,8 @@ EXPORTS = [
     "e4hip_block_connect",
     "e4hip_block_upload", "e4hip_block_download", "e4hip_block_ncells", "e4hip_config",
     "e4hip_explicit_increment", "e4hip_determine_dt", "e4hip_profile_enable", "e4hip_profile_read",
-    "e4hip_gas_tp_set_guess_model", "e4hip_tp_guess", "e4hip_tp_decode_stats", "e4hip_mlp_grad", "e4hip_mlp_train",
+    "e4hip_gas_tp_set_guess_model", "e4hip_tp_guess", "e4hip_tp_decode_stats", "e4hip_tp_face_stats",
+    "e4hip_mlp_grad", "e4hip_mlp_train",
 ]
 
 
@@ -75,6 +76,7 @@ def load(path=LIB_PATH):
     L.e4hip_gas_tp_set_guess_model.argtypes = [C.c_int, _fp, _fp, _fp, C.c_float, C.c_float]
     L.e4hip_tp_guess.argtypes = [C.c_int, C.c_int, _dp, _dp]
     L.e4hip_tp_decode_stats.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.c_int]
+    L.e4hip_tp_face_stats.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.c_int]
     L.e4hip_mlp_grad.argtypes = [_fp, _fp, _fp, C.c_int, _fp, _fp]
     L.e4hip_mlp_train.argtypes = [_fp, _fp, _fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _fp]
     _lib = L

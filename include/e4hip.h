@@ -140,9 +140,14 @@ int e4hip_gas_tp_set_guess_model(int gm, const float* params, const float* in_me
                                  float T_mean, float T_std);
 /* Guesses for n states x[n][7] (rho, u, Y0..Y4) -> T[n] (host arrays). */
 int e4hip_tp_guess(int gm, int n, const double* x, double* T);
-/* Energy evaluations and learned-guess fallbacks of the decode temperature
- * solves since the last reset (summed over blocks and stages). */
+/* Energy evaluations of the decode temperature solves, and the decodes whose
+ * polish (mixed mode or learned guess) fell back to the bracketed solve, since
+ * the last reset (summed over blocks and stages).  No counterpart in the
+ * reference: work counts of the solves that replace therm_perf_gas_mix_eos.d:70-153. */
 int e4hip_tp_decode_stats(long long* energy_evals, long long* fallbacks, int reset);
+/* The same for the face-state solves of the reconstruction (two per face,
+ * onedinterp.d:534-551); fallbacks count only in the mixed mode. */
+int e4hip_tp_face_stats(long long* energy_evals, long long* fallbacks, int reset);
 /* Training of the guess network with this library's own kernels (FP32).
  * X[n][8] normalised features (8th = 0), T[n] normalised targets.
  * e4hip_mlp_grad: mean squared error and its gradient for one batch.
