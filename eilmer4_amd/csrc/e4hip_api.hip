@@ -384,6 +384,12 @@ int launch_stage_batch(const std::vector<const BlockH*>& blks, const std::vector
 
 // Stage sequencing of the thermally perfect path: the same U/dUdt buffer
 // use as launch_stage; the primitive state goes P[0] -> P[1] -> P[0] ...
+// Whether a mixture can use the fast modes' mixture form (tp_gas_device.h).
+bool tp_mixture_form(const e4::TPGas& g)
+{
+    return g.shared != 0 && g.curve[0].nseg == e4::TP_FAST_NSEG;
+}
+
 int launch_tp_stage(BlockH& b, int stage, double dt, unsigned int* counters)
 {
     const E4hipStepConfig& cfg = g_state.cfg;
@@ -456,7 +462,10 @@ int launch_tp_stage(BlockH& b, int stage, double dt, unsigned int* counters)
     const int flux = (cfg.flux_calculator == E4HIP_FLUX_HLLC) ? e4::FLUX_HLLC : e4::FLUX_AUSMDV;
     hipEvent_t ev = nullptr;
     prof_begin(&ev);
-    if (cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT) e4::launch_tp_stage_exact(A, g_state.stream, flux);
+    // the fast modes evaluate the mixture form, which needs three-segment curves
+    // with shared break points; other mixtures run the reference-form kernels
+    const bool mix_form = tp_mixture_form(g_state.gases[b.gm].tp);
+    if (cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT || !mix_form) e4::launch_tp_stage_exact(A, g_state.stream, flux);
     else if (cfg.precision_mode == E4HIP_PRECISION_MIXED) e4::launch_tp_stage_mixed(A, g_state.stream, flux);
     else e4::launch_tp_stage_fast64(A, g_state.stream, flux);
     prof_end(0, ev);
@@ -851,7 +860,8 @@ int e4hip_block_upload(int blk, const double* U, const double* Tguess)
         e4::TPArgs A = tp_args_for(*b);
         A.Pout = b->P[0];
         A.guess = g_state.cfg.learned_eos_guess ? g_state.gases[b->gm].guess_dev : nullptr;
-        if (g_state.cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT) e4::launch_tp_decode_exact(A, b->U[0], Tg, g_state.stream);
+        if (g_state.cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT || !tp_mixture_form(g_state.gases[b->gm].tp))
+            e4::launch_tp_decode_exact(A, b->U[0], Tg, g_state.stream);
         else e4::launch_tp_decode_fast64(A, b->U[0], Tg, g_state.stream);
         HIP_TRY(hipGetLastError());
         unsigned int cnt[e4::COUNTER_SLOT] = {};

@@ -96,65 +96,68 @@ __device__ __forceinline__ float mlp_guess_wave(const GuessModel& G, const doubl
 // temperature.  Returns 0 when accepted, -1 to fall back to the reference's
 // bracketed solve.
 //
-// Fast modes with shared breakpoints: Halley steps (cubic convergence) on the
-// exact u, du/dT and d2u/dT2 of the blended curves, the species sums computed
-// once per curve piece.  From a face's cell temperature (typically tens to a
-// few hundred kelvin from the face root) two Halley steps reach the judge's
-// bound; Newton would need three.  Otherwise: Newton steps with the
-// reference's dzdT, sum_s Y_s (Cp_s - R_s), which inside a blend is not the
+// Fast modes (which require curves with shared breakpoints): Halley steps
+// (cubic convergence) on the exact u, du/dT and d2u/dT2 of the blended
+// curves, the species sums of every segment formed once per solve.  From a face's cell
+// temperature (typically tens to a few hundred kelvin from the face root) two
+// Halley steps reach the judge's bound; Newton would need three.  ModeExact:
+// Newton steps with the reference's dzdT, sum_s Y_s (Cp_s - R_s), which inside a blend is not the
 // derivative of the blended energy (convergence there is only linear, hence a
 // judge on the residual rather than on the size of the last step).
 template <class M, int NSP>
-__device__ __forceinline__ int tp_polish(const TPGas& g, TPState<NSP>& Q, double e_tgt, double T0, int* fevals)
+__device__ __forceinline__ int tp_polish_x(const TPGas& g, const TPMix& X, double Rmix, TPState<NSP>& Q, double e_tgt,
+                                           double T0, int* fevals)
 {
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
     if (!(T0 >= 20.0 && T0 <= 100000.0)) return -1;
     if constexpr (M::kFast) {
-        if (g.shared) {
-            const CeaCurve& c0 = g.curve[0];
-            const double Tlo = c0.Tb[0], Thi = c0.Tb[c0.nseg];
-            const double Rmix = tp_Rmix<NSP>(g, Q.Y);
-            TPPiece P;
-            P.lo = 1.0; P.hi = 0.0;   // empty: the first evaluation finds its piece
-            double T = T0;
+        double T = T0;
 #pragma nounroll
-            for (int it = 0;; ++it) {
-                if (it == 2 && !(T >= 20.0 && T <= 100000.0)) return -1;
-                const double Tc = fmin(fmax(T, Tlo), Thi);
-                if (!(Tc >= P.lo && Tc <= P.hi)) tp_mix_piece<NSP>(g, Q.Y, Tc, P);
-                double u, du, d2u;
-                tp_piece_eval(P, Rmix, Tlo, Thi, T, &u, &du, &d2u);
-                (*fevals)++;
-                const double f = u - e_tgt;
-                if (it == 2) {
-                    const double err = fabs(dv<MD>(f, du));
-                    if (!(err <= 1.0e-11 * T)) return -1;
-                    Q.T = T;
-                    Q.u = u;
-                    return 0;
-                }
-                T -= dv<MD>(2.0 * f * du, 2.0 * du * du - f * d2u);
+        for (int it = 0;; ++it) {
+            if (it == 2 && !(T >= 20.0 && T <= 100000.0)) return -1;
+            double u, du, d2u;
+            tp_mix_eval<true>(X, g.curve[0], Rmix, T, &u, &du, &d2u);
+            (*fevals)++;
+            const double f = u - e_tgt;
+            if (it == 2) {
+                if (!(fabs(f) <= 1.0e-11 * T * fabs(du))) return -1;
+                Q.T = T;
+                Q.u = u;
+                return 0;
             }
+            T -= dv<MD>(2.0 * f * du, 2.0 * du * du - f * d2u);
         }
-    }
-    // two Newton steps, then the energy at the result (one evaluation site)
-    double T = T0, df = 1.0;
+    } else {
+        // two Newton steps, then the energy at the result (one evaluation site)
+        double T = T0, df = 1.0;
 #pragma nounroll
-    for (int it = 0;; ++it) {
-        if (it == 2 && !(T >= 20.0 && T <= 100000.0)) return -1;
-        double u, dz;
-        tp_u_dzdT<M, NSP>(g, Q.Y, T, it < 2, &u, &dz);
-        (*fevals)++;
-        if (it == 2) {
-            const double err = fabs(dv<MD>(e_tgt - u, df));
-            if (!(err <= 1.0e-11 * T)) return -1;
-            Q.T = T;
-            Q.u = u;
-            return 0;
+        for (int it = 0;; ++it) {
+            if (it == 2 && !(T >= 20.0 && T <= 100000.0)) return -1;
+            double u, dz;
+            tp_u_dzdT<M, NSP>(g, Q.Y, T, it < 2, &u, &dz);
+            (*fevals)++;
+            if (it == 2) {
+                if (!(fabs(e_tgt - u) <= 1.0e-11 * T * fabs(df))) return -1;
+                Q.T = T;
+                Q.u = u;
+                return 0;
+            }
+            df = dz;
+            T -= dv<MD>(e_tgt - u, df);
         }
-        df = dz;
-        T -= dv<MD>(e_tgt - u, df);
     }
+}
+
+template <class M, int NSP>
+__device__ __forceinline__ int tp_polish(const TPGas& g, TPState<NSP>& Q, double e_tgt, double T0, int* fevals)
+{
+    TPMix X;
+    double Rmix = 0.0;
+    if constexpr (M::kFast) {
+        tp_mix_all<NSP>(g, Q.Y, X);
+        Rmix = tp_Rmix<NSP>(g, Q.Y);
+    }
+    return tp_polish_x<M, NSP>(g, X, Rmix, Q, e_tgt, T0, fevals);
 }
 
 } // namespace e4

@@ -192,78 +192,84 @@ __device__ __forceinline__ void tp_mix_u_cv(const TPGas& g, const double* Y, dou
     *cvmix = cp - Rmix;
 }
 
-// One piece of the mixture's blended curves (fast modes, shared breakpoints):
-// for T in [lo, hi] the mixture coefficients are M + (T - t0) dM, with dM = 0
-// outside the blend ranges (cea_thermo_curves.d:136-169 applied to the
-// mixture).  A solve that stays on one piece computes the species sums once.
-struct TPPiece {
-    double lo, hi, t0;
-    double M[8], dM[8];
-    int blend;
+// The fast modes' curves: three segments (two blend ranges), every species on
+// the same breakpoints; other mixtures run the reference-form kernels.
+constexpr int TP_FAST_NSEG = 3;
+
+// The mixture's coefficient sums of every segment, M[q][k] = sum_s Y_s R_s
+// a_{s,q,k}: formed once per temperature solve (the composition is fixed
+// during the solve), so an evaluation at any T only weights the three sets.
+struct TPMix {
+    double M[TP_FAST_NSEG][8];
 };
 
 template <int NSP>
-__device__ __forceinline__ void tp_mix_piece(const TPGas& g, const double* Y, double Tc, TPPiece& P)
+__device__ __forceinline__ void tp_mix_all(const TPGas& g, const double* Y, TPMix& X)
 {
-    const CeaCurve& cv = g.curve[0];
-    const int nb = cv.nseg + 1;
-    int s0 = -1, s1 = -1;
-    double lo = -1.0e300, hi = 1.0e300;
-    if (Tc < (cv.Tb[1] - 0.5 * cv.Tbl[0])) {
-        s0 = 0;
-        hi = cv.Tb[1] - 0.5 * cv.Tbl[0];
-    } else if (Tc > (cv.Tb[nb - 2] + 0.5 * cv.Tbl[nb - 3])) {
-        s0 = cv.nseg - 1;
-        lo = cv.Tb[nb - 2] + 0.5 * cv.Tbl[nb - 3];
-    } else {
-        for (int i = 1; i < nb - 1; ++i) {
-            const double blo = cv.Tb[i] - 0.5 * cv.Tbl[i - 1], bhi = cv.Tb[i] + 0.5 * cv.Tbl[i - 1];
-            if (Tc >= blo && Tc <= bhi) { s0 = i - 1; s1 = i; lo = blo; hi = bhi; break; }
-            const double next = cv.Tb[i + 1] - 0.5 * cv.Tbl[i];
-            if (Tc > bhi && Tc < next) { s0 = i; lo = bhi; hi = next; break; }
-        }
-        if (s0 < 0) { s0 = 0; lo = Tc; hi = Tc; }   // unreachable for Tc inside the curve range
-    }
-    P.lo = lo; P.hi = hi; P.t0 = lo;
-    P.blend = s1 >= 0;
-    const double rbl = (s1 >= 0) ? dv_seed(1.0, cv.Tbl[s1 - 1], rcp_seed(cv.Tbl[s1 - 1])) : 0.0;
-    const int sb = (s1 >= 0) ? s1 : s0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        double a = 0.0, b = 0.0;
+    for (int q = 0; q < TP_FAST_NSEG; ++q)
 #pragma unroll
-        for (int s = 0; s < NSP; ++s) {
-            a += Y[s] * g.RC[s0][k][s];
-            b += Y[s] * g.RC[sb][k][s];
+        for (int k = 0; k < 8; ++k) {
+            double m = 0.0;
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) m += Y[s] * g.RC[q][k][s];
+            X.M[q][k] = m;
         }
-        P.M[k] = a;
-        P.dM[k] = (b - a) * rbl;
-    }
 }
 
-// u, du/dT and d2u/dT2 of the mixture at T from its piece (exact derivatives
-// of the blended curves; constant-Cp extension outside the curve range,
-// cea_thermo_curves.d:63-91).
-__device__ __forceinline__ void tp_piece_eval(const TPPiece& P, double Rmix, double Tlo, double Thi, double T,
-                                              double* u, double* du, double* d2u)
+// Weights of the three segments at Tc and their T derivatives: one segment,
+// or the linear blend of two inside a blend range (cea_thermo_curves.d:136-169,
+// the same comparisons).
+__device__ __forceinline__ void tp_seg_weights(const CeaCurve& cv, double Tc, double w[3], double dw[3])
 {
+    const double lo1 = cv.Tb[1] - 0.5 * cv.Tbl[0], hi1 = cv.Tb[1] + 0.5 * cv.Tbl[0];
+    const double lo2 = cv.Tb[2] - 0.5 * cv.Tbl[1], hi2 = cv.Tb[2] + 0.5 * cv.Tbl[1];
+    const double r1 = dv_seed(1.0, cv.Tbl[0], rcp_seed(cv.Tbl[0]));
+    const double r2 = dv_seed(1.0, cv.Tbl[1], rcp_seed(cv.Tbl[1]));
+    const bool in1 = Tc >= lo1 && Tc <= hi1, in2 = Tc >= lo2 && Tc <= hi2;
+    const double wb = in1 ? r1 * (Tc - lo1) : r2 * (Tc - lo2);
+    const double rb = in1 ? r1 : r2;
+    w[0] = (Tc < lo1) ? 1.0 : (in1 ? 1.0 - wb : 0.0);
+    w[1] = in1 ? wb : (in2 ? 1.0 - wb : ((Tc > hi1 && Tc < lo2) ? 1.0 : 0.0));
+    w[2] = in2 ? wb : ((Tc > hi2) ? 1.0 : 0.0);
+    dw[0] = in1 ? -rb : 0.0;
+    dw[1] = in1 ? rb : (in2 ? -rb : 0.0);
+    dw[2] = in2 ? rb : 0.0;
+}
+
+// u, du/dT and d2u/dT2 of the mixture at T from its segment sums.  SLOPE:
+// du/dT is the exact derivative of the blended curve, which inside a blend
+// adds (dw_B/dT)(h_B - h_A) to the blended Cv; without it du/dT is the blended
+// Cv (the reference's dzdT, therm_perf_gas_mix_eos.d:116-123) and d2u is not
+// formed.  Outside the curve range: the constant-Cp extension
+// (cea_thermo_curves.d:63-91).
+template <bool SLOPE>
+__device__ __forceinline__ void tp_mix_eval(const TPMix& X, const CeaCurve& c0, double Rmix, double T, double* u,
+                                            double* du, double* d2u)
+{
+    const double Tlo = c0.Tb[0], Thi = c0.Tb[TP_FAST_NSEG];
     const double Tc = fmin(fmax(T, Tlo), Thi);
-    const double w = Tc - P.t0;
+    double w[3], dw[3];
+    tp_seg_weights(c0, Tc, w, dw);
     double Mk[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) Mk[k] = P.M[k] + w * P.dM[k];
+    for (int k = 0; k < 8; ++k) Mk[k] = w[0] * X.M[0][k] + w[1] * X.M[1][k] + w[2] * X.M[2][k];
     const double r = dv_seed(1.0, Tc, rcp_seed(Tc));
     const double lnT = log(Tc);
     const double T2 = Tc * Tc, T3 = T2 * Tc, T4 = T2 * T2;
     double h = tp_mix_h(Mk, Tc, r, lnT);
     const double cp = Mk[0] * r * r + Mk[1] * r + Mk[2] + Mk[3] * Tc + Mk[4] * T2 + Mk[5] * T3 + Mk[6] * T4;
-    double dh = cp;
-    double d2h = -2.0 * Mk[0] * r * r * r - Mk[1] * r * r + Mk[3] + 2.0 * Mk[4] * Tc + 3.0 * Mk[5] * T2 +
-                 4.0 * Mk[6] * T3;
-    if (P.blend) {
-        dh += tp_mix_h(P.dM, Tc, r, lnT);
-        d2h += 2.0 * (P.dM[0] * r * r + P.dM[1] * r + P.dM[2] + P.dM[3] * Tc + P.dM[4] * T2 + P.dM[5] * T3 +
-                      P.dM[6] * T4);
+    double dh = cp, d2h = 0.0;
+    if constexpr (SLOPE) {
+        d2h = -2.0 * Mk[0] * r * r * r - Mk[1] * r * r + Mk[3] + 2.0 * Mk[4] * Tc + 3.0 * Mk[5] * T2 +
+              4.0 * Mk[6] * T3;
+        if (dw[0] != 0.0 || dw[1] != 0.0 || dw[2] != 0.0) {
+            double dM[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) dM[k] = dw[0] * X.M[0][k] + dw[1] * X.M[1][k] + dw[2] * X.M[2][k];
+            dh += tp_mix_h(dM, Tc, r, lnT);
+            d2h += 2.0 * (dM[0] * r * r + dM[1] * r + dM[2] + dM[3] * Tc + dM[4] * T2 + dM[5] * T3 + dM[6] * T4);
+        }
     }
     if (T < Tlo) { h -= cp * (Tlo - T); dh = cp; d2h = 0.0; }
     else if (T > Thi) { h += cp * (T - Thi); dh = cp; d2h = 0.0; }
@@ -277,19 +283,18 @@ template <class M, int NSP>
 __device__ __forceinline__ double tp_energy(const TPGas& g, const double* Y, double T)
 {
     if constexpr (M::kFast) {
-        if (g.shared) {
-            double u, cv;
-            tp_mix_u_cv<NSP>(g, Y, tp_Rmix<NSP>(g, Y), T, &u, &cv);
-            return u;
-        }
-    }
-    double result = 0.0;
+        double u, cv;
+        tp_mix_u_cv<NSP>(g, Y, tp_Rmix<NSP>(g, Y), T, &u, &cv);
+        return u;
+    } else {
+        double result = 0.0;
 #pragma unroll
-    for (int s = 0; s < NSP; ++s) {
-        double e = cea_h<M>(g.curve[s], T) - g.R[s] * T;
-        result += Y[s] * e;
+        for (int s = 0; s < NSP; ++s) {
+            double e = cea_h<M>(g.curve[s], T) - g.R[s] * T;
+            result += Y[s] * e;
+        }
+        return result;
     }
-    return result;
 }
 
 // -(sum Y_s (Cp_s - R_s)), the dzdT closure of therm_perf_gas_mix_eos.d:116-123
@@ -297,19 +302,18 @@ template <class M, int NSP>
 __device__ __forceinline__ double tp_dzdT(const TPGas& g, const double* Y, double T)
 {
     if constexpr (M::kFast) {
-        if (g.shared) {
-            double u, cv;
-            tp_mix_u_cv<NSP>(g, Y, tp_Rmix<NSP>(g, Y), T, &u, &cv);
-            return -1.0 * cv;
-        }
-    }
-    double result = 0.0;
+        double u, cv;
+        tp_mix_u_cv<NSP>(g, Y, tp_Rmix<NSP>(g, Y), T, &u, &cv);
+        return -1.0 * cv;
+    } else {
+        double result = 0.0;
 #pragma unroll
-    for (int s = 0; s < NSP; ++s) {
-        double Cv = cea_Cp<M>(g.curve[s], T) - g.R[s];
-        result += Y[s] * Cv;
+        for (int s = 0; s < NSP; ++s) {
+            double Cv = cea_Cp<M>(g.curve[s], T) - g.R[s];
+            result += Y[s] * Cv;
+        }
+        return -1.0 * result;
     }
-    return -1.0 * result;
 }
 
 // Mixture energy at T and, when want_d, the dzdT closure's value
@@ -320,15 +324,13 @@ __device__ __forceinline__ void tp_u_dzdT(const TPGas& g, const double* Y, doubl
                                           double* dz)
 {
     if constexpr (M::kFast) {
-        if (g.shared) {
-            double cv;
-            tp_mix_u_cv<NSP>(g, Y, tp_Rmix<NSP>(g, Y), T, u, &cv);
-            *dz = -1.0 * cv;
-            return;
-        }
+        double cv;
+        tp_mix_u_cv<NSP>(g, Y, tp_Rmix<NSP>(g, Y), T, u, &cv);
+        *dz = -1.0 * cv;
+    } else {
+        *u = tp_energy<M, NSP>(g, Y, T);
+        *dz = want_d ? tp_dzdT<M, NSP>(g, Y, T) : 0.0;
     }
-    *u = tp_energy<M, NSP>(g, Y, T);
-    *dz = want_d ? tp_dzdT<M, NSP>(g, Y, T) : 0.0;
 }
 
 // nm/newton.d:40-107 with the zeroFn/dzdT closures; Q.T and Q.u receive the
@@ -337,8 +339,8 @@ __device__ __forceinline__ void tp_u_dzdT(const TPGas& g, const double* Y, doubl
 // go through one call site in one loop, which keeps the inlined code to one
 // copy of the energy evaluation.
 template <class M, int NSP>
-__device__ __forceinline__ int tp_newton(const TPGas& g, TPState<NSP>& Q, double e_tgt, double x0, double xMin,
-                                         double xMax, double tol, double* root, int* fevals)
+__device__ __forceinline__ int tp_newton(const TPGas& g, const TPMix& X, double Rmix, TPState<NSP>& Q, double e_tgt,
+                                         double x0, double xMin, double xMax, double tol, double* root, int* fevals)
 {
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
     const int MAXIT = 30;
@@ -351,7 +353,13 @@ __device__ __forceinline__ int tp_newton(const TPGas& g, TPState<NSP>& Q, double
         // evaluation n: 0 the lower bracket end, 1 the upper, 2+ the iterates
         const double T = (n == 0) ? xMin : (n == 1) ? xMax : rts;
         double u, df0;
-        tp_u_dzdT<M, NSP>(g, Q.Y, T, n >= 2, &u, &df0);
+        if constexpr (M::kFast) {
+            double du, d2u;
+            tp_mix_eval<false>(X, g.curve[0], Rmix, T, &u, &du, &d2u);
+            df0 = -1.0 * du;
+        } else {
+            tp_u_dzdT<M, NSP>(g, Q.Y, T, n >= 2, &u, &df0);
+        }
         Q.T = T;
         Q.u = u;
         (*fevals)++;
@@ -385,9 +393,11 @@ __device__ __forceinline__ int tp_newton(const TPGas& g, TPState<NSP>& Q, double
 }
 
 // therm_perf_gas_mix_eos.d:70-153: guess = Q.T, bracket [max(T-500, 20), +1000],
-// then [20, 1e5]; tolerance 1e-6 K.
+// then [20, 1e5]; tolerance 1e-6 K.  X and Rmix: the composition's segment
+// sums and gas constant (fast modes; unused by ModeExact).
 template <class M, int NSP>
-__device__ int tp_update_temperature(const TPGas& g, TPState<NSP>& Q, int* fevals)
+__device__ __forceinline__ int tp_update_temperature_x(const TPGas& g, const TPMix& X, double Rmix, TPState<NSP>& Q,
+                                                       int* fevals)
 {
     const double Tsave = Q.T;
     const double e_tgt = Q.u;
@@ -397,11 +407,39 @@ __device__ int tp_update_temperature(const TPGas& g, TPState<NSP>& Q, int* feval
     for (int attempt = 0; attempt < 2; ++attempt) {
         const double lo = attempt ? 20.0 : T1, hi = attempt ? 100000.0 : T1 + delT;
         double root;
-        if (tp_newton<M, NSP>(g, Q, e_tgt, Tsave, lo, hi, 1.0e-6, &root, fevals) == 0) { Q.T = root; return 0; }
+        if (tp_newton<M, NSP>(g, X, Rmix, Q, e_tgt, Tsave, lo, hi, 1.0e-6, &root, fevals) == 0) { Q.T = root; return 0; }
     }
     Q.T = Tsave;
-    Q.u = tp_energy<M, NSP>(g, Q.Y, Tsave);
+    if constexpr (M::kFast) {
+        double du, d2u;
+        tp_mix_eval<false>(X, g.curve[0], Rmix, Tsave, &Q.u, &du, &d2u);
+    } else {
+        Q.u = tp_energy<M, NSP>(g, Q.Y, Tsave);
+    }
     return -1;
+}
+
+template <class M, int NSP>
+__device__ __forceinline__ int tp_update_temperature(const TPGas& g, TPState<NSP>& Q, int* fevals)
+{
+    TPMix X;
+    double Rmix = 0.0;
+    if constexpr (M::kFast) {
+        tp_mix_all<NSP>(g, Q.Y, X);
+        Rmix = tp_Rmix<NSP>(g, Q.Y);
+    }
+    return tp_update_temperature_x<M, NSP>(g, X, Rmix, Q, fevals);
+}
+
+// Frozen sound speed a = sqrt(Cp/Cv R T) (therm_perf_gas.d:345-385) from the
+// composition's segment sums (fast modes).
+template <class M>
+__device__ __forceinline__ double tp_sound_speed_x(const TPGas& g, const TPMix& X, double Rmix, double T)
+{
+    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    double u, cv, d2u;
+    tp_mix_eval<false>(X, g.curve[0], Rmix, T, &u, &cv, &d2u);
+    return sqrt(dv<MD>(cv + Rmix, cv) * (Rmix * T));
 }
 
 // therm_perf_gas.d:187-191 with perf_gas_mix_eos.d:43-49
@@ -429,20 +467,19 @@ __device__ __forceinline__ double tp_sound_speed(const TPGas& g, const TPState<N
 {
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
     if constexpr (M::kFast) {
-        if (g.shared) {
-            const double R = tp_Rmix<NSP>(g, Q.Y);
-            double u, cv;
-            tp_mix_u_cv<NSP>(g, Q.Y, R, Q.T, &u, &cv);
-            return sqrt(dv<MD>(cv + R, cv) * (R * Q.T));
-        }
+        const double R = tp_Rmix<NSP>(g, Q.Y);
+        double u, cv;
+        tp_mix_u_cv<NSP>(g, Q.Y, R, Q.T, &u, &cv);
+        return sqrt(dv<MD>(cv + R, cv) * (R * Q.T));
+    } else {
+        double Cp = 0.0, Cv = 0.0;
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) Cp += Q.Y[s] * cea_Cp<M>(g.curve[s], Q.T);
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) Cv += Q.Y[s] * (cea_Cp<M>(g.curve[s], Q.T) - g.R[s]);
+        double R = tp_Rmix<NSP>(g, Q.Y);
+        return sqrt(dv<MD>(Cp, Cv) * (R * Q.T));
     }
-    double Cp = 0.0, Cv = 0.0;
-#pragma unroll
-    for (int s = 0; s < NSP; ++s) Cp += Q.Y[s] * cea_Cp<M>(g.curve[s], Q.T);
-#pragma unroll
-    for (int s = 0; s < NSP; ++s) Cv += Q.Y[s] * (cea_Cp<M>(g.curve[s], Q.T) - g.R[s]);
-    double R = tp_Rmix<NSP>(g, Q.Y);
-    return sqrt(dv<MD>(Cp, Cv) * (R * Q.T));
 }
 
 // gas/gas_model.d:296-328 (tolerance 0, assert_error_tolerance 0.1)

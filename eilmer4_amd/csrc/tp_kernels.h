@@ -34,14 +34,29 @@ template <> struct TPArith<ModeMixed> { using type = ModeFast64; static constexp
 template <class M, int NSP, bool POLISH>
 __device__ __forceinline__ int tp_face_eos(const TPGas& g, TPState<NSP>& Q, int* fevals, int* fallbacks)
 {
-    if constexpr (POLISH) {
-        if (tp_polish<M, NSP>(g, Q, Q.u, Q.T, fevals) == 0) {
-            Q.p = Q.rho * tp_Rmix<NSP>(g, Q.Y) * Q.T;
-            return 0;
+    if constexpr (M::kFast) {
+        // one set of segment sums serves the polish and the bracketed solve
+        TPMix X;
+        tp_mix_all<NSP>(g, Q.Y, X);
+        const double Rmix = tp_Rmix<NSP>(g, Q.Y);
+        int rc = -1;
+        if constexpr (POLISH) {
+            rc = tp_polish_x<M, NSP>(g, X, Rmix, Q, Q.u, Q.T, fevals);
+            if (rc != 0) (*fallbacks)++;
         }
-        (*fallbacks)++;
+        if (rc != 0) rc = tp_update_temperature_x<M, NSP>(g, X, Rmix, Q, fevals);
+        if (rc == 0) Q.p = Q.rho * Rmix * Q.T;
+        return rc;
+    } else {
+        if constexpr (POLISH) {
+            if (tp_polish<M, NSP>(g, Q, Q.u, Q.T, fevals) == 0) {
+                Q.p = Q.rho * tp_Rmix<NSP>(g, Q.Y) * Q.T;
+                return 0;
+            }
+            (*fallbacks)++;
+        }
+        return tp_update_thermo_from_rhou<M, NSP>(g, Q, fevals);
     }
-    return tp_update_thermo_from_rhou<M, NSP>(g, Q, fevals);
 }
 
 template <int NSP>
@@ -278,29 +293,22 @@ __global__ __launch_bounds__(128) void tp_face_kernel(TPArgs A)
                                tp_val(r[3], P_U), Lf.u, Rf.u);
         // update_thermo_from_rhou on each side (guess: the side's cell T); on
         // failure the side is the cell's state (onedinterp.d:1039-1068); the
-        // sound speed is the cell's (sfluidblock.d:2199-2208).  The polish of
-        // the mixed mode starts from T reconstructed like rho and u instead,
-        // which is within a few kelvin of the face's root where the field is
-        // smooth (the cell T can be hundreds of kelvin away).
-        double TgL = tp_val(r[1], P_T), TgR = tp_val(r[2], P_T);
-        if constexpr (kPolish) {
-            double tl, tr;
-            interp_l2r2_scalar<MA>(w, lim, clip, tp_val(r[0], P_T), TgL, TgR, tp_val(r[3], P_T), tl, tr);
-            TgL = (tl > 20.0 && tl < 100000.0) ? tl : TgL;
-            TgR = (tr > 20.0 && tr < 100000.0) ? tr : TgR;
-        }
-        TPState<NSP> Q;
-        Q.rho = Lf.rho; Q.T = TgL; Q.u = Lf.u;
+        // sound speed is the cell's (sfluidblock.d:2199-2208).  The two sides
+        // share one copy of the solver code (a loop that is not unrolled).
+#pragma nounroll
+        for (int sd = 0; sd < 2; ++sd) {
+            const TPRef rc = sd ? r[2] : r[1];
+            TPState<NSP> Q;
+            Q.rho = sd ? Rf.rho : Lf.rho;
+            Q.u = sd ? Rf.u : Lf.u;
+            Q.T = tp_val(rc, P_T);
 #pragma unroll
-        for (int s = 0; s < NSP; ++s) Q.Y[s] = Lf.Y[s];
-        if (tp_face_eos<MA, NSP, kPolish>(*A.gas, Q, &fe, &fb) == 0) {
-            Lf.T = Q.T; Lf.u = Q.u; Lf.p = Q.p; Lf.a = tp_val(r[1], P_A); okL = true;
-        }
-        Q.rho = Rf.rho; Q.T = TgR; Q.u = Rf.u;
-#pragma unroll
-        for (int s = 0; s < NSP; ++s) Q.Y[s] = Rf.Y[s];
-        if (tp_face_eos<MA, NSP, kPolish>(*A.gas, Q, &fe, &fb) == 0) {
-            Rf.T = Q.T; Rf.u = Q.u; Rf.p = Q.p; Rf.a = tp_val(r[2], P_A); okR = true;
+            for (int s = 0; s < NSP; ++s) Q.Y[s] = sd ? Rf.Y[s] : Lf.Y[s];
+            if (tp_face_eos<MA, NSP, kPolish>(*A.gas, Q, &fe, &fb) == 0) {
+                const double a = tp_val(rc, P_A);
+                if (sd == 0) { Lf.T = Q.T; Lf.u = Q.u; Lf.p = Q.p; Lf.a = a; okL = true; }
+                else { Rf.T = Q.T; Rf.u = Q.u; Rf.p = Q.p; Rf.a = a; okR = true; }
+            }
         }
     }
     if (!okL) Lf = tp_cell<NSP>(r[1], pn, pt1, pt2);
@@ -415,23 +423,39 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
     }
     if (have_state) {
         const double e_tgt = Q.u;
-        int rc;
-        if constexpr (GUESS) {
-            rc = tp_polish<MA, NSP>(*A.gas, Q, e_tgt, Tguess, &fe);
-            if (rc == 0) {
-                double Rmix = tp_Rmix<NSP>(*A.gas, Q.Y);
-                Q.p = Q.rho * Rmix * Q.T;
-            } else {
-                fell_back = true;
+        const double Tprev = A.Pin[P_T * qs + c];   // the previous temperature: the bracketed solve's guess
+        int rc = -1;
+        if constexpr (MA::kFast) {
+            // one set of segment sums for the polish, the bracketed solve and
+            // the sound speed
+            TPMix X;
+            tp_mix_all<NSP>(*A.gas, Q.Y, X);
+            const double Rmix = tp_Rmix<NSP>(*A.gas, Q.Y);
+            if constexpr (GUESS || kPolish) {
+                rc = tp_polish_x<MA, NSP>(*A.gas, X, Rmix, Q, e_tgt, GUESS ? Tguess : Tprev, &fe);
+                fell_back = rc != 0;
+            }
+            if (rc != 0) {
                 Q.u = e_tgt;
-                Q.T = A.Pin[P_T * qs + c];
-                rc = tp_update_thermo_from_rhou<MA, NSP>(*A.gas, Q, &fe);
+                Q.T = Tprev;
+                rc = tp_update_temperature_x<MA, NSP>(*A.gas, X, Rmix, Q, &fe);
+            }
+            if (rc == 0) {
+                Q.p = Q.rho * Rmix * Q.T;
+                Q.a = tp_sound_speed_x<MA>(*A.gas, X, Rmix, Q.T);
             }
         } else {
-            Q.T = A.Pin[P_T * qs + c];     // the previous temperature is the Newton guess
-            int nfb_cell = 0;
-            rc = tp_face_eos<MA, NSP, kPolish>(*A.gas, Q, &fe, &nfb_cell);
-            fell_back = nfb_cell != 0;
+            if constexpr (GUESS) {
+                rc = tp_polish<MA, NSP>(*A.gas, Q, e_tgt, Tguess, &fe);
+                if (rc == 0) Q.p = Q.rho * tp_Rmix<NSP>(*A.gas, Q.Y) * Q.T;
+                else fell_back = true;
+            }
+            if (rc != 0) {
+                Q.u = e_tgt;
+                Q.T = Tprev;
+                rc = tp_update_thermo_from_rhou<MA, NSP>(*A.gas, Q, &fe);
+            }
+            if (rc == 0) Q.a = tp_sound_speed<MA, NSP>(*A.gas, Q);
         }
         if (rc != 0) {
             // ignore_low_T_thermo_update_failure: T = 200 K and re-encode U
@@ -443,8 +467,8 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
             Un_out[4 * qs] = rho * (Q.u + ke2);
 #pragma unroll
             for (int s = 0; s < NSP; ++s) Un_out[(5 + s) * qs] = rho * Q.Y[s];
+            Q.a = tp_sound_speed<MA, NSP>(*A.gas, Q);
         }
-        Q.a = tp_sound_speed<MA, NSP>(*A.gas, Q);
         // check_data (flowstate.d:345-388)
         double fsum = 0.0;
 #pragma unroll
