@@ -123,3 +123,44 @@ def test_tp_mixed_mode_close_to_plain(gpu):
     e_T = float(np.max(np.abs(fb[2] - fa[2]) / fa[2]))
     print("  rho %.3e T %.3e" % (e_rho, e_T), flush=True)
     assert e_rho <= 1e-5 and e_T <= 1e-8
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tp_unshared_breakpoints_run_reference_form(gpu, precision, tmp_path):
+    """A mixture whose species do not share blend ranges cannot use the fast
+    modes' mixture form; the library runs the reference-form kernels for it,
+    so the fast modes then match the oracle like fp64_exact."""
+    import json
+    import os
+    from eilmer4_amd import gas as gas_module
+    desc = json.load(open(os.path.join(gas_module.DATA, "therm-perf-5-species-air.json")))
+    desc["thermo"][4]["T_blend_ranges"] = [300.0, 900.0]
+    path = str(tmp_path / "air5-unshared.json")
+    json.dump(desc, open(path, "w"))
+    n = 10
+    gas = e4.ThermallyPerfectGas.from_json(path)
+    geom, fields, bcs, _ = workloads.thermally_perfect_box_3d(gas, n=n)
+    U0, Tg = workloads.initial_U_tp(gas, fields)
+    og = O.Gas.from_json(path)
+    ob = O.Block(og, 3, n, n, n, geom["dx"], geom["dy"], geom["dz"])
+    for face, (kind, fs) in bcs.items():
+        ob.set_bc(int(face), int(kind), fs)
+    ob.set_U(U0, Tg)
+    db = e4.SFluidBlock(gas, 3, n, n, n, geom["dx"], geom["dy"], geom["dz"])
+    for face, (kind, fs) in bcs.items():
+        db.set_bc(face, kind, fs)
+    sim = e4.Simulation([db], e4.GlobalConfig(precision_mode=e4.PrecisionMode[precision]))
+    db.upload(U0, Tg)
+    print("case tp unshared blend ranges n=%d precision=%s steps=3" % (n, precision), flush=True)
+    dt, _, _ = ob.determine_dt(1.0e-3, 0.5, False)
+    sim.cfg.fixed_time_step = True
+    for _ in range(3):
+        assert ob.step(dt)[0] == 0
+        sim.state.dt_global = dt
+        sim.gasdynamic_explicit_increment_with_fixed_grid()
+    U, fs = db.download(with_fs=True)
+    err = rel_err(U, ob.get_U())
+    errT = float(np.max(np.abs(fs[2] - ob.get_fs()[2]) / ob.get_fs()[2]))
+    print("  U rel err", err, "T", errT, flush=True)
+    assert err.max() <= TOL and errT <= TOL
+    db.destroy()
