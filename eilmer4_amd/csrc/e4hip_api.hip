@@ -58,6 +58,9 @@ struct BlockH {
     double* F[3] = {nullptr, nullptr, nullptr};
     long fqs[3] = {0, 0, 0};
     double bc_fs_tp[6][8 + e4::TP_NSP_MAX] = {};
+    // shock indicators per cell (adaptive_hanel_ausmdv): S[0] this stage's
+    // detection, S[1] the previous one
+    unsigned char* S[2] = {nullptr, nullptr};
     bool is_tp() const;
 };
 
@@ -85,7 +88,8 @@ struct State {
     BatchBuffers batch;
     int batch_min_blocks = 2;                  // batch the ideal blocks of a stage from this many on
     std::vector<BlockH> blocks;
-    E4hipStepConfig cfg{E4HIP_FLUX_AUSMDV, E4HIP_UPDATE_PC, 2, 1, 1, 1, E4HIP_THERMO_RHOU, E4HIP_PRECISION_FP64, 0, 0};
+    E4hipStepConfig cfg{E4HIP_FLUX_AUSMDV, E4HIP_UPDATE_PC, 2, 1, 1, 1, E4HIP_THERMO_RHOU, E4HIP_PRECISION_FP64, 0, 0,
+                        0, 0.20, -0.30};
     bool profiling = false;
     std::vector<ProfEvent> prof_pending;
     double prof_ms[2] = {0, 0};
@@ -210,6 +214,34 @@ e4::Connection connection_of(const BlockH& b, int f)
     return c;
 }
 
+// The kernels' flux instantiation of a FluxCalculator value.
+int kernel_flux(int fc)
+{
+    if (fc == E4HIP_FLUX_HLLC) return e4::FLUX_HLLC;
+    if (fc == E4HIP_FLUX_HANEL || fc == E4HIP_FLUX_ADAPTIVE_HANEL_AUSMDV) return e4::FLUX_SHOCK;
+    return e4::FLUX_AUSMDV;
+}
+
+// Shock-indicator arguments of a block's stage (StageArgs or TPArgs).
+template <class Args>
+int shock_fields(const BlockH& b, Args& A)
+{
+    const E4hipStepConfig& cfg = g_state.cfg;
+    A.S = b.S[0];
+    A.Sprev = b.S[1];
+    A.Snew = b.S[0];
+    for (int f = 0; f < 6; ++f) {
+        if (b.bc_type[f] != E4HIP_BC_CONNECTED) continue;
+        const BlockH* o = get_block(b.nbr_blk[f]);
+        if (!o) return fail(E4HIP_ERR_HANDLE, "face %d is connected to a destroyed block", f);
+        A.nbrS[f] = o->S[1];
+    }
+    A.shear_tol = cfg.shear_tolerance;
+    A.compression_tol = cfg.compression_tolerance;
+    A.always_hanel = cfg.flux_calculator == E4HIP_FLUX_HANEL;
+    return E4HIP_OK;
+}
+
 // The stage kernel's arguments for one block; kchunk <= 0 picks the
 // single-launch chunk.
 int stage_args_for(const BlockH& b, int stage, double dt, unsigned int* counters, int kchunk, e4::StageArgs& A,
@@ -283,6 +315,8 @@ int stage_args_for(const BlockH& b, int stage, double dt, unsigned int* counters
     }
     A.dt = dt; A.g1 = g1; A.g2 = g2; A.g3 = g3;
     A.c1 = dt * g1;
+    const int src = shock_fields(b, A);
+    if (src) return src;
     const int nk = A.nk;
     A.tiles_x = (b.ni + TILE_X - 1) / TILE_X;
     A.tiles_y = (b.nj + TILE_Y - 1) / TILE_Y;
@@ -299,7 +333,7 @@ int launch_stage(const BlockH& b, int stage, double dt, unsigned int* counters)
     if (rc) return rc;
     hipEvent_t ev = nullptr;
     prof_begin(&ev);
-    const int flux = (cfg.flux_calculator == E4HIP_FLUX_HLLC) ? e4::FLUX_HLLC : e4::FLUX_AUSMDV;
+    const int flux = kernel_flux(cfg.flux_calculator);
     switch (cfg.precision_mode) {
     case E4HIP_PRECISION_FP64_EXACT: e4::launch_stage_exact(A, grid, g_state.stream, b.dims, flux); break;
     case E4HIP_PRECISION_MIXED: e4::launch_stage_mixed(A, grid, g_state.stream, b.dims, flux); break;
@@ -368,7 +402,7 @@ int launch_stage_batch(const std::vector<const BlockH*>& blks, const std::vector
     B.table = (unsigned long long)(uintptr_t)ad;
     B.wg_entry = (const int*)md;
     const bool def = args[0].high_order && args[0].apply_limiter && args[0].extrema_clipping;
-    const int flux = (cfg.flux_calculator == E4HIP_FLUX_HLLC) ? e4::FLUX_HLLC : e4::FLUX_AUSMDV;
+    const int flux = kernel_flux(cfg.flux_calculator);
     hipEvent_t ev = nullptr;
     prof_begin(&ev);
     const int nwg = (int)map.size();
@@ -390,12 +424,12 @@ bool tp_mixture_form(const e4::TPGas& g)
     return g.shared != 0 && g.curve[0].nseg == e4::TP_FAST_NSEG;
 }
 
-int launch_tp_stage(BlockH& b, int stage, double dt, unsigned int* counters)
+// The thermally perfect kernels' arguments for one block's stage.
+int tp_args_for(const BlockH& b, int stage, double dt, unsigned int* counters, e4::TPArgs& A)
 {
     const E4hipStepConfig& cfg = g_state.cfg;
     const int sch = cfg.gasdynamic_update_scheme;
     const int ns = nstages(sch);
-    e4::TPArgs A;
     memset(&A, 0, sizeof(A));
     A.qs = b.qs;
     for (int dd = 0; dd < 3; ++dd) { A.F[dd] = b.F[dd]; A.fqs[dd] = b.fqs[dd]; }
@@ -459,7 +493,16 @@ int launch_tp_stage(BlockH& b, int stage, double dt, unsigned int* counters)
     A.c1 = dt * g1;
     A.Pin = b.P[0];
     A.Pout = b.P[1];
-    const int flux = (cfg.flux_calculator == E4HIP_FLUX_HLLC) ? e4::FLUX_HLLC : e4::FLUX_AUSMDV;
+    return shock_fields(b, A);
+}
+
+int launch_tp_stage(BlockH& b, int stage, double dt, unsigned int* counters)
+{
+    const E4hipStepConfig& cfg = g_state.cfg;
+    e4::TPArgs A;
+    int rc = tp_args_for(b, stage, dt, counters, A);
+    if (rc) return rc;
+    const int flux = kernel_flux(cfg.flux_calculator);
     hipEvent_t ev = nullptr;
     prof_begin(&ev);
     // the fast modes evaluate the mixture form, which needs three-segment curves
@@ -471,6 +514,32 @@ int launch_tp_stage(BlockH& b, int stage, double dt, unsigned int* counters)
     prof_end(0, ev);
     HIP_TRY(hipGetLastError());
     std::swap(b.P[0], b.P[1]);
+    return E4HIP_OK;
+}
+
+// Shock detection of one block for a stage (fluidblock.d:415-431): every
+// block's indicators are swapped first (S[1] keeps the previous detection,
+// which connected blocks' ghost cells carry), then each block detects from the
+// stencil state of the stage into S[0].
+int launch_shock_detect(const BlockH& b, int stage, double dt)
+{
+    const E4hipStepConfig& cfg = g_state.cfg;
+    const bool exact = cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT;
+    if (b.is_tp()) {
+        e4::TPArgs A;
+        int rc = tp_args_for(b, stage, dt, nullptr, A);
+        if (rc) return rc;
+        if (exact || !tp_mixture_form(g_state.gases[b.gm].tp)) e4::launch_tp_shock_detect_exact(A, g_state.stream);
+        else e4::launch_tp_shock_detect_fast64(A, g_state.stream);
+    } else {
+        e4::StageArgs A;
+        dim3 grid;
+        int rc = stage_args_for(b, stage, dt, nullptr, 0, A, grid);
+        if (rc) return rc;
+        if (exact) e4::launch_shock_detect_exact(A, b.dims, g_state.stream);
+        else e4::launch_shock_detect_fast64(A, b.dims, g_state.stream);
+    }
+    HIP_TRY(hipGetLastError());
     return E4HIP_OK;
 }
 
@@ -490,6 +559,11 @@ int alloc_block_buffers(BlockH& b)
         }
     }
     if (!b.bc_dev) HIP_TRY(hipMalloc(&b.bc_dev, sizeof(e4::BCTable)));
+    for (int t = 0; t < 2; ++t) {
+        if (b.S[t]) continue;
+        HIP_TRY(hipMalloc(&b.S[t], (size_t)b.qs));
+        HIP_TRY(hipMemset(b.S[t], 0, (size_t)b.qs));
+    }
     if (!b.U[0]) HIP_TRY(hipMalloc(&b.U[0], bytes));
     if (!b.U[1]) HIP_TRY(hipMalloc(&b.U[1], bytes));
     if (ns >= 2 && !b.d[0]) HIP_TRY(hipMalloc(&b.d[0], bytes));
@@ -506,6 +580,7 @@ int free_block_buffers(BlockH& b)
     for (auto& p : b.d) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     for (auto& p : b.P) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     for (auto& p : b.F) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
+    for (auto& p : b.S) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     if (b.bc_dev) { HIP_TRY(hipFree(b.bc_dev)); b.bc_dev = nullptr; }
     return E4HIP_OK;
 }
@@ -848,6 +923,8 @@ int e4hip_block_upload(int blk, const double* U, const double* Tguess)
     if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
     if (!U) return fail(E4HIP_ERR_ARG, "null U");
     if (b->is_tp() && !Tguess) return fail(E4HIP_ERR_ARG, "a thermally perfect gas needs the temperature guess");
+    // an uploaded flow state carries no shock indicator (flowstate.d:44, S = 0)
+    for (auto* sp : b->S) if (sp) HIP_TRY(hipMemsetAsync(sp, 0, (size_t)b->qs, g_state.stream));
     for (int q = 0; q < b->nq; ++q)
         HIP_TRY(hipMemcpyAsync(b->U[0] + (size_t)q * b->qs, U + (size_t)q * b->ncells, sizeof(double) * b->ncells,
                                hipMemcpyHostToDevice, g_state.stream));
@@ -915,8 +992,11 @@ int e4hip_block_download(int blk, double* U, double* fs)
 int e4hip_config(const E4hipStepConfig* c)
 {
     if (!c) return fail(E4HIP_ERR_ARG, "null config");
-    if (c->flux_calculator != E4HIP_FLUX_AUSMDV && c->flux_calculator != E4HIP_FLUX_HLLC)
+    if (c->flux_calculator != E4HIP_FLUX_AUSMDV && c->flux_calculator != E4HIP_FLUX_HLLC &&
+        c->flux_calculator != E4HIP_FLUX_HANEL && c->flux_calculator != E4HIP_FLUX_ADAPTIVE_HANEL_AUSMDV)
         return fail(E4HIP_ERR_ARG, "flux_calculator %d is not available on the device path", c->flux_calculator);
+    if (!std::isfinite(c->shear_tolerance) || !std::isfinite(c->compression_tolerance))
+        return fail(E4HIP_ERR_ARG, "shock tolerances must be finite");
     if (c->gasdynamic_update_scheme < E4HIP_UPDATE_EULER || c->gasdynamic_update_scheme > E4HIP_UPDATE_DENMAN_RK3)
         return fail(E4HIP_ERR_ARG, "gasdynamic_update_scheme %d is not available", c->gasdynamic_update_scheme);
     if (c->interpolation_order != 1 && c->interpolation_order != 2)
@@ -948,6 +1028,18 @@ int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_ce
         g_state.stage_pin.assign(g_state.blocks.size(), nullptr);
         for (size_t h = 0; h < g_state.blocks.size(); ++h)
             if (g_state.blocks[h].alive) g_state.stage_pin[h] = g_state.blocks[h].P[0];
+        // adaptive_hanel_ausmdv turns the shock detector on (globalconfig.d:1942-1954);
+        // it runs after the ghost-cell fill and before the fluxes (simcore.d:2352-2358)
+        if (g_state.cfg.flux_calculator == E4HIP_FLUX_ADAPTIVE_HANEL_AUSMDV) {
+            for (int n = 0; n < nblk; ++n) {
+                BlockH& b = *get_block(blks[n]);
+                std::swap(b.S[0], b.S[1]);
+            }
+            for (int n = 0; n < nblk; ++n) {
+                rc = launch_shock_detect(*get_block(blks[n]), stage, dt);
+                if (rc) return rc;
+            }
+        }
         // ideal-gas blocks: one batched launch per dimensionality when there
         // are several; thermally perfect blocks: their kernel sequence each
         std::vector<const BlockH*> batch[2];

@@ -20,7 +20,11 @@
 
 namespace e4 {
 
-enum { FLUX_AUSMDV = 0, FLUX_HLLC = 9 };
+// Flux calculators (FluxCalculator order, eilmer/fvcore.d:184-194; hllc
+// appended).  FLUX_SHOCK is the kernels' instantiation for the calculators
+// that read the shock indicator: adaptive_hanel_ausmdv, and hanel as its
+// always-Hanel case.
+enum { FLUX_AUSMDV = 0, FLUX_HANEL = 1, FLUX_ADAPTIVE_HANEL_AUSMDV = 5, FLUX_HLLC = 9, FLUX_SHOCK = 5 };
 
 struct ModeExact { using R = double; static constexpr bool kFast = false; };
 struct ModeFast64 { using R = double; static constexpr bool kFast = true; };
@@ -343,12 +347,85 @@ __device__ __forceinline__ Flux5T<typename M::R> hllc(const FaceStateT<typename 
     return F;
 }
 
+// fluxcalc.d:532-623: Hanel's flux vector splitting (van Leer's split
+// velocities and pressures), in the reference's operation order.
+template <class M>
+__device__ __forceinline__ Flux5T<typename M::R> hanel(const FaceStateT<typename M::R>& L,
+                                                       const FaceStateT<typename M::R>& Rs)
+{
+    using R = typename M::R;
+    const R rL = L.rho, pL = L.p;
+    const R pLrL = dv<M>(pL, rL);
+    const R uL = L.vn, vL = L.vt1, wL = L.vt2;
+    const R eL = L.u, aL = L.a;
+    const R keL = R(0.5) * (uL * uL + vL * vL + wL * wL);
+    const R HL = eL + pLrL + keL;
+    const R rR = Rs.rho, pR = Rs.p;
+    const R pRrR = dv<M>(pR, rR);
+    const R uR = Rs.vn, vR = Rs.vt1, wR = Rs.vt2;
+    const R eR = Rs.u, aR = Rs.a;
+    const R keR = R(0.5) * (uR * uR + vR * vR + wR * wR);
+    const R HR = eR + pRrR + keR;
+    R pLplus, uLplus;
+    if (ab(uL) <= aL) {
+        uLplus = dv<M>(R(1.0), R(4.0) * aL) * (uL + aL) * (uL + aL);
+        pLplus = pL * uLplus * (dv<M>(R(1.0), aL) * (R(2.0) - dv<M>(uL, aL)));
+    } else {
+        uLplus = R(0.5) * (uL + ab(uL));
+        pLplus = pL * uLplus * dv<M>(R(1.0), uL);
+    }
+    R pRminus, uRminus;
+    if (ab(uR) <= aR) {
+        uRminus = dv<M>(R(-1.0), R(4.0) * aR) * (uR - aR) * (uR - aR);
+        pRminus = pR * uRminus * (dv<M>(R(1.0), aR) * (R(-2.0) - dv<M>(uR, aR)));
+    } else {
+        uRminus = R(0.5) * (uR - ab(uR));
+        pRminus = pR * uRminus * dv<M>(R(1.0), uR);
+    }
+    const R p_half = pLplus + pRminus;
+    Flux5T<R> F;
+    F.mass = uLplus * rL + uRminus * rR;
+    F.mn = uLplus * rL * uL + uRminus * rR * uR + p_half;
+    F.mt1 = uLplus * rL * vL + uRminus * rR * vR;
+    F.mt2 = uLplus * rL * wL + uRminus * rR * wR;
+    F.E = uLplus * rL * HL + uRminus * rR * HR;
+    return F;
+}
+
+// Whether adaptive_hanel_ausmdv (fluxcalc.d:822-836) takes Hanel at a face:
+// either side is a shock point and the tangential velocity jump, scaled by the
+// mean sound speed, is within the shear tolerance.
+template <class M>
+__device__ __forceinline__ bool hanel_at_face(const FaceStateT<typename M::R>& L, const FaceStateT<typename M::R>& Rs,
+                                              bool shock, typename M::R shear_tol)
+{
+    using R = typename M::R;
+    const R sound_speed = R(0.5) * (L.a + Rs.a);
+    const R shear_y = dv<M>(ab(L.vt1 - Rs.vt1), sound_speed);
+    const R shear_z = dv<M>(ab(L.vt2 - Rs.vt2), sound_speed);
+    const bool shear_is_small = mx(shear_y, shear_z) <= shear_tol;
+    return shock && shear_is_small;
+}
+
+// Options of the shock-indicator calculators: the shear tolerance, and whether
+// every face takes Hanel (the hanel calculator itself).
+template <class R>
+struct ShockOpt {
+    R shear_tol;
+    bool always_hanel;
+};
+
 template <class M, int FLUX>
 __device__ __forceinline__ Flux5T<typename M::R> flux_calc(const FaceStateT<typename M::R>& L,
-                                                           const FaceStateT<typename M::R>& R)
+                                                           const FaceStateT<typename M::R>& R,
+                                                           bool shock = false,
+                                                           ShockOpt<typename M::R> opt = {})
 {
     if constexpr (FLUX == FLUX_HLLC) return hllc<M>(L, R);
-    else return ausmdv<M>(L, R);
+    else if constexpr (FLUX == FLUX_SHOCK) {
+        if (opt.always_hanel || hanel_at_face<M>(L, R, shock, opt.shear_tol)) return hanel<M>(L, R);
+        return ausmdv<M>(L, R);
+    } else return ausmdv<M>(L, R);
 }
 
 } // namespace e4

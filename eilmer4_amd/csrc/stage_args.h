@@ -92,6 +92,15 @@ struct StageArgs {
     const double* nbrU[6];            // stencil-state arrays of the blocks connected to each face
     int tiles_x, tiles_y;             // workgroup tiles along i and j
     int wg0;                          // first workgroup of this block in a batched launch
+    // shock indicators (adaptive_hanel_ausmdv, hanel): S of this stage's
+    // detection, Sprev of the previous one (what the ghost cells were filled
+    // with), the connected blocks' Sprev; Snew is written by the detector
+    const unsigned char* S;
+    const unsigned char* Sprev;
+    const unsigned char* nbrS[6];
+    unsigned char* Snew;
+    double shear_tol, compression_tol;
+    int always_hanel;
 };
 
 // A batched stage launch: one workgroup per (block, tile, k-chunk) over all
@@ -134,6 +143,13 @@ struct TPArgs {
     const GuessModel* guess;          // learned temperature guess of the decode, or null
     const BCTable* bc;                // boundary effects (device memory)
     const double* nbrP[6];            // primitive arrays of the connected blocks at the stage start
+    // shock indicators, as in StageArgs
+    const unsigned char* S;
+    const unsigned char* Sprev;
+    const unsigned char* nbrS[6];
+    unsigned char* Snew;
+    double shear_tol, compression_tol;
+    int always_hanel;
 };
 
 // Launchers, one per arithmetic mode; each lives in its own translation unit
@@ -151,6 +167,12 @@ void launch_stage_batch_mixed(const BatchArgs& B, int nwg, bool def, hipStream_t
 void launch_tp_stage_exact(const TPArgs& A, hipStream_t stream, int flux);
 void launch_tp_stage_fast64(const TPArgs& A, hipStream_t stream, int flux);
 void launch_tp_stage_mixed(const TPArgs& A, hipStream_t stream, int flux);
+// shock indicators of a stage's stencil state (adaptive_hanel_ausmdv); the
+// mixed mode uses the fast64 detector (FP64 arithmetic)
+void launch_shock_detect_exact(const StageArgs& A, int dims, hipStream_t stream);
+void launch_shock_detect_fast64(const StageArgs& A, int dims, hipStream_t stream);
+void launch_tp_shock_detect_exact(const TPArgs& A, hipStream_t stream);
+void launch_tp_shock_detect_fast64(const TPArgs& A, hipStream_t stream);
 void launch_tp_decode_exact(const TPArgs& A, const double* U, const double* Tguess, hipStream_t stream);
 void launch_tp_decode_fast64(const TPArgs& A, const double* U, const double* Tguess, hipStream_t stream);
 

@@ -103,7 +103,11 @@ __device__ __forceinline__ long cell_index(KArgs A, int i, int j, int k)
     return ((long)k * A->nj + j) * A->ni + i;
 }
 
-template <class M>
+// With SH the shock indicator travels in the sign of the sound speed (a < 0:
+// shock point): interior cells carry this stage's indicator, ghost cells the
+// one of the cell they were copied from at the previous detection (the
+// ghost-cell fill precedes the detector, simcore.d:2338-2358), inflow ghosts 0.
+template <class M, bool SH = false>
 __device__ Prim fetch_ghost(KArgs A, int i, int j, int k)
 {
     Prim dummy{1.0, 0.0, 0.0, 0.0, 1.0, 1.0};
@@ -136,7 +140,9 @@ __device__ Prim fetch_ghost(KArgs A, int i, int j, int k)
         const double* p = A->nbrU[face] + c;
         const long qs = nb.qs;
         const IdealGas g = A->gas;
-        return decode_ideal<M>(p[0], p[qs], p[2 * qs], p[3 * qs], p[4 * qs], g);
+        Prim P = decode_ideal<M>(p[0], p[qs], p[2 * qs], p[3 * qs], p[4 * qs], g);
+        if (SH && A->nbrS[face][c]) P.a = -(P.a);
+        return P;
     }
     int src;
     if (type == BC_EXTRAPOLATE) src = plus ? n - 1 : 0;
@@ -144,7 +150,9 @@ __device__ Prim fetch_ghost(KArgs A, int i, int j, int k)
     else src = plus ? n - layer : layer - 1;     // slip wall mirror
     int si = i, sj = j, sk = k;
     if (dir == 0) si = src; else if (dir == 1) sj = src; else sk = src;
-    Prim P = load_decode<M>(A, cell_index(A, si, sj, sk));
+    const long sc = cell_index(A, si, sj, sk);
+    Prim P = load_decode<M>(A, sc);
+    if (SH && A->Sprev[sc]) P.a = -(P.a);
     if (type == BC_SLIP_WALL) {
         // reflect_normal_velocity (ghost_cell.d:32-43): negate the face-normal component
         if (dir == 0) P.v0 = -(P.v0); else if (dir == 1) P.v1 = -(P.v1); else P.v2 = -(P.v2);
@@ -152,13 +160,18 @@ __device__ Prim fetch_ghost(KArgs A, int i, int j, int k)
     return P;
 }
 
-template <class M>
+template <class M, bool SH = false>
 __device__ __forceinline__ PrimT<typename M::R> fetch(KArgs A, int i, int j, int k)
 {
     using R = typename M::R;
     Prim P;
-    if (i >= 0 && i < A->ni && j >= 0 && j < A->nj && k >= 0 && k < A->nk) P = load_decode<M>(A, cell_index(A, i, j, k));
-    else P = fetch_ghost<M>(A, i, j, k);
+    if (i >= 0 && i < A->ni && j >= 0 && j < A->nj && k >= 0 && k < A->nk) {
+        const long c = cell_index(A, i, j, k);
+        P = load_decode<M>(A, c);
+        if (SH && A->S[c]) P.a = -(P.a);
+    } else {
+        P = fetch_ghost<M, SH>(A, i, j, k);
+    }
     return PrimT<R>{R(P.rho), R(P.v0), R(P.v1), R(P.v2), R(P.u), R(P.a)};
 }
 
@@ -253,15 +266,24 @@ __device__ __forceinline__ R vcomp(const SideT<R>& S)
 // an exact sign flip that every operation of the face path commutes with, so
 // it is not applied.  The face EOS is p = rho Rgas (u / Cv) (ideal_gas.d:98-106).
 // Fg receives the flux in the global frame (mass, x-, y-, z-momentum, energy).
+// With FLUX_SHOCK the sides' sound speeds carry their cells' shock indicators
+// in the sign (fetch).
 template <class M, int FLUX, int PN, int PT1, int PT2>
 __device__ __forceinline__ void side_flux(const SideT<typename M::R>& L, const SideT<typename M::R>& Rs,
-                                          const IdealGasT<typename M::R>& g, typename M::R Fg[5])
+                                          const IdealGasT<typename M::R>& g, typename M::R Fg[5],
+                                          ShockOpt<typename M::R> opt = {})
 {
     using R = typename M::R;
     FaceStateT<R> Lf, Rf;
+    bool shock = false;
     Lf.rho = L.rho; Lf.u = L.u; Lf.a = L.a;
-    Lf.vn = vcomp<PN>(L); Lf.vt1 = vcomp<PT1>(L); Lf.vt2 = vcomp<PT2>(L);
     Rf.rho = Rs.rho; Rf.u = Rs.u; Rf.a = Rs.a;
+    if constexpr (FLUX == FLUX_SHOCK) {
+        shock = L.a < R(0) || Rs.a < R(0);
+        Lf.a = ab(L.a);
+        Rf.a = ab(Rs.a);
+    }
+    Lf.vn = vcomp<PN>(L); Lf.vt1 = vcomp<PT1>(L); Lf.vt2 = vcomp<PT2>(L);
     Rf.vn = vcomp<PN>(Rs); Rf.vt1 = vcomp<PT1>(Rs); Rf.vt2 = vcomp<PT2>(Rs);
     R TL = Lf.u * g.Cvinv;
     R TR = Rf.u * g.Cvinv;
@@ -275,12 +297,18 @@ __device__ __forceinline__ void side_flux(const SideT<typename M::R>& L, const S
         Rf.p = Rf.rho * g.Rgas * TR;
         Lf.RT = Rf.RT = R(0);
     }
-    Flux5T<R> F = flux_calc<M, FLUX>(Lf, Rf);
+    Flux5T<R> F = flux_calc<M, FLUX>(Lf, Rf, shock, opt);
     Fg[0] = F.mass;
     Fg[1 + PN] = F.mn;
     Fg[1 + PT1] = F.mt1;
     Fg[1 + PT2] = F.mt2;
     Fg[4] = F.E;
+}
+
+template <class R>
+__device__ __forceinline__ ShockOpt<R> shock_opt(KArgs A)
+{
+    return ShockOpt<R>{R(A->shear_tol), A->always_hanel != 0};
 }
 
 template <class R>
@@ -338,6 +366,7 @@ void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type A
     const int entry = BATCH ? batch_entry() : 0;
     auto args = [&]() { return BATCH ? kargs_entry(entry) : kargs(); };
     using R = typename M::R;
+    constexpr bool SH = (FLUX == FLUX_SHOCK);
     constexpr int NT = TX * TY;
     constexpr int PX = TX + 4, PY = TY + 4, PLANE = PX * PY;
     constexpr int NSI = (TX + 1) * TY;       // i-face slots: face fc (0..TX) of row r is r*(TX+1)+fc
@@ -382,17 +411,17 @@ void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type A
     if constexpr (DIMS == 3) {
         const L2R2UT<R> wk = cast_wu<R>(L2R2U(ap->wu[2]));
         const IdealGasT<R> gR = cast_gas<R>(IdealGas(ap->gas));
-        PrimT<R> qm2 = fetch<M>(ap, i, j, k0 - 2);
-        PrimT<R> qm1 = fetch<M>(ap, i, j, k0 - 1);
-        own = fetch<M>(ap, i, j, k0);
-        qa = fetch<M>(ap, i, j, k0 + 1);
-        qb = fetch<M>(ap, i, j, k0 + 2);
+        PrimT<R> qm2 = fetch<M, SH>(ap, i, j, k0 - 2);
+        PrimT<R> qm1 = fetch<M, SH>(ap, i, j, k0 - 1);
+        own = fetch<M, SH>(ap, i, j, k0);
+        qa = fetch<M, SH>(ap, i, j, k0 + 1);
+        qb = fetch<M, SH>(ap, i, j, k0 + 2);
         SideT<R> Lbelow, Rdummy, Lown, Rown, Rabove;
         recon_cell<M>(wk, ho, lim, clip, qm2, qm1, own, Lbelow, Rdummy);
         recon_cell<M>(wk, ho, lim, clip, qm1, own, qa, Lown, Rown);
-        if (own_ij) side_flux<M, FLUX, 2, 0, 1>(Lbelow, Rown, gR, Fb);
+        if (own_ij) side_flux<M, FLUX, 2, 0, 1>(Lbelow, Rown, gR, Fb, shock_opt<R>(ap));
         recon_cell<M>(wk, ho, lim, clip, own, qa, qb, LpC, Rabove);
-        if (own_ij) side_flux<M, FLUX, 2, 0, 1>(Lown, Rabove, gR, Ft);
+        if (own_ij) side_flux<M, FLUX, 2, 0, 1>(Lown, Rabove, gR, Ft, shock_opt<R>(ap));
     }
 
     for (int k = k0; k < k1; ++k) {
@@ -403,7 +432,7 @@ void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type A
         const int txk = tidk % TX, tyk = tidk / TX;
         const int ik = i0 + txk, jk = j0 + tyk;
         const bool own_k = (ik < ap->ni) && (jk < ap->nj);
-        if constexpr (DIMS == 2) own = fetch<M>(ap, ik, jk, 0);
+        if constexpr (DIMS == 2) own = fetch<M, SH>(ap, ik, jk, 0);
         // conserved state of plane k+3, loaded now and decoded at the end
         double raw[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         const bool raw_ok = (DIMS == 3) && own_k && (k + 1 < k1) && (k + 3 < ap->nk);
@@ -431,7 +460,7 @@ void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type A
                 jj = side ? TY + 2 + row : row;
                 ii = col + 2;
             }
-            PrimT<R> h = fetch<M>(ap, i0 + ii - 2, j0 + jj - 2, k);
+            PrimT<R> h = fetch<M, SH>(ap, i0 + ii - 2, j0 + jj - 2, k);
             const int s = jj * PX + ii;
             sP[0 * PLANE + s] = h.rho; sP[1 * PLANE + s] = h.v0; sP[2 * PLANE + s] = h.v1;
             sP[3 * PLANE + s] = h.v2; sP[4 * PLANE + s] = h.u; sP[5 * PLANE + s] = h.a;
@@ -481,9 +510,9 @@ void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type A
         if (own_k) {
             R Fx[5];
             const int si = tyk * (TX + 1) + txk + 1, sj = (tyk + 1) * TX + txk;
-            side_flux<M, FLUX, 0, 1, 2>(LpI, lds_side(sI, NSI, si), gR, Fx);
+            side_flux<M, FLUX, 0, 1, 2>(LpI, lds_side(sI, NSI, si), gR, Fx, shock_opt<R>(ap));
             lds_put_flux(sI, NSI, si, Fx);
-            side_flux<M, FLUX, JN, JT1, JT2>(LpJ, lds_side(sJ, NSJ, sj), gR, Fx);
+            side_flux<M, FLUX, JN, JT1, JT2>(LpJ, lds_side(sJ, NSJ, sj), gR, Fx, shock_opt<R>(ap));
             lds_put_flux(sJ, NSJ, sj, Fx);
         }
         if (tidk < NBND) {
@@ -491,13 +520,13 @@ void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type A
             if (tidk < TY) {
                 if (j0 + tidk < ap->nj) {
                     const int si = tidk * (TX + 1);
-                    side_flux<M, FLUX, 0, 1, 2>(lds_side(sLw, TY, tidk), lds_side(sI, NSI, si), gR, Fx);
+                    side_flux<M, FLUX, 0, 1, 2>(lds_side(sLw, TY, tidk), lds_side(sI, NSI, si), gR, Fx, shock_opt<R>(ap));
                     lds_put_flux(sI, NSI, si, Fx);
                 }
             } else {
                 const int c = tidk - TY;
                 if (i0 + c < ap->ni) {
-                    side_flux<M, FLUX, JN, JT1, JT2>(lds_side(sLs, TX, c), lds_side(sJ, NSJ, c), gR, Fx);
+                    side_flux<M, FLUX, JN, JT1, JT2>(lds_side(sLs, TX, c), lds_side(sJ, NSJ, c), gR, Fx, shock_opt<R>(ap));
                     lds_put_flux(sJ, NSJ, c, Fx);
                 }
             }
@@ -583,15 +612,16 @@ void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type A
                 PrimT<R> q3;
                 if (raw_ok) {
                     Prim P = decode_ideal<M>(raw[0], raw[1], raw[2], raw[3], raw[4], IdealGas(ap->gas));
+                    if (SH && ap->S[cell_index(ap, ik, jk, k + 3)]) P.a = -(P.a);
                     q3 = PrimT<R>{R(P.rho), R(P.v0), R(P.v1), R(P.v2), R(P.u), R(P.a)};
                 } else {
-                    q3 = fetch<M>(ap, ik, jk, k + 3);
+                    q3 = fetch<M, SH>(ap, ik, jk, k + 3);
                 }
                 SideT<R> Lnext, Rnext;
                 recon_cell<M>(cast_wu<R>(L2R2U(ap->wu[2])), ho, lim, clip, qa, qb, q3, Lnext, Rnext);
 #pragma unroll
                 for (int q = 0; q < 5; ++q) Fb[q] = Ft[q];
-                if (own_k) side_flux<M, FLUX, 2, 0, 1>(LpC, Rnext, gR, Ft);
+                if (own_k) side_flux<M, FLUX, 2, 0, 1>(LpC, Rnext, gR, Ft, shock_opt<R>(ap));
                 LpC = Lnext;
                 own = qa; qa = qb; qb = q3;
             }
@@ -622,9 +652,11 @@ void launch_stage_batch_mode(const BatchArgs& B, int nwg, bool def, hipStream_t 
 {
     if (dims == 3) {
         if (flux == FLUX_HLLC) launch_stage_batch_df<M, 3, FLUX_HLLC>(B, nwg, def, stream);
+        else if (flux == FLUX_SHOCK) launch_stage_batch_df<M, 3, FLUX_SHOCK>(B, nwg, def, stream);
         else launch_stage_batch_df<M, 3, FLUX_AUSMDV>(B, nwg, def, stream);
     } else {
         if (flux == FLUX_HLLC) launch_stage_batch_df<M, 2, FLUX_HLLC>(B, nwg, def, stream);
+        else if (flux == FLUX_SHOCK) launch_stage_batch_df<M, 2, FLUX_SHOCK>(B, nwg, def, stream);
         else launch_stage_batch_df<M, 2, FLUX_AUSMDV>(B, nwg, def, stream);
     }
 }
@@ -635,11 +667,62 @@ void launch_stage_mode(const StageArgs& A, dim3 grid, hipStream_t stream, int di
 {
     if (dims == 3) {
         if (flux == FLUX_HLLC) launch_stage_df<M, 3, FLUX_HLLC>(A, grid, stream);
+        else if (flux == FLUX_SHOCK) launch_stage_df<M, 3, FLUX_SHOCK>(A, grid, stream);
         else launch_stage_df<M, 3, FLUX_AUSMDV>(A, grid, stream);
     } else {
         if (flux == FLUX_HLLC) launch_stage_df<M, 2, FLUX_HLLC>(A, grid, stream);
+        else if (flux == FLUX_SHOCK) launch_stage_df<M, 2, FLUX_SHOCK>(A, grid, stream);
         else launch_stage_df<M, 2, FLUX_AUSMDV>(A, grid, stream);
     }
+}
+
+// fluidblock.d:415-431 with PJ_ShockDetector (shockdetectors.d) for every
+// active cell from the stage's stencil state: a face is a shock point when
+// (u_n,right - u_n,left) / min(a_left, a_right) < compression tolerance (the
+// face normal of this axis-aligned grid is a unit axis); a cell is one when
+// any of its faces is.  Ghost neighbours are the ghost-cell effects' states.
+template <class M>
+__device__ __forceinline__ Prim prim_at(KArgs A, int i, int j, int k)
+{
+    if (i >= 0 && i < A->ni && j >= 0 && j < A->nj && k >= 0 && k < A->nk) return load_decode<M>(A, cell_index(A, i, j, k));
+    return fetch_ghost<M, false>(A, i, j, k);
+}
+
+__device__ __forceinline__ double prim_vel(const Prim& P, int d) { return d == 0 ? P.v0 : (d == 1 ? P.v1 : P.v2); }
+
+template <class M, int DIMS>
+__global__ __launch_bounds__(256) void shock_detect_kernel(StageArgs)
+{
+    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    KArgs A = kargs();
+    const long n = (long)A->ni * A->nj * A->nk;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int i = (int)(c % A->ni), j = (int)((c / A->ni) % A->nj), k = (int)(c / ((long)A->ni * A->nj));
+    const Prim own = load_decode<M>(A, c);
+    const double tol = A->compression_tol;
+    auto face = [&](const Prim& L, const Prim& R, int d) {
+        const double uL = prim_vel(L, d), uR = prim_vel(R, d);
+        const double a_min = (L.a < R.a) ? L.a : R.a;
+        return dv<MD>(uR - uL, a_min) < tol;
+    };
+    bool S = false;
+#pragma unroll
+    for (int d = 0; d < DIMS; ++d) {
+        const int di = (d == 0), dj = (d == 1), dk = (d == 2);
+        S = S || face(prim_at<M>(A, i - di, j - dj, k - dk), own, d);
+        S = S || face(own, prim_at<M>(A, i + di, j + dj, k + dk), d);
+    }
+    A->Snew[c] = S ? 1 : 0;
+}
+
+template <class M>
+void launch_shock_detect_mode(const StageArgs& A, int dims, hipStream_t stream)
+{
+    const long n = (long)A.ni * A.nj * A.nk;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    if (dims == 3) hipLaunchKernelGGL((shock_detect_kernel<M, 3>), grid, block, 0, stream, A);
+    else hipLaunchKernelGGL((shock_detect_kernel<M, 2>), grid, block, 0, stream, A);
 }
 
 } // namespace e4

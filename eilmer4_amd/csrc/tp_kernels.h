@@ -69,18 +69,24 @@ struct TPCell {
 // Where the values of one stencil position come from: field f (P_* order)
 // is base[f * stride] -- the primitive array of this block or of a connected
 // block (stride qs), or a fixed inflow state (stride 1); a slip wall negates
-// the velocity component `flip` (-1: none).  The same mapping as the
-// ideal-gas stage kernel's fetch_ghost.
+// the velocity component `flip` (-1: none).  s points at the position's
+// shock indicator (this stage's for a cell of the block, the previous
+// detection's for a cell a ghost was copied from; null for an inflow ghost,
+// whose indicator is 0).  The same mapping as the ideal-gas stage kernel's
+// fetch_ghost.
 struct TPRef {
     const double* base;
     long stride;
     int flip;
+    const unsigned char* s;
 };
 
 __device__ __forceinline__ TPRef tp_ref(const TPArgs& A, int i, int j, int k)
 {
-    if (i >= 0 && i < A.ni && j >= 0 && j < A.nj && k >= 0 && k < A.nk)
-        return TPRef{A.Pin + (((long)k * A.nj + j) * A.ni + i), A.qs, -1};
+    if (i >= 0 && i < A.ni && j >= 0 && j < A.nj && k >= 0 && k < A.nk) {
+        const long c = ((long)k * A.nj + j) * A.ni + i;
+        return TPRef{A.Pin + c, A.qs, -1, A.S + c};
+    }
     int dir, idx, n;
     if (i < 0 || i >= A.ni) { dir = 0; idx = i; n = A.ni; }
     else if (j < 0 || j >= A.nj) { dir = 1; idx = j; n = A.nj; }
@@ -92,7 +98,7 @@ __device__ __forceinline__ TPRef tp_ref(const TPArgs& A, int i, int j, int k)
                                 : (plus ? FACE_TOP : FACE_BOTTOM);
     const BCTable* bc = A.bc;
     const int type = bc->type[face];
-    if (type == BC_INFLOW) return TPRef{bc->fs_tp[face], 1, -1};
+    if (type == BC_INFLOW) return TPRef{bc->fs_tp[face], 1, -1, nullptr};
     if (type == BC_CONNECTED) {
         // full_face_copy.d:1277-1357: the mapped interior cell of the other block
         const Connection nb = bc->nbr[face];
@@ -100,7 +106,8 @@ __device__ __forceinline__ TPRef tp_ref(const TPArgs& A, int i, int j, int k)
         const int fb = (face >= FACE_TOP) ? j : k;
         int sc[3];
         connection_source(nb.dims, face, nb.face, nb.orient, nb.n, layer, fa, fb, sc);
-        return TPRef{A.nbrP[face] + (((long)sc[2] * nb.n[1] + sc[1]) * nb.n[0] + sc[0]), nb.qs, -1};
+        const long c = ((long)sc[2] * nb.n[1] + sc[1]) * nb.n[0] + sc[0];
+        return TPRef{A.nbrP[face] + c, nb.qs, -1, A.nbrS[face] + c};
     }
     int src;
     if (type == BC_EXTRAPOLATE) src = plus ? n - 1 : 0;
@@ -108,7 +115,8 @@ __device__ __forceinline__ TPRef tp_ref(const TPArgs& A, int i, int j, int k)
     else src = plus ? n - layer : layer - 1;
     int si = i, sj = j, sk = k;
     if (dir == 0) si = src; else if (dir == 1) sj = src; else sk = src;
-    return TPRef{A.Pin + (((long)sk * A.nj + sj) * A.ni + si), A.qs, (type == BC_SLIP_WALL) ? dir : -1};
+    const long c = ((long)sk * A.nj + sj) * A.ni + si;
+    return TPRef{A.Pin + c, A.qs, (type == BC_SLIP_WALL) ? dir : -1, A.Sprev + c};
 }
 
 __device__ __forceinline__ double tp_val(const TPRef& r, int field) { return r.base[field * r.stride]; }
@@ -201,6 +209,48 @@ __device__ __forceinline__ void tp_ausmdv(const TPCell<NSP>& L, const TPCell<NSP
 #pragma unroll
         for (int i = 0; i < NSP; ++i) F[5 + i] -= d_ua * (rR * Rt.Y[i] - rL * L.Y[i]);
     }
+}
+
+// fluxcalc.d:532-623 with species (Hanel's flux vector splitting).
+template <class M, int NSP>
+__device__ __forceinline__ void tp_hanel(const TPCell<NSP>& L, const TPCell<NSP>& Rt, double* F)
+{
+    const double rL = L.rho, pL = L.p;
+    const double pLrL = dv<M>(pL, rL);
+    const double uL = L.v[0], vL = L.v[1], wL = L.v[2];
+    const double eL = L.u, aL = L.a;
+    const double keL = 0.5 * (uL * uL + vL * vL + wL * wL);
+    const double HL = eL + pLrL + keL;
+    const double rR = Rt.rho, pR = Rt.p;
+    const double pRrR = dv<M>(pR, rR);
+    const double uR = Rt.v[0], vR = Rt.v[1], wR = Rt.v[2];
+    const double eR = Rt.u, aR = Rt.a;
+    const double keR = 0.5 * (uR * uR + vR * vR + wR * wR);
+    const double HR = eR + pRrR + keR;
+    double pLplus, uLplus;
+    if (fabs(uL) <= aL) {
+        uLplus = dv<M>(1.0, 4.0 * aL) * (uL + aL) * (uL + aL);
+        pLplus = pL * uLplus * (dv<M>(1.0, aL) * (2.0 - dv<M>(uL, aL)));
+    } else {
+        uLplus = 0.5 * (uL + fabs(uL));
+        pLplus = pL * uLplus * dv<M>(1.0, uL);
+    }
+    double pRminus, uRminus;
+    if (fabs(uR) <= aR) {
+        uRminus = dv<M>(-1.0, 4.0 * aR) * (uR - aR) * (uR - aR);
+        pRminus = pR * uRminus * (dv<M>(1.0, aR) * (-2.0 - dv<M>(uR, aR)));
+    } else {
+        uRminus = 0.5 * (uR - fabs(uR));
+        pRminus = pR * uRminus * dv<M>(1.0, uR);
+    }
+    const double p_half = pLplus + pRminus;
+    F[0] = uLplus * rL + uRminus * rR;
+    F[1] = uLplus * rL * uL + uRminus * rR * uR + p_half;
+    F[2] = uLplus * rL * vL + uRminus * rR * vR;
+    F[3] = uLplus * rL * wL + uRminus * rR * wR;
+    F[4] = uLplus * rL * HL + uRminus * rR * HR;
+#pragma unroll
+    for (int i = 0; i < NSP; ++i) F[5 + i] = uLplus * rL * L.Y[i] + uRminus * rR * Rt.Y[i];
 }
 
 // HLLC with species carried by the contact wave (the oracle's hllc).
@@ -314,8 +364,20 @@ __global__ __launch_bounds__(128) void tp_face_kernel(TPArgs A)
     if (!okL) Lf = tp_cell<NSP>(r[1], pn, pt1, pt2);
     if (!okR) Rf = tp_cell<NSP>(r[2], pn, pt1, pt2);
     double F[5 + NSP];
-    if constexpr (FLUX == FLUX_HLLC) tp_hllc<MA, NSP>(Lf, Rf, F);
-    else tp_ausmdv<MA, NSP>(Lf, Rf, F);
+    if constexpr (FLUX == FLUX_HLLC) {
+        tp_hllc<MA, NSP>(Lf, Rf, F);
+    } else if constexpr (FLUX == FLUX_SHOCK) {
+        // adaptive_hanel_ausmdv (fluxcalc.d:822-836), or hanel throughout
+        const bool shock = (r[1].s && *r[1].s) || (r[2].s && *r[2].s);
+        const double sound_speed = 0.5 * (Lf.a + Rf.a);
+        const double shear_y = dv<MA>(fabs(Lf.v[1] - Rf.v[1]), sound_speed);
+        const double shear_z = dv<MA>(fabs(Lf.v[2] - Rf.v[2]), sound_speed);
+        const bool shear_is_small = fmax(shear_y, shear_z) <= A.shear_tol;
+        if (A.always_hanel || (shock && shear_is_small)) tp_hanel<MA, NSP>(Lf, Rf, F);
+        else tp_ausmdv<MA, NSP>(Lf, Rf, F);
+    } else {
+        tp_ausmdv<MA, NSP>(Lf, Rf, F);
+    }
     if (active) {
         double* dst = A.F[DIR];
         const long fq = A.fqs[DIR];
@@ -570,6 +632,70 @@ __global__ __launch_bounds__(128) void tp_decode_kernel(TPArgs A, const double* 
         if (fsum_w) atomicAdd(&A.counters[2], (unsigned int)fsum_w);
         if (nfb) atomicAdd(&A.counters[3], nfb);
     }
+}
+
+// fluidblock.d:415-431 with PJ_ShockDetector (shockdetectors.d) for a
+// mixture block: the shock indicator of every active cell from the primitive
+// state the stage starts from (ghost neighbours as the ghost-cell effects
+// give them).
+template <class M>
+__global__ __launch_bounds__(128) void tp_shock_detect_kernel(TPArgs A)
+{
+    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    const long n = (long)A.ni * A.nj * A.nk;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int i = (int)(c % A.ni), j = (int)((c / A.ni) % A.nj), k = (int)(c / ((long)A.ni * A.nj));
+    const TPRef own = tp_ref(A, i, j, k);
+    const double tol = A.compression_tol;
+    auto face = [&](const TPRef& L, const TPRef& R, int d) {
+        const double uL = tp_vel(L, d), uR = tp_vel(R, d);
+        const double aL = tp_val(L, P_A), aR = tp_val(R, P_A);
+        const double a_min = (aL < aR) ? aL : aR;
+        return dv<MD>(uR - uL, a_min) < tol;
+    };
+    bool S = false;
+    const int ndir = (A.dims == 3) ? 3 : 2;
+    for (int d = 0; d < ndir; ++d) {
+        const int di = (d == 0), dj = (d == 1), dk = (d == 2);
+        S = S || face(tp_ref(A, i - di, j - dj, k - dk), own, d);
+        S = S || face(own, tp_ref(A, i + di, j + dj, k + dk), d);
+    }
+    A.Snew[c] = S ? 1 : 0;
+}
+
+// Launch of one stage of a mixture block: the face kernels of every
+// direction, then the update.
+template <class M, int FLUX>
+void launch_tp_faces(const TPArgs& A, hipStream_t stream)
+{
+    const int nfk = (A.dims == 3) ? A.nk : 1;
+    const long nf0 = (long)(A.ni + 1) * A.nj * nfk, nf1 = (long)A.ni * (A.nj + 1) * nfk;
+    const long nf2 = (long)A.ni * A.nj * (A.nk + 1);
+    const dim3 b(128);
+    auto g = [](long n) { return dim3((unsigned)((n + 127) / 128)); };
+    hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX, 0>), g(nf0), b, 0, stream, A);
+    hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX, 1>), g(nf1), b, 0, stream, A);
+    if (A.dims == 3) hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX, 2>), g(nf2), b, 0, stream, A);
+}
+
+template <class M>
+void launch_tp_stage_m(const TPArgs& A, hipStream_t stream, int flux)
+{
+    if (flux == FLUX_HLLC) launch_tp_faces<M, FLUX_HLLC>(A, stream);
+    else if (flux == FLUX_SHOCK) launch_tp_faces<M, FLUX_SHOCK>(A, stream);
+    else launch_tp_faces<M, FLUX_AUSMDV>(A, stream);
+    const long nc = (long)A.ni * A.nj * ((A.dims == 3) ? A.nk : 1);
+    const dim3 grid((unsigned)((nc + 127) / 128)), b(128);
+    if (A.guess) hipLaunchKernelGGL((tp_update_kernel<M, 5, true>), grid, b, 0, stream, A);
+    else hipLaunchKernelGGL((tp_update_kernel<M, 5, false>), grid, b, 0, stream, A);
+}
+
+template <class M>
+void launch_tp_shock_detect_m(const TPArgs& A, hipStream_t stream)
+{
+    const long nc = (long)A.ni * A.nj * ((A.dims == 3) ? A.nk : 1);
+    hipLaunchKernelGGL((tp_shock_detect_kernel<M>), dim3((unsigned)((nc + 127) / 128)), dim3(128), 0, stream, A);
 }
 
 } // namespace e4

@@ -24,7 +24,8 @@ class FluxCalculator(IntEnum):          # fvcore.d:184-194
 
 
 # calculators the device path implements
-DEVICE_FLUX_CALCULATORS = (FluxCalculator.ausmdv, FluxCalculator.hllc)
+DEVICE_FLUX_CALCULATORS = (FluxCalculator.ausmdv, FluxCalculator.hanel, FluxCalculator.adaptive_hanel_ausmdv,
+                           FluxCalculator.hllc)
 
 
 def flux_calculator_name(fcalc):

@@ -38,7 +38,8 @@ class StepConfig(C.Structure):
                 ("interpolation_order", C.c_int), ("apply_limiter", C.c_int),
                 ("extrema_clipping", C.c_int), ("interpolate_in_local_frame", C.c_int),
                 ("thermo_interpolator", C.c_int), ("precision_mode", C.c_int),
-                ("max_invalid_cells", C.c_int), ("learned_eos_guess", C.c_int), ("batch_blocks", C.c_int)]
+                ("max_invalid_cells", C.c_int), ("learned_eos_guess", C.c_int), ("batch_blocks", C.c_int),
+                ("shear_tolerance", C.c_double), ("compression_tolerance", C.c_double)]
 
 
 _lib = None

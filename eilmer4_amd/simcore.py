@@ -34,6 +34,8 @@ class GlobalConfig:
     max_invalid_cells: int = 0
     learned_eos_guess: bool = False
     batch_blocks: bool = True
+    shear_tolerance: float = 0.20           # globalconfig.d:585
+    compression_tolerance: float = -0.30    # globalconfig.d:600
     cfl_value: float = 0.5
     cfl_count: int = 10
     dt_init: float = 1.0e-3
@@ -50,7 +52,8 @@ class GlobalConfig:
                                int(self.interpolation_order), int(bool(self.apply_limiter)),
                                int(bool(self.extrema_clipping)), int(bool(self.interpolate_in_local_frame)),
                                int(self.thermo_interpolator), int(self.precision_mode), int(self.max_invalid_cells),
-                               int(bool(self.learned_eos_guess)), int(bool(self.batch_blocks)))
+                               int(bool(self.learned_eos_guess)), int(bool(self.batch_blocks)),
+                               float(self.shear_tolerance), float(self.compression_tolerance))
 
 
 class SimState:

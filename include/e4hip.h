@@ -43,6 +43,10 @@ extern "C" {
 
 /* ---- FluxCalculator (eilmer/fvcore.d:184-194; values kept, hllc appended) */
 #define E4HIP_FLUX_AUSMDV      0
+#define E4HIP_FLUX_HANEL       1   /* fluxcalc.d:532-623 */
+#define E4HIP_FLUX_ADAPTIVE_HANEL_AUSMDV 5  /* the reference's default (globalconfig.d:533):
+                                              Hanel at shock points with small shear,
+                                              AUSMDV elsewhere (fluxcalc.d:822-836) */
 #define E4HIP_FLUX_HLLC        9
 
 /* ---- GasdynamicUpdate (eilmer/fvcore.d:30-41) */
@@ -101,6 +105,8 @@ typedef struct E4hipStepConfig {
     int batch_blocks;                /* 1: the ideal-gas blocks of one stage (of one
                                         dimensionality) go into one kernel launch when there
                                         are two or more; 0: one launch per block */
+    double shear_tolerance;          /* adaptive_hanel_ausmdv: default 0.20 (globalconfig.d:585) */
+    double compression_tolerance;    /* shock detector: default -0.30 (globalconfig.d:600) */
 } E4hipStepConfig;
 
 /* Library set-up on one GPU; creates the library's HIP stream.

@@ -31,7 +31,7 @@ enum { NSP_MAX = 8, NSEG_MAX = 4, NQ_MAX = 5 + NSP_MAX };
 enum { GAS_IDEAL = 0, GAS_TP = 1 };
 // FluxCalculator values follow the reference's enum order (eilmer/fvcore.d:184-194);
 // hllc is the new member appended after roe.
-enum { FLUX_AUSMDV = 0, FLUX_HLLC = 9 };
+enum { FLUX_AUSMDV = 0, FLUX_HANEL = 1, FLUX_ADAPTIVE_HANEL_AUSMDV = 5, FLUX_HLLC = 9 };
 // GasdynamicUpdate (eilmer/fvcore.d:30-41).
 enum { UPD_EULER = 0, UPD_PC = 1, UPD_MIDPOINT = 2, UPD_CLASSIC_RK3 = 3, UPD_TVD_RK3 = 4, UPD_DENMAN_RK3 = 5 };
 // InterpolateOption (eilmer/fvcore.d:155).
@@ -66,7 +66,8 @@ struct Gas {
 };
 
 struct GasState { double rho, p, T, u, a; double massf[NSP_MAX]; };
-struct FlowState { GasState gas; double vel[3]; };
+// S: the shock indicator (flowstate.d:44), 0 or 1; copied with the flow state
+struct FlowState { GasState gas; double vel[3]; int S; };
 
 // ---------------------------------------------------------------- gas models
 
@@ -509,6 +510,8 @@ struct Config {
     int extrema_clipping;
     int thermo_interp;
     int max_invalid_cells;
+    double shear_tolerance;         // globalconfig.d:585
+    double compression_tolerance;   // globalconfig.d:600
 };
 
 // onedinterp.d:435-585 together with the EOS mixins of onedinterp.d:1039-1068.
@@ -716,6 +719,63 @@ static void hllc(const Gas* g, const FlowState& Lft, const FlowState& Rght, doub
     for (int q = 0; q < nq; ++q) F[q] = FK[q] + SK * (Us[q] - UK[q]);
 }
 
+// fluxcalc.d:532-623 (Hanel's flux vector splitting; velocities in the face frame)
+static void hanel(const Gas* g, const FlowState& Lft, const FlowState& Rght, double* F)
+{
+    const double rL = Lft.gas.rho, pL = Lft.gas.p;
+    const double pLrL = pL / rL;
+    const double uL = Lft.vel[0], vL = Lft.vel[1], wL = Lft.vel[2];
+    const double eL = Lft.gas.u, aL = Lft.gas.a;
+    const double keL = 0.5 * (uL * uL + vL * vL + wL * wL);
+    const double HL = eL + pLrL + keL;
+    const double rR = Rght.gas.rho, pR = Rght.gas.p;
+    const double pRrR = pR / rR;
+    const double uR = Rght.vel[0], vR = Rght.vel[1], wR = Rght.vel[2];
+    const double eR = Rght.gas.u, aR = Rght.gas.a;
+    const double keR = 0.5 * (uR * uR + vR * vR + wR * wR);
+    const double HR = eR + pRrR + keR;
+    // left: pressure and velocity splitting (Wada and Liou 1997, eqs. 7, 9)
+    double pLplus, uLplus;
+    if (fabs(uL) <= aL) {
+        uLplus = 1.0 / (4.0 * aL) * (uL + aL) * (uL + aL);
+        pLplus = pL * uLplus * (1.0 / aL * (2.0 - uL / aL));
+    } else {
+        uLplus = 0.5 * (uL + fabs(uL));
+        pLplus = pL * uLplus * (1.0 / uL);
+    }
+    double pRminus, uRminus;
+    if (fabs(uR) <= aR) {
+        uRminus = -1.0 / (4.0 * aR) * (uR - aR) * (uR - aR);
+        pRminus = pR * uRminus * (1.0 / aR * (-2.0 - uR / aR));
+    } else {
+        uRminus = 0.5 * (uR - fabs(uR));
+        pRminus = pR * uRminus * (1.0 / uR);
+    }
+    const double p_half = pLplus + pRminus;
+    F[0] = uLplus * rL + uRminus * rR;
+    F[1] = uLplus * rL * uL + uRminus * rR * uR + p_half;
+    F[2] = uLplus * rL * vL + uRminus * rR * vR;
+    F[3] = uLplus * rL * wL + uRminus * rR * wR;
+    F[4] = uLplus * rL * HL + uRminus * rR * HR;
+    const int nsp = g->nsp;
+    if (nsp > 1)
+        for (int i = 0; i < nsp; ++i)
+            F[5 + i] = uLplus * rL * Lft.gas.massf[i] + uRminus * rR * Rght.gas.massf[i];
+}
+
+// fluxcalc.d:822-836: Hanel across a shock (either side flagged) where the
+// tangential velocity jump is small, AUSMDV elsewhere.
+static void adaptive_hanel_ausmdv(const Gas* g, const Config& cfg, const FlowState& Lft, const FlowState& Rght,
+                                  double* F)
+{
+    const double sound_speed = 0.5 * (Lft.gas.a + Rght.gas.a);
+    const double shear_y = fabs(Lft.vel[1] - Rght.vel[1]) / sound_speed;
+    const double shear_z = fabs(Lft.vel[2] - Rght.vel[2]) / sound_speed;
+    const bool shear_is_small = fmax(shear_y, shear_z) <= cfg.shear_tolerance;
+    if ((Lft.S == 1 || Rght.S == 1) && shear_is_small) hanel(g, Lft, Rght, F);
+    else ausmdv(g, Lft, Rght, F);
+}
+
 // fluxcalc.d:26-128 on a fixed grid (gvel = 0, omegaz = 0).
 static void compute_interface_flux(const Gas* g, const Config& cfg, FlowState Lft, FlowState Rght,
                                    const Frame& fr, double* F)
@@ -723,6 +783,8 @@ static void compute_interface_flux(const Gas* g, const Config& cfg, FlowState Lf
     to_local(Lft.vel, fr);
     to_local(Rght.vel, fr);
     if (cfg.flux_calc == FLUX_HLLC) hllc(g, Lft, Rght, F);
+    else if (cfg.flux_calc == FLUX_HANEL) hanel(g, Lft, Rght, F);
+    else if (cfg.flux_calc == FLUX_ADAPTIVE_HANEL_AUSMDV) adaptive_hanel_ausmdv(g, cfg, Lft, Rght, F);
     else ausmdv(g, Lft, Rght, F);
     // The grid-velocity terms of fluxcalc.d:112-121 add exact zeros on a fixed grid.
     double m[3] = {F[1], F[2], F[3]};
@@ -741,7 +803,10 @@ int ref_flux(Gas* g, int calc, const double* L, const double* R, double* F)
         for (int i = 0; i < g->nsp; ++i) f.gas.massf[i] = s[8 + i];
     };
     unpack(L, l); unpack(R, r);
-    if (calc == FLUX_HLLC) hllc(g, l, r, F); else ausmdv(g, l, r, F);
+    l.S = r.S = 0;
+    if (calc == FLUX_HLLC) hllc(g, l, r, F);
+    else if (calc == FLUX_HANEL) hanel(g, l, r, F);
+    else ausmdv(g, l, r, F);
     return 0;
 }
 
@@ -773,7 +838,7 @@ Block* ref_block_new(Gas* g, int dims, int ni, int nj, int nk, double dx, double
 {
     Block* b = new Block();
     b->gas = g;
-    b->cfg = Config{dims, FLUX_AUSMDV, UPD_PC, 2, 1, 1, INTERP_RHOU, 0};
+    b->cfg = Config{dims, FLUX_AUSMDV, UPD_PC, 2, 1, 1, INTERP_RHOU, 0, 0.20, -0.30};
     b->ni = ni; b->nj = nj; b->nk = (dims == 3) ? nk : 1;
     b->nidim = ni + 4; b->njdim = nj + 4; b->nkdim = (dims == 3) ? nk + 4 : 1;
     b->imin = 2; b->imax = ni + 1; b->jmin = 2; b->jmax = nj + 1;
@@ -803,6 +868,12 @@ Block* ref_block_new(Gas* g, int dims, int ni, int nj, int nk, double dx, double
 }
 
 void ref_block_free(Block* b) { delete b; }
+
+void ref_block_set_shock_tolerances(Block* b, double shear_tolerance, double compression_tolerance)
+{
+    b->cfg.shear_tolerance = shear_tolerance;
+    b->cfg.compression_tolerance = compression_tolerance;
+}
 
 void ref_block_set_config(Block* b, int flux_calc, int update_scheme, int interp_order, int apply_limiter,
                           int extrema_clipping, int thermo_interp, int max_invalid_cells)
@@ -944,6 +1015,15 @@ void ref_block_get_U(const Block* b, int ftl, double* U)
             }
 }
 
+// shock indicators of the active cells, i fastest
+void ref_block_get_S(const Block* b, int* out)
+{
+    size_t c = 0;
+    for (int k = b->kmin; k <= b->kmax; ++k)
+        for (int j = b->jmin; j <= b->jmax; ++j)
+            for (int i = b->imin; i <= b->imax; ++i, ++c) out[c] = b->fs[cidx(b, i, j, k)].S;
+}
+
 // fs fields: rho,p,T,u,a,vx,vy,vz,massf... as [nfield][ncells]
 void ref_block_get_fs(const Block* b, double* out)
 {
@@ -1082,6 +1162,43 @@ static void apply_bcs(Block* b)
                     (void)n;
                 }
     }
+}
+
+// fluidblock.d:415-431 with shockdetectors.d (PJ_ShockDetector): a face is a
+// shock point when the normal velocity jump across it, scaled by the smaller
+// sound speed, is below the compression tolerance; a cell is a shock point
+// when any of its faces is.  Every face has a cell on both sides here (the
+// ghost cells of all these boundary conditions hold data), and the ghost
+// cells keep the indicator they were filled with.
+static inline int pj_shock(const FlowState& cL, const FlowState& cR, const Frame& fr, double tol)
+{
+    const double uL = cL.vel[0] * fr.n[0] + cL.vel[1] * fr.n[1] + cL.vel[2] * fr.n[2];
+    const double uR = cR.vel[0] * fr.n[0] + cR.vel[1] * fr.n[1] + cR.vel[2] * fr.n[2];
+    const double aL = cL.gas.a, aR = cR.gas.a;
+    const double a_min = (aL < aR) ? aL : aR;
+    return ((uR - uL) / a_min) < tol;
+}
+
+static void detect_shock_points(Block* b)
+{
+    const int ndir = (b->cfg.dims == 3) ? 3 : 2;
+    const double tol = b->cfg.compression_tolerance;
+    const size_t st[3] = {1, (size_t)b->nidim, (size_t)b->nidim * b->njdim};
+    std::vector<int> S((size_t)b->nidim * b->njdim * b->nkdim, 0);
+    for (int k = b->kmin; k <= b->kmax; ++k)
+        for (int j = b->jmin; j <= b->jmax; ++j)
+            for (int i = b->imin; i <= b->imax; ++i) {
+                const size_t c = cidx(b, i, j, k);
+                int s = 0;
+                for (int d = 0; d < ndir; ++d) {
+                    s |= pj_shock(b->fs[c - st[d]], b->fs[c], b->frame[d], tol);   // minus face
+                    s |= pj_shock(b->fs[c], b->fs[c + st[d]], b->frame[d], tol);   // plus face
+                }
+                S[c] = s;
+            }
+    for (int k = b->kmin; k <= b->kmax; ++k)
+        for (int j = b->jmin; j <= b->jmax; ++j)
+            for (int i = b->imin; i <= b->imax; ++i) b->fs[cidx(b, i, j, k)].S = S[cidx(b, i, j, k)];
 }
 
 // sfluidblock.d:2016-2346, the classic (l2r2) branch; all BCs here keep ghost
@@ -1235,8 +1352,13 @@ int ref_block_step(Block* b, double dt, int* nbad)
 int ref_multi_step(Block** blks, int nblk, double dt, int* nbad)
 {
     const int ns = nstages(blks[0]->cfg.update_scheme);
+    // the adaptive flux calculators switch the shock detector on (globalconfig.d:1942-1954)
+    const bool detect = blks[0]->cfg.flux_calc == FLUX_ADAPTIVE_HANEL_AUSMDV;
     for (int stage = 1; stage <= ns; ++stage) {
         for (int n = 0; n < nblk; ++n) apply_bcs(blks[n]);
+        // simcore.d:2352-2358: after the ghost-cell fill, before the fluxes
+        if (detect)
+            for (int n = 0; n < nblk; ++n) detect_shock_points(blks[n]);
         int total = 0, any_fail = 0, too_many = 0;
         for (int n = 0; n < nblk; ++n) {
             int decode_fail = 0, invalid = 0;

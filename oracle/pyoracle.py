@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libe4ref.so")
 DATA = os.path.join(os.path.dirname(HERE), "eilmer4_amd", "data")
 
 # enum values shared with the product ABI (reference enums, eilmer/fvcore.d)
-AUSMDV, HLLC = 0, 9
+AUSMDV, HANEL, ADAPTIVE_HANEL_AUSMDV, HLLC = 0, 1, 5, 9
 EULER, PC, MIDPOINT, CLASSIC_RK3, TVD_RK3, DENMAN_RK3 = range(6)
 INTERP_PT, INTERP_RHOU, INTERP_RHOP, INTERP_RHOT = range(4)
 BC_SLIP_WALL, BC_INFLOW, BC_EXTRAPOLATE, BC_PERIODIC, BC_CONNECTED = range(5)
@@ -57,6 +57,8 @@ def lib():
         L.ref_block_new.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
         L.ref_block_free.argtypes = [vp]
         L.ref_block_set_config.argtypes = [vp] + [C.c_int] * 7
+        L.ref_block_set_shock_tolerances.argtypes = [vp, C.c_double, C.c_double]
+        L.ref_block_get_S.argtypes = [vp, C.POINTER(C.c_int)]
         L.ref_block_set_bc.argtypes = [vp, C.c_int, C.c_int, _dp]
         L.ref_block_set_from_pT.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.ref_block_set_U.argtypes = [vp, _dp, _dp]
@@ -191,6 +193,14 @@ class Block:
                    extrema_clipping=1, thermo_interp=INTERP_RHOU, max_invalid_cells=0):
         lib().ref_block_set_config(self.h, flux_calc, update_scheme, interp_order, apply_limiter,
                                    extrema_clipping, thermo_interp, max_invalid_cells)
+
+    def set_shock_tolerances(self, shear_tolerance=0.20, compression_tolerance=-0.30):
+        lib().ref_block_set_shock_tolerances(self.h, shear_tolerance, compression_tolerance)
+
+    def get_S(self):
+        out = np.zeros(self.ncells, dtype=np.int32)
+        lib().ref_block_get_S(self.h, out.ctypes.data_as(C.POINTER(C.c_int)))
+        return out
 
     def set_bc(self, face, kind, fs=None):
         lib().ref_block_set_bc(self.h, face, kind, _p(_f64(fs)) if fs is not None else None)

@@ -24,10 +24,11 @@ def physical_flux(s):
     return np.array([rho * vn, rho * vn * vn + p, rho * vn * vt1, rho * vn * vt2, vn * (E + p)])
 
 
-@pytest.mark.parametrize("calc", [O.AUSMDV, O.HLLC])
+@pytest.mark.parametrize("calc", [O.AUSMDV, O.HANEL, O.HLLC])
 def test_flux_consistency(calc):
     """F(U, U) is the physical flux (AUSMDV: Wada & Liou 1994 eq. 29-37 reduce
-    to it for equal states; HLLC: Toro 10.4)."""
+    to it for equal states; Hanel: the van Leer split velocities and
+    pressures sum to u and p; HLLC: Toro 10.4)."""
     gas = O.Gas.ideal()
     rng = np.random.default_rng(1)
     for _ in range(200):
@@ -41,7 +42,7 @@ def test_flux_consistency(calc):
         assert np.max(np.abs(F - Fp)) / scale < 1e-12
 
 
-@pytest.mark.parametrize("calc", [O.AUSMDV, O.HLLC])
+@pytest.mark.parametrize("calc", [O.AUSMDV, O.HANEL, O.HLLC])
 def test_supersonic_upwinding(calc):
     """Both sides supersonic to the right: the flux is the left state's."""
     gas = O.Gas.ideal()
@@ -56,7 +57,7 @@ def test_supersonic_upwinding(calc):
         assert np.max(np.abs(F - Fp)) / np.abs(Fp).max() < 1e-12
 
 
-@pytest.mark.parametrize("calc", [O.AUSMDV, O.HLLC])
+@pytest.mark.parametrize("calc", [O.AUSMDV, O.HANEL, O.HLLC])
 def test_flux_mirror_symmetry(calc):
     """Swapping the states and negating normal velocities negates the mass
     and energy flux and keeps the normal momentum flux."""
@@ -110,7 +111,7 @@ def periodic_box(n, rng, calc=O.AUSMDV, amp=0.1):
     return b
 
 
-@pytest.mark.parametrize("calc", [O.AUSMDV, O.HLLC])
+@pytest.mark.parametrize("calc", [O.AUSMDV, O.HANEL, O.ADAPTIVE_HANEL_AUSMDV, O.HLLC])
 def test_free_stream_preservation(calc):
     """A uniform state in a periodic box does not change (every flux pair cancels)."""
     g = O.Gas.ideal()
@@ -126,7 +127,7 @@ def test_free_stream_preservation(calc):
     assert np.array_equal(b.get_U(), U0)
 
 
-@pytest.mark.parametrize("calc", [O.AUSMDV, O.HLLC])
+@pytest.mark.parametrize("calc", [O.AUSMDV, O.HANEL, O.ADAPTIVE_HANEL_AUSMDV, O.HLLC])
 def test_conservation_periodic(calc):
     """Sum over cells of mass, momentum and energy is conserved in a periodic
     box (flux differencing); only summation rounding remains."""
@@ -177,13 +178,13 @@ def test_axis_permutation_symmetry():
         assert np.max(np.abs(Ub[qb] - ref)) / np.abs(ref).max() < 1e-12
 
 
-def test_stationary_normal_shock():
-    """A Mach-3 normal shock with Rankine-Hugoniot states (the relations of
-    gasdyn/idealgasflow.d:115-186) stays within one cell over 200 steps."""
+def shock_block(calc=O.AUSMDV, ni=64):
+    """A Mach-3 normal shock at x = 0.5 with Rankine-Hugoniot states (the
+    relations of gasdyn/idealgasflow.d:115-186), supersonic inflow, extrapolated
+    outflow."""
     g = O.Gas.ideal()
-    ni = 64
     b = O.Block(g, 3, ni, 2, 2, 1.0 / ni, 0.05, 0.05)
-    b.set_config()
+    b.set_config(flux_calc=calc)
     M = 3.0
     rho1, p1 = 1.0, 1e5
     a1 = np.sqrt(G * p1 / rho1)
@@ -201,6 +202,14 @@ def test_stationary_normal_shock():
                   0.0, 0.0)
     b.set_bc(O.WEST, O.BC_INFLOW, O.flowstate_pT(g, p1, p1 / (rho1 * RGAS), (u1, 0, 0)))
     b.set_bc(O.EAST, O.BC_EXTRAPOLATE)
+    return b, shape
+
+
+@pytest.mark.parametrize("calc", [O.AUSMDV, O.ADAPTIVE_HANEL_AUSMDV])
+def test_stationary_normal_shock(calc):
+    """The Mach-3 shock stays within one cell over 200 steps."""
+    ni = 64
+    b, shape = shock_block(calc, ni)
     sim = O.Simulation(b)
     sim.run(max_step=200)
     r_ = b.get_fs()[0].reshape(shape)[0, 0]
@@ -208,7 +217,39 @@ def test_stationary_normal_shock():
     assert abs(pos + 1 - ni // 2) <= 1
 
 
-@pytest.mark.parametrize("calc", [O.AUSMDV, O.HLLC])
+def test_shock_detector_flags_the_shock_cells_only():
+    """PJ_ShockDetector (shockdetectors.d): across the shock face (u2 - u1)/a1
+    = -2.2 < -0.3, so the two cells beside it are shock points; in the uniform
+    regions nothing is.  Away from the flagged cells the adaptive flux is
+    AUSMDV, so one step of each differs only near the shock."""
+    ni = 64
+    b, shape = shock_block(O.ADAPTIVE_HANEL_AUSMDV, ni)
+    a, _ = shock_block(O.AUSMDV, ni)
+    dt = 1.0e-6
+    assert b.step(dt)[0] == 0 and a.step(dt)[0] == 0
+    S = b.get_S().reshape(shape)
+    assert np.all(S == S[:1, :1, :])                   # the same on every (j, k) line
+    flagged = np.nonzero(S[0, 0])[0]
+    assert set(flagged) == {ni // 2 - 1, ni // 2}, flagged
+    Ub, Ua = b.get_U().reshape(5, *shape), a.get_U().reshape(5, *shape)
+    differ = np.nonzero(np.any(Ub != Ua, axis=(0, 1, 2)))[0]
+    assert len(differ) > 0 and differ.min() >= ni // 2 - 3 and differ.max() <= ni // 2 + 2, differ
+
+
+def test_shock_detector_compression_tolerance():
+    """With the tolerance below the shock's compression nothing is flagged and
+    the adaptive flux reduces to AUSMDV bit for bit."""
+    ni = 32
+    b, shape = shock_block(O.ADAPTIVE_HANEL_AUSMDV, ni)
+    b.set_shock_tolerances(0.20, -3.0)
+    a, _ = shock_block(O.AUSMDV, ni)
+    for _ in range(3):
+        assert b.step(1.0e-6)[0] == 0 and a.step(1.0e-6)[0] == 0
+    assert not b.get_S().any()
+    assert np.array_equal(b.get_U(), a.get_U())
+
+
+@pytest.mark.parametrize("calc", [O.AUSMDV, O.HANEL, O.ADAPTIVE_HANEL_AUSMDV, O.HLLC])
 @pytest.mark.parametrize("case", [
     # Toro 2009, table 4.1 tests 1, 2 and 4 (left/right rho, u, p; end time;
     # diaphragm; pressure scale to air-like units)
