@@ -98,7 +98,8 @@ def tp_workload(e4, workloads, L, n=128, steps=5, warmup=1):
         blk = e4.SFluidBlock(gas, 3, n, n, n, 1.0 / n, 1.0 / n, 1.0 / n)
         for face, (kind, fs) in bcs.items():
             blk.set_bc(face, kind, fs)
-        sim = e4.Simulation([blk], e4.GlobalConfig(learned_eos_guess=learned,
+        sim = e4.Simulation([blk], e4.GlobalConfig(flux_calculator=e4.FluxCalculator.ausmdv,
+                                                   learned_eos_guess=learned,
                                                    precision_mode=e4.PrecisionMode[precision]))
         blk.upload(U0, Tg)
         if dt_fixed is None:
@@ -166,7 +167,8 @@ def c5_workload(e4, workloads, steps=3, warmup=1):
             blks.append(b)
         for (a, fa, b, fb, o) in conns:
             e4.connect_blocks(blks[a], fa, blks[b], fb, o)
-        sim = e4.Simulation(blks, e4.GlobalConfig(precision_mode=e4.PrecisionMode[prec], batch_blocks=batch,
+        sim = e4.Simulation(blks, e4.GlobalConfig(flux_calculator=e4.FluxCalculator.ausmdv,
+                                                  precision_mode=e4.PrecisionMode[prec], batch_blocks=batch,
                                                   learned_eos_guess=learned))
         for b, d in zip(blks, descs):
             b.upload(d["U0"], d["Tguess"])
@@ -221,7 +223,8 @@ def accuracy_study(e4, workloads, sizes=(256, 384), checkpoints=(1, 100, 1000), 
             blk = e4.SFluidBlock(gas, 3, n, n, n, geom["dx"], geom["dy"], geom["dz"])
             for face, (kind, fs) in bcs.items():
                 blk.set_bc(face, kind, fs)
-            sim = e4.Simulation([blk], e4.GlobalConfig(precision_mode=e4.PrecisionMode[prec]))
+            sim = e4.Simulation([blk], e4.GlobalConfig(flux_calculator=e4.FluxCalculator.ausmdv,
+                                                       precision_mode=e4.PrecisionMode[prec]))
             blk.upload(U0)
             runs[prec] = (blk, sim)
         del U0

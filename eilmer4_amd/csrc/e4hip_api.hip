@@ -508,7 +508,8 @@ int launch_tp_stage(BlockH& b, int stage, double dt, unsigned int* counters)
     // the fast modes evaluate the mixture form, which needs three-segment curves
     // with shared break points; other mixtures run the reference-form kernels
     const bool mix_form = tp_mixture_form(g_state.gases[b.gm].tp);
-    if (cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT || !mix_form) e4::launch_tp_stage_exact(A, g_state.stream, flux);
+    if (cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT || !mix_form)
+        e4::launch_tp_stage_exact(A, g_state.stream, flux);
     else if (cfg.precision_mode == E4HIP_PRECISION_MIXED) e4::launch_tp_stage_mixed(A, g_state.stream, flux);
     else e4::launch_tp_stage_fast64(A, g_state.stream, flux);
     prof_end(0, ev);

@@ -520,13 +520,15 @@ void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type A
             if (tidk < TY) {
                 if (j0 + tidk < ap->nj) {
                     const int si = tidk * (TX + 1);
-                    side_flux<M, FLUX, 0, 1, 2>(lds_side(sLw, TY, tidk), lds_side(sI, NSI, si), gR, Fx, shock_opt<R>(ap));
+                    side_flux<M, FLUX, 0, 1, 2>(lds_side(sLw, TY, tidk), lds_side(sI, NSI, si), gR, Fx,
+                                                shock_opt<R>(ap));
                     lds_put_flux(sI, NSI, si, Fx);
                 }
             } else {
                 const int c = tidk - TY;
                 if (i0 + c < ap->ni) {
-                    side_flux<M, FLUX, JN, JT1, JT2>(lds_side(sLs, TX, c), lds_side(sJ, NSJ, c), gR, Fx, shock_opt<R>(ap));
+                    side_flux<M, FLUX, JN, JT1, JT2>(lds_side(sLs, TX, c), lds_side(sJ, NSJ, c), gR, Fx,
+                                                     shock_opt<R>(ap));
                     lds_put_flux(sJ, NSJ, c, Fx);
                 }
             }
@@ -684,7 +686,8 @@ void launch_stage_mode(const StageArgs& A, dim3 grid, hipStream_t stream, int di
 template <class M>
 __device__ __forceinline__ Prim prim_at(KArgs A, int i, int j, int k)
 {
-    if (i >= 0 && i < A->ni && j >= 0 && j < A->nj && k >= 0 && k < A->nk) return load_decode<M>(A, cell_index(A, i, j, k));
+    if (i >= 0 && i < A->ni && j >= 0 && j < A->nj && k >= 0 && k < A->nk)
+        return load_decode<M>(A, cell_index(A, i, j, k));
     return fetch_ghost<M, false>(A, i, j, k);
 }
 

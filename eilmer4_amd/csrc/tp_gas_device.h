@@ -407,7 +407,10 @@ __device__ __forceinline__ int tp_update_temperature_x(const TPGas& g, const TPM
     for (int attempt = 0; attempt < 2; ++attempt) {
         const double lo = attempt ? 20.0 : T1, hi = attempt ? 100000.0 : T1 + delT;
         double root;
-        if (tp_newton<M, NSP>(g, X, Rmix, Q, e_tgt, Tsave, lo, hi, 1.0e-6, &root, fevals) == 0) { Q.T = root; return 0; }
+        if (tp_newton<M, NSP>(g, X, Rmix, Q, e_tgt, Tsave, lo, hi, 1.0e-6, &root, fevals) == 0) {
+            Q.T = root;
+            return 0;
+        }
     }
     Q.T = Tsave;
     if constexpr (M::kFast) {

@@ -21,9 +21,8 @@ from .fvcore import (FlowSolverException, FluxCalculator, GasdynamicUpdate, Inte
 @dataclass
 class GlobalConfig:
     """The GlobalConfig/LocalConfig fields the path reads, with the reference
-    defaults (eilmer/globalconfig.d:440-569, 707-734) except flux_calculator,
-    whose reference default (adaptive_hanel_ausmdv) is not on the device path."""
-    flux_calculator: int = FluxCalculator.ausmdv
+    defaults (eilmer/globalconfig.d:440-569, 585, 600, 707-734)."""
+    flux_calculator: int = FluxCalculator.adaptive_hanel_ausmdv
     gasdynamic_update_scheme: int = GasdynamicUpdate.pc
     interpolation_order: int = 2
     apply_limiter: bool = True

@@ -89,7 +89,8 @@ extern "C" {
 /* Step configuration: the LocalConfig fields the path reads
  * (eilmer/globalconfig.d:440-569, 875, 1008). */
 typedef struct E4hipStepConfig {
-    int flux_calculator;             /* E4HIP_FLUX_*; reference default ausmdv here */
+    int flux_calculator;             /* E4HIP_FLUX_*; the reference's default is
+                                        E4HIP_FLUX_ADAPTIVE_HANEL_AUSMDV (globalconfig.d:533) */
     int gasdynamic_update_scheme;    /* E4HIP_UPDATE_*; default pc */
     int interpolation_order;         /* 1 or 2; default 2 */
     int apply_limiter;               /* van Albada on/off; default 1 */

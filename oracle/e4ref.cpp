@@ -838,7 +838,8 @@ Block* ref_block_new(Gas* g, int dims, int ni, int nj, int nk, double dx, double
 {
     Block* b = new Block();
     b->gas = g;
-    b->cfg = Config{dims, FLUX_AUSMDV, UPD_PC, 2, 1, 1, INTERP_RHOU, 0, 0.20, -0.30};
+    // the reference's defaults (globalconfig.d:440-600): adaptive_hanel_ausmdv, pc, ...
+    b->cfg = Config{dims, FLUX_ADAPTIVE_HANEL_AUSMDV, UPD_PC, 2, 1, 1, INTERP_RHOU, 0, 0.20, -0.30};
     b->ni = ni; b->nj = nj; b->nk = (dims == 3) ? nk : 1;
     b->nidim = ni + 4; b->njdim = nj + 4; b->nkdim = (dims == 3) ? nk + 4 : 1;
     b->imin = 2; b->imax = ni + 1; b->jmin = 2; b->jmax = nj + 1;

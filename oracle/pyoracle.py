@@ -189,7 +189,7 @@ class Block:
         except Exception:
             pass
 
-    def set_config(self, flux_calc=AUSMDV, update_scheme=PC, interp_order=2, apply_limiter=1,
+    def set_config(self, flux_calc=ADAPTIVE_HANEL_AUSMDV, update_scheme=PC, interp_order=2, apply_limiter=1,
                    extrema_clipping=1, thermo_interp=INTERP_RHOU, max_invalid_cells=0):
         lib().ref_block_set_config(self.h, flux_calc, update_scheme, interp_order, apply_limiter,
                                    extrema_clipping, thermo_interp, max_invalid_cells)

@@ -125,7 +125,8 @@ def test_sod_shock_tube_on_device(gpu):
     p = np.broadcast_to(np.where(x < 0.5, 1.0e5, 1.0e4), (nk, nj, ni))
     T = np.broadcast_to(np.where(x < 0.5, 348.4, 278.8), (nk, nj, ni))
     b.set_from_pT(p, T, 0.0, 0.0, 0.0)
-    sim = e4.Simulation([b], e4.GlobalConfig(max_time=0.6e-3, max_step=600))
+    sim = e4.Simulation([b], e4.GlobalConfig(flux_calculator=e4.FluxCalculator.ausmdv,   # sod.lua
+                                             max_time=0.6e-3, max_step=600))
     steps = sim.integrate_in_time()
     U, fs = b.download(with_fs=True)
     fs = fs.reshape(8, nk, nj, ni)

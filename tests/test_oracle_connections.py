@@ -78,7 +78,7 @@ def test_decomposition_invariance_3d():
     ni, nj, nk, cut = 12, 10, 8, 5
     p, T, v = _field((nk, nj, ni), 7)
     whole = O.Block(gas, 3, ni, nj, nk, 0.1, 0.1, 0.1)
-    whole.set_config()
+    whole.set_config(flux_calc=O.AUSMDV)
     whole.set_from_pT(p, T, *v)
     whole.set_bc(O.NORTH, O.BC_PERIODIC)
     whole.set_bc(O.SOUTH, O.BC_PERIODIC)
@@ -86,7 +86,7 @@ def test_decomposition_invariance_3d():
     A = O.Block(gas, 3, cut, nj, nk, 0.1, 0.1, 0.1)
     B = O.Block(gas, 3, ni - cut, nj, nk, 0.1, 0.1, 0.1)
     for blk, sl in ((A, slice(0, cut)), (B, slice(cut, ni))):
-        blk.set_config()
+        blk.set_config(flux_calc=O.AUSMDV)
         blk.set_from_pT(p[:, :, sl], T[:, :, sl], *[x[:, :, sl] for x in v])
         O.connect_blocks(blk, O.NORTH, blk, O.SOUTH, 0)
     O.connect_blocks(A, O.EAST, B, O.WEST, 0)

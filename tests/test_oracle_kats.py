@@ -86,7 +86,7 @@ def _sod_block():
     g = O.Gas.ideal()
     ni, nj, nk = 100, 2, 2
     b = O.Block(g, 3, ni, nj, nk, 1.0 / ni, 0.1 / nj, 0.1 / nk)
-    b.set_config()
+    b.set_config(flux_calc=O.AUSMDV)          # sod.lua: config.flux_calculator = "ausmdv"
     x = (np.arange(ni) + 0.5) / ni
     P = np.broadcast_to(np.where(x < 0.5, 1.0e5, 1.0e4), (nk, nj, ni))
     T = np.broadcast_to(np.where(x < 0.5, 348.4, 278.8), (nk, nj, ni))
