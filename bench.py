@@ -270,7 +270,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--flux", default="ausmdv,hllc", help="comma list of flux calculators to run")
+    ap.add_argument("--flux", default="ausmdv,hllc,adaptive_hanel_ausmdv",
+                    help="comma list of flux calculators to run; the first is the headline")
     ap.add_argument("--precision", default="fp64,mixed,fp64_exact",
                     help="comma list of precision modes; the first is the headline")
     ap.add_argument("--acc-steps", type=int, default=20, help="steps of the mixed-vs-fp64 accuracy check")

@@ -693,37 +693,78 @@ __device__ __forceinline__ Prim prim_at(KArgs A, int i, int j, int k)
 
 __device__ __forceinline__ double prim_vel(const Prim& P, int d) { return d == 0 ? P.v0 : (d == 1 ? P.v1 : P.v2); }
 
+// A workgroup owns a 32 x 8 column of cells and marches a chunk of k-planes:
+// each plane, with a one-cell ring in i and j, is decoded once into LDS
+// (velocity and sound speed; double-buffered so that plane k+1 loads while
+// plane k is tested); the k-direction neighbours come from the next plane
+// and from the thread's previous cell.
+constexpr int SD_TX = 32, SD_TY = 8, SD_KC = 16;
+
 template <class M, int DIMS>
 __global__ __launch_bounds__(256) void shock_detect_kernel(StageArgs)
 {
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    constexpr int TX = SD_TX, TY = SD_TY, PX = TX + 2, PY = TY + 2, PL = PX * PY;
+    __shared__ double sv[2][4][PL];          // vx, vy, vz, a of a plane and its ring
     KArgs A = kargs();
-    const long n = (long)A->ni * A->nj * A->nk;
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    const int i = (int)(c % A->ni), j = (int)((c / A->ni) % A->nj), k = (int)(c / ((long)A->ni * A->nj));
-    const Prim own = load_decode<M>(A, c);
+    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const int k0 = (DIMS == 3) ? blockIdx.z * SD_KC : 0;
+    const int k1 = (DIMS == 3) ? min(k0 + SD_KC, A->nk) : 1;
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool own = i < A->ni && j < A->nj;
+    const int h = (ty + 1) * PX + (tx + 1);
+    auto load_plane = [&](int buf, int k) {
+        for (int t = threadIdx.x; t < PL; t += blockDim.x) {
+            const int hx = t % PX, hy = t / PX;
+            // ring corners are never read; positions outside in two
+            // directions give a dummy state
+            const Prim P = prim_at<M>(A, i0 + hx - 1, j0 + hy - 1, k);
+            sv[buf][0][t] = P.v0; sv[buf][1][t] = P.v1; sv[buf][2][t] = P.v2; sv[buf][3][t] = P.a;
+        }
+    };
     const double tol = A->compression_tol;
-    auto face = [&](const Prim& L, const Prim& R, int d) {
-        const double uL = prim_vel(L, d), uR = prim_vel(R, d);
-        const double a_min = (L.a < R.a) ? L.a : R.a;
+    auto face = [&](double uL, double aL, double uR, double aR) {
+        const double a_min = (aL < aR) ? aL : aR;
         return dv<MD>(uR - uL, a_min) < tol;
     };
-    bool S = false;
-#pragma unroll
-    for (int d = 0; d < DIMS; ++d) {
-        const int di = (d == 0), dj = (d == 1), dk = (d == 2);
-        S = S || face(prim_at<M>(A, i - di, j - dj, k - dk), own, d);
-        S = S || face(own, prim_at<M>(A, i + di, j + dj, k + dk), d);
+    double vz_below = 0.0, a_below = 1.0;
+    if constexpr (DIMS == 3) {
+        if (own) {
+            const Prim P = prim_at<M>(A, i, j, k0 - 1);
+            vz_below = P.v2;
+            a_below = P.a;
+        }
     }
-    A->Snew[c] = S ? 1 : 0;
+    load_plane(0, k0);
+    int buf = 0;
+    for (int k = k0; k < k1; ++k) {
+        if constexpr (DIMS == 3) load_plane(buf ^ 1, k + 1);
+        __syncthreads();
+        if (own) {
+            const double (*c)[PL] = sv[buf];
+            bool S = face(c[0][h - 1], c[3][h - 1], c[0][h], c[3][h]) || face(c[0][h], c[3][h], c[0][h + 1], c[3][h + 1]) ||
+                     face(c[1][h - PX], c[3][h - PX], c[1][h], c[3][h]) ||
+                     face(c[1][h], c[3][h], c[1][h + PX], c[3][h + PX]);
+            if constexpr (DIMS == 3) {
+                const double (*n)[PL] = sv[buf ^ 1];
+                S = S || face(vz_below, a_below, c[2][h], c[3][h]) || face(c[2][h], c[3][h], n[2][h], n[3][h]);
+                vz_below = c[2][h];
+                a_below = c[3][h];
+            }
+            A->Snew[cell_index(A, i, j, k)] = S ? 1 : 0;
+        }
+        __syncthreads();
+        buf ^= 1;
+    }
 }
 
 template <class M>
 void launch_shock_detect_mode(const StageArgs& A, int dims, hipStream_t stream)
 {
-    const long n = (long)A.ni * A.nj * A.nk;
-    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    const dim3 block(SD_TX * SD_TY);
+    const dim3 grid((A.ni + SD_TX - 1) / SD_TX, (A.nj + SD_TY - 1) / SD_TY,
+                    (dims == 3) ? (A.nk + SD_KC - 1) / SD_KC : 1);
     if (dims == 3) hipLaunchKernelGGL((shock_detect_kernel<M, 3>), grid, block, 0, stream, A);
     else hipLaunchKernelGGL((shock_detect_kernel<M, 2>), grid, block, 0, stream, A);
 }
