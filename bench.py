@@ -56,7 +56,7 @@ def cpu_baseline(n_sample=128, min_seconds=10.0, max_steps=100):
     geom, fields, bcs, _ = workloads.shocked_box_3d(gas, n=n_sample)
     U0 = workloads.initial_U(gas, fields)
     b = O.Block(O.Gas.ideal(), 3, geom["ni"], geom["nj"], geom["nk"], geom["dx"], geom["dy"], geom["dz"])
-    b.set_config()
+    b.set_config(flux_calc=O.AUSMDV)      # the headline workload's calculator
     for face, (kind, fs) in bcs.items():
         b.set_bc(int(face), int(kind), fs)
     b.set_U(U0)
