@@ -37,7 +37,11 @@ enum { UPD_EULER = 0, UPD_PC = 1, UPD_MIDPOINT = 2, UPD_CLASSIC_RK3 = 3, UPD_TVD
 // InterpolateOption (eilmer/fvcore.d:155).
 enum { INTERP_PT = 0, INTERP_RHOU = 1, INTERP_RHOP = 2, INTERP_RHOT = 3 };
 // Boundary kinds (ghost-cell effects, eilmer/bc/ghost_cell_effect/*).
-enum { BC_SLIP_WALL = 0, BC_INFLOW = 1, BC_EXTRAPOLATE = 2, BC_PERIODIC = 3, BC_CONNECTED = 4 };
+// 5: OutFlowBC_SimpleFlux (ghosts by x_order-0 extrapolation, boundary-face flux
+// by BFE_SimpleOutflowFlux); 6: WallBC_WithSlip1 (no ghost-cell data, one-sided
+// reconstruction and the wall flux); 7: OutFlowBC_SimpleExtrapolate x_order 1
+enum { BC_SLIP_WALL = 0, BC_INFLOW = 1, BC_EXTRAPOLATE = 2, BC_PERIODIC = 3, BC_CONNECTED = 4,
+       BC_OUTFLOW_SIMPLE_FLUX = 5, BC_SLIP_WALL_1 = 6, BC_EXTRAPOLATE_1 = 7 };
 // Face order of a cell (eilmer/fvcore.d Face enum): north, east, south, west, top, bottom.
 enum { F_NORTH = 0, F_EAST = 1, F_SOUTH = 2, F_WEST = 3, F_TOP = 4, F_BOTTOM = 5 };
 
@@ -583,6 +587,150 @@ static void interp_l2r2(Gas* g, const Config& cfg, const Frame& fr,
     to_global(Rght.vel, fr);
 }
 
+// ---------------------------------------- one-sided reconstruction (no ghost data)
+// onedinterp.d:165-258.  Weights of a linear combination q0*w0 + q1*w1:
+// interpolation to the face between two cell centres, or extrapolation past
+// q0 to the boundary len0/2 away.
+struct Lin { double w0, w1; };
+static inline Lin linear_interp_prepare(double len0, double len1)
+{
+    return Lin{len1 / (len0 + len1), len0 / (len0 + len1)};
+}
+static inline Lin linear_extrap_prepare(double len0, double len1)
+{
+    return Lin{(2.0 * len0 + len1) / (len0 + len1), -len0 / (len0 + len1)};
+}
+static inline double weight_scalar(const Lin& w, int extrema_clipping, double q0, double q1)
+{
+    double q = q0 * w.w0 + q1 * w.w1;
+    if (extrema_clipping) q = clip_to_limits(q, q0, q1);
+    return q;
+}
+
+// onedinterp.d:178-196: the left state as l2r2 has it; the right state
+// interpolated between the two cells, or the cell value where the limiter
+// sees an extremum.  (l2r1_prepare sets the same work variables as
+// l2r2_prepare for the terms used here.)
+static void interp_l2r1_scalar(const L2R2& w, const Lin& lin, int apply_limiter, int extrema_clipping,
+                               double qL1, double qL0, double qR0, double& qL, double& qR)
+{
+    double delLminus = (qL0 - qL1) * w.two_over_lenL0_plus_lenL1;
+    double del = (qR0 - qL0) * w.two_over_lenR0_plus_lenL0;
+    double sL = 1.0;
+    if (apply_limiter)
+        sL = (delLminus * del + fabs(delLminus * del) + epsilon_van_albada) /
+             (delLminus * delLminus + del * del + epsilon_van_albada);
+    qL = qL0 + sL * w.aL0 * (del * w.two_lenL0_plus_lenL1 + delLminus * w.lenR0_);
+    if (apply_limiter && (delLminus * del < 0.0)) qR = qR0;
+    else qR = weight_scalar(lin, extrema_clipping, qL0, qR0);
+    if (extrema_clipping) qL = clip_to_limits(qL, qL0, qR0);
+}
+
+// onedinterp.d:209-227: the mirror image of l2r1
+static void interp_l1r2_scalar(const L2R2& w, const Lin& lin, int apply_limiter, int extrema_clipping,
+                               double qL0, double qR0, double qR1, double& qL, double& qR)
+{
+    double del = (qR0 - qL0) * w.two_over_lenR0_plus_lenL0;
+    double delRplus = (qR1 - qR0) * w.two_over_lenR1_plus_lenR0;
+    double sR = 1.0;
+    if (apply_limiter)
+        sR = (del * delRplus + fabs(del * delRplus) + epsilon_van_albada) /
+             (del * del + delRplus * delRplus + epsilon_van_albada);
+    qR = qR0 - sR * w.aR0 * (delRplus * w.lenL0_ + del * w.two_lenR0_plus_lenR1);
+    if (apply_limiter && (delRplus * del < 0.0)) qL = qL0;
+    else qL = weight_scalar(lin, extrema_clipping, qL0, qR0);
+    if (extrema_clipping) qR = clip_to_limits(qR, qL0, qR0);
+}
+
+static int thermo_update(Gas* g, int thermo_interp, GasState& Q)
+{
+    switch (thermo_interp) {
+    case INTERP_PT: return update_thermo_from_pT(g, Q);
+    case INTERP_RHOP: return update_thermo_from_rhop(g, Q);
+    case INTERP_RHOT: return update_thermo_from_rhoT(g, Q);
+    default: return update_thermo_from_rhou(g, Q);
+    }
+}
+
+// The two thermodynamic scalars each thermo_interpolator reconstructs.
+static double* thermo_pair(GasState& Q, int thermo_interp, int which)
+{
+    switch (thermo_interp) {
+    case INTERP_PT: return which ? &Q.T : &Q.p;
+    case INTERP_RHOP: return which ? &Q.p : &Q.rho;
+    case INTERP_RHOT: return which ? &Q.T : &Q.rho;
+    default: return which ? &Q.u : &Q.rho;
+    }
+}
+
+// onedinterp.d:587-818: interp_l2r1 (left_full = true; cells cL1, cL0, cR0 =
+// cA, cB, cC; the face next to a right boundary without ghost data) and
+// interp_l1r2 (left_full = false; cells cL0, cR0, cR1 = cA, cB, cC).
+static void interp_l2r1_l1r2(Gas* g, const Config& cfg, const Frame& fr, bool left_full, FlowState cA,
+                             FlowState cB, FlowState cC, double len, FlowState& Lft, FlowState& Rght)
+{
+    const int nsp = g->nsp;
+    to_local(cA.vel, fr); to_local(cB.vel, fr); to_local(cC.vel, fr);
+    L2R2 w;
+    if (left_full) l2r2_prepare(w, len, len, len, len);   // l2r1_prepare(lenL1, lenL0, lenR0)
+    else l2r2_prepare(w, len, len, len, len);             // l1r2_prepare(lenL0, lenR0, lenR1)
+    const Lin lin = linear_interp_prepare(len, len);
+    const int lim = cfg.apply_limiter, clip = cfg.extrema_clipping;
+    auto scalar = [&](double a, double b, double c, double& qL, double& qR) {
+        if (left_full) interp_l2r1_scalar(w, lin, lim, clip, a, b, c, qL, qR);
+        else interp_l1r2_scalar(w, lin, lim, clip, a, b, c, qL, qR);
+    };
+    const FlowState& cL0 = left_full ? cB : cA;
+    const FlowState& cR0 = left_full ? cC : cB;
+    for (int d = 0; d < 3; ++d) scalar(cA.vel[d], cB.vel[d], cC.vel[d], Lft.vel[d], Rght.vel[d]);
+    if (nsp > 1) {
+        for (int s = 0; s < nsp; ++s)
+            scalar(cA.gas.massf[s], cB.gas.massf[s], cC.gas.massf[s], Lft.gas.massf[s], Rght.gas.massf[s]);
+        if (scale_mass_fractions(Lft.gas.massf, nsp) != 0)
+            for (int s = 0; s < nsp; ++s) Lft.gas.massf[s] = cL0.gas.massf[s];
+        if (scale_mass_fractions(Rght.gas.massf, nsp) != 0)
+            for (int s = 0; s < nsp; ++s) Rght.gas.massf[s] = cR0.gas.massf[s];
+    } else {
+        Lft.gas.massf[0] = 1.0;
+        Rght.gas.massf[0] = 1.0;
+    }
+    for (int which = 0; which < 2; ++which)
+        scalar(*thermo_pair(cA.gas, cfg.thermo_interp, which), *thermo_pair(cB.gas, cfg.thermo_interp, which),
+               *thermo_pair(cC.gas, cfg.thermo_interp, which), *thermo_pair(Lft.gas, cfg.thermo_interp, which),
+               *thermo_pair(Rght.gas, cfg.thermo_interp, which));
+    if (thermo_update(g, cfg.thermo_interp, Lft.gas) != 0) Lft = cL0;
+    if (thermo_update(g, cfg.thermo_interp, Rght.gas) != 0) Rght = cR0;
+    to_global(Lft.vel, fr);
+    to_global(Rght.vel, fr);
+}
+
+// onedinterp.d:820-1037: interp_l2r0 / interp_l0r2, the state at a wall face
+// without ghost data, extrapolated linearly from the two cells next to it
+// (c0 at the wall, c1 behind it).  With extrema clipping the reference
+// returns at once and the face keeps the copy of the wall cell's state.
+static void interp_wall_side(Gas* g, const Config& cfg, const Frame& fr, FlowState c0, FlowState c1, double len,
+                             FlowState& side)
+{
+    if (cfg.extrema_clipping) return;
+    const int nsp = g->nsp;
+    to_local(c0.vel, fr); to_local(c1.vel, fr);
+    const Lin w = linear_extrap_prepare(len, len);
+    for (int d = 0; d < 3; ++d) side.vel[d] = weight_scalar(w, 0, c0.vel[d], c1.vel[d]);
+    if (nsp > 1) {
+        for (int s = 0; s < nsp; ++s) side.gas.massf[s] = weight_scalar(w, 0, c0.gas.massf[s], c1.gas.massf[s]);
+        if (scale_mass_fractions(side.gas.massf, nsp) != 0)
+            for (int s = 0; s < nsp; ++s) side.gas.massf[s] = c0.gas.massf[s];
+    } else {
+        side.gas.massf[0] = 1.0;
+    }
+    for (int which = 0; which < 2; ++which)
+        *thermo_pair(side.gas, cfg.thermo_interp, which) =
+            weight_scalar(w, 0, *thermo_pair(c0.gas, cfg.thermo_interp, which),
+                          *thermo_pair(c1.gas, cfg.thermo_interp, which));
+    if (thermo_update(g, cfg.thermo_interp, side.gas) != 0) side = c0;
+    to_global(side.vel, fr);
+}
+
 // ------------------------------------------------------------------- fluxes
 // Conserved layout: [mass, mom_x, mom_y, mom_z, total_energy, rhoY_0..rhoY_{nsp-1}]
 // (species only when nsp > 1).
@@ -792,6 +940,90 @@ static void compute_interface_flux(const Gas* g, const Config& cfg, FlowState Lf
     F[1] = m[0]; F[2] = m[1]; F[3] = m[2];
 }
 
+// gas_model.d:162: gamma = Cp/Cv of the state (ideal gas: _Cp/_Cv)
+static double gas_gamma(const Gas* g, const GasState& Q)
+{
+    if (g->type == GAS_IDEAL) return g->Cp / g->Cv;
+    double Cp[NSP_MAX], Cv[NSP_MAX];
+    for (int i = 0; i < g->nsp; ++i) Cp[i] = eval_Cp(g->curve[i], Q.T);
+    double Cpm = mass_average(g, Q, Cp);
+    for (int i = 0; i < g->nsp; ++i) Cv[i] = eval_Cp(g->curve[i], Q.T) - g->R[i];
+    double Cvm = mass_average(g, Q, Cv);
+    return Cpm / Cvm;
+}
+
+// fluxcalc.d:130-298, compute_flux_at_left_wall (left = true, S the state on
+// the right of the face) and compute_flux_at_right_wall, on a fixed grid
+// (wall velocity vstar = 0): the wall pressure from the Riemann invariant of
+// the wave that brings the gas to rest, an isentropic estimate refined by a
+// secant iteration when it is a shock, capped at ten times the gas pressure.
+static void wall_flux(const Gas* g, FlowState S, const Frame& fr, bool left, int nq, double* F)
+{
+    to_local(S.vel, fr);
+    const double vstar = 0.0;
+    const double a = S.gas.a, v = S.vel[0];
+    const double gam = gas_gamma(g, S.gas);
+    const double rho = S.gas.rho, p = S.gas.p;
+    double tmp;
+    if (left) {
+        double Ubar = v - 2.0 * a / (gam - 1.0);
+        tmp = (vstar - Ubar) * (gam - 1.0) / (2.0 * sqrt(gam)) * sqrt(rho / pow(p, 1.0 / gam));
+    } else {
+        double Ubar = v + 2.0 * a / (gam - 1.0);
+        tmp = (Ubar - vstar) * (gam - 1.0) / (2.0 * sqrt(gam)) * sqrt(rho / pow(p, 1.0 / gam));
+    }
+    const double ptiny = 0.1;   // FlowStateLimits.min_pressure, fvcore.d:26
+    double pstar = (tmp > 0.0) ? pow(tmp, 2.0 * gam / (gam - 1.0)) : ptiny;
+    if (pstar > 1.1 * p) {
+        auto f = [&](double ps) {
+            double xi = ps / p;
+            double M1sq = 1.0 + (gam + 1.0) / 2.0 / gam * (xi - 1.0);
+            double u1 = sqrt(M1sq) * a;
+            double u2 = u1 * ((gam - 1.0) * M1sq + 2.0) / ((gam + 1.0) * M1sq);
+            return left ? vstar - u1 + u2 - v : vstar + u1 - u2 - v;
+        };
+        int count = 0;
+        double incr;
+        do {
+            double f0 = f(pstar);
+            double dp = 0.001 * pstar;
+            double f1 = f(pstar + dp);
+            incr = -f0 * dp / (f1 - f0);
+            pstar += incr;
+            count += 1;
+        } while (fabs(incr) / pstar > 0.01 && count < 10);
+    }
+    pstar = fmin(pstar, p * 10.0);
+    for (int q = 0; q < nq; ++q) F[q] = 0.0;
+    double m[3] = {pstar, 0.0, 0.0};
+    to_global(m, fr);
+    F[1] = m[0]; F[2] = m[1]; F[3] = m[2];
+    F[4] = pstar * vstar;
+}
+
+// boundary_flux_effect.d:481-546 (BFE_SimpleOutflowFlux.compute_outflow_flux) on
+// a fixed grid: the flux of the interior cell's own state when it leaves the
+// domain (outsign: +1 on a plus face, -1 on a minus face), else a closed face
+// carrying only the pressure.
+static void outflow_flux(const Gas* g, const FlowState& fs, int outsign, const Frame& fr, int nq, double* F)
+{
+    const double* n = fr.n;
+    const double vn = fs.vel[0] * n[0] + fs.vel[1] * n[1] + fs.vel[2] * n[2];
+    const double mass_flux = fs.gas.rho * vn;
+    for (int q = 0; q < nq; ++q) F[q] = 0.0;
+    if ((outsign * mass_flux) > 0.0) {
+        F[0] = mass_flux;
+        for (int d = 0; d < 3; ++d) F[1 + d] = fs.gas.p * n[d] + fs.vel[d] * mass_flux;
+        double utot = fs.gas.u + 0.5 * (fs.vel[0] * fs.vel[0] + fs.vel[1] * fs.vel[1] + fs.vel[2] * fs.vel[2]);
+        F[4] = mass_flux * utot + fs.gas.p * vn;
+        if (g->nsp > 1)
+            for (int s = 0; s < g->nsp; ++s) F[5 + s] = mass_flux * fs.gas.massf[s];
+    } else {
+        for (int d = 0; d < 3; ++d) F[1 + d] = n[d] * fs.gas.p;
+        F[4] = fs.gas.p * 0.0;
+    }
+}
+
 // Stand-alone flux entry for tests.  fs layout: [rho,p,T,u,a,vx,vy,vz,massf..]
 // in the face frame already (n, t1, t2) = (x, y, z).
 int ref_flux(Gas* g, int calc, const double* L, const double* R, double* F)
@@ -808,6 +1040,44 @@ int ref_flux(Gas* g, int calc, const double* L, const double* R, double* F)
     else if (calc == FLUX_HANEL) hanel(g, l, r, F);
     else ausmdv(g, l, r, F);
     return 0;
+}
+
+// Stand-alone boundary fluxes for tests, in the frame (n, t1, t2) = (x, y, z):
+// the wall flux (left != 0: the wall is on the left of the gas state) and the
+// simple outflow flux through a face of outward sign outsign.
+int ref_wall_flux(Gas* g, int left, const double* st, double* F)
+{
+    FlowState f{};
+    f.gas.rho = st[0]; f.gas.p = st[1]; f.gas.T = st[2]; f.gas.u = st[3]; f.gas.a = st[4];
+    f.vel[0] = st[5]; f.vel[1] = st[6]; f.vel[2] = st[7];
+    for (int i = 0; i < g->nsp; ++i) f.gas.massf[i] = st[8 + i];
+    const Frame fr{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    wall_flux(g, f, fr, left != 0, 5 + (g->nsp > 1 ? g->nsp : 0), F);
+    return 0;
+}
+
+int ref_outflow_flux(Gas* g, int outsign, const double* st, double* F)
+{
+    FlowState f{};
+    f.gas.rho = st[0]; f.gas.p = st[1]; f.gas.T = st[2]; f.gas.u = st[3]; f.gas.a = st[4];
+    f.vel[0] = st[5]; f.vel[1] = st[6]; f.vel[2] = st[7];
+    for (int i = 0; i < g->nsp; ++i) f.gas.massf[i] = st[8 + i];
+    const Frame fr{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    outflow_flux(g, f, outsign, fr, 5 + (g->nsp > 1 ? g->nsp : 0), F);
+    return 0;
+}
+
+// One-sided scalar reconstructions on a uniform grid of cell length len:
+// kind 0 l2r1 (q = qL1, qL0, qR0), 1 l1r2 (q = qL0, qR0, qR1), 2 the wall-side
+// linear extrapolation (q = wall cell, the cell behind it).  out = (qL, qR),
+// or the extrapolated value in out[0].
+void ref_interp_onesided_scalar(int kind, double len, const double* q, int apply_limiter, int extrema_clipping,
+                                double* out)
+{
+    L2R2 w; l2r2_prepare(w, len, len, len, len);
+    if (kind == 0) interp_l2r1_scalar(w, linear_interp_prepare(len, len), apply_limiter, extrema_clipping, q[0], q[1], q[2], out[0], out[1]);
+    else if (kind == 1) interp_l1r2_scalar(w, linear_interp_prepare(len, len), apply_limiter, extrema_clipping, q[0], q[1], q[2], out[0], out[1]);
+    else out[0] = weight_scalar(linear_extrap_prepare(len, len), 0, q[0], q[1]);
 }
 
 // ------------------------------------------------------------------- blocks
@@ -831,6 +1101,7 @@ struct Block {
 };
 
 static inline size_t cidx(const Block* b, int i, int j, int k) { return ((size_t)k * b->njdim + j) * b->nidim + i; }
+static void apply_bcs(Block* b);
 
 // sfluidblock.d:94-126, 205-209 (ghost layers = 2); geometry for a uniform
 // Cartesian grid (the values properties.d:97-139, 568-650 produce for it).
@@ -860,7 +1131,15 @@ Block* ref_block_new(Gas* g, int dims, int ni, int nj, int nk, double dx, double
         b->frame[2] = Frame{{0, 0, 1}, {1, 0, 0}, {0, 1, 0}};
     }
     size_t n = (size_t)b->nidim * b->njdim * b->nkdim;
-    b->fs.assign(n, FlowState{});
+    // every cell, ghost cells included, starts as the flow state FVCell's
+    // constructor makes (fvcell.d:170, gas_state.d:64-88): p = 100 kPa,
+    // T = 300 K, at rest, all of the first species, with its sound speed; a
+    // ghost cell that no ghost-cell effect writes keeps it
+    FlowState init{};
+    init.gas.p = 100.0e3; init.gas.T = 300.0; init.gas.massf[0] = 1.0;
+    update_thermo_from_pT(g, init.gas);
+    update_sound_speed(g, init.gas);
+    b->fs.assign(n, init);
     for (int t = 0; t < 4; ++t) { b->U[t].assign(n * b->nq, 0.0); b->dUdt[t].assign(n * b->nq, 0.0); }
     for (int d = 0; d < 3; ++d) b->F[d].assign(n * b->nq, 0.0);
     b->bad.assign(n, 0);
@@ -1041,6 +1320,23 @@ void ref_block_get_fs(const Block* b, double* out)
             }
 }
 
+// Every cell's flow state, ghost cells included ([8 + nsp][nidim*njdim*nkdim],
+// the reference's own slot order), after the ghost cells are filled, and the
+// shock indicators of all slots.
+void ref_block_get_fs_full(Block* b, int fill, double* out, int* S)
+{
+    if (fill) apply_bcs(b);
+    const size_t n = b->fs.size();
+    for (size_t c = 0; c < n; ++c) {
+        const FlowState& f = b->fs[c];
+        out[0 * n + c] = f.gas.rho; out[1 * n + c] = f.gas.p; out[2 * n + c] = f.gas.T;
+        out[3 * n + c] = f.gas.u; out[4 * n + c] = f.gas.a;
+        out[5 * n + c] = f.vel[0]; out[6 * n + c] = f.vel[1]; out[7 * n + c] = f.vel[2];
+        for (int s = 0; s < b->gas->nsp; ++s) out[(8 + s) * n + c] = f.gas.massf[s];
+        if (S) S[c] = f.S;
+    }
+}
+
 // ------------------------------------------------------- block connections
 //
 // full_face_copy.d:142-962 (set_up_cell_mapping_phase0): the ghost cell at
@@ -1129,9 +1425,37 @@ static void apply_bcs(Block* b)
                         dest = bc.fs;
                         break;
                     case BC_EXTRAPOLATE:
+                    case BC_OUTFLOW_SIMPLE_FLUX:
                         s[dir] = plus ? hi[dir] : lo[dir];
                         dest = b->fs[cidx(b, s[0], s[1], s[2])];
                         break;
+                    case BC_EXTRAPOLATE_1: {
+                        // extrapolate_copy.d:102-186: linear in the primitive
+                        // variables from the two cells inside (for layer 2: the
+                        // first ghost and the boundary cell), then T and p from
+                        // the EOS; the sound speed and shock indicator are not
+                        // written
+                        int s1[3] = {s[0], s[1], s[2]}, s2[3] = {s[0], s[1], s[2]};
+                        const int edge = plus ? hi[dir] : lo[dir], in = plus ? -1 : 1;
+                        s1[dir] = (layer == 1) ? edge : edge - in;
+                        s2[dir] = (layer == 1) ? edge + in : edge;
+                        const FlowState c1 = b->fs[cidx(b, s1[0], s1[1], s1[2])];
+                        const FlowState c2 = b->fs[cidx(b, s2[0], s2[1], s2[2])];
+                        dest.gas.rho = 2.0 * c1.gas.rho - c2.gas.rho;
+                        dest.gas.u = 2.0 * c1.gas.u - c2.gas.u;
+                        const int nsp = b->gas->nsp;
+                        if (nsp > 1) {
+                            for (int q = 0; q < nsp; ++q) dest.gas.massf[q] = 2.0 * c1.gas.massf[q] - c2.gas.massf[q];
+                            scale_mass_fractions(dest.gas.massf, nsp);
+                        } else {
+                            dest.gas.massf[0] = 1.0;
+                        }
+                        update_thermo_from_rhou(b->gas, dest.gas);
+                        for (int d = 0; d < 3; ++d) dest.vel[d] = 2.0 * c1.vel[d] - c2.vel[d];
+                        break;
+                    }
+                    case BC_SLIP_WALL_1:
+                        break;   // no ghost-cell data
                     case BC_PERIODIC:
                         s[dir] = plus ? lo[dir] + layer - 1 : hi[dir] - layer + 1;
                         dest = b->fs[cidx(b, s[0], s[1], s[2])];
@@ -1168,9 +1492,13 @@ static void apply_bcs(Block* b)
 // fluidblock.d:415-431 with shockdetectors.d (PJ_ShockDetector): a face is a
 // shock point when the normal velocity jump across it, scaled by the smaller
 // sound speed, is below the compression tolerance; a cell is a shock point
-// when any of its faces is.  Every face has a cell on both sides here (the
-// ghost cells of all these boundary conditions hold data), and the ghost
-// cells keep the indicator they were filled with.
+// when any of its faces is.  The ghost cells keep the indicator they were
+// filled with (none for x_order-1 extrapolation, which does not write it).
+static const int MINUS_FACE[3] = {F_WEST, F_SOUTH, F_BOTTOM};
+static const int PLUS_FACE[3] = {F_EAST, F_NORTH, F_TOP};
+// bc.lua: WallBC_WithSlip1 is the one boundary here without ghost-cell data
+static inline bool ghost_data_available(const Block* b, int face) { return b->bc[face].type != BC_SLIP_WALL_1; }
+
 static inline int pj_shock(const FlowState& cL, const FlowState& cR, const Frame& fr, double tol)
 {
     const double uL = cL.vel[0] * fr.n[0] + cL.vel[1] * fr.n[1] + cL.vel[2] * fr.n[2];
@@ -1190,10 +1518,18 @@ static void detect_shock_points(Block* b)
         for (int j = b->jmin; j <= b->jmax; ++j)
             for (int i = b->imin; i <= b->imax; ++i) {
                 const size_t c = cidx(b, i, j, k);
+                const int pos[3] = {i, j, k}, lo[3] = {b->imin, b->jmin, b->kmin}, hi[3] = {b->imax, b->jmax, b->kmax};
                 int s = 0;
                 for (int d = 0; d < ndir; ++d) {
-                    s |= pj_shock(b->fs[c - st[d]], b->fs[c], b->frame[d], tol);   // minus face
-                    s |= pj_shock(b->fs[c], b->fs[c + st[d]], b->frame[d], tol);   // plus face
+                    const Frame& fr = b->frame[d];
+                    const FlowState& cs = b->fs[c];
+                    const double un = cs.vel[0] * fr.n[0] + cs.vel[1] * fr.n[1] + cs.vel[2] * fr.n[2];
+                    // a boundary face without ghost data has one cell:
+                    // shockdetectors.d:39-54 compares its velocity towards the wall
+                    if (pos[d] == lo[d] && !ghost_data_available(b, MINUS_FACE[d])) s |= (un / cs.gas.a) < tol;
+                    else s |= pj_shock(b->fs[c - st[d]], cs, fr, tol);   // minus face
+                    if (pos[d] == hi[d] && !ghost_data_available(b, PLUS_FACE[d])) s |= ((-un) / cs.gas.a) < tol;
+                    else s |= pj_shock(cs, b->fs[c + st[d]], fr, tol);   // plus face
                 }
                 S[c] = s;
             }
@@ -1202,8 +1538,10 @@ static void detect_shock_points(Block* b)
             for (int i = b->imin; i <= b->imax; ++i) b->fs[cidx(b, i, j, k)].S = S[cidx(b, i, j, k)];
 }
 
-// sfluidblock.d:2016-2346, the classic (l2r2) branch; all BCs here keep ghost
-// data, so the one-sided variants and wall-flux calls are not reached.
+// sfluidblock.d:2016-2346, the general branch: l2r2 reconstruction, the
+// one-sided variants and wall fluxes next to boundaries without ghost data,
+// faces whose flux the boundary computes skipped; then the boundary fluxes
+// (postConvFluxAction SimpleOutflowFlux, boundary_flux_effect.d:367-546).
 static void convective_flux(Block* b)
 {
     Gas* g = b->gas;
@@ -1216,22 +1554,55 @@ static void convective_flux(Block* b)
         int st[3] = {0, 0, 0}; st[dir] = 1;
         const size_t stride = (size_t)st[0] + (size_t)st[1] * b->nidim + (size_t)st[2] * b->nidim * b->njdim;
         const int ihi = b->imax + st[0], jhi = b->jmax + st[1], khi = b->kmax + st[2];
+        const bool ng_m = !ghost_data_available(b, MINUS_FACE[dir]), ng_p = !ghost_data_available(b, PLUS_FACE[dir]);
+        const bool bf_m = b->bc[MINUS_FACE[dir]].type == BC_OUTFLOW_SIMPLE_FLUX;
+        const bool bf_p = b->bc[PLUS_FACE[dir]].type == BC_OUTFLOW_SIMPLE_FLUX;
+        const int lo = (dir == 0) ? b->imin : (dir == 1) ? b->jmin : b->kmin;
+        const int hi = (dir == 0) ? b->imax : (dir == 1) ? b->jmax : b->kmax;
+        const Frame& fr = b->frame[dir];
         #pragma omp parallel for collapse(2) schedule(static)
         for (int k = b->kmin; k <= khi; ++k)
             for (int j = b->jmin; j <= jhi; ++j)
                 for (int i = b->imin; i <= ihi; ++i) {
+                    const int pos = (dir == 0) ? i : (dir == 1) ? j : k;
                     size_t cR0 = cidx(b, i, j, k);
                     size_t cL0 = cR0 - stride, cL1 = cL0 - stride, cR1 = cR0 + stride;
-                    FlowState Lft = b->fs[cL0];
-                    FlowState Rght = b->fs[cR0];
-                    if (do_reconstruction)
-                        interp_l2r2(g, cfg, b->frame[dir], b->fs[cL1], b->fs[cL0], b->fs[cR0], b->fs[cR1],
-                                    len[dir], len[dir], len[dir], len[dir], Lft, Rght);
+                    const bool wall_m = (pos == lo) && ng_m, wall_p = (pos == hi + 1) && ng_p;
+                    FlowState Lft = wall_m ? b->fs[cR0] : b->fs[cL0];
+                    FlowState Rght = wall_p ? b->fs[cL0] : b->fs[cR0];
+                    if (do_reconstruction) {
+                        const double l = len[dir];
+                        if (wall_m) interp_wall_side(g, cfg, fr, b->fs[cR0], b->fs[cR1], l, Rght);
+                        else if (pos == lo + 1 && ng_m)
+                            interp_l2r1_l1r2(g, cfg, fr, false, b->fs[cL0], b->fs[cR0], b->fs[cR1], l, Lft, Rght);
+                        else if (pos == hi && ng_p)
+                            interp_l2r1_l1r2(g, cfg, fr, true, b->fs[cL1], b->fs[cL0], b->fs[cR0], l, Lft, Rght);
+                        else if (wall_p) interp_wall_side(g, cfg, fr, b->fs[cL0], b->fs[cL1], l, Lft);
+                        else interp_l2r2(g, cfg, fr, b->fs[cL1], b->fs[cL0], b->fs[cR0], b->fs[cR1], l, l, l, l,
+                                         Lft, Rght);
+                    }
+                    if ((pos == lo && bf_m) || (pos == hi + 1 && bf_p)) continue;
                     double Fq[NQ_MAX];
-                    compute_interface_flux(g, cfg, Lft, Rght, b->frame[dir], Fq);
+                    if (wall_m) wall_flux(g, Rght, fr, true, nq, Fq);
+                    else if (wall_p) wall_flux(g, Lft, fr, false, nq, Fq);
+                    else compute_interface_flux(g, cfg, Lft, Rght, fr, Fq);
                     double* dst = &b->F[dir][cR0 * nq];
                     for (int q = 0; q < nq; ++q) dst[q] = Fq[q];
                 }
+        // boundary faces whose flux the boundary condition computes, from the
+        // interior cell next to them
+        for (int side = 0; side < 2; ++side) {
+            if (!(side ? bf_p : bf_m)) continue;
+            for (int k = b->kmin; k <= b->kmax; ++k)
+                for (int j = b->jmin; j <= b->jmax; ++j)
+                    for (int i = b->imin; i <= b->imax; ++i) {
+                        const int pos = (dir == 0) ? i : (dir == 1) ? j : k;
+                        if (pos != (side ? hi : lo)) continue;
+                        const size_t c = cidx(b, i, j, k);
+                        const size_t face = side ? c + stride : c;
+                        outflow_flux(g, b->fs[c], side ? 1 : -1, fr, nq, &b->F[dir][face * nq]);
+                    }
+        }
     }
 }
 

@@ -19,6 +19,8 @@ AUSMDV, HANEL, ADAPTIVE_HANEL_AUSMDV, HLLC = 0, 1, 5, 9
 EULER, PC, MIDPOINT, CLASSIC_RK3, TVD_RK3, DENMAN_RK3 = range(6)
 INTERP_PT, INTERP_RHOU, INTERP_RHOP, INTERP_RHOT = range(4)
 BC_SLIP_WALL, BC_INFLOW, BC_EXTRAPOLATE, BC_PERIODIC, BC_CONNECTED = range(5)
+# OutFlowBC_SimpleFlux, WallBC_WithSlip1, OutFlowBC_SimpleExtrapolate x_order 1
+BC_OUTFLOW_SIMPLE_FLUX, BC_SLIP_WALL_1, BC_EXTRAPOLATE_1 = 5, 6, 7
 NORTH, EAST, SOUTH, WEST, TOP, BOTTOM = range(6)
 
 _lib = None
@@ -53,6 +55,10 @@ def lib():
         L.ref_gas_update_many.argtypes = [vp, C.c_int, C.c_int, _dp]
         L.ref_interp_l2r2_scalar.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp]
         L.ref_flux.argtypes = [vp, C.c_int, _dp, _dp, _dp]
+        L.ref_wall_flux.argtypes = [vp, C.c_int, _dp, _dp]
+        L.ref_outflow_flux.argtypes = [vp, C.c_int, _dp, _dp]
+        L.ref_interp_onesided_scalar.argtypes = [C.c_int, C.c_double, _dp, C.c_int, C.c_int, _dp]
+        L.ref_block_get_fs_full.argtypes = [vp, C.c_int, _dp, C.POINTER(C.c_int)]
         L.ref_block_new.restype = vp
         L.ref_block_new.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
         L.ref_block_free.argtypes = [vp]
@@ -162,6 +168,33 @@ def interp_l2r2_scalar(lens, q, apply_limiter=True, extrema_clipping=True):
     return out[0], out[1]
 
 
+def wall_flux(gas, left, st):
+    """fluxcalc.d:130-298 for the state st (face frame n = x) beside a wall."""
+    nq = 5 + (gas.nsp if gas.nsp > 1 else 0)
+    F = np.zeros(nq)
+    lib().ref_wall_flux(gas.h, int(left), _p(_f64(st)), _p(F))
+    return F
+
+
+def outflow_flux(gas, outsign, st):
+    """BFE_SimpleOutflowFlux for the interior state st (face frame n = x)."""
+    nq = 5 + (gas.nsp if gas.nsp > 1 else 0)
+    F = np.zeros(nq)
+    lib().ref_outflow_flux(gas.h, int(outsign), _p(_f64(st)), _p(F))
+    return F
+
+
+def interp_onesided_scalar(kind, q, length=1.0, apply_limiter=True, extrema_clipping=True):
+    """kind 0: l2r1 (qL1, qL0, qR0) -> (qL, qR); 1: l1r2 (qL0, qR0, qR1) -> (qL, qR);
+    2: wall-side extrapolation (q0, q1) -> value."""
+    q3 = np.zeros(3)
+    q3[:len(q)] = q
+    out = np.zeros(2)
+    lib().ref_interp_onesided_scalar(int(kind), float(length), _p(q3), int(apply_limiter), int(extrema_clipping),
+                                     _p(out))
+    return out[0] if kind == 2 else (out[0], out[1])
+
+
 def flux(gas, calc, L, R):
     """L, R: [rho,p,T,u,a,vn,vt1,vt2,massf...] in the face frame."""
     nq = 5 + (gas.nsp if gas.nsp > 1 else 0)
@@ -227,6 +260,17 @@ class Block:
         out = np.zeros((8 + self.gas.nsp, self.ncells))
         lib().ref_block_get_fs(self.h, _p(out))
         return out
+
+    def get_fs_full(self, fill=True):
+        """Flow states of all slots, ghost cells included, shaped
+        [field][k][j][i] over the (n+4)-wide slot arrays, and the slots' S."""
+        nk = self.shape[0] + 4 if self.dims == 3 else 1
+        shp = (nk, self.shape[1] + 4, self.shape[2] + 4)
+        n = int(np.prod(shp))
+        out = np.zeros((8 + self.gas.nsp, n))
+        S = np.zeros(n, dtype=np.int32)
+        lib().ref_block_get_fs_full(self.h, int(fill), _p(out), S.ctypes.data_as(C.POINTER(C.c_int)))
+        return out.reshape((8 + self.gas.nsp,) + shp), S.reshape(shp)
 
     def step(self, dt):
         nbad = (C.c_int * 3)()
