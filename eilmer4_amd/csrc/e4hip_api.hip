@@ -35,6 +35,7 @@ struct GasModelH {
     e4::TPGas* tp_dev = nullptr;     // device copy read by the kernels
     e4::GuessModel* guess_dev = nullptr;
     __bf16* guess_w = nullptr;       // W1 [64][32], W2 [64][64], W3 [64][64] in BF16
+    double a_init = 0.0;             // mixture: sound speed of the constructed cell state
 };
 
 struct BlockH {
@@ -244,6 +245,8 @@ int shock_fields(const BlockH& b, Args& A)
 
 // The stage kernel's arguments for one block; kchunk <= 0 picks the
 // single-launch chunk.
+void boundary_slabs(const BlockH& b, int* lo, int* hi, int* off, int& noghost);
+
 int stage_args_for(const BlockH& b, int stage, double dt, unsigned int* counters, int kchunk, e4::StageArgs& A,
                    dim3& grid)
 {
@@ -315,6 +318,7 @@ int stage_args_for(const BlockH& b, int stage, double dt, unsigned int* counters
     }
     A.dt = dt; A.g1 = g1; A.g2 = g2; A.g3 = g3;
     A.c1 = dt * g1;
+    boundary_slabs(b, A.fix_lo, A.fix_hi, A.fix_off, A.noghost);
     const int src = shock_fields(b, A);
     if (src) return src;
     const int nk = A.nk;
@@ -322,6 +326,44 @@ int stage_args_for(const BlockH& b, int stage, double dt, unsigned int* counters
     A.tiles_y = (b.nj + TILE_Y - 1) / TILE_Y;
     grid = dim3(A.tiles_x, A.tiles_y, (nk + A.kchunk - 1) / A.kchunk);
     return E4HIP_OK;
+}
+
+// Cells the boundary kernel finishes: the layers beside faces whose flux is
+// not the interior code's (two for a wall without ghost data, one for a
+// simple-outflow-flux face), slab by slab.
+void boundary_slabs(const BlockH& b, int* lo, int* hi, int* off, int& noghost)
+{
+    static const int FM[3] = {E4HIP_FACE_WEST, E4HIP_FACE_SOUTH, E4HIP_FACE_BOTTOM};
+    static const int FP[3] = {E4HIP_FACE_EAST, E4HIP_FACE_NORTH, E4HIP_FACE_TOP};
+    const int n[3] = {b.ni, b.nj, (b.dims == 3) ? b.nk : 1};
+    auto layers = [&](int f) {
+        return (b.bc_type[f] == E4HIP_BC_SLIP_WALL_1) ? 2 : (b.bc_type[f] == E4HIP_BC_OUTFLOW_SIMPLE_FLUX ? 1 : 0);
+    };
+    noghost = 0;
+    for (int d = 0; d < 3; ++d) {
+        const bool on = d < b.dims;
+        lo[d] = on ? layers(FM[d]) : 0;
+        hi[d] = on ? layers(FP[d]) : 0;
+        if (on && b.bc_type[FM[d]] == E4HIP_BC_SLIP_WALL_1) noghost |= 1 << FM[d];
+        if (on && b.bc_type[FP[d]] == E4HIP_BC_SLIP_WALL_1) noghost |= 1 << FP[d];
+    }
+    int total = 0;
+    for (int s = 0; s < 6; ++s) {
+        const int d = s >> 1;
+        off[s] = total;
+        total += ((s & 1) ? hi[d] : lo[d]) * n[(d + 1) % 3] * n[(d + 2) % 3];
+    }
+    off[6] = total;
+}
+
+void launch_boundary_cells(const e4::StageArgs& A, int dims, int flux)
+{
+    if (A.fix_off[6] <= 0) return;
+    switch (g_state.cfg.precision_mode) {
+    case E4HIP_PRECISION_FP64_EXACT: e4::launch_boundary_cells_exact(A, g_state.stream, dims, flux); break;
+    case E4HIP_PRECISION_MIXED: e4::launch_boundary_cells_mixed(A, g_state.stream, dims, flux); break;
+    default: e4::launch_boundary_cells_fast64(A, g_state.stream, dims, flux); break;
+    }
 }
 
 int launch_stage(const BlockH& b, int stage, double dt, unsigned int* counters)
@@ -339,6 +381,7 @@ int launch_stage(const BlockH& b, int stage, double dt, unsigned int* counters)
     case E4HIP_PRECISION_MIXED: e4::launch_stage_mixed(A, grid, g_state.stream, b.dims, flux); break;
     default: e4::launch_stage_fast64(A, grid, g_state.stream, b.dims, flux); break;
     }
+    launch_boundary_cells(A, b.dims, flux);
     prof_end(0, ev);
     HIP_TRY(hipGetLastError());
     return E4HIP_OK;
@@ -411,6 +454,7 @@ int launch_stage_batch(const std::vector<const BlockH*>& blks, const std::vector
     case E4HIP_PRECISION_MIXED: e4::launch_stage_batch_mixed(B, nwg, def, g_state.stream, blks[0]->dims, flux); break;
     default: e4::launch_stage_batch_fast64(B, nwg, def, g_state.stream, blks[0]->dims, flux); break;
     }
+    for (int e = 0; e < n; ++e) launch_boundary_cells(args[e], blks[e]->dims, flux);
     prof_end(0, ev);
     HIP_TRY(hipGetLastError());
     return E4HIP_OK;
@@ -595,6 +639,22 @@ int upload_bc(const BlockH& b)
     for (int f = 0; f < 6; ++f) {
         t.type[f] = b.bc_type[f]; t.fs[f] = b.bc_fs[f]; t.nbr[f] = connection_of(b, f);
         memcpy(t.fs_tp[f], b.bc_fs_tp[f], sizeof(t.fs_tp[f]));
+    }
+    const GasModelH& g = g_state.gases[b.gm];
+    if (g.type == GAS_IDEAL) {
+        // the constructed cell's sound speed, sqrt(gamma R T) at 300 K
+        // (ideal_gas.d:137-144), and gamma(Q) = Cp/Cv (gas_model.d:162)
+        t.a_init = sqrt(g.ideal.gammaR * 300.0);
+        const double Cp = g.ideal.Rgas * g.ideal.gamma / (g.ideal.gamma - 1.0);
+        t.wall_gamma = Cp / g.ideal.Cv;
+    } else {
+        t.a_init = g.a_init;
+    }
+    // linear_extrap_prepare(len, len) (onedinterp.d:229-235)
+    const double len[3] = {b.dx, b.dy, b.dz};
+    for (int d = 0; d < 3; ++d) {
+        t.ext_w[d][0] = (2.0 * len[d] + len[d]) / (len[d] + len[d]);
+        t.ext_w[d][1] = -len[d] / (len[d] + len[d]);
     }
     HIP_TRY(hipStreamSynchronize(g_state.stream));
     HIP_TRY(hipMemcpy(b.bc_dev, &t, sizeof(t), hipMemcpyHostToDevice));
@@ -824,7 +884,16 @@ int e4hip_block_set_bc(int blk, int face, int type, const double* fs)
     BlockH* b = get_block(blk);
     if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
     if (face < 0 || face > 5 || (b->dims == 2 && face > 3)) return fail(E4HIP_ERR_ARG, "bad face %d", face);
-    if (type < E4HIP_BC_SLIP_WALL || type > E4HIP_BC_PERIODIC) return fail(E4HIP_ERR_ARG, "bad bc type %d", type);
+    if (type < E4HIP_BC_SLIP_WALL || type > E4HIP_BC_EXTRAPOLATE_1 || type == E4HIP_BC_CONNECTED)
+        return fail(E4HIP_ERR_ARG, "bad bc type %d", type);
+    if (b->is_tp() && type >= E4HIP_BC_OUTFLOW_SIMPLE_FLUX)
+        return fail(E4HIP_ERR_ARG, "bc type %d is not available for a thermally perfect gas", type);
+    if (type == E4HIP_BC_SLIP_WALL_1) {
+        // one-sided reconstruction reaches three cells in from the wall
+        const int n = (face == E4HIP_FACE_EAST || face == E4HIP_FACE_WEST) ? b->ni
+                    : (face == E4HIP_FACE_NORTH || face == E4HIP_FACE_SOUTH) ? b->nj : b->nk;
+        if (n < 3) return fail(E4HIP_ERR_ARG, "a wall without ghost cells needs at least 3 cells across it");
+    }
     if (type == E4HIP_BC_INFLOW && b->is_tp()) {
         if (!fs) return fail(E4HIP_ERR_ARG, "inflow needs a flow state");
         if (!(fs[0] > 0) || !(fs[2] > 0) || !(fs[4] > 0)) return fail(E4HIP_ERR_ARG, "invalid inflow state");

@@ -7,7 +7,12 @@
 
 namespace e4 {
 
-enum { BC_SLIP_WALL = 0, BC_INFLOW = 1, BC_EXTRAPOLATE = 2, BC_PERIODIC = 3, BC_CONNECTED = 4 };
+// 5: OutFlowBC_SimpleFlux, ghosts extrapolated (x_order 0) and the boundary
+// face's flux from the interior cell; 6: WallBC_WithSlip1, no ghost-cell data,
+// one-sided reconstruction and the wall flux; 7: OutFlowBC_SimpleExtrapolate
+// with x_order 1.
+enum { BC_SLIP_WALL = 0, BC_INFLOW = 1, BC_EXTRAPOLATE = 2, BC_PERIODIC = 3, BC_CONNECTED = 4,
+       BC_OUTFLOW_SIMPLE_FLUX = 5, BC_SLIP_WALL_1 = 6, BC_EXTRAPOLATE_1 = 7 };
 enum { FACE_NORTH = 0, FACE_EAST = 1, FACE_SOUTH = 2, FACE_WEST = 3, FACE_TOP = 4, FACE_BOTTOM = 5 };
 
 // A face connected to a face of another block (full-face copy): the other
@@ -68,6 +73,12 @@ struct BCTable {
     Prim fs[6];                       // inflow ghost states (global frame)
     Connection nbr[6];                // connected faces
     double fs_tp[6][8 + TP_NSP_MAX];  // mixture inflow states in the primitive-array order (P_*)
+    // the sound speed of the flow state a cell is constructed with (p = 100
+    // kPa, T = 300 K, at rest, first species; fvcell.d:170), which x_order-1
+    // ghost cells keep (extrapolate_copy.d:102-186 does not write it)
+    double a_init;
+    double wall_gamma;                // Cp/Cv of the ideal gas (gas_model.d:162) for the wall flux
+    double ext_w[3][2];               // linear_extrap_prepare(len, len) weights per direction
 };
 
 struct StageArgs {
@@ -101,6 +112,15 @@ struct StageArgs {
     unsigned char* Snew;
     double shear_tol, compression_tol;
     int always_hanel;
+    // boundaries whose faces the interior code does not handle (one-sided
+    // reconstruction, wall or outflow flux): the stage kernel leaves fix_lo[d]
+    // and fix_hi[d] layers of cells at the two ends of direction d to the
+    // boundary kernel, which covers fix_off[6] cells, slab by slab in the
+    // order (d, side) = (0,-), (0,+), (1,-), (1,+), (2,-), (2,+) starting at
+    // fix_off[slab]; noghost has bit f set when face f has no ghost-cell data
+    int fix_lo[3], fix_hi[3];
+    int fix_off[7];
+    int noghost;
 };
 
 // A batched stage launch: one workgroup per (block, tile, k-chunk) over all
@@ -177,6 +197,10 @@ void launch_tp_decode_exact(const TPArgs& A, const double* U, const double* Tgue
 void launch_tp_decode_fast64(const TPArgs& A, const double* U, const double* Tguess, hipStream_t stream);
 
 void launch_mlp_guess(const GuessModel* G, const double* x, int n, double* T, hipStream_t stream);
+// the cells next to one-sided boundaries, after the stage kernel (A.fix_off[6] > 0)
+void launch_boundary_cells_exact(const StageArgs& A, hipStream_t stream, int dims, int flux);
+void launch_boundary_cells_fast64(const StageArgs& A, hipStream_t stream, int dims, int flux);
+void launch_boundary_cells_mixed(const StageArgs& A, hipStream_t stream, int dims, int flux);
 
 constexpr int TILE_X = 32;
 constexpr int TILE_Y = 8;

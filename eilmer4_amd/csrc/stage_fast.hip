@@ -25,6 +25,16 @@ void launch_stage_batch_mixed(const BatchArgs& B, int nwg, bool def, hipStream_t
     launch_stage_batch_mode<ModeMixed>(B, nwg, def, stream, dims, flux);
 }
 
+void launch_boundary_cells_fast64(const StageArgs& A, hipStream_t stream, int dims, int flux)
+{
+    launch_boundary_cells_mode<ModeFast64>(A, stream, dims, flux);
+}
+
+void launch_boundary_cells_mixed(const StageArgs& A, hipStream_t stream, int dims, int flux)
+{
+    launch_boundary_cells_mode<ModeMixed>(A, stream, dims, flux);
+}
+
 void launch_tp_stage_fast64(const TPArgs& A, hipStream_t stream, int flux)
 {
     launch_tp_stage_m<ModeFast64>(A, stream, flux);

@@ -144,16 +144,37 @@ __device__ Prim fetch_ghost(KArgs A, int i, int j, int k)
         if (SH && A->nbrS[face][c]) P.a = -(P.a);
         return P;
     }
-    int src;
-    if (type == BC_EXTRAPOLATE) src = plus ? n - 1 : 0;
-    else if (type == BC_PERIODIC) src = plus ? layer - 1 : n - layer;
-    else src = plus ? n - layer : layer - 1;     // slip wall mirror
     int si = i, sj = j, sk = k;
+    if (type == BC_EXTRAPOLATE_1) {
+        // extrapolate_copy.d:102-186: linear in rho, u and velocity from the
+        // boundary cell c1 and the one behind it c2 (layer 2 from the first
+        // ghost and c1); the sound speed stays the constructed cell's and the
+        // shock indicator 0, which the extrapolation does not write
+        const int e = plus ? n - 1 : 0, in = plus ? -1 : 1;
+        auto cell = [&](int s) {
+            int ci = i, cj = j, ck = k;
+            if (dir == 0) ci = s; else if (dir == 1) cj = s; else ck = s;
+            return load_decode<M>(A, cell_index(A, ci, cj, ck));
+        };
+        auto lin = [&](const Prim& c1, const Prim& c2) {
+            return Prim{2.0 * c1.rho - c2.rho, 2.0 * c1.v0 - c2.v0, 2.0 * c1.v1 - c2.v1, 2.0 * c1.v2 - c2.v2,
+                        2.0 * c1.u - c2.u, A->bc->a_init};
+        };
+        const Prim c1 = cell(e);
+        const Prim g1 = lin(c1, cell(e + in));
+        return (layer == 1) ? g1 : lin(g1, c1);
+    }
+    int src;
+    if (type == BC_EXTRAPOLATE || type == BC_OUTFLOW_SIMPLE_FLUX) src = plus ? n - 1 : 0;
+    else if (type == BC_PERIODIC) src = plus ? layer - 1 : n - layer;
+    // slip-wall mirror; a wall without ghost data (BC_SLIP_WALL_1) gets the
+    // same values, which only faces the boundary kernel recomputes read
+    else src = plus ? n - layer : layer - 1;
     if (dir == 0) si = src; else if (dir == 1) sj = src; else sk = src;
     const long sc = cell_index(A, si, sj, sk);
     Prim P = load_decode<M>(A, sc);
     if (SH && A->Sprev[sc]) P.a = -(P.a);
-    if (type == BC_SLIP_WALL) {
+    if (type == BC_SLIP_WALL || type == BC_SLIP_WALL_1) {
         // reflect_normal_velocity (ghost_cell.d:32-43): negate the face-normal component
         if (dir == 0) P.v0 = -(P.v0); else if (dir == 1) P.v1 = -(P.v1); else P.v2 = -(P.v2);
     }
@@ -337,6 +358,70 @@ __device__ __forceinline__ void lds_put_flux(R* dst, int nslot, int s, const R F
 {
 #pragma unroll
     for (int q = 0; q < 5; ++q) dst[q * nslot + s] = F[q];
+}
+
+// The rest of a stage for one cell from its flux integral (the sum over its
+// faces of -F * outsign * area): time derivative, stage update, decode checks
+// with the low-T fallback (fvcell.d:570-589, 680-697), check_data
+// (flowstate.d:345-388, gas_state.d:199-241), write, and the wave's counts of
+// invalid cells and decode failures.  All in FP64.
+template <class M>
+__device__ __forceinline__ void finish_cell(KArgs ap, long c, const double integ[5])
+{
+    const long qs = opaque_qs(ap->qs);
+    double Un[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        double d = ap->vol_inv * integ[q] + 0.0;
+        double dd0 = (ap->stage >= 2) ? ap->d0[c + q * qs] : 0.0;
+        double dd1 = (ap->stage >= 3) ? ap->d1[c + q * qs] : 0.0;
+        if (ap->dout) ap->dout[c + q * qs] = d;
+        Un[q] = stage_update(ap, ap->Uold[c + q * qs], dd0, dd1, d);
+    }
+    bool decode_fail = false, invalid = false;
+    double rho = Un[0];
+    if (!(rho > 0.0)) {
+        decode_fail = true;
+    } else {
+        using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+        double dinv = dv<MD>(1.0, rho);
+        double vx = Un[1] * dinv, vy = Un[2] * dinv, vz = Un[3] * dinv;
+        double u = Un[4] * dinv;
+        double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
+        u -= ke;
+        double T;
+        if (u <= 0.0) {
+            T = 200.0;
+            u = ap->gas.Cv * T;
+            Un[1] = rho * vx; Un[2] = rho * vy; Un[3] = rho * vz;
+            Un[4] = rho * (u + ke);
+        } else {
+            T = u * ap->gas.Cvinv;
+        }
+        double p = rho * ap->gas.Rgas * T;
+        if (!isfinite(rho)) invalid = true;
+        if (!isfinite(T) || T < 1.01 * 1.0) invalid = true;
+        if (!isfinite(p)) invalid = true;
+        if (fabs(vx) > 30000.0 || fabs(vy) > 30000.0 || fabs(vz) > 30000.0) invalid = true;
+        if (T > 50000.0) invalid = true;
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) ap->Unew[c + q * qs] = Un[q];
+    unsigned int nbad = wave_count(invalid), nfail = wave_count(decode_fail);
+    if ((nbad | nfail) && (__lane_id() == __ffsll((unsigned long long)__ballot(true)) - 1)) {
+        if (nbad) atomicAdd(&ap->counters[0], nbad);
+        if (nfail) atomicAdd(&ap->counters[1], nfail);
+    }
+}
+
+// Whether the cell is in the layers next to a one-sided boundary that the
+// boundary kernel finishes.
+template <int DIMS>
+__device__ __forceinline__ bool left_to_boundary(KArgs A, int i, int j, int k)
+{
+    bool f = i < A->fix_lo[0] || i >= A->ni - A->fix_hi[0] || j < A->fix_lo[1] || j >= A->nj - A->fix_hi[1];
+    if constexpr (DIMS == 3) f = f || k < A->fix_lo[2] || k >= A->nk - A->fix_hi[2];
+    return f;
 }
 
 // Waves per SIMD the register allocation of the stage kernel aims for
@@ -535,9 +620,10 @@ void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type A
         }
         __syncthreads();
         // 4. flux divergence in face order N, E, S, W, T, B (fvcell.d:789-825),
-        // stage update, decode check, write; all in FP64.
+        // stage update, decode check, write; all in FP64.  Cells next to a
+        // one-sided boundary are the boundary kernel's.
         ap = args();
-        if (own_k) {
+        if (own_k && !left_to_boundary<DIMS>(ap, ik, jk, k)) {
             const int iW = tyk * (TX + 1) + txk, jS = tyk * TX + txk;
             const int iE = iW + 1, jN = jS + TX;
             double integ[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -557,53 +643,7 @@ void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type A
                 add_face(Ft, 1, 0, 1.0, ap->area[2]);
                 add_face(Fb, 1, 0, -1.0, ap->area[2]);
             }
-            const long c = cell_index(ap, ik, jk, k);
-            const long qs = opaque_qs(ap->qs);
-            double Un[5];
-#pragma unroll
-            for (int q = 0; q < 5; ++q) {
-                double d = ap->vol_inv * integ[q] + 0.0;
-                double dd0 = (ap->stage >= 2) ? ap->d0[c + q * qs] : 0.0;
-                double dd1 = (ap->stage >= 3) ? ap->d1[c + q * qs] : 0.0;
-                if (ap->dout) ap->dout[c + q * qs] = d;
-                Un[q] = stage_update(ap, ap->Uold[c + q * qs], dd0, dd1, d);
-            }
-            // decode_conserved checks and the low-T fallback (fvcell.d:570-589, 680-697)
-            bool decode_fail = false, invalid = false;
-            double rho = Un[0];
-            if (!(rho > 0.0)) {
-                decode_fail = true;
-            } else {
-                using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
-                double dinv = dv<MD>(1.0, rho);
-                double vx = Un[1] * dinv, vy = Un[2] * dinv, vz = Un[3] * dinv;
-                double u = Un[4] * dinv;
-                double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
-                u -= ke;
-                double T;
-                if (u <= 0.0) {
-                    T = 200.0;
-                    u = ap->gas.Cv * T;
-                    Un[1] = rho * vx; Un[2] = rho * vy; Un[3] = rho * vz;
-                    Un[4] = rho * (u + ke);
-                } else {
-                    T = u * ap->gas.Cvinv;
-                }
-                double p = rho * ap->gas.Rgas * T;
-                // check_data (flowstate.d:345-388, gas_state.d:199-241)
-                if (!isfinite(rho)) invalid = true;
-                if (!isfinite(T) || T < 1.01 * 1.0) invalid = true;
-                if (!isfinite(p)) invalid = true;
-                if (fabs(vx) > 30000.0 || fabs(vy) > 30000.0 || fabs(vz) > 30000.0) invalid = true;
-                if (T > 50000.0) invalid = true;
-            }
-#pragma unroll
-            for (int q = 0; q < 5; ++q) ap->Unew[c + q * qs] = Un[q];
-            unsigned int nbad = wave_count(invalid), nfail = wave_count(decode_fail);
-            if ((nbad | nfail) && (__lane_id() == __ffsll((unsigned long long)__ballot(true)) - 1)) {
-                if (nbad) atomicAdd(&ap->counters[0], nbad);
-                if (nfail) atomicAdd(&ap->counters[1], nfail);
-            }
+            finish_cell<M>(ap, cell_index(ap, ik, jk, k), integ);
         }
         // 5. k-direction work for plane k+1: face states of cell k+2 and the
         // flux of the top face of cell k+1
@@ -628,6 +668,264 @@ void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type A
                 own = qa; qa = qb; qb = q3;
             }
         }
+    }
+}
+
+// ------------------------------------------------------------ boundaries
+// Faces whose flux the reference computes otherwise than from two
+// reconstructed sides (sfluidblock.d:2189-2345): next to a boundary without
+// ghost-cell data (WallBC_WithSlip1) the boundary face takes the wall flux of
+// fluxcalc.d:130-298 and the face after it the one-sided reconstruction
+// l1r2/l2r1 of onedinterp.d:165-818; on an OutFlowBC_SimpleFlux face the
+// flux is the interior cell's (boundary_flux_effect.d:367-546).  The stage
+// kernel leaves the cells beside such faces (two layers, one layer) to
+// boundary_cell_kernel, which evaluates all faces of each such cell, interior
+// ones with the stage kernel's functions, and finishes the cell.
+
+// compute_flux_at_left_wall (left) / compute_flux_at_right_wall with the wall
+// at rest: rho, p, a, vn the gas state beside the wall (vn along the face
+// normal), gam its Cp/Cv.  Returns the wall pressure; the flux is that
+// pressure in the normal momentum and zero elsewhere.  FP64 in every mode.
+__device__ inline double wall_pressure(double rho, double p, double a, double vn, double gam, bool left)
+{
+    const double vstar = 0.0;
+    double tmp;
+    if (left) {
+        const double Ubar = vn - 2.0 * a / (gam - 1.0);
+        tmp = (vstar - Ubar) * (gam - 1.0) / (2.0 * sqrt(gam)) * sqrt(rho / pow(p, 1.0 / gam));
+    } else {
+        const double Ubar = vn + 2.0 * a / (gam - 1.0);
+        tmp = (Ubar - vstar) * (gam - 1.0) / (2.0 * sqrt(gam)) * sqrt(rho / pow(p, 1.0 / gam));
+    }
+    const double ptiny = 0.1;   // FlowStateLimits.min_pressure (fvcore.d:26)
+    double pstar = (tmp > 0.0) ? pow(tmp, 2.0 * gam / (gam - 1.0)) : ptiny;
+    if (pstar > 1.1 * p) {
+        // the wave is a shock: secant steps on the shock relation, at most 10
+        auto f = [&](double ps) {
+            const double xi = ps / p;
+            const double M1sq = 1.0 + (gam + 1.0) / 2.0 / gam * (xi - 1.0);
+            const double u1 = sqrt(M1sq) * a;
+            const double u2 = u1 * ((gam - 1.0) * M1sq + 2.0) / ((gam + 1.0) * M1sq);
+            return left ? vstar - u1 + u2 - vn : vstar + u1 - u2 - vn;
+        };
+        int count = 0;
+        double incr;
+        do {
+            const double f0 = f(pstar);
+            const double dp = 0.001 * pstar;
+            const double f1 = f(pstar + dp);
+            incr = -f0 * dp / (f1 - f0);
+            pstar += incr;
+            count += 1;
+        } while (fabs(incr) / pstar > 0.01 && count < 10);
+    }
+    return fmin(pstar, p * 10.0);
+}
+
+__device__ __forceinline__ double prim_comp(const Prim& P, int d) { return d == 0 ? P.v0 : (d == 1 ? P.v1 : P.v2); }
+
+// The cell (i, j, k) with its index along direction D replaced by p.
+template <int D>
+__device__ __forceinline__ void along(int& i, int& j, int& k, int p)
+{
+    if constexpr (D == 0) i = p;
+    else if constexpr (D == 1) j = p;
+    else k = p;
+}
+
+// The side of an l1r2 (or, mirrored, l2r1) face next to a boundary without
+// ghost data, per scalar (onedinterp.d:187-191, 218-222): the mean of the two
+// cells beside the face (linear_interp_prepare on equal lengths gives weights
+// 1/2), or the boundary-side cell's value where the limiter sees an extremum
+// over (boundary-side, far, beyond).  The reference's product of the two
+// scaled differences is formed from the negated pair on the l2r1 side, the
+// same number.
+template <class M>
+__device__ __forceinline__ typename M::R lin_side(const L2R2UT<typename M::R>& w, bool lim, bool clip,
+                                                 typename M::R qn, typename M::R qf, typename M::R qb)
+{
+    using R = typename M::R;
+    const R del = (qf - qn) * w.c;
+    const R delp = (qb - qf) * w.c;
+    if (lim && delp * del < R(0)) return qn;
+    R q = qn * R(0.5) + qf * R(0.5);
+    if (clip) q = clip_limits<M>(q, qn, qf);
+    return q;
+}
+
+template <class M>
+__device__ __forceinline__ SideT<typename M::R> lin_side_state(const L2R2UT<typename M::R>& w, bool lim, bool clip,
+                                                               const PrimT<typename M::R>& n,
+                                                               const PrimT<typename M::R>& f,
+                                                               const PrimT<typename M::R>& b)
+{
+    using R = typename M::R;
+    SideT<R> s{lin_side<M>(w, lim, clip, n.rho, f.rho, b.rho), lin_side<M>(w, lim, clip, n.v0, f.v0, b.v0),
+               lin_side<M>(w, lim, clip, n.v1, f.v1, b.v1), lin_side<M>(w, lim, clip, n.v2, f.v2, b.v2),
+               lin_side<M>(w, lim, clip, n.u, f.u, b.u), n.a};
+    // the EOS fall-back to the cell's state (onedinterp.d:1039-1068)
+    if (s.u <= R(0) || s.rho <= R(0)) s = SideT<R>{n.rho, n.v0, n.v1, n.v2, n.u, n.a};
+    return s;
+}
+
+// Flux (global frame, FP64) of the direction-D face whose plus-side cell is
+// (i, j, k); pos is that cell's index along D (0 .. n).
+template <class M, int FLUX, int D, int DIMS>
+__device__ __forceinline__ void boundary_face_flux(KArgs A, int i, int j, int k, double F[5])
+{
+    using R = typename M::R;
+    constexpr bool SH = (FLUX == FLUX_SHOCK);
+    constexpr int PN = D, PT1 = (D == 0) ? 1 : (D == 1 ? ((DIMS == 3) ? 2 : 0) : 0);
+    constexpr int PT2 = (D == 0) ? 2 : (D == 1 ? ((DIMS == 3) ? 0 : 2) : 1);
+    const int n = (D == 0) ? A->ni : (D == 1 ? A->nj : A->nk);
+    const int pos = (D == 0) ? i : (D == 1 ? j : k);
+    const int fm = (D == 0) ? FACE_WEST : (D == 1 ? FACE_SOUTH : FACE_BOTTOM);
+    const int fp = (D == 0) ? FACE_EAST : (D == 1 ? FACE_NORTH : FACE_TOP);
+    const int tm = A->bc->type[fm], tp = A->bc->type[fp];
+    auto cell64 = [&](int p) {
+        int ci = i, cj = j, ck = k;
+        along<D>(ci, cj, ck, p);
+        return load_decode<M>(A, cell_index(A, ci, cj, ck));
+    };
+    auto cellR = [&](int p) {
+        int ci = i, cj = j, ck = k;
+        along<D>(ci, cj, ck, p);
+        return fetch<M, SH>(A, ci, cj, ck);
+    };
+#pragma unroll
+    for (int q = 0; q < 5; ++q) F[q] = 0.0;
+    const bool ho = A->high_order != 0, lim = A->apply_limiter != 0, clip = A->extrema_clipping != 0;
+    const IdealGas g = A->gas;
+    // outflow flux of the interior cell's state (its p = rho Rgas T, T = u/Cv,
+    // as the decode has it)
+    if ((pos == 0 && tm == BC_OUTFLOW_SIMPLE_FLUX) || (pos == n && tp == BC_OUTFLOW_SIMPLE_FLUX)) {
+        const bool plus = (pos == n);
+        const Prim c = cell64(plus ? n - 1 : 0);
+        const double p = c.rho * g.Rgas * (c.u * g.Cvinv);
+        const double vn = prim_comp(c, D);
+        const double mass_flux = c.rho * vn;
+        if ((plus ? 1.0 : -1.0) * mass_flux > 0.0) {
+            F[0] = mass_flux;
+            F[1] = c.v0 * mass_flux; F[2] = c.v1 * mass_flux; F[3] = c.v2 * mass_flux;
+            F[1 + D] = p + vn * mass_flux;
+            const double utot = c.u + 0.5 * (c.v0 * c.v0 + c.v1 * c.v1 + c.v2 * c.v2);
+            F[4] = mass_flux * utot + p * vn;
+        } else {
+            F[1 + D] = p;
+        }
+        return;
+    }
+    // wall face: the wall cell's state, or with reconstruction and no
+    // clipping its linear extrapolation (onedinterp.d:820-1037)
+    if ((pos == 0 && tm == BC_SLIP_WALL_1) || (pos == n && tp == BC_SLIP_WALL_1)) {
+        const bool left = (pos == 0);
+        const Prim c0 = cell64(left ? 0 : n - 1);
+        double rho = c0.rho, u = c0.u, vn = prim_comp(c0, D);
+        if (ho && !clip) {
+            const Prim c1 = cell64(left ? 1 : n - 2);
+            const double w0 = A->bc->ext_w[D][0], w1 = A->bc->ext_w[D][1];
+            const double r = c0.rho * w0 + c1.rho * w1, e = c0.u * w0 + c1.u * w1;
+            if (e > 0.0 && r > 0.0) {
+                rho = r; u = e;
+                vn = prim_comp(c0, D) * w0 + prim_comp(c1, D) * w1;
+            }
+        }
+        const double p = rho * g.Rgas * (u * g.Cvinv);
+        F[1 + D] = wall_pressure(rho, p, c0.a, vn, A->bc->wall_gamma, left);
+        return;
+    }
+    const L2R2UT<R> w = cast_wu<R>(L2R2U(A->wu[D]));
+    SideT<R> L, Rr, unused;
+    if (pos == 1 && tm == BC_SLIP_WALL_1) {
+        const PrimT<R> q0 = cellR(0), q1 = cellR(1), q2 = cellR(2);
+        recon_cell<M>(w, ho, lim, clip, q0, q1, q2, unused, Rr);
+        L = ho ? lin_side_state<M>(w, lim, clip, q0, q1, q2) : SideT<R>{q0.rho, q0.v0, q0.v1, q0.v2, q0.u, q0.a};
+    } else if (pos == n - 1 && tp == BC_SLIP_WALL_1) {
+        const PrimT<R> qa = cellR(n - 3), qb = cellR(n - 2), qc = cellR(n - 1);
+        recon_cell<M>(w, ho, lim, clip, qa, qb, qc, L, unused);
+        Rr = ho ? lin_side_state<M>(w, lim, clip, qc, qb, qa) : SideT<R>{qc.rho, qc.v0, qc.v1, qc.v2, qc.u, qc.a};
+    } else {
+        const PrimT<R> qm2 = cellR(pos - 2), qm1 = cellR(pos - 1), q0 = cellR(pos), qp1 = cellR(pos + 1);
+        recon_cell<M>(w, ho, lim, clip, qm2, qm1, q0, L, unused);
+        recon_cell<M>(w, ho, lim, clip, qm1, q0, qp1, unused, Rr);
+    }
+    R Fg[5];
+    side_flux<M, FLUX, PN, PT1, PT2>(L, Rr, cast_gas<R>(g), Fg, shock_opt<R>(A));
+#pragma unroll
+    for (int q = 0; q < 5; ++q) F[q] = double(Fg[q]);
+}
+
+// One thread per cell of the slabs beside one-sided boundaries (each cell
+// once: a cell in several slabs belongs to the first).
+template <class M, int DIMS, int FLUX>
+__global__ __launch_bounds__(64) void boundary_cell_kernel(StageArgs)
+{
+    KArgs A = kargs();
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A->fix_off[6]) return;
+    int slab = 0;
+    while (slab < 5 && t >= A->fix_off[slab + 1]) ++slab;
+    const int d = slab >> 1, side = slab & 1;
+    const int n[3] = {A->ni, A->nj, (DIMS == 3) ? A->nk : 1};
+    const int L = side ? ((d == 0) ? A->fix_hi[0] : (d == 1 ? A->fix_hi[1] : A->fix_hi[2]))
+                       : ((d == 0) ? A->fix_lo[0] : (d == 1 ? A->fix_lo[1] : A->fix_lo[2]));
+    const int da = (d + 1) % 3, db = (d + 2) % 3;
+    const int local = t - A->fix_off[slab];
+    const int layer = local % L, rest = local / L;
+    int c[3];
+    c[d] = side ? n[d] - 1 - layer : layer;
+    c[da] = rest % n[da];
+    c[db] = rest / n[da];
+    for (int s2 = 0; s2 < slab; ++s2) {
+        const int d2 = s2 >> 1;
+        const int L2 = (s2 & 1) ? A->fix_hi[d2] : A->fix_lo[d2];
+        if ((s2 & 1) ? (c[d2] >= n[d2] - L2) : (c[d2] < L2)) return;
+    }
+    const int i = c[0], j = c[1], k = c[2];
+    double FN[5], FE[5], FS[5], FW[5], FT[5], FB[5];
+    boundary_face_flux<M, FLUX, 1, DIMS>(A, i, j + 1, k, FN);
+    boundary_face_flux<M, FLUX, 0, DIMS>(A, i + 1, j, k, FE);
+    boundary_face_flux<M, FLUX, 1, DIMS>(A, i, j, k, FS);
+    boundary_face_flux<M, FLUX, 0, DIMS>(A, i, j, k, FW);
+    if constexpr (DIMS == 3) {
+        boundary_face_flux<M, FLUX, 2, DIMS>(A, i, j, k + 1, FT);
+        boundary_face_flux<M, FLUX, 2, DIMS>(A, i, j, k, FB);
+    }
+    // flux divergence in face order N, E, S, W, T, B (fvcell.d:789-825)
+    double integ[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    auto add_face = [&](const double* Fq, double outsign, double Aface) {
+        const double area = outsign * Aface;
+        integ[0] -= Fq[0] * area;
+        integ[1] -= Fq[1] * area;
+        integ[2] -= Fq[2] * area;
+        if (DIMS == 3) integ[3] -= Fq[3] * area;
+        integ[4] -= Fq[4] * area;
+    };
+    add_face(FN, 1.0, A->area[1]);
+    add_face(FE, 1.0, A->area[0]);
+    add_face(FS, -1.0, A->area[1]);
+    add_face(FW, -1.0, A->area[0]);
+    if constexpr (DIMS == 3) {
+        add_face(FT, 1.0, A->area[2]);
+        add_face(FB, -1.0, A->area[2]);
+    }
+    finish_cell<M>(A, cell_index(A, i, j, k), integ);
+}
+
+template <class M>
+void launch_boundary_cells_mode(const StageArgs& A, hipStream_t stream, int dims, int flux)
+{
+    const int n = A.fix_off[6];
+    if (n <= 0) return;
+    const dim3 block(64), grid((unsigned)((n + 63) / 64));
+    if (dims == 3) {
+        if (flux == FLUX_HLLC) hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_HLLC>), grid, block, 0, stream, A);
+        else if (flux == FLUX_SHOCK) hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_SHOCK>), grid, block, 0, stream, A);
+        else hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_AUSMDV>), grid, block, 0, stream, A);
+    } else {
+        if (flux == FLUX_HLLC) hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_HLLC>), grid, block, 0, stream, A);
+        else if (flux == FLUX_SHOCK) hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_SHOCK>), grid, block, 0, stream, A);
+        else hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_AUSMDV>), grid, block, 0, stream, A);
     }
 }
 
@@ -743,12 +1041,24 @@ __global__ __launch_bounds__(256) void shock_detect_kernel(StageArgs)
         __syncthreads();
         if (own) {
             const double (*c)[PL] = sv[buf];
-            bool S = face(c[0][h - 1], c[3][h - 1], c[0][h], c[3][h]) || face(c[0][h], c[3][h], c[0][h + 1], c[3][h + 1]) ||
-                     face(c[1][h - PX], c[3][h - PX], c[1][h], c[3][h]) ||
-                     face(c[1][h], c[3][h], c[1][h + PX], c[3][h + PX]);
+            // a boundary face without ghost data has the cell on one side only
+            // (shockdetectors.d:39-54): its velocity towards the wall
+            const int ng = A->noghost;
+            const double a0 = c[3][h];
+            auto minus_face = [&](bool wall, double uL, double aL, double u0) {
+                return wall ? dv<MD>(u0, a0) < tol : face(uL, aL, u0, a0);
+            };
+            auto plus_face = [&](bool wall, double u0, double uR, double aR) {
+                return wall ? dv<MD>(-u0, a0) < tol : face(u0, a0, uR, aR);
+            };
+            bool S = minus_face(i == 0 && (ng & (1 << FACE_WEST)), c[0][h - 1], c[3][h - 1], c[0][h]) ||
+                     plus_face(i == A->ni - 1 && (ng & (1 << FACE_EAST)), c[0][h], c[0][h + 1], c[3][h + 1]) ||
+                     minus_face(j == 0 && (ng & (1 << FACE_SOUTH)), c[1][h - PX], c[3][h - PX], c[1][h]) ||
+                     plus_face(j == A->nj - 1 && (ng & (1 << FACE_NORTH)), c[1][h], c[1][h + PX], c[3][h + PX]);
             if constexpr (DIMS == 3) {
                 const double (*n)[PL] = sv[buf ^ 1];
-                S = S || face(vz_below, a_below, c[2][h], c[3][h]) || face(c[2][h], c[3][h], n[2][h], n[3][h]);
+                S = S || minus_face(k == 0 && (ng & (1 << FACE_BOTTOM)), vz_below, a_below, c[2][h]) ||
+                    plus_face(k == A->nk - 1 && (ng & (1 << FACE_TOP)), c[2][h], n[2][h], n[3][h]);
                 vz_below = c[2][h];
                 a_below = c[3][h];
             }

@@ -94,6 +94,9 @@ class BCType(IntEnum):
     extrapolate = 2        # OutFlowBC_SimpleExtrapolate, x_order 0
     periodic = 3           # ExchangeBC_FullFace of a block with itself
     connected = 4          # ExchangeBC_FullFace to another block (SFluidBlock.connect)
+    outflow_simple_flux = 5  # OutFlowBC_SimpleFlux / OutFlowBC_Simple (bc.lua:1124-1148)
+    slip_wall_1 = 6        # WallBC_WithSlip1: no ghost data, one-sided reconstruction, wall flux
+    extrapolate_1 = 7      # OutFlowBC_SimpleExtrapolate{xOrder=1}
 
 
 class PrecisionMode(IntEnum):

@@ -69,6 +69,14 @@ extern "C" {
                                      opposite faces: full_face_copy.d:1277-1357 */
 #define E4HIP_BC_CONNECTED     4  /* ExchangeBC_FullFace to another block (set by
                                      e4hip_block_connect): full_face_copy.d:142-962, 1277-1357 */
+#define E4HIP_BC_OUTFLOW_SIMPLE_FLUX 5  /* OutFlowBC_SimpleFlux (alias OutFlowBC_Simple, bc.lua:1124-1148):
+                                     ghosts by x_order-0 extrapolation, the boundary face's flux
+                                     from the interior cell (boundary_flux_effect.d:367-546) */
+#define E4HIP_BC_SLIP_WALL_1   6  /* WallBC_WithSlip1 (bc.lua:554-577): no ghost-cell data; one-sided
+                                     reconstruction onedinterp.d:165-1037 and the wall flux
+                                     fluxcalc.d:130-298 (sfluidblock.d:2189-2345); needs >= 3 cells
+                                     across the wall */
+#define E4HIP_BC_EXTRAPOLATE_1 7  /* OutFlowBC_SimpleExtrapolate{xOrder=1}: extrapolate_copy.d:102-186 */
 
 /* ---- face names (eilmer/fvcore.d Face enum order) */
 #define E4HIP_FACE_NORTH  0

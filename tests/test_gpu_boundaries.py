@@ -1,0 +1,161 @@
+"""Boundary flavours that change face fluxes on the device against the oracle:
+WallBC_WithSlip1 (no ghost data: one-sided reconstruction, wall flux),
+OutFlowBC_SimpleFlux (boundary outflow flux) and x_order-1 extrapolation,
+through the stage kernel plus the boundary kernel that finishes the cells
+beside such faces.  Bars: fp64_exact bit-identical except where the wall
+flux's pow() enters (the device's and the host's pow may differ in the last
+bit: 1e-13 there), fp64 1e-11, mixed density 1e-5."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+import eilmer4_amd as e4
+from eilmer4_amd.fvcore import BCType
+from parity_helpers import oracle_block, device_block, device_sim, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _field(gas, shape, seed, vmean=(0.0, 0.0, 0.0), vamp=300.0):
+    rng = np.random.default_rng(seed)
+    p = 1.0e5 * (1.0 + 0.4 * rng.random(shape))
+    T = 300.0 * (1.0 + 0.4 * rng.random(shape))
+    v = [vmean[d] + vamp * (rng.random(shape) - 0.5) for d in range(3)]
+    rho, u, _ = gas.rho_u_a_from_pT(p.ravel(), T.ravel())
+    return gas.encode(rho, u, *[x.ravel() for x in v])
+
+
+def _run(dims, n, bcs, U0, steps, flux, precision, clipping=True):
+    gas = e4.IdealGas()
+    ni, nj, nk = n
+    geom = dict(dims=dims, ni=ni, nj=nj, nk=nk, dx=1.0 / ni, dy=1.0 / nj, dz=1.0 / nk)
+    fc = e4.flux_calculator_from_name(flux)
+    ob = oracle_block(geom, bcs, U0, flux=int(fc), clipping=int(clipping))
+    db = device_block(gas, geom, bcs, U0)
+    dsim = device_sim([db], flux=fc, clipping=clipping, precision=precision)
+    osim = O.Simulation(ob)
+    for s in range(steps):
+        osim.run(max_step=s + 1)
+        dsim.state.dt_global = osim.dt_global
+        dsim.gasdynamic_explicit_increment_with_fixed_grid()
+        dsim.state.step += 1
+    Uo, Ud = ob.get_U(), db.download()
+    db.destroy()
+    return Uo, Ud
+
+
+def _check(precision, Ud, Uo, wall):
+    err = rel_err(Ud, Uo)
+    bitwise = bool(np.array_equal(Ud, Uo))
+    print("  %s: rel err %s bitwise %s" % (precision, err, bitwise), flush=True)
+    assert np.all(np.isfinite(Ud))
+    if precision == "fp64_exact":
+        if wall:
+            assert err.max() <= 1.0e-13
+        else:
+            assert bitwise
+    elif precision == "mixed":
+        assert err[0] <= 1.0e-5
+    else:
+        assert err.max() <= 1.0e-11
+
+
+CASES = {
+    # name: (bcs, mean velocity, wall flux involved)
+    "slip1_box": ({f: (BCType.slip_wall_1, None) for f in range(6)}, (0.0, 0.0, 0.0), True),
+    "slip1_west_east": ({O.WEST: (BCType.slip_wall_1, None), O.EAST: (BCType.slip_wall_1, None),
+                         O.NORTH: (BCType.periodic, None), O.SOUTH: (BCType.periodic, None),
+                         O.TOP: (BCType.slip_wall, None), O.BOTTOM: (BCType.extrapolate, None)},
+                        (150.0, 0.0, 0.0), True),
+    "simple_flux": ({O.EAST: (BCType.outflow_simple_flux, None), O.WEST: (BCType.extrapolate, None),
+                     O.NORTH: (BCType.outflow_simple_flux, None), O.SOUTH: (BCType.slip_wall, None),
+                     O.TOP: (BCType.outflow_simple_flux, None), O.BOTTOM: (BCType.outflow_simple_flux, None)},
+                    (200.0, -120.0, 60.0), False),
+    "extrapolate_1": ({O.EAST: (BCType.extrapolate_1, None), O.WEST: (BCType.extrapolate_1, None),
+                       O.NORTH: (BCType.extrapolate_1, None), O.SOUTH: (BCType.slip_wall, None),
+                       O.TOP: (BCType.periodic, None), O.BOTTOM: (BCType.periodic, None)},
+                      (100.0, 50.0, 0.0), False),
+    "mixed_flavours": ({O.EAST: (BCType.outflow_simple_flux, None), O.WEST: (BCType.slip_wall_1, None),
+                        O.NORTH: (BCType.extrapolate_1, None), O.SOUTH: (BCType.slip_wall_1, None),
+                        O.TOP: (BCType.outflow_simple_flux, None), O.BOTTOM: (BCType.slip_wall_1, None)},
+                       (80.0, -40.0, 30.0), True),
+}
+
+
+@pytest.mark.parametrize("precision", ["fp64_exact", "fp64", "mixed"])
+@pytest.mark.parametrize("case", list(CASES))
+def test_boundaries_3d(gpu, case, precision):
+    bcs, vmean, wall = CASES[case]
+    n = (11, 9, 7)
+    U0 = _field(e4.IdealGas(), (n[2], n[1], n[0]), 31, vmean)
+    print("case %s 3D %s precision=%s steps=4" % (case, n, precision), flush=True)
+    Uo, Ud = _run(3, n, bcs, U0, 4, "ausmdv", precision)
+    _check(precision, Ud, Uo, wall)
+
+
+@pytest.mark.parametrize("flux", ["hllc", "adaptive_hanel_ausmdv"])
+def test_boundaries_fluxes(gpu, flux):
+    """The boundary kernel's interior faces use the same flux calculator,
+    shock indicators included (a fast compression into a wall without ghost
+    data flags the wall cell by the one-sided detector test)."""
+    bcs, vmean, wall = CASES["mixed_flavours"]
+    n = (12, 10, 8)
+    U0 = _field(e4.IdealGas(), (n[2], n[1], n[0]), 32, (-250.0, 0.0, 0.0), vamp=500.0)
+    print("case mixed_flavours flux=%s fp64_exact steps=4" % flux, flush=True)
+    Uo, Ud = _run(3, n, bcs, U0, 4, flux, "fp64_exact")
+    _check("fp64_exact", Ud, Uo, wall)
+
+
+@pytest.mark.parametrize("precision", ["fp64_exact", "fp64"])
+def test_boundaries_no_clipping(gpu, precision):
+    """Without extrema clipping the wall face extrapolates the two cells next
+    to it (interp_l0r2 / interp_l2r0) instead of copying the wall cell."""
+    bcs, vmean, wall = CASES["mixed_flavours"]
+    n = (10, 8, 6)
+    U0 = _field(e4.IdealGas(), (n[2], n[1], n[0]), 33, vmean)
+    print("case mixed_flavours clipping off precision=%s" % precision, flush=True)
+    Uo, Ud = _run(3, n, bcs, U0, 4, "ausmdv", precision, clipping=False)
+    _check(precision, Ud, Uo, wall)
+
+
+@pytest.mark.parametrize("precision", ["fp64_exact", "fp64"])
+def test_boundaries_2d(gpu, precision):
+    """The 2D block (C1-like): WallBC_WithSlip1 on north and south,
+    OutFlowBC_Simple on the east, supersonic inflow on the west."""
+    gas = e4.IdealGas()
+    ni, nj = 40, 16
+    U0 = _field(gas, (1, nj, ni), 34, (600.0, 0.0, 0.0), vamp=200.0)
+    rho, u, a = gas.rho_u_a_from_pT(np.array([1.0e5]), np.array([300.0]))
+    fs = np.array([rho[0], 1.0e5, 300.0, u[0], a[0], 700.0, 0.0, 0.0])
+    bcs = {O.WEST: (BCType.inflow, fs), O.EAST: (BCType.outflow_simple_flux, None),
+           O.NORTH: (BCType.slip_wall_1, None), O.SOUTH: (BCType.slip_wall_1, None)}
+    print("case 2D %dx%d slip1 + simple flux precision=%s steps=20" % (ni, nj, precision), flush=True)
+    Uo, Ud = _run(2, (ni, nj, 1), bcs, U0, 20, "ausmdv", precision)
+    _check(precision, Ud, Uo, True)
+
+
+def test_boundaries_batched(gpu):
+    """Two blocks in one batched launch, each with one-sided boundaries: the
+    boundary kernel runs per block after the batched stage kernel."""
+    gas = e4.IdealGas()
+    n = 9
+    fc = int(e4.FluxCalculator.ausmdv)
+    Us = [_field(gas, (n, n, n), 40 + s, (60.0, 0.0, 0.0)) for s in range(2)]
+    obs, dbs = [], []
+    geom = dict(dims=3, ni=n, nj=n, nk=n, dx=0.1, dy=0.1, dz=0.1)
+    bcs = [{O.WEST: (BCType.slip_wall_1, None), O.EAST: (BCType.outflow_simple_flux, None)},
+           {O.NORTH: (BCType.extrapolate_1, None), O.BOTTOM: (BCType.slip_wall_1, None)}]
+    for s in range(2):
+        obs.append(oracle_block(geom, bcs[s], Us[s], flux=fc))
+        dbs.append(device_block(gas, geom, bcs[s], Us[s]))
+    dsim = device_sim(dbs, flux=e4.FluxCalculator.ausmdv, precision="fp64_exact")   # batch_blocks on by default
+    for _ in range(3):
+        dt = min(b.determine_dt(1e-3, 0.5, False)[0] for b in obs)
+        assert O.multi_step(obs, dt)[0] == 0
+        dsim.state.dt_global = dt
+        dsim.gasdynamic_explicit_increment_with_fixed_grid()
+    for s in range(2):
+        err = rel_err(dbs[s].download(), obs[s].get_U())
+        print("  block %d rel err %s" % (s, err), flush=True)
+        assert err.max() <= 1.0e-13
+        dbs[s].destroy()
