@@ -4,6 +4,7 @@
 // (eilmer/simcore.d:2303-2952) and determine_time_step_size
 // (eilmer/simcore.d:1347-1452).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -59,6 +60,7 @@ struct BlockH {
     double* F[3] = {nullptr, nullptr, nullptr};
     long fqs[3] = {0, 0, 0};
     double bc_fs_tp[6][8 + e4::TP_NSP_MAX] = {};
+    double* gx[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   // mixture x_order-1 ghosts
     // shock indicators per cell (adaptive_hanel_ausmdv): S[0] this stage's
     // detection, S[1] the previous one
     unsigned char* S[2] = {nullptr, nullptr};
@@ -537,6 +539,23 @@ int tp_args_for(const BlockH& b, int stage, double dt, unsigned int* counters, e
     A.c1 = dt * g1;
     A.Pin = b.P[0];
     A.Pout = b.P[1];
+    // faces the boundary-face kernel recomputes: per slab, the face layers
+    // (wall and the face after it; a simple-outflow-flux face)
+    {
+        int lo[3], hi[3], off[7];
+        boundary_slabs(b, lo, hi, off, A.noghost);
+        const int n[3] = {b.ni, b.nj, (b.dims == 3) ? b.nk : 1};
+        int total = 0;
+        for (int sl = 0; sl < 6; ++sl) {
+            const int d = sl >> 1;
+            const int L = (sl & 1) ? hi[d] : lo[d];   // cell layers: 2 wall, 1 outflow; face layers alike
+            A.fix_layers[sl] = L;
+            A.fix_off[sl] = total;
+            total += L * n[(d + 1) % 3] * n[(d + 2) % 3];
+        }
+        A.fix_off[6] = total;
+    }
+    for (int f = 0; f < 6; ++f) A.gx[f] = b.gx[f];
     return shock_fields(b, A);
 }
 
@@ -626,6 +645,7 @@ int free_block_buffers(BlockH& b)
     for (auto& p : b.P) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     for (auto& p : b.F) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     for (auto& p : b.S) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
+    for (auto& p : b.gx) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     if (b.bc_dev) { HIP_TRY(hipFree(b.bc_dev)); b.bc_dev = nullptr; }
     return E4HIP_OK;
 }
@@ -733,6 +753,16 @@ int e4hip_gas_ideal(double mMass, double gamma, int* gm)
     return E4HIP_OK;
 }
 
+// the sound speed of the constructed cell state of a mixture (fvcell.d:170:
+// p = 100 kPa, T = 300 K, all of the first species; therm_perf_gas.d:345-355)
+__global__ void tp_init_sound_speed_kernel(const e4::TPGas* g, double* a)
+{
+    e4::TPState<5> Q;
+    Q.T = 300.0;
+    for (int s = 0; s < 5; ++s) Q.Y[s] = (s == 0) ? 1.0 : 0.0;
+    *a = e4::tp_sound_speed<e4::ModeExact, 5>(*g, Q);
+}
+
 int e4hip_gas_tp(int nsp, const double* M, int nseg, const double* Tbreaks, const double* Tblends,
                  const double* coeffs, int* gm)
 {
@@ -767,6 +797,14 @@ int e4hip_gas_tp(int nsp, const double* M, int nseg, const double* Tbreaks, cons
             for (int s = 0; s < nsp; ++s) g.tp.RC[q][k][s] = g.tp.R[s] * g.tp.curve[s].c[q][k];
     HIP_TRY(hipMalloc(&g.tp_dev, sizeof(e4::TPGas)));
     HIP_TRY(hipMemcpy(g.tp_dev, &g.tp, sizeof(e4::TPGas), hipMemcpyHostToDevice));
+    {
+        double* da = nullptr;
+        HIP_TRY(hipMalloc(&da, sizeof(double)));
+        hipLaunchKernelGGL(tp_init_sound_speed_kernel, dim3(1), dim3(1), 0, g_state.stream, g.tp_dev, da);
+        HIP_TRY(hipMemcpyAsync(&g.a_init, da, sizeof(double), hipMemcpyDeviceToHost, g_state.stream));
+        HIP_TRY(hipStreamSynchronize(g_state.stream));
+        HIP_TRY(hipFree(da));
+    }
     g_state.gases.push_back(g);
     *gm = (int)g_state.gases.size() - 1;
     return E4HIP_OK;
@@ -886,8 +924,6 @@ int e4hip_block_set_bc(int blk, int face, int type, const double* fs)
     if (face < 0 || face > 5 || (b->dims == 2 && face > 3)) return fail(E4HIP_ERR_ARG, "bad face %d", face);
     if (type < E4HIP_BC_SLIP_WALL || type > E4HIP_BC_EXTRAPOLATE_1 || type == E4HIP_BC_CONNECTED)
         return fail(E4HIP_ERR_ARG, "bad bc type %d", type);
-    if (b->is_tp() && type >= E4HIP_BC_OUTFLOW_SIMPLE_FLUX)
-        return fail(E4HIP_ERR_ARG, "bc type %d is not available for a thermally perfect gas", type);
     if (type == E4HIP_BC_SLIP_WALL_1) {
         // one-sided reconstruction reaches three cells in from the wall
         const int n = (face == E4HIP_FACE_EAST || face == E4HIP_FACE_WEST) ? b->ni
@@ -909,6 +945,25 @@ int e4hip_block_set_bc(int blk, int face, int type, const double* fs)
         P.rho = fs[0]; P.u = fs[3]; P.a = fs[4]; P.v0 = fs[5]; P.v1 = fs[6]; P.v2 = fs[7];
         if (!(P.rho > 0) || !(P.u > 0) || !(P.a > 0)) return fail(E4HIP_ERR_ARG, "invalid inflow state");
         b->bc_fs[face] = P;
+    }
+    if (type == E4HIP_BC_EXTRAPOLATE_1 && b->is_tp() && !b->gx[face]) {
+        // the face's two ghost layers, holding the constructed cell state
+        // (p = 100 kPa, T = 300 K, first species, at rest) until the first fill
+        const int dir = (face == E4HIP_FACE_EAST || face == E4HIP_FACE_WEST) ? 0
+                      : (face == E4HIP_FACE_NORTH || face == E4HIP_FACE_SOUTH) ? 1 : 2;
+        const long nfc = (long)((dir == 0) ? b->nj : b->ni) * ((dir == 2) ? b->nj : b->nk);
+        const int np = 8 + b->nsp;
+        std::vector<double> h((size_t)np * 2 * nfc, 0.0);
+        const GasModelH& g = g_state.gases[b->gm];
+        auto fill = [&](int field, double v) { std::fill(h.begin() + field * 2 * nfc, h.begin() + (field + 1) * 2 * nfc, v); };
+        fill(e4::P_T, 300.0);
+        fill(e4::P_P, 100.0e3);
+        fill(e4::P_A, g.a_init);
+        fill(e4::P_RHO, 100.0e3 / (g.tp.R[0] * 300.0));
+        fill(e4::P_Y, 1.0);
+        HIP_TRY(hipStreamSynchronize(g_state.stream));
+        HIP_TRY(hipMalloc(&b->gx[face], sizeof(double) * h.size()));
+        HIP_TRY(hipMemcpy(b->gx[face], h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
     }
     b->bc_type[face] = type;
     b->nbr_blk[face] = -1;
@@ -1098,6 +1153,23 @@ int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_ce
         g_state.stage_pin.assign(g_state.blocks.size(), nullptr);
         for (size_t h = 0; h < g_state.blocks.size(); ++h)
             if (g_state.blocks[h].alive) g_state.stage_pin[h] = g_state.blocks[h].P[0];
+        // ghost cells that are stored rather than read through: the mixture
+        // blocks' x_order-1 extrapolation (the rest is done on the fly)
+        for (int n = 0; n < nblk; ++n) {
+            BlockH& b = *get_block(blks[n]);
+            int faces = 0;
+            for (int f = 0; f < 6; ++f)
+                if (b.gx[f] && b.bc_type[f] == E4HIP_BC_EXTRAPOLATE_1) faces |= 1 << f;
+            if (!faces) continue;
+            e4::TPArgs A;
+            rc = tp_args_for(b, stage, dt, nullptr, A);
+            if (rc) return rc;
+            if (g_state.cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT || !tp_mixture_form(g_state.gases[b.gm].tp))
+                e4::launch_tp_extrap1_exact(A, faces, g_state.stream);
+            else
+                e4::launch_tp_extrap1_fast64(A, faces, g_state.stream);
+            HIP_TRY(hipGetLastError());
+        }
         // adaptive_hanel_ausmdv turns the shock detector on (globalconfig.d:1942-1954);
         // it runs after the ghost-cell fill and before the fluxes (simcore.d:2352-2358)
         if (g_state.cfg.flux_calculator == E4HIP_FLUX_ADAPTIVE_HANEL_AUSMDV) {

@@ -428,4 +428,44 @@ __device__ __forceinline__ Flux5T<typename M::R> flux_calc(const FaceStateT<type
     } else return ausmdv<M>(L, R);
 }
 
+// compute_flux_at_left_wall (left) / compute_flux_at_right_wall with the wall
+// at rest: rho, p, a, vn the gas state beside the wall (vn along the face
+// normal), gam its Cp/Cv.  Returns the wall pressure; the flux is that
+// pressure in the normal momentum and zero elsewhere.  FP64 in every mode.
+__device__ inline double wall_pressure(double rho, double p, double a, double vn, double gam, bool left)
+{
+    const double vstar = 0.0;
+    double tmp;
+    if (left) {
+        const double Ubar = vn - 2.0 * a / (gam - 1.0);
+        tmp = (vstar - Ubar) * (gam - 1.0) / (2.0 * sqrt(gam)) * sqrt(rho / pow(p, 1.0 / gam));
+    } else {
+        const double Ubar = vn + 2.0 * a / (gam - 1.0);
+        tmp = (Ubar - vstar) * (gam - 1.0) / (2.0 * sqrt(gam)) * sqrt(rho / pow(p, 1.0 / gam));
+    }
+    const double ptiny = 0.1;   // FlowStateLimits.min_pressure (fvcore.d:26)
+    double pstar = (tmp > 0.0) ? pow(tmp, 2.0 * gam / (gam - 1.0)) : ptiny;
+    if (pstar > 1.1 * p) {
+        // the wave is a shock: secant steps on the shock relation, at most 10
+        auto f = [&](double ps) {
+            const double xi = ps / p;
+            const double M1sq = 1.0 + (gam + 1.0) / 2.0 / gam * (xi - 1.0);
+            const double u1 = sqrt(M1sq) * a;
+            const double u2 = u1 * ((gam - 1.0) * M1sq + 2.0) / ((gam + 1.0) * M1sq);
+            return left ? vstar - u1 + u2 - vn : vstar + u1 - u2 - vn;
+        };
+        int count = 0;
+        double incr;
+        do {
+            const double f0 = f(pstar);
+            const double dp = 0.001 * pstar;
+            const double f1 = f(pstar + dp);
+            incr = -f0 * dp / (f1 - f0);
+            pstar += incr;
+            count += 1;
+        } while (fabs(incr) / pstar > 0.01 && count < 10);
+    }
+    return fmin(pstar, p * 10.0);
+}
+
 } // namespace e4

@@ -170,6 +170,17 @@ struct TPArgs {
     unsigned char* Snew;
     double shear_tol, compression_tol;
     int always_hanel;
+    // faces whose flux the face kernels do not compute as the reference does
+    // (next to a wall without ghost data: the wall face and the one after it;
+    // a simple-outflow-flux face), recomputed by the boundary-face kernel:
+    // fix_off[6] faces, slab by slab in the order (d, side) = (0,-), (0,+),
+    // (1,-), (1,+), (2,-), (2,+); fix_layers[slab] face layers each
+    int fix_layers[6];
+    int fix_off[7];
+    int noghost;                      // bit f: face f has no ghost-cell data
+    // x_order-1 extrapolated ghost cells of face f, [8+nsp][2 * face cells]
+    // (layer-major, in-face index a + b * na), filled at the start of a stage
+    double* gx[6];
 };
 
 // Launchers, one per arithmetic mode; each lives in its own translation unit
@@ -194,6 +205,9 @@ void launch_shock_detect_fast64(const StageArgs& A, int dims, hipStream_t stream
 void launch_tp_shock_detect_exact(const TPArgs& A, hipStream_t stream);
 void launch_tp_shock_detect_fast64(const TPArgs& A, hipStream_t stream);
 void launch_tp_decode_exact(const TPArgs& A, const double* U, const double* Tguess, hipStream_t stream);
+// x_order-1 ghost cells of the mixture block's faces (face mask: bit f)
+void launch_tp_extrap1_exact(const TPArgs& A, int faces, hipStream_t stream);
+void launch_tp_extrap1_fast64(const TPArgs& A, int faces, hipStream_t stream);
 void launch_tp_decode_fast64(const TPArgs& A, const double* U, const double* Tguess, hipStream_t stream);
 
 void launch_mlp_guess(const GuessModel* G, const double* x, int n, double* T, hipStream_t stream);

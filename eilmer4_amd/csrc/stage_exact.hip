@@ -35,6 +35,11 @@ void launch_tp_shock_detect_exact(const TPArgs& A, hipStream_t stream)
     launch_tp_shock_detect_m<ModeExact>(A, stream);
 }
 
+void launch_tp_extrap1_exact(const TPArgs& A, int faces, hipStream_t stream)
+{
+    launch_tp_extrap1_m<ModeExact>(A, faces, stream);
+}
+
 void launch_tp_decode_exact(const TPArgs& A, const double* U, const double* Tguess, hipStream_t stream)
 {
     const long nc = (long)A.ni * A.nj * ((A.dims == 3) ? A.nk : 1);

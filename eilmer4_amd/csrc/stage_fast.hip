@@ -55,6 +55,11 @@ void launch_tp_shock_detect_fast64(const TPArgs& A, hipStream_t stream)
     launch_tp_shock_detect_m<ModeFast64>(A, stream);
 }
 
+void launch_tp_extrap1_fast64(const TPArgs& A, int faces, hipStream_t stream)
+{
+    launch_tp_extrap1_m<ModeFast64>(A, faces, stream);
+}
+
 void launch_tp_decode_fast64(const TPArgs& A, const double* U, const double* Tguess, hipStream_t stream)
 {
     const long nc = (long)A.ni * A.nj * ((A.dims == 3) ? A.nk : 1);

@@ -109,14 +109,24 @@ __device__ __forceinline__ TPRef tp_ref(const TPArgs& A, int i, int j, int k)
         const long c = ((long)sc[2] * nb.n[1] + sc[1]) * nb.n[0] + sc[0];
         return TPRef{A.nbrP[face] + c, nb.qs, -1, A.nbrS[face] + c};
     }
+    if (type == BC_EXTRAPOLATE_1) {
+        // the face's x_order-1 ghost cells, filled at the start of the stage;
+        // their shock indicator is 0 (the extrapolation does not write it)
+        const int na = (dir == 0) ? A.nj : A.ni;
+        const int fa = (dir == 0) ? j : i, fb = (dir == 2) ? j : k;
+        const long nfc = (long)na * ((dir == 2) ? A.nj : A.nk);
+        return TPRef{A.gx[face] + (layer - 1) * nfc + fa + (long)fb * na, 2 * nfc, -1, nullptr};
+    }
     int src;
-    if (type == BC_EXTRAPOLATE) src = plus ? n - 1 : 0;
+    if (type == BC_EXTRAPOLATE || type == BC_OUTFLOW_SIMPLE_FLUX) src = plus ? n - 1 : 0;
     else if (type == BC_PERIODIC) src = plus ? layer - 1 : n - layer;
+    // slip-wall mirror; a wall without ghost data gets the same values, read
+    // only by faces the boundary-face kernel recomputes
     else src = plus ? n - layer : layer - 1;
     int si = i, sj = j, sk = k;
     if (dir == 0) si = src; else if (dir == 1) sj = src; else sk = src;
     const long c = ((long)sk * A.nj + sj) * A.ni + si;
-    return TPRef{A.Pin + c, A.qs, (type == BC_SLIP_WALL) ? dir : -1, A.Sprev + c};
+    return TPRef{A.Pin + c, A.qs, (type == BC_SLIP_WALL || type == BC_SLIP_WALL_1) ? dir : -1, A.Sprev + c};
 }
 
 __device__ __forceinline__ double tp_val(const TPRef& r, int field) { return r.base[field * r.stride]; }
@@ -656,12 +666,280 @@ __global__ __launch_bounds__(128) void tp_shock_detect_kernel(TPArgs A)
     };
     bool S = false;
     const int ndir = (A.dims == 3) ? 3 : 2;
+    const int FM[3] = {FACE_WEST, FACE_SOUTH, FACE_BOTTOM}, FP[3] = {FACE_EAST, FACE_NORTH, FACE_TOP};
     for (int d = 0; d < ndir; ++d) {
         const int di = (d == 0), dj = (d == 1), dk = (d == 2);
-        S = S || face(tp_ref(A, i - di, j - dj, k - dk), own, d);
-        S = S || face(own, tp_ref(A, i + di, j + dj, k + dk), d);
+        const int pos = (d == 0) ? i : (d == 1 ? j : k), n = (d == 0) ? A.ni : (d == 1 ? A.nj : A.nk);
+        // a boundary face without ghost data has the cell on one side only
+        // (shockdetectors.d:39-54): its velocity towards the wall
+        const double un = tp_vel(own, d), a0 = tp_val(own, P_A);
+        if (pos == 0 && (A.noghost & (1 << FM[d]))) S = S || dv<MD>(un, a0) < tol;
+        else S = S || face(tp_ref(A, i - di, j - dj, k - dk), own, d);
+        if (pos == n - 1 && (A.noghost & (1 << FP[d]))) S = S || dv<MD>(-un, a0) < tol;
+        else S = S || face(own, tp_ref(A, i + di, j + dj, k + dk), d);
     }
     A.Snew[c] = S ? 1 : 0;
+}
+
+// gamma = Cp/Cv of a mixture state (gas_model.d:162; therm_perf_gas.d:366-380)
+template <class M, int NSP>
+__device__ __forceinline__ double tp_gamma(const TPGas& g, const double* Y, double T)
+{
+    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    if constexpr (M::kFast) {
+        const double R = tp_Rmix<NSP>(g, Y);
+        double u, cv;
+        tp_mix_u_cv<NSP>(g, Y, R, T, &u, &cv);
+        return dv<MD>(cv + R, cv);
+    } else {
+        double Cp = 0.0, Cv = 0.0;
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) Cp += Y[s] * cea_Cp<M>(g.curve[s], T);
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) Cv += Y[s] * (cea_Cp<M>(g.curve[s], T) - g.R[s]);
+        return dv<MD>(Cp, Cv);
+    }
+}
+
+// The faces of a mixture block whose flux is not the two-sided reconstruction
+// of tp_face_kernel (sfluidblock.d:2189-2345): next to a wall without ghost
+// data (WallBC_WithSlip1) the wall face takes the wall flux of
+// fluxcalc.d:130-298 from the wall cell's state (or, unclipped, its linear
+// extrapolation, onedinterp.d:820-1037) and the face after it the one-sided
+// reconstruction l1r2 / l2r1 (onedinterp.d:165-818); a simple-outflow-flux
+// face takes the interior cell's flux (boundary_flux_effect.d:367-546).
+// One thread per such face; it overwrites what the face kernel stored.
+template <class M, int NSP, int FLUX>
+__global__ __launch_bounds__(64) void tp_boundary_face_kernel(TPArgs A)
+{
+    using MA = typename TPArith<M>::type;
+    constexpr bool kPolish = TPArith<M>::polish;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.fix_off[6]) return;
+    int slab = 0;
+    while (slab < 5 && t >= A.fix_off[slab + 1]) ++slab;
+    const int d = slab >> 1, side = slab & 1;
+    const int n3[3] = {A.ni, A.nj, (A.dims == 3) ? A.nk : 1};
+    const int nl = A.fix_layers[slab];
+    const int da = (d + 1) % 3, db = (d + 2) % 3;
+    const int local = t - A.fix_off[slab];
+    const int layer = local % nl, rest = local / nl;
+    int c[3];
+    const int n = n3[d];
+    c[d] = side ? n - layer : layer;            // the face's plus-side cell index along d
+    c[da] = rest % n3[da];
+    c[db] = rest / n3[da];
+    const int i = c[0], j = c[1], k = c[2];
+    const int FM[3] = {FACE_WEST, FACE_SOUTH, FACE_BOTTOM}, FP[3] = {FACE_EAST, FACE_NORTH, FACE_TOP};
+    const int tm = A.bc->type[FM[d]], tpl = A.bc->type[FP[d]];
+    const int pos = c[d];
+    const bool d3 = (A.dims == 3);
+    const int pn = d, pt1 = (d == 0) ? 1 : (d == 1) ? (d3 ? 2 : 0) : 0;
+    const int pt2 = (d == 0) ? 2 : (d == 1) ? (d3 ? 0 : 2) : 1;
+    const int di = (d == 0), dj = (d == 1), dk = (d == 2);
+    TPRef r[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) r[m] = tp_ref(A, i + (m - 2) * di, j + (m - 2) * dj, k + (m - 2) * dk);
+    const TPGas& g = *A.gas;
+    double F[5 + NSP];
+#pragma unroll
+    for (int q = 0; q < 5 + NSP; ++q) F[q] = 0.0;
+    int fe = 0, fb = 0;
+    const bool ho = A.high_order, lim = A.apply_limiter, clip = A.extrema_clipping;
+    if ((pos == 0 && tm == BC_OUTFLOW_SIMPLE_FLUX) || (pos == n && tpl == BC_OUTFLOW_SIMPLE_FLUX)) {
+        const bool plus = (pos == n);
+        const TPRef& cr = plus ? r[1] : r[2];
+        const double rho = tp_val(cr, P_RHO), p = tp_val(cr, P_P), u = tp_val(cr, P_U);
+        const double vx = tp_vel(cr, 0), vy = tp_vel(cr, 1), vz = tp_vel(cr, 2);
+        const double vn = tp_vel(cr, pn);
+        const double mass_flux = rho * vn;
+        if ((plus ? 1.0 : -1.0) * mass_flux > 0.0) {
+            F[0] = mass_flux;
+            F[1] = p + vn * mass_flux;
+            F[2] = tp_vel(cr, pt1) * mass_flux;
+            F[3] = tp_vel(cr, pt2) * mass_flux;
+            const double utot = u + 0.5 * (vx * vx + vy * vy + vz * vz);
+            F[4] = mass_flux * utot + p * vn;
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) F[5 + s] = mass_flux * tp_val(cr, P_Y + s);
+        } else {
+            F[1] = p;
+        }
+    } else if ((pos == 0 && tm == BC_SLIP_WALL_1) || (pos == n && tpl == BC_SLIP_WALL_1)) {
+        const bool left = (pos == 0);
+        const TPRef& c0 = left ? r[2] : r[1];
+        const TPRef& c1 = left ? r[3] : r[0];
+        TPCell<NSP> S = tp_cell<NSP>(c0, pn, pt1, pt2);
+        if (ho && !clip) {
+            const double w0 = A.bc->ext_w[d][0], w1 = A.bc->ext_w[d][1];
+            TPState<NSP> Q;
+            double v[3];
+            v[0] = tp_vel(c0, pn) * w0 + tp_vel(c1, pn) * w1;
+            v[1] = tp_vel(c0, pt1) * w0 + tp_vel(c1, pt1) * w1;
+            v[2] = tp_vel(c0, pt2) * w0 + tp_vel(c1, pt2) * w1;
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) Q.Y[s] = tp_val(c0, P_Y + s) * w0 + tp_val(c1, P_Y + s) * w1;
+            if (scale_mass_fractions<MA, NSP>(Q.Y) != 0) {
+#pragma unroll
+                for (int s = 0; s < NSP; ++s) Q.Y[s] = tp_val(c0, P_Y + s);
+            }
+            Q.rho = tp_val(c0, P_RHO) * w0 + tp_val(c1, P_RHO) * w1;
+            Q.u = tp_val(c0, P_U) * w0 + tp_val(c1, P_U) * w1;
+            Q.T = S.T;
+            if (tp_face_eos<MA, NSP, kPolish>(g, Q, &fe, &fb) == 0) {
+                S.rho = Q.rho; S.u = Q.u; S.T = Q.T; S.p = Q.p;
+                S.v[0] = v[0]; S.v[1] = v[1]; S.v[2] = v[2];
+#pragma unroll
+                for (int s = 0; s < NSP; ++s) S.Y[s] = Q.Y[s];
+            }
+        }
+        F[1] = wall_pressure(S.rho, S.p, S.a, S.v[0], tp_gamma<MA, NSP>(g, S.Y, S.T), left);
+    } else {
+        // l1r2 (kind 1: the face after a wall on the minus side) or l2r1 (kind 2)
+        const int kind = (pos == 1 && tm == BC_SLIP_WALL_1) ? 1 : 2;
+        TPCell<NSP> Lf, Rf;
+        bool okL = false, okR = false;
+        if (ho) {
+            const L2R2W& w = A.w[d];
+            auto lin = [&](double q0, double q1) {
+                double q = q0 * 0.5 + q1 * 0.5;
+                return clip ? clip_limits<MA>(q, q0, q1) : q;
+            };
+            auto recon = [&](double q0, double q1, double q2, double q3, double& qL, double& qR) {
+                double unused;
+                if (kind == 1) {
+                    interp_l2r2_scalar<MA>(w, lim, clip, q1, q1, q2, q3, unused, qR);
+                    const double del = (q2 - q1) * w.two_over_lenR0_plus_lenL0;
+                    const double delRplus = (q3 - q2) * w.two_over_lenR1_plus_lenR0;
+                    qL = (lim && delRplus * del < 0.0) ? q1 : lin(q1, q2);
+                } else {
+                    interp_l2r2_scalar<MA>(w, lim, clip, q0, q1, q2, q2, qL, unused);
+                    const double delLminus = (q1 - q0) * w.two_over_lenL0_plus_lenL1;
+                    const double del = (q2 - q1) * w.two_over_lenR0_plus_lenL0;
+                    qR = (lim && delLminus * del < 0.0) ? q2 : lin(q1, q2);
+                }
+            };
+            const int comp[3] = {pn, pt1, pt2};
+            for (int dd = 0; dd < 3; ++dd)
+                recon(tp_vel(r[0], comp[dd]), tp_vel(r[1], comp[dd]), tp_vel(r[2], comp[dd]), tp_vel(r[3], comp[dd]),
+                      Lf.v[dd], Rf.v[dd]);
+            for (int s = 0; s < NSP; ++s)
+                recon(tp_val(r[0], P_Y + s), tp_val(r[1], P_Y + s), tp_val(r[2], P_Y + s), tp_val(r[3], P_Y + s),
+                      Lf.Y[s], Rf.Y[s]);
+            if (scale_mass_fractions<MA, NSP>(Lf.Y) != 0)
+                for (int s = 0; s < NSP; ++s) Lf.Y[s] = tp_val(r[1], P_Y + s);
+            if (scale_mass_fractions<MA, NSP>(Rf.Y) != 0)
+                for (int s = 0; s < NSP; ++s) Rf.Y[s] = tp_val(r[2], P_Y + s);
+            recon(tp_val(r[0], P_RHO), tp_val(r[1], P_RHO), tp_val(r[2], P_RHO), tp_val(r[3], P_RHO), Lf.rho, Rf.rho);
+            recon(tp_val(r[0], P_U), tp_val(r[1], P_U), tp_val(r[2], P_U), tp_val(r[3], P_U), Lf.u, Rf.u);
+            for (int sd = 0; sd < 2; ++sd) {
+                const TPRef rc = sd ? r[2] : r[1];
+                TPState<NSP> Q;
+                Q.rho = sd ? Rf.rho : Lf.rho;
+                Q.u = sd ? Rf.u : Lf.u;
+                Q.T = tp_val(rc, P_T);
+                for (int s = 0; s < NSP; ++s) Q.Y[s] = sd ? Rf.Y[s] : Lf.Y[s];
+                if (tp_face_eos<MA, NSP, kPolish>(g, Q, &fe, &fb) == 0) {
+                    const double a = tp_val(rc, P_A);
+                    if (sd == 0) { Lf.T = Q.T; Lf.u = Q.u; Lf.p = Q.p; Lf.a = a; okL = true; }
+                    else { Rf.T = Q.T; Rf.u = Q.u; Rf.p = Q.p; Rf.a = a; okR = true; }
+                }
+            }
+        }
+        if (!okL) Lf = tp_cell<NSP>(r[1], pn, pt1, pt2);
+        if (!okR) Rf = tp_cell<NSP>(r[2], pn, pt1, pt2);
+        if constexpr (FLUX == FLUX_HLLC) {
+            tp_hllc<MA, NSP>(Lf, Rf, F);
+        } else if constexpr (FLUX == FLUX_SHOCK) {
+            const bool shock = (r[1].s && *r[1].s) || (r[2].s && *r[2].s);
+            const double sound_speed = 0.5 * (Lf.a + Rf.a);
+            const double shear_y = dv<MA>(fabs(Lf.v[1] - Rf.v[1]), sound_speed);
+            const double shear_z = dv<MA>(fabs(Lf.v[2] - Rf.v[2]), sound_speed);
+            const bool shear_is_small = fmax(shear_y, shear_z) <= A.shear_tol;
+            if (A.always_hanel || (shock && shear_is_small)) tp_hanel<MA, NSP>(Lf, Rf, F);
+            else tp_ausmdv<MA, NSP>(Lf, Rf, F);
+        } else {
+            tp_ausmdv<MA, NSP>(Lf, Rf, F);
+        }
+    }
+    const int nfi = A.ni + (d == 0), nfj = A.nj + (d == 1);
+    const long f = ((long)k * nfj + j) * nfi + i;
+    double* dst = A.F[d];
+    const long fq = A.fqs[d];
+    dst[f] = F[0];
+    dst[(1 + pn) * fq + f] = F[1];
+    dst[(1 + pt1) * fq + f] = F[2];
+    dst[(1 + pt2) * fq + f] = F[3];
+    dst[4 * fq + f] = F[4];
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) dst[(5 + s) * fq + f] = F[5 + s];
+}
+
+// x_order-1 ghost cells of one face of a mixture block (extrapolate_copy.d:
+// 102-186): layer 1 linear from the boundary cell c1 and the cell behind it,
+// layer 2 from layer 1 and c1, in rho, u, velocity and mass fractions
+// (scaled); T by the EOS from the ghost's previous T, then p.  The sound
+// speed is not written (the buffer holds the constructed cell's).
+template <class M>
+__global__ __launch_bounds__(128) void tp_extrap1_kernel(TPArgs A, int face)
+{
+    using MA = typename TPArith<M>::type;
+    constexpr int NSP = 5;
+    const int dir = (face == FACE_EAST || face == FACE_WEST) ? 0 : ((face == FACE_NORTH || face == FACE_SOUTH) ? 1 : 2);
+    const bool plus = (face == FACE_EAST || face == FACE_NORTH || face == FACE_TOP);
+    const int na = (dir == 0) ? A.nj : A.ni;
+    const int nb = (dir == 2) ? A.nj : A.nk;
+    const long nfc = (long)na * nb;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nfc) return;
+    const int fa = (int)(t % na), fb = (int)(t / na);
+    const int n = (dir == 0) ? A.ni : ((dir == 1) ? A.nj : A.nk);
+    auto cell = [&](int p) {
+        int ci, cj, ck;
+        if (dir == 0) { ci = p; cj = fa; ck = fb; }
+        else if (dir == 1) { ci = fa; cj = p; ck = fb; }
+        else { ci = fa; cj = fb; ck = p; }
+        return A.Pin + ((long)ck * A.nj + cj) * A.ni + ci;
+    };
+    double* G = A.gx[face];
+    const long gs = 2 * nfc;
+    const int e = plus ? n - 1 : 0, in = plus ? -1 : 1;
+    const double* c1 = cell(e);
+    for (int layer = 1; layer <= 2; ++layer) {
+        const double* p1 = (layer == 1) ? c1 : G + t;
+        const long s1 = (layer == 1) ? A.qs : gs;
+        const double* p2 = (layer == 1) ? cell(e + in) : c1;
+        const long s2 = (layer == 1) ? A.qs : A.qs;
+        double* dst = G + (layer - 1) * nfc + t;
+        TPState<NSP> Q;
+        Q.rho = 2.0 * p1[P_RHO * s1] - p2[P_RHO * s2];
+        Q.u = 2.0 * p1[P_U * s1] - p2[P_U * s2];
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) Q.Y[s] = 2.0 * p1[(P_Y + s) * s1] - p2[(P_Y + s) * s2];
+        scale_mass_fractions<MA, NSP>(Q.Y);
+        Q.T = dst[P_T * gs];
+        Q.p = dst[P_P * gs];
+        int fe = 0;
+        tp_update_thermo_from_rhou<MA, NSP>(*A.gas, Q, &fe);
+        const double vx = 2.0 * p1[P_VX * s1] - p2[P_VX * s2];
+        const double vy = 2.0 * p1[P_VY * s1] - p2[P_VY * s2];
+        const double vz = 2.0 * p1[P_VZ * s1] - p2[P_VZ * s2];
+        dst[P_RHO * gs] = Q.rho; dst[P_U * gs] = Q.u; dst[P_T * gs] = Q.T; dst[P_P * gs] = Q.p;
+        dst[P_VX * gs] = vx; dst[P_VY * gs] = vy; dst[P_VZ * gs] = vz;
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) dst[(P_Y + s) * gs] = Q.Y[s];
+    }
+}
+
+template <class M>
+void launch_tp_extrap1_m(const TPArgs& A, int faces, hipStream_t stream)
+{
+    for (int f = 0; f < 6; ++f) {
+        if (!(faces & (1 << f))) continue;
+        const int dir = (f == FACE_EAST || f == FACE_WEST) ? 0 : ((f == FACE_NORTH || f == FACE_SOUTH) ? 1 : 2);
+        const long nfc = (long)((dir == 0) ? A.nj : A.ni) * ((dir == 2) ? A.nj : A.nk);
+        hipLaunchKernelGGL((tp_extrap1_kernel<M>), dim3((unsigned)((nfc + 127) / 128)), dim3(128), 0, stream, A, f);
+    }
 }
 
 // Launch of one stage of a mixture block: the face kernels of every
@@ -685,6 +963,12 @@ void launch_tp_stage_m(const TPArgs& A, hipStream_t stream, int flux)
     if (flux == FLUX_HLLC) launch_tp_faces<M, FLUX_HLLC>(A, stream);
     else if (flux == FLUX_SHOCK) launch_tp_faces<M, FLUX_SHOCK>(A, stream);
     else launch_tp_faces<M, FLUX_AUSMDV>(A, stream);
+    if (A.fix_off[6] > 0) {
+        const dim3 g((unsigned)((A.fix_off[6] + 63) / 64)), b(64);
+        if (flux == FLUX_HLLC) hipLaunchKernelGGL((tp_boundary_face_kernel<M, 5, FLUX_HLLC>), g, b, 0, stream, A);
+        else if (flux == FLUX_SHOCK) hipLaunchKernelGGL((tp_boundary_face_kernel<M, 5, FLUX_SHOCK>), g, b, 0, stream, A);
+        else hipLaunchKernelGGL((tp_boundary_face_kernel<M, 5, FLUX_AUSMDV>), g, b, 0, stream, A);
+    }
     const long nc = (long)A.ni * A.nj * ((A.dims == 3) ? A.nk : 1);
     const dim3 grid((unsigned)((nc + 127) / 128)), b(128);
     if (A.guess) hipLaunchKernelGGL((tp_update_kernel<M, 5, true>), grid, b, 0, stream, A);

@@ -159,3 +159,79 @@ def test_boundaries_batched(gpu):
         print("  block %d rel err %s" % (s, err), flush=True)
         assert err.max() <= 1.0e-13
         dbs[s].destroy()
+
+
+# ------------------------------------------------------------------ mixtures
+
+@pytest.mark.parametrize("precision", ["fp64_exact", "fp64", "mixed"])
+@pytest.mark.parametrize("flux", ["ausmdv", "adaptive_hanel_ausmdv"])
+def test_boundaries_thermally_perfect(gpu, flux, precision):
+    """5-species air with every new flavour on one block: walls without ghost
+    data on three faces (the boundary-face kernel's wall flux with gamma =
+    Cp/Cv of the mixture state, and the one-sided faces after them), simple
+    outflow flux, and x_order-1 ghosts (stored per face, T by the EOS from the
+    previous ghost T).  Bar: 1e-10 (the logarithm differs between the device
+    and host libraries); mixed: density 1e-8."""
+    from test_gpu_tp import setup_pair
+    bcs = {e4.Face.west: (BCType.slip_wall_1, None), e4.Face.east: (BCType.outflow_simple_flux, None),
+           e4.Face.north: (BCType.extrapolate_1, None), e4.Face.south: (BCType.slip_wall_1, None),
+           e4.Face.top: (BCType.extrapolate_1, None), e4.Face.bottom: (BCType.slip_wall_1, None)}
+    print("case tp new bcs n=10 flux=%s precision=%s steps=3" % (flux, precision), flush=True)
+    ob, db, sim = setup_pair(10, flux=flux, precision=precision, bcs=bcs, seed=77)
+    dt, _, _ = ob.determine_dt(1.0e-3, 0.5, False)
+    for _ in range(3):
+        assert ob.step(dt)[0] == 0
+        sim.state.dt_global = dt
+        sim.gasdynamic_explicit_increment_with_fixed_grid()
+    err = rel_err(db.download(), ob.get_U())
+    print("  U rel err per quantity", err, flush=True)
+    assert np.all(np.isfinite(err))
+    if precision == "mixed":
+        assert err[0] <= 1.0e-8
+    else:
+        assert err.max() <= 1.0e-10
+    db.destroy()
+
+
+def test_boundaries_thermally_perfect_no_clipping(gpu):
+    """Without clipping the wall faces' states are extrapolated and go
+    through the mixture EOS before the wall flux.  A mild field (T 1500-1800 K,
+    uniform composition): on the 300-8000 K workload the unclipped wall
+    extrapolation drives the reference's own algorithm to invalid cells."""
+    from test_gpu_tp import TOL
+    gas = e4.ThermallyPerfectGas.from_json()
+    n = 10
+    rng = np.random.default_rng(5)
+    shape = (n, n, n)
+    Y = np.array([0.7, 0.2, 0.05, 0.03, 0.02])
+    Yf = np.broadcast_to(Y[:, None], (5, n ** 3)).copy()
+    T = 1500.0 * (1.0 + 0.2 * rng.random(shape))
+    p = 1.0e5 * (1.0 + 0.3 * rng.random(shape))
+    rho = (p.ravel() / (gas.Rmix(Yf) * T.ravel())).reshape(shape)
+    fields = dict(p=p, T=T, rho=rho, vx=200.0 * (rng.random(shape) - 0.5), vy=200.0 * (rng.random(shape) - 0.5),
+                  vz=200.0 * (rng.random(shape) - 0.5), Y=Yf)
+    from eilmer4_amd import workloads
+    U0, Tg = workloads.initial_U_tp(gas, fields)
+    bcs = {e4.Face.west: BCType.slip_wall_1, e4.Face.east: BCType.slip_wall_1,
+           e4.Face.north: BCType.outflow_simple_flux, e4.Face.south: BCType.extrapolate_1,
+           e4.Face.top: BCType.periodic, e4.Face.bottom: BCType.periodic}
+    ob = O.Block(O.Gas.from_json("therm-perf-5-species-air.json"), 3, n, n, n, 0.1, 0.1, 0.1)
+    ob.set_config(flux_calc=O.AUSMDV, extrema_clipping=0)
+    db = e4.SFluidBlock(gas, 3, n, n, n, 0.1, 0.1, 0.1)
+    for face, kind in bcs.items():
+        ob.set_bc(int(face), int(kind), None)
+        db.set_bc(face, kind, None)
+    ob.set_U(U0, Tg)
+    sim = e4.Simulation([db], e4.GlobalConfig(flux_calculator=e4.FluxCalculator.ausmdv, extrema_clipping=False,
+                                              precision_mode=e4.PrecisionMode.fp64_exact))
+    db.upload(U0, Tg)
+    print("case tp new bcs clipping off n=10", flush=True)
+    dt, _, _ = ob.determine_dt(1.0e-3, 0.5, False)
+    for _ in range(3):
+        assert ob.step(dt)[0] == 0
+        sim.state.dt_global = dt
+        sim.gasdynamic_explicit_increment_with_fixed_grid()
+    err = rel_err(db.download(), ob.get_U())
+    print("  U rel err per quantity", err, flush=True)
+    assert err.max() <= TOL
+    db.destroy()
