@@ -60,7 +60,10 @@ struct BlockH {
     double* F[3] = {nullptr, nullptr, nullptr};
     long fqs[3] = {0, 0, 0};
     double bc_fs_tp[6][8 + e4::TP_NSP_MAX] = {};
-    double* gx[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   // mixture x_order-1 ghosts
+    // stored ghost cells per face (x_order-1 extrapolation of a mixture, static
+    // profiles): [8+nsp][2 * face cells]; gS the profile's shock indicators
+    double* gx[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    unsigned char* gS[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     // shock indicators per cell (adaptive_hanel_ausmdv): S[0] this stage's
     // detection, S[1] the previous one
     unsigned char* S[2] = {nullptr, nullptr};
@@ -321,6 +324,7 @@ int stage_args_for(const BlockH& b, int stage, double dt, unsigned int* counters
     A.dt = dt; A.g1 = g1; A.g2 = g2; A.g3 = g3;
     A.c1 = dt * g1;
     boundary_slabs(b, A.fix_lo, A.fix_hi, A.fix_off, A.noghost);
+    for (int f = 0; f < 6; ++f) { A.gx[f] = b.gx[f]; A.gS[f] = b.gS[f]; }
     const int src = shock_fields(b, A);
     if (src) return src;
     const int nk = A.nk;
@@ -555,7 +559,7 @@ int tp_args_for(const BlockH& b, int stage, double dt, unsigned int* counters, e
         }
         A.fix_off[6] = total;
     }
-    for (int f = 0; f < 6; ++f) A.gx[f] = b.gx[f];
+    for (int f = 0; f < 6; ++f) { A.gx[f] = b.gx[f]; A.gS[f] = b.gS[f]; }
     return shock_fields(b, A);
 }
 
@@ -646,6 +650,7 @@ int free_block_buffers(BlockH& b)
     for (auto& p : b.F) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     for (auto& p : b.S) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     for (auto& p : b.gx) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
+    for (auto& p : b.gS) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     if (b.bc_dev) { HIP_TRY(hipFree(b.bc_dev)); b.bc_dev = nullptr; }
     return E4HIP_OK;
 }
@@ -922,8 +927,39 @@ int e4hip_block_set_bc(int blk, int face, int type, const double* fs)
     BlockH* b = get_block(blk);
     if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
     if (face < 0 || face > 5 || (b->dims == 2 && face > 3)) return fail(E4HIP_ERR_ARG, "bad face %d", face);
-    if (type < E4HIP_BC_SLIP_WALL || type > E4HIP_BC_EXTRAPOLATE_1 || type == E4HIP_BC_CONNECTED)
+    if (type < E4HIP_BC_SLIP_WALL || type > E4HIP_BC_INFLOW_PROFILE || type == E4HIP_BC_CONNECTED)
         return fail(E4HIP_ERR_ARG, "bad bc type %d", type);
+    const int dir = (face == E4HIP_FACE_EAST || face == E4HIP_FACE_WEST) ? 0
+                  : (face == E4HIP_FACE_NORTH || face == E4HIP_FACE_SOUTH) ? 1 : 2;
+    const long nfc = (long)((dir == 0) ? b->nj : b->ni) * ((dir == 2) ? b->nj : b->nk);
+    const int np = 8 + b->nsp;
+    auto ghost_buffer = [&](std::vector<double>& h) -> int {
+        HIP_TRY(hipStreamSynchronize(g_state.stream));
+        if (!b->gx[face]) HIP_TRY(hipMalloc(&b->gx[face], sizeof(double) * (size_t)np * 2 * nfc));
+        HIP_TRY(hipMemcpy(b->gx[face], h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+        return E4HIP_OK;
+    };
+    if (type == E4HIP_BC_INFLOW_PROFILE) {
+        // fs: [2][face cells][9 + nsp] flow states (rho, p, T, u, a, vx, vy, vz, S,
+        // Y...), layer-major, in-face index a + b * na
+        if (!fs) return fail(E4HIP_ERR_ARG, "a static profile needs the ghost flow states");
+        const int w = 9 + b->nsp;
+        std::vector<double> h((size_t)np * 2 * nfc, 0.0);
+        std::vector<unsigned char> S(2 * nfc, 0);
+        static const int src_of[8] = {0, 5, 6, 7, 3, 2, 1, 4};   // P_* field <- fs column
+        for (long g = 0; g < 2 * nfc; ++g) {
+            const double* st = fs + g * w;
+            if (!(st[0] > 0) || !(st[4] > 0) || !(b->is_tp() || st[3] > 0))
+                return fail(E4HIP_ERR_ARG, "invalid profile state at ghost %ld", g);
+            for (int q = 0; q < 8; ++q) h[(size_t)q * 2 * nfc + g] = st[src_of[q]];
+            for (int s = 0; s < b->nsp; ++s) h[(size_t)(e4::P_Y + s) * 2 * nfc + g] = b->is_tp() ? st[9 + s] : 1.0;
+            S[g] = st[8] != 0.0;
+        }
+        int rc = ghost_buffer(h);
+        if (rc) return rc;
+        if (!b->gS[face]) HIP_TRY(hipMalloc(&b->gS[face], 2 * nfc));
+        HIP_TRY(hipMemcpy(b->gS[face], S.data(), 2 * nfc, hipMemcpyHostToDevice));
+    }
     if (type == E4HIP_BC_SLIP_WALL_1) {
         // one-sided reconstruction reaches three cells in from the wall
         const int n = (face == E4HIP_FACE_EAST || face == E4HIP_FACE_WEST) ? b->ni
@@ -946,13 +982,9 @@ int e4hip_block_set_bc(int blk, int face, int type, const double* fs)
         if (!(P.rho > 0) || !(P.u > 0) || !(P.a > 0)) return fail(E4HIP_ERR_ARG, "invalid inflow state");
         b->bc_fs[face] = P;
     }
-    if (type == E4HIP_BC_EXTRAPOLATE_1 && b->is_tp() && !b->gx[face]) {
+    if (type == E4HIP_BC_EXTRAPOLATE_1 && b->is_tp()) {
         // the face's two ghost layers, holding the constructed cell state
         // (p = 100 kPa, T = 300 K, first species, at rest) until the first fill
-        const int dir = (face == E4HIP_FACE_EAST || face == E4HIP_FACE_WEST) ? 0
-                      : (face == E4HIP_FACE_NORTH || face == E4HIP_FACE_SOUTH) ? 1 : 2;
-        const long nfc = (long)((dir == 0) ? b->nj : b->ni) * ((dir == 2) ? b->nj : b->nk);
-        const int np = 8 + b->nsp;
         std::vector<double> h((size_t)np * 2 * nfc, 0.0);
         const GasModelH& g = g_state.gases[b->gm];
         auto fill = [&](int field, double v) { std::fill(h.begin() + field * 2 * nfc, h.begin() + (field + 1) * 2 * nfc, v); };
@@ -961,9 +993,8 @@ int e4hip_block_set_bc(int blk, int face, int type, const double* fs)
         fill(e4::P_A, g.a_init);
         fill(e4::P_RHO, 100.0e3 / (g.tp.R[0] * 300.0));
         fill(e4::P_Y, 1.0);
-        HIP_TRY(hipStreamSynchronize(g_state.stream));
-        HIP_TRY(hipMalloc(&b->gx[face], sizeof(double) * h.size()));
-        HIP_TRY(hipMemcpy(b->gx[face], h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+        int rc = ghost_buffer(h);
+        if (rc) return rc;
     }
     b->bc_type[face] = type;
     b->nbr_blk[face] = -1;

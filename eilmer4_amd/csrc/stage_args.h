@@ -11,8 +11,9 @@ namespace e4 {
 // face's flux from the interior cell; 6: WallBC_WithSlip1, no ghost-cell data,
 // one-sided reconstruction and the wall flux; 7: OutFlowBC_SimpleExtrapolate
 // with x_order 1.
+// 8: InFlowBC_StaticProfile, a stored flow state per ghost cell.
 enum { BC_SLIP_WALL = 0, BC_INFLOW = 1, BC_EXTRAPOLATE = 2, BC_PERIODIC = 3, BC_CONNECTED = 4,
-       BC_OUTFLOW_SIMPLE_FLUX = 5, BC_SLIP_WALL_1 = 6, BC_EXTRAPOLATE_1 = 7 };
+       BC_OUTFLOW_SIMPLE_FLUX = 5, BC_SLIP_WALL_1 = 6, BC_EXTRAPOLATE_1 = 7, BC_INFLOW_PROFILE = 8 };
 enum { FACE_NORTH = 0, FACE_EAST = 1, FACE_SOUTH = 2, FACE_WEST = 3, FACE_TOP = 4, FACE_BOTTOM = 5 };
 
 // A face connected to a face of another block (full-face copy): the other
@@ -121,6 +122,11 @@ struct StageArgs {
     int fix_lo[3], fix_hi[3];
     int fix_off[7];
     int noghost;
+    // stored ghost cells of face f (static profile): [8][2 * face cells] in the
+    // P_* field order, layer-major, in-face index a + b * na; gS their shock
+    // indicators
+    const double* gx[6];
+    const unsigned char* gS[6];
 };
 
 // A batched stage launch: one workgroup per (block, tile, k-chunk) over all
@@ -178,9 +184,11 @@ struct TPArgs {
     int fix_layers[6];
     int fix_off[7];
     int noghost;                      // bit f: face f has no ghost-cell data
-    // x_order-1 extrapolated ghost cells of face f, [8+nsp][2 * face cells]
-    // (layer-major, in-face index a + b * na), filled at the start of a stage
+    // stored ghost cells of face f, [8+nsp][2 * face cells] (layer-major,
+    // in-face index a + b * na): x_order-1 extrapolation, filled at the start
+    // of a stage, or a static profile; gS the profile's shock indicators
     double* gx[6];
+    const unsigned char* gS[6];
 };
 
 // Launchers, one per arithmetic mode; each lives in its own translation unit

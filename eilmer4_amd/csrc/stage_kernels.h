@@ -145,6 +145,19 @@ __device__ Prim fetch_ghost(KArgs A, int i, int j, int k)
         return P;
     }
     int si = i, sj = j, sk = k;
+    if (type == BC_INFLOW_PROFILE) {
+        // the stored ghost state (flow_state_copy_from_profile.d), with its
+        // profile point's shock indicator
+        const int na = (dir == 0) ? A->nj : A->ni;
+        const int fa = (dir == 0) ? j : i, fb = (dir == 2) ? j : k;
+        const long nfc = (long)na * ((dir == 2) ? A->nj : A->nk);
+        const long g = (layer - 1) * nfc + fa + (long)fb * na;
+        const double* G = A->gx[face] + g;
+        const long gs = 2 * nfc;
+        Prim P{G[P_RHO * gs], G[P_VX * gs], G[P_VY * gs], G[P_VZ * gs], G[P_U * gs], G[P_A * gs]};
+        if (SH && A->gS[face][g]) P.a = -(P.a);
+        return P;
+    }
     if (type == BC_EXTRAPOLATE_1) {
         // extrapolate_copy.d:102-186: linear in rho, u and velocity from the
         // boundary cell c1 and the one behind it c2 (layer 2 from the first

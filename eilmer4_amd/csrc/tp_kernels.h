@@ -109,13 +109,15 @@ __device__ __forceinline__ TPRef tp_ref(const TPArgs& A, int i, int j, int k)
         const long c = ((long)sc[2] * nb.n[1] + sc[1]) * nb.n[0] + sc[0];
         return TPRef{A.nbrP[face] + c, nb.qs, -1, A.nbrS[face] + c};
     }
-    if (type == BC_EXTRAPOLATE_1) {
-        // the face's x_order-1 ghost cells, filled at the start of the stage;
-        // their shock indicator is 0 (the extrapolation does not write it)
+    if (type == BC_EXTRAPOLATE_1 || type == BC_INFLOW_PROFILE) {
+        // the face's stored ghost cells: x_order-1 ones, filled at the start of
+        // the stage, with shock indicator 0 (the extrapolation does not write
+        // it); a static profile's, with the profile's indicators
         const int na = (dir == 0) ? A.nj : A.ni;
         const int fa = (dir == 0) ? j : i, fb = (dir == 2) ? j : k;
         const long nfc = (long)na * ((dir == 2) ? A.nj : A.nk);
-        return TPRef{A.gx[face] + (layer - 1) * nfc + fa + (long)fb * na, 2 * nfc, -1, nullptr};
+        const long g = (layer - 1) * nfc + fa + (long)fb * na;
+        return TPRef{A.gx[face] + g, 2 * nfc, -1, (type == BC_INFLOW_PROFILE) ? A.gS[face] + g : nullptr};
     }
     int src;
     if (type == BC_EXTRAPOLATE || type == BC_OUTFLOW_SIMPLE_FLUX) src = plus ? n - 1 : 0;

@@ -292,3 +292,120 @@ def conserved_from_flow(flow, gas):
     if nsp > 1:
         return gas.encode(rho, u, *vel, flow.massf(nsp)), flow.get("T", "T[0]").copy()
     return gas.encode(rho, u, *vel), None
+
+
+# ------------------------------------------------------- static profiles
+
+class FlowProfile:
+    """The flow states of a profile file for InFlowBC_StaticProfile
+    (flowstate.d:589-736): lines of cell data in the fixed flow-file order
+    (scan_cell_data_from_fixed_order_string, fvcell.d:2580-), lines holding
+    '#' skipped; each ghost cell takes the state of the profile point nearest
+    to its centre under the match rule, the first such point on ties."""
+
+    MATCHES = ("xyz-to-xyz", "xyA-to-xyA", "AyA-to-AyA", "xy-to-xR", "Ay-to-AR")
+
+    def __init__(self, path, gas, match="xyz-to-xyz"):
+        if match not in self.MATCHES:
+            raise ValueError("Invalid match option: %r" % match)
+        self.match = match
+        self.names = flow_variable_list(gas)
+        self.nsp = getattr(gas, "n_species", 1)
+        rows = []
+        opener = gzip.open if _is_gzip(path) else open
+        with opener(path, "rt") as f:
+            for line in f:
+                if "#" in line or not line.strip():
+                    continue
+                rows.append([float(x) for x in line.split()[:len(self.names)]])
+        self.data = np.array(rows, dtype=np.float64)
+        col = {n: q for q, n in enumerate(self.names)}
+        self.pos = self.data[:, [col["pos.x"], col["pos.y"], col["pos.z"]]]
+        self.col = col
+
+    def distance(self, my, other):
+        """compute_distance (flowstate.d:641-689) from one position to many."""
+        if self.match == "xyz-to-xyz":
+            d = my[None, :] - other
+            return np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2])
+        if self.match == "xyA-to-xyA":
+            dx, dy = my[0] - other[:, 0], my[1] - other[:, 1]
+            return np.sqrt(dx ** 2 + dy ** 2)
+        if self.match == "AyA-to-AyA":
+            return np.abs(my[1] - other[:, 1])
+        other_r = np.sqrt(other[:, 1] ** 2 + other[:, 2] ** 2)
+        my_r = np.sqrt(my[1] ** 2 + my[2] ** 2)
+        dr = my_r - other_r
+        if self.match == "xy-to-xR":
+            dx = my[0] - other[:, 0]
+            return np.sqrt(dx * dx + dr * dr)
+        return np.abs(dr)
+
+    def nearest(self, my):
+        """find_nearest_profile_point: the first point of least distance."""
+        return int(np.argmin(self.distance(np.asarray(my, dtype=np.float64), self.pos)))
+
+    def state(self, my):
+        """[rho, p, T, u, a, vx, vy, vz, S, Y...] of the nearest point, with
+        adjust_velocity (flowstate.d:717-735) for the radial matches."""
+        row = self.data[self.nearest(my)]
+        c = self.col
+        vx, vy, vz = row[c["vel.x"]], row[c["vel.y"]], row[c["vel.z"]]
+        if self.match in ("xy-to-xR", "Ay-to-AR"):
+            r = np.sqrt(my[1] ** 2 + my[2] ** 2)
+            vel_yz = np.sqrt(vy ** 2 + vz ** 2)
+            sign = -1.0 if vy < 0.0 else 1.0
+            vy, vz = sign * vel_yz * my[1] / r, sign * vel_yz * my[2] / r
+        Y = [row[c[n]] for n in self.names if n.startswith("massf[")]
+        return np.array([row[c["rho"]], row[c["p"]], row[c["T"]], row[c["u"]], row[c["a"]], vx, vy, vz,
+                         row[c["S"]]] + (Y if self.nsp > 1 else [1.0]), dtype=np.float64)
+
+
+def ghost_centres(dims, ni, nj, nk, dx, dy, dz, face, origin=(0.0, 0.0, 0.0)):
+    """Centres of the two ghost layers of a face of a uniform Cartesian block,
+    [2][face cells][3], layer-major with the in-face index a + b * na
+    (north/south: a = i, b = k; east/west: a = j, b = k; top/bottom: a = i,
+    b = j)."""
+    nk = nk if dims == 3 else 1
+    d = 0 if face in (1, 3) else (1 if face in (0, 2) else 2)
+    plus = face in (0, 1, 4)
+    n = (ni, nj, nk)
+    na = nj if d == 0 else ni
+    nb = nj if d == 2 else nk
+    out = np.empty((2, na * nb, 3))
+    h = (dx, dy, dz)
+    for layer in (1, 2):
+        for b in range(nb):
+            for a in range(na):
+                idx = [0, 0, 0]
+                idx[d] = n[d] - 1 + layer if plus else -layer
+                if d == 0:
+                    idx[1], idx[2] = a, b
+                elif d == 1:
+                    idx[0], idx[2] = a, b
+                else:
+                    idx[0], idx[1] = a, b
+                c = [origin[q] + (idx[q] + 0.5) * h[q] for q in range(3)]
+                if dims == 2:
+                    c[2] = origin[2]
+                out[layer - 1, a + b * na] = c
+    return out
+
+
+def profile_ghost_states(profile, block, face, origin=(0.0, 0.0, 0.0)):
+    """The flow states InFlowBC_StaticProfile gives the ghost cells of a face
+    of the block (E4HIP_BC_INFLOW_PROFILE layout [2][face cells][9 + nsp])."""
+    g = ghost_centres(block.dims, block.ni, block.nj, block.nk, block.dx, block.dy, block.dz, face, origin)
+    return np.stack([np.stack([profile.state(p) for p in layer]) for layer in g])
+
+
+def write_profile(path, flow, cells):
+    """A profile file: a '#' line of the variable names, then the fixed-order
+    lines of the chosen cells of a flow solution (as the reference's sampling
+    of a solution writes them)."""
+    cols = [np.asarray(flow.values[n], dtype=np.float64).ravel() for n in flow.variables]
+    with open(path, "w") as f:
+        f.write("# " + " ".join(flow.variables) + "\n")
+        for c in cells:
+            f.write(" ".join(("%d" % int(col[c])) if n == "S" else ("%.18e" % col[c])
+                             for n, col in zip(flow.variables, cols)) + "\n")

@@ -38,6 +38,15 @@ class SFluidBlock:
 
     def set_bc(self, face, kind, fs=None):
         fsv = None if fs is None else np.ascontiguousarray(fs, dtype=np.float64)
+        nsp = getattr(self.gas, "n_species", 1)
+        if kind == BCType.inflow_profile:
+            # one state per ghost cell: [2][face cells][9 + nsp]
+            d = 0 if Face(face) in (Face.east, Face.west) else (1 if Face(face) in (Face.north, Face.south) else 2)
+            nfc = (self.nj if d == 0 else self.ni) * (self.nj if d == 2 else self.nk)
+            if fsv is None or fsv.size != 2 * nfc * (9 + nsp):
+                raise ValueError("a static profile needs %d ghost states of %d values" % (2 * nfc, 9 + nsp))
+        elif kind == BCType.inflow and (fsv is None or fsv.size < 8 + (nsp if nsp > 1 else 0)):
+            raise ValueError("an inflow needs a flow state")
         _lib.check(_lib.load().e4hip_block_set_bc(self.id, int(face), int(kind),
                                                   _p(fsv) if fsv is not None else None))
         self.bc[Face(face)] = BCType(kind)

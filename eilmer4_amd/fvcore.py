@@ -97,6 +97,7 @@ class BCType(IntEnum):
     outflow_simple_flux = 5  # OutFlowBC_SimpleFlux / OutFlowBC_Simple (bc.lua:1124-1148)
     slip_wall_1 = 6        # WallBC_WithSlip1: no ghost data, one-sided reconstruction, wall flux
     extrapolate_1 = 7      # OutFlowBC_SimpleExtrapolate{xOrder=1}
+    inflow_profile = 8     # InFlowBC_StaticProfile (states from flowfiles.profile_ghost_states)
 
 
 class PrecisionMode(IntEnum):

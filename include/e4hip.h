@@ -77,6 +77,10 @@ extern "C" {
                                      fluxcalc.d:130-298 (sfluidblock.d:2189-2345); needs >= 3 cells
                                      across the wall */
 #define E4HIP_BC_EXTRAPOLATE_1 7  /* OutFlowBC_SimpleExtrapolate{xOrder=1}: extrapolate_copy.d:102-186 */
+#define E4HIP_BC_INFLOW_PROFILE 8 /* InFlowBC_StaticProfile (bc.lua:996-1019): each ghost cell copies
+                                     the flow state of its nearest profile point
+                                     (flow_state_copy_from_profile.d, flowstate.d:589-736);
+                                     the caller does the matching and passes the states */
 
 /* ---- face names (eilmer/fvcore.d Face enum order) */
 #define E4HIP_FACE_NORTH  0
@@ -181,8 +185,13 @@ int e4hip_block_create(int gm, int dims, int ni, int nj, int nk,
                        double dx, double dy, double dz, int* blk);
 int e4hip_block_destroy(int blk);
 /* Boundary condition of one face (E4HIP_FACE_*, E4HIP_BC_*).  For
- * E4HIP_BC_INFLOW, fs gives the ghost flow state [8 + nsp] as above; else
- * fs may be NULL.  Periodic faces must be set on both opposite faces.
+ * E4HIP_BC_INFLOW, fs gives the ghost flow state [8 + nsp] as above; for
+ * E4HIP_BC_INFLOW_PROFILE, fs holds one state per ghost cell,
+ * [2 layers][face cells][9 + nsp] = rho, p, T, u, a, vx, vy, vz, S, Y..., with
+ * the face cells in-face index a + b * na (north/south: a = i, b = k;
+ * east/west: a = j, b = k; top/bottom: a = i, b = j); else fs may be NULL.
+ * A wall without ghost data (E4HIP_BC_SLIP_WALL_1) needs at least 3 cells
+ * across it.  Periodic faces must be set on both opposite faces.
  * Replaces BoundaryCondition.applyPreReconAction for these effects
  * (eilmer/bc/boundary_condition.d:274-292) and the exchange of
  * simcore.d:1599-1668 for a block connected to itself. */

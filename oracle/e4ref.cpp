@@ -39,9 +39,10 @@ enum { INTERP_PT = 0, INTERP_RHOU = 1, INTERP_RHOP = 2, INTERP_RHOT = 3 };
 // Boundary kinds (ghost-cell effects, eilmer/bc/ghost_cell_effect/*).
 // 5: OutFlowBC_SimpleFlux (ghosts by x_order-0 extrapolation, boundary-face flux
 // by BFE_SimpleOutflowFlux); 6: WallBC_WithSlip1 (no ghost-cell data, one-sided
-// reconstruction and the wall flux); 7: OutFlowBC_SimpleExtrapolate x_order 1
+// reconstruction and the wall flux); 7: OutFlowBC_SimpleExtrapolate x_order 1;
+// 8: InFlowBC_StaticProfile (a flow state per ghost cell)
 enum { BC_SLIP_WALL = 0, BC_INFLOW = 1, BC_EXTRAPOLATE = 2, BC_PERIODIC = 3, BC_CONNECTED = 4,
-       BC_OUTFLOW_SIMPLE_FLUX = 5, BC_SLIP_WALL_1 = 6, BC_EXTRAPOLATE_1 = 7 };
+       BC_OUTFLOW_SIMPLE_FLUX = 5, BC_SLIP_WALL_1 = 6, BC_EXTRAPOLATE_1 = 7, BC_INFLOW_PROFILE = 8 };
 // Face order of a cell (eilmer/fvcore.d Face enum): north, east, south, west, top, bottom.
 enum { F_NORTH = 0, F_EAST = 1, F_SOUTH = 2, F_WEST = 3, F_TOP = 4, F_BOTTOM = 5 };
 
@@ -1083,7 +1084,9 @@ void ref_interp_onesided_scalar(int kind, double len, const double* q, int apply
 // ------------------------------------------------------------------- blocks
 
 struct Block;
-struct BC { int type; FlowState fs; Block* other; int other_face, orientation; };
+// profile: the ghost states of a static profile, [layer][a + b * na] over the
+// face's in-face axes (north/south (i, k), east/west (j, k), top/bottom (i, j))
+struct BC { int type; FlowState fs; Block* other; int other_face, orientation; std::vector<FlowState> profile; };
 
 struct Block {
     Gas* gas;
@@ -1161,6 +1164,26 @@ void ref_block_set_config(Block* b, int flux_calc, int update_scheme, int interp
     b->cfg.flux_calc = flux_calc; b->cfg.update_scheme = update_scheme; b->cfg.interp_order = interp_order;
     b->cfg.apply_limiter = apply_limiter; b->cfg.extrema_clipping = extrema_clipping;
     b->cfg.thermo_interp = thermo_interp; b->cfg.max_invalid_cells = max_invalid_cells;
+}
+
+// A static profile: states [2][face cells][9 + nsp] = rho, p, T, u, a, vx, vy,
+// vz, S, massf...
+void ref_block_set_bc_profile(Block* b, int face, const double* states)
+{
+    const int dir = (face == F_EAST || face == F_WEST) ? 0 : (face == F_NORTH || face == F_SOUTH) ? 1 : 2;
+    const size_t nfc = (size_t)((dir == 0) ? b->nj : b->ni) * ((dir == 2) ? b->nj : b->nk);
+    const int w = 9 + b->gas->nsp;
+    BC& bc = b->bc[face];
+    bc.type = BC_INFLOW_PROFILE;
+    bc.profile.assign(2 * nfc, FlowState{});
+    for (size_t g = 0; g < 2 * nfc; ++g) {
+        const double* st = states + g * w;
+        FlowState& f = bc.profile[g];
+        f.gas.rho = st[0]; f.gas.p = st[1]; f.gas.T = st[2]; f.gas.u = st[3]; f.gas.a = st[4];
+        f.vel[0] = st[5]; f.vel[1] = st[6]; f.vel[2] = st[7];
+        f.S = st[8] != 0.0;
+        for (int s = 0; s < b->gas->nsp; ++s) f.gas.massf[s] = st[9 + s];
+    }
 }
 
 // fs vector layout: [rho,p,T,u,a,vx,vy,vz,massf...]
@@ -1424,6 +1447,16 @@ static void apply_bcs(Block* b)
                     case BC_INFLOW:
                         dest = bc.fs;
                         break;
+                    case BC_INFLOW_PROFILE: {
+                        // flow_state_copy_from_profile.d: the nearest profile
+                        // point's state, matched by the caller
+                        const int gi[3] = {g[0] - b->imin, g[1] - b->jmin, g[2] - b->kmin};
+                        const int na = (dir == 0) ? b->nj : b->ni;
+                        const int fa = (dir == 0) ? gi[1] : gi[0], fb = (dir == 2) ? gi[1] : ((dims == 3) ? gi[2] : 0);
+                        const size_t nfc = (size_t)na * ((dir == 2) ? b->nj : b->nk);
+                        dest = bc.profile[(layer - 1) * nfc + fa + (size_t)fb * na];
+                        break;
+                    }
                     case BC_EXTRAPOLATE:
                     case BC_OUTFLOW_SIMPLE_FLUX:
                         s[dir] = plus ? hi[dir] : lo[dir];

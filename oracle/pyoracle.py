@@ -21,6 +21,7 @@ INTERP_PT, INTERP_RHOU, INTERP_RHOP, INTERP_RHOT = range(4)
 BC_SLIP_WALL, BC_INFLOW, BC_EXTRAPOLATE, BC_PERIODIC, BC_CONNECTED = range(5)
 # OutFlowBC_SimpleFlux, WallBC_WithSlip1, OutFlowBC_SimpleExtrapolate x_order 1
 BC_OUTFLOW_SIMPLE_FLUX, BC_SLIP_WALL_1, BC_EXTRAPOLATE_1 = 5, 6, 7
+BC_INFLOW_PROFILE = 8          # InFlowBC_StaticProfile: a state per ghost cell
 NORTH, EAST, SOUTH, WEST, TOP, BOTTOM = range(6)
 
 _lib = None
@@ -66,6 +67,7 @@ def lib():
         L.ref_block_set_shock_tolerances.argtypes = [vp, C.c_double, C.c_double]
         L.ref_block_get_S.argtypes = [vp, C.POINTER(C.c_int)]
         L.ref_block_set_bc.argtypes = [vp, C.c_int, C.c_int, _dp]
+        L.ref_block_set_bc_profile.argtypes = [vp, C.c_int, _dp]
         L.ref_block_set_from_pT.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.ref_block_set_U.argtypes = [vp, _dp, _dp]
         L.ref_block_get_U.argtypes = [vp, C.c_int, _dp]
@@ -236,6 +238,9 @@ class Block:
         return out
 
     def set_bc(self, face, kind, fs=None):
+        if kind == BC_INFLOW_PROFILE:
+            lib().ref_block_set_bc_profile(self.h, face, _p(_f64(fs).ravel()))
+            return
         lib().ref_block_set_bc(self.h, face, kind, _p(_f64(fs)) if fs is not None else None)
 
     def set_from_pT(self, p, T, vx, vy, vz, massf=None):

@@ -130,3 +130,72 @@ def test_conserved_from_reference_flow():
     assert np.max(np.abs(U[1] / rho - f.get("vel.x"))) <= 1e-12 * np.abs(f.get("vel.x")).max()
     ke = 0.5 * (f.get("vel.x") ** 2 + f.get("vel.y") ** 2 + f.get("vel.z") ** 2)
     assert np.max(np.abs(U[4] / rho - ke - f.get("e[0]")) / f.get("e[0]")) < 1e-12
+
+
+def _profile_flow(gas, ny):
+    """A column of ny cells at x = 0.5, y = (j + 0.5)/ny varying in y."""
+    y = (np.arange(ny) + 0.5) / ny
+    p = 1.0e5 * (1.0 + 0.2 * y)
+    T = 300.0 + 100.0 * y
+    rho, u, a = gas.rho_u_a_from_pT(p, T)
+    zero = np.zeros(ny)
+    v = {"pos.x": np.full(ny, 0.5), "pos.y": y, "pos.z": zero, "volume": np.full(ny, 1.0 / ny), "rho": rho,
+         "vel.x": 600.0 + 50.0 * y, "vel.y": 10.0 * y, "vel.z": zero, "p": p, "a": a, "mu": zero, "k": zero,
+         "mu_t": zero, "k_t": zero, "S": (np.arange(ny) % 3 == 0).astype(float), "tke": zero, "omega": np.ones(ny),
+         "massf[0]-air": np.ones(ny), "u": u, "T": T}
+    return ff.FlowSolution(ff.flow_variable_list(gas), v, 2, 1, ny, 1)
+
+
+def test_static_profile_matching(tmp_path):
+    """flowstate.d:589-736: each ghost cell takes the nearest profile point
+    (first on ties).  Profile points at y = (m + 0.5)/8; a block of 16 cells
+    in y has its west ghosts at y = (j + 0.5)/16, whose nearest points are
+    m = j // 2 (the lower one on the exact tie is impossible here: no ghost
+    is equidistant)."""
+    gas = e4.IdealGas()
+    fl = _profile_flow(gas, 8)
+    ff.write_profile(tmp_path / "prof.dat", fl, range(8))
+    prof = ff.FlowProfile(tmp_path / "prof.dat", gas)
+    assert prof.data.shape == (8, len(ff.flow_variable_list(gas)))
+    g = ff.ghost_centres(2, 4, 16, 1, 0.25, 1.0 / 16, 1.0, 3)       # west face
+    assert g.shape == (2, 16, 3)
+    assert np.allclose(g[0, :, 0], -0.125) and np.allclose(g[1, :, 0], -0.375)
+    for j in range(16):
+        assert prof.nearest(g[0, j]) == j // 2
+        st = prof.state(g[0, j])
+        m = j // 2
+        assert st[0] == fl.values["rho"][m] and st[5] == fl.values["vel.x"][m] and st[8] == fl.values["S"][m]
+    # AyA-to-AyA ignores x and z
+    prof_y = ff.FlowProfile(tmp_path / "prof.dat", gas, "AyA-to-AyA")
+    assert prof_y.nearest([100.0, 0.3, -7.0]) == 2
+    with pytest.raises(ValueError):
+        ff.FlowProfile(tmp_path / "prof.dat", gas, "xyz-to-abc")
+
+
+def test_static_profile_uniform_is_supersonic_inflow():
+    """A profile of one state everywhere is InFlowBC_Supersonic with that
+    state (the oracle, bit for bit)."""
+    import pyoracle as O
+    gas = e4.IdealGas()
+    og = O.Gas.ideal()
+    rng = np.random.default_rng(3)
+    n = 8
+    shape = (n, n, n)
+    U0 = gas.encode(*[x.ravel() for x in (np.full(shape, 1.2) + 0.1 * rng.random(shape),
+                                          2.2e5 + 1e4 * rng.random(shape), 100 * rng.random(shape),
+                                          np.zeros(shape), np.zeros(shape))])
+    fs = O.flowstate_pT(og, 1.0e5, 300.0, (700.0, 5.0, -3.0))
+    states = np.zeros((2, n * n, 10))
+    states[..., :8] = fs[:8]
+    states[..., 9] = 1.0
+    blocks = []
+    for kind, arg in ((O.BC_INFLOW, fs), (O.BC_INFLOW_PROFILE, states)):
+        b = O.Block(og, 3, n, n, n, 0.1, 0.1, 0.1)
+        b.set_config(flux_calc=O.AUSMDV)
+        b.set_bc(O.WEST, kind, arg)
+        b.set_bc(O.EAST, O.BC_EXTRAPOLATE)
+        b.set_U(U0)
+        for _ in range(3):
+            assert b.step(1e-5)[0] == 0
+        blocks.append(b.get_U())
+    assert np.array_equal(blocks[0], blocks[1])

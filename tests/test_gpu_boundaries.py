@@ -235,3 +235,82 @@ def test_boundaries_thermally_perfect_no_clipping(gpu):
     print("  U rel err per quantity", err, flush=True)
     assert err.max() <= TOL
     db.destroy()
+
+
+# ---------------------------------------------------------- static profiles
+
+def _profile_file(gas, path, seed):
+    """Profile points on a 5 x 4 lattice in (y, z) at x = -0.05, states
+    varying over it, every third point a shock point."""
+    from eilmer4_amd import flowfiles as ff
+    rng = np.random.default_rng(seed)
+    yy, zz = np.meshgrid((np.arange(5) + 0.5) / 5, (np.arange(4) + 0.5) / 4, indexing="xy")
+    y, z = yy.ravel(), zz.ravel()
+    m = y.size
+    p = 1.0e5 * (1.0 + 0.3 * rng.random(m))
+    T = 300.0 * (1.0 + 0.3 * rng.random(m))
+    rho, u, a = gas.rho_u_a_from_pT(p, T)
+    zero = np.zeros(m)
+    v = {"pos.x": np.full(m, -0.05), "pos.y": y, "pos.z": z, "volume": np.full(m, 1e-3), "rho": rho,
+         "vel.x": 500.0 + 200.0 * rng.random(m), "vel.y": 50.0 * (rng.random(m) - 0.5),
+         "vel.z": 50.0 * (rng.random(m) - 0.5), "p": p, "a": a, "mu": zero, "k": zero, "mu_t": zero, "k_t": zero,
+         "S": (np.arange(m) % 3 == 0).astype(float), "tke": zero, "omega": np.ones(m), "massf[0]-air": np.ones(m),
+         "u": u, "T": T}
+    fl = ff.FlowSolution(ff.flow_variable_list(gas), v, 3, m, 1, 1)
+    ff.write_profile(path, fl, range(m))
+    return ff.FlowProfile(path, gas)
+
+
+@pytest.mark.parametrize("precision", ["fp64_exact", "fp64"])
+@pytest.mark.parametrize("flux", ["ausmdv", "adaptive_hanel_ausmdv"])
+def test_static_profile_inflow(gpu, tmp_path, flux, precision):
+    """InFlowBC_StaticProfile on the west face: the ghost states of the
+    nearest profile points (shock indicators included) on the device and in
+    the oracle."""
+    from eilmer4_amd import flowfiles as ff
+    gas = e4.IdealGas()
+    n = (10, 8, 6)
+    prof = _profile_file(gas, tmp_path / "inflow.dat", 51)
+    geom = dict(dims=3, ni=n[0], nj=n[1], nk=n[2], dx=1.0 / n[0], dy=1.0 / n[1], dz=1.0 / n[2])
+    probe = e4.SFluidBlock(gas, 3, *n, geom["dx"], geom["dy"], geom["dz"])
+    states = ff.profile_ghost_states(prof, probe, int(e4.Face.west))
+    probe.destroy()
+    assert states.shape == (2, n[1] * n[2], 10)
+    bcs = {O.WEST: (BCType.inflow_profile, states), O.EAST: (BCType.outflow_simple_flux, None),
+           O.NORTH: (BCType.extrapolate, None), O.SOUTH: (BCType.slip_wall, None),
+           O.TOP: (BCType.periodic, None), O.BOTTOM: (BCType.periodic, None)}
+    U0 = _field(gas, (n[2], n[1], n[0]), 52, (550.0, 0.0, 0.0), vamp=100.0)
+    print("case static profile 3D %s flux=%s precision=%s steps=4" % (n, flux, precision), flush=True)
+    Uo, Ud = _run(3, n, bcs, U0, 4, flux, precision)
+    _check(precision, Ud, Uo, False)
+
+
+def test_static_profile_inflow_thermally_perfect(gpu):
+    """The mixture path reads the stored ghost states through the same
+    per-face buffers as its x_order-1 ghosts."""
+    from test_gpu_tp import setup_pair
+    gas = e4.ThermallyPerfectGas.from_json()
+    n = 10
+    rng = np.random.default_rng(53)
+    Y = np.array([0.74, 0.2, 0.03, 0.02, 0.01])
+    states = np.zeros((2, n * n, 9 + 5))
+    for g in range(2 * n * n):
+        fs = gas.flowstate(5.0e4 * (1 + 0.2 * rng.random()), 1500.0 * (1 + 0.2 * rng.random()),
+                           (900.0, 20.0 * rng.random(), 0.0), Y)
+        states.reshape(-1, 14)[g, :8] = fs[:8]
+        states.reshape(-1, 14)[g, 8] = float(g % 4 == 0)
+        states.reshape(-1, 14)[g, 9:] = fs[8:]
+    bcs = {e4.Face.west: (BCType.inflow_profile, states), e4.Face.east: (BCType.extrapolate, None),
+           e4.Face.north: (BCType.slip_wall, None), e4.Face.south: (BCType.slip_wall, None),
+           e4.Face.top: (BCType.periodic, None), e4.Face.bottom: (BCType.periodic, None)}
+    print("case tp static profile n=10", flush=True)
+    ob, db, sim = setup_pair(10, flux="adaptive_hanel_ausmdv", precision="fp64", bcs=bcs, seed=54)
+    dt, _, _ = ob.determine_dt(1.0e-3, 0.5, False)
+    for _ in range(3):
+        assert ob.step(dt)[0] == 0
+        sim.state.dt_global = dt
+        sim.gasdynamic_explicit_increment_with_fixed_grid()
+    err = rel_err(db.download(), ob.get_U())
+    print("  U rel err per quantity", err, flush=True)
+    assert err.max() <= 1.0e-10
+    db.destroy()
