@@ -64,6 +64,7 @@ struct BlockH {
     // profiles): [8+nsp][2 * face cells]; gS the profile's shock indicators
     double* gx[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     unsigned char* gS[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    double* axi = nullptr;                        // 2D axisymmetric geometry (e4::axi_geom layout)
     // shock indicators per cell (adaptive_hanel_ausmdv): S[0] this stage's
     // detection, S[1] the previous one
     unsigned char* S[2] = {nullptr, nullptr};
@@ -325,6 +326,7 @@ int stage_args_for(const BlockH& b, int stage, double dt, unsigned int* counters
     A.c1 = dt * g1;
     boundary_slabs(b, A.fix_lo, A.fix_hi, A.fix_off, A.noghost);
     for (int f = 0; f < 6; ++f) { A.gx[f] = b.gx[f]; A.gS[f] = b.gS[f]; }
+    A.axi = b.axi;
     const int src = shock_fields(b, A);
     if (src) return src;
     const int nk = A.nk;
@@ -560,6 +562,7 @@ int tp_args_for(const BlockH& b, int stage, double dt, unsigned int* counters, e
         A.fix_off[6] = total;
     }
     for (int f = 0; f < 6; ++f) { A.gx[f] = b.gx[f]; A.gS[f] = b.gS[f]; }
+    A.axi = b.axi;
     return shock_fields(b, A);
 }
 
@@ -651,6 +654,7 @@ int free_block_buffers(BlockH& b)
     for (auto& p : b.S) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     for (auto& p : b.gx) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     for (auto& p : b.gS) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
+    if (b.axi) { HIP_TRY(hipFree(b.axi)); b.axi = nullptr; }
     if (b.bc_dev) { HIP_TRY(hipFree(b.bc_dev)); b.bc_dev = nullptr; }
     return E4HIP_OK;
 }
@@ -999,6 +1003,55 @@ int e4hip_block_set_bc(int blk, int face, int type, const double* fs)
     b->bc_type[face] = type;
     b->nbr_blk[face] = -1;
     return upload_bc(*b);
+}
+
+int e4hip_block_set_axisymmetric(int blk, double x0, double y0)
+{
+    BlockH* b = get_block(blk);
+    if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
+    if (b->dims != 2) return fail(E4HIP_ERR_ARG, "axisymmetric geometry is for 2D blocks");
+    if (!(y0 >= 0.0) || !std::isfinite(x0)) return fail(E4HIP_ERR_ARG, "the block must lie at y >= 0");
+    // xyplane_quad_cell_properties (properties.d:177-237; volume = area *
+    // centroid y, fvcell.d:314-351) and update_2D_geometric_data
+    // (fvinterface.d:265-297; area = length * mid-point y) of the cells and
+    // faces of vertices (x0 + i dx, y0 + j dy); vertex orders sfluidblock.d:512-600
+    const int ni = b->ni, nj = b->nj;
+    auto X = [&](int i) { return x0 + i * b->dx; };
+    auto Y = [&](int j) { return y0 + j * b->dy; };
+    const long nc = (long)ni * nj;
+    std::vector<double> h(2 * nc + (long)(ni + 1) * nj + (long)ni * (nj + 1));
+    e4::AxiGeom g = e4::axi_geom(h.data(), ni, nj);
+    double* vol = const_cast<double*>(g.vol);
+    double* axy = const_cast<double*>(g.areaxy);
+    double* ai = const_cast<double*>(g.ai);
+    double* aj = const_cast<double*>(g.aj);
+    for (int j = 0; j < nj; ++j)
+        for (int i = 0; i < ni; ++i) {
+            const double xA = X(i + 1), yA = Y(j), xB = X(i + 1), yB = Y(j + 1);
+            const double xC = X(i), yC = Y(j + 1), xD = X(i), yD = Y(j);
+            const double area = 0.5 * ((xB + xA) * (yB - yA) + (xC + xB) * (yC - yB) +
+                                       (xD + xC) * (yD - yC) + (xA + xD) * (yA - yD));
+            const double yc = -1.0 / (area * 6.0) *
+                              ((xB - xA) * (yA * yA + yA * yB + yB * yB) + (xC - xB) * (yB * yB + yB * yC + yC * yC) +
+                               (xD - xC) * (yC * yC + yC * yD + yD * yD) + (xA - xD) * (yD * yD + yD * yA + yA * yA));
+            vol[(long)j * ni + i] = area * yc;
+            axy[(long)j * ni + i] = area;
+        }
+    auto face_area = [](double xA, double yA, double xB, double yB) {
+        const double LAB = sqrt((xB - xA) * (xB - xA) + (yB - yA) * (yB - yA));
+        const double Ybar = 0.5 * (yA + yB);
+        return LAB * Ybar;
+    };
+    for (int j = 0; j < nj; ++j)
+        for (int i = 0; i <= ni; ++i) ai[(long)j * (ni + 1) + i] = face_area(X(i), Y(j), X(i), Y(j + 1));
+    for (int j = 0; j <= nj; ++j)
+        for (int i = 0; i < ni; ++i) aj[(long)j * ni + i] = face_area(X(i + 1), Y(j), X(i), Y(j));
+    for (long c = 0; c < nc; ++c)
+        if (!(vol[c] > 0.0)) return fail(E4HIP_ERR_ARG, "a cell has no volume (on the axis)");
+    HIP_TRY(hipStreamSynchronize(g_state.stream));
+    if (!b->axi) HIP_TRY(hipMalloc(&b->axi, sizeof(double) * h.size()));
+    HIP_TRY(hipMemcpy(b->axi, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    return E4HIP_OK;
 }
 
 int e4hip_block_connect(int blk, int face, int other, int other_face, int orientation)

@@ -127,7 +127,21 @@ struct StageArgs {
     // indicators
     const double* gx[6];
     const unsigned char* gS[6];
+    // 2D axisymmetric geometry (null: planar): cell volumes [ni*nj], cell
+    // (x,y)-plane areas [ni*nj], i-face areas [(ni+1)*nj], j-face areas
+    // [ni*(nj+1)], one array after the other
+    const double* axi;
 };
+
+// Per-radian geometry of a 2D axisymmetric block, laid out as above.
+struct AxiGeom {
+    const double *vol, *areaxy, *ai, *aj;
+};
+__host__ __device__ inline AxiGeom axi_geom(const double* g, int ni, int nj)
+{
+    const long nc = (long)ni * nj;
+    return AxiGeom{g, g + nc, g + 2 * nc, g + 2 * nc + (long)(ni + 1) * nj};
+}
 
 // A batched stage launch: one workgroup per (block, tile, k-chunk) over all
 // blocks of the batch; workgroup w belongs to table entry wg_entry[w].  The
@@ -189,6 +203,7 @@ struct TPArgs {
     // of a stage, or a static profile; gS the profile's shock indicators
     double* gx[6];
     const unsigned char* gS[6];
+    const double* axi;                // 2D axisymmetric geometry, as in StageArgs
 };
 
 // Launchers, one per arithmetic mode; each lives in its own translation unit

@@ -374,18 +374,20 @@ __device__ __forceinline__ void lds_put_flux(R* dst, int nslot, int s, const R F
 }
 
 // The rest of a stage for one cell from its flux integral (the sum over its
-// faces of -F * outsign * area): time derivative, stage update, decode checks
+// faces of -F * outsign * area), its inverse volume and the y-momentum source
+// (the axisymmetric pressure term, 0 for planar flow): time derivative
+// (fvcell.d:826-832), stage update, decode checks
 // with the low-T fallback (fvcell.d:570-589, 680-697), check_data
 // (flowstate.d:345-388, gas_state.d:199-241), write, and the wave's counts of
 // invalid cells and decode failures.  All in FP64.
 template <class M>
-__device__ __forceinline__ void finish_cell(KArgs ap, long c, const double integ[5])
+__device__ __forceinline__ void finish_cell(KArgs ap, long c, const double integ[5], double vol_inv, double qy)
 {
     const long qs = opaque_qs(ap->qs);
     double Un[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-        double d = ap->vol_inv * integ[q] + 0.0;
+        double d = vol_inv * integ[q] + ((q == 2) ? qy : 0.0);
         double dd0 = (ap->stage >= 2) ? ap->d0[c + q * qs] : 0.0;
         double dd1 = (ap->stage >= 3) ? ap->d1[c + q * qs] : 0.0;
         if (ap->dout) ap->dout[c + q * qs] = d;
@@ -648,15 +650,34 @@ void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type A
                 if (DIMS == 3) integ[3] -= double(Fsrc[3 * nface + fidx]) * area;
                 integ[4] -= double(Fsrc[4 * nface + fidx]) * area;
             };
-            add_face(sJ, NSJ, jN, 1.0, ap->area[1]);
-            add_face(sI, NSI, iE, 1.0, ap->area[0]);
-            add_face(sJ, NSJ, jS, -1.0, ap->area[1]);
-            add_face(sI, NSI, iW, -1.0, ap->area[0]);
-            if constexpr (DIMS == 3) {
-                add_face(Ft, 1, 0, 1.0, ap->area[2]);
-                add_face(Fb, 1, 0, -1.0, ap->area[2]);
+            bool axi = false;
+            if constexpr (DIMS == 2) {
+                if (ap->axi) {
+                    // per-radian face areas, volume and the pressure source
+                    axi = true;
+                    const AxiGeom g = axi_geom(ap->axi, ap->ni, ap->nj);
+                    const long c = cell_index(ap, ik, jk, 0);
+                    add_face(sJ, NSJ, jN, 1.0, g.aj[(long)(jk + 1) * ap->ni + ik]);
+                    add_face(sI, NSI, iE, 1.0, g.ai[(long)jk * (ap->ni + 1) + ik + 1]);
+                    add_face(sJ, NSJ, jS, -1.0, g.aj[(long)jk * ap->ni + ik]);
+                    add_face(sI, NSI, iW, -1.0, g.ai[(long)jk * (ap->ni + 1) + ik]);
+                    const Prim P = load_decode<M>(ap, c);
+                    const double p = P.rho * ap->gas.Rgas * (P.u * ap->gas.Cvinv);
+                    const double vol = g.vol[c];
+                    finish_cell<M>(ap, c, integ, 1.0 / vol, 0.0 + p * g.areaxy[c] / vol);
+                }
             }
-            finish_cell<M>(ap, cell_index(ap, ik, jk, k), integ);
+            if (!axi) {
+                add_face(sJ, NSJ, jN, 1.0, ap->area[1]);
+                add_face(sI, NSI, iE, 1.0, ap->area[0]);
+                add_face(sJ, NSJ, jS, -1.0, ap->area[1]);
+                add_face(sI, NSI, iW, -1.0, ap->area[0]);
+                if constexpr (DIMS == 3) {
+                    add_face(Ft, 1, 0, 1.0, ap->area[2]);
+                    add_face(Fb, 1, 0, -1.0, ap->area[2]);
+                }
+                finish_cell<M>(ap, cell_index(ap, ik, jk, k), integ, ap->vol_inv, 0.0);
+            }
         }
         // 5. k-direction work for plane k+1: face states of cell k+2 and the
         // flux of the top face of cell k+1
@@ -874,6 +895,19 @@ __global__ __launch_bounds__(64) void boundary_cell_kernel(StageArgs)
         if (DIMS == 3) integ[3] -= Fq[3] * area;
         integ[4] -= Fq[4] * area;
     };
+    const long cc = cell_index(A, i, j, k);
+    if (DIMS == 2 && A->axi) {
+        const AxiGeom g = axi_geom(A->axi, A->ni, A->nj);
+        add_face(FN, 1.0, g.aj[(long)(j + 1) * A->ni + i]);
+        add_face(FE, 1.0, g.ai[(long)j * (A->ni + 1) + i + 1]);
+        add_face(FS, -1.0, g.aj[(long)j * A->ni + i]);
+        add_face(FW, -1.0, g.ai[(long)j * (A->ni + 1) + i]);
+        const Prim P = load_decode<M>(A, cc);
+        const double p = P.rho * A->gas.Rgas * (P.u * A->gas.Cvinv);
+        const double vol = g.vol[cc];
+        finish_cell<M>(A, cc, integ, 1.0 / vol, 0.0 + p * g.areaxy[cc] / vol);
+        return;
+    }
     add_face(FN, 1.0, A->area[1]);
     add_face(FE, 1.0, A->area[0]);
     add_face(FS, -1.0, A->area[1]);
@@ -882,7 +916,7 @@ __global__ __launch_bounds__(64) void boundary_cell_kernel(StageArgs)
         add_face(FT, 1.0, A->area[2]);
         add_face(FB, -1.0, A->area[2]);
     }
-    finish_cell<M>(A, cell_index(A, i, j, k), integ);
+    finish_cell<M>(A, cc, integ, A->vol_inv, 0.0);
 }
 
 template <class M>

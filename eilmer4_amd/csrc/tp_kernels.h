@@ -448,17 +448,30 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
                 integ[q] -= Fsrc[q * fq + fidx] * area;
             }
         };
-        add_face(A.F[1], A.fqs[1], jS + A.ni, 1.0, A.area[1]);
-        add_face(A.F[0], A.fqs[0], iW + 1, 1.0, A.area[0]);
-        add_face(A.F[1], A.fqs[1], jS, -1.0, A.area[1]);
-        add_face(A.F[0], A.fqs[0], iW, -1.0, A.area[0]);
-        if (A.dims == 3) {
-            add_face(A.F[2], A.fqs[2], kB + (long)A.ni * A.nj, 1.0, A.area[2]);
-            add_face(A.F[2], A.fqs[2], kB, -1.0, A.area[2]);
+        double vol_inv = A.vol_inv, qy = 0.0;
+        if (A.axi) {
+            // 2D axisymmetric: per-radian face areas and volume, and the
+            // pressure source of the y-momentum (fvcell.d:2177-2181)
+            const AxiGeom g = axi_geom(A.axi, A.ni, A.nj);
+            add_face(A.F[1], A.fqs[1], jS + A.ni, 1.0, g.aj[jS + A.ni]);
+            add_face(A.F[0], A.fqs[0], iW + 1, 1.0, g.ai[iW + 1]);
+            add_face(A.F[1], A.fqs[1], jS, -1.0, g.aj[jS]);
+            add_face(A.F[0], A.fqs[0], iW, -1.0, g.ai[iW]);
+            vol_inv = 1.0 / g.vol[c];
+            qy = 0.0 + A.Pin[c + P_P * qs] * g.areaxy[c] / g.vol[c];
+        } else {
+            add_face(A.F[1], A.fqs[1], jS + A.ni, 1.0, A.area[1]);
+            add_face(A.F[0], A.fqs[0], iW + 1, 1.0, A.area[0]);
+            add_face(A.F[1], A.fqs[1], jS, -1.0, A.area[1]);
+            add_face(A.F[0], A.fqs[0], iW, -1.0, A.area[0]);
+            if (A.dims == 3) {
+                add_face(A.F[2], A.fqs[2], kB + (long)A.ni * A.nj, 1.0, A.area[2]);
+                add_face(A.F[2], A.fqs[2], kB, -1.0, A.area[2]);
+            }
         }
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            double d = A.vol_inv * integ[q] + 0.0;
+            double d = vol_inv * integ[q] + ((q == 2) ? qy : 0.0);
             double dd0 = (A.stage >= 2) ? A.d0[c + q * qs] : 0.0;
             double dd1 = (A.stage >= 3) ? A.d1[c + q * qs] : 0.0;
             if (A.dout) A.dout[c + q * qs] = d;

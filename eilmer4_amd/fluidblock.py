@@ -51,6 +51,13 @@ class SFluidBlock:
                                                   _p(fsv) if fsv is not None else None))
         self.bc[Face(face)] = BCType(kind)
 
+    def set_axisymmetric(self, x0=0.0, y0=0.0):
+        """2D axisymmetric geometry (GlobalConfig.axisymmetric) for this block,
+        lower-left vertex at (x0, y0): per-radian volumes and face areas and
+        the pressure source of the radial momentum."""
+        _lib.check(_lib.load().e4hip_block_set_axisymmetric(self.id, float(x0), float(y0)))
+        self.axisymmetric = (float(x0), float(y0))
+
     def connect(self, face, other, other_face, orientation=0):
         """One side of a full-face connection: the ghost cells of `face` copy
         the mapped interior cells of `other` next to `other_face` at every

@@ -205,6 +205,13 @@ int e4hip_block_set_bc(int blk, int face, int type, const double* fs);
  * orientation.  The blocks must share the gas model and dimensionality and the
  * face sizes must match under the orientation.  blk == other is allowed. */
 int e4hip_block_connect(int blk, int face, int other, int other_face, int orientation);
+/* 2D axisymmetric geometry (GlobalConfig.axisymmetric, x the axis, y the
+ * radius) for a block whose lower-left vertex is (x0, y0), y0 >= 0: per-radian
+ * cell volumes (area * centroid y, fvcell.d:314-351; properties.d:177-237) and
+ * face areas (length * mid-point y, fvinterface.d:265-297), and the pressure
+ * source p * areaxy / volume of the y-momentum (add_inviscid_source_vector,
+ * fvcell.d:2177-2181).  Planar blocks need no call. */
+int e4hip_block_set_axisymmetric(int blk, double x0, double y0);
 /* Host -> device copy of U[0] ([nq][ncells]); the flow state is the decode of
  * U (fvcell.d:563-732).  For a thermally perfect gas, Tguess[ncells] is the
  * temperature the Newton solve starts from (the flow state's T on restart);

@@ -1101,6 +1101,10 @@ struct Block {
     std::vector<double> F[3];     // face fluxes, indexed by the cell on the + side
     std::vector<char> bad;
     BC bc[6];
+    // 2D axisymmetric geometry (per radian): cell volumes and (x,y)-plane
+    // areas [ni*nj], i-face areas [(ni+1)*nj], j-face areas [ni*(nj+1)]
+    bool axi = false;
+    std::vector<double> axi_vol, axi_areaxy, axi_ai, axi_aj;
 };
 
 static inline size_t cidx(const Block* b, int i, int j, int k) { return ((size_t)k * b->njdim + j) * b->nidim + i; }
@@ -1639,7 +1643,9 @@ static void convective_flux(Block* b)
     }
 }
 
-// fvcell.d:735-875 (sources are zero: planar, non-rotating, inviscid)
+// fvcell.d:735-875 with add_inviscid_source_vector (fvcell.d:2156-2199): the
+// only source on this path is the axisymmetric pressure term of the y-momentum
+// equation, p * areaxy / volume.
 static void time_derivatives(Block* b, size_t c, double* dUdt)
 {
     const int nq = b->nq;
@@ -1650,7 +1656,18 @@ static void time_derivatives(Block* b, size_t c, double* dUdt)
                            &b->F[0][c * nq], three_d ? &b->F[2][(c + sk) * nq] : nullptr,
                            three_d ? &b->F[2][c * nq] : nullptr};
     const double outsign[6] = {1.0, 1.0, -1.0, -1.0, 1.0, -1.0};
-    const double A[6] = {b->area[1], b->area[0], b->area[1], b->area[0], b->area[2], b->area[2]};
+    double A[6] = {b->area[1], b->area[0], b->area[1], b->area[0], b->area[2], b->area[2]};
+    double volume = b->vol, Qy = 0.0;
+    if (b->axi) {
+        const int i = (int)(c % b->nidim) - b->imin, j = (int)(c / b->nidim) - b->jmin;
+        const size_t cc = (size_t)j * b->ni + i;
+        A[0] = b->axi_aj[(size_t)(j + 1) * b->ni + i];
+        A[1] = b->axi_ai[(size_t)j * (b->ni + 1) + i + 1];
+        A[2] = b->axi_aj[(size_t)j * b->ni + i];
+        A[3] = b->axi_ai[(size_t)j * (b->ni + 1) + i];
+        volume = b->axi_vol[cc];
+        Qy = 0.0 + b->fs[c].gas.p * b->axi_areaxy[cc] / volume;
+    }
     double integral[NQ_MAX];
     for (int q = 0; q < nq; ++q) integral[q] = 0.0;
     const int nf = three_d ? 6 : 4;
@@ -1664,8 +1681,47 @@ static void time_derivatives(Block* b, size_t c, double* dUdt)
         integral[4] -= myF[4] * area;
         for (int q = 5; q < nq; ++q) integral[q] -= myF[q] * area;
     }
-    double vol_inv = 1.0 / b->vol;
-    for (int q = 0; q < nq; ++q) dUdt[q] = vol_inv * integral[q] + 0.0;
+    double vol_inv = 1.0 / volume;
+    for (int q = 0; q < nq; ++q) dUdt[q] = vol_inv * integral[q] + ((q == 2) ? Qy : 0.0);
+}
+
+// Axisymmetric geometry of a 2D block whose lower-left vertex is (x0, y0),
+// vertex (i, j) at (x0 + i dx, y0 + j dy): xyplane_quad_cell_properties
+// (properties.d:177-237, volume = area * centroid y, fvcell.d:314-351) and
+// FVInterface.update_2D_geometric_data (fvinterface.d:265-297, area = length *
+// mid-point y); face vertex orders sfluidblock.d:538-645.
+void ref_block_set_axisymmetric(Block* b, double x0, double y0)
+{
+    const int ni = b->ni, nj = b->nj;
+    auto X = [&](int i) { return x0 + i * b->dx; };
+    auto Y = [&](int j) { return y0 + j * b->dy; };
+    b->axi = true;
+    b->axi_vol.assign((size_t)ni * nj, 0.0);
+    b->axi_areaxy.assign((size_t)ni * nj, 0.0);
+    b->axi_ai.assign((size_t)(ni + 1) * nj, 0.0);
+    b->axi_aj.assign((size_t)ni * (nj + 1), 0.0);
+    for (int j = 0; j < nj; ++j)
+        for (int i = 0; i < ni; ++i) {
+            // p0 = D (i, j), p1 = A (i+1, j), p2 = B (i+1, j+1), p3 = C (i, j+1)
+            const double xA = X(i + 1), yA = Y(j), xB = X(i + 1), yB = Y(j + 1);
+            const double xC = X(i), yC = Y(j + 1), xD = X(i), yD = Y(j);
+            const double area = 0.5 * ((xB + xA) * (yB - yA) + (xC + xB) * (yC - yB) +
+                                       (xD + xC) * (yD - yC) + (xA + xD) * (yA - yD));
+            const double yc = -1.0 / (area * 6.0) *
+                              ((xB - xA) * (yA * yA + yA * yB + yB * yB) + (xC - xB) * (yB * yB + yB * yC + yC * yC) +
+                               (xD - xC) * (yC * yC + yC * yD + yD * yD) + (xA - xD) * (yD * yD + yD * yA + yA * yA));
+            b->axi_vol[(size_t)j * ni + i] = area * yc;
+            b->axi_areaxy[(size_t)j * ni + i] = area;
+        }
+    auto face_area = [](double xA, double yA, double xB, double yB) {
+        const double LAB = sqrt((xB - xA) * (xB - xA) + (yB - yA) * (yB - yA));
+        const double Ybar = 0.5 * (yA + yB);
+        return LAB * Ybar;
+    };
+    for (int j = 0; j < nj; ++j)
+        for (int i = 0; i <= ni; ++i) b->axi_ai[(size_t)j * (ni + 1) + i] = face_area(X(i), Y(j), X(i), Y(j + 1));
+    for (int j = 0; j <= nj; ++j)
+        for (int i = 0; i < ni; ++i) b->axi_aj[(size_t)j * ni + i] = face_area(X(i + 1), Y(j), X(i), Y(j));
 }
 
 static int nstages(int scheme)

@@ -68,6 +68,7 @@ def lib():
         L.ref_block_get_S.argtypes = [vp, C.POINTER(C.c_int)]
         L.ref_block_set_bc.argtypes = [vp, C.c_int, C.c_int, _dp]
         L.ref_block_set_bc_profile.argtypes = [vp, C.c_int, _dp]
+        L.ref_block_set_axisymmetric.argtypes = [vp, C.c_double, C.c_double]
         L.ref_block_set_from_pT.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.ref_block_set_U.argtypes = [vp, _dp, _dp]
         L.ref_block_get_U.argtypes = [vp, C.c_int, _dp]
@@ -228,6 +229,10 @@ class Block:
                    extrema_clipping=1, thermo_interp=INTERP_RHOU, max_invalid_cells=0):
         lib().ref_block_set_config(self.h, flux_calc, update_scheme, interp_order, apply_limiter,
                                    extrema_clipping, thermo_interp, max_invalid_cells)
+
+    def set_axisymmetric(self, x0=0.0, y0=0.0):
+        """2D axisymmetric geometry, lower-left vertex at (x0, y0)."""
+        lib().ref_block_set_axisymmetric(self.h, x0, y0)
 
     def set_shock_tolerances(self, shear_tolerance=0.20, compression_tolerance=-0.30):
         lib().ref_block_set_shock_tolerances(self.h, shear_tolerance, compression_tolerance)

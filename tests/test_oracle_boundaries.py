@@ -299,3 +299,72 @@ def test_slip1_detector_uses_the_wall_relative_velocity():
     assert b.step(1e-7)[0] == 0
     S = b.get_S().reshape(2, 2, ni)[0, 0]
     assert S[0] == 1 and S[-1] == 0
+
+
+# --------------------------------------------------------------- axisymmetric
+
+def axi_block(ni, nj, y0, bc, rng=None, v=(300.0, 0.0), amp=0.0, calc=O.AUSMDV):
+    g = O.Gas.ideal()
+    b = O.Block(g, 2, ni, nj, 1, 1.0 / ni, 0.5 / nj)
+    b.set_config(flux_calc=calc)
+    for f, t in bc.items():
+        b.set_bc(f, t)
+    b.set_axisymmetric(0.0, y0)
+    shape = (1, nj, ni)
+    rng = rng or np.random.default_rng(0)
+    p = 1e5 * (1 + amp * rng.standard_normal(shape))
+    T = 300 * (1 + amp * rng.standard_normal(shape))
+    b.set_from_pT(p, T, v[0] + 50 * amp * rng.standard_normal(shape), v[1] + 50 * amp * rng.standard_normal(shape), 0.0)
+    return b
+
+
+def test_axisymmetric_uniform_flow_along_the_axis():
+    """Uniform flow parallel to the axis: the pressure on a cell's outer and
+    inner faces (areas dx y_N and dx y_S per radian) is balanced by the source
+    p areaxy / V (add_inviscid_source_vector), so nothing changes beyond
+    rounding; without the source the cells would accelerate outwards."""
+    b = axi_block(12, 8, 0.25, {O.EAST: O.BC_PERIODIC, O.WEST: O.BC_PERIODIC, O.NORTH: O.BC_SLIP_WALL,
+                                O.SOUTH: O.BC_SLIP_WALL})
+    U0 = b.get_U()
+    for _ in range(10):
+        assert b.step(1e-5)[0] == 0
+    U = b.get_U()
+    assert np.max(np.abs(U[0] - U0[0])) / U0[0].max() < 1e-13
+    assert np.max(np.abs(U[2])) < 1e-13 * 1e5 * 10 * 1e-5 / (0.5 / 8)
+
+
+def test_axisymmetric_closed_box_conserves_mass_and_energy():
+    """Per-radian volumes: sum of V rho and V rho E over a box closed by slip
+    walls is conserved to rounding (radial momentum is not: the walls push)."""
+    rng = np.random.default_rng(7)
+    b = axi_block(10, 10, 0.0, {f: O.BC_SLIP_WALL for f in range(4)}, rng, v=(0.0, 0.0), amp=0.05)
+    # per-radian cell volumes of the uniform grid, y from 0: dx (y_{j+1}^2 - y_j^2) / 2
+    dx, dy = 0.1, 0.05
+    yj = dy * np.arange(11)
+    vol = np.broadcast_to((dx * 0.5 * (yj[1:] ** 2 - yj[:-1] ** 2))[:, None], (10, 10)).ravel()
+    U0 = b.get_U()
+    for _ in range(10):
+        assert b.step(2e-6)[0] == 0
+    U = b.get_U()
+    for q in (0, 4):
+        assert abs((vol * U[q]).sum() - (vol * U0[q]).sum()) / (vol * U0[q]).sum() < 1e-14
+
+
+def test_axisymmetric_far_from_the_axis_is_planar():
+    """Far from the axis (y0 = 1e4, cells 0.06 high) the per-radian geometry is
+    a scaled planar one and the source is small: one step agrees with the
+    planar block to O(dy / y0)."""
+    rng = np.random.default_rng(8)
+    bc = {O.EAST: O.BC_EXTRAPOLATE, O.WEST: O.BC_EXTRAPOLATE, O.NORTH: O.BC_SLIP_WALL, O.SOUTH: O.BC_SLIP_WALL}
+    a = axi_block(8, 8, 1.0e4, bc, rng, amp=0.1)
+    g = O.Gas.ideal()
+    pl = O.Block(g, 2, 8, 8, 1, 1.0 / 8, 0.5 / 8)
+    pl.set_config(flux_calc=O.AUSMDV)
+    for f, t in bc.items():
+        pl.set_bc(f, t)
+    pl.set_U(a.get_U())
+    U0 = a.get_U()
+    assert a.step(2e-6)[0] == 0 and pl.step(2e-6)[0] == 0
+    dA, dP = a.get_U() - U0, pl.get_U() - U0
+    rel = np.max(np.abs(dA - dP)) / np.max(np.abs(dP))
+    assert 0.0 < rel < 1e-4, rel
