@@ -409,3 +409,65 @@ def write_profile(path, flow, cells):
         for c in cells:
             f.write(" ".join(("%d" % int(col[c])) if n == "S" else ("%.18e" % col[c])
                              for n, col in zip(flow.variables, cols)) + "\n")
+
+
+# ------------------------------------------------------ job directory layout
+
+def file_ext(fmt):
+    """gz for gziptext, bin for rawbinary (flowsolution.d:88-89)."""
+    return "bin" if fmt == "rawbinary" else "gz"
+
+
+def make_path_name(mytype, tindx):
+    """fileutil.d:15-22: grid/t0003, flow/t0003."""
+    return "%s/t%04d" % (mytype, tindx)
+
+
+def make_file_name(mytype, job, blk_id, tindx, fmt="gziptext"):
+    """fileutil.d:24-33: flow/t0003/job.flow.b0001.t0003.gz."""
+    return "%s/t%04d/%s.%s.b%04d.t%04d.%s" % (mytype, tindx, job, mytype, blk_id, tindx, file_ext(fmt))
+
+
+def append_times(job_dir, job, tindx, sim_time, dt_global):
+    """The times file connecting tindx to the simulation time
+    (simcore.d:678-683; its header from prep.lua:2017-2022)."""
+    import os
+    path = os.path.join(job_dir, "config", job + ".times")
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    new = not os.path.exists(path)
+    with open(path, "a") as f:
+        if new:
+            f.write("# tindx sim_time dt_global\n")
+        f.write("%04d %.18e %.18e\n" % (tindx, sim_time, dt_global))
+
+
+def read_times(job_dir, job):
+    """[(tindx, sim_time, dt_global)] of a times file."""
+    import os
+    out = []
+    with open(os.path.join(job_dir, "config", job + ".times")) as f:
+        for line in f:
+            if line.startswith("#") or not line.strip():
+                continue
+            t, s, d = line.split()[:3]
+            out.append((int(t), float(s), float(d)))
+    return out
+
+
+def write_solution_files(job_dir, job, blocks, tindx, sim_time, dt_global, fmt="gziptext", origins=None,
+                         with_grid=False):
+    """simcore.d:637-684 for these blocks: one flow file per block under
+    flow/tNNNN (and grid files when asked), then the times-file line."""
+    import os
+    for n, blk in enumerate(blocks):
+        origin = origins[n] if origins is not None else (0.0, 0.0, 0.0)
+        fl = flow_from_block(blk, sim_time, origin)
+        path = os.path.join(job_dir, make_file_name("flow", job, n, tindx, fmt))
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        write_flow(path, fl, fmt)
+        if with_grid:
+            g = cartesian_grid(blk.dims, blk.ni, blk.nj, blk.nk, blk.dx, blk.dy, blk.dz, origin)
+            gpath = os.path.join(job_dir, make_file_name("grid", job, n, tindx, fmt))
+            os.makedirs(os.path.dirname(gpath), exist_ok=True)
+            write_grid(gpath, g, fmt)
+    append_times(job_dir, job, tindx, sim_time, dt_global)

@@ -37,6 +37,13 @@ class SFluidBlock:
         return (self.nk, self.nj, self.ni)
 
     def set_bc(self, face, kind, fs=None):
+        """The boundary of one face: a BCType with its data, or one of the
+        reference-named conditions of eilmer4_amd.bc (WallBC_WithSlip1(),
+        OutFlowBC_Simple(), ...)."""
+        from .bc import BoundaryCondition
+        if isinstance(kind, BoundaryCondition):
+            fs = kind.fs(self, face) if callable(kind.fs) else kind.fs
+            kind = kind.kind
         fsv = None if fs is None else np.ascontiguousarray(fs, dtype=np.float64)
         nsp = getattr(self.gas, "n_species", 1)
         if kind == BCType.inflow_profile:

@@ -199,3 +199,28 @@ def test_static_profile_uniform_is_supersonic_inflow():
             assert b.step(1e-5)[0] == 0
         blocks.append(b.get_U())
     assert np.array_equal(blocks[0], blocks[1])
+
+
+def test_job_layout_and_times(tmp_path):
+    """fileutil.d:15-33 names and the times file of simcore.d:678-683."""
+    assert ff.make_file_name("flow", "cone", 3, 12) == "flow/t0012/cone.flow.b0003.t0012.gz"
+    assert ff.make_file_name("grid", "cone", 0, 0, "rawbinary") == "grid/t0000/cone.grid.b0000.t0000.bin"
+    ff.append_times(tmp_path, "cone", 0, 0.0, 1.0e-6)
+    ff.append_times(tmp_path, "cone", 1, 2.5e-3, 3.0e-6)
+    lines = open(tmp_path / "config" / "cone.times").read().splitlines()
+    assert lines[0] == "# tindx sim_time dt_global"
+    assert lines[2] == "0001 2.500000000000000052e-03 3.000000000000000076e-06"
+    assert ff.read_times(tmp_path, "cone") == [(0, 0.0, 1.0e-6), (1, 2.5e-3, 3.0e-6)]
+
+
+def test_reference_named_boundary_conditions():
+    from eilmer4_amd import bc
+    from eilmer4_amd.fvcore import BCType
+    assert bc.WallBC_WithSlip().kind == BCType.slip_wall and bc.WallBC_WithSlip1().kind == BCType.slip_wall_1
+    assert bc.OutFlowBC_Simple().kind == BCType.outflow_simple_flux
+    assert bc.OutFlowBC_SimpleExtrapolate(1).kind == BCType.extrapolate_1
+    fs = e4.IdealGas().flowstate(1e5, 300.0, (500.0, 0.0, 0.0))
+    b = bc.InFlowBC_Supersonic(fs)
+    assert b.kind == BCType.inflow and np.array_equal(b.fs, fs)
+    with pytest.raises(ValueError):
+        bc.OutFlowBC_SimpleExtrapolate(2)
