@@ -251,7 +251,7 @@ int shock_fields(const BlockH& b, Args& A)
 
 // The stage kernel's arguments for one block; kchunk <= 0 picks the
 // single-launch chunk.
-void boundary_slabs(const BlockH& b, int* lo, int* hi, int* off, int& noghost);
+void boundary_slabs(const BlockH& b, int* lo, int* hi, int* off, int& noghost, bool stored_ghosts);
 
 int stage_args_for(const BlockH& b, int stage, double dt, unsigned int* counters, int kchunk, e4::StageArgs& A,
                    dim3& grid)
@@ -324,7 +324,7 @@ int stage_args_for(const BlockH& b, int stage, double dt, unsigned int* counters
     }
     A.dt = dt; A.g1 = g1; A.g2 = g2; A.g3 = g3;
     A.c1 = dt * g1;
-    boundary_slabs(b, A.fix_lo, A.fix_hi, A.fix_off, A.noghost);
+    boundary_slabs(b, A.fix_lo, A.fix_hi, A.fix_off, A.noghost, true);
     for (int f = 0; f < 6; ++f) { A.gx[f] = b.gx[f]; A.gS[f] = b.gS[f]; }
     A.axi = b.axi;
     const int src = shock_fields(b, A);
@@ -338,14 +338,20 @@ int stage_args_for(const BlockH& b, int stage, double dt, unsigned int* counters
 
 // Cells the boundary kernel finishes: the layers beside faces whose flux is
 // not the interior code's (two for a wall without ghost data, one for a
-// simple-outflow-flux face), slab by slab.
-void boundary_slabs(const BlockH& b, int* lo, int* hi, int* off, int& noghost)
+// simple-outflow-flux face) and, with stored_ghosts (the ideal-gas stage
+// kernel, which does not read stored ghost states), two beside faces whose
+// ghosts are stored (static profile, x_order-1 extrapolation); slab by slab.
+void boundary_slabs(const BlockH& b, int* lo, int* hi, int* off, int& noghost, bool stored_ghosts)
 {
     static const int FM[3] = {E4HIP_FACE_WEST, E4HIP_FACE_SOUTH, E4HIP_FACE_BOTTOM};
     static const int FP[3] = {E4HIP_FACE_EAST, E4HIP_FACE_NORTH, E4HIP_FACE_TOP};
     const int n[3] = {b.ni, b.nj, (b.dims == 3) ? b.nk : 1};
     auto layers = [&](int f) {
-        return (b.bc_type[f] == E4HIP_BC_SLIP_WALL_1) ? 2 : (b.bc_type[f] == E4HIP_BC_OUTFLOW_SIMPLE_FLUX ? 1 : 0);
+        const int t = b.bc_type[f];
+        if (t == E4HIP_BC_SLIP_WALL_1) return 2;
+        if (t == E4HIP_BC_OUTFLOW_SIMPLE_FLUX) return 1;
+        if (stored_ghosts && (t == E4HIP_BC_INFLOW_PROFILE || t == E4HIP_BC_EXTRAPOLATE_1)) return 2;
+        return 0;
     };
     noghost = 0;
     for (int d = 0; d < 3; ++d) {
@@ -549,7 +555,7 @@ int tp_args_for(const BlockH& b, int stage, double dt, unsigned int* counters, e
     // (wall and the face after it; a simple-outflow-flux face)
     {
         int lo[3], hi[3], off[7];
-        boundary_slabs(b, lo, hi, off, A.noghost);
+        boundary_slabs(b, lo, hi, off, A.noghost, false);
         const int n[3] = {b.ni, b.nj, (b.dims == 3) ? b.nk : 1};
         int total = 0;
         for (int sl = 0; sl < 6; ++sl) {
@@ -671,13 +677,9 @@ int upload_bc(const BlockH& b)
     }
     const GasModelH& g = g_state.gases[b.gm];
     if (g.type == GAS_IDEAL) {
-        // the constructed cell's sound speed, sqrt(gamma R T) at 300 K
-        // (ideal_gas.d:137-144), and gamma(Q) = Cp/Cv (gas_model.d:162)
-        t.a_init = sqrt(g.ideal.gammaR * 300.0);
+        // gamma(Q) = Cp/Cv (gas_model.d:162) for the wall flux
         const double Cp = g.ideal.Rgas * g.ideal.gamma / (g.ideal.gamma - 1.0);
         t.wall_gamma = Cp / g.ideal.Cv;
-    } else {
-        t.a_init = g.a_init;
     }
     // linear_extrap_prepare(len, len) (onedinterp.d:229-235)
     const double len[3] = {b.dx, b.dy, b.dz};
@@ -986,16 +988,19 @@ int e4hip_block_set_bc(int blk, int face, int type, const double* fs)
         if (!(P.rho > 0) || !(P.u > 0) || !(P.a > 0)) return fail(E4HIP_ERR_ARG, "invalid inflow state");
         b->bc_fs[face] = P;
     }
-    if (type == E4HIP_BC_EXTRAPOLATE_1 && b->is_tp()) {
+    if (type == E4HIP_BC_EXTRAPOLATE_1) {
         // the face's two ghost layers, holding the constructed cell state
-        // (p = 100 kPa, T = 300 K, first species, at rest) until the first fill
+        // (p = 100 kPa, T = 300 K, first species, at rest) until the first
+        // fill; the fill does not write the sound speed
         std::vector<double> h((size_t)np * 2 * nfc, 0.0);
         const GasModelH& g = g_state.gases[b->gm];
         auto fill = [&](int field, double v) { std::fill(h.begin() + field * 2 * nfc, h.begin() + (field + 1) * 2 * nfc, v); };
         fill(e4::P_T, 300.0);
         fill(e4::P_P, 100.0e3);
-        fill(e4::P_A, g.a_init);
-        fill(e4::P_RHO, 100.0e3 / (g.tp.R[0] * 300.0));
+        // the constructed cell's sound speed (ideal: sqrt(gamma R T) at 300 K,
+        // ideal_gas.d:137-144)
+        fill(e4::P_A, b->is_tp() ? g.a_init : sqrt(g.ideal.gammaR * 300.0));
+        fill(e4::P_RHO, 100.0e3 / ((b->is_tp() ? g.tp.R[0] : g.ideal.Rgas) * 300.0));
         fill(e4::P_Y, 1.0);
         int rc = ghost_buffer(h);
         if (rc) return rc;
@@ -1237,21 +1242,29 @@ int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_ce
         g_state.stage_pin.assign(g_state.blocks.size(), nullptr);
         for (size_t h = 0; h < g_state.blocks.size(); ++h)
             if (g_state.blocks[h].alive) g_state.stage_pin[h] = g_state.blocks[h].P[0];
-        // ghost cells that are stored rather than read through: the mixture
-        // blocks' x_order-1 extrapolation (the rest is done on the fly)
+        // ghost cells that are stored rather than read through: the x_order-1
+        // extrapolation (the other effects are applied on the fly)
         for (int n = 0; n < nblk; ++n) {
             BlockH& b = *get_block(blks[n]);
             int faces = 0;
             for (int f = 0; f < 6; ++f)
                 if (b.gx[f] && b.bc_type[f] == E4HIP_BC_EXTRAPOLATE_1) faces |= 1 << f;
             if (!faces) continue;
-            e4::TPArgs A;
-            rc = tp_args_for(b, stage, dt, nullptr, A);
-            if (rc) return rc;
-            if (g_state.cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT || !tp_mixture_form(g_state.gases[b.gm].tp))
-                e4::launch_tp_extrap1_exact(A, faces, g_state.stream);
-            else
-                e4::launch_tp_extrap1_fast64(A, faces, g_state.stream);
+            const bool exact = g_state.cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT;
+            if (b.is_tp()) {
+                e4::TPArgs A;
+                rc = tp_args_for(b, stage, dt, nullptr, A);
+                if (rc) return rc;
+                if (exact || !tp_mixture_form(g_state.gases[b.gm].tp)) e4::launch_tp_extrap1_exact(A, faces, g_state.stream);
+                else e4::launch_tp_extrap1_fast64(A, faces, g_state.stream);
+            } else {
+                e4::StageArgs A;
+                dim3 grid;
+                rc = stage_args_for(b, stage, dt, nullptr, 0, A, grid);
+                if (rc) return rc;
+                if (exact) e4::launch_extrapolate_ghosts_exact(A, faces, g_state.stream);
+                else e4::launch_extrapolate_ghosts_fast64(A, faces, g_state.stream);
+            }
             HIP_TRY(hipGetLastError());
         }
         // adaptive_hanel_ausmdv turns the shock detector on (globalconfig.d:1942-1954);

@@ -74,10 +74,6 @@ struct BCTable {
     Prim fs[6];                       // inflow ghost states (global frame)
     Connection nbr[6];                // connected faces
     double fs_tp[6][8 + TP_NSP_MAX];  // mixture inflow states in the primitive-array order (P_*)
-    // the sound speed of the flow state a cell is constructed with (p = 100
-    // kPa, T = 300 K, at rest, first species; fvcell.d:170), which x_order-1
-    // ghost cells keep (extrapolate_copy.d:102-186 does not write it)
-    double a_init;
     double wall_gamma;                // Cp/Cv of the ideal gas (gas_model.d:162) for the wall flux
     double ext_w[3][2];               // linear_extrap_prepare(len, len) weights per direction
 };
@@ -236,6 +232,9 @@ void launch_tp_decode_fast64(const TPArgs& A, const double* U, const double* Tgu
 void launch_mlp_guess(const GuessModel* G, const double* x, int n, double* T, hipStream_t stream);
 // the cells next to one-sided boundaries, after the stage kernel (A.fix_off[6] > 0)
 void launch_boundary_cells_exact(const StageArgs& A, hipStream_t stream, int dims, int flux);
+// x_order-1 ghost cells of an ideal-gas block's faces (face mask: bit f), at the start of a stage
+void launch_extrapolate_ghosts_exact(const StageArgs& A, int faces, hipStream_t stream);
+void launch_extrapolate_ghosts_fast64(const StageArgs& A, int faces, hipStream_t stream);
 void launch_boundary_cells_fast64(const StageArgs& A, hipStream_t stream, int dims, int flux);
 void launch_boundary_cells_mixed(const StageArgs& A, hipStream_t stream, int dims, int flux);
 

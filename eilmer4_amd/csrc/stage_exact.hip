@@ -15,6 +15,11 @@ void launch_stage_batch_exact(const BatchArgs& B, int nwg, bool def, hipStream_t
     launch_stage_batch_mode<ModeExact>(B, nwg, def, stream, dims, flux);
 }
 
+void launch_extrapolate_ghosts_exact(const StageArgs& A, int faces, hipStream_t stream)
+{
+    launch_extrapolate_ghosts_mode<ModeExact>(A, faces, stream);
+}
+
 void launch_boundary_cells_exact(const StageArgs& A, hipStream_t stream, int dims, int flux)
 {
     launch_boundary_cells_mode<ModeExact>(A, stream, dims, flux);

@@ -25,6 +25,11 @@ void launch_stage_batch_mixed(const BatchArgs& B, int nwg, bool def, hipStream_t
     launch_stage_batch_mode<ModeMixed>(B, nwg, def, stream, dims, flux);
 }
 
+void launch_extrapolate_ghosts_fast64(const StageArgs& A, int faces, hipStream_t stream)
+{
+    launch_extrapolate_ghosts_mode<ModeFast64>(A, faces, stream);
+}
+
 void launch_boundary_cells_fast64(const StageArgs& A, hipStream_t stream, int dims, int flux)
 {
     launch_boundary_cells_mode<ModeFast64>(A, stream, dims, flux);

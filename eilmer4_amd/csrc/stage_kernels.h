@@ -107,7 +107,11 @@ __device__ __forceinline__ long cell_index(KArgs A, int i, int j, int k)
 // shock point): interior cells carry this stage's indicator, ghost cells the
 // one of the cell they were copied from at the previous detection (the
 // ghost-cell fill precedes the detector, simcore.d:2338-2358), inflow ghosts 0.
-template <class M, bool SH = false>
+// STORED: ghost states stored per face (static profiles, x_order-1
+// extrapolation) are read; without it those faces give the slip-wall mirror,
+// which only faces of the cells the boundary kernel finishes read (the stage
+// kernel leaves two layers next to such faces to it).
+template <class M, bool SH = false, bool STORED = false>
 __device__ Prim fetch_ghost(KArgs A, int i, int j, int k)
 {
     Prim dummy{1.0, 0.0, 0.0, 0.0, 1.0, 1.0};
@@ -145,9 +149,11 @@ __device__ Prim fetch_ghost(KArgs A, int i, int j, int k)
         return P;
     }
     int si = i, sj = j, sk = k;
-    if (type == BC_INFLOW_PROFILE) {
-        // the stored ghost state (flow_state_copy_from_profile.d), with its
-        // profile point's shock indicator
+    if (STORED && (type == BC_INFLOW_PROFILE || type == BC_EXTRAPOLATE_1)) {
+        // stored ghost states: a static profile's (flow_state_copy_from_profile.d),
+        // with its points' shock indicators, or the x_order-1 extrapolation
+        // filled at the start of the stage (extrapolate_ghosts_kernel), whose
+        // indicator is 0
         const int na = (dir == 0) ? A->nj : A->ni;
         const int fa = (dir == 0) ? j : i, fb = (dir == 2) ? j : k;
         const long nfc = (long)na * ((dir == 2) ? A->nj : A->nk);
@@ -155,27 +161,8 @@ __device__ Prim fetch_ghost(KArgs A, int i, int j, int k)
         const double* G = A->gx[face] + g;
         const long gs = 2 * nfc;
         Prim P{G[P_RHO * gs], G[P_VX * gs], G[P_VY * gs], G[P_VZ * gs], G[P_U * gs], G[P_A * gs]};
-        if (SH && A->gS[face][g]) P.a = -(P.a);
+        if (SH && type == BC_INFLOW_PROFILE && A->gS[face][g]) P.a = -(P.a);
         return P;
-    }
-    if (type == BC_EXTRAPOLATE_1) {
-        // extrapolate_copy.d:102-186: linear in rho, u and velocity from the
-        // boundary cell c1 and the one behind it c2 (layer 2 from the first
-        // ghost and c1); the sound speed stays the constructed cell's and the
-        // shock indicator 0, which the extrapolation does not write
-        const int e = plus ? n - 1 : 0, in = plus ? -1 : 1;
-        auto cell = [&](int s) {
-            int ci = i, cj = j, ck = k;
-            if (dir == 0) ci = s; else if (dir == 1) cj = s; else ck = s;
-            return load_decode<M>(A, cell_index(A, ci, cj, ck));
-        };
-        auto lin = [&](const Prim& c1, const Prim& c2) {
-            return Prim{2.0 * c1.rho - c2.rho, 2.0 * c1.v0 - c2.v0, 2.0 * c1.v1 - c2.v1, 2.0 * c1.v2 - c2.v2,
-                        2.0 * c1.u - c2.u, A->bc->a_init};
-        };
-        const Prim c1 = cell(e);
-        const Prim g1 = lin(c1, cell(e + in));
-        return (layer == 1) ? g1 : lin(g1, c1);
     }
     int src;
     if (type == BC_EXTRAPOLATE || type == BC_OUTFLOW_SIMPLE_FLUX) src = plus ? n - 1 : 0;
@@ -194,7 +181,7 @@ __device__ Prim fetch_ghost(KArgs A, int i, int j, int k)
     return P;
 }
 
-template <class M, bool SH = false>
+template <class M, bool SH = false, bool STORED = false>
 __device__ __forceinline__ PrimT<typename M::R> fetch(KArgs A, int i, int j, int k)
 {
     using R = typename M::R;
@@ -204,7 +191,7 @@ __device__ __forceinline__ PrimT<typename M::R> fetch(KArgs A, int i, int j, int
         P = load_decode<M>(A, c);
         if (SH && A->S[c]) P.a = -(P.a);
     } else {
-        P = fetch_ghost<M, SH>(A, i, j, k);
+        P = fetch_ghost<M, SH, STORED>(A, i, j, k);
     }
     return PrimT<R>{R(P.rho), R(P.v0), R(P.v1), R(P.v2), R(P.u), R(P.a)};
 }
@@ -638,7 +625,7 @@ void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type A
         // stage update, decode check, write; all in FP64.  Cells next to a
         // one-sided boundary are the boundary kernel's.
         ap = args();
-        if (own_k && !left_to_boundary<DIMS>(ap, ik, jk, k)) {
+        if (own_k && !(ap->fix_off[6] != 0 && left_to_boundary<DIMS>(ap, ik, jk, k))) {
             const int iW = tyk * (TX + 1) + txk, jS = tyk * TX + txk;
             const int iE = iW + 1, jN = jS + TX;
             double integ[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -784,7 +771,7 @@ __device__ __forceinline__ void boundary_face_flux(KArgs A, int i, int j, int k,
     auto cellR = [&](int p) {
         int ci = i, cj = j, ck = k;
         along<D>(ci, cj, ck, p);
-        return fetch<M, SH>(A, ci, cj, ck);
+        return fetch<M, SH, true>(A, ci, cj, ck);
     };
 #pragma unroll
     for (int q = 0; q < 5; ++q) F[q] = 0.0;
@@ -919,6 +906,59 @@ __global__ __launch_bounds__(64) void boundary_cell_kernel(StageArgs)
     finish_cell<M>(A, cc, integ, A->vol_inv, 0.0);
 }
 
+// x_order-1 ghost cells of one face of an ideal-gas block
+// (extrapolate_copy.d:102-186), stored per face: layer 1 linear in rho, u and
+// velocity from the boundary cell c1 and the cell behind it, layer 2 from
+// layer 1 and c1; the sound speed stays the constructed cell's (the buffer's
+// initial value), which the extrapolation does not write.
+template <class M>
+__global__ __launch_bounds__(128) void extrapolate_ghosts_kernel(StageArgs, int face, double* G)
+{
+    KArgs A = kargs();
+    const int dir = (face == FACE_EAST || face == FACE_WEST) ? 0 : ((face == FACE_NORTH || face == FACE_SOUTH) ? 1 : 2);
+    const bool plus = (face == FACE_EAST || face == FACE_NORTH || face == FACE_TOP);
+    const int na = (dir == 0) ? A->nj : A->ni;
+    const long nfc = (long)na * ((dir == 2) ? A->nj : A->nk);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nfc) return;
+    const int fa = (int)(t % na), fb = (int)(t / na);
+    const int n = (dir == 0) ? A->ni : ((dir == 1) ? A->nj : A->nk);
+    auto cell = [&](int p) {
+        int ci, cj, ck;
+        if (dir == 0) { ci = p; cj = fa; ck = fb; }
+        else if (dir == 1) { ci = fa; cj = p; ck = fb; }
+        else { ci = fa; cj = fb; ck = p; }
+        return load_decode<M>(A, cell_index(A, ci, cj, ck));
+    };
+    const int e = plus ? n - 1 : 0, in = plus ? -1 : 1;
+    const Prim c1 = cell(e), c2 = cell(e + in);
+    const long gs = 2 * nfc;
+    auto put = [&](int layer, const Prim& a, const Prim& b) {
+        double* d = G + (layer - 1) * nfc + t;
+        d[P_RHO * gs] = 2.0 * a.rho - b.rho;
+        d[P_VX * gs] = 2.0 * a.v0 - b.v0;
+        d[P_VY * gs] = 2.0 * a.v1 - b.v1;
+        d[P_VZ * gs] = 2.0 * a.v2 - b.v2;
+        d[P_U * gs] = 2.0 * a.u - b.u;
+    };
+    put(1, c1, c2);
+    const Prim g1{2.0 * c1.rho - c2.rho, 2.0 * c1.v0 - c2.v0, 2.0 * c1.v1 - c2.v1, 2.0 * c1.v2 - c2.v2,
+                  2.0 * c1.u - c2.u, 0.0};
+    put(2, g1, c1);
+}
+
+template <class M>
+void launch_extrapolate_ghosts_mode(const StageArgs& A, int faces, hipStream_t stream)
+{
+    for (int f = 0; f < 6; ++f) {
+        if (!(faces & (1 << f))) continue;
+        const int dir = (f == FACE_EAST || f == FACE_WEST) ? 0 : ((f == FACE_NORTH || f == FACE_SOUTH) ? 1 : 2);
+        const long nfc = (long)((dir == 0) ? A.nj : A.ni) * ((dir == 2) ? A.nj : A.nk);
+        hipLaunchKernelGGL((extrapolate_ghosts_kernel<M>), dim3((unsigned)((nfc + 127) / 128)), dim3(128), 0, stream,
+                           A, f, const_cast<double*>(A.gx[f]));
+    }
+}
+
 template <class M>
 void launch_boundary_cells_mode(const StageArgs& A, hipStream_t stream, int dims, int flux)
 {
@@ -993,7 +1033,7 @@ __device__ __forceinline__ Prim prim_at(KArgs A, int i, int j, int k)
 {
     if (i >= 0 && i < A->ni && j >= 0 && j < A->nj && k >= 0 && k < A->nk)
         return load_decode<M>(A, cell_index(A, i, j, k));
-    return fetch_ghost<M, false>(A, i, j, k);
+    return fetch_ghost<M, false, true>(A, i, j, k);
 }
 
 __device__ __forceinline__ double prim_vel(const Prim& P, int d) { return d == 0 ? P.v0 : (d == 1 ? P.v1 : P.v2); }
