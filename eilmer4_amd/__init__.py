@@ -9,9 +9,9 @@ from .gas import IdealGas, ThermallyPerfectGas
 from .lib import E4hipError, load
 from .fluidblock import SFluidBlock, connect_blocks
 from .simcore import GlobalConfig, Simulation
-from . import bc, flowfiles
+from . import bc, flowfiles, udf
 
 __all__ = ["BCType", "Face", "FlowSolverException", "FluxCalculator", "GasdynamicUpdate", "InterpolateOption",
            "PrecisionMode", "flux_calculator_from_name", "flux_calculator_name", "update_scheme_from_name",
            "IdealGas", "ThermallyPerfectGas", "E4hipError", "load", "SFluidBlock", "connect_blocks", "GlobalConfig",
-           "Simulation", "bc", "flowfiles"]
+           "Simulation", "bc", "flowfiles", "udf"]

@@ -7,6 +7,7 @@ SFluidBlock.connect / connect_blocks.
     blk.set_bc(Face.east, OutFlowBC_Simple())
     blk.set_bc(Face.north, WallBC_WithSlip1())
     blk.set_bc(Face.west, InFlowBC_StaticProfile("profile.dat", gas, match="xyz-to-xyz"))
+    blk.set_bc(Face.south, UserDefinedBC(lambda t, x, y, z: {"p": p(x, y), "T": T(x, y)}))
 """
 from dataclasses import dataclass
 
@@ -68,5 +69,38 @@ def InFlowBC_StaticProfile(filename, gas, match="xyz-to-xyz", origin=(0.0, 0.0, 
                              lambda blk, face: profile_ghost_states(prof, blk, int(face), origin))
 
 
+def UserDefinedGhostCellBC(ghost_cell, t=0.0, origin=(0.0, 0.0, 0.0)):
+    """bc.lua:1286-1311 (UserDefinedBC): the ghost cells take the flow state a
+    user function gives at their centres (user_defined_effects.d:179-226:
+    p, T and massf, then update_thermo_from_pT and update_sound_speed, then
+    velx, vely, velz, each 0 when absent).  ghost_cell(t, x, y, z) returns a
+    dict of those names; it is evaluated once, at time t, when the condition
+    is set on a block (the reference calls its Lua function every stage)."""
+    def states(blk, face):
+        from .flowfiles import ghost_centres
+        g = ghost_centres(blk.dims, blk.ni, blk.nj, blk.nk, blk.dx, blk.dy, blk.dz, int(face), origin)
+        nsp = getattr(blk.gas, "n_species", 1)
+        out = np.empty(g.shape[:2] + (9 + nsp,))
+        for layer in range(g.shape[0]):
+            for n in range(g.shape[1]):
+                x, y, z = g[layer, n]
+                d = ghost_cell(t, x, y, z)
+                vel = (d.get("velx", 0.0), d.get("vely", 0.0), d.get("velz", 0.0))
+                if nsp > 1:
+                    Y = d.get("massf")
+                    Y = np.zeros(nsp) if Y is None else np.asarray(Y, dtype=np.float64)
+                    fs = blk.gas.flowstate(d["p"], d["T"], vel, Y)
+                else:
+                    fs = blk.gas.flowstate(d["p"], d["T"], vel)[:8]
+                    fs = np.concatenate([fs, [1.0]])
+                out[layer, n] = np.concatenate([fs[:8], [0.0], fs[8:]])
+        return out
+    return BoundaryCondition(BCType.inflow_profile, states)
+
+
+UserDefinedBC = UserDefinedGhostCellBC      # bc.lua:1311
+
+
 __all__ = ["BoundaryCondition", "WallBC_WithSlip0", "WallBC_WithSlip", "WallBC_WithSlip1", "InFlowBC_Supersonic",
-           "OutFlowBC_SimpleExtrapolate", "OutFlowBC_SimpleFlux", "OutFlowBC_Simple", "InFlowBC_StaticProfile"]
+           "OutFlowBC_SimpleExtrapolate", "OutFlowBC_SimpleFlux", "OutFlowBC_Simple", "InFlowBC_StaticProfile",
+           "UserDefinedGhostCellBC", "UserDefinedBC"]

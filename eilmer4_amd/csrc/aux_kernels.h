@@ -10,12 +10,13 @@ __device__ __forceinline__ unsigned int wave_count_aux(bool pred)
     return (unsigned int)__popcll(__ballot(pred));
 }
 
-// CFL signal (fvcell.d:1624-1661, non-stringent) and its maximum over the
-// block; the maximum of a non-negative double is taken on its bit pattern.
+// CFL signal (fvcell.d:1624-1661; stringent: the fastest normal speed over
+// the shortest cell length) and its maximum over the block; the maximum of a
+// non-negative double is taken on its bit pattern.
 struct CflArgs {
     const double* __restrict__ U;
     long qs;
-    int ni, nj, nk, dims;
+    int ni, nj, nk, dims, stringent;
     double iLength, jLength, kLength;
     IdealGas gas;
     unsigned long long* smax_bits;
@@ -35,13 +36,23 @@ __global__ __launch_bounds__(256) void cfl_kernel(CflArgs A)
         double un_N = fabs(P.v1);
         double un_E = fabs(P.v0);
         double un_T = (A.dims == 3) ? fabs(P.v2) : 0.0;
-        double signalN = (un_N + P.a) / A.jLength;
-        s = fmax(s, signalN);
-        double signalE = (un_E + P.a) / A.iLength;
-        s = fmax(s, signalE);
-        if (A.dims == 3) {
-            double signalT = (un_T + P.a) / A.kLength;
-            s = fmax(s, signalT);
+        if (A.stringent) {
+            double un_max = fmax(un_N, un_E);
+            double minLength = fmin(A.iLength, A.jLength);
+            if (A.dims == 3) {
+                un_max = fmax(un_max, un_T);
+                minLength = fmin(minLength, A.kLength);
+            }
+            s = (un_max + P.a) / minLength;
+        } else {
+            double signalN = (un_N + P.a) / A.jLength;
+            s = fmax(s, signalN);
+            double signalE = (un_E + P.a) / A.iLength;
+            s = fmax(s, signalE);
+            if (A.dims == 3) {
+                double signalT = (un_T + P.a) / A.kLength;
+                s = fmax(s, signalT);
+            }
         }
         if (c == ncells - 1) *A.last_signal = s;
         signal = fmax(signal, s);
@@ -100,7 +111,7 @@ __global__ void reset_counters_kernel(unsigned int* counters, int n, unsigned lo
 
 // CFL signal from the primitive array (fvcell.d:1624-1661).
 __global__ __launch_bounds__(256) void tp_cfl_kernel(const double* __restrict__ P, long qs, int ni, int nj, int nk,
-                                                     int dims, double iL, double jL, double kL,
+                                                     int dims, int stringent, double iL, double jL, double kL,
                                                      unsigned long long* smax_bits, double* last_signal)
 {
     __shared__ double wave_max[4];
@@ -109,9 +120,19 @@ __global__ __launch_bounds__(256) void tp_cfl_kernel(const double* __restrict__ 
     for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += (long)gridDim.x * blockDim.x) {
         double a = P[P_A * qs + c];
         double s = 0.0;
-        s = fmax(s, (fabs(P[P_VY * qs + c]) + a) / jL);
-        s = fmax(s, (fabs(P[P_VX * qs + c]) + a) / iL);
-        if (dims == 3) s = fmax(s, (fabs(P[P_VZ * qs + c]) + a) / kL);
+        if (stringent) {
+            double un_max = fmax(fabs(P[P_VY * qs + c]), fabs(P[P_VX * qs + c]));
+            double minLength = fmin(iL, jL);
+            if (dims == 3) {
+                un_max = fmax(un_max, fabs(P[P_VZ * qs + c]));
+                minLength = fmin(minLength, kL);
+            }
+            s = (un_max + a) / minLength;
+        } else {
+            s = fmax(s, (fabs(P[P_VY * qs + c]) + a) / jL);
+            s = fmax(s, (fabs(P[P_VX * qs + c]) + a) / iL);
+            if (dims == 3) s = fmax(s, (fabs(P[P_VZ * qs + c]) + a) / kL);
+        }
         if (c == ncells - 1) *last_signal = s;
         sig = fmax(sig, s);
     }

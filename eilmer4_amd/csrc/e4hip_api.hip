@@ -65,6 +65,7 @@ struct BlockH {
     double* gx[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     unsigned char* gS[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double* axi = nullptr;                        // 2D axisymmetric geometry (e4::axi_geom layout)
+    double* src = nullptr;                        // user-defined source terms [nq][qs]
     // shock indicators per cell (adaptive_hanel_ausmdv): S[0] this stage's
     // detection, S[1] the previous one
     unsigned char* S[2] = {nullptr, nullptr};
@@ -327,6 +328,7 @@ int stage_args_for(const BlockH& b, int stage, double dt, unsigned int* counters
     boundary_slabs(b, A.fix_lo, A.fix_hi, A.fix_off, A.noghost, true);
     for (int f = 0; f < 6; ++f) { A.gx[f] = b.gx[f]; A.gS[f] = b.gS[f]; }
     A.axi = b.axi;
+    A.src = b.src;
     const int src = shock_fields(b, A);
     if (src) return src;
     const int nk = A.nk;
@@ -569,6 +571,7 @@ int tp_args_for(const BlockH& b, int stage, double dt, unsigned int* counters, e
     }
     for (int f = 0; f < 6; ++f) { A.gx[f] = b.gx[f]; A.gS[f] = b.gS[f]; }
     A.axi = b.axi;
+    A.src = b.src;
     return shock_fields(b, A);
 }
 
@@ -661,6 +664,7 @@ int free_block_buffers(BlockH& b)
     for (auto& p : b.gx) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     for (auto& p : b.gS) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     if (b.axi) { HIP_TRY(hipFree(b.axi)); b.axi = nullptr; }
+    if (b.src) { HIP_TRY(hipFree(b.src)); b.src = nullptr; }
     if (b.bc_dev) { HIP_TRY(hipFree(b.bc_dev)); b.bc_dev = nullptr; }
     return E4HIP_OK;
 }
@@ -1059,6 +1063,27 @@ int e4hip_block_set_axisymmetric(int blk, double x0, double y0)
     return E4HIP_OK;
 }
 
+int e4hip_block_set_source_terms(int blk, const double* Q)
+{
+    BlockH* b = get_block(blk);
+    if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
+    HIP_TRY(hipStreamSynchronize(g_state.stream));
+    if (!Q) {
+        if (b->src) { HIP_TRY(hipFree(b->src)); b->src = nullptr; }
+        return E4HIP_OK;
+    }
+    for (long i = 0; i < (long)b->nq * b->ncells; ++i)
+        if (!std::isfinite(Q[i])) return fail(E4HIP_ERR_ARG, "source term %ld is not finite", i);
+    if (!b->src) {
+        HIP_TRY(hipMalloc(&b->src, sizeof(double) * (size_t)b->nq * b->qs));
+        HIP_TRY(hipMemset(b->src, 0, sizeof(double) * (size_t)b->nq * b->qs));
+    }
+    for (int q = 0; q < b->nq; ++q)
+        HIP_TRY(hipMemcpy(b->src + (size_t)q * b->qs, Q + (size_t)q * b->ncells, sizeof(double) * b->ncells,
+                          hipMemcpyHostToDevice));
+    return E4HIP_OK;
+}
+
 int e4hip_block_connect(int blk, int face, int other, int other_face, int orientation)
 {
     BlockH* b = get_block(blk);
@@ -1350,6 +1375,7 @@ int e4hip_determine_dt(const int* blks, int nblk, double dt_current, double cfl_
                            g_state.smax);
         e4::CflArgs C;
         C.U = b->U[0]; C.qs = b->qs; C.ni = b->ni; C.nj = b->nj; C.nk = b->nk; C.dims = b->dims;
+        C.stringent = g_state.cfg.stringent_cfl != 0;
         C.iLength = b->dx; C.jLength = b->dy; C.kLength = b->dz;
         C.gas = g_state.gases[b->gm].ideal;
         C.smax_bits = g_state.smax; C.last_signal = g_state.last_signal;
@@ -1359,7 +1385,8 @@ int e4hip_determine_dt(const int* blks, int nblk, double dt_current, double cfl_
         if (nwg > 4096) nwg = 4096;
         if (b->is_tp())
             hipLaunchKernelGGL(e4::tp_cfl_kernel, dim3((unsigned)nwg), dim3(256), 0, g_state.stream, b->P[0], b->qs,
-                               b->ni, b->nj, b->nk, b->dims, b->dx, b->dy, b->dz, g_state.smax, g_state.last_signal);
+                               b->ni, b->nj, b->nk, b->dims, C.stringent, b->dx, b->dy, b->dz, g_state.smax,
+                               g_state.last_signal);
         else
             hipLaunchKernelGGL(e4::cfl_kernel<e4::ModeExact>, dim3((unsigned)nwg), dim3(256), 0, g_state.stream, C);
         prof_end(1, ev);

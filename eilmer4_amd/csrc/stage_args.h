@@ -127,6 +127,9 @@ struct StageArgs {
     // (x,y)-plane areas [ni*nj], i-face areas [(ni+1)*nj], j-face areas
     // [ni*(nj+1)], one array after the other
     const double* axi;
+    // user-defined source terms per cell, constant in time, in U's layout
+    // (null: none); added to the time derivative after the axisymmetric term
+    const double* src;
 };
 
 // Per-radian geometry of a 2D axisymmetric block, laid out as above.
@@ -200,6 +203,7 @@ struct TPArgs {
     double* gx[6];
     const unsigned char* gS[6];
     const double* axi;                // 2D axisymmetric geometry, as in StageArgs
+    const double* src;                // user-defined source terms, as in StageArgs
 };
 
 // Launchers, one per arithmetic mode; each lives in its own translation unit

@@ -363,7 +363,8 @@ __device__ __forceinline__ void lds_put_flux(R* dst, int nslot, int s, const R F
 // The rest of a stage for one cell from its flux integral (the sum over its
 // faces of -F * outsign * area), its inverse volume and the y-momentum source
 // (the axisymmetric pressure term, 0 for planar flow): time derivative
-// (fvcell.d:826-832), stage update, decode checks
+// (fvcell.d:826-832) with the block's user-defined source terms added to the
+// source vector after that term (simcore.d:2442-2465), stage update, decode checks
 // with the low-T fallback (fvcell.d:570-589, 680-697), check_data
 // (flowstate.d:345-388, gas_state.d:199-241), write, and the wave's counts of
 // invalid cells and decode failures.  All in FP64.
@@ -371,10 +372,13 @@ template <class M>
 __device__ __forceinline__ void finish_cell(KArgs ap, long c, const double integ[5], double vol_inv, double qy)
 {
     const long qs = opaque_qs(ap->qs);
+    const double* __restrict__ src = ap->src;
     double Un[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-        double d = vol_inv * integ[q] + ((q == 2) ? qy : 0.0);
+        double Q = (q == 2) ? qy : 0.0;
+        if (src) Q += src[c + q * qs];
+        double d = vol_inv * integ[q] + Q;
         double dd0 = (ap->stage >= 2) ? ap->d0[c + q * qs] : 0.0;
         double dd1 = (ap->stage >= 3) ? ap->d1[c + q * qs] : 0.0;
         if (ap->dout) ap->dout[c + q * qs] = d;

@@ -471,7 +471,9 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
         }
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            double d = vol_inv * integ[q] + ((q == 2) ? qy : 0.0);
+            double Qs = (q == 2) ? qy : 0.0;
+            if (A.src) Qs += A.src[c + q * qs];   // user-defined source terms
+            double d = vol_inv * integ[q] + Qs;
             double dd0 = (A.stage >= 2) ? A.d0[c + q * qs] : 0.0;
             double dd1 = (A.stage >= 3) ? A.d1[c + q * qs] : 0.0;
             if (A.dout) A.dout[c + q * qs] = d;

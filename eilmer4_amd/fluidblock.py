@@ -65,6 +65,18 @@ class SFluidBlock:
         _lib.check(_lib.load().e4hip_block_set_axisymmetric(self.id, float(x0), float(y0)))
         self.axisymmetric = (float(x0), float(y0))
 
+    def set_source_terms(self, Q):
+        """User-defined source terms (config.udf_source_terms): Q [nq][cells]
+        in the conserved order (mass, x/y/z momentum, total energy, species
+        densities), held constant until the next call; None removes them."""
+        if Q is None:
+            _lib.check(_lib.load().e4hip_block_set_source_terms(self.id, None))
+            return
+        Qv = np.ascontiguousarray(Q, dtype=np.float64)
+        if Qv.size != self.nq * self.ncells:
+            raise ValueError("source terms need [%d][%d] values" % (self.nq, self.ncells))
+        _lib.check(_lib.load().e4hip_block_set_source_terms(self.id, _p(Qv)))
+
     def connect(self, face, other, other_face, orientation=0):
         """One side of a full-face connection: the ghost cells of `face` copy
         the mapped interior cells of `other` next to `other_face` at every

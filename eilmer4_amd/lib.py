@@ -18,7 +18,7 @@ ERRORS = {-1: "E4HIP_ERR_ARG", -2: "E4HIP_ERR_HIP", -3: "E4HIP_ERR_HANDLE", -4: 
 EXPORTS = [
     "e4hip_init", "e4hip_finalize", "e4hip_last_error", "e4hip_stream", "e4hip_synchronize",
     "e4hip_gas_ideal", "e4hip_gas_tp", "e4hip_block_create", "e4hip_block_destroy", "e4hip_block_set_bc",
-    "e4hip_block_connect", "e4hip_block_set_axisymmetric",
+    "e4hip_block_connect", "e4hip_block_set_axisymmetric", "e4hip_block_set_source_terms",
     "e4hip_block_upload", "e4hip_block_download", "e4hip_block_ncells", "e4hip_config",
     "e4hip_explicit_increment", "e4hip_determine_dt", "e4hip_profile_enable", "e4hip_profile_read",
     "e4hip_gas_tp_set_guess_model", "e4hip_tp_guess", "e4hip_tp_decode_stats", "e4hip_tp_face_stats",
@@ -39,7 +39,8 @@ class StepConfig(C.Structure):
                 ("extrema_clipping", C.c_int), ("interpolate_in_local_frame", C.c_int),
                 ("thermo_interpolator", C.c_int), ("precision_mode", C.c_int),
                 ("max_invalid_cells", C.c_int), ("learned_eos_guess", C.c_int), ("batch_blocks", C.c_int),
-                ("shear_tolerance", C.c_double), ("compression_tolerance", C.c_double)]
+                ("shear_tolerance", C.c_double), ("compression_tolerance", C.c_double),
+                ("stringent_cfl", C.c_int)]
 
 
 _lib = None
@@ -66,6 +67,7 @@ def load(path=LIB_PATH):
     L.e4hip_block_set_bc.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
     L.e4hip_block_connect.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
     L.e4hip_block_set_axisymmetric.argtypes = [C.c_int, C.c_double, C.c_double]
+    L.e4hip_block_set_source_terms.argtypes = [C.c_int, _dp]
     L.e4hip_block_upload.argtypes = [C.c_int, _dp, _dp]
     L.e4hip_block_download.argtypes = [C.c_int, _dp, _dp]
     L.e4hip_block_ncells.argtypes = [C.c_int, C.POINTER(C.c_long), _ip]

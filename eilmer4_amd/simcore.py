@@ -42,6 +42,7 @@ class GlobalConfig:
     max_time: float = 1.0e-3
     max_step: int = 100
     fixed_time_step: bool = False
+    stringent_cfl: bool = False             # globalconfig.d:729
 
     def step_config(self):
         if FluxCalculator(self.flux_calculator) not in DEVICE_FLUX_CALCULATORS:
@@ -52,7 +53,8 @@ class GlobalConfig:
                                int(bool(self.extrema_clipping)), int(bool(self.interpolate_in_local_frame)),
                                int(self.thermo_interpolator), int(self.precision_mode), int(self.max_invalid_cells),
                                int(bool(self.learned_eos_guess)), int(bool(self.batch_blocks)),
-                               float(self.shear_tolerance), float(self.compression_tolerance))
+                               float(self.shear_tolerance), float(self.compression_tolerance),
+                               int(bool(self.stringent_cfl)))
 
 
 class SimState:

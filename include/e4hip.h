@@ -120,6 +120,9 @@ typedef struct E4hipStepConfig {
                                         are two or more; 0: one launch per block */
     double shear_tolerance;          /* adaptive_hanel_ausmdv: default 0.20 (globalconfig.d:585) */
     double compression_tolerance;    /* shock detector: default -0.30 (globalconfig.d:600) */
+    int stringent_cfl;               /* CFL signal from the fastest normal velocity over the
+                                        shortest cell length (fvcell.d:1642-1650); default 0
+                                        (globalconfig.d:729) */
 } E4hipStepConfig;
 
 /* Library set-up on one GPU; creates the library's HIP stream.
@@ -212,6 +215,15 @@ int e4hip_block_connect(int blk, int face, int other, int other_face, int orient
  * source p * areaxy / volume of the y-momentum (add_inviscid_source_vector,
  * fvcell.d:2177-2181).  Planar blocks need no call. */
 int e4hip_block_set_axisymmetric(int blk, double x0, double y0);
+/* User-defined source terms of a block (GlobalConfig.udf_source_terms;
+ * addUDFSourceTermsToCell, user_defined_source_terms.d:23-56, called from
+ * simcore.d:2445-2464): Q is [nq][ncells] in the conserved order (mass, x/y/z
+ * momentum, total energy, then the species densities of a mixture), added to
+ * each cell's source vector after the axisymmetric term at every stage.  The
+ * reference calls its Lua function at every stage with the cell's state; here
+ * the values are held constant until the next call (a manufactured-solution
+ * source, or a caller refreshing them between steps).  NULL removes them. */
+int e4hip_block_set_source_terms(int blk, const double* Q);
 /* Host -> device copy of U[0] ([nq][ncells]); the flow state is the decode of
  * U (fvcell.d:563-732).  For a thermally perfect gas, Tguess[ncells] is the
  * temperature the Newton solve starts from (the flow state's T on restart);

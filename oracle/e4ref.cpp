@@ -1105,6 +1105,9 @@ struct Block {
     // areas [ni*nj], i-face areas [(ni+1)*nj], j-face areas [ni*(nj+1)]
     bool axi = false;
     std::vector<double> axi_vol, axi_areaxy, axi_ai, axi_aj;
+    // user-defined source terms per cell slot [slot * nq + q] (empty: none)
+    std::vector<double> udf_src;
+    bool stringent_cfl = false;     // GlobalConfig.stringent_cfl (globalconfig.d:729)
 };
 
 static inline size_t cidx(const Block* b, int i, int j, int k) { return ((size_t)k * b->njdim + j) * b->nidim + i; }
@@ -1643,9 +1646,11 @@ static void convective_flux(Block* b)
     }
 }
 
-// fvcell.d:735-875 with add_inviscid_source_vector (fvcell.d:2156-2199): the
-// only source on this path is the axisymmetric pressure term of the y-momentum
-// equation, p * areaxy / volume.
+// fvcell.d:735-875 with the source vector Q built as simcore.d:2442-2465
+// builds it: cleared, then add_inviscid_source_vector (fvcell.d:2156-2199;
+// the axisymmetric pressure term of the y-momentum equation, p * areaxy /
+// volume), then the user-defined source terms of the cell
+// (user_defined_source_terms.d:23-56 adds them to Q); dUdt = integral / V + Q.
 static void time_derivatives(Block* b, size_t c, double* dUdt)
 {
     const int nq = b->nq;
@@ -1682,7 +1687,29 @@ static void time_derivatives(Block* b, size_t c, double* dUdt)
         for (int q = 5; q < nq; ++q) integral[q] -= myF[q] * area;
     }
     double vol_inv = 1.0 / volume;
-    for (int q = 0; q < nq; ++q) dUdt[q] = vol_inv * integral[q] + ((q == 2) ? Qy : 0.0);
+    const double* udf = b->udf_src.empty() ? nullptr : &b->udf_src[c * nq];
+    for (int q = 0; q < nq; ++q) {
+        double Q = (q == 2) ? Qy : 0.0;
+        if (udf) Q += udf[q];
+        dUdt[q] = vol_inv * integral[q] + Q;
+    }
+}
+
+// User-defined source terms, constant in time: Q [nq][ncells] in the block's
+// conserved order (mass, x/y/z momentum, total energy, then species
+// densities), or NULL for none.
+void ref_block_set_source_terms(Block* b, const double* Q)
+{
+    if (!Q) { b->udf_src.clear(); return; }
+    const size_t nc = (size_t)b->ni * b->nj * b->nk;
+    b->udf_src.assign(b->U[0].size(), 0.0);
+    size_t c = 0;
+    for (int k = b->kmin; k <= b->kmax; ++k)
+        for (int j = b->jmin; j <= b->jmax; ++j)
+            for (int i = b->imin; i <= b->imax; ++i, ++c) {
+                const size_t ci = cidx(b, i, j, k);
+                for (int q = 0; q < b->nq; ++q) b->udf_src[ci * b->nq + q] = Q[q * nc + c];
+            }
 }
 
 // Axisymmetric geometry of a 2D block whose lower-left vertex is (x0, y0),
@@ -1858,13 +1885,24 @@ void ref_block_dt(Block* b, double dt_current, double cfl_value, int check_cfl, 
                 double un_E = fabs(v[0] * fE.n[0] + v[1] * fE.n[1] + v[2] * fE.n[2]);
                 double un_T = three_d ? fabs(v[0] * fT.n[0] + v[1] * fT.n[1] + v[2] * fT.n[2]) : 0.0;
                 signal = 0;
-                double signalN = (un_N + fs.gas.a) / jLength;
-                signal = fmax(signal, signalN);
-                double signalE = (un_E + fs.gas.a) / iLength;
-                signal = fmax(signal, signalE);
-                if (three_d) {
-                    double signalT = (un_T + fs.gas.a) / kLength;
-                    signal = fmax(signal, signalT);
+                if (b->stringent_cfl) {
+                    // fvcell.d:1642-1650: the fastest normal speed over the shortest length
+                    double un_max = fmax(un_N, un_E);
+                    double minLength = fmin(iLength, jLength);
+                    if (three_d) {
+                        un_max = fmax(un_max, un_T);
+                        minLength = fmin(minLength, kLength);
+                    }
+                    signal = (un_max + fs.gas.a) / minLength;
+                } else {
+                    double signalN = (un_N + fs.gas.a) / jLength;
+                    signal = fmax(signal, signalN);
+                    double signalE = (un_E + fs.gas.a) / iLength;
+                    signal = fmax(signal, signalE);
+                    if (three_d) {
+                        double signalT = (un_T + fs.gas.a) / kLength;
+                        signal = fmax(signal, signalT);
+                    }
                 }
                 smax = fmax(smax, signal);
                 double cfl_local = dt_current * signal;
@@ -1881,5 +1919,7 @@ void ref_block_dt(Block* b, double dt_current, double cfl_value, int check_cfl, 
 }
 
 int ref_block_nq(const Block* b) { return b->nq; }
+
+void ref_block_set_stringent_cfl(Block* b, int on) { b->stringent_cfl = (on != 0); }
 
 } // extern "C"

@@ -69,6 +69,8 @@ def lib():
         L.ref_block_set_bc.argtypes = [vp, C.c_int, C.c_int, _dp]
         L.ref_block_set_bc_profile.argtypes = [vp, C.c_int, _dp]
         L.ref_block_set_axisymmetric.argtypes = [vp, C.c_double, C.c_double]
+        L.ref_block_set_source_terms.argtypes = [vp, _dp]
+        L.ref_block_set_stringent_cfl.argtypes = [vp, C.c_int]
         L.ref_block_set_from_pT.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.ref_block_set_U.argtypes = [vp, _dp, _dp]
         L.ref_block_get_U.argtypes = [vp, C.c_int, _dp]
@@ -233,6 +235,14 @@ class Block:
     def set_axisymmetric(self, x0=0.0, y0=0.0):
         """2D axisymmetric geometry, lower-left vertex at (x0, y0)."""
         lib().ref_block_set_axisymmetric(self.h, x0, y0)
+
+    def set_source_terms(self, Q):
+        """User-defined source terms, constant in time: [nq][cells] in the
+        conserved order, or None."""
+        lib().ref_block_set_source_terms(self.h, _p(_f64(Q).reshape(self.nq, -1)) if Q is not None else None)
+
+    def set_stringent_cfl(self, on=True):
+        lib().ref_block_set_stringent_cfl(self.h, int(bool(on)))
 
     def set_shock_tolerances(self, shear_tolerance=0.20, compression_tolerance=-0.30):
         lib().ref_block_set_shock_tolerances(self.h, shear_tolerance, compression_tolerance)
