@@ -177,8 +177,12 @@ def density_norms(blocks_rho, layout, h, case=1):
             "Linf": float(np.max(np.abs(e)))}
 
 
-def run_oracle(po, case=1, ncells=16, max_time=60.0e-3, nib=2, njb=2):
-    """The case on the oracle: steps taken and the density error norms."""
+def run_oracle(po, case=1, ncells=16, max_time=60.0e-3, nib=2, njb=2, decode_initial=False):
+    """The case on the oracle: steps taken and the density error norms.
+    decode_initial: start from decode(encode(initial flow state)), as a device
+    block does after its upload, instead of the flow state itself (as the
+    reference's solver does after reading its initial flow file); the two
+    differ by rounding."""
     gas = po.Gas.ideal(mMass=MMASS, gamma=GAMMA)
     layout, h = block_layout(ncells, nib, njb)
     blks = {}
@@ -188,6 +192,8 @@ def run_oracle(po, case=1, ncells=16, max_time=60.0e-3, nib=2, njb=2):
                      extrema_clipping=0)
         b.set_stringent_cfl(True)
         b.set_from_pT(1.0e5, 1.0e5 / RGAS, 800.0, 800.0, 0.0)
+        if decode_initial:
+            b.set_U(b.get_U())
         b.set_source_terms(cell_sources(ni, nj, h, origin, case))
         if ib == nib - 1:
             b.set_bc(FACE_EAST, po.BC_EXTRAPOLATE_1)
