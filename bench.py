@@ -263,6 +263,53 @@ def accuracy_study(e4, workloads, sizes=(256, 384), checkpoints=(1, 100, 1000), 
     return out
 
 
+def graph_study(e4, nb=2, n=8, steps=2000):
+    """Per-step wall time of many small connected blocks with and without
+    graph replay (use_graphs): nb x nb 2D blocks of n x n cells on the unit
+    square, Mach-2.3 flow at 45 degrees, supersonic inflow west/south,
+    x_order-1 extrapolation north/east, ausmdv, pc, fixed dt."""
+    import time
+    gas = e4.IdealGas()
+    h = 1.0 / (nb * n)
+    fs = gas.flowstate(1.0e5, 300.0, (560.0, 560.0, 0.0))
+    out = {}
+    for graphs in (False, True, False, True):
+        blks = {}
+        for jb in range(nb):
+            for ib in range(nb):
+                b = e4.SFluidBlock(gas, 2, n, n, 1, h, h)
+                b.set_from_pT(1.0e5, 300.0, 560.0, 560.0, 0.0)
+                if ib == 0:
+                    b.set_bc(e4.Face.west, e4.BCType.inflow, fs)
+                if jb == 0:
+                    b.set_bc(e4.Face.south, e4.BCType.inflow, fs)
+                if ib == nb - 1:
+                    b.set_bc(e4.Face.east, e4.BCType.extrapolate_1)
+                if jb == nb - 1:
+                    b.set_bc(e4.Face.north, e4.BCType.extrapolate_1)
+                blks[(ib, jb)] = b
+        for (ib, jb), b in blks.items():
+            if ib + 1 < nb:
+                e4.connect_blocks(b, e4.Face.east, blks[(ib + 1, jb)], e4.Face.west)
+            if jb + 1 < nb:
+                e4.connect_blocks(b, e4.Face.north, blks[(ib, jb + 1)], e4.Face.south)
+        blocks = list(blks.values())
+        sim = e4.Simulation(blocks, e4.GlobalConfig(flux_calculator=e4.FluxCalculator.ausmdv, use_graphs=graphs))
+        sim.state.dt_global = 1.0e-6
+        for _ in range(20):
+            sim.gasdynamic_explicit_increment_with_fixed_grid()
+        sim.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.gasdynamic_explicit_increment_with_fixed_grid()
+        sim.synchronize()
+        out["graphs" if graphs else "eager"] = round(1.0e6 * (time.perf_counter() - t0) / steps, 1)
+        for b in blocks:
+            b.destroy()
+    return {"workload": "%d x %d 2D blocks of %d x %d cells, ausmdv, pc, fixed dt" % (nb, nb, n, n),
+            "us_per_step": out, "steps": steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,6 +326,8 @@ def main():
     ap.add_argument("--c5", type=int, default=1, help="run the batched multi-block workload (0: skip)")
     ap.add_argument("--accuracy-study", action="store_true",
                     help="only run the mixed-vs-FP64 study (256^3 and 384^3, 1/100/1000 steps)")
+    ap.add_argument("--graph-study", action="store_true",
+                    help="only time many small blocks with and without graph replay of steps")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM-counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -302,6 +351,10 @@ def main():
 
     L = e4.load()
     e4.lib.check(L.e4hip_init(0))
+    if args.graph_study:
+        print(json.dumps({"graph_study": [graph_study(e4, nb, n) for nb, n in ((2, 8), (4, 16), (8, 32))]}),
+              flush=True)
+        return
     if args.accuracy_study:
         print(json.dumps({"accuracy_study": accuracy_study(e4, workloads), "workload": "shocked box seed 20261203",
                           "vs": "plain FP64 device run, same dt sequence"}), flush=True)

@@ -104,12 +104,29 @@ struct State {
     long prof_n[2] = {0, 0};
     long long decode_evals = 0, decode_fallbacks = 0;
     long long face_evals = 0, face_fallbacks = 0;      // face temperature solves of the thermally perfect path
+    // graph replay of a step (use_graphs): the instantiated graph of the last
+    // captured step, the key of the state it was captured from (generation,
+    // blocks, dt, buffer pointers), the blocks' buffer pointers after it, and
+    // the key of the last step run without the graph
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uintptr_t> graph_key, graph_post, graph_last_key;
+    unsigned long long generation = 0;         // bumped by every call that changes blocks, boundaries or config
     std::string err;
 };
 
 State g_state;
 
 bool BlockH::is_tp() const { return g_state.gases[gm].type == GAS_TP; }
+
+// Any call that changes what a step's kernels are given drops the captured step.
+void graph_invalidate()
+{
+    ++g_state.generation;
+    if (g_state.graph_exec) { (void)hipGraphExecDestroy(g_state.graph_exec); g_state.graph_exec = nullptr; }
+    g_state.graph_key.clear();
+    g_state.graph_post.clear();
+    g_state.graph_last_key.clear();
+}
 
 int fail(int code, const char* fmt, ...)
 {
@@ -716,6 +733,7 @@ int e4hip_init(int device)
 
 int e4hip_finalize(void)
 {
+    graph_invalidate();
     if (!g_state.init) return E4HIP_OK;
     HIP_TRY(hipStreamSynchronize(g_state.stream));
     for (auto& b : g_state.blocks) if (b.alive) { free_block_buffers(b); b.alive = false; }
@@ -755,6 +773,7 @@ int e4hip_synchronize(void)
 
 int e4hip_gas_ideal(double mMass, double gamma, int* gm)
 {
+    graph_invalidate();
     if (!(mMass > 0.0) || !(gamma > 1.0) || !gm) return fail(E4HIP_ERR_ARG, "bad ideal-gas parameters");
     GasModelH g;
     g.mMass = mMass;
@@ -781,6 +800,7 @@ __global__ void tp_init_sound_speed_kernel(const e4::TPGas* g, double* a)
 int e4hip_gas_tp(int nsp, const double* M, int nseg, const double* Tbreaks, const double* Tblends,
                  const double* coeffs, int* gm)
 {
+    graph_invalidate();
     if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
     if (nsp != 5) return fail(E4HIP_ERR_ARG, "the device path is built for 5 species (got %d)", nsp);
     if (nseg < 1 || nseg > e4::TP_NSEG_MAX || !M || !Tbreaks || !Tblends || !coeffs || !gm)
@@ -828,6 +848,7 @@ int e4hip_gas_tp(int nsp, const double* M, int nseg, const double* Tbreaks, cons
 int e4hip_gas_tp_set_guess_model(int gm, const float* params, const float* in_mean, const float* in_std,
                                  float T_mean, float T_std)
 {
+    graph_invalidate();
     if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
     if (gm < 0 || gm >= (int)g_state.gases.size() || g_state.gases[gm].type != GAS_TP)
         return fail(E4HIP_ERR_HANDLE, "not a thermally perfect gas model: %d", gm);
@@ -899,6 +920,7 @@ int e4hip_tp_face_stats(long long* energy_evals, long long* fallbacks, int reset
 
 int e4hip_block_create(int gm, int dims, int ni, int nj, int nk, double dx, double dy, double dz, int* blk)
 {
+    graph_invalidate();
     if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
     if (gm < 0 || gm >= (int)g_state.gases.size()) return fail(E4HIP_ERR_HANDLE, "unknown gas model %d", gm);
     if (!blk || (dims != 2 && dims != 3)) return fail(E4HIP_ERR_ARG, "dims must be 2 or 3");
@@ -924,6 +946,7 @@ int e4hip_block_create(int gm, int dims, int ni, int nj, int nk, double dx, doub
 
 int e4hip_block_destroy(int blk)
 {
+    graph_invalidate();
     BlockH* b = get_block(blk);
     if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
     HIP_TRY(hipStreamSynchronize(g_state.stream));
@@ -934,6 +957,7 @@ int e4hip_block_destroy(int blk)
 
 int e4hip_block_set_bc(int blk, int face, int type, const double* fs)
 {
+    graph_invalidate();
     BlockH* b = get_block(blk);
     if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
     if (face < 0 || face > 5 || (b->dims == 2 && face > 3)) return fail(E4HIP_ERR_ARG, "bad face %d", face);
@@ -1016,6 +1040,7 @@ int e4hip_block_set_bc(int blk, int face, int type, const double* fs)
 
 int e4hip_block_set_axisymmetric(int blk, double x0, double y0)
 {
+    graph_invalidate();
     BlockH* b = get_block(blk);
     if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
     if (b->dims != 2) return fail(E4HIP_ERR_ARG, "axisymmetric geometry is for 2D blocks");
@@ -1065,6 +1090,7 @@ int e4hip_block_set_axisymmetric(int blk, double x0, double y0)
 
 int e4hip_block_set_source_terms(int blk, const double* Q)
 {
+    graph_invalidate();
     BlockH* b = get_block(blk);
     if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
     HIP_TRY(hipStreamSynchronize(g_state.stream));
@@ -1086,6 +1112,7 @@ int e4hip_block_set_source_terms(int blk, const double* Q)
 
 int e4hip_block_connect(int blk, int face, int other, int other_face, int orientation)
 {
+    graph_invalidate();
     BlockH* b = get_block(blk);
     BlockH* o = get_block(other);
     if (!b || !o) return fail(E4HIP_ERR_HANDLE, "unknown block %d or %d", blk, other);
@@ -1158,6 +1185,7 @@ static e4::TPArgs tp_args_for(const BlockH& b)
 
 int e4hip_block_upload(int blk, const double* U, const double* Tguess)
 {
+    graph_invalidate();
     BlockH* b = get_block(blk);
     if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
     if (!U) return fail(E4HIP_ERR_ARG, "null U");
@@ -1230,6 +1258,7 @@ int e4hip_block_download(int blk, double* U, double* fs)
 
 int e4hip_config(const E4hipStepConfig* c)
 {
+    graph_invalidate();
     if (!c) return fail(E4HIP_ERR_ARG, "null config");
     if (c->flux_calculator != E4HIP_FLUX_AUSMDV && c->flux_calculator != E4HIP_FLUX_HLLC &&
         c->flux_calculator != E4HIP_FLUX_HANEL && c->flux_calculator != E4HIP_FLUX_ADAPTIVE_HANEL_AUSMDV)
@@ -1250,16 +1279,35 @@ int e4hip_config(const E4hipStepConfig* c)
     return E4HIP_OK;
 }
 
-int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_cells)
+// The buffers a step swaps between its stages, per block.
+static void step_pointers(const int* blks, int nblk, std::vector<uintptr_t>& v)
 {
-    if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
-    if (!blks || nblk <= 0) return fail(E4HIP_ERR_ARG, "no blocks");
-    if (!(dt > 0.0)) return fail(E4HIP_ERR_ARG, "dt must be positive");
-    for (int n = 0; n < nblk; ++n)
-        if (!get_block(blks[n])) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blks[n]);
-    const int ns = nstages(g_state.cfg.gasdynamic_update_scheme);
-    int rc = ensure_counters(nblk * 3);
-    if (rc) return rc;
+    for (int n = 0; n < nblk; ++n) {
+        const BlockH& b = *get_block(blks[n]);
+        for (auto p : b.U) v.push_back((uintptr_t)p);
+        for (auto p : b.d) v.push_back((uintptr_t)p);
+        for (auto p : b.P) v.push_back((uintptr_t)p);
+        for (auto p : b.S) v.push_back((uintptr_t)p);
+    }
+}
+
+static void set_step_pointers(const int* blks, int nblk, const std::vector<uintptr_t>& v)
+{
+    size_t i = 0;
+    for (int n = 0; n < nblk; ++n) {
+        BlockH& b = *get_block(blks[n]);
+        for (auto& p : b.U) p = (double*)v[i++];
+        for (auto& p : b.d) p = (double*)v[i++];
+        for (auto& p : b.P) p = (double*)v[i++];
+        for (auto& p : b.S) p = (unsigned char*)v[i++];
+    }
+}
+
+// Everything one step puts on the library's stream (counter reset, the stages'
+// ghost fills, detector and stage kernels), with the host's buffer swaps.
+static int enqueue_step(const int* blks, int nblk, double dt, int ns)
+{
+    int rc = 0;
     const int nslots = nblk * 3;
     hipLaunchKernelGGL(e4::reset_counters_kernel, dim3(1), dim3(1024), 0, g_state.stream, g_state.counters,
                        e4::COUNTER_SLOT * nslots, nullptr);
@@ -1334,6 +1382,71 @@ int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_ce
     for (int n = 0; n < nblk; ++n) {
         BlockH& b = *get_block(blks[n]);
         if (ns == 1) std::swap(b.U[0], b.U[1]);   // Euler: the new state is in U[1]
+    }
+    return E4HIP_OK;
+}
+
+int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_cells)
+{
+    if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
+    if (!blks || nblk <= 0) return fail(E4HIP_ERR_ARG, "no blocks");
+    if (!(dt > 0.0)) return fail(E4HIP_ERR_ARG, "dt must be positive");
+    for (int n = 0; n < nblk; ++n)
+        if (!get_block(blks[n])) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blks[n]);
+    const int ns = nstages(g_state.cfg.gasdynamic_update_scheme);
+    int rc = ensure_counters(nblk * 3);
+    if (rc) return rc;
+    const int nslots = nblk * 3;
+    // use_graphs: a step whose starting state (generation, blocks, dt, buffer
+    // pointers) repeats that of the step before is captured into a graph and
+    // launched; later steps from the same state replay it.  The kernels read
+    // and write device memory only, so a replay is the same work.
+    const bool graphs = g_state.cfg.use_graphs != 0 && !g_state.profiling;
+    std::vector<uintptr_t> key;
+    if (graphs) {
+        uint64_t dtb;
+        memcpy(&dtb, &dt, sizeof(dtb));
+        key = {(uintptr_t)g_state.generation, (uintptr_t)nblk, (uintptr_t)dtb, (uintptr_t)g_state.counters,
+               (uintptr_t)g_state.batch.args_dev, (uintptr_t)g_state.batch.map_dev};
+        for (int n = 0; n < nblk; ++n) key.push_back((uintptr_t)blks[n]);
+        step_pointers(blks, nblk, key);
+    }
+    if (graphs && g_state.graph_exec && key == g_state.graph_key) {
+        HIP_TRY(hipGraphLaunch(g_state.graph_exec, g_state.stream));
+        set_step_pointers(blks, nblk, g_state.graph_post);
+    } else {
+        // the batched launches stage their arguments in pinned host buffers that
+        // a captured graph copies from when it runs: a step enqueued now rewrites
+        // them, so the graph captured before it is dropped
+        if (g_state.graph_exec) {
+            HIP_TRY(hipGraphExecDestroy(g_state.graph_exec));
+            g_state.graph_exec = nullptr;
+            g_state.graph_key.clear();
+        }
+        const bool capture = graphs && key == g_state.graph_last_key;
+        if (capture) HIP_TRY(hipStreamBeginCapture(g_state.stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_step(blks, nblk, dt, ns);
+        if (capture) {
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(g_state.stream, &graph);
+            if (rc) {
+                if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            HIP_TRY(e);
+            hipGraphExec_t ex = nullptr;
+            e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIP_TRY(e);
+            g_state.graph_exec = ex;
+            g_state.graph_key = key;
+            g_state.graph_post.clear();
+            step_pointers(blks, nblk, g_state.graph_post);
+            HIP_TRY(hipGraphLaunch(ex, g_state.stream));
+        } else if (rc) {
+            return rc;
+        }
+        if (graphs) g_state.graph_last_key = key;
     }
     const int CS = e4::COUNTER_SLOT;
     std::vector<unsigned int> h(CS * nslots);
@@ -1416,6 +1529,7 @@ int e4hip_determine_dt(const int* blks, int nblk, double dt_current, double cfl_
 
 int e4hip_profile_enable(int on)
 {
+    graph_invalidate();
     g_state.profiling = on != 0;
     return E4HIP_OK;
 }

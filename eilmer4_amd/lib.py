@@ -40,7 +40,7 @@ class StepConfig(C.Structure):
                 ("thermo_interpolator", C.c_int), ("precision_mode", C.c_int),
                 ("max_invalid_cells", C.c_int), ("learned_eos_guess", C.c_int), ("batch_blocks", C.c_int),
                 ("shear_tolerance", C.c_double), ("compression_tolerance", C.c_double),
-                ("stringent_cfl", C.c_int)]
+                ("stringent_cfl", C.c_int), ("use_graphs", C.c_int)]
 
 
 _lib = None

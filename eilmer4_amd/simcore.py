@@ -43,6 +43,7 @@ class GlobalConfig:
     max_step: int = 100
     fixed_time_step: bool = False
     stringent_cfl: bool = False             # globalconfig.d:729
+    use_graphs: bool = False                # replay repeated steps from a captured HIP graph
 
     def step_config(self):
         if FluxCalculator(self.flux_calculator) not in DEVICE_FLUX_CALCULATORS:
@@ -54,7 +55,7 @@ class GlobalConfig:
                                int(self.thermo_interpolator), int(self.precision_mode), int(self.max_invalid_cells),
                                int(bool(self.learned_eos_guess)), int(bool(self.batch_blocks)),
                                float(self.shear_tolerance), float(self.compression_tolerance),
-                               int(bool(self.stringent_cfl)))
+                               int(bool(self.stringent_cfl)), int(bool(self.use_graphs)))
 
 
 class SimState:

@@ -123,6 +123,12 @@ typedef struct E4hipStepConfig {
     int stringent_cfl;               /* CFL signal from the fastest normal velocity over the
                                         shortest cell length (fvcell.d:1642-1650); default 0
                                         (globalconfig.d:729) */
+    int use_graphs;                  /* 1: a step that starts from the same state (blocks,
+                                        boundaries, dt, buffers) as the step before is captured
+                                        into a HIP graph, and later such steps replay it (fewer
+                                        launches for many small blocks); any call that changes
+                                        blocks, boundaries or the config drops the graph.
+                                        Default 0 */
 } E4hipStepConfig;
 
 /* Library set-up on one GPU; creates the library's HIP stream.
