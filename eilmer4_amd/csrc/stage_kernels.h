@@ -840,72 +840,88 @@ __device__ __forceinline__ void boundary_face_flux(KArgs A, int i, int j, int k,
     for (int q = 0; q < 5; ++q) F[q] = double(Fg[q]);
 }
 
-// One thread per cell of the slabs beside one-sided boundaries (each cell
-// once: a cell in several slabs belongs to the first).
+// The cells of the slabs beside one-sided boundaries (each cell once: a cell
+// in several slabs belongs to the first), 64 per workgroup: wave f evaluates
+// face f (north, east, south, west, top, bottom) of all 64 cells, so every
+// lane computes one face flux rather than all of its cell's; the fluxes meet
+// in LDS and the first wave sums them in face order and finishes the cells.
 template <class M, int DIMS, int FLUX>
-__global__ __launch_bounds__(64) void boundary_cell_kernel(StageArgs)
+__global__ __launch_bounds__(128 * DIMS) void boundary_cell_kernel(StageArgs)
 {
+    constexpr int NF = 2 * DIMS;
+    __shared__ double sF[NF][5][64];
     KArgs A = kargs();
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= A->fix_off[6]) return;
-    int slab = 0;
-    while (slab < 5 && t >= A->fix_off[slab + 1]) ++slab;
-    const int d = slab >> 1, side = slab & 1;
-    const int n[3] = {A->ni, A->nj, (DIMS == 3) ? A->nk : 1};
-    const int L = side ? ((d == 0) ? A->fix_hi[0] : (d == 1 ? A->fix_hi[1] : A->fix_hi[2]))
-                       : ((d == 0) ? A->fix_lo[0] : (d == 1 ? A->fix_lo[1] : A->fix_lo[2]));
-    const int da = (d + 1) % 3, db = (d + 2) % 3;
-    const int local = t - A->fix_off[slab];
-    const int layer = local % L, rest = local / L;
-    int c[3];
-    c[d] = side ? n[d] - 1 - layer : layer;
-    c[da] = rest % n[da];
-    c[db] = rest / n[da];
-    for (int s2 = 0; s2 < slab; ++s2) {
-        const int d2 = s2 >> 1;
-        const int L2 = (s2 & 1) ? A->fix_hi[d2] : A->fix_lo[d2];
-        if ((s2 & 1) ? (c[d2] >= n[d2] - L2) : (c[d2] < L2)) return;
+    const int lane = threadIdx.x & 63;
+    const int f = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int t = blockIdx.x * 64 + lane;
+    bool valid = t < A->fix_off[6];
+    int i = 0, j = 0, k = 0;
+    if (valid) {
+        int slab = 0;
+        while (slab < 5 && t >= A->fix_off[slab + 1]) ++slab;
+        const int d = slab >> 1, side = slab & 1;
+        const int n[3] = {A->ni, A->nj, (DIMS == 3) ? A->nk : 1};
+        const int L = side ? ((d == 0) ? A->fix_hi[0] : (d == 1 ? A->fix_hi[1] : A->fix_hi[2]))
+                           : ((d == 0) ? A->fix_lo[0] : (d == 1 ? A->fix_lo[1] : A->fix_lo[2]));
+        const int da = (d + 1) % 3, db = (d + 2) % 3;
+        const int local = t - A->fix_off[slab];
+        const int layer = local % L, rest = local / L;
+        int c[3];
+        c[d] = side ? n[d] - 1 - layer : layer;
+        c[da] = rest % n[da];
+        c[db] = rest / n[da];
+        for (int s2 = 0; s2 < slab; ++s2) {
+            const int d2 = s2 >> 1;
+            const int L2 = (s2 & 1) ? A->fix_hi[d2] : A->fix_lo[d2];
+            if ((s2 & 1) ? (c[d2] >= n[d2] - L2) : (c[d2] < L2)) valid = false;
+        }
+        i = c[0]; j = c[1]; k = c[2];
     }
-    const int i = c[0], j = c[1], k = c[2];
-    double FN[5], FE[5], FS[5], FW[5], FT[5], FB[5];
-    boundary_face_flux<M, FLUX, 1, DIMS>(A, i, j + 1, k, FN);
-    boundary_face_flux<M, FLUX, 0, DIMS>(A, i + 1, j, k, FE);
-    boundary_face_flux<M, FLUX, 1, DIMS>(A, i, j, k, FS);
-    boundary_face_flux<M, FLUX, 0, DIMS>(A, i, j, k, FW);
-    if constexpr (DIMS == 3) {
-        boundary_face_flux<M, FLUX, 2, DIMS>(A, i, j, k + 1, FT);
-        boundary_face_flux<M, FLUX, 2, DIMS>(A, i, j, k, FB);
+    if (valid) {
+        double F[5];
+        switch (f) {
+        case 0: boundary_face_flux<M, FLUX, 1, DIMS>(A, i, j + 1, k, F); break;
+        case 1: boundary_face_flux<M, FLUX, 0, DIMS>(A, i + 1, j, k, F); break;
+        case 2: boundary_face_flux<M, FLUX, 1, DIMS>(A, i, j, k, F); break;
+        case 3: boundary_face_flux<M, FLUX, 0, DIMS>(A, i, j, k, F); break;
+        case 4: if constexpr (DIMS == 3) boundary_face_flux<M, FLUX, 2, DIMS>(A, i, j, k + 1, F); break;
+        default: if constexpr (DIMS == 3) boundary_face_flux<M, FLUX, 2, DIMS>(A, i, j, k, F); break;
+        }
+#pragma unroll
+        for (int q = 0; q < 5; ++q) sF[f][q][lane] = F[q];
     }
+    __syncthreads();
+    if (f != 0 || !valid) return;
     // flux divergence in face order N, E, S, W, T, B (fvcell.d:789-825)
     double integ[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    auto add_face = [&](const double* Fq, double outsign, double Aface) {
+    auto add_face = [&](int face, double outsign, double Aface) {
         const double area = outsign * Aface;
-        integ[0] -= Fq[0] * area;
-        integ[1] -= Fq[1] * area;
-        integ[2] -= Fq[2] * area;
-        if (DIMS == 3) integ[3] -= Fq[3] * area;
-        integ[4] -= Fq[4] * area;
+        integ[0] -= sF[face][0][lane] * area;
+        integ[1] -= sF[face][1][lane] * area;
+        integ[2] -= sF[face][2][lane] * area;
+        if (DIMS == 3) integ[3] -= sF[face][3][lane] * area;
+        integ[4] -= sF[face][4][lane] * area;
     };
     const long cc = cell_index(A, i, j, k);
     if (DIMS == 2 && A->axi) {
         const AxiGeom g = axi_geom(A->axi, A->ni, A->nj);
-        add_face(FN, 1.0, g.aj[(long)(j + 1) * A->ni + i]);
-        add_face(FE, 1.0, g.ai[(long)j * (A->ni + 1) + i + 1]);
-        add_face(FS, -1.0, g.aj[(long)j * A->ni + i]);
-        add_face(FW, -1.0, g.ai[(long)j * (A->ni + 1) + i]);
+        add_face(0, 1.0, g.aj[(long)(j + 1) * A->ni + i]);
+        add_face(1, 1.0, g.ai[(long)j * (A->ni + 1) + i + 1]);
+        add_face(2, -1.0, g.aj[(long)j * A->ni + i]);
+        add_face(3, -1.0, g.ai[(long)j * (A->ni + 1) + i]);
         const Prim P = load_decode<M>(A, cc);
         const double p = P.rho * A->gas.Rgas * (P.u * A->gas.Cvinv);
         const double vol = g.vol[cc];
         finish_cell<M>(A, cc, integ, 1.0 / vol, 0.0 + p * g.areaxy[cc] / vol);
         return;
     }
-    add_face(FN, 1.0, A->area[1]);
-    add_face(FE, 1.0, A->area[0]);
-    add_face(FS, -1.0, A->area[1]);
-    add_face(FW, -1.0, A->area[0]);
+    add_face(0, 1.0, A->area[1]);
+    add_face(1, 1.0, A->area[0]);
+    add_face(2, -1.0, A->area[1]);
+    add_face(3, -1.0, A->area[0]);
     if constexpr (DIMS == 3) {
-        add_face(FT, 1.0, A->area[2]);
-        add_face(FB, -1.0, A->area[2]);
+        add_face(4, 1.0, A->area[2]);
+        add_face(5, -1.0, A->area[2]);
     }
     finish_cell<M>(A, cc, integ, A->vol_inv, 0.0);
 }
@@ -968,7 +984,7 @@ void launch_boundary_cells_mode(const StageArgs& A, hipStream_t stream, int dims
 {
     const int n = A.fix_off[6];
     if (n <= 0) return;
-    const dim3 block(64), grid((unsigned)((n + 63) / 64));
+    const dim3 block(128 * dims), grid((unsigned)((n + 63) / 64));
     if (dims == 3) {
         if (flux == FLUX_HLLC) hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_HLLC>), grid, block, 0, stream, A);
         else if (flux == FLUX_SHOCK) hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_SHOCK>), grid, block, 0, stream, A);
