@@ -452,6 +452,15 @@ int launch_stage_batch(const std::vector<const BlockH*>& blks, const std::vector
         const long nwg = (long)grid.x * grid.y * grid.z;
         map.insert(map.end(), nwg, e);
     }
+    // the boundary-cell workgroups of all blocks, mapped after the stage's
+    std::vector<int> bmap;
+    for (int e = 0; e < n; ++e) {
+        args[e].bwg0 = (int)bmap.size();
+        const int nb = (args[e].fix_off[6] > 0) ? (args[e].fix_off[6] + 63) / 64 : 0;
+        bmap.insert(bmap.end(), nb, e);
+    }
+    const size_t nstage_wg = map.size();
+    map.insert(map.end(), bmap.begin(), bmap.end());
     // grow the staging buffers (no work in flight may read them then)
     const size_t slot_args = sizeof(e4::StageArgs) * n, slot_map = sizeof(int) * map.size();
     BatchBuffers& bb = g_state.batch;
@@ -481,13 +490,26 @@ int launch_stage_batch(const std::vector<const BlockH*>& blks, const std::vector
     const int flux = kernel_flux(cfg.flux_calculator);
     hipEvent_t ev = nullptr;
     prof_begin(&ev);
-    const int nwg = (int)map.size();
+    const int nwg = (int)nstage_wg;
+    const int dims = blks[0]->dims;
+    e4::BatchArgs Bb;
+    Bb.table = B.table;
+    Bb.wg_entry = (const int*)md + nstage_wg;
+    const int nbwg = (int)bmap.size();
     switch (cfg.precision_mode) {
-    case E4HIP_PRECISION_FP64_EXACT: e4::launch_stage_batch_exact(B, nwg, def, g_state.stream, blks[0]->dims, flux); break;
-    case E4HIP_PRECISION_MIXED: e4::launch_stage_batch_mixed(B, nwg, def, g_state.stream, blks[0]->dims, flux); break;
-    default: e4::launch_stage_batch_fast64(B, nwg, def, g_state.stream, blks[0]->dims, flux); break;
+    case E4HIP_PRECISION_FP64_EXACT:
+        e4::launch_stage_batch_exact(B, nwg, def, g_state.stream, dims, flux);
+        e4::launch_boundary_cells_batch_exact(Bb, nbwg, g_state.stream, dims, flux);
+        break;
+    case E4HIP_PRECISION_MIXED:
+        e4::launch_stage_batch_mixed(B, nwg, def, g_state.stream, dims, flux);
+        e4::launch_boundary_cells_batch_mixed(Bb, nbwg, g_state.stream, dims, flux);
+        break;
+    default:
+        e4::launch_stage_batch_fast64(B, nwg, def, g_state.stream, dims, flux);
+        e4::launch_boundary_cells_batch_fast64(Bb, nbwg, g_state.stream, dims, flux);
+        break;
     }
-    for (int e = 0; e < n; ++e) launch_boundary_cells(args[e], blks[e]->dims, flux);
     prof_end(0, ev);
     HIP_TRY(hipGetLastError());
     return E4HIP_OK;

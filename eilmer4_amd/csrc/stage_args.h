@@ -100,6 +100,7 @@ struct StageArgs {
     const double* nbrU[6];            // stencil-state arrays of the blocks connected to each face
     int tiles_x, tiles_y;             // workgroup tiles along i and j
     int wg0;                          // first workgroup of this block in a batched launch
+    int bwg0;                         // first workgroup of this block in a batched boundary-cell launch
     // shock indicators (adaptive_hanel_ausmdv, hanel): S of this stage's
     // detection, Sprev of the previous one (what the ghost cells were filled
     // with), the connected blocks' Sprev; Snew is written by the detector
@@ -214,6 +215,10 @@ void launch_stage_mixed(const StageArgs& A, dim3 grid, hipStream_t stream, int d
 // batched: nwg workgroups over the blocks of one table (same dims; default
 // reconstruction options or not, for all of them)
 void launch_stage_batch_exact(const BatchArgs& B, int nwg, bool def, hipStream_t stream, int dims, int flux);
+// the boundary-cell kernel over the blocks of a batched stage (64 cells per workgroup)
+void launch_boundary_cells_batch_exact(const BatchArgs& B, int nwg, hipStream_t stream, int dims, int flux);
+void launch_boundary_cells_batch_fast64(const BatchArgs& B, int nwg, hipStream_t stream, int dims, int flux);
+void launch_boundary_cells_batch_mixed(const BatchArgs& B, int nwg, hipStream_t stream, int dims, int flux);
 void launch_stage_batch_fast64(const BatchArgs& B, int nwg, bool def, hipStream_t stream, int dims, int flux);
 void launch_stage_batch_mixed(const BatchArgs& B, int nwg, bool def, hipStream_t stream, int dims, int flux);
 // thermally perfect mixture (five species): faces of every direction, then the

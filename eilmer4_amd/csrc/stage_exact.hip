@@ -20,6 +20,11 @@ void launch_extrapolate_ghosts_exact(const StageArgs& A, int faces, hipStream_t 
     launch_extrapolate_ghosts_mode<ModeExact>(A, faces, stream);
 }
 
+void launch_boundary_cells_batch_exact(const BatchArgs& B, int nwg, hipStream_t stream, int dims, int flux)
+{
+    launch_boundary_cells_batch_mode<ModeExact>(B, nwg, stream, dims, flux);
+}
+
 void launch_boundary_cells_exact(const StageArgs& A, hipStream_t stream, int dims, int flux)
 {
     launch_boundary_cells_mode<ModeExact>(A, stream, dims, flux);

@@ -30,6 +30,16 @@ void launch_extrapolate_ghosts_fast64(const StageArgs& A, int faces, hipStream_t
     launch_extrapolate_ghosts_mode<ModeFast64>(A, faces, stream);
 }
 
+void launch_boundary_cells_batch_fast64(const BatchArgs& B, int nwg, hipStream_t stream, int dims, int flux)
+{
+    launch_boundary_cells_batch_mode<ModeFast64>(B, nwg, stream, dims, flux);
+}
+
+void launch_boundary_cells_batch_mixed(const BatchArgs& B, int nwg, hipStream_t stream, int dims, int flux)
+{
+    launch_boundary_cells_batch_mode<ModeMixed>(B, nwg, stream, dims, flux);
+}
+
 void launch_boundary_cells_fast64(const StageArgs& A, hipStream_t stream, int dims, int flux)
 {
     launch_boundary_cells_mode<ModeFast64>(A, stream, dims, flux);

@@ -845,15 +845,25 @@ __device__ __forceinline__ void boundary_face_flux(KArgs A, int i, int j, int k,
 // face f (north, east, south, west, top, bottom) of all 64 cells, so every
 // lane computes one face flux rather than all of its cell's; the fluxes meet
 // in LDS and the first wave sums them in face order and finishes the cells.
-template <class M, int DIMS, int FLUX>
-__global__ __launch_bounds__(128 * DIMS) void boundary_cell_kernel(StageArgs)
+// A batched launch covers the blocks of a batched stage: each workgroup reads
+// its block's arguments from the table (bwg0: the block's first workgroup).
+template <class M, int DIMS, int FLUX, bool BATCH>
+__global__ __launch_bounds__(128 * DIMS) void boundary_cell_kernel(
+    typename std::conditional<BATCH, BatchArgs, StageArgs>::type)
 {
     constexpr int NF = 2 * DIMS;
     __shared__ double sF[NF][5][64];
-    KArgs A = kargs();
+    KArgs A;
+    int wg = (int)blockIdx.x;
+    if constexpr (BATCH) {
+        A = kargs_entry(batch_entry());
+        wg -= A->bwg0;
+    } else {
+        A = kargs();
+    }
     const int lane = threadIdx.x & 63;
     const int f = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int t = blockIdx.x * 64 + lane;
+    const int t = wg * 64 + lane;
     bool valid = t < A->fix_off[6];
     int i = 0, j = 0, k = 0;
     if (valid) {
@@ -986,13 +996,31 @@ void launch_boundary_cells_mode(const StageArgs& A, hipStream_t stream, int dims
     if (n <= 0) return;
     const dim3 block(128 * dims), grid((unsigned)((n + 63) / 64));
     if (dims == 3) {
-        if (flux == FLUX_HLLC) hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_HLLC>), grid, block, 0, stream, A);
-        else if (flux == FLUX_SHOCK) hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_SHOCK>), grid, block, 0, stream, A);
-        else hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_AUSMDV>), grid, block, 0, stream, A);
+        if (flux == FLUX_HLLC) hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_HLLC, false>), grid, block, 0, stream, A);
+        else if (flux == FLUX_SHOCK) hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_SHOCK, false>), grid, block, 0, stream, A);
+        else hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_AUSMDV, false>), grid, block, 0, stream, A);
     } else {
-        if (flux == FLUX_HLLC) hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_HLLC>), grid, block, 0, stream, A);
-        else if (flux == FLUX_SHOCK) hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_SHOCK>), grid, block, 0, stream, A);
-        else hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_AUSMDV>), grid, block, 0, stream, A);
+        if (flux == FLUX_HLLC) hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_HLLC, false>), grid, block, 0, stream, A);
+        else if (flux == FLUX_SHOCK) hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_SHOCK, false>), grid, block, 0, stream, A);
+        else hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_AUSMDV, false>), grid, block, 0, stream, A);
+    }
+}
+
+// nwg workgroups: the sum over the batch's blocks of ceil(boundary cells / 64),
+// each mapped to its block by B.wg_entry.
+template <class M>
+void launch_boundary_cells_batch_mode(const BatchArgs& B, int nwg, hipStream_t stream, int dims, int flux)
+{
+    if (nwg <= 0) return;
+    const dim3 block(128 * dims), grid((unsigned)nwg);
+    if (dims == 3) {
+        if (flux == FLUX_HLLC) hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_HLLC, true>), grid, block, 0, stream, B);
+        else if (flux == FLUX_SHOCK) hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_SHOCK, true>), grid, block, 0, stream, B);
+        else hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_AUSMDV, true>), grid, block, 0, stream, B);
+    } else {
+        if (flux == FLUX_HLLC) hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_HLLC, true>), grid, block, 0, stream, B);
+        else if (flux == FLUX_SHOCK) hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_SHOCK, true>), grid, block, 0, stream, B);
+        else hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_AUSMDV, true>), grid, block, 0, stream, B);
     }
 }
 
