@@ -94,6 +94,7 @@ struct State {
     std::vector<GasModelH> gases;
     std::vector<const double*> stage_pin;      // P[0] of every block at the start of the current stage
     BatchBuffers batch;
+    BatchBuffers fill;                         // staging of the batched x_order-1 ghost fill
     int batch_min_blocks = 2;                  // batch the ideal blocks of a stage from this many on
     std::vector<BlockH> blocks;
     E4hipStepConfig cfg{E4HIP_FLUX_AUSMDV, E4HIP_UPDATE_PC, 2, 1, 1, 1, E4HIP_THERMO_RHOU, E4HIP_PRECISION_FP64, 0, 0,
@@ -1325,6 +1326,33 @@ static void set_step_pointers(const int* blks, int nblk, const std::vector<uintp
     }
 }
 
+// Copy a batched launch's argument table and workgroup map into the stage's
+// slot of the staging buffers (grown, with the stream drained, when too small).
+static int stage_batch_buffers(BatchBuffers& bb, int stage, const void* a, size_t na, const void* m, size_t nm,
+                               char** ad, char** md)
+{
+    if (na > bb.args_slot || nm > bb.map_slot) {
+        HIP_TRY(hipStreamSynchronize(g_state.stream));
+        if (bb.args_dev) { HIP_TRY(hipFree(bb.args_dev)); HIP_TRY(hipHostFree(bb.args_host)); }
+        if (bb.map_dev) { HIP_TRY(hipFree(bb.map_dev)); HIP_TRY(hipHostFree(bb.map_host)); }
+        bb.args_slot = std::max(na, bb.args_slot) * 2;
+        bb.map_slot = std::max(nm, bb.map_slot) * 2;
+        HIP_TRY(hipMalloc(&bb.args_dev, bb.args_slot * 3));
+        HIP_TRY(hipHostMalloc(&bb.args_host, bb.args_slot * 3));
+        HIP_TRY(hipMalloc(&bb.map_dev, bb.map_slot * 3));
+        HIP_TRY(hipHostMalloc(&bb.map_host, bb.map_slot * 3));
+    }
+    char* ah = (char*)bb.args_host + bb.args_slot * (stage - 1);
+    char* mh = (char*)bb.map_host + bb.map_slot * (stage - 1);
+    *ad = (char*)bb.args_dev + bb.args_slot * (stage - 1);
+    *md = (char*)bb.map_dev + bb.map_slot * (stage - 1);
+    memcpy(ah, a, na);
+    memcpy(mh, m, nm);
+    HIP_TRY(hipMemcpyAsync(*ad, ah, na, hipMemcpyHostToDevice, g_state.stream));
+    HIP_TRY(hipMemcpyAsync(*md, mh, nm, hipMemcpyHostToDevice, g_state.stream));
+    return E4HIP_OK;
+}
+
 // Everything one step puts on the library's stream (counter reset, the stages'
 // ghost fills, detector and stage kernels), with the host's buffer swaps.
 static int enqueue_step(const int* blks, int nblk, double dt, int ns)
@@ -1338,14 +1366,37 @@ static int enqueue_step(const int* blks, int nblk, double dt, int ns)
         for (size_t h = 0; h < g_state.blocks.size(); ++h)
             if (g_state.blocks[h].alive) g_state.stage_pin[h] = g_state.blocks[h].P[0];
         // ghost cells that are stored rather than read through: the x_order-1
-        // extrapolation (the other effects are applied on the fly)
+        // extrapolation (the other effects are applied on the fly); the ideal
+        // blocks' faces in one launch when blocks are batched
+        const bool exact = g_state.cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT;
+        std::vector<e4::StageArgs> fill_args;
+        std::vector<int> fill_map;
         for (int n = 0; n < nblk; ++n) {
             BlockH& b = *get_block(blks[n]);
             int faces = 0;
             for (int f = 0; f < 6; ++f)
                 if (b.gx[f] && b.bc_type[f] == E4HIP_BC_EXTRAPOLATE_1) faces |= 1 << f;
             if (!faces) continue;
-            const bool exact = g_state.cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT;
+            if (!b.is_tp() && g_state.cfg.batch_blocks) {
+                e4::StageArgs A;
+                dim3 grid;
+                rc = stage_args_for(b, stage, dt, nullptr, 0, A, grid);
+                if (rc) return rc;
+                const int e = (int)fill_args.size();
+                fill_args.push_back(A);
+                for (int f = 0; f < 6; ++f) {
+                    if (!(faces & (1 << f))) continue;
+                    const int dir = (f == E4HIP_FACE_EAST || f == E4HIP_FACE_WEST)
+                                        ? 0 : ((f == E4HIP_FACE_NORTH || f == E4HIP_FACE_SOUTH) ? 1 : 2);
+                    const long nfc = (long)((dir == 0) ? A.nj : A.ni) * ((dir == 2) ? A.nj : A.nk);
+                    const int nw = (int)((nfc + 127) / 128);
+                    for (int w = 0; w < nw; ++w) {
+                        fill_map.push_back(e * 8 + f);
+                        fill_map.push_back(w);
+                    }
+                }
+                continue;
+            }
             if (b.is_tp()) {
                 e4::TPArgs A;
                 rc = tp_args_for(b, stage, dt, nullptr, A);
@@ -1360,6 +1411,19 @@ static int enqueue_step(const int* blks, int nblk, double dt, int ns)
                 if (exact) e4::launch_extrapolate_ghosts_exact(A, faces, g_state.stream);
                 else e4::launch_extrapolate_ghosts_fast64(A, faces, g_state.stream);
             }
+            HIP_TRY(hipGetLastError());
+        }
+        if (!fill_args.empty()) {
+            char *ad = nullptr, *md = nullptr;
+            rc = stage_batch_buffers(g_state.fill, stage, fill_args.data(), sizeof(e4::StageArgs) * fill_args.size(),
+                                     fill_map.data(), sizeof(int) * fill_map.size(), &ad, &md);
+            if (rc) return rc;
+            e4::BatchArgs B;
+            B.table = (unsigned long long)(uintptr_t)ad;
+            B.wg_entry = (const int*)md;
+            const int nwg = (int)(fill_map.size() / 2);
+            if (exact) e4::launch_extrapolate_ghosts_batch_exact(B, nwg, g_state.stream);
+            else e4::launch_extrapolate_ghosts_batch_fast64(B, nwg, g_state.stream);
             HIP_TRY(hipGetLastError());
         }
         // adaptive_hanel_ausmdv turns the shock detector on (globalconfig.d:1942-1954);
@@ -1429,7 +1493,8 @@ int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_ce
         uint64_t dtb;
         memcpy(&dtb, &dt, sizeof(dtb));
         key = {(uintptr_t)g_state.generation, (uintptr_t)nblk, (uintptr_t)dtb, (uintptr_t)g_state.counters,
-               (uintptr_t)g_state.batch.args_dev, (uintptr_t)g_state.batch.map_dev};
+               (uintptr_t)g_state.batch.args_dev, (uintptr_t)g_state.batch.map_dev,
+               (uintptr_t)g_state.fill.args_dev, (uintptr_t)g_state.fill.map_dev};
         for (int n = 0; n < nblk; ++n) key.push_back((uintptr_t)blks[n]);
         step_pointers(blks, nblk, key);
     }

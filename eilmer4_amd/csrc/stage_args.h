@@ -215,6 +215,9 @@ void launch_stage_mixed(const StageArgs& A, dim3 grid, hipStream_t stream, int d
 // batched: nwg workgroups over the blocks of one table (same dims; default
 // reconstruction options or not, for all of them)
 void launch_stage_batch_exact(const BatchArgs& B, int nwg, bool def, hipStream_t stream, int dims, int flux);
+// the x_order-1 ghost fill of several blocks' faces in one launch (128 face cells per workgroup)
+void launch_extrapolate_ghosts_batch_exact(const BatchArgs& B, int nwg, hipStream_t stream);
+void launch_extrapolate_ghosts_batch_fast64(const BatchArgs& B, int nwg, hipStream_t stream);
 // the boundary-cell kernel over the blocks of a batched stage (64 cells per workgroup)
 void launch_boundary_cells_batch_exact(const BatchArgs& B, int nwg, hipStream_t stream, int dims, int flux);
 void launch_boundary_cells_batch_fast64(const BatchArgs& B, int nwg, hipStream_t stream, int dims, int flux);

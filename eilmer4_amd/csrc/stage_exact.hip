@@ -15,6 +15,11 @@ void launch_stage_batch_exact(const BatchArgs& B, int nwg, bool def, hipStream_t
     launch_stage_batch_mode<ModeExact>(B, nwg, def, stream, dims, flux);
 }
 
+void launch_extrapolate_ghosts_batch_exact(const BatchArgs& B, int nwg, hipStream_t stream)
+{
+    launch_extrapolate_ghosts_batch_mode<ModeExact>(B, nwg, stream);
+}
+
 void launch_extrapolate_ghosts_exact(const StageArgs& A, int faces, hipStream_t stream)
 {
     launch_extrapolate_ghosts_mode<ModeExact>(A, faces, stream);

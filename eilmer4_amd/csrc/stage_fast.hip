@@ -25,6 +25,11 @@ void launch_stage_batch_mixed(const BatchArgs& B, int nwg, bool def, hipStream_t
     launch_stage_batch_mode<ModeMixed>(B, nwg, def, stream, dims, flux);
 }
 
+void launch_extrapolate_ghosts_batch_fast64(const BatchArgs& B, int nwg, hipStream_t stream)
+{
+    launch_extrapolate_ghosts_batch_mode<ModeFast64>(B, nwg, stream);
+}
+
 void launch_extrapolate_ghosts_fast64(const StageArgs& A, int faces, hipStream_t stream)
 {
     launch_extrapolate_ghosts_mode<ModeFast64>(A, faces, stream);

@@ -79,11 +79,19 @@ __device__ __forceinline__ int batch_entry()
     KBatch b = (KBatch)__builtin_amdgcn_kernarg_segment_ptr();
     return __builtin_amdgcn_readfirstlane(b->wg_entry[blockIdx.x]);
 }
+
+// word idx of a batched launch's workgroup map (uniform)
+__device__ __forceinline__ int batch_word(int idx)
+{
+    KBatch b = (KBatch)__builtin_amdgcn_kernarg_segment_ptr();
+    return __builtin_amdgcn_readfirstlane(b->wg_entry[idx]);
+}
 #else
 using KArgs = const StageArgs*;
 __device__ __forceinline__ KArgs kargs() { return nullptr; }
 __device__ __forceinline__ KArgs kargs_entry(int) { return nullptr; }
 __device__ __forceinline__ int batch_entry() { return 0; }
+__device__ __forceinline__ int batch_word(int) { return 0; }
 #endif
 
 // Decode of the cell (i,j,k); positions one or two cells outside the block in
@@ -941,15 +949,29 @@ __global__ __launch_bounds__(128 * DIMS) void boundary_cell_kernel(
 // velocity from the boundary cell c1 and the cell behind it, layer 2 from
 // layer 1 and c1; the sound speed stays the constructed cell's (the buffer's
 // initial value), which the extrapolation does not write.
-template <class M>
-__global__ __launch_bounds__(128) void extrapolate_ghosts_kernel(StageArgs, int face, double* G)
+// A batched launch covers the x_order-1 faces of several blocks: word 2w of
+// the workgroup map holds (table entry << 3 | face) and word 2w+1 the
+// workgroup's index within that face.
+template <class M, bool BATCH>
+__global__ __launch_bounds__(128) void extrapolate_ghosts_kernel(
+    typename std::conditional<BATCH, BatchArgs, StageArgs>::type, int face, double* G)
 {
-    KArgs A = kargs();
+    KArgs A;
+    int wg = (int)blockIdx.x;
+    if constexpr (BATCH) {
+        const int code = batch_word(2 * (int)blockIdx.x);
+        wg = batch_word(2 * (int)blockIdx.x + 1);
+        A = kargs_entry(code >> 3);
+        face = code & 7;
+        G = const_cast<double*>(A->gx[face]);
+    } else {
+        A = kargs();
+    }
     const int dir = (face == FACE_EAST || face == FACE_WEST) ? 0 : ((face == FACE_NORTH || face == FACE_SOUTH) ? 1 : 2);
     const bool plus = (face == FACE_EAST || face == FACE_NORTH || face == FACE_TOP);
     const int na = (dir == 0) ? A->nj : A->ni;
     const long nfc = (long)na * ((dir == 2) ? A->nj : A->nk);
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = (long)wg * blockDim.x + threadIdx.x;
     if (t >= nfc) return;
     const int fa = (int)(t % na), fb = (int)(t / na);
     const int n = (dir == 0) ? A->ni : ((dir == 1) ? A->nj : A->nk);
@@ -984,9 +1006,17 @@ void launch_extrapolate_ghosts_mode(const StageArgs& A, int faces, hipStream_t s
         if (!(faces & (1 << f))) continue;
         const int dir = (f == FACE_EAST || f == FACE_WEST) ? 0 : ((f == FACE_NORTH || f == FACE_SOUTH) ? 1 : 2);
         const long nfc = (long)((dir == 0) ? A.nj : A.ni) * ((dir == 2) ? A.nj : A.nk);
-        hipLaunchKernelGGL((extrapolate_ghosts_kernel<M>), dim3((unsigned)((nfc + 127) / 128)), dim3(128), 0, stream,
-                           A, f, const_cast<double*>(A.gx[f]));
+        hipLaunchKernelGGL((extrapolate_ghosts_kernel<M, false>), dim3((unsigned)((nfc + 127) / 128)), dim3(128), 0,
+                           stream, A, f, const_cast<double*>(A.gx[f]));
     }
+}
+
+template <class M>
+void launch_extrapolate_ghosts_batch_mode(const BatchArgs& B, int nwg, hipStream_t stream)
+{
+    if (nwg <= 0) return;
+    hipLaunchKernelGGL((extrapolate_ghosts_kernel<M, true>), dim3((unsigned)nwg), dim3(128), 0, stream, B, 0,
+                       (double*)nullptr);
 }
 
 template <class M>
