@@ -81,6 +81,8 @@ struct BatchBuffers {
     void* map_dev = nullptr;
     void* map_host = nullptr;
     size_t args_slot = 0, map_slot = 0;
+    // bytes of each slot whose device copy equals its host copy (0 after a regrow)
+    size_t args_valid[3] = {0, 0, 0}, map_valid[3] = {0, 0, 0};
 };
 
 struct State {
@@ -427,6 +429,43 @@ int launch_stage(const BlockH& b, int stage, double dt, unsigned int* counters)
 // after the previous step's synchronisation), and each workgroup finds its
 // block in a workgroup-to-entry map.  The k-chunk of every block is chosen so
 // that the whole launch has >= 2048 workgroups.
+// Copy a batched launch's argument table and workgroup map into the stage's
+// slot of the staging buffers (grown, with the stream drained, when too small);
+// a table or map equal to what the slot's device copy already holds is not
+// copied again.
+static int stage_batch_buffers(BatchBuffers& bb, int stage, const void* a, size_t na, const void* m, size_t nm,
+                               char** ad, char** md)
+{
+    if (na > bb.args_slot || nm > bb.map_slot) {
+        HIP_TRY(hipStreamSynchronize(g_state.stream));
+        if (bb.args_dev) { HIP_TRY(hipFree(bb.args_dev)); HIP_TRY(hipHostFree(bb.args_host)); }
+        if (bb.map_dev) { HIP_TRY(hipFree(bb.map_dev)); HIP_TRY(hipHostFree(bb.map_host)); }
+        bb.args_slot = std::max(na, bb.args_slot) * 2;
+        bb.map_slot = std::max(nm, bb.map_slot) * 2;
+        HIP_TRY(hipMalloc(&bb.args_dev, bb.args_slot * 3));
+        HIP_TRY(hipHostMalloc(&bb.args_host, bb.args_slot * 3));
+        HIP_TRY(hipMalloc(&bb.map_dev, bb.map_slot * 3));
+        HIP_TRY(hipHostMalloc(&bb.map_host, bb.map_slot * 3));
+        for (int q = 0; q < 3; ++q) bb.args_valid[q] = bb.map_valid[q] = 0;
+    }
+    char* ah = (char*)bb.args_host + bb.args_slot * (stage - 1);
+    char* mh = (char*)bb.map_host + bb.map_slot * (stage - 1);
+    *ad = (char*)bb.args_dev + bb.args_slot * (stage - 1);
+    *md = (char*)bb.map_dev + bb.map_slot * (stage - 1);
+    const int sl = stage - 1;
+    if (!(bb.args_valid[sl] == na && memcmp(ah, a, na) == 0)) {
+        memcpy(ah, a, na);
+        HIP_TRY(hipMemcpyAsync(*ad, ah, na, hipMemcpyHostToDevice, g_state.stream));
+        bb.args_valid[sl] = na;
+    }
+    if (!(bb.map_valid[sl] == nm && memcmp(mh, m, nm) == 0)) {
+        memcpy(mh, m, nm);
+        HIP_TRY(hipMemcpyAsync(*md, mh, nm, hipMemcpyHostToDevice, g_state.stream));
+        bb.map_valid[sl] = nm;
+    }
+    return E4HIP_OK;
+}
+
 int launch_stage_batch(const std::vector<const BlockH*>& blks, const std::vector<unsigned int*>& counters,
                        int stage, double dt)
 {
@@ -462,28 +501,10 @@ int launch_stage_batch(const std::vector<const BlockH*>& blks, const std::vector
     }
     const size_t nstage_wg = map.size();
     map.insert(map.end(), bmap.begin(), bmap.end());
-    // grow the staging buffers (no work in flight may read them then)
-    const size_t slot_args = sizeof(e4::StageArgs) * n, slot_map = sizeof(int) * map.size();
-    BatchBuffers& bb = g_state.batch;
-    if (slot_args > bb.args_slot || slot_map > bb.map_slot) {
-        HIP_TRY(hipStreamSynchronize(g_state.stream));
-        if (bb.args_dev) { HIP_TRY(hipFree(bb.args_dev)); HIP_TRY(hipHostFree(bb.args_host)); }
-        if (bb.map_dev) { HIP_TRY(hipFree(bb.map_dev)); HIP_TRY(hipHostFree(bb.map_host)); }
-        bb.args_slot = std::max(slot_args, bb.args_slot) * 2;
-        bb.map_slot = std::max(slot_map, bb.map_slot) * 2;
-        HIP_TRY(hipMalloc(&bb.args_dev, bb.args_slot * 3));
-        HIP_TRY(hipHostMalloc(&bb.args_host, bb.args_slot * 3));
-        HIP_TRY(hipMalloc(&bb.map_dev, bb.map_slot * 3));
-        HIP_TRY(hipHostMalloc(&bb.map_host, bb.map_slot * 3));
-    }
-    char* ah = (char*)bb.args_host + bb.args_slot * (stage - 1);
-    char* ad = (char*)bb.args_dev + bb.args_slot * (stage - 1);
-    char* mh = (char*)bb.map_host + bb.map_slot * (stage - 1);
-    char* md = (char*)bb.map_dev + bb.map_slot * (stage - 1);
-    memcpy(ah, args.data(), slot_args);
-    memcpy(mh, map.data(), slot_map);
-    HIP_TRY(hipMemcpyAsync(ad, ah, slot_args, hipMemcpyHostToDevice, g_state.stream));
-    HIP_TRY(hipMemcpyAsync(md, mh, slot_map, hipMemcpyHostToDevice, g_state.stream));
+    char *ad = nullptr, *md = nullptr;
+    int rc = stage_batch_buffers(g_state.batch, stage, args.data(), sizeof(e4::StageArgs) * n, map.data(),
+                                 sizeof(int) * map.size(), &ad, &md);
+    if (rc) return rc;
     e4::BatchArgs B;
     B.table = (unsigned long long)(uintptr_t)ad;
     B.wg_entry = (const int*)md;
@@ -1326,33 +1347,6 @@ static void set_step_pointers(const int* blks, int nblk, const std::vector<uintp
     }
 }
 
-// Copy a batched launch's argument table and workgroup map into the stage's
-// slot of the staging buffers (grown, with the stream drained, when too small).
-static int stage_batch_buffers(BatchBuffers& bb, int stage, const void* a, size_t na, const void* m, size_t nm,
-                               char** ad, char** md)
-{
-    if (na > bb.args_slot || nm > bb.map_slot) {
-        HIP_TRY(hipStreamSynchronize(g_state.stream));
-        if (bb.args_dev) { HIP_TRY(hipFree(bb.args_dev)); HIP_TRY(hipHostFree(bb.args_host)); }
-        if (bb.map_dev) { HIP_TRY(hipFree(bb.map_dev)); HIP_TRY(hipHostFree(bb.map_host)); }
-        bb.args_slot = std::max(na, bb.args_slot) * 2;
-        bb.map_slot = std::max(nm, bb.map_slot) * 2;
-        HIP_TRY(hipMalloc(&bb.args_dev, bb.args_slot * 3));
-        HIP_TRY(hipHostMalloc(&bb.args_host, bb.args_slot * 3));
-        HIP_TRY(hipMalloc(&bb.map_dev, bb.map_slot * 3));
-        HIP_TRY(hipHostMalloc(&bb.map_host, bb.map_slot * 3));
-    }
-    char* ah = (char*)bb.args_host + bb.args_slot * (stage - 1);
-    char* mh = (char*)bb.map_host + bb.map_slot * (stage - 1);
-    *ad = (char*)bb.args_dev + bb.args_slot * (stage - 1);
-    *md = (char*)bb.map_dev + bb.map_slot * (stage - 1);
-    memcpy(ah, a, na);
-    memcpy(mh, m, nm);
-    HIP_TRY(hipMemcpyAsync(*ad, ah, na, hipMemcpyHostToDevice, g_state.stream));
-    HIP_TRY(hipMemcpyAsync(*md, mh, nm, hipMemcpyHostToDevice, g_state.stream));
-    return E4HIP_OK;
-}
-
 // Everything one step puts on the library's stream (counter reset, the stages'
 // ghost fills, detector and stage kernels), with the host's buffer swaps.
 static int enqueue_step(const int* blks, int nblk, double dt, int ns)
@@ -1378,7 +1372,7 @@ static int enqueue_step(const int* blks, int nblk, double dt, int ns)
                 if (b.gx[f] && b.bc_type[f] == E4HIP_BC_EXTRAPOLATE_1) faces |= 1 << f;
             if (!faces) continue;
             if (!b.is_tp() && g_state.cfg.batch_blocks) {
-                e4::StageArgs A;
+                e4::StageArgs A{};
                 dim3 grid;
                 rc = stage_args_for(b, stage, dt, nullptr, 0, A, grid);
                 if (rc) return rc;
