@@ -90,6 +90,7 @@ struct State {
     int device = 0;
     hipStream_t stream = nullptr;
     unsigned int* counters = nullptr;          // per-stage counters, e4::COUNTER_SLOT uints per block-stage slot
+    unsigned int* counters_host = nullptr;     // pinned host copy the step reads them back into
     unsigned long long* smax = nullptr;
     double* last_signal = nullptr;
     int counter_slots = 0;
@@ -210,7 +211,12 @@ int ensure_counters(int slots)
 {
     if (slots <= g_state.counter_slots) return E4HIP_OK;
     if (g_state.counters) HIP_TRY(hipFree(g_state.counters));
+    if (g_state.counters_host) HIP_TRY(hipHostFree(g_state.counters_host));
+    g_state.counters = nullptr;
+    g_state.counters_host = nullptr;
+    g_state.counter_slots = 0;
     HIP_TRY(hipMalloc(&g_state.counters, sizeof(unsigned int) * e4::COUNTER_SLOT * slots));
+    HIP_TRY(hipHostMalloc(&g_state.counters_host, sizeof(unsigned int) * e4::COUNTER_SLOT * slots));
     g_state.counter_slots = slots;
     return E4HIP_OK;
 }
@@ -786,9 +792,13 @@ int e4hip_finalize(void)
     for (auto& pe : g_state.prof_pending) { hipEventDestroy(pe.a); hipEventDestroy(pe.b); }
     g_state.prof_pending.clear();
     if (g_state.counters) HIP_TRY(hipFree(g_state.counters));
-    if (g_state.batch.args_dev) { HIP_TRY(hipFree(g_state.batch.args_dev)); HIP_TRY(hipHostFree(g_state.batch.args_host)); }
-    if (g_state.batch.map_dev) { HIP_TRY(hipFree(g_state.batch.map_dev)); HIP_TRY(hipHostFree(g_state.batch.map_host)); }
-    g_state.batch = BatchBuffers();
+    if (g_state.counters_host) HIP_TRY(hipHostFree(g_state.counters_host));
+    g_state.counters_host = nullptr;
+    for (BatchBuffers* bb : {&g_state.batch, &g_state.fill}) {
+        if (bb->args_dev) { HIP_TRY(hipFree(bb->args_dev)); HIP_TRY(hipHostFree(bb->args_host)); }
+        if (bb->map_dev) { HIP_TRY(hipFree(bb->map_dev)); HIP_TRY(hipHostFree(bb->map_host)); }
+        *bb = BatchBuffers();
+    }
     if (g_state.smax) HIP_TRY(hipFree(g_state.smax));
     if (g_state.last_signal) HIP_TRY(hipFree(g_state.last_signal));
     g_state.counters = nullptr; g_state.smax = nullptr; g_state.last_signal = nullptr;
@@ -1530,9 +1540,9 @@ int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_ce
         if (graphs) g_state.graph_last_key = key;
     }
     const int CS = e4::COUNTER_SLOT;
-    std::vector<unsigned int> h(CS * nslots);
-    HIP_TRY(hipMemcpyAsync(h.data(), g_state.counters, sizeof(unsigned int) * CS * nslots, hipMemcpyDeviceToHost,
-                           g_state.stream));
+    const unsigned int* h = g_state.counters_host;
+    HIP_TRY(hipMemcpyAsync(g_state.counters_host, g_state.counters, sizeof(unsigned int) * CS * nslots,
+                           hipMemcpyDeviceToHost, g_state.stream));
     HIP_TRY(hipStreamSynchronize(g_state.stream));
     int stage_bad[3] = {0, 0, 0};
     bool too_many = false, decode_fail = false;
