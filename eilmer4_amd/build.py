@@ -1,12 +1,15 @@
 """In-tree build of libe4hip.so for gfx950 with hipcc.
 
-Four translation units:
-  stage_exact.hip  -ffp-contract=off: the FP64 stage kernel with the
-                   reference's roundings (bit-identical to the CPU oracle)
-  stage_fast.hip   -ffp-contract=fast: the plain FP64 and mixed FP32/FP64
-                   stage kernels
-  e4hip_api.hip    -ffp-contract=off: C ABI, host bookkeeping, CFL/decode kernels
-  mlp_train.hip    -ffp-contract=fast: training kernels of the temperature-guess network
+Translation units, compiled in parallel:
+  stage_exact.hip   -ffp-contract=off: the FP64 ideal-gas kernels with the
+                    reference's roundings (bit-identical to the CPU oracle)
+  tp_exact.hip      -ffp-contract=off: the same for the thermally perfect mixture
+  stage_fast64.hip  -ffp-contract=fast: the plain FP64 ideal-gas kernels
+  stage_mixed.hip   -ffp-contract=fast: the mixed FP32/FP64 ideal-gas kernels
+  tp_fast.hip       -ffp-contract=fast: the plain and mixed mixture kernels and
+                    the learned-guess inference
+  e4hip_api.hip     -ffp-contract=off: C ABI, host bookkeeping, CFL/decode kernels
+  mlp_train.hip     -ffp-contract=fast: training kernels of the temperature-guess network
 Packed FP32 instructions are disabled in all of them (-packed-fp32-ops).
 """
 import os
@@ -23,7 +26,10 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", "-Wno-unused-result", "-Wno-unused-value"]
 UNITS = [("stage_exact.hip", ["-ffp-contract=off"]),
-         ("stage_fast.hip", ["-ffp-contract=fast"]),
+         ("tp_exact.hip", ["-ffp-contract=off"]),
+         ("stage_fast64.hip", ["-ffp-contract=fast"]),
+         ("stage_mixed.hip", ["-ffp-contract=fast"]),
+         ("tp_fast.hip", ["-ffp-contract=fast"]),
          ("e4hip_api.hip", ["-ffp-contract=off"]),
          ("mlp_train.hip", ["-ffp-contract=fast"])]
 

@@ -47,7 +47,10 @@ struct BlockH {
     long ncells = 0, qs = 0;
     int nq = 5;
     int nsp = 1;
-    double* U[3] = {nullptr, nullptr, nullptr};   // U[0] is the current state
+    // U[0] is the current state; stage s writes U[s], and a step that ends
+    // without bad cells swaps U[0] and U[stages] (simcore.d:2936-2941), so a
+    // step that stops leaves U[0] as it was
+    double* U[4] = {nullptr, nullptr, nullptr, nullptr};
     double* d[2] = {nullptr, nullptr};            // dUdt of stages 1 and 2
     int bc_type[6] = {0, 0, 0, 0, 0, 0};
     Prim bc_fs[6] = {};
@@ -55,8 +58,9 @@ struct BlockH {
     int nbr_blk[6] = {-1, -1, -1, -1, -1, -1};    // connected block per face (E4HIP_BC_CONNECTED)
     int nbr_face[6] = {0, 0, 0, 0, 0, 0};
     int nbr_orient[6] = {0, 0, 0, 0, 0, 0};
-    // thermally perfect gas only: primitive arrays (ping-pong) and face fluxes
-    double* P[2] = {nullptr, nullptr};
+    // thermally perfect gas only: primitive arrays (stage s reads P[s-1] and
+    // writes P[s]; swapped like U) and face fluxes
+    double* P[4] = {nullptr, nullptr, nullptr, nullptr};
     double* F[3] = {nullptr, nullptr, nullptr};
     long fqs[3] = {0, 0, 0};
     double bc_fs_tp[6][8 + e4::TP_NSP_MAX] = {};
@@ -74,28 +78,52 @@ struct BlockH {
 
 struct ProfEvent { int cls; hipEvent_t a, b; };
 
-// Staging of the batched stage launches: three slots (one per stage of a step).
+// Staging of the batched launches: a slot per (set, stage, dimensionality).
+// Set 0 serves steps run eagerly; sets 1 and 2 belong to the two captured
+// step graphs, whose copy nodes read their pinned host slots whenever they
+// replay, so only the capture of that graph writes them.
+constexpr int BATCH_SETS = 3;
+constexpr int BATCH_SLOTS = BATCH_SETS * 3 * 2;
 struct BatchBuffers {
     void* args_dev = nullptr;
     void* args_host = nullptr;
     void* map_dev = nullptr;
     void* map_host = nullptr;
     size_t args_slot = 0, map_slot = 0;
-    // bytes of each slot whose device copy equals its host copy (0 after a regrow)
-    size_t args_valid[3] = {0, 0, 0}, map_valid[3] = {0, 0, 0};
+    // set 0: bytes of each slot whose device copy equals its host copy (0 after a regrow)
+    size_t args_valid[BATCH_SLOTS] = {}, map_valid[BATCH_SLOTS] = {};
 };
+
+inline int batch_slot(int set, int stage, int dims) { return (set * 3 + (stage - 1)) * 2 + (dims - 2); }
+
+// A step captured into a HIP graph: the key of the state it starts from
+// (generation, blocks, dt, buffer pointers), the blocks' buffer pointers after
+// it, and the batch staging set its copy nodes read.
+struct StepGraph {
+    std::vector<uintptr_t> key, post;
+    hipGraphExec_t exec = nullptr;
+    int set = 1;
+    unsigned long long last_use = 0;
+};
+
+// Words before the block-stage counter slots: the abort flag of each stage.
+constexpr int ABORT_WORDS = 8;
+// pointers per block in a step's snapshot (U[4], d[2], P[4], S[2])
+constexpr int PTRS_PER_BLOCK = 12;
 
 struct State {
     bool init = false;
     int device = 0;
     hipStream_t stream = nullptr;
+    unsigned int* counters_raw = nullptr;      // ABORT_WORDS abort flags, then the counter slots
+    unsigned int* abort = nullptr;             // = counters_raw: abort flag of stage s at [s-1]
     unsigned int* counters = nullptr;          // per-stage counters, e4::COUNTER_SLOT uints per block-stage slot
-    unsigned int* counters_host = nullptr;     // pinned host copy the step reads them back into
+    unsigned int* counters_host = nullptr;     // pinned host copy the step reads the flags and counters back into
+    unsigned int* step_abort = nullptr;        // abort flags of the step being enqueued
     unsigned long long* smax = nullptr;
     double* last_signal = nullptr;
     int counter_slots = 0;
     std::vector<GasModelH> gases;
-    std::vector<const double*> stage_pin;      // P[0] of every block at the start of the current stage
     BatchBuffers batch;
     BatchBuffers fill;                         // staging of the batched x_order-1 ghost fill
     int batch_min_blocks = 2;                  // batch the ideal blocks of a stage from this many on
@@ -108,12 +136,13 @@ struct State {
     long prof_n[2] = {0, 0};
     long long decode_evals = 0, decode_fallbacks = 0;
     long long face_evals = 0, face_fallbacks = 0;      // face temperature solves of the thermally perfect path
-    // graph replay of a step (use_graphs): the instantiated graph of the last
-    // captured step, the key of the state it was captured from (generation,
-    // blocks, dt, buffer pointers), the blocks' buffer pointers after it, and
-    // the key of the last step run without the graph
-    hipGraphExec_t graph_exec = nullptr;
-    std::vector<uintptr_t> graph_key, graph_post, graph_last_key;
+    // graph replay of steps (use_graphs): up to two captured steps (a step
+    // swaps U[0] and U[stages], so the starting states of consecutive steps
+    // alternate between two keys), and the keys of the last two steps run
+    // without a graph
+    std::vector<StepGraph> graphs;
+    std::vector<std::vector<uintptr_t>> recent_keys;
+    unsigned long long step_counter = 0;
     unsigned long long generation = 0;         // bumped by every call that changes blocks, boundaries or config
     std::string err;
 };
@@ -122,14 +151,19 @@ State g_state;
 
 bool BlockH::is_tp() const { return g_state.gases[gm].type == GAS_TP; }
 
-// Any call that changes what a step's kernels are given drops the captured step.
+void graphs_drop()
+{
+    for (auto& g : g_state.graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g_state.graphs.clear();
+    g_state.recent_keys.clear();
+}
+
+// Any call that changes what a step's kernels are given drops the captured steps.
 void graph_invalidate()
 {
     ++g_state.generation;
-    if (g_state.graph_exec) { (void)hipGraphExecDestroy(g_state.graph_exec); g_state.graph_exec = nullptr; }
-    g_state.graph_key.clear();
-    g_state.graph_post.clear();
-    g_state.graph_last_key.clear();
+    graphs_drop();
 }
 
 int fail(int code, const char* fmt, ...)
@@ -210,13 +244,19 @@ int kchunk_for(const BlockH& b)
 int ensure_counters(int slots)
 {
     if (slots <= g_state.counter_slots) return E4HIP_OK;
-    if (g_state.counters) HIP_TRY(hipFree(g_state.counters));
+    // captured steps hold the old counter addresses
+    graph_invalidate();
+    HIP_TRY(hipStreamSynchronize(g_state.stream));
+    if (g_state.counters_raw) HIP_TRY(hipFree(g_state.counters_raw));
     if (g_state.counters_host) HIP_TRY(hipHostFree(g_state.counters_host));
-    g_state.counters = nullptr;
+    g_state.counters_raw = g_state.abort = g_state.counters = nullptr;
     g_state.counters_host = nullptr;
     g_state.counter_slots = 0;
-    HIP_TRY(hipMalloc(&g_state.counters, sizeof(unsigned int) * e4::COUNTER_SLOT * slots));
-    HIP_TRY(hipHostMalloc(&g_state.counters_host, sizeof(unsigned int) * e4::COUNTER_SLOT * slots));
+    const size_t n = ABORT_WORDS + (size_t)e4::COUNTER_SLOT * slots;
+    HIP_TRY(hipMalloc(&g_state.counters_raw, sizeof(unsigned int) * n));
+    HIP_TRY(hipHostMalloc(&g_state.counters_host, sizeof(unsigned int) * n));
+    g_state.abort = g_state.counters_raw;
+    g_state.counters = g_state.counters_raw + ABORT_WORDS;
     g_state.counter_slots = slots;
     return E4HIP_OK;
 }
@@ -276,6 +316,42 @@ int shock_fields(const BlockH& b, Args& A)
     return E4HIP_OK;
 }
 
+// The coefficients of the stage updates of fvcell.d:1153-1393:
+// stage 1 U1 = U0 + dt g1 d0; stage 2 U2 = Uold + dt (g1 d0 + g2 d1);
+// stage 3 U3 = Uold + dt (g1 d0 + g2 d1 + g3 d2).
+void stage_coefficients(int sch, int stage, double& g1, double& g2, double& g3)
+{
+    g1 = 1.0; g2 = 0.0; g3 = 0.0;
+    if (stage == 1) {
+        if (sch == E4HIP_UPDATE_MIDPOINT || sch == E4HIP_UPDATE_CLASSIC_RK3) g1 = 0.5;
+        if (sch == E4HIP_UPDATE_DENMAN_RK3) g1 = 8.0 / 15.0;
+    } else if (stage == 2) {
+        g1 = 0.5; g2 = 0.5;
+        switch (sch) {
+        case E4HIP_UPDATE_MIDPOINT: g1 = 0.0; g2 = 1.0; break;
+        case E4HIP_UPDATE_CLASSIC_RK3: g1 = -1.0; g2 = 2.0; break;
+        case E4HIP_UPDATE_TVD_RK3: g1 = 0.25; g2 = 0.25; break;
+        case E4HIP_UPDATE_DENMAN_RK3: g1 = -17.0 / 60.0; g2 = 5.0 / 12.0; break;
+        default: break;
+        }
+    } else {
+        g1 = 1.0 / 6.0; g2 = 1.0 / 6.0; g3 = 4.0 / 6.0;
+        if (sch == E4HIP_UPDATE_CLASSIC_RK3) { g1 = 1.0 / 6.0; g2 = 4.0 / 6.0; g3 = 1.0 / 6.0; }
+        if (sch == E4HIP_UPDATE_DENMAN_RK3) { g1 = 0.0; g2 = -5.0 / 12.0; g3 = 3.0 / 4.0; }
+    }
+}
+
+// The bad-cell stop of the step being enqueued: stage s reads the flag of
+// stage s-1 and sets its own (StageArgs and TPArgs).
+template <class Args>
+void abort_fields(int stage, Args& A)
+{
+    unsigned int* ab = g_state.step_abort;
+    A.abort_prev = (ab && stage > 1) ? ab + (stage - 2) : nullptr;
+    A.abort_this = ab ? ab + (stage - 1) : nullptr;
+    A.max_invalid = (unsigned int)g_state.cfg.max_invalid_cells;
+}
+
 // The stage kernel's arguments for one block; kchunk <= 0 picks the
 // single-launch chunk.
 void boundary_slabs(const BlockH& b, int* lo, int* hi, int* off, int& noghost, bool stored_ghosts);
@@ -313,44 +389,27 @@ int stage_args_for(const BlockH& b, int stage, double dt, unsigned int* counters
     }
     if (!l2r2_uniform(b.dx, A.wu[0]) || !l2r2_uniform(b.dy, A.wu[1]) || !l2r2_uniform(b.dz, A.wu[2]))
         return fail(E4HIP_ERR_ARG, "l2r2 weights of a uniform direction are not uniform");
-    // stage coefficients (fvcell.d:1168-1180, 1254-1266, 1333-1342) and buffers
+    // stage coefficients (fvcell.d:1168-1180, 1254-1266, 1333-1342) and
+    // buffers: stage s evaluates its fluxes from U[s-1] and writes U[s]
     double g1 = 1.0, g2 = 0.0, g3 = 0.0;
+    stage_coefficients(sch, stage, g1, g2, g3);
+    A.Us = b.U[stage - 1];
+    A.Unew = b.U[stage];
     if (stage == 1) {
-        if (sch == E4HIP_UPDATE_MIDPOINT || sch == E4HIP_UPDATE_CLASSIC_RK3) g1 = 0.5;
-        if (sch == E4HIP_UPDATE_DENMAN_RK3) g1 = 8.0 / 15.0;
-        A.Us = b.U[0]; A.Uold = b.U[0];
-        A.Unew = (ns == 1) ? b.U[1] : b.U[1];
+        A.Uold = b.U[0];
         A.dout = (ns > 1) ? b.d[0] : nullptr;
     } else if (stage == 2) {
-        g1 = 0.5; g2 = 0.5;
-        switch (sch) {
-        case E4HIP_UPDATE_MIDPOINT: g1 = 0.0; g2 = 1.0; break;
-        case E4HIP_UPDATE_CLASSIC_RK3: g1 = -1.0; g2 = 2.0; break;
-        case E4HIP_UPDATE_TVD_RK3: g1 = 0.25; g2 = 0.25; break;
-        case E4HIP_UPDATE_DENMAN_RK3: g1 = -17.0 / 60.0; g2 = 5.0 / 12.0; break;
-        default: break;
-        }
-        A.Us = b.U[1];
         A.d0 = b.d[0];
-        if (ns == 2) {            // pc, midpoint: new state written over U[0] in place
-            A.Uold = b.U[0]; A.Unew = b.U[0]; A.dout = nullptr;
-        } else {
-            A.Uold = (sch == E4HIP_UPDATE_DENMAN_RK3) ? b.U[1] : b.U[0];
-            A.Unew = b.U[2]; A.dout = b.d[1];
-        }
+        A.Uold = (sch == E4HIP_UPDATE_DENMAN_RK3) ? b.U[1] : b.U[0];
+        A.dout = (ns > 2) ? b.d[1] : nullptr;
     } else {
-        g1 = 1.0 / 6.0; g2 = 1.0 / 6.0; g3 = 4.0 / 6.0;
-        if (sch == E4HIP_UPDATE_CLASSIC_RK3) { g1 = 1.0 / 6.0; g2 = 4.0 / 6.0; g3 = 1.0 / 6.0; }
-        if (sch == E4HIP_UPDATE_DENMAN_RK3) { g1 = 0.0; g2 = -5.0 / 12.0; g3 = 3.0 / 4.0; }
-        A.Us = b.U[2];
         A.d0 = b.d[0]; A.d1 = b.d[1];
         A.Uold = (sch == E4HIP_UPDATE_DENMAN_RK3) ? b.U[2] : b.U[0];
-        A.Unew = b.U[0];          // U[0] is not read as a stencil in stage 3
         A.dout = nullptr;
-        if (sch == E4HIP_UPDATE_DENMAN_RK3) A.Unew = b.U[0];
     }
     A.dt = dt; A.g1 = g1; A.g2 = g2; A.g3 = g3;
     A.c1 = dt * g1;
+    abort_fields(stage, A);
     boundary_slabs(b, A.fix_lo, A.fix_hi, A.fix_off, A.noghost, true);
     for (int f = 0; f < 6; ++f) { A.gx[f] = b.gx[f]; A.gS[f] = b.gS[f]; }
     A.axi = b.axi;
@@ -439,41 +498,50 @@ int launch_stage(const BlockH& b, int stage, double dt, unsigned int* counters)
 // slot of the staging buffers (grown, with the stream drained, when too small);
 // a table or map equal to what the slot's device copy already holds is not
 // copied again.
-static int stage_batch_buffers(BatchBuffers& bb, int stage, const void* a, size_t na, const void* m, size_t nm,
+// A slot of set 0 skips the copy when its device copy already holds the
+// table; the slots of a captured graph's set always record the copy (a capture
+// runs nothing, so their device copies are only what the graph writes).
+// Growing the buffers drops the captured graphs, whose nodes hold the old ones.
+static int stage_batch_buffers(BatchBuffers& bb, int slot, const void* a, size_t na, const void* m, size_t nm,
                                char** ad, char** md)
 {
     if (na > bb.args_slot || nm > bb.map_slot) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIP_TRY(hipStreamIsCapturing(g_state.stream, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            return fail(E4HIP_ERR_ARG, "batched argument tables outgrew their buffers during a graph capture");
+        graph_invalidate();
         HIP_TRY(hipStreamSynchronize(g_state.stream));
         if (bb.args_dev) { HIP_TRY(hipFree(bb.args_dev)); HIP_TRY(hipHostFree(bb.args_host)); }
         if (bb.map_dev) { HIP_TRY(hipFree(bb.map_dev)); HIP_TRY(hipHostFree(bb.map_host)); }
-        bb.args_slot = std::max(na, bb.args_slot) * 2;
-        bb.map_slot = std::max(nm, bb.map_slot) * 2;
-        HIP_TRY(hipMalloc(&bb.args_dev, bb.args_slot * 3));
-        HIP_TRY(hipHostMalloc(&bb.args_host, bb.args_slot * 3));
-        HIP_TRY(hipMalloc(&bb.map_dev, bb.map_slot * 3));
-        HIP_TRY(hipHostMalloc(&bb.map_host, bb.map_slot * 3));
-        for (int q = 0; q < 3; ++q) bb.args_valid[q] = bb.map_valid[q] = 0;
+        bb.args_slot = (std::max(na, bb.args_slot) * 2 + 255) / 256 * 256;
+        bb.map_slot = (std::max(nm, bb.map_slot) * 2 + 255) / 256 * 256;
+        HIP_TRY(hipMalloc(&bb.args_dev, bb.args_slot * BATCH_SLOTS));
+        HIP_TRY(hipHostMalloc(&bb.args_host, bb.args_slot * BATCH_SLOTS));
+        HIP_TRY(hipMalloc(&bb.map_dev, bb.map_slot * BATCH_SLOTS));
+        HIP_TRY(hipHostMalloc(&bb.map_host, bb.map_slot * BATCH_SLOTS));
+        for (int q = 0; q < BATCH_SLOTS; ++q) bb.args_valid[q] = bb.map_valid[q] = 0;
     }
-    char* ah = (char*)bb.args_host + bb.args_slot * (stage - 1);
-    char* mh = (char*)bb.map_host + bb.map_slot * (stage - 1);
-    *ad = (char*)bb.args_dev + bb.args_slot * (stage - 1);
-    *md = (char*)bb.map_dev + bb.map_slot * (stage - 1);
-    const int sl = stage - 1;
-    if (!(bb.args_valid[sl] == na && memcmp(ah, a, na) == 0)) {
+    char* ah = (char*)bb.args_host + bb.args_slot * slot;
+    char* mh = (char*)bb.map_host + bb.map_slot * slot;
+    *ad = (char*)bb.args_dev + bb.args_slot * slot;
+    *md = (char*)bb.map_dev + bb.map_slot * slot;
+    const bool eager_set = slot < 6;
+    if (!(eager_set && bb.args_valid[slot] == na && memcmp(ah, a, na) == 0)) {
         memcpy(ah, a, na);
         HIP_TRY(hipMemcpyAsync(*ad, ah, na, hipMemcpyHostToDevice, g_state.stream));
-        bb.args_valid[sl] = na;
+        if (eager_set) bb.args_valid[slot] = na;
     }
-    if (!(bb.map_valid[sl] == nm && memcmp(mh, m, nm) == 0)) {
+    if (!(eager_set && bb.map_valid[slot] == nm && memcmp(mh, m, nm) == 0)) {
         memcpy(mh, m, nm);
         HIP_TRY(hipMemcpyAsync(*md, mh, nm, hipMemcpyHostToDevice, g_state.stream));
-        bb.map_valid[sl] = nm;
+        if (eager_set) bb.map_valid[slot] = nm;
     }
     return E4HIP_OK;
 }
 
 int launch_stage_batch(const std::vector<const BlockH*>& blks, const std::vector<unsigned int*>& counters,
-                       int stage, double dt)
+                       int stage, double dt, int set)
 {
     const E4hipStepConfig& cfg = g_state.cfg;
     const int n = (int)blks.size();
@@ -508,8 +576,8 @@ int launch_stage_batch(const std::vector<const BlockH*>& blks, const std::vector
     const size_t nstage_wg = map.size();
     map.insert(map.end(), bmap.begin(), bmap.end());
     char *ad = nullptr, *md = nullptr;
-    int rc = stage_batch_buffers(g_state.batch, stage, args.data(), sizeof(e4::StageArgs) * n, map.data(),
-                                 sizeof(int) * map.size(), &ad, &md);
+    int rc = stage_batch_buffers(g_state.batch, batch_slot(set, stage, blks[0]->dims), args.data(),
+                                 sizeof(e4::StageArgs) * n, map.data(), sizeof(int) * map.size(), &ad, &md);
     if (rc) return rc;
     e4::BatchArgs B;
     B.table = (unsigned long long)(uintptr_t)ad;
@@ -573,9 +641,8 @@ int tp_args_for(const BlockH& b, int stage, double dt, unsigned int* counters, e
         if (b.bc_type[f] != E4HIP_BC_CONNECTED) continue;
         const BlockH* o = get_block(b.nbr_blk[f]);
         if (!o) return fail(E4HIP_ERR_HANDLE, "face %d is connected to a destroyed block", f);
-        // the other block's primitive state at the start of this stage (its
-        // own stage launch, earlier in the same stage, swaps P[0] and P[1])
-        A.nbrP[f] = g_state.stage_pin[b.nbr_blk[f]];
+        // the other block's primitive state at the start of this stage
+        A.nbrP[f] = o->P[stage - 1];
     }
     if (b.dims == 3) {
         A.vol_inv = 1.0 / (b.dx * b.dy * b.dz);
@@ -588,38 +655,24 @@ int tp_args_for(const BlockH& b, int stage, double dt, unsigned int* counters, e
     A.w[1] = l2r2_weights(b.dy);
     A.w[2] = l2r2_weights(b.dz);
     double g1 = 1.0, g2 = 0.0, g3 = 0.0;
+    stage_coefficients(sch, stage, g1, g2, g3);
+    A.Unew = b.U[stage];
     if (stage == 1) {
-        if (sch == E4HIP_UPDATE_MIDPOINT || sch == E4HIP_UPDATE_CLASSIC_RK3) g1 = 0.5;
-        if (sch == E4HIP_UPDATE_DENMAN_RK3) g1 = 8.0 / 15.0;
-        A.Uold = b.U[0]; A.Unew = b.U[1];
+        A.Uold = b.U[0];
         A.dout = (ns > 1) ? b.d[0] : nullptr;
     } else if (stage == 2) {
-        g1 = 0.5; g2 = 0.5;
-        switch (sch) {
-        case E4HIP_UPDATE_MIDPOINT: g1 = 0.0; g2 = 1.0; break;
-        case E4HIP_UPDATE_CLASSIC_RK3: g1 = -1.0; g2 = 2.0; break;
-        case E4HIP_UPDATE_TVD_RK3: g1 = 0.25; g2 = 0.25; break;
-        case E4HIP_UPDATE_DENMAN_RK3: g1 = -17.0 / 60.0; g2 = 5.0 / 12.0; break;
-        default: break;
-        }
         A.d0 = b.d[0];
-        if (ns == 2) { A.Uold = b.U[0]; A.Unew = b.U[0]; A.dout = nullptr; }
-        else {
-            A.Uold = (sch == E4HIP_UPDATE_DENMAN_RK3) ? b.U[1] : b.U[0];
-            A.Unew = b.U[2]; A.dout = b.d[1];
-        }
+        A.Uold = (sch == E4HIP_UPDATE_DENMAN_RK3) ? b.U[1] : b.U[0];
+        A.dout = (ns > 2) ? b.d[1] : nullptr;
     } else {
-        g1 = 1.0 / 6.0; g2 = 1.0 / 6.0; g3 = 4.0 / 6.0;
-        if (sch == E4HIP_UPDATE_CLASSIC_RK3) { g1 = 1.0 / 6.0; g2 = 4.0 / 6.0; g3 = 1.0 / 6.0; }
-        if (sch == E4HIP_UPDATE_DENMAN_RK3) { g1 = 0.0; g2 = -5.0 / 12.0; g3 = 3.0 / 4.0; }
         A.d0 = b.d[0]; A.d1 = b.d[1];
         A.Uold = (sch == E4HIP_UPDATE_DENMAN_RK3) ? b.U[2] : b.U[0];
-        A.Unew = b.U[0];
     }
     A.dt = dt; A.g1 = g1; A.g2 = g2; A.g3 = g3;
     A.c1 = dt * g1;
-    A.Pin = b.P[0];
-    A.Pout = b.P[1];
+    A.Pin = b.P[stage - 1];
+    A.Pout = b.P[stage];
+    abort_fields(stage, A);
     // faces the boundary-face kernel recomputes: per slab, the face layers
     // (wall and the face after it; a simple-outflow-flux face)
     {
@@ -660,7 +713,6 @@ int launch_tp_stage(BlockH& b, int stage, double dt, unsigned int* counters)
     else e4::launch_tp_stage_fast64(A, g_state.stream, flux);
     prof_end(0, ev);
     HIP_TRY(hipGetLastError());
-    std::swap(b.P[0], b.P[1]);
     return E4HIP_OK;
 }
 
@@ -696,7 +748,7 @@ int alloc_block_buffers(BlockH& b)
     const int ns = nstages(g_state.cfg.gasdynamic_update_scheme);
     if (b.is_tp()) {
         const size_t pbytes = sizeof(double) * (size_t)b.qs * (8 + b.nsp);
-        for (int t = 0; t < 2; ++t) if (!b.P[t]) HIP_TRY(hipMalloc(&b.P[t], pbytes));
+        for (int t = 0; t <= ns; ++t) if (!b.P[t]) HIP_TRY(hipMalloc(&b.P[t], pbytes));
         const long nk = (b.dims == 3) ? b.nk : 1;
         const long nf[3] = {(long)(b.ni + 1) * b.nj * nk, (long)b.ni * (b.nj + 1) * nk,
                             (b.dims == 3) ? (long)b.ni * b.nj * (b.nk + 1) : 1};
@@ -711,13 +763,8 @@ int alloc_block_buffers(BlockH& b)
         HIP_TRY(hipMalloc(&b.S[t], (size_t)b.qs));
         HIP_TRY(hipMemset(b.S[t], 0, (size_t)b.qs));
     }
-    if (!b.U[0]) HIP_TRY(hipMalloc(&b.U[0], bytes));
-    if (!b.U[1]) HIP_TRY(hipMalloc(&b.U[1], bytes));
-    if (ns >= 2 && !b.d[0]) HIP_TRY(hipMalloc(&b.d[0], bytes));
-    if (ns >= 3) {
-        if (!b.U[2]) HIP_TRY(hipMalloc(&b.U[2], bytes));
-        if (!b.d[1]) HIP_TRY(hipMalloc(&b.d[1], bytes));
-    }
+    for (int t = 0; t <= ns; ++t) if (!b.U[t]) HIP_TRY(hipMalloc(&b.U[t], bytes));
+    for (int t = 0; t + 1 < ns; ++t) if (!b.d[t]) HIP_TRY(hipMalloc(&b.d[t], bytes));
     return E4HIP_OK;
 }
 
@@ -791,9 +838,10 @@ int e4hip_finalize(void)
     g_state.gases.clear();
     for (auto& pe : g_state.prof_pending) { hipEventDestroy(pe.a); hipEventDestroy(pe.b); }
     g_state.prof_pending.clear();
-    if (g_state.counters) HIP_TRY(hipFree(g_state.counters));
+    if (g_state.counters_raw) HIP_TRY(hipFree(g_state.counters_raw));
     if (g_state.counters_host) HIP_TRY(hipHostFree(g_state.counters_host));
     g_state.counters_host = nullptr;
+    g_state.counters_raw = g_state.abort = nullptr;
     for (BatchBuffers* bb : {&g_state.batch, &g_state.fill}) {
         if (bb->args_dev) { HIP_TRY(hipFree(bb->args_dev)); HIP_TRY(hipHostFree(bb->args_host)); }
         if (bb->map_dev) { HIP_TRY(hipFree(bb->map_dev)); HIP_TRY(hipHostFree(bb->map_host)); }
@@ -1327,6 +1375,7 @@ int e4hip_config(const E4hipStepConfig* c)
     if (c->thermo_interpolator != E4HIP_THERMO_RHOU) return fail(E4HIP_ERR_ARG, "thermo_interpolator must be rhou");
     if (c->precision_mode < E4HIP_PRECISION_FP64 || c->precision_mode > E4HIP_PRECISION_MIXED)
         return fail(E4HIP_ERR_ARG, "precision_mode %d not available", c->precision_mode);
+    if (c->max_invalid_cells < 0) return fail(E4HIP_ERR_ARG, "max_invalid_cells must be >= 0");
     g_state.cfg = *c;
     for (auto& b : g_state.blocks)
         if (b.alive) { int rc = alloc_block_buffers(b); if (rc) return rc; }
@@ -1334,18 +1383,24 @@ int e4hip_config(const E4hipStepConfig* c)
 }
 
 // The buffers a step swaps between its stages, per block.
-static void step_pointers(const int* blks, int nblk, std::vector<uintptr_t>& v)
+// The buffers a step swaps between its stages, per block (PTRS_PER_BLOCK each).
+static size_t step_pointers_to(const int* blks, int nblk, uintptr_t* v)
 {
+    static_assert(sizeof(BlockH::U) / sizeof(double*) + sizeof(BlockH::d) / sizeof(double*) +
+                      sizeof(BlockH::P) / sizeof(double*) + sizeof(BlockH::S) / sizeof(unsigned char*) ==
+                      PTRS_PER_BLOCK, "PTRS_PER_BLOCK");
+    size_t i = 0;
     for (int n = 0; n < nblk; ++n) {
         const BlockH& b = *get_block(blks[n]);
-        for (auto p : b.U) v.push_back((uintptr_t)p);
-        for (auto p : b.d) v.push_back((uintptr_t)p);
-        for (auto p : b.P) v.push_back((uintptr_t)p);
-        for (auto p : b.S) v.push_back((uintptr_t)p);
+        for (auto p : b.U) v[i++] = (uintptr_t)p;
+        for (auto p : b.d) v[i++] = (uintptr_t)p;
+        for (auto p : b.P) v[i++] = (uintptr_t)p;
+        for (auto p : b.S) v[i++] = (uintptr_t)p;
     }
+    return i;
 }
 
-static void set_step_pointers(const int* blks, int nblk, const std::vector<uintptr_t>& v)
+static void set_step_pointers_from(const int* blks, int nblk, const uintptr_t* v)
 {
     size_t i = 0;
     for (int n = 0; n < nblk; ++n) {
@@ -1357,18 +1412,30 @@ static void set_step_pointers(const int* blks, int nblk, const std::vector<uintp
     }
 }
 
-// Everything one step puts on the library's stream (counter reset, the stages'
-// ghost fills, detector and stage kernels), with the host's buffer swaps.
-static int enqueue_step(const int* blks, int nblk, double dt, int ns)
+static void step_pointers(const int* blks, int nblk, std::vector<uintptr_t>& v)
+{
+    const size_t o = v.size();
+    v.resize(o + (size_t)PTRS_PER_BLOCK * nblk);
+    step_pointers_to(blks, nblk, v.data() + o);
+}
+
+static void set_step_pointers(const int* blks, int nblk, const std::vector<uintptr_t>& v)
+{
+    set_step_pointers_from(blks, nblk, v.data());
+}
+
+// Everything one step puts on the library's stream (flag and counter reset,
+// the stages' ghost fills, detector and stage kernels), with the host's
+// swaps of the shock-indicator buffers.  counters: the block-stage counter
+// slots; set: the batch staging set (0 eager, 1 and 2 captured graphs);
+// batch: whether ideal-gas blocks may share a launch.
+static int enqueue_step(const int* blks, int nblk, double dt, int ns, unsigned int* counters, int set, bool batch)
 {
     int rc = 0;
     const int nslots = nblk * 3;
-    hipLaunchKernelGGL(e4::reset_counters_kernel, dim3(1), dim3(1024), 0, g_state.stream, g_state.counters,
-                       e4::COUNTER_SLOT * nslots, nullptr);
+    hipLaunchKernelGGL(e4::reset_counters_kernel, dim3(1), dim3(1024), 0, g_state.stream, g_state.step_abort,
+                       ABORT_WORDS + e4::COUNTER_SLOT * nslots, nullptr);
     for (int stage = 1; stage <= ns; ++stage) {
-        g_state.stage_pin.assign(g_state.blocks.size(), nullptr);
-        for (size_t h = 0; h < g_state.blocks.size(); ++h)
-            if (g_state.blocks[h].alive) g_state.stage_pin[h] = g_state.blocks[h].P[0];
         // ghost cells that are stored rather than read through: the x_order-1
         // extrapolation (the other effects are applied on the fly); the ideal
         // blocks' faces in one launch when blocks are batched
@@ -1381,7 +1448,7 @@ static int enqueue_step(const int* blks, int nblk, double dt, int ns)
             for (int f = 0; f < 6; ++f)
                 if (b.gx[f] && b.bc_type[f] == E4HIP_BC_EXTRAPOLATE_1) faces |= 1 << f;
             if (!faces) continue;
-            if (!b.is_tp() && g_state.cfg.batch_blocks) {
+            if (!b.is_tp() && batch && g_state.cfg.batch_blocks) {
                 e4::StageArgs A{};
                 dim3 grid;
                 rc = stage_args_for(b, stage, dt, nullptr, 0, A, grid);
@@ -1419,8 +1486,9 @@ static int enqueue_step(const int* blks, int nblk, double dt, int ns)
         }
         if (!fill_args.empty()) {
             char *ad = nullptr, *md = nullptr;
-            rc = stage_batch_buffers(g_state.fill, stage, fill_args.data(), sizeof(e4::StageArgs) * fill_args.size(),
-                                     fill_map.data(), sizeof(int) * fill_map.size(), &ad, &md);
+            rc = stage_batch_buffers(g_state.fill, batch_slot(set, stage, 2), fill_args.data(),
+                                     sizeof(e4::StageArgs) * fill_args.size(), fill_map.data(),
+                                     sizeof(int) * fill_map.size(), &ad, &md);
             if (rc) return rc;
             e4::BatchArgs B;
             B.table = (unsigned long long)(uintptr_t)ad;
@@ -1444,35 +1512,114 @@ static int enqueue_step(const int* blks, int nblk, double dt, int ns)
         }
         // ideal-gas blocks: one batched launch per dimensionality when there
         // are several; thermally perfect blocks: their kernel sequence each
-        std::vector<const BlockH*> batch[2];
+        std::vector<const BlockH*> bl[2];
         std::vector<unsigned int*> bcount[2];
         for (int n = 0; n < nblk; ++n) {
             BlockH& b = *get_block(blks[n]);
-            unsigned int* cnt = g_state.counters + e4::COUNTER_SLOT * (n * 3 + (stage - 1));
+            unsigned int* cnt = counters + e4::COUNTER_SLOT * (n * 3 + (stage - 1));
             if (b.is_tp()) {
                 rc = launch_tp_stage(b, stage, dt, cnt);
                 if (rc) return rc;
-            } else {
-                batch[b.dims - 2].push_back(&b);
+            } else if (batch) {
+                bl[b.dims - 2].push_back(&b);
                 bcount[b.dims - 2].push_back(cnt);
+            } else {
+                rc = launch_stage(b, stage, dt, cnt);
+                if (rc) return rc;
             }
         }
         for (int d = 0; d < 2; ++d) {
-            if (g_state.cfg.batch_blocks && (int)batch[d].size() >= g_state.batch_min_blocks) {
-                rc = launch_stage_batch(batch[d], bcount[d], stage, dt);
+            if (g_state.cfg.batch_blocks && (int)bl[d].size() >= g_state.batch_min_blocks) {
+                rc = launch_stage_batch(bl[d], bcount[d], stage, dt, set);
                 if (rc) return rc;
             } else {
-                for (size_t e = 0; e < batch[d].size(); ++e) {
-                    rc = launch_stage(*batch[d][e], stage, dt, bcount[d][e]);
+                for (size_t e = 0; e < bl[d].size(); ++e) {
+                    rc = launch_stage(*bl[d][e], stage, dt, bcount[d][e]);
                     if (rc) return rc;
                 }
             }
         }
     }
+    return E4HIP_OK;
+}
+
+// The outcome of a step from its abort flags and counters read back into h
+// (ABORT_WORDS flags, then the block-stage slots): per-stage bad-cell counts
+// summed over the blocks, and the status of simcore.d:2480-2493 -- a stage
+// with a failed decode or a block over max_invalid_cells ends the step there.
+static int step_outcome(const unsigned int* h, int nblk, int ns, int* n_bad_cells, bool* stopped)
+{
+    const int CS = e4::COUNTER_SLOT;
+    const unsigned int* slots = h + ABORT_WORDS;
+    int stage_bad[3] = {0, 0, 0};
+    bool too_many = false, decode_fail = false;
+    *stopped = false;
+    for (int s = 0; s < ns && !*stopped; ++s) {
+        for (int n = 0; n < nblk; ++n) {
+            const unsigned int* c = &slots[CS * (n * 3 + s)];
+            const unsigned int inv = c[0], dec = c[1];
+            g_state.decode_evals += c[2];
+            g_state.decode_fallbacks += c[3];
+            g_state.face_evals += c[4];
+            g_state.face_fallbacks += c[5];
+            stage_bad[s] += (int)(inv + dec);
+            if ((int)inv > g_state.cfg.max_invalid_cells) too_many = true;
+            if (dec) decode_fail = true;
+        }
+        if (too_many || decode_fail || h[s]) *stopped = true;
+    }
+    if (n_bad_cells) for (int s = 0; s < 3; ++s) n_bad_cells[s] = stage_bad[s];
+    if (decode_fail) return fail(E4HIP_ERR_DECODE, "Bad cell with negative mass.");
+    if (too_many) return fail(E4HIP_ERR_BAD_CELLS, "Too many bad cells; go home.");
+    if (*stopped) return fail(E4HIP_ERR_BAD_CELLS, "a stage stopped the step");
+    return E4HIP_OK;
+}
+
+// After a step: one that ended without bad cells makes its last stage's state
+// the current one (swap(U[0], U[stages]), simcore.d:2936-2941, and the same for
+// a mixture's primitive state); one that stopped restores every buffer pointer
+// the step started with, so U[0] holds the state before the step.
+static void finish_step(const int* blks, int nblk, int ns, bool stopped, const uintptr_t* pre)
+{
+    if (stopped) {
+        set_step_pointers_from(blks, nblk, pre);
+        return;
+    }
     for (int n = 0; n < nblk; ++n) {
         BlockH& b = *get_block(blks[n]);
-        if (ns == 1) std::swap(b.U[0], b.U[1]);   // Euler: the new state is in U[1]
+        std::swap(b.U[0], b.U[ns]);
+        if (b.is_tp()) std::swap(b.P[0], b.P[ns]);
     }
+}
+
+// Capture of one step into a graph (batch staging set `set`); the graph is
+// launched once here.
+static int capture_step(const int* blks, int nblk, double dt, int ns, int set, const std::vector<uintptr_t>& key,
+                        const std::vector<uintptr_t>& pre)
+{
+    HIP_TRY(hipStreamBeginCapture(g_state.stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(blks, nblk, dt, ns, g_state.counters, set, true);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(g_state.stream, &graph);
+    if (rc || e != hipSuccess) {
+        if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+        set_step_pointers(blks, nblk, pre);
+        if (rc) return rc;
+        HIP_TRY(e);
+    }
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) set_step_pointers(blks, nblk, pre);
+    HIP_TRY(e);
+    StepGraph G;
+    G.key = key;
+    step_pointers(blks, nblk, G.post);
+    G.exec = ex;
+    G.set = set;
+    G.last_use = g_state.step_counter;
+    g_state.graphs.push_back(G);
+    HIP_TRY(hipGraphLaunch(ex, g_state.stream));
     return E4HIP_OK;
 }
 
@@ -1486,82 +1633,71 @@ int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_ce
     const int ns = nstages(g_state.cfg.gasdynamic_update_scheme);
     int rc = ensure_counters(nblk * 3);
     if (rc) return rc;
+    g_state.step_abort = g_state.abort;
+    ++g_state.step_counter;
     const int nslots = nblk * 3;
+    std::vector<uintptr_t> pre;
+    step_pointers(blks, nblk, pre);
     // use_graphs: a step whose starting state (generation, blocks, dt, buffer
-    // pointers) repeats that of the step before is captured into a graph and
-    // launched; later steps from the same state replay it.  The kernels read
-    // and write device memory only, so a replay is the same work.
+    // pointers) was that of one of the last two steps is captured into a graph
+    // and launched; later steps from the same state replay it.  Consecutive
+    // steps alternate between two states (the swap of U[0] and U[stages]), so
+    // two graphs are kept, each with its own batch staging set.
     const bool graphs = g_state.cfg.use_graphs != 0 && !g_state.profiling;
     std::vector<uintptr_t> key;
     if (graphs) {
         uint64_t dtb;
         memcpy(&dtb, &dt, sizeof(dtb));
-        key = {(uintptr_t)g_state.generation, (uintptr_t)nblk, (uintptr_t)dtb, (uintptr_t)g_state.counters,
+        key = {(uintptr_t)g_state.generation, (uintptr_t)nblk, (uintptr_t)dtb, (uintptr_t)g_state.counters_raw,
                (uintptr_t)g_state.batch.args_dev, (uintptr_t)g_state.batch.map_dev,
                (uintptr_t)g_state.fill.args_dev, (uintptr_t)g_state.fill.map_dev};
         for (int n = 0; n < nblk; ++n) key.push_back((uintptr_t)blks[n]);
-        step_pointers(blks, nblk, key);
+        key.insert(key.end(), pre.begin(), pre.end());
     }
-    if (graphs && g_state.graph_exec && key == g_state.graph_key) {
-        HIP_TRY(hipGraphLaunch(g_state.graph_exec, g_state.stream));
-        set_step_pointers(blks, nblk, g_state.graph_post);
+    StepGraph* hit = nullptr;
+    if (graphs)
+        for (auto& G : g_state.graphs)
+            if (G.key == key) hit = &G;
+    if (hit) {
+        HIP_TRY(hipGraphLaunch(hit->exec, g_state.stream));
+        hit->last_use = g_state.step_counter;
+        set_step_pointers(blks, nblk, hit->post);
     } else {
-        // the batched launches stage their arguments in pinned host buffers that
-        // a captured graph copies from when it runs: a step enqueued now rewrites
-        // them, so the graph captured before it is dropped
-        if (g_state.graph_exec) {
-            HIP_TRY(hipGraphExecDestroy(g_state.graph_exec));
-            g_state.graph_exec = nullptr;
-            g_state.graph_key.clear();
-        }
-        const bool capture = graphs && key == g_state.graph_last_key;
-        if (capture) HIP_TRY(hipStreamBeginCapture(g_state.stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_step(blks, nblk, dt, ns);
-        if (capture) {
-            hipGraph_t graph = nullptr;
-            hipError_t e = hipStreamEndCapture(g_state.stream, &graph);
+        bool seen = false;
+        for (auto& k : g_state.recent_keys) if (k == key) seen = true;
+        if (graphs && seen) {
+            // a free staging set, or the least recently used graph's
+            int set = 1;
+            if (g_state.graphs.size() >= 2) {
+                size_t lru = (g_state.graphs[0].last_use <= g_state.graphs[1].last_use) ? 0 : 1;
+                set = g_state.graphs[lru].set;
+                (void)hipGraphExecDestroy(g_state.graphs[lru].exec);
+                g_state.graphs.erase(g_state.graphs.begin() + lru);
+            } else if (g_state.graphs.size() == 1) {
+                set = 3 - g_state.graphs[0].set;
+            }
+            rc = capture_step(blks, nblk, dt, ns, set, key, pre);
+            if (rc) return rc;
+        } else {
+            rc = enqueue_step(blks, nblk, dt, ns, g_state.counters, 0, true);
             if (rc) {
-                if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+                set_step_pointers(blks, nblk, pre);
                 return rc;
             }
-            HIP_TRY(e);
-            hipGraphExec_t ex = nullptr;
-            e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            HIP_TRY(e);
-            g_state.graph_exec = ex;
-            g_state.graph_key = key;
-            g_state.graph_post.clear();
-            step_pointers(blks, nblk, g_state.graph_post);
-            HIP_TRY(hipGraphLaunch(ex, g_state.stream));
-        } else if (rc) {
-            return rc;
+            if (graphs) {
+                g_state.recent_keys.push_back(key);
+                if (g_state.recent_keys.size() > 2) g_state.recent_keys.erase(g_state.recent_keys.begin());
+            }
         }
-        if (graphs) g_state.graph_last_key = key;
     }
-    const int CS = e4::COUNTER_SLOT;
-    const unsigned int* h = g_state.counters_host;
-    HIP_TRY(hipMemcpyAsync(g_state.counters_host, g_state.counters, sizeof(unsigned int) * CS * nslots,
-                           hipMemcpyDeviceToHost, g_state.stream));
+    HIP_TRY(hipMemcpyAsync(g_state.counters_host, g_state.counters_raw,
+                           sizeof(unsigned int) * (ABORT_WORDS + e4::COUNTER_SLOT * nslots), hipMemcpyDeviceToHost,
+                           g_state.stream));
     HIP_TRY(hipStreamSynchronize(g_state.stream));
-    int stage_bad[3] = {0, 0, 0};
-    bool too_many = false, decode_fail = false;
-    for (int n = 0; n < nblk; ++n)
-        for (int s = 0; s < 3; ++s) {
-            const unsigned int* c = &h[CS * (n * 3 + s)];
-            unsigned int inv = c[0], dec = c[1];
-            g_state.decode_evals += c[2];
-            g_state.decode_fallbacks += c[3];
-            g_state.face_evals += c[4];
-            g_state.face_fallbacks += c[5];
-            stage_bad[s] += (int)(inv + dec);
-            if ((int)inv > g_state.cfg.max_invalid_cells) too_many = true;
-            if (dec) decode_fail = true;
-        }
-    if (n_bad_cells) for (int s = 0; s < 3; ++s) n_bad_cells[s] = stage_bad[s];
-    if (decode_fail) return fail(E4HIP_ERR_DECODE, "Bad cell with negative mass.");
-    if (too_many) return fail(E4HIP_ERR_BAD_CELLS, "Too many bad cells; go home.");
-    return E4HIP_OK;
+    bool stopped = false;
+    rc = step_outcome(g_state.counters_host, nblk, ns, n_bad_cells, &stopped);
+    finish_step(blks, nblk, ns, stopped, pre.data());
+    return rc;
 }
 
 int e4hip_determine_dt(const int* blks, int nblk, double dt_current, double cfl_value, int check_cfl,

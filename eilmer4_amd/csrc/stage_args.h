@@ -131,7 +131,41 @@ struct StageArgs {
     // user-defined source terms per cell, constant in time, in U's layout
     // (null: none); added to the time derivative after the axisymmetric term
     const double* src;
+    // bad-cell stop of a step (simcore.d:2480-2493): every kernel of a stage
+    // whose abort_prev word is set (the stage before counted more invalid
+    // cells in some block than max_invalid, or a decode failed) does nothing;
+    // this stage's counts set abort_this
+    const unsigned int* abort_prev;
+    unsigned int* abort_this;
+    unsigned int max_invalid;
 };
+
+// Whether the stage before this one stopped the step: the word is read with
+// an agent-scope load (the stage before wrote it with device atomics), once
+// per wave.
+__device__ __forceinline__ bool stage_aborted(const unsigned int* abort_prev)
+{
+    if (!abort_prev) return false;
+    const unsigned int v = __hip_atomic_load(abort_prev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __builtin_amdgcn_readfirstlane(v) != 0u;
+}
+
+// The wave leader's share of a stage's counts: invalid cells and decode
+// failures added to the block's counters; the step stops after this stage
+// when the block's invalid count passes max_invalid (the atomic's old value
+// tells which wave pushed it over) or any decode failed.
+__device__ __forceinline__ void count_bad_cells(unsigned int* counters, unsigned int* abort_this, unsigned int max_invalid,
+                                                unsigned int nbad, unsigned int nfail)
+{
+    if (nbad) {
+        const unsigned int old = atomicAdd(&counters[0], nbad);
+        if (abort_this && old + nbad > max_invalid) atomicOr(abort_this, 1u);
+    }
+    if (nfail) {
+        atomicAdd(&counters[1], nfail);
+        if (abort_this) atomicOr(abort_this, 1u);
+    }
+}
 
 // Per-radian geometry of a 2D axisymmetric block, laid out as above.
 struct AxiGeom {
@@ -205,6 +239,9 @@ struct TPArgs {
     const unsigned char* gS[6];
     const double* axi;                // 2D axisymmetric geometry, as in StageArgs
     const double* src;                // user-defined source terms, as in StageArgs
+    const unsigned int* abort_prev;   // bad-cell stop of the step, as in StageArgs
+    unsigned int* abort_this;
+    unsigned int max_invalid;
 };
 
 // Launchers, one per arithmetic mode; each lives in its own translation unit

@@ -422,10 +422,8 @@ __device__ __forceinline__ void finish_cell(KArgs ap, long c, const double integ
 #pragma unroll
     for (int q = 0; q < 5; ++q) ap->Unew[c + q * qs] = Un[q];
     unsigned int nbad = wave_count(invalid), nfail = wave_count(decode_fail);
-    if ((nbad | nfail) && (__lane_id() == __ffsll((unsigned long long)__ballot(true)) - 1)) {
-        if (nbad) atomicAdd(&ap->counters[0], nbad);
-        if (nfail) atomicAdd(&ap->counters[1], nfail);
-    }
+    if ((nbad | nfail) && (__lane_id() == __ffsll((unsigned long long)__ballot(true)) - 1))
+        count_bad_cells(ap->counters, ap->abort_this, ap->max_invalid, nbad, nfail);
 }
 
 // Whether the cell is in the layers next to a one-sided boundary that the
@@ -482,6 +480,7 @@ void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type A
     __shared__ R sLs[6 * TX];                // left states of the tile's south faces
 
     KArgs ap = args();
+    if (stage_aborted(ap->abort_prev)) return;
     int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
     if constexpr (BATCH) {
         const int w = (int)blockIdx.x - ap->wg0;
@@ -869,6 +868,7 @@ __global__ __launch_bounds__(128 * DIMS) void boundary_cell_kernel(
     } else {
         A = kargs();
     }
+    if (stage_aborted(A->abort_prev)) return;
     const int lane = threadIdx.x & 63;
     const int f = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int t = wg * 64 + lane;
@@ -967,6 +967,7 @@ __global__ __launch_bounds__(128) void extrapolate_ghosts_kernel(
     } else {
         A = kargs();
     }
+    if (stage_aborted(A->abort_prev)) return;
     const int dir = (face == FACE_EAST || face == FACE_WEST) ? 0 : ((face == FACE_NORTH || face == FACE_SOUTH) ? 1 : 2);
     const bool plus = (face == FACE_EAST || face == FACE_NORTH || face == FACE_TOP);
     const int na = (dir == 0) ? A->nj : A->ni;
@@ -1130,6 +1131,7 @@ __global__ __launch_bounds__(256) void shock_detect_kernel(StageArgs)
     constexpr int TX = SD_TX, TY = SD_TY, PX = TX + 2, PY = TY + 2, PL = PX * PY;
     __shared__ double sv[2][4][PL];          // vx, vy, vz, a of a plane and its ring
     KArgs A = kargs();
+    if (stage_aborted(A->abort_prev)) return;
     const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
     const int k0 = (DIMS == 3) ? blockIdx.z * SD_KC : 0;
     const int k1 = (DIMS == 3) ? min(k0 + SD_KC, A->nk) : 1;

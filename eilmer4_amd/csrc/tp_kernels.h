@@ -309,6 +309,7 @@ __global__ __launch_bounds__(128) void tp_face_kernel(TPArgs A)
 {
     using MA = typename TPArith<M>::type;
     constexpr bool kPolish = TPArith<M>::polish;
+    if (stage_aborted(A.abort_prev)) return;
     const int nfi = A.ni + (DIR == 0), nfj = A.nj + (DIR == 1), nfk = (A.dims == 3) ? A.nk + (DIR == 2) : 1;
     const long nf = (long)nfi * nfj * nfk;
     const long f0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -421,6 +422,7 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
     constexpr bool kPolish = TPArith<M>::polish;
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
     constexpr int NQ = 5 + NSP;
+    if (stage_aborted(A.abort_prev)) return;
     const long ncells = (long)A.ni * A.nj * A.nk;
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = c < ncells;
@@ -580,8 +582,7 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
     int fsum_w = fe;
     for (int off = 32; off > 0; off >>= 1) fsum_w += __shfl_xor(fsum_w, off);
     if (__lane_id() == 0) {
-        if (nbad) atomicAdd(&A.counters[0], nbad);
-        if (nfail) atomicAdd(&A.counters[1], nfail);
+        count_bad_cells(A.counters, A.abort_this, A.max_invalid, nbad, nfail);
         if (fsum_w) atomicAdd(&A.counters[2], (unsigned int)fsum_w);
         if (nfb) atomicAdd(&A.counters[3], nfb);
     }
@@ -669,6 +670,7 @@ template <class M>
 __global__ __launch_bounds__(128) void tp_shock_detect_kernel(TPArgs A)
 {
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    if (stage_aborted(A.abort_prev)) return;
     const long n = (long)A.ni * A.nj * A.nk;
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
@@ -731,6 +733,7 @@ __global__ __launch_bounds__(64) void tp_boundary_face_kernel(TPArgs A)
 {
     using MA = typename TPArith<M>::type;
     constexpr bool kPolish = TPArith<M>::polish;
+    if (stage_aborted(A.abort_prev)) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= A.fix_off[6]) return;
     int slab = 0;
@@ -902,6 +905,7 @@ __global__ __launch_bounds__(128) void tp_extrap1_kernel(TPArgs A, int face)
 {
     using MA = typename TPArith<M>::type;
     constexpr int NSP = 5;
+    if (stage_aborted(A.abort_prev)) return;
     const int dir = (face == FACE_EAST || face == FACE_WEST) ? 0 : ((face == FACE_NORTH || face == FACE_SOUTH) ? 1 : 2);
     const bool plus = (face == FACE_EAST || face == FACE_NORTH || face == FACE_TOP);
     const int na = (dir == 0) ? A.nj : A.ni;

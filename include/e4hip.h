@@ -244,17 +244,23 @@ int e4hip_block_ncells(int blk, long* ncells, int* nq);
 int e4hip_config(const E4hipStepConfig* c);
 
 /* gasdynamic_explicit_increment_with_fixed_grid() (simcore.d:2303-2952) for
- * the listed blocks and time step dt: every stage of the configured scheme,
- * ending with U[0] holding the new state (the U swap of simcore.d:2936-2941).
+ * the listed blocks and time step dt: every stage of the configured scheme
+ * (stage s writes U[s] from U[s-1]), ending with U[0] holding the new state
+ * (the swap of U[0] and U[stages], simcore.d:2936-2941).
  * n_bad_cells (may be NULL) receives, per stage (3 ints), the invalid-cell
- * count summed over the blocks (count_invalid_cells, fluidblock.d:433-499).
- * Returns E4HIP_ERR_BAD_CELLS when a stage's count of any block exceeds
- * max_invalid_cells and E4HIP_ERR_DECODE when a decode would throw; as in the
- * reference, the caller then ends the run. */
+ * count (count_invalid_cells, fluidblock.d:433-499) plus failed decodes,
+ * summed over the blocks.  As in the reference, the counts are checked after
+ * each stage: when a block's count exceeds max_invalid_cells the later stages
+ * do nothing and E4HIP_ERR_BAD_CELLS is returned (simcore.d:2480-2493,
+ * 2694-2696, 2897-2899); a decode that would throw (non-positive density)
+ * gives E4HIP_ERR_DECODE the same way.  Either way U[0] (and a mixture's flow
+ * state) is left as it was before the call, and the caller ends the run. */
 int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_cells);
 
-/* determine_time_step_size for the listed blocks (non-stringent CFL,
- * fvcell.d:1624-1705; fluidblock.d:776-862): dt_allow = min over cells of
+/* determine_time_step_size for the listed blocks (fvcell.d:1624-1705;
+ * fluidblock.d:776-862): the CFL signal of each cell is the non-stringent one
+ * or, with E4hipStepConfig.stringent_cfl, the fastest normal velocity over the
+ * shortest cell length (fvcell.d:1642-1650); dt_allow = min over cells of
  * cfl_value/signal and cfl_max = max over cells of dt_current*signal,
  * reduced over blocks (simcore.d:1385-1394), with the reference's rescaling
  * when check_cfl and cfl_max exceeds cfl_allow. */

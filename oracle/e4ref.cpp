@@ -24,6 +24,7 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <utility>
 
 extern "C" {
 
@@ -67,7 +68,6 @@ struct Gas {
     // thermally perfect mix (gas/therm_perf_gas.d:41-110)
     double R[NSP_MAX];
     CEACurve curve[NSP_MAX];
-    long newton_f_evals;   // instrumentation: zeroFn evaluations
 };
 
 struct GasState { double rho, p, T, u, a; double massf[NSP_MAX]; };
@@ -108,7 +108,6 @@ Gas* ref_gas_new_tp(int nsp, const double* M, int nseg, const double* Tbreaks,
 }
 
 void ref_gas_free(Gas* g) { free(g); }
-long ref_gas_newton_evals(Gas* g) { long n = g->newton_f_evals; g->newton_f_evals = 0; return n; }
 
 // cea_thermo_curves.d:136-169 (returns 0 on success)
 static int determineCoefficients(const CEACurve& cv, double T, double a[9])
@@ -192,7 +191,7 @@ static void tp_update_energy(const Gas* g, GasState& Q)
 static int tp_newton_solve(Gas* g, GasState& Q, double e_tgt, double x0, double xMin, double xMax,
                            double tol, double* root)
 {
-    auto f = [&](double T) { Q.T = T; tp_update_energy(g, Q); g->newton_f_evals++; return e_tgt - Q.u; };
+    auto f = [&](double T) { Q.T = T; tp_update_energy(g, Q); return e_tgt - Q.u; };
     auto dfdx = [&](double T) {
         double Cv[NSP_MAX];
         for (int s = 0; s < g->nsp; ++s) Cv[s] = eval_Cp(g->curve[s], T) - g->R[s];
@@ -1083,6 +1082,33 @@ void ref_interp_onesided_scalar(int kind, double len, const double* q, int apply
 
 // ------------------------------------------------------------------- blocks
 
+// Large per-slot arrays are sized without value-initialisation and then
+// filled by all threads, so that their pages are first touched in parallel
+// (a 256^3 block holds about 10 GB of them).
+extern "C++" {
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U> struct rebind { using other = NoInitAlloc<U>; };
+    NoInitAlloc() = default;
+    template <class U> NoInitAlloc(const NoInitAlloc<U>&) {}
+    template <class U, class... A> void construct(U* p, A&&... a)
+    {
+        if constexpr (sizeof...(A) == 0) ::new ((void*)p) U;
+        else ::new ((void*)p) U(std::forward<A>(a)...);
+    }
+};
+template <class T> using BigVec = std::vector<T, NoInitAlloc<T>>;
+
+template <class T>
+static void par_fill(BigVec<T>& v, size_t n, const T& value)
+{
+    v.resize(n);
+    T* d = v.data();
+    #pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; ++i) d[i] = value;
+}
+}   // extern "C++"
+
 struct Block;
 // profile: the ghost states of a static profile, [layer][a + b * na] over the
 // face's in-face axes (north/south (i, k), east/west (j, k), top/bottom (i, j))
@@ -1096,9 +1122,9 @@ struct Block {
     int nq;
     double dx, dy, dz, vol, area[3];
     Frame frame[3];
-    std::vector<FlowState> fs;
-    std::vector<double> U[4], dUdt[4];
-    std::vector<double> F[3];     // face fluxes, indexed by the cell on the + side
+    BigVec<FlowState> fs;
+    BigVec<double> U[4], dUdt[4];
+    BigVec<double> F[3];          // face fluxes, indexed by the cell on the + side
     std::vector<char> bad;
     BC bc[6];
     // 2D axisymmetric geometry (per radian): cell volumes and (x,y)-plane
@@ -1149,9 +1175,9 @@ Block* ref_block_new(Gas* g, int dims, int ni, int nj, int nk, double dx, double
     init.gas.p = 100.0e3; init.gas.T = 300.0; init.gas.massf[0] = 1.0;
     update_thermo_from_pT(g, init.gas);
     update_sound_speed(g, init.gas);
-    b->fs.assign(n, init);
-    for (int t = 0; t < 4; ++t) { b->U[t].assign(n * b->nq, 0.0); b->dUdt[t].assign(n * b->nq, 0.0); }
-    for (int d = 0; d < 3; ++d) b->F[d].assign(n * b->nq, 0.0);
+    par_fill(b->fs, n, init);
+    for (int t = 0; t < 4; ++t) { par_fill(b->U[t], n * b->nq, 0.0); par_fill(b->dUdt[t], n * b->nq, 0.0); }
+    for (int d = 0; d < 3; ++d) par_fill(b->F[d], n * b->nq, 0.0);
     b->bad.assign(n, 0);
     for (int f = 0; f < 6; ++f) b->bc[f].type = BC_SLIP_WALL;   // prep.lua default WallBC_WithSlip
     return b;
@@ -1300,29 +1326,34 @@ int ref_block_set_from_pT(Block* b, const double* p, const double* T, const doub
 int ref_block_set_U(Block* b, const double* U, const double* Tguess)
 {
     const size_t nc = (size_t)b->ni * b->nj * b->nk;
-    size_t c = 0;
+    int bad = 0;
+    #pragma omp parallel for collapse(2) schedule(static) reduction(+:bad)
     for (int k = b->kmin; k <= b->kmax; ++k)
-        for (int j = b->jmin; j <= b->jmax; ++j)
+        for (int j = b->jmin; j <= b->jmax; ++j) {
+            size_t c = ((size_t)(k - b->kmin) * b->nj + (j - b->jmin)) * b->ni;
             for (int i = b->imin; i <= b->imax; ++i, ++c) {
                 size_t ci = cidx(b, i, j, k);
                 double* u = &b->U[0][ci * b->nq];
                 for (int q = 0; q < b->nq; ++q) u[q] = U[q * nc + c];
                 if (Tguess) b->fs[ci].gas.T = Tguess[c];
-                if (decode_conserved(b, b->fs[ci], u) != 0) return -1;
+                if (decode_conserved(b, b->fs[ci], u) != 0) bad++;
             }
-    return 0;
+        }
+    return bad ? -1 : 0;
 }
 
 void ref_block_get_U(const Block* b, int ftl, double* U)
 {
     const size_t nc = (size_t)b->ni * b->nj * b->nk;
-    size_t c = 0;
+    #pragma omp parallel for collapse(2) schedule(static)
     for (int k = b->kmin; k <= b->kmax; ++k)
-        for (int j = b->jmin; j <= b->jmax; ++j)
+        for (int j = b->jmin; j <= b->jmax; ++j) {
+            size_t c = ((size_t)(k - b->kmin) * b->nj + (j - b->jmin)) * b->ni;
             for (int i = b->imin; i <= b->imax; ++i, ++c) {
                 const double* u = &b->U[ftl][cidx(b, i, j, k) * b->nq];
                 for (int q = 0; q < b->nq; ++q) U[q * nc + c] = u[q];
             }
+        }
 }
 
 // shock indicators of the active cells, i fastest
@@ -1338,9 +1369,10 @@ void ref_block_get_S(const Block* b, int* out)
 void ref_block_get_fs(const Block* b, double* out)
 {
     const size_t nc = (size_t)b->ni * b->nj * b->nk;
-    size_t c = 0;
+    #pragma omp parallel for collapse(2) schedule(static)
     for (int k = b->kmin; k <= b->kmax; ++k)
-        for (int j = b->jmin; j <= b->jmax; ++j)
+        for (int j = b->jmin; j <= b->jmax; ++j) {
+            size_t c = ((size_t)(k - b->kmin) * b->nj + (j - b->jmin)) * b->ni;
             for (int i = b->imin; i <= b->imax; ++i, ++c) {
                 const FlowState& f = b->fs[cidx(b, i, j, k)];
                 out[0 * nc + c] = f.gas.rho; out[1 * nc + c] = f.gas.p; out[2 * nc + c] = f.gas.T;
@@ -1348,6 +1380,7 @@ void ref_block_get_fs(const Block* b, double* out)
                 out[5 * nc + c] = f.vel[0]; out[6 * nc + c] = f.vel[1]; out[7 * nc + c] = f.vel[2];
                 for (int s = 0; s < b->gas->nsp; ++s) out[(8 + s) * nc + c] = f.gas.massf[s];
             }
+        }
 }
 
 // Every cell's flow state, ghost cells included ([8 + nsp][nidim*njdim*nkdim],

@@ -45,8 +45,6 @@ def lib():
         L.ref_gas_new_tp.restype = vp
         L.ref_gas_new_tp.argtypes = [C.c_int, _dp, C.c_int, _dp, _dp, _dp]
         L.ref_gas_free.argtypes = [vp]
-        L.ref_gas_newton_evals.restype = C.c_long
-        L.ref_gas_newton_evals.argtypes = [vp]
         L.ref_cea_eval_h.restype = C.c_double
         L.ref_cea_eval_h.argtypes = [vp, C.c_int, C.c_double]
         L.ref_cea_eval_Cp.restype = C.c_double
@@ -132,9 +130,6 @@ class Gas:
         co = _f64(d["segments"]).ravel()
         h = lib().ref_gas_new_tp(1, _p(M), d["nsegments"], _p(Tb), _p(Tbl), _p(co))
         return Gas(h, 1, {"model": "curve"})
-
-    def newton_evals(self):
-        return lib().ref_gas_newton_evals(self.h)
 
     def state(self, rho=0.0, p=0.0, T=0.0, u=0.0, a=0.0, massf=None):
         q = np.zeros(5 + self.nsp)

@@ -137,3 +137,33 @@ def test_graph_dropped_on_change(gpu):
         for b in blocks:
             b.destroy()
     _compare(runs[0], runs[1], "change")
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_batched_2d_and_3d_blocks_in_one_call(gpu, graphs):
+    """Two 2D and two 3D ideal-gas blocks in one call: each dimensionality is
+    one batched launch per stage with its own argument table; the result
+    equals one launch per block, bit for bit, eager and under graph replay."""
+    print("case batch: 2D + 3D blocks in one call, graphs=%s" % graphs, flush=True)
+    from eilmer4_amd import workloads
+    gas = e4.IdealGas()
+    g3, f3, bc3, _ = workloads.shocked_box_3d(gas, n=12, seed=4)
+    g2, f2, bc2, _ = workloads.planar_bump_2d(gas, ni=40, nj=24)
+    U3 = workloads.initial_U(gas, f3)
+    U2 = workloads.initial_U(gas, f2)
+    runs = []
+    for batch in (False, True):
+        blocks = []
+        for geom, bcs, U in ((g2, bc2, U2), (g3, bc3, U3), (g2, bc2, U2 * 1.01), (g3, bc3, U3 * 0.99)):
+            b = e4.SFluidBlock(gas, geom["dims"], geom["ni"], geom["nj"], geom["nk"], geom["dx"], geom["dy"],
+                               geom["dz"])
+            for face, (kind, fs) in bcs.items():
+                b.set_bc(face, kind, fs)
+            b.upload(U)
+            blocks.append(b)
+        cfg = e4.GlobalConfig(flux_calculator=e4.FluxCalculator.ausmdv, precision_mode=e4.PrecisionMode.fp64,
+                              batch_blocks=batch, use_graphs=graphs and batch)
+        runs.append(_run(blocks, cfg, 8, fixed_dt=2.0e-6))
+        for b in blocks:
+            b.destroy()
+    _compare(runs[0], runs[1], "2d+3d")
