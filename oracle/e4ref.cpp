@@ -1134,10 +1134,208 @@ struct Block {
     // user-defined source terms per cell slot [slot * nq + q] (empty: none)
     std::vector<double> udf_src;
     bool stringent_cfl = false;     // GlobalConfig.stringent_cfl (globalconfig.d:729)
+    // geometry computed from the grid vertices (ref_block_new_grid): per slot
+    // the cell volume and i/j/k lengths (ghost slots: the lengths the
+    // reference gives its ghost cells), per face of direction d the area and
+    // the n, t1, t2 frame, indexed by the slot of the cell on the + side
+    bool general = false;
+    std::vector<double> gvol, glen[3], garea[3];
+    std::vector<Frame> gfr[3];
 };
 
 static inline size_t cidx(const Block* b, int i, int j, int k) { return ((size_t)k * b->njdim + j) * b->nidim + i; }
 static void apply_bcs(Block* b);
+
+// Geometry of a face of direction dir (indexed by its + side slot) and of a
+// cell slot: the block's uniform values, or the per-face / per-cell arrays of
+// a block built from grid vertices.
+static inline const Frame& face_frame(const Block* b, int dir, size_t slot)
+{
+    return b->general ? b->gfr[dir][slot] : b->frame[dir];
+}
+static inline double face_area(const Block* b, int dir, size_t slot)
+{
+    return b->general ? b->garea[dir][slot] : b->area[dir];
+}
+static inline double cell_len(const Block* b, int dir, size_t slot)
+{
+    if (b->general) return b->glen[dir][slot];
+    return (dir == 0) ? b->dx : (dir == 1) ? b->dy : b->dz;
+}
+static inline double cell_vol(const Block* b, size_t slot) { return b->general ? b->gvol[slot] : b->vol; }
+
+// ------------------------------------------------- cell and face geometry
+// geom/elements/vector3.d, properties.d, fvinterface.d in the reference's
+// operation order.
+struct V3 { double x, y, z; };
+
+// vector3.d:386-407: unit vector, small components flushed to zero
+static void v3_normalize(double v[3])
+{
+    double magnitude = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (magnitude > 0.0) {
+        v[0] /= magnitude; v[1] /= magnitude; v[2] /= magnitude;
+    } else {
+        v[0] = v[1] = v[2] = 0.0;
+    }
+    const double small = 1.0e-30;
+    for (int q = 0; q < 3; ++q) if (fabs(v[q]) < small) v[q] = 0.0;
+}
+
+// vector3.d:604-609: v3 = v1 x v2
+static void v3_cross(double v3[3], const double v1[3], const double v2[3])
+{
+    v3[0] = v1[1] * v2[2] - v2[1] * v1[2];
+    v3[1] = v2[0] * v1[2] - v1[0] * v2[2];
+    v3[2] = v1[0] * v2[1] - v2[0] * v1[1];
+}
+
+// properties.d:97-139: area and n, t1, t2 of the quadrilateral p0 p1 p2 p3
+// (normal by the right-hand rule over the cycle)
+static void quad_properties(const V3& p0, const V3& p1, const V3& p2, const V3& p3, Frame& fr, double& area)
+{
+    double p01x = p1.x - p0.x + p2.x - p3.x;
+    double p01y = p1.y - p0.y + p2.y - p3.y;
+    double p01z = p1.z - p0.z + p2.z - p3.z;
+    double p03x = p3.x - p0.x + p2.x - p1.x;
+    double p03y = p3.y - p0.y + p2.y - p1.y;
+    double p03z = p3.z - p0.z + p2.z - p1.z;
+    // vector3.d:634-641 cross_product
+    double vax = p01y * p03z - p01z * p03y;
+    double vay = p01z * p03x - p01x * p03z;
+    double vaz = p01x * p03y - p01y * p03x;
+    vax *= 0.25; vay *= 0.25; vaz *= 0.25;
+    area = sqrt(vax * vax + vay * vay + vaz * vaz);
+    if (area > 1.0e-20) {
+        fr.n[0] = vax / area; fr.n[1] = vay / area; fr.n[2] = vaz / area;
+        fr.t1[0] = p01x; fr.t1[1] = p01y; fr.t1[2] = p01z;
+        v3_normalize(fr.t1);
+        v3_cross(fr.t2, fr.n, fr.t1);
+        v3_normalize(fr.t2);
+    } else {
+        fr = Frame{{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+    }
+}
+
+// properties.d:293-316: volume and centroid of the tetrahedron p0 p1 p2 p3
+static void tetrahedron_properties(const V3& p0, const V3& p1, const V3& p2, const V3& p3, V3& centroid,
+                                   double& volume)
+{
+    centroid = V3{0.25 * (p0.x + p1.x + p2.x + p3.x), 0.25 * (p0.y + p1.y + p2.y + p3.y),
+                  0.25 * (p0.z + p1.z + p2.z + p3.z)};
+    double dx01 = p1.x - p0.x, dy01 = p1.y - p0.y, dz01 = p1.z - p0.z;
+    double dx02 = p2.x - p0.x, dy02 = p2.y - p0.y, dz02 = p2.z - p0.z;
+    double cx = dy01 * dz02 - dz01 * dy02;
+    double cy = dz01 * dx02 - dx01 * dz02;
+    double cz = dx01 * dy02 - dy01 * dx02;
+    double dx03 = p3.x - p0.x, dy03 = p3.y - p0.y, dz03 = p3.z - p0.z;
+    volume = (dx03 * cx + dy03 * cy + dz03 * cz) / 6.0;
+}
+
+// properties.d:390-419: the quadrilateral-based pyramid p0 p1 p2 p3 / p4 as
+// four tetrahedra about the base mid-point
+static void pyramid_properties(const V3& p0, const V3& p1, const V3& p2, const V3& p3, const V3& p4, V3& centroid,
+                               double& volume)
+{
+    V3 pmB{0.25 * (p0.x + p1.x + p2.x + p3.x), 0.25 * (p0.y + p1.y + p2.y + p3.y), 0.25 * (p0.z + p1.z + p2.z + p3.z)};
+    volume = 0.0;
+    V3 moment{0.0, 0.0, 0.0}, tc;
+    double tv;
+    const V3* base[5] = {&p0, &p1, &p2, &p3, &p0};
+    for (int t = 0; t < 4; ++t) {
+        tetrahedron_properties(*base[t], *base[t + 1], pmB, p4, tc, tv);
+        volume += tv;
+        tc.x *= tv; tc.y *= tv; tc.z *= tv;
+        moment.x += tc.x; moment.y += tc.y; moment.z += tc.z;
+    }
+    if (fabs(volume) > 0.0) { moment.x /= volume; moment.y /= volume; moment.z /= volume; }
+    centroid = moment;
+}
+
+// properties.d:568-650: volume (six pyramids on the faces about the vertex
+// mean) and the i, j, k widths between opposite face mid-points
+static void hex_cell_properties(const V3 p[8], double& volume, double& iLen, double& jLen, double& kLen)
+{
+    V3 c{0.125 * (p[0].x + p[1].x + p[2].x + p[3].x + p[4].x + p[5].x + p[6].x + p[7].x),
+         0.125 * (p[0].y + p[1].y + p[2].y + p[3].y + p[4].y + p[5].y + p[6].y + p[7].y),
+         0.125 * (p[0].z + p[1].z + p[2].z + p[3].z + p[4].z + p[5].z + p[6].z + p[7].z)};
+    auto mid = [&](int a, int b2, int c2, int d) {
+        return V3{0.25 * (p[a].x + p[b2].x + p[c2].x + p[d].x), 0.25 * (p[a].y + p[b2].y + p[c2].y + p[d].y),
+                  0.25 * (p[a].z + p[b2].z + p[c2].z + p[d].z)};
+    };
+    V3 pmN = mid(3, 2, 6, 7), pmE = mid(1, 2, 6, 5), pmS = mid(0, 1, 5, 4);
+    V3 pmW = mid(0, 3, 7, 4), pmT = mid(4, 5, 6, 7), pmB = mid(0, 1, 2, 3);
+    double dx = pmE.x - pmW.x, dy = pmE.y - pmW.y, dz = pmE.z - pmW.z;
+    iLen = sqrt(dx * dx + dy * dy + dz * dz);
+    dx = pmN.x - pmS.x; dy = pmN.y - pmS.y; dz = pmN.z - pmS.z;
+    jLen = sqrt(dx * dx + dy * dy + dz * dz);
+    dx = pmT.x - pmB.x; dy = pmT.y - pmB.y; dz = pmT.z - pmB.z;
+    kLen = sqrt(dx * dx + dy * dy + dz * dz);
+    static const int pyr[6][4] = {{6, 7, 3, 2}, {5, 6, 2, 1}, {4, 5, 1, 0}, {7, 4, 0, 3}, {7, 6, 5, 4}, {0, 1, 2, 3}};
+    volume = 0.0;
+    V3 sc;
+    double sv;
+    for (int q = 0; q < 6; ++q) {
+        pyramid_properties(p[pyr[q][0]], p[pyr[q][1]], p[pyr[q][2]], p[pyr[q][3]], c, sc, sv);
+        volume += sv;
+    }
+    if ((volume < 0.0 && fabs(volume) < 1.0e-12) || (volume >= 0.0 && volume < 1.0e-20)) volume = 0.0;
+}
+
+// properties.d:177-237: (x,y)-plane area and i, j widths of the quadrilateral
+// cell p0 p1 p2 p3
+static void xyplane_quad_cell_properties(const V3& p0, const V3& p1, const V3& p2, const V3& p3, double& area,
+                                         double& iLen, double& jLen)
+{
+    double xA = p1.x, yA = p1.y, xB = p2.x, yB = p2.y, xC = p3.x, yC = p3.y, xD = p0.x, yD = p0.y;
+    area = 0.5 * ((xB + xA) * (yB - yA) + (xC + xB) * (yC - yB) + (xD + xC) * (yD - yC) + (xA + xD) * (yA - yD));
+    double xN = 0.5 * (xC + xB), yN = 0.5 * (yC + yB);
+    double xS = 0.5 * (xD + xA), yS = 0.5 * (yD + yA);
+    double xE = 0.5 * (xA + xB), yE = 0.5 * (yA + yB);
+    double xW = 0.5 * (xD + xC), yW = 0.5 * (yD + yC);
+    double dx = xN - xS, dy = yN - yS;
+    jLen = sqrt(dx * dx + dy * dy);
+    dx = xE - xW; dy = yE - yW;
+    iLen = sqrt(dx * dx + dy * dy);
+}
+
+// fvinterface.d:265-297 (planar): the face from A to B, normal to the right
+static void face_2d_properties(const V3& A, const V3& B, Frame& fr, double& area)
+{
+    double LAB = sqrt((B.x - A.x) * (B.x - A.x) + (B.y - A.y) * (B.y - A.y));
+    if (LAB > 1.0e-12) {
+        fr.n[0] = (B.y - A.y) / LAB; fr.n[1] = -(B.x - A.x) / LAB; fr.n[2] = 0.0;
+        fr.t2[0] = 0.0; fr.t2[1] = 0.0; fr.t2[2] = 1.0;
+        v3_cross(fr.t1, fr.n, fr.t2);
+        area = LAB;
+    } else {
+        fr = Frame{{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+        area = 0.0;
+    }
+}
+
+// Ghost-cell lengths of compute_primary_cell_geometric_data
+// (sfluidblock.d:673-713): the two ghost layers beside each face copy the
+// lengths of the first and second interior cells.
+static void mirror_ghost_lengths(Block* b)
+{
+    const int ndir = (b->cfg.dims == 3) ? 3 : 2;
+    const int lo[3] = {b->imin, b->jmin, b->kmin}, hi[3] = {b->imax, b->jmax, b->kmax};
+    for (int dir = 0; dir < ndir; ++dir) {
+        const int a1 = (dir + 1) % 3, a2 = (dir + 2) % 3;
+        for (int q2 = lo[a2]; q2 <= hi[a2]; ++q2)
+            for (int q1 = lo[a1]; q1 <= hi[a1]; ++q1)
+                for (int side = 0; side < 2; ++side)
+                    for (int layer = 1; layer <= 2; ++layer) {
+                        int g[3], s[3];
+                        g[a1] = s[a1] = q1; g[a2] = s[a2] = q2;
+                        g[dir] = side ? hi[dir] + layer : lo[dir] - layer;
+                        s[dir] = side ? hi[dir] - layer + 1 : lo[dir] + layer - 1;
+                        const size_t gc = cidx(b, g[0], g[1], g[2]), sc = cidx(b, s[0], s[1], s[2]);
+                        for (int d = 0; d < 3; ++d) b->glen[d][gc] = b->glen[d][sc];
+                    }
+    }
+}
 
 // sfluidblock.d:94-126, 205-209 (ghost layers = 2); geometry for a uniform
 // Cartesian grid (the values properties.d:97-139, 568-650 produce for it).
@@ -1185,6 +1383,116 @@ Block* ref_block_new(Gas* g, int dims, int ni, int nj, int nk, double dx, double
 
 void ref_block_free(Block* b) { delete b; }
 
+// A block whose geometry is computed from its grid vertices
+// (compute_primary_cell_geometric_data, sfluidblock.d:650-842, with the cell
+// and face vertex orders of sfluidblock.d:486-645): V holds (ni+1)(nj+1)(nk+1)
+// vertices (x, y, z), i fastest (3D; in 2D (ni+1)(nj+1) with z ignored).
+// Returns null when a cell has no positive volume.
+Block* ref_block_new_grid(Gas* g, int dims, int ni, int nj, int nk, const double* V)
+{
+    Block* b = ref_block_new(g, dims, ni, nj, nk, 1.0, 1.0, 1.0);
+    b->general = true;
+    const size_t n = (size_t)b->nidim * b->njdim * b->nkdim;
+    b->gvol.assign(n, 0.0);
+    for (int d = 0; d < 3; ++d) {
+        b->glen[d].assign(n, 0.0);
+        b->garea[d].assign(n, 0.0);
+        b->gfr[d].assign(n, Frame{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}});
+    }
+    const int nkv = (dims == 3) ? nk + 1 : 1;
+    (void)nkv;
+    auto vtx = [&](int i, int j, int k) {
+        const double* q = V + 3 * (((size_t)k * (nj + 1) + j) * (ni + 1) + i);
+        return V3{q[0], q[1], (dims == 3) ? q[2] : 0.0};
+    };
+    const int nkc = (dims == 3) ? nk : 1;
+    for (int k = 0; k < nkc; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i) {
+                const size_t c = cidx(b, i + b->imin, j + b->jmin, k + b->kmin);
+                double vol, iL, jL, kL = 0.0;
+                if (dims == 3) {
+                    const V3 p[8] = {vtx(i, j, k), vtx(i + 1, j, k), vtx(i + 1, j + 1, k), vtx(i, j + 1, k),
+                                     vtx(i, j, k + 1), vtx(i + 1, j, k + 1), vtx(i + 1, j + 1, k + 1),
+                                     vtx(i, j + 1, k + 1)};
+                    hex_cell_properties(p, vol, iL, jL, kL);
+                } else {
+                    xyplane_quad_cell_properties(vtx(i, j, 0), vtx(i + 1, j, 0), vtx(i + 1, j + 1, 0), vtx(i, j + 1, 0),
+                                                 vol, iL, jL);
+                }
+                if (!(vol > 0.0)) { delete b; return nullptr; }
+                b->gvol[c] = vol;
+                b->glen[0][c] = iL; b->glen[1][c] = jL; b->glen[2][c] = kL;
+            }
+    // i-faces: west faces of cells i (normal towards +i); j-faces south, k-faces bottom
+    for (int k = 0; k < nkc; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i <= ni; ++i) {
+                const size_t f = cidx(b, i + b->imin, j + b->jmin, k + b->kmin);
+                if (dims == 3) quad_properties(vtx(i, j, k), vtx(i, j + 1, k), vtx(i, j + 1, k + 1), vtx(i, j, k + 1),
+                                               b->gfr[0][f], b->garea[0][f]);
+                else face_2d_properties(vtx(i, j, 0), vtx(i, j + 1, 0), b->gfr[0][f], b->garea[0][f]);
+            }
+    for (int k = 0; k < nkc; ++k)
+        for (int j = 0; j <= nj; ++j)
+            for (int i = 0; i < ni; ++i) {
+                const size_t f = cidx(b, i + b->imin, j + b->jmin, k + b->kmin);
+                if (dims == 3) quad_properties(vtx(i, j, k), vtx(i, j, k + 1), vtx(i + 1, j, k + 1), vtx(i + 1, j, k),
+                                               b->gfr[1][f], b->garea[1][f]);
+                else face_2d_properties(vtx(i + 1, j, 0), vtx(i, j, 0), b->gfr[1][f], b->garea[1][f]);
+            }
+    if (dims == 3)
+        for (int k = 0; k <= nk; ++k)
+            for (int j = 0; j < nj; ++j)
+                for (int i = 0; i < ni; ++i) {
+                    const size_t f = cidx(b, i + b->imin, j + b->jmin, k + b->kmin);
+                    quad_properties(vtx(i, j, k), vtx(i + 1, j, k), vtx(i + 1, j + 1, k), vtx(i, j + 1, k),
+                                    b->gfr[2][f], b->garea[2][f]);
+                }
+    mirror_ghost_lengths(b);
+    return b;
+}
+
+// Geometry of a block built from its grid: cell volumes and i/j/k lengths
+// [4][ncells] and, per direction, face areas and frames [10][nfaces(d)]
+// (area, n, t1, t2), faces of direction d indexed i fastest over
+// (ni + [d==0]) x (nj + [d==1]) x (nk + [d==2]).
+int ref_block_get_geometry(const Block* b, double* cells, double* faces0, double* faces1, double* faces2)
+{
+    if (!b->general) return -1;
+    const int nkc = (b->cfg.dims == 3) ? b->nk : 1;
+    const size_t nc = (size_t)b->ni * b->nj * nkc;
+    size_t c = 0;
+    for (int k = 0; k < nkc; ++k)
+        for (int j = 0; j < b->nj; ++j)
+            for (int i = 0; i < b->ni; ++i, ++c) {
+                const size_t s = cidx(b, i + b->imin, j + b->jmin, k + b->kmin);
+                cells[c] = b->gvol[s];
+                for (int d = 0; d < 3; ++d) cells[(1 + d) * nc + c] = b->glen[d][s];
+            }
+    double* out[3] = {faces0, faces1, faces2};
+    const int ndir = (b->cfg.dims == 3) ? 3 : 2;
+    for (int d = 0; d < ndir; ++d) {
+        if (!out[d]) continue;
+        const int fi = b->ni + (d == 0), fj = b->nj + (d == 1), fk = nkc + (d == 2);
+        const size_t nf = (size_t)fi * fj * fk;
+        size_t f = 0;
+        for (int k = 0; k < fk; ++k)
+            for (int j = 0; j < fj; ++j)
+                for (int i = 0; i < fi; ++i, ++f) {
+                    const size_t s = cidx(b, i + b->imin, j + b->jmin, k + b->kmin);
+                    const Frame& fr = b->gfr[d][s];
+                    out[d][f] = b->garea[d][s];
+                    for (int q = 0; q < 3; ++q) {
+                        out[d][(1 + q) * nf + f] = fr.n[q];
+                        out[d][(4 + q) * nf + f] = fr.t1[q];
+                        out[d][(7 + q) * nf + f] = fr.t2[q];
+                    }
+                }
+    }
+    return 0;
+}
+
 void ref_block_set_shock_tolerances(Block* b, double shear_tolerance, double compression_tolerance)
 {
     b->cfg.shear_tolerance = shear_tolerance;
@@ -1219,16 +1527,23 @@ void ref_block_set_bc_profile(Block* b, int face, const double* states)
     }
 }
 
-// fs vector layout: [rho,p,T,u,a,vx,vy,vz,massf...]
-void ref_block_set_bc(Block* b, int face, int type, const double* fs)
+static void face_ghost_lengths(Block* b, int face);
+
+// fs vector layout: [rho,p,T,u,a,vx,vy,vz,massf...].  Returns -1 for a
+// boundary the oracle does not provide on a block built from its grid (those
+// without ghost data, whose one-sided reconstruction takes one cell length).
+int ref_block_set_bc(Block* b, int face, int type, const double* fs)
 {
+    if (b->general && type == BC_SLIP_WALL_1) return -1;
     b->bc[face].type = type;
+    if (b->general) face_ghost_lengths(b, face);
     if (fs) {
         FlowState& f = b->bc[face].fs;
         f.gas.rho = fs[0]; f.gas.p = fs[1]; f.gas.T = fs[2]; f.gas.u = fs[3]; f.gas.a = fs[4];
         f.vel[0] = fs[5]; f.vel[1] = fs[6]; f.vel[2] = fs[7];
         for (int i = 0; i < b->gas->nsp; ++i) f.gas.massf[i] = fs[8 + i];
     }
+    return 0;
 }
 
 // fvcell.d:494-560 (ftl = 0 unless given)
@@ -1459,6 +1774,50 @@ void ref_block_connect(Block* b, int face, Block* other, int other_face, int ori
     b->bc[face].other = other;
     b->bc[face].other_face = other_face;
     b->bc[face].orientation = orientation;
+    if (b->general) face_ghost_lengths(b, face);
+}
+
+// The ghost-cell lengths of one face of a block built from its grid: mirrored
+// from the interior (sfluidblock.d:673-713), or, across a full-face
+// connection (a connected or periodic face), the i/j/k lengths of the mapped
+// cell itself, as exchange_geometry_phase2 copies them
+// (full_face_copy.d:1042-1085, CopyDataOption.grid, fvcell.d:220-230).
+static void face_ghost_lengths(Block* b, int face)
+{
+    const int dims = b->cfg.dims;
+    const int dir = (face == F_EAST || face == F_WEST) ? 0 : (face == F_NORTH || face == F_SOUTH) ? 1 : 2;
+    const bool plus = (face == F_EAST || face == F_NORTH || face == F_TOP);
+    const int lo[3] = {b->imin, b->jmin, b->kmin}, hi[3] = {b->imax, b->jmax, b->kmax};
+    const int a1 = (dir + 1) % 3, a2 = (dir + 2) % 3;
+    const BC& bc = b->bc[face];
+    for (int q2 = lo[a2]; q2 <= hi[a2]; ++q2)
+        for (int q1 = lo[a1]; q1 <= hi[a1]; ++q1)
+            for (int layer = 1; layer <= 2; ++layer) {
+                int g[3], s[3];
+                g[a1] = s[a1] = q1; g[a2] = s[a2] = q2;
+                g[dir] = plus ? hi[dir] + layer : lo[dir] - layer;
+                const size_t gc = cidx(b, g[0], g[1], g[2]);
+                const Block* src_blk = b;
+                size_t sc;
+                if (bc.type == BC_CONNECTED && bc.other) {
+                    const Block* o = bc.other;
+                    const int on[3] = {o->ni, o->nj, o->nk};
+                    const int gidx[3] = {g[0] - b->imin, g[1] - b->jmin, g[2] - b->kmin};
+                    const int fa = gidx[FACE_AX[face][0]];
+                    const int fb = (dims == 3) ? gidx[FACE_AX[face][1]] : 0;
+                    int src[3];
+                    ref_connection_source(dims, face, bc.other_face, bc.orientation, on, layer, fa, fb, src);
+                    src_blk = o;
+                    sc = cidx(o, src[0] + o->imin, src[1] + o->jmin, src[2] + o->kmin);
+                } else if (bc.type == BC_PERIODIC) {
+                    s[dir] = plus ? lo[dir] + layer - 1 : hi[dir] - layer + 1;
+                    sc = cidx(b, s[0], s[1], s[2]);
+                } else {
+                    s[dir] = plus ? hi[dir] - layer + 1 : lo[dir] + layer - 1;
+                    sc = cidx(b, s[0], s[1], s[2]);
+                }
+                for (int d = 0; d < 3; ++d) b->glen[d][gc] = cell_len(src_blk, d, sc);
+            }
 }
 
 // Ghost-cell effects (bc/ghost_cell_effect): flow_state_copy.d:66-169,
@@ -1550,7 +1909,10 @@ static void apply_bcs(Block* b)
                     default: {
                         s[dir] = plus ? hi[dir] - layer + 1 : lo[dir] + layer - 1;
                         dest = b->fs[cidx(b, s[0], s[1], s[2])];
-                        const Frame& fr = b->frame[dir];
+                        // reflected about the boundary face of the boundary cell
+                        int fpos[3] = {g[0], g[1], g[2]};
+                        fpos[dir] = plus ? hi[dir] + 1 : lo[dir];
+                        const Frame& fr = face_frame(b, dir, cidx(b, fpos[0], fpos[1], fpos[2]));
                         to_local(dest.vel, fr);
                         dest.vel[0] = -(dest.vel[0]);
                         to_global(dest.vel, fr);
@@ -1594,15 +1956,23 @@ static void detect_shock_points(Block* b)
                 const int pos[3] = {i, j, k}, lo[3] = {b->imin, b->jmin, b->kmin}, hi[3] = {b->imax, b->jmax, b->kmax};
                 int s = 0;
                 for (int d = 0; d < ndir; ++d) {
-                    const Frame& fr = b->frame[d];
+                    const Frame& fm = face_frame(b, d, c);
+                    const Frame& fp = face_frame(b, d, c + st[d]);
                     const FlowState& cs = b->fs[c];
-                    const double un = cs.vel[0] * fr.n[0] + cs.vel[1] * fr.n[1] + cs.vel[2] * fr.n[2];
                     // a boundary face without ghost data has one cell:
                     // shockdetectors.d:39-54 compares its velocity towards the wall
-                    if (pos[d] == lo[d] && !ghost_data_available(b, MINUS_FACE[d])) s |= (un / cs.gas.a) < tol;
-                    else s |= pj_shock(b->fs[c - st[d]], cs, fr, tol);   // minus face
-                    if (pos[d] == hi[d] && !ghost_data_available(b, PLUS_FACE[d])) s |= ((-un) / cs.gas.a) < tol;
-                    else s |= pj_shock(cs, b->fs[c + st[d]], fr, tol);   // plus face
+                    if (pos[d] == lo[d] && !ghost_data_available(b, MINUS_FACE[d])) {
+                        const double un = cs.vel[0] * fm.n[0] + cs.vel[1] * fm.n[1] + cs.vel[2] * fm.n[2];
+                        s |= (un / cs.gas.a) < tol;
+                    } else {
+                        s |= pj_shock(b->fs[c - st[d]], cs, fm, tol);   // minus face
+                    }
+                    if (pos[d] == hi[d] && !ghost_data_available(b, PLUS_FACE[d])) {
+                        const double un = cs.vel[0] * fp.n[0] + cs.vel[1] * fp.n[1] + cs.vel[2] * fp.n[2];
+                        s |= ((-un) / cs.gas.a) < tol;
+                    } else {
+                        s |= pj_shock(cs, b->fs[c + st[d]], fp, tol);   // plus face
+                    }
                 }
                 S[c] = s;
             }
@@ -1632,7 +2002,6 @@ static void convective_flux(Block* b)
         const bool bf_p = b->bc[PLUS_FACE[dir]].type == BC_OUTFLOW_SIMPLE_FLUX;
         const int lo = (dir == 0) ? b->imin : (dir == 1) ? b->jmin : b->kmin;
         const int hi = (dir == 0) ? b->imax : (dir == 1) ? b->jmax : b->kmax;
-        const Frame& fr = b->frame[dir];
         #pragma omp parallel for collapse(2) schedule(static)
         for (int k = b->kmin; k <= khi; ++k)
             for (int j = b->jmin; j <= jhi; ++j)
@@ -1640,6 +2009,7 @@ static void convective_flux(Block* b)
                     const int pos = (dir == 0) ? i : (dir == 1) ? j : k;
                     size_t cR0 = cidx(b, i, j, k);
                     size_t cL0 = cR0 - stride, cL1 = cL0 - stride, cR1 = cR0 + stride;
+                    const Frame& fr = face_frame(b, dir, cR0);
                     const bool wall_m = (pos == lo) && ng_m, wall_p = (pos == hi + 1) && ng_p;
                     FlowState Lft = wall_m ? b->fs[cR0] : b->fs[cL0];
                     FlowState Rght = wall_p ? b->fs[cL0] : b->fs[cR0];
@@ -1651,6 +2021,9 @@ static void convective_flux(Block* b)
                         else if (pos == hi && ng_p)
                             interp_l2r1_l1r2(g, cfg, fr, true, b->fs[cL1], b->fs[cL0], b->fs[cR0], l, Lft, Rght);
                         else if (wall_p) interp_wall_side(g, cfg, fr, b->fs[cL0], b->fs[cL1], l, Lft);
+                        else if (b->general)
+                            interp_l2r2(g, cfg, fr, b->fs[cL1], b->fs[cL0], b->fs[cR0], b->fs[cR1], cell_len(b, dir, cL1),
+                                        cell_len(b, dir, cL0), cell_len(b, dir, cR0), cell_len(b, dir, cR1), Lft, Rght);
                         else interp_l2r2(g, cfg, fr, b->fs[cL1], b->fs[cL0], b->fs[cR0], b->fs[cR1], l, l, l, l,
                                          Lft, Rght);
                     }
@@ -1673,7 +2046,7 @@ static void convective_flux(Block* b)
                         if (pos != (side ? hi : lo)) continue;
                         const size_t c = cidx(b, i, j, k);
                         const size_t face = side ? c + stride : c;
-                        outflow_flux(g, b->fs[c], side ? 1 : -1, fr, nq, &b->F[dir][face * nq]);
+                        outflow_flux(g, b->fs[c], side ? 1 : -1, face_frame(b, dir, face), nq, &b->F[dir][face * nq]);
                     }
         }
     }
@@ -1694,8 +2067,9 @@ static void time_derivatives(Block* b, size_t c, double* dUdt)
                            &b->F[0][c * nq], three_d ? &b->F[2][(c + sk) * nq] : nullptr,
                            three_d ? &b->F[2][c * nq] : nullptr};
     const double outsign[6] = {1.0, 1.0, -1.0, -1.0, 1.0, -1.0};
-    double A[6] = {b->area[1], b->area[0], b->area[1], b->area[0], b->area[2], b->area[2]};
-    double volume = b->vol, Qy = 0.0;
+    double A[6] = {face_area(b, 1, c + sj), face_area(b, 0, c + 1), face_area(b, 1, c), face_area(b, 0, c),
+                   three_d ? face_area(b, 2, c + sk) : 0.0, three_d ? face_area(b, 2, c) : 0.0};
+    double volume = cell_vol(b, c), Qy = 0.0;
     if (b->axi) {
         const int i = (int)(c % b->nidim) - b->imin, j = (int)(c / b->nidim) - b->jmin;
         const size_t cc = (size_t)j * b->ni + i;
@@ -1905,15 +2279,17 @@ void ref_block_dt(Block* b, double dt_current, double cfl_value, int check_cfl, 
     switch (nstages(b->cfg.update_scheme)) { case 1: cfl_allow = 0.9; break; case 2: cfl_allow = 1.2; break; default: cfl_allow = 1.6; }
     bool first = true;
     double dt_allow = 0, cfl_min = 0, cfl_max = 0, signal = 0, smax = 0;
-    const double iLength = b->dx, jLength = b->dy, kLength = b->dz;
+    const size_t sj = b->nidim, sk = (size_t)b->nidim * b->njdim;
     for (int k = b->kmin; k <= b->kmax; ++k)
         for (int j = b->jmin; j <= b->jmax; ++j)
             for (int i = b->imin; i <= b->imax; ++i) {
-                const FlowState& fs = b->fs[cidx(b, i, j, k)];
+                const size_t c = cidx(b, i, j, k);
+                const FlowState& fs = b->fs[c];
                 const double* v = fs.vel;
-                const Frame& fN = b->frame[1];
-                const Frame& fE = b->frame[0];
-                const Frame& fT = b->frame[2];
+                const double iLength = cell_len(b, 0, c), jLength = cell_len(b, 1, c), kLength = cell_len(b, 2, c);
+                const Frame& fN = face_frame(b, 1, c + sj);
+                const Frame& fE = face_frame(b, 0, c + 1);
+                const Frame& fT = three_d ? face_frame(b, 2, c + sk) : b->frame[2];
                 double un_N = fabs(v[0] * fN.n[0] + v[1] * fN.n[1] + v[2] * fN.n[2]);
                 double un_E = fabs(v[0] * fE.n[0] + v[1] * fE.n[1] + v[2] * fE.n[2]);
                 double un_T = three_d ? fabs(v[0] * fT.n[0] + v[1] * fT.n[1] + v[2] * fT.n[2]) : 0.0;

@@ -65,6 +65,10 @@ def lib():
         L.ref_block_set_shock_tolerances.argtypes = [vp, C.c_double, C.c_double]
         L.ref_block_get_S.argtypes = [vp, C.POINTER(C.c_int)]
         L.ref_block_set_bc.argtypes = [vp, C.c_int, C.c_int, _dp]
+        L.ref_block_set_bc.restype = C.c_int
+        L.ref_block_new_grid.restype = vp
+        L.ref_block_new_grid.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+        L.ref_block_get_geometry.argtypes = [vp, _dp, _dp, _dp, _dp]
         L.ref_block_set_bc_profile.argtypes = [vp, C.c_int, _dp]
         L.ref_block_set_axisymmetric.argtypes = [vp, C.c_double, C.c_double]
         L.ref_block_set_source_terms.argtypes = [vp, _dp]
@@ -208,13 +212,46 @@ class Block:
 
     FS_FIELDS = ["rho", "p", "T", "u", "a", "vx", "vy", "vz"]
 
-    def __init__(self, gas, dims, ni, nj, nk, dx, dy, dz=1.0):
+    def __init__(self, gas, dims, ni, nj, nk, dx, dy, dz=1.0, grid=None):
         self.gas = gas
         self.dims = dims
         self.shape = (nk if dims == 3 else 1, nj, ni)
         self.ncells = int(np.prod(self.shape))
-        self.h = lib().ref_block_new(gas.h, dims, ni, nj, nk if dims == 3 else 1, dx, dy, dz)
+        if grid is None:
+            self.h = lib().ref_block_new(gas.h, dims, ni, nj, nk if dims == 3 else 1, dx, dy, dz)
+        else:
+            V = _f64(grid)
+            nv = (ni + 1) * (nj + 1) * ((nk + 1) if dims == 3 else 1)
+            if V.size != 3 * nv:
+                raise ValueError("grid needs %d vertices (x, y, z)" % nv)
+            self.h = lib().ref_block_new_grid(gas.h, dims, ni, nj, nk if dims == 3 else 1, _p(V.ravel()))
+            if not self.h:
+                raise RuntimeError("a cell of the grid has no positive volume")
+        self.general = grid is not None
         self.nq = lib().ref_block_nq(self.h)
+
+    @staticmethod
+    def from_grid(gas, grid):
+        """A block from its grid vertices: grid [nk+1][nj+1][ni+1][3] (3D) or
+        [nj+1][ni+1][3] (2D)."""
+        g = np.asarray(grid, dtype=np.float64)
+        if g.ndim == 4:
+            return Block(gas, 3, g.shape[2] - 1, g.shape[1] - 1, g.shape[0] - 1, 1.0, 1.0, 1.0, grid=g)
+        return Block(gas, 2, g.shape[1] - 1, g.shape[0] - 1, 1, 1.0, 1.0, 1.0, grid=g)
+
+    def geometry(self):
+        """(cells [4][ncells] = volume, i/j/k lengths; faces per direction
+        [10][nfaces] = area, n, t1, t2) of a block built from its grid."""
+        nk, nj, ni = self.shape
+        cells = np.zeros((4, self.ncells))
+        faces = []
+        for d in range(self.dims):
+            nf = (ni + (d == 0)) * (nj + (d == 1)) * (nk + (d == 2))
+            faces.append(np.zeros((10, nf)))
+        ptrs = [_p(f) for f in faces] + [None] * (3 - len(faces))
+        if lib().ref_block_get_geometry(self.h, _p(cells), *ptrs) != 0:
+            raise RuntimeError("not a block built from its grid")
+        return cells, faces
 
     def __del__(self):
         try:
@@ -251,7 +288,8 @@ class Block:
         if kind == BC_INFLOW_PROFILE:
             lib().ref_block_set_bc_profile(self.h, face, _p(_f64(fs).ravel()))
             return
-        lib().ref_block_set_bc(self.h, face, kind, _p(_f64(fs)) if fs is not None else None)
+        if lib().ref_block_set_bc(self.h, face, kind, _p(_f64(fs)) if fs is not None else None) != 0:
+            raise ValueError("boundary kind %d is not available on a block built from its grid" % kind)
 
     def set_from_pT(self, p, T, vx, vy, vz, massf=None):
         args = [_f64(np.broadcast_to(a, self.shape)).ravel() for a in (p, T, vx, vy, vz)]
