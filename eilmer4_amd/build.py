@@ -8,6 +8,9 @@ Translation units, compiled in parallel:
   stage_mixed.hip   -ffp-contract=fast: the mixed FP32/FP64 ideal-gas kernels
   tp_fast.hip       -ffp-contract=fast: the plain and mixed mixture kernels and
                     the learned-guess inference
+  gen_exact.hip     -ffp-contract=off: blocks built from their grid (general
+                    structured geometry), reference roundings; their CFL kernel
+  gen_fast.hip      -ffp-contract=fast: the same in the plain and mixed forms
   e4hip_api.hip     -ffp-contract=off: C ABI, host bookkeeping, CFL/decode kernels
   mlp_train.hip     -ffp-contract=fast: training kernels of the temperature-guess network
 Packed FP32 instructions are disabled in all of them (-packed-fp32-ops).
@@ -30,6 +33,8 @@ UNITS = [("stage_exact.hip", ["-ffp-contract=off"]),
          ("stage_fast64.hip", ["-ffp-contract=fast"]),
          ("stage_mixed.hip", ["-ffp-contract=fast"]),
          ("tp_fast.hip", ["-ffp-contract=fast"]),
+         ("gen_exact.hip", ["-ffp-contract=off"]),
+         ("gen_fast.hip", ["-ffp-contract=fast"]),
          ("e4hip_api.hip", ["-ffp-contract=off"]),
          ("mlp_train.hip", ["-ffp-contract=fast"])]
 

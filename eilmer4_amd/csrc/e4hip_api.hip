@@ -73,7 +73,15 @@ struct BlockH {
     // shock indicators per cell (adaptive_hanel_ausmdv): S[0] this stage's
     // detection, S[1] the previous one
     unsigned char* S[2] = {nullptr, nullptr};
+    // a block built from its grid (e4hip_block_create_geometry): cell geometry
+    // [4][qs] (volume, i/j/k lengths) and face geometry [10][fqs[d]] (area, n,
+    // t1, t2); its stages run on the face kernels of gen_kernels.h with a
+    // flow-state array P, whatever the gas
+    bool general = false;
+    double* geo = nullptr;
+    double* fgeo[3] = {nullptr, nullptr, nullptr};
     bool is_tp() const;
+    bool has_P() const { return general || is_tp(); }
 };
 
 struct ProfEvent { int cls; hipEvent_t a, b; };
@@ -634,15 +642,19 @@ int tp_args_for(const BlockH& b, int stage, double dt, unsigned int* counters, e
     A.apply_limiter = cfg.apply_limiter;
     A.extrema_clipping = cfg.extrema_clipping;
     A.counters = counters;
-    A.gas = g_state.gases[b.gm].tp_dev;
-    A.guess = cfg.learned_eos_guess ? g_state.gases[b.gm].guess_dev : nullptr;
+    A.gas = b.is_tp() ? g_state.gases[b.gm].tp_dev : nullptr;
+    A.ig = g_state.gases[b.gm].ideal;
+    A.guess = (cfg.learned_eos_guess && b.is_tp()) ? g_state.gases[b.gm].guess_dev : nullptr;
     A.bc = b.bc_dev;
+    A.geo = b.geo;
+    for (int dd = 0; dd < 3; ++dd) A.fgeo[dd] = b.fgeo[dd];
     for (int f = 0; f < 6; ++f) {
         if (b.bc_type[f] != E4HIP_BC_CONNECTED) continue;
         const BlockH* o = get_block(b.nbr_blk[f]);
         if (!o) return fail(E4HIP_ERR_HANDLE, "face %d is connected to a destroyed block", f);
         // the other block's primitive state at the start of this stage
         A.nbrP[f] = o->P[stage - 1];
+        A.nbrGeo[f] = o->geo;
     }
     if (b.dims == 3) {
         A.vol_inv = 1.0 / (b.dx * b.dy * b.dz);
@@ -704,6 +716,17 @@ int launch_tp_stage(BlockH& b, int stage, double dt, unsigned int* counters)
     const int flux = kernel_flux(cfg.flux_calculator);
     hipEvent_t ev = nullptr;
     prof_begin(&ev);
+    if (b.general) {
+        const int nsp = b.is_tp() ? 5 : 0;
+        const bool exact = cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT ||
+                           (b.is_tp() && !tp_mixture_form(g_state.gases[b.gm].tp));
+        if (exact) e4::launch_gen_stage_exact(A, g_state.stream, flux, nsp);
+        else if (cfg.precision_mode == E4HIP_PRECISION_MIXED) e4::launch_gen_stage_mixed(A, g_state.stream, flux, nsp);
+        else e4::launch_gen_stage_fast64(A, g_state.stream, flux, nsp);
+        prof_end(0, ev);
+        HIP_TRY(hipGetLastError());
+        return E4HIP_OK;
+    }
     // the fast modes evaluate the mixture form, which needs three-segment curves
     // with shared break points; other mixtures run the reference-form kernels
     const bool mix_form = tp_mixture_form(g_state.gases[b.gm].tp);
@@ -724,7 +747,13 @@ int launch_shock_detect(const BlockH& b, int stage, double dt)
 {
     const E4hipStepConfig& cfg = g_state.cfg;
     const bool exact = cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT;
-    if (b.is_tp()) {
+    if (b.general) {
+        e4::TPArgs A;
+        int rc = tp_args_for(b, stage, dt, nullptr, A);
+        if (rc) return rc;
+        if (exact || (b.is_tp() && !tp_mixture_form(g_state.gases[b.gm].tp))) e4::launch_gen_aux_exact(A, g_state.stream, 0, nullptr);
+        else e4::launch_gen_aux_fast64(A, g_state.stream, 0, nullptr);
+    } else if (b.is_tp()) {
         e4::TPArgs A;
         int rc = tp_args_for(b, stage, dt, nullptr, A);
         if (rc) return rc;
@@ -746,7 +775,7 @@ int alloc_block_buffers(BlockH& b)
 {
     const size_t bytes = sizeof(double) * (size_t)b.qs * b.nq;
     const int ns = nstages(g_state.cfg.gasdynamic_update_scheme);
-    if (b.is_tp()) {
+    if (b.has_P()) {
         const size_t pbytes = sizeof(double) * (size_t)b.qs * (8 + b.nsp);
         for (int t = 0; t <= ns; ++t) if (!b.P[t]) HIP_TRY(hipMalloc(&b.P[t], pbytes));
         const long nk = (b.dims == 3) ? b.nk : 1;
@@ -779,6 +808,8 @@ int free_block_buffers(BlockH& b)
     for (auto& p : b.gS) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     if (b.axi) { HIP_TRY(hipFree(b.axi)); b.axi = nullptr; }
     if (b.src) { HIP_TRY(hipFree(b.src)); b.src = nullptr; }
+    if (b.geo) { HIP_TRY(hipFree(b.geo)); b.geo = nullptr; }
+    for (auto& p : b.fgeo) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     if (b.bc_dev) { HIP_TRY(hipFree(b.bc_dev)); b.bc_dev = nullptr; }
     return E4HIP_OK;
 }
@@ -1020,7 +1051,14 @@ int e4hip_tp_face_stats(long long* energy_evals, long long* fallbacks, int reset
     return E4HIP_OK;
 }
 
-int e4hip_block_create(int gm, int dims, int ni, int nj, int nk, double dx, double dy, double dz, int* blk)
+} // extern "C"
+
+namespace {
+
+// Block creation: uniform cells of dx*dy*dz, or (geom != null) the geometry
+// arrays of a block built from its grid (layout of e4hip_block_create_geometry).
+int create_block(int gm, int dims, int ni, int nj, int nk, double dx, double dy, double dz, const double* geom,
+                 int* blk)
 {
     graph_invalidate();
     if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
@@ -1038,12 +1076,57 @@ int e4hip_block_create(int gm, int dims, int ni, int nj, int nk, double dx, doub
     b.qs = (b.ncells + 63) / 64 * 64;
     b.nsp = g_state.gases[gm].nsp;
     b.nq = (b.nsp > 1) ? 5 + b.nsp : 5;
+    b.general = geom != nullptr;
+    if (b.general) {
+        // every cell needs a positive volume and positive lengths
+        const long nc = b.ncells;
+        for (long c = 0; c < nc; ++c)
+            for (int q = 0; q <= dims; ++q)
+                if (!(geom[q * nc + c] > 0.0))
+                    return fail(E4HIP_ERR_ARG, "cell %ld has a non-positive %s", c, q ? "length" : "volume");
+    }
     int rc = alloc_block_buffers(b);
     if (!rc) rc = upload_bc(b);
+    if (!rc && b.general) {
+        auto put = [&](double* dst, long stride, const double* src, int rows, long n) -> int {
+            for (int q = 0; q < rows; ++q)
+                HIP_TRY(hipMemcpy(dst + (size_t)q * stride, src + (size_t)q * n, sizeof(double) * n,
+                                  hipMemcpyHostToDevice));
+            return E4HIP_OK;
+        };
+        const long nc = b.ncells;
+        const long nf[3] = {(long)(ni + 1) * nj * nk, (long)ni * (nj + 1) * nk, (long)ni * nj * (nk + 1)};
+        rc = hipMalloc(&b.geo, sizeof(double) * 4 * b.qs) == hipSuccess ? E4HIP_OK
+                                                                        : fail(E4HIP_ERR_HIP, "hipMalloc of the cell geometry failed");
+        if (!rc) rc = put(b.geo, b.qs, geom, 4, nc);
+        const double* fsrc = geom + 4 * nc;
+        for (int d = 0; d < dims && !rc; ++d) {
+            if (hipMalloc(&b.fgeo[d], sizeof(double) * 10 * b.fqs[d]) != hipSuccess)
+                rc = fail(E4HIP_ERR_HIP, "hipMalloc of the face geometry failed");
+            if (!rc) rc = put(b.fgeo[d], b.fqs[d], fsrc, 10, nf[d]);
+            fsrc += 10 * nf[d];
+        }
+    }
     if (rc) { free_block_buffers(b); return rc; }
     g_state.blocks.push_back(b);
     *blk = (int)g_state.blocks.size() - 1;
     return E4HIP_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int e4hip_block_create(int gm, int dims, int ni, int nj, int nk, double dx, double dy, double dz, int* blk)
+{
+    return create_block(gm, dims, ni, nj, nk, dx, dy, dz, nullptr, blk);
+}
+
+int e4hip_block_create_geometry(int gm, int dims, int ni, int nj, int nk, const double* geom, int cartesian, int* blk)
+{
+    if (!geom) return fail(E4HIP_ERR_ARG, "null geometry");
+    if (cartesian) return create_block(gm, dims, ni, nj, nk, geom[0], geom[1], (dims == 3) ? geom[2] : 1.0, nullptr, blk);
+    return create_block(gm, dims, ni, nj, nk, 1.0, 1.0, 1.0, geom, blk);
 }
 
 int e4hip_block_destroy(int blk)
@@ -1096,6 +1179,8 @@ int e4hip_block_set_bc(int blk, int face, int type, const double* fs)
         if (!b->gS[face]) HIP_TRY(hipMalloc(&b->gS[face], 2 * nfc));
         HIP_TRY(hipMemcpy(b->gS[face], S.data(), 2 * nfc, hipMemcpyHostToDevice));
     }
+    if (b->general && (type == E4HIP_BC_SLIP_WALL_1 || (type == E4HIP_BC_EXTRAPOLATE_1 && !b->is_tp())))
+        return fail(E4HIP_ERR_ARG, "boundary kind %d is not available on a block built from its grid", type);
     if (type == E4HIP_BC_SLIP_WALL_1) {
         // one-sided reconstruction reaches three cells in from the wall
         const int n = (face == E4HIP_FACE_EAST || face == E4HIP_FACE_WEST) ? b->ni
@@ -1117,6 +1202,10 @@ int e4hip_block_set_bc(int blk, int face, int type, const double* fs)
         P.rho = fs[0]; P.u = fs[3]; P.a = fs[4]; P.v0 = fs[5]; P.v1 = fs[6]; P.v2 = fs[7];
         if (!(P.rho > 0) || !(P.u > 0) || !(P.a > 0)) return fail(E4HIP_ERR_ARG, "invalid inflow state");
         b->bc_fs[face] = P;
+        // the same state in the primitive-array order (blocks built from their grid)
+        double* t = b->bc_fs_tp[face];
+        t[e4::P_RHO] = fs[0]; t[e4::P_P] = fs[1]; t[e4::P_T] = fs[2]; t[e4::P_U] = fs[3]; t[e4::P_A] = fs[4];
+        t[e4::P_VX] = fs[5]; t[e4::P_VY] = fs[6]; t[e4::P_VZ] = fs[7];
     }
     if (type == E4HIP_BC_EXTRAPOLATE_1) {
         // the face's two ghost layers, holding the constructed cell state
@@ -1146,6 +1235,7 @@ int e4hip_block_set_axisymmetric(int blk, double x0, double y0)
     BlockH* b = get_block(blk);
     if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
     if (b->dims != 2) return fail(E4HIP_ERR_ARG, "axisymmetric geometry is for 2D blocks");
+    if (b->general) return fail(E4HIP_ERR_ARG, "axisymmetric geometry is for uniform blocks");
     if (!(y0 >= 0.0) || !std::isfinite(x0)) return fail(E4HIP_ERR_ARG, "the block must lie at y >= 0");
     // xyplane_quad_cell_properties (properties.d:177-237; volume = area *
     // centroid y, fvcell.d:314-351) and update_2D_geometric_data
@@ -1222,6 +1312,8 @@ int e4hip_block_connect(int blk, int face, int other, int other_face, int orient
     if (face < 0 || face >= nf || other_face < 0 || other_face >= nf) return fail(E4HIP_ERR_ARG, "bad face");
     if (b->dims != o->dims) return fail(E4HIP_ERR_ARG, "connected blocks must have the same dimensionality");
     if (b->gm != o->gm) return fail(E4HIP_ERR_ARG, "connected blocks must share the gas model");
+    if (b->general != o->general)
+        return fail(E4HIP_ERR_ARG, "a block built from its grid connects only to blocks built from their grids");
     if (orientation < 0 || orientation > 3 || (b->dims == 2 && orientation != 0))
         return fail(E4HIP_ERR_ARG, "bad orientation %d", orientation);
     // every ghost cell must map into the other block: check the corners of the face
@@ -1280,6 +1372,7 @@ static e4::TPArgs tp_args_for(const BlockH& b)
     e4::TPArgs A;
     memset(&A, 0, sizeof(A));
     A.qs = b.qs; A.ni = b.ni; A.nj = b.nj; A.nk = (b.dims == 3) ? b.nk : 1; A.dims = b.dims;
+    for (int d = 0; d < 3; ++d) A.fqs[d] = b.fqs[d];
     A.gas = g_state.gases[b.gm].tp_dev;
     A.counters = g_state.counters;
     return A;
@@ -1321,8 +1414,21 @@ int e4hip_block_upload(int blk, const double* U, const double* Tguess)
         return E4HIP_OK;
     }
     unsigned int nbad = 0;
-    int rc = run_decode(*b, nullptr, &nbad);
-    if (rc) return rc;
+    if (b->general) {
+        hipLaunchKernelGGL(e4::reset_counters_kernel, dim3(1), dim3(64), 0, g_state.stream, g_state.counters,
+                           e4::COUNTER_SLOT, nullptr);
+        e4::TPArgs A = tp_args_for(*b);
+        A.Pout = b->P[0];
+        A.ig = g_state.gases[b->gm].ideal;
+        if (g_state.cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT) e4::launch_gen_aux_exact(A, g_state.stream, 1, b->U[0]);
+        else e4::launch_gen_aux_fast64(A, g_state.stream, 1, b->U[0]);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(&nbad, g_state.counters, sizeof(unsigned int), hipMemcpyDeviceToHost, g_state.stream));
+        HIP_TRY(hipStreamSynchronize(g_state.stream));
+    } else {
+        int rc = run_decode(*b, nullptr, &nbad);
+        if (rc) return rc;
+    }
     if (nbad) return fail(E4HIP_ERR_DECODE, "%u cells of block %d do not decode to a valid state", nbad, blk);
     return E4HIP_OK;
 }
@@ -1336,10 +1442,10 @@ int e4hip_block_download(int blk, double* U, double* fs)
             HIP_TRY(hipMemcpyAsync(U + (size_t)q * b->ncells, b->U[0] + (size_t)q * b->qs, sizeof(double) * b->ncells,
                                    hipMemcpyDeviceToHost, g_state.stream));
     }
-    if (fs && b->is_tp()) {
+    if (fs && b->has_P()) {
         // primitive layout rho,vx,vy,vz,u,T,p,a,Y -> flow-state layout rho,p,T,u,a,vx,vy,vz,Y
         const int map[8] = {e4::P_RHO, e4::P_P, e4::P_T, e4::P_U, e4::P_A, e4::P_VX, e4::P_VY, e4::P_VZ};
-        for (int q = 0; q < 8 + b->nsp; ++q) {
+        for (int q = 0; q < 8 + (b->is_tp() ? b->nsp : 0); ++q) {
             int src = (q < 8) ? map[q] : q;
             HIP_TRY(hipMemcpyAsync(fs + (size_t)q * b->ncells, b->P[0] + (size_t)src * b->qs,
                                    sizeof(double) * b->ncells, hipMemcpyDeviceToHost, g_state.stream));
@@ -1382,7 +1488,6 @@ int e4hip_config(const E4hipStepConfig* c)
     return E4HIP_OK;
 }
 
-// The buffers a step swaps between its stages, per block.
 // The buffers a step swaps between its stages, per block (PTRS_PER_BLOCK each).
 static size_t step_pointers_to(const int* blks, int nblk, uintptr_t* v)
 {
@@ -1448,7 +1553,7 @@ static int enqueue_step(const int* blks, int nblk, double dt, int ns, unsigned i
             for (int f = 0; f < 6; ++f)
                 if (b.gx[f] && b.bc_type[f] == E4HIP_BC_EXTRAPOLATE_1) faces |= 1 << f;
             if (!faces) continue;
-            if (!b.is_tp() && batch && g_state.cfg.batch_blocks) {
+            if (!b.has_P() && batch && g_state.cfg.batch_blocks) {
                 e4::StageArgs A{};
                 dim3 grid;
                 rc = stage_args_for(b, stage, dt, nullptr, 0, A, grid);
@@ -1517,7 +1622,7 @@ static int enqueue_step(const int* blks, int nblk, double dt, int ns, unsigned i
         for (int n = 0; n < nblk; ++n) {
             BlockH& b = *get_block(blks[n]);
             unsigned int* cnt = counters + e4::COUNTER_SLOT * (n * 3 + (stage - 1));
-            if (b.is_tp()) {
+            if (b.has_P()) {
                 rc = launch_tp_stage(b, stage, dt, cnt);
                 if (rc) return rc;
             } else if (batch) {
@@ -1588,7 +1693,7 @@ static void finish_step(const int* blks, int nblk, int ns, bool stopped, const u
     for (int n = 0; n < nblk; ++n) {
         BlockH& b = *get_block(blks[n]);
         std::swap(b.U[0], b.U[ns]);
-        if (b.is_tp()) std::swap(b.P[0], b.P[ns]);
+        if (b.has_P()) std::swap(b.P[0], b.P[ns]);
     }
 }
 
@@ -1700,52 +1805,87 @@ int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_ce
     return rc;
 }
 
+} // extern "C"
+
+namespace {
+
+// The CFL-signal kernel of one block on the library stream: the block's
+// largest signal as a bit pattern into *smax_bits (atomic max; the caller
+// clears it first) and the last cell's signal into *last_signal.
+int enqueue_cfl(const BlockH& b, unsigned long long* smax_bits, double* last_signal)
+{
+    const int stringent = g_state.cfg.stringent_cfl != 0;
+    long nwg = (b.ncells + 255) / 256;
+    if (nwg > 4096) nwg = 4096;
+    hipEvent_t ev = nullptr;
+    prof_begin(&ev);
+    if (b.general) {
+        e4::TPArgs A = ::tp_args_for(b);
+        A.Pin = b.P[0];
+        A.geo = b.geo;
+        for (int d = 0; d < 3; ++d) A.fgeo[d] = b.fgeo[d];
+        e4::launch_gen_cfl(A, stringent, smax_bits, last_signal, (int)nwg, g_state.stream);
+    } else if (b.is_tp()) {
+        hipLaunchKernelGGL(e4::tp_cfl_kernel, dim3((unsigned)nwg), dim3(256), 0, g_state.stream, b.P[0], b.qs, b.ni,
+                           b.nj, b.nk, b.dims, stringent, b.dx, b.dy, b.dz, smax_bits, last_signal);
+    } else {
+        e4::CflArgs C;
+        C.U = b.U[0]; C.qs = b.qs; C.ni = b.ni; C.nj = b.nj; C.nk = b.nk; C.dims = b.dims;
+        C.stringent = stringent;
+        C.iLength = b.dx; C.jLength = b.dy; C.kLength = b.dz;
+        C.gas = g_state.gases[b.gm].ideal;
+        C.smax_bits = smax_bits; C.last_signal = last_signal;
+        hipLaunchKernelGGL(e4::cfl_kernel<e4::ModeExact>, dim3((unsigned)nwg), dim3(256), 0, g_state.stream, C);
+    }
+    prof_end(1, ev);
+    HIP_TRY(hipGetLastError());
+    return E4HIP_OK;
+}
+
+// A block's dt_allow and cfl_max from its largest signal and its last cell's
+// signal (fluidblock.d:776-862): min over cells of cfl/signal == cfl/max(signal)
+// and max of dt*signal == dt*max(signal), since rounding is monotone; the
+// rescaling when check_cfl and cfl_max exceeds cfl_allow uses the last cell's
+// signal, as fluidblock.d:856 does.
+void block_dt(unsigned long long smax_bits, double last, double dt_current, double cfl_value, int check_cfl,
+              double& dt_allow, double& cfl_max)
+{
+    double cfl_allow = 0.9;   // fluidblock.d:798-803
+    switch (nstages(g_state.cfg.gasdynamic_update_scheme)) { case 1: cfl_allow = 0.9; break; case 2: cfl_allow = 1.2; break; default: cfl_allow = 1.6; }
+    double smax;
+    memcpy(&smax, &smax_bits, sizeof(smax));
+    dt_allow = cfl_value / smax;
+    cfl_max = dt_current * smax;
+    if (check_cfl && (cfl_max < 0.0 || cfl_max > cfl_allow)) {
+        cfl_max = 0.5 * cfl_allow;
+        dt_allow = cfl_max / last;
+    }
+}
+
+} // namespace
+
+extern "C" {
+
 int e4hip_determine_dt(const int* blks, int nblk, double dt_current, double cfl_value, int check_cfl,
                        double* dt_allow_out, double* cfl_max_out)
 {
     if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
     if (!blks || nblk <= 0 || !dt_allow_out || !cfl_max_out) return fail(E4HIP_ERR_ARG, "bad arguments");
-    double cfl_allow = 0.9;   // fluidblock.d:798-803
-    switch (nstages(g_state.cfg.gasdynamic_update_scheme)) { case 1: cfl_allow = 0.9; break; case 2: cfl_allow = 1.2; break; default: cfl_allow = 1.6; }
     double dt_allow_all = 1.7976931348623157e308, cfl_max_all = 0.0;
     for (int n = 0; n < nblk; ++n) {
         BlockH* b = get_block(blks[n]);
         if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blks[n]);
         hipLaunchKernelGGL(e4::reset_counters_kernel, dim3(1), dim3(64), 0, g_state.stream, g_state.counters, 0,
                            g_state.smax);
-        e4::CflArgs C;
-        C.U = b->U[0]; C.qs = b->qs; C.ni = b->ni; C.nj = b->nj; C.nk = b->nk; C.dims = b->dims;
-        C.stringent = g_state.cfg.stringent_cfl != 0;
-        C.iLength = b->dx; C.jLength = b->dy; C.kLength = b->dz;
-        C.gas = g_state.gases[b->gm].ideal;
-        C.smax_bits = g_state.smax; C.last_signal = g_state.last_signal;
-        hipEvent_t ev = nullptr;
-        prof_begin(&ev);
-        long nwg = (b->ncells + 255) / 256;
-        if (nwg > 4096) nwg = 4096;
-        if (b->is_tp())
-            hipLaunchKernelGGL(e4::tp_cfl_kernel, dim3((unsigned)nwg), dim3(256), 0, g_state.stream, b->P[0], b->qs,
-                               b->ni, b->nj, b->nk, b->dims, C.stringent, b->dx, b->dy, b->dz, g_state.smax,
-                               g_state.last_signal);
-        else
-            hipLaunchKernelGGL(e4::cfl_kernel<e4::ModeExact>, dim3((unsigned)nwg), dim3(256), 0, g_state.stream, C);
-        prof_end(1, ev);
-        HIP_TRY(hipGetLastError());
+        int rc = enqueue_cfl(*b, g_state.smax, g_state.last_signal);
+        if (rc) return rc;
         unsigned long long bits = 0;
         double last = 0.0;
         HIP_TRY(hipMemcpyAsync(&bits, g_state.smax, sizeof(bits), hipMemcpyDeviceToHost, g_state.stream));
         HIP_TRY(hipMemcpyAsync(&last, g_state.last_signal, sizeof(last), hipMemcpyDeviceToHost, g_state.stream));
         HIP_TRY(hipStreamSynchronize(g_state.stream));
-        double smax;
-        memcpy(&smax, &bits, sizeof(smax));
-        // min over cells of cfl/signal == cfl/max(signal) and max of dt*signal ==
-        // dt*max(signal), since rounding is monotone
-        double dt_allow = cfl_value / smax;
-        double cfl_max = dt_current * smax;
-        if (check_cfl && (cfl_max < 0.0 || cfl_max > cfl_allow)) {
-            cfl_max = 0.5 * cfl_allow;
-            dt_allow = cfl_max / last;   // the last cell's signal, as fluidblock.d:856 uses it
-        }
+        double dt_allow, cfl_max;
+        block_dt(bits, last, dt_current, cfl_value, check_cfl, dt_allow, cfl_max);
         dt_allow_all = fmin(dt_allow_all, dt_allow);
         cfl_max_all = fmax(cfl_max_all, cfl_max);
     }

@@ -242,6 +242,13 @@ struct TPArgs {
     const unsigned int* abort_prev;   // bad-cell stop of the step, as in StageArgs
     unsigned int* abort_this;
     unsigned int max_invalid;
+    // a block built from its grid (null: uniform Cartesian cells): cell
+    // geometry [4][qs] (volume, i/j/k lengths), face geometry of direction d
+    // [10][fqs[d]] (area, n, t1, t2), the connected blocks' cell geometry
+    const double* geo;
+    const double* fgeo[3];
+    const double* nbrGeo[6];
+    IdealGas ig;                      // ideal-gas constants (ideal-gas blocks built from their grid)
 };
 
 // Launchers, one per arithmetic mode; each lives in its own translation unit
@@ -277,6 +284,16 @@ void launch_tp_decode_exact(const TPArgs& A, const double* U, const double* Tgue
 void launch_tp_extrap1_exact(const TPArgs& A, int faces, hipStream_t stream);
 void launch_tp_extrap1_fast64(const TPArgs& A, int faces, hipStream_t stream);
 void launch_tp_decode_fast64(const TPArgs& A, const double* U, const double* Tguess, hipStream_t stream);
+// blocks built from their grid (nsp 0: ideal gas, 5: the mixture): one stage;
+// what = 0 the shock detector, 1 the decode of an uploaded ideal-gas U into the
+// flow-state array (U is rewritten where the low-temperature fallback re-encodes it)
+void launch_gen_stage_exact(const TPArgs& A, hipStream_t stream, int flux, int nsp);
+void launch_gen_stage_fast64(const TPArgs& A, hipStream_t stream, int flux, int nsp);
+void launch_gen_stage_mixed(const TPArgs& A, hipStream_t stream, int flux, int nsp);
+void launch_gen_aux_exact(const TPArgs& A, hipStream_t stream, int what, double* U);
+void launch_gen_aux_fast64(const TPArgs& A, hipStream_t stream, int what, double* U);
+void launch_gen_cfl(const TPArgs& A, int stringent, unsigned long long* smax_bits, double* last_signal, int nwg,
+                    hipStream_t stream);
 
 void launch_mlp_guess(const GuessModel* G, const double* x, int n, double* T, hipStream_t stream);
 // the cells next to one-sided boundaries, after the stage kernel (A.fix_off[6] > 0)

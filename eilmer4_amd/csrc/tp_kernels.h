@@ -461,6 +461,17 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
             add_face(A.F[0], A.fqs[0], iW, -1.0, g.ai[iW]);
             vol_inv = 1.0 / g.vol[c];
             qy = 0.0 + A.Pin[c + P_P * qs] * g.areaxy[c] / g.vol[c];
+        } else if (A.geo) {
+            // a block built from its grid: each face's own area, the cell's volume
+            add_face(A.F[1], A.fqs[1], jS + A.ni, 1.0, A.fgeo[1][jS + A.ni]);
+            add_face(A.F[0], A.fqs[0], iW + 1, 1.0, A.fgeo[0][iW + 1]);
+            add_face(A.F[1], A.fqs[1], jS, -1.0, A.fgeo[1][jS]);
+            add_face(A.F[0], A.fqs[0], iW, -1.0, A.fgeo[0][iW]);
+            if (A.dims == 3) {
+                add_face(A.F[2], A.fqs[2], kB + (long)A.ni * A.nj, 1.0, A.fgeo[2][kB + (long)A.ni * A.nj]);
+                add_face(A.F[2], A.fqs[2], kB, -1.0, A.fgeo[2][kB]);
+            }
+            vol_inv = 1.0 / A.geo[c];
         } else {
             add_face(A.F[1], A.fqs[1], jS + A.ni, 1.0, A.area[1]);
             add_face(A.F[0], A.fqs[0], iW + 1, 1.0, A.area[0]);

@@ -1,5 +1,6 @@
 """Structured fluid block on the device (the SFluidBlock role,
-eilmer/sfluidblock.d:48-126), restricted to uniform Cartesian cells."""
+eilmer/sfluidblock.d:48-126): uniform Cartesian cells, or any structured grid
+given by its vertices (SFluidBlock.from_grid)."""
 import ctypes as C
 
 import numpy as np
@@ -16,21 +17,36 @@ class SFluidBlock:
     """A block of ni x nj (x nk) cells of size dx x dy (x dz) whose conserved
     state lives in device memory; every face starts as a slip wall."""
 
-    def __init__(self, gas, dims, ni, nj, nk=1, dx=1.0, dy=1.0, dz=1.0, label=""):
+    def __init__(self, gas, dims, ni, nj, nk=1, dx=1.0, dy=1.0, dz=1.0, label="", geom=None):
         self.gas = gas
         self.dims = dims
         self.ni, self.nj, self.nk = ni, nj, (nk if dims == 3 else 1)
         self.dx, self.dy, self.dz = dx, dy, (dz if dims == 3 else 1.0)
         self.label = label
+        self.geom = geom
         h = C.c_int(-1)
-        _lib.check(_lib.load().e4hip_block_create(gas.handle(), dims, ni, nj, self.nk, dx, dy, self.dz,
-                                                  C.byref(h)))
+        if geom is None:
+            _lib.check(_lib.load().e4hip_block_create(gas.handle(), dims, ni, nj, self.nk, dx, dy, self.dz,
+                                                      C.byref(h)))
+        else:
+            g = np.ascontiguousarray(geom, dtype=np.float64)
+            _lib.check(_lib.load().e4hip_block_create_geometry(gas.handle(), dims, ni, nj, self.nk, _p(g), 0,
+                                                               C.byref(h)))
         self.id = h.value
         n = C.c_long(0)
         nq = C.c_int(0)
         _lib.check(_lib.load().e4hip_block_ncells(self.id, C.byref(n), C.byref(nq)))
         self.ncells, self.nq = n.value, nq.value
         self.bc = {f: BCType.slip_wall for f in Face}
+
+    @classmethod
+    def from_grid(cls, gas, grid, label=""):
+        """A block of the structured grid with vertices grid [nk+1][nj+1][ni+1][3]
+        (3D) or [nj+1][ni+1][2|3] (2D): cell volumes and lengths, face areas and
+        frames from the vertices (eilmer4_amd.geometry)."""
+        from .geometry import block_geometry
+        dims, ni, nj, nk, geom = block_geometry(grid)
+        return cls(gas, dims, ni, nj, nk, label=label, geom=geom)
 
     @property
     def shape(self):

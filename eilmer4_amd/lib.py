@@ -17,7 +17,8 @@ ERRORS = {-1: "E4HIP_ERR_ARG", -2: "E4HIP_ERR_HIP", -3: "E4HIP_ERR_HANDLE", -4: 
 # every symbol include/e4hip.h declares
 EXPORTS = [
     "e4hip_init", "e4hip_finalize", "e4hip_last_error", "e4hip_stream", "e4hip_synchronize",
-    "e4hip_gas_ideal", "e4hip_gas_tp", "e4hip_block_create", "e4hip_block_destroy", "e4hip_block_set_bc",
+    "e4hip_gas_ideal", "e4hip_gas_tp", "e4hip_block_create", "e4hip_block_create_geometry", "e4hip_block_destroy",
+    "e4hip_block_set_bc",
     "e4hip_block_connect", "e4hip_block_set_axisymmetric", "e4hip_block_set_source_terms",
     "e4hip_block_upload", "e4hip_block_download", "e4hip_block_ncells", "e4hip_config",
     "e4hip_explicit_increment", "e4hip_determine_dt", "e4hip_profile_enable", "e4hip_profile_read",
@@ -63,6 +64,7 @@ def load(path=LIB_PATH):
     L.e4hip_gas_tp.argtypes = [C.c_int, _dp, C.c_int, _dp, _dp, _dp, _ip]
     L.e4hip_block_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                      C.c_double, _ip]
+    L.e4hip_block_create_geometry.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_int, _ip]
     L.e4hip_block_destroy.argtypes = [C.c_int]
     L.e4hip_block_set_bc.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
     L.e4hip_block_connect.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]

@@ -192,6 +192,26 @@ int e4hip_mlp_train(float* params, const float* X, const float* T, int n, int ep
  * needs at least 2 cells. */
 int e4hip_block_create(int gm, int dims, int ni, int nj, int nk,
                        double dx, double dy, double dz, int* blk);
+/* Blocks of any structured grid (compute_primary_cell_geometric_data,
+ * eilmer/sfluidblock.d:650-842, as the D side has already computed it for its
+ * SFluidBlock).  cartesian != 0: geom = {dx, dy, dz}, the same as
+ * e4hip_block_create.  cartesian == 0: geom holds the cell geometry
+ * [4][ncells] (volume -- the (x,y)-plane area in 2D --, iLength, jLength,
+ * kLength; fvcell.d:58-154) and then, for each direction d < dims, the face
+ * geometry [10][nfaces(d)] (area, n.x, n.y, n.z, t1.x, t1.y, t1.z, t2.x, t2.y,
+ * t2.z; fvinterface.d:34-90), cells i fastest and faces of direction d i
+ * fastest over (ni + [d==0]) x (nj + [d==1]) x (nk + [d==2]); d = 0 are the
+ * i-faces (west faces of the cells, normal towards +i), d = 1 the j-faces and
+ * d = 2 the k-faces.  The ghost cells' lengths are the reference's: mirrored
+ * from the interior (sfluidblock.d:673-713), or across a full-face connection
+ * the mapped cell's own (full_face_copy.d:1042-1085).  Such a block's stages
+ * reconstruct with the four cells' lengths (onedinterp.d:118-162) in each
+ * face's own frame (vector3.d:425-450, fluxcalc.d:26-128).  It takes every
+ * boundary kind except E4HIP_BC_SLIP_WALL_1 (and, for an ideal gas,
+ * E4HIP_BC_EXTRAPOLATE_1), and connects only to blocks also built this way.
+ * Volumes and lengths must be positive. */
+int e4hip_block_create_geometry(int gm, int dims, int ni, int nj, int nk, const double* geom, int cartesian,
+                                int* blk);
 int e4hip_block_destroy(int blk);
 /* Boundary condition of one face (E4HIP_FACE_*, E4HIP_BC_*).  For
  * E4HIP_BC_INFLOW, fs gives the ghost flow state [8 + nsp] as above; for
