@@ -137,7 +137,7 @@ struct State {
     int batch_min_blocks = 2;                  // batch the ideal blocks of a stage from this many on
     std::vector<BlockH> blocks;
     E4hipStepConfig cfg{E4HIP_FLUX_AUSMDV, E4HIP_UPDATE_PC, 2, 1, 1, 1, E4HIP_THERMO_RHOU, E4HIP_PRECISION_FP64, 0, 0,
-                        0, 0.20, -0.30};
+                        0, 0.20, -0.30, 0, 0, 0.01, 1};
     bool profiling = false;
     std::vector<ProfEvent> prof_pending;
     double prof_ms[2] = {0, 0};
@@ -152,6 +152,9 @@ struct State {
     std::vector<std::vector<uintptr_t>> recent_keys;
     unsigned long long step_counter = 0;
     unsigned long long generation = 0;         // bumped by every call that changes blocks, boundaries or config
+    // k-chunk autotuner: chunk height of the stage kernel per block shape and
+    // arithmetic (ni, nj, nk, precision, kernel flux, default reconstruction)
+    std::vector<std::pair<std::vector<int>, int>> kchunk_tuned;
     std::string err;
 };
 
@@ -236,10 +239,22 @@ bool l2r2_uniform(double len, e4::L2R2U& u)
            w.lenL0 == u.len && w.two_lenR0_plus_lenR1 == u.len3;
 }
 
+int kernel_flux(int fc);
+
+std::vector<int> tune_key(const BlockH& b)
+{
+    const E4hipStepConfig& c = g_state.cfg;
+    const int def = c.interpolation_order > 1 && c.apply_limiter && c.extrema_clipping;
+    return {b.ni, b.nj, b.nk, c.precision_mode, kernel_flux(c.flux_calculator), def};
+}
+
 int kchunk_for(const BlockH& b)
 {
     const char* env = getenv("E4HIP_KCHUNK");
     if (env && atoi(env) > 0) return atoi(env);
+    const std::vector<int> key = tune_key(b);
+    for (const auto& t : g_state.kchunk_tuned)
+        if (t.first == key) return t.second;
     long nxy = (long)((b.ni + TILE_X - 1) / TILE_X) * ((b.nj + TILE_Y - 1) / TILE_Y);
     // aim for >= 2048 workgroups (8 per CU) while marching at least 8 planes
     long chunks = (2048 + nxy - 1) / nxy;
@@ -300,8 +315,18 @@ e4::Connection connection_of(const BlockH& b, int f)
 int kernel_flux(int fc)
 {
     if (fc == E4HIP_FLUX_HLLC) return e4::FLUX_HLLC;
-    if (fc == E4HIP_FLUX_HANEL || fc == E4HIP_FLUX_ADAPTIVE_HANEL_AUSMDV) return e4::FLUX_SHOCK;
+    if (fc == E4HIP_FLUX_HANEL || fc == E4HIP_FLUX_ADAPTIVE_HANEL_AUSMDV || fc == E4HIP_FLUX_EFM ||
+        fc == E4HIP_FLUX_ADAPTIVE_EFM_AUSMDV)
+        return e4::FLUX_SHOCK;
+    if (fc == E4HIP_FLUX_AUSM_PLUS_UP || fc == E4HIP_FLUX_ROE) return e4::FLUX_OTHER;
     return e4::FLUX_AUSMDV;
+}
+
+// Whether the flux calculator switches the shock detector on
+// (globalconfig.d:1942-1954; adaptive_hlle_roe is not available here).
+bool flux_detects_shocks(int fc)
+{
+    return fc == E4HIP_FLUX_ADAPTIVE_HANEL_AUSMDV || fc == E4HIP_FLUX_ADAPTIVE_EFM_AUSMDV;
 }
 
 // Shock-indicator arguments of a block's stage (StageArgs or TPArgs).
@@ -320,7 +345,11 @@ int shock_fields(const BlockH& b, Args& A)
     }
     A.shear_tol = cfg.shear_tolerance;
     A.compression_tol = cfg.compression_tolerance;
-    A.always_hanel = cfg.flux_calculator == E4HIP_FLUX_HANEL;
+    const int fc = cfg.flux_calculator;
+    A.always_hanel = fc == E4HIP_FLUX_HANEL || fc == E4HIP_FLUX_EFM;
+    A.shock_efm = fc == E4HIP_FLUX_EFM || fc == E4HIP_FLUX_ADAPTIVE_EFM_AUSMDV;
+    A.other_roe = fc == E4HIP_FLUX_ROE;
+    A.M_inf = cfg.M_inf;
     return E4HIP_OK;
 }
 
@@ -496,6 +525,120 @@ int launch_stage(const BlockH& b, int stage, double dt, unsigned int* counters)
     return E4HIP_OK;
 }
 
+// The k-chunk autotuner.  For a uniform 3D ideal-gas block whose shape has
+// not been tuned under the current arithmetic, the stage kernel of stage 1 is
+// timed (HIP events, three launches after a warm-up) at candidate chunk
+// heights -- the heights that split nk into 1, 2, 3, 4, 6, 8, 12, 16 or 32
+// chunks of at least four planes -- and the fastest is kept for that shape.
+// Stage 1 only reads U[0] and writes U[1], d[0] and counters that the step
+// resets, and every chunk height gives the same bits, so tuning before a step
+// changes its timing only.
+std::vector<int> batch_tune_key(const std::vector<const BlockH*>& blks);
+int launch_stage_batch(const std::vector<const BlockH*>& blks, const std::vector<unsigned int*>& counters,
+                       int stage, double dt, int set, long force_chunks);
+
+int autotune_blocks(const int* blks, int nblk, double dt)
+{
+    if (!g_state.cfg.autotune) return E4HIP_OK;
+    const char* env = getenv("E4HIP_KCHUNK");
+    if (env && atoi(env) > 0) return E4HIP_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(g_state.stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) return E4HIP_OK;
+    const int flux = kernel_flux(g_state.cfg.flux_calculator);
+    for (int n = 0; n < nblk; ++n) {
+        const BlockH& b = *get_block(blks[n]);
+        if (b.dims != 3 || b.has_P() || b.general) continue;
+        const std::vector<int> key = tune_key(b);
+        bool known = false;
+        for (const auto& t : g_state.kchunk_tuned) known = known || t.first == key;
+        if (known) continue;
+        std::vector<int> cand;
+        for (int chunks : {1, 2, 3, 4, 6, 8, 12, 16, 32}) {
+            const int kc = (b.nk + chunks - 1) / chunks;
+            if (kc >= 4 && std::find(cand.begin(), cand.end(), kc) == cand.end()) cand.push_back(kc);
+        }
+        if (cand.empty()) cand.push_back(b.nk);
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        int best = cand[0];
+        float best_ms = 1e30f;
+        for (int kc : cand) {
+            e4::StageArgs A;
+            dim3 grid;
+            int rc = stage_args_for(b, 1, dt, g_state.counters, kc, A, grid);
+            if (rc) return rc;
+            auto launch = [&]() {
+                switch (g_state.cfg.precision_mode) {
+                case E4HIP_PRECISION_FP64_EXACT: e4::launch_stage_exact(A, grid, g_state.stream, 3, flux); break;
+                case E4HIP_PRECISION_MIXED: e4::launch_stage_mixed(A, grid, g_state.stream, 3, flux); break;
+                default: e4::launch_stage_fast64(A, grid, g_state.stream, 3, flux); break;
+                }
+            };
+            launch();
+            HIP_TRY(hipEventRecord(e0, g_state.stream));
+            for (int r = 0; r < 3; ++r) launch();
+            HIP_TRY(hipEventRecord(e1, g_state.stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            HIP_TRY(hipGetLastError());
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) { best_ms = ms; best = kc; }
+        }
+        HIP_TRY(hipEventDestroy(e0));
+        HIP_TRY(hipEventDestroy(e1));
+        g_state.kchunk_tuned.push_back({key, best});
+        graph_invalidate();
+    }
+    // the ideal 3D blocks of a batched launch: the number of chunks per block
+    // over the whole launch (one table upload per candidate, into the eager
+    // staging set's stage-1 slot)
+    if (g_state.cfg.batch_blocks) {
+        std::vector<const BlockH*> bl;
+        std::vector<unsigned int*> cnt;
+        for (int n = 0; n < nblk; ++n) {
+            const BlockH& b = *get_block(blks[n]);
+            if (b.dims == 3 && !b.has_P()) {
+                bl.push_back(&b);
+                cnt.push_back(g_state.counters);
+            }
+        }
+        if ((int)bl.size() >= g_state.batch_min_blocks) {
+            const std::vector<int> key = batch_tune_key(bl);
+            bool known = false;
+            for (const auto& t : g_state.kchunk_tuned) known = known || t.first == key;
+            if (!known) {
+                hipEvent_t e0, e1;
+                HIP_TRY(hipEventCreate(&e0));
+                HIP_TRY(hipEventCreate(&e1));
+                long best = 1;
+                float best_ms = 1e30f;
+                for (long chunks : {1L, 2L, 3L, 4L, 6L, 8L, 12L, 16L, 32L}) {
+                    int rc = launch_stage_batch(bl, cnt, 1, dt, 0, chunks);
+                    if (rc) return rc;
+                    HIP_TRY(hipEventRecord(e0, g_state.stream));
+                    for (int r = 0; r < 3; ++r) {
+                        rc = launch_stage_batch(bl, cnt, 1, dt, 0, chunks);
+                        if (rc) return rc;
+                    }
+                    HIP_TRY(hipEventRecord(e1, g_state.stream));
+                    HIP_TRY(hipEventSynchronize(e1));
+                    HIP_TRY(hipGetLastError());
+                    float ms = 0.0f;
+                    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                    if (ms < best_ms) { best_ms = ms; best = chunks; }
+                }
+                HIP_TRY(hipEventDestroy(e0));
+                HIP_TRY(hipEventDestroy(e1));
+                g_state.kchunk_tuned.push_back({key, (int)best});
+                graph_invalidate();
+            }
+        }
+    }
+    return E4HIP_OK;
+}
+
 // One launch for one stage of several ideal-gas blocks of the same
 // dimensionality: their StageArgs go to a device table (one slot per stage of
 // the step, staged through pinned host memory; the host rewrites a slot only
@@ -548,15 +691,37 @@ static int stage_batch_buffers(BatchBuffers& bb, int slot, const void* a, size_t
     return E4HIP_OK;
 }
 
-int launch_stage_batch(const std::vector<const BlockH*>& blks, const std::vector<unsigned int*>& counters,
-                       int stage, double dt, int set)
+// The key of a batched launch for the autotuner: its blocks' shapes and the
+// arithmetic.
+std::vector<int> batch_tune_key(const std::vector<const BlockH*>& blks)
 {
-    const E4hipStepConfig& cfg = g_state.cfg;
-    const int n = (int)blks.size();
+    std::vector<int> key = {-1};
+    for (const BlockH* b : blks) {
+        const std::vector<int> k = tune_key(*b);
+        key.insert(key.end(), k.begin(), k.end());
+    }
+    return key;
+}
+
+// The number of k-chunks per block of a batched launch: the tuned one, else
+// enough for >= 2048 workgroups in the launch.
+long batch_chunks(const std::vector<const BlockH*>& blks)
+{
+    const std::vector<int> key = batch_tune_key(blks);
+    for (const auto& t : g_state.kchunk_tuned)
+        if (t.first == key) return t.second;
     long txy = 0;
     for (const BlockH* b : blks)
         txy += (long)((b->ni + TILE_X - 1) / TILE_X) * ((b->nj + TILE_Y - 1) / TILE_Y);
-    const long chunks = (2048 + txy - 1) / txy;
+    return (2048 + txy - 1) / txy;
+}
+
+int launch_stage_batch(const std::vector<const BlockH*>& blks, const std::vector<unsigned int*>& counters,
+                       int stage, double dt, int set, long force_chunks)
+{
+    const E4hipStepConfig& cfg = g_state.cfg;
+    const int n = (int)blks.size();
+    const long chunks = force_chunks > 0 ? force_chunks : batch_chunks(blks);
     std::vector<e4::StageArgs> args(n);
     std::vector<int> map;
     for (int e = 0; e < n; ++e) {
@@ -564,7 +729,7 @@ int launch_stage_batch(const std::vector<const BlockH*>& blks, const std::vector
         int kc = 1;
         if (b.dims == 3) {
             kc = (int)((b.nk + chunks - 1) / chunks);
-            if (kc < 8) kc = 8;
+            if (kc < 4) kc = 4;
             if (kc > b.nk) kc = b.nk;
         }
         dim3 grid;
@@ -915,6 +1080,7 @@ int e4hip_gas_ideal(double mMass, double gamma, int* gm)
     g.ideal.Cv = g.ideal.Rgas / (gamma - 1.0);
     g.ideal.Cvinv = 1.0 / g.ideal.Cv;
     g.ideal.gammaR = gamma * g.ideal.Rgas;
+    g.ideal.CpCv = (g.ideal.Rgas * gamma / (gamma - 1.0)) / g.ideal.Cv;   // gas_model.d:162, ideal_gas.d:68
     g_state.gases.push_back(g);
     *gm = (int)g_state.gases.size() - 1;
     return E4HIP_OK;
@@ -1417,11 +1583,13 @@ int e4hip_block_upload(int blk, const double* U, const double* Tguess)
     if (b->general) {
         hipLaunchKernelGGL(e4::reset_counters_kernel, dim3(1), dim3(64), 0, g_state.stream, g_state.counters,
                            e4::COUNTER_SLOT, nullptr);
+        // the decode of an uploaded state is the reference's arithmetic in
+        // every precision mode (as decode_kernel's for uniform blocks): the
+        // stored flow state must not depend on the mode configured at upload
         e4::TPArgs A = tp_args_for(*b);
         A.Pout = b->P[0];
         A.ig = g_state.gases[b->gm].ideal;
-        if (g_state.cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT) e4::launch_gen_aux_exact(A, g_state.stream, 1, b->U[0]);
-        else e4::launch_gen_aux_fast64(A, g_state.stream, 1, b->U[0]);
+        e4::launch_gen_aux_exact(A, g_state.stream, 1, b->U[0]);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(&nbad, g_state.counters, sizeof(unsigned int), hipMemcpyDeviceToHost, g_state.stream));
         HIP_TRY(hipStreamSynchronize(g_state.stream));
@@ -1468,11 +1636,15 @@ int e4hip_config(const E4hipStepConfig* c)
 {
     graph_invalidate();
     if (!c) return fail(E4HIP_ERR_ARG, "null config");
-    if (c->flux_calculator != E4HIP_FLUX_AUSMDV && c->flux_calculator != E4HIP_FLUX_HLLC &&
-        c->flux_calculator != E4HIP_FLUX_HANEL && c->flux_calculator != E4HIP_FLUX_ADAPTIVE_HANEL_AUSMDV)
-        return fail(E4HIP_ERR_ARG, "flux_calculator %d is not available on the device path", c->flux_calculator);
+    // hlle is MHD only (fluxcalc.d:1197) and adaptive_hlle_roe takes it near shocks
+    const int fc = c->flux_calculator;
+    if (fc != E4HIP_FLUX_AUSMDV && fc != E4HIP_FLUX_HLLC && fc != E4HIP_FLUX_HANEL && fc != E4HIP_FLUX_EFM &&
+        fc != E4HIP_FLUX_AUSM_PLUS_UP && fc != E4HIP_FLUX_ADAPTIVE_EFM_AUSMDV &&
+        fc != E4HIP_FLUX_ADAPTIVE_HANEL_AUSMDV && fc != E4HIP_FLUX_ROE)
+        return fail(E4HIP_ERR_ARG, "flux_calculator %d is not available on the device path", fc);
     if (!std::isfinite(c->shear_tolerance) || !std::isfinite(c->compression_tolerance))
         return fail(E4HIP_ERR_ARG, "shock tolerances must be finite");
+    if (!std::isfinite(c->M_inf) || !(c->M_inf >= 0.0)) return fail(E4HIP_ERR_ARG, "M_inf must be finite and >= 0");
     if (c->gasdynamic_update_scheme < E4HIP_UPDATE_EULER || c->gasdynamic_update_scheme > E4HIP_UPDATE_DENMAN_RK3)
         return fail(E4HIP_ERR_ARG, "gasdynamic_update_scheme %d is not available", c->gasdynamic_update_scheme);
     if (c->interpolation_order != 1 && c->interpolation_order != 2)
@@ -1605,7 +1777,7 @@ static int enqueue_step(const int* blks, int nblk, double dt, int ns, unsigned i
         }
         // adaptive_hanel_ausmdv turns the shock detector on (globalconfig.d:1942-1954);
         // it runs after the ghost-cell fill and before the fluxes (simcore.d:2352-2358)
-        if (g_state.cfg.flux_calculator == E4HIP_FLUX_ADAPTIVE_HANEL_AUSMDV) {
+        if (flux_detects_shocks(g_state.cfg.flux_calculator)) {
             for (int n = 0; n < nblk; ++n) {
                 BlockH& b = *get_block(blks[n]);
                 std::swap(b.S[0], b.S[1]);
@@ -1635,7 +1807,7 @@ static int enqueue_step(const int* blks, int nblk, double dt, int ns, unsigned i
         }
         for (int d = 0; d < 2; ++d) {
             if (g_state.cfg.batch_blocks && (int)bl[d].size() >= g_state.batch_min_blocks) {
-                rc = launch_stage_batch(bl[d], bcount[d], stage, dt, set);
+                rc = launch_stage_batch(bl[d], bcount[d], stage, dt, set, 0);
                 if (rc) return rc;
             } else {
                 for (size_t e = 0; e < bl[d].size(); ++e) {
@@ -1748,6 +1920,8 @@ int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_ce
     // and launched; later steps from the same state replay it.  Consecutive
     // steps alternate between two states (the swap of U[0] and U[stages]), so
     // two graphs are kept, each with its own batch staging set.
+    rc = autotune_blocks(blks, nblk, dt);
+    if (rc) return rc;
     const bool graphs = g_state.cfg.use_graphs != 0 && !g_state.profiling;
     std::vector<uintptr_t> key;
     if (graphs) {
@@ -1898,6 +2072,15 @@ int e4hip_profile_enable(int on)
 {
     graph_invalidate();
     g_state.profiling = on != 0;
+    return E4HIP_OK;
+}
+
+int e4hip_block_kchunk(int blk, int* kchunk)
+{
+    const BlockH* b = get_block(blk);
+    if (!b) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blk);
+    if (!kchunk) return fail(E4HIP_ERR_ARG, "null kchunk");
+    *kchunk = (b->dims == 3) ? kchunk_for(*b) : 1;
     return E4HIP_OK;
 }
 

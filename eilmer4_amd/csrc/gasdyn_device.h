@@ -24,7 +24,12 @@ namespace e4 {
 // appended).  FLUX_SHOCK is the kernels' instantiation for the calculators
 // that read the shock indicator: adaptive_hanel_ausmdv, and hanel as its
 // always-Hanel case.
-enum { FLUX_AUSMDV = 0, FLUX_HANEL = 1, FLUX_ADAPTIVE_HANEL_AUSMDV = 5, FLUX_HLLC = 9, FLUX_SHOCK = 5 };
+// FLUX_OTHER is the instantiation of the calculators without a shock
+// indicator other than ausmdv and hllc: ausm_plus_up and roe, chosen at run
+// time.  FLUX_SHOCK also serves efm and adaptive_efm_ausmdv (EFM as the
+// near-shock calculator).
+enum { FLUX_AUSMDV = 0, FLUX_HANEL = 1, FLUX_EFM = 2, FLUX_AUSM_PLUS_UP = 3, FLUX_ADAPTIVE_EFM_AUSMDV = 4,
+       FLUX_ADAPTIVE_HANEL_AUSMDV = 5, FLUX_ROE = 8, FLUX_HLLC = 9, FLUX_SHOCK = 5, FLUX_OTHER = 3 };
 
 struct ModeExact { using R = double; static constexpr bool kFast = false; };
 struct ModeFast64 { using R = double; static constexpr bool kFast = true; };
@@ -85,9 +90,11 @@ using L2R2W = L2R2WT<double>;
 
 // Ideal-gas constants (gas/ideal_gas.d:65-68); gammaR = gamma*Rgas as the
 // sound-speed product gamma*Rgas*T is associated (ideal_gas.d:143).
+// CpCv is gamma(Q) = Cp/Cv of the gas model (gas_model.d:162), with Cp as
+// ideal_gas.d:68 forms it.
 template <class R>
 struct IdealGasT {
-    R Rgas, gamma, Cv, Cvinv, gammaR;
+    R Rgas, gamma, Cv, Cvinv, gammaR, CpCv;
 };
 using IdealGas = IdealGasT<double>;
 
@@ -117,7 +124,7 @@ __device__ __forceinline__ L2R2UT<R> cast_wu(const L2R2U& w)
 template <class R>
 __device__ __forceinline__ IdealGasT<R> cast_gas(const IdealGas& g)
 {
-    return IdealGasT<R>{R(g.Rgas), R(g.gamma), R(g.Cv), R(g.Cvinv), R(g.gammaR)};
+    return IdealGasT<R>{R(g.Rgas), R(g.gamma), R(g.Cv), R(g.Cvinv), R(g.gammaR), R(g.CpCv)};
 }
 
 // limiters.d:30-38
@@ -192,10 +199,11 @@ __device__ __forceinline__ Prim decode_ideal(double U0, double U1, double U2, do
 }
 
 // A face-side state after reconstruction and the face EOS; RT = Rgas T, the
-// ratio p/rho of the ideal gas, is what the fast modes use for p/rho.
+// ratio p/rho of the ideal gas, is what the fast modes use for p/rho; T is
+// the face temperature u/Cv (used by efm).
 template <class R>
 struct FaceStateT {
-    R rho, p, u, a, vn, vt1, vt2, RT;
+    R rho, p, u, a, vn, vt1, vt2, RT, T;
 };
 
 // Flux in the face frame: mass, normal momentum, two tangential momenta, energy.
@@ -407,24 +415,255 @@ __device__ __forceinline__ bool hanel_at_face(const FaceStateT<typename M::R>& L
     return shock && shear_is_small;
 }
 
-// Options of the shock-indicator calculators: the shear tolerance, and whether
-// every face takes Hanel (the hanel calculator itself).
+template <class R> __device__ __forceinline__ R sq_rt(R x) { return sqrt(x); }
+template <> __device__ __forceinline__ float sq_rt<float>(float x) { return sqrtf(x); }
+template <class R> __device__ __forceinline__ R ex_p(R x) { return exp(x); }
+template <> __device__ __forceinline__ float ex_p<float>(float x) { return expf(x); }
+
+// fluxcalc.d:792-820: exp(-x^2) and erf(x) by the reference's polynomial
+// (Abramowitz and Stegun 7.1.26), clamped beyond |x| = 5.
+template <class M>
+__device__ __forceinline__ void exxef(typename M::R sn, typename M::R& exx, typename M::R& ef)
+{
+    using R = typename M::R;
+    const R P = R(0.327591100), A1 = R(0.254829592), A2 = R(-0.284496736), A3 = R(1.421413741);
+    const R A4 = R(-1.453152027), A5 = R(1.061405429);
+    // a zero argument counts as +0: the polynomial's erf(0) is 1e-9, so
+    // copysign makes the sign of a zero matter; the reference's velocities
+    // reach the flux through frame rotations that give +0 where the device's
+    // mirrored ghost states carry -0
+    sn = sn + R(0.0);
+    R ef1;
+    if (ab(sn) > R(5.0)) {
+        exx = R(0.138879e-10);
+        ef1 = R(1.0);
+    } else {
+        const R snsq = sn * sn;
+        exx = ex_p<R>(-snsq);
+        const R y = dv<M>(R(1.0), R(1.0) + P * ab(sn));
+        ef1 = R(1.0) - y * (A1 + y * (A2 + y * (A3 + y * (A4 + A5 * y)))) * exx;
+    }
+    ef = copysign(ef1, sn);
+}
+
+// fluxcalc.d:626-790 (efmflx): Macrossan and Pullin's equilibrium flux method
+// with the gas "constants" of the two sides weighted by sqrt(rho); Cv of an
+// ideal gas is the constant Cv, R of each side p/(rho T), and the right side's
+// RT is p/rho where the left one's is R T, as the reference forms them.
+template <class M>
+__device__ __forceinline__ Flux5T<typename M::R> efm(const FaceStateT<typename M::R>& L,
+                                                     const FaceStateT<typename M::R>& Rs,
+                                                     const IdealGasT<typename M::R>& g)
+{
+    using R = typename M::R;
+    const R dtwspi = R(0.282094792);
+    const R rhoL = L.rho, presL = L.p, eL = L.u, tL = L.T;
+    const R hL = eL + dv<M>(presL, rhoL);
+    const R vnL = L.vn, vpL = L.vt1, vqL = L.vt2;
+    const R rhoR = Rs.rho, presR = Rs.p, eR = Rs.u, tR = Rs.T;
+    const R hR = eR + dv<M>(presR, rhoR);
+    const R vnR = Rs.vn, vpR = Rs.vt1, vqR = Rs.vt2;
+    const R cvL = g.Cv, cvR = g.Cv;
+    const R RgasL = dv<M>(presL, rhoL * tL);
+    const R RgasR = dv<M>(presR, rhoR * tR);
+    const R rLsqrt = sq_rt<R>(rhoL), rRsqrt = sq_rt<R>(rhoR);
+    const R alpha = dv<M>(rLsqrt, rLsqrt + rRsqrt);
+    const R cv = alpha * cvL + (R(1.0) - alpha) * cvR;
+    const R Rgas = alpha * RgasL + (R(1.0) - alpha) * RgasR;
+    const R cp = cv + Rgas;
+    const R gam = dv<M>(cp, cv);
+    const R con = dv<M>(R(0.5) * (gam + R(1.0)), gam - R(1.0));
+    const R rtL = Rgas * tL;
+    const R cmpL = sq_rt<R>(R(2.0) * rtL);
+    const R hvsqL = R(0.5) * (vnL * vnL + vpL * vpL + vqL * vqL);
+    const R snL = dv<M>(vnL, R(1.0) * cmpL);
+    R exL, efL;
+    exxef<M>(snL, exL, efL);
+    const R wL = R(0.5) * (R(1.0) + efL);
+    const R dL = exL * dtwspi;
+    const R rtR = dv<M>(presR, rhoR);
+    const R cmpR = sq_rt<R>(R(2.0) * rtR);
+    const R hvsqR = R(0.5) * (vnR * vnR + vpR * vpR + vqR * vqR);
+    const R snR = dv<M>(vnR, R(1.0) * cmpR);
+    R exR, efR;
+    exxef<M>(snR, exR, efR);
+    const R wR = R(0.5) * (R(1.0) - efR);
+    const R dR = -exR * dtwspi;
+    const R fmsL = (wL * rhoL * vnL) + (dL * cmpL * rhoL);
+    const R fmsR = (wR * rhoR * vnR) + (dR * cmpR * rhoR);
+    Flux5T<R> F;
+    F.mass = fmsL + fmsR;
+    F.mn = fmsL * vnL + fmsR * vnR + wL * presL + wR * presR;
+    F.mt1 = fmsL * vpL + fmsR * vpR;
+    F.mt2 = fmsL * vqL + fmsR * vqR;
+    F.E = (wL * rhoL * vnL) * (hvsqL + hL) + (wR * rhoR * vnR) * (hvsqR + hR) +
+          (dL * cmpL * rhoL) * (hvsqL + con * rtL) + (dR * cmpR * rhoR) * (hvsqR + con * rtR);
+    return F;
+}
+
+// fluxcalc.d:860-1037: Liou's AUSM+-up (2006) with the mean sound speed at
+// the face and the reference's M_inf floor on M0^2.  The split functions
+// (M1, M2, M4, P5) are those of the reference's nested helpers.
+template <class R> __device__ __forceinline__ R aup_M1p(R M) { return R(0.5) * (M + ab(M)); }
+template <class R> __device__ __forceinline__ R aup_M1m(R M) { return R(0.5) * (M - ab(M)); }
+template <class R> __device__ __forceinline__ R aup_M2p(R M) { return R(0.25) * (M + R(1.0)) * (M + R(1.0)); }
+template <class R> __device__ __forceinline__ R aup_M2m(R M) { return R(-0.25) * (M - R(1.0)) * (M - R(1.0)); }
+
+template <class M>
+__device__ __forceinline__ Flux5T<typename M::R> ausm_plus_up(const FaceStateT<typename M::R>& L,
+                                                              const FaceStateT<typename M::R>& Rs,
+                                                              typename M::R M_inf)
+{
+    using R = typename M::R;
+    const R rL = L.rho, pL = L.p, uL = L.vn, vL = L.vt1, wL = L.vt2, eL = L.u, aL = L.a;
+    const R keL = R(0.5) * (uL * uL + vL * vL + wL * wL);
+    const R HL = eL + dv<M>(pL, rL) + keL;
+    const R rR = Rs.rho, pR = Rs.p, uR = Rs.vn, vR = Rs.vt1, wR = Rs.vt2, eR = Rs.u, aR = Rs.a;
+    const R keR = R(0.5) * (uR * uR + vR * vR + wR * wR);
+    const R HR = eR + dv<M>(pR, rR) + keR;
+    const R a_half = R(0.5) * (aR + aL);
+    const R ML = dv<M>(uL, a_half);
+    const R MR = dv<M>(uR, a_half);
+    const R MbarSq = dv<M>(uL * uL + uR * uR, R(2.0) * a_half * a_half);
+    const R M0Sq = mn(R(1.0), mx(MbarSq, M_inf));
+    const R sM0 = sq_rt<R>(M0Sq);
+    const R fa = sM0 * (R(2.0) - sM0);
+    const R alpha = R(0.1875) * (R(-4.0) + R(5.0) * fa * fa);
+    const R beta = R(0.125);
+    R M4plus_ML, P5plus_ML, M4minus_MR, P5minus_MR;
+    if (ab(ML) >= R(1.0)) {
+        M4plus_ML = aup_M1p(ML);
+        P5plus_ML = dv<M>(R(1.0), ML) * aup_M1p(ML);
+    } else {
+        const R M2p = aup_M2p(ML), M2m = aup_M2m(ML);
+        M4plus_ML = M2p * (R(1.0) - R(16.0) * beta * M2m);
+        P5plus_ML = M2p * ((R(2.0) - ML) - R(16.0) * alpha * ML * M2m);
+    }
+    if (ab(MR) >= R(1.0)) {
+        M4minus_MR = aup_M1m(MR);
+        P5minus_MR = dv<M>(R(1.0), MR) * aup_M1m(MR);
+    } else {
+        const R M2p = aup_M2p(MR), M2m = aup_M2m(MR);
+        M4minus_MR = M2m * (R(1.0) + R(16.0) * beta * M2p);
+        P5minus_MR = M2m * ((R(-2.0) - MR) + R(16.0) * alpha * MR * M2p);
+    }
+    const R KP = R(0.25), KU = R(0.75), SIGMA = R(1.0);
+    const R r_half = R(0.5) * (rL + rR);
+    const R Mp = dv<M>(dv<M>(-KP, fa) * mx((R(1.0) - SIGMA * MbarSq), R(0.0)) * (pR - pL), r_half * a_half * a_half);
+    const R Pu = -KU * P5plus_ML * P5minus_MR * (rL + rR) * fa * a_half * (uR - uL);
+    const R M_half = M4plus_ML + M4minus_MR + Mp;
+    R ru_half = a_half * M_half;
+    if (M_half > R(0.0)) ru_half *= rL;
+    else ru_half *= rR;
+    const R p_half = P5plus_ML * pL + P5minus_MR * pR + Pu;
+    Flux5T<R> F;
+    F.mass = ru_half;
+    if (ru_half >= R(0.0)) {
+        F.mn = ru_half * uL + p_half; F.mt1 = ru_half * vL; F.mt2 = ru_half * wL; F.E = ru_half * HL;
+    } else {
+        F.mn = ru_half * uR + p_half; F.mt1 = ru_half * vR; F.mt2 = ru_half * wR; F.E = ru_half * HR;
+    }
+    return F;
+}
+
+// fluxcalc.d:1202-1371: Roe-Pike (Toro 2009, section 11.3) with the
+// Kermani-Plett entropy fix on the acoustic eigenvalues; gamma of each side is
+// the gas model's Cp/Cv (for an ideal gas one constant).  Each flux is the
+// central part less the five wave terms in the reference's order (eigenvector
+// entries 0 and 1 included, as the reference sums them).
+template <class M>
+__device__ __forceinline__ Flux5T<typename M::R> roe(const FaceStateT<typename M::R>& L,
+                                                     const FaceStateT<typename M::R>& Rs, typename M::R gL,
+                                                     typename M::R gR)
+{
+    using R = typename M::R;
+    const R rL = L.rho, pL = L.p, uL = L.vn, vL = L.vt1, wL = L.vt2, eL = L.u, aL = L.a;
+    const R pLrL = dv<M>(pL, rL);
+    const R keL = R(0.5) * (uL * uL + vL * vL + wL * wL);
+    const R HL = eL + pLrL + keL;
+    const R rR = Rs.rho, pR = Rs.p, uR = Rs.vn, vR = Rs.vt1, wR = Rs.vt2, eR = Rs.u, aR = Rs.a;
+    const R pRrR = dv<M>(pR, rR);
+    const R keR = R(0.5) * (uR * uR + vR * vR + wR * wR);
+    const R HR = eR + pRrR + keR;
+    const R ghat = (gR + gL) / R(2.0);
+    const R rhat = sq_rt<R>(rL * rR);
+    const R sL = sq_rt<R>(rL), sR = sq_rt<R>(rR);
+    const R den = sL + sR;
+    const R uhat = dv<M>(sL * uL + sR * uR, den);
+    const R vhat = dv<M>(sL * vL + sR * vR, den);
+    const R what = dv<M>(sL * wL + sR * wR, den);
+    const R Hhat = dv<M>(sL * HL + sR * HR, den);
+    const R ke_hat = R(0.5) * (uhat * uhat + vhat * vhat + what * what);
+    const R ahat2 = (ghat - R(1.0)) * (Hhat - ke_hat);
+    const R ahat = sq_rt<R>(ahat2);
+    R lam[5] = {uhat - ahat, uhat, uhat, uhat, uhat + ahat};
+    const R lambdaL = uL - aL, lambdaR = uR - aR;
+    const R eps = R(1.0e-12);
+    const R EPS = R(4.0) * mx(R(0.0), (lambdaR - lambdaL));
+    if (ab(lam[0]) < EPS) lam[0] = dv<M>(lam[0] * lam[0] + EPS * EPS, R(2.0) * EPS + eps);
+    if (ab(lam[4]) < EPS) lam[4] = dv<M>(lam[4] * lam[4] + EPS * EPS, R(2.0) * EPS + eps);
+    const R K[5][5] = {{R(1.0), uhat - ahat, vhat, what, Hhat - uhat * ahat},
+                       {R(1.0), uhat, vhat, what, ke_hat},
+                       {R(0.0), R(0.0), R(1.0), R(0.0), vhat},
+                       {R(0.0), R(0.0), R(0.0), R(1.0), what},
+                       {R(1.0), uhat + ahat, vhat, what, Hhat + uhat * ahat}};
+    const R inv2a2 = dv<M>(R(1.0), R(2.0) * ahat2);
+    R alpha[5];
+    alpha[0] = inv2a2 * ((pR - pL) - rhat * ahat * (uR - uL));
+    alpha[1] = (rR - rL) - dv<M>(pR - pL, ahat2);
+    alpha[2] = rhat * (vR - vL);
+    alpha[3] = rhat * (wR - wL);
+    alpha[4] = inv2a2 * ((pR - pL) + rhat * ahat * (uR - uL));
+    R c[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) c[i] = R(0.5) * alpha[i] * ab(lam[i]);
+    R Fq[5];
+    Fq[0] = R(0.5) * (rL * uL + rR * uR);
+    Fq[1] = R(0.5) * ((pL + rL * uL * uL) + (pR + rR * uR * uR));
+    Fq[2] = R(0.5) * (rL * uL * vL + rR * uR * vR);
+    Fq[3] = R(0.5) * (rL * uL * wL + rR * uR * wR);
+    Fq[4] = R(0.5) * ((rL * eL + rL * (uL * uL + vL * vL + wL * wL) / R(2.0) + pL) * uL +
+                      (rR * eR + rR * (uR * uR + vR * vR + wR * wR) / R(2.0) + pR) * uR);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) Fq[q] -= c[i] * K[i][q];
+    }
+    Flux5T<R> F;
+    F.mass = Fq[0]; F.mn = Fq[1]; F.mt1 = Fq[2]; F.mt2 = Fq[3]; F.E = Fq[4];
+    return F;
+}
+
+// Options of the flux calculators: the shear tolerance of the adaptive ones,
+// whether every face takes the near-shock calculator (hanel, efm), whether
+// that calculator is efm rather than hanel, whether FLUX_OTHER is roe rather
+// than ausm_plus_up, and ausm_plus_up's M_inf.
 template <class R>
-struct ShockOpt {
+struct FluxOpt {
     R shear_tol;
-    bool always_hanel;
+    bool always_shock_flux;
+    bool efm;
+    bool roe;
+    R M_inf;
 };
 
 template <class M, int FLUX>
 __device__ __forceinline__ Flux5T<typename M::R> flux_calc(const FaceStateT<typename M::R>& L,
                                                            const FaceStateT<typename M::R>& R,
+                                                           const IdealGasT<typename M::R>& g,
                                                            bool shock = false,
-                                                           ShockOpt<typename M::R> opt = {})
+                                                           FluxOpt<typename M::R> opt = {})
 {
     if constexpr (FLUX == FLUX_HLLC) return hllc<M>(L, R);
     else if constexpr (FLUX == FLUX_SHOCK) {
-        if (opt.always_hanel || hanel_at_face<M>(L, R, shock, opt.shear_tol)) return hanel<M>(L, R);
+        // adaptive_hanel_ausmdv / adaptive_efm_ausmdv (fluxcalc.d:802-836), or
+        // hanel / efm at every face
+        if (opt.always_shock_flux || hanel_at_face<M>(L, R, shock, opt.shear_tol))
+            return opt.efm ? efm<M>(L, R, g) : hanel<M>(L, R);
         return ausmdv<M>(L, R);
+    } else if constexpr (FLUX == FLUX_OTHER) {
+        if (opt.roe) return roe<M>(L, R, g.CpCv, g.CpCv);
+        return ausm_plus_up<M>(L, R, opt.M_inf);
     } else return ausmdv<M>(L, R);
 }
 

@@ -201,14 +201,15 @@ __device__ __forceinline__ void gen_flux(const TPArgs& A, const GSide<NSP>& L, c
         Lf.vn = R(L.v[0]); Lf.vt1 = R(L.v[1]); Lf.vt2 = R(L.v[2]);
         Rf.rho = R(Rt.rho); Rf.p = R(Rt.p); Rf.u = R(Rt.u); Rf.a = R(Rt.a);
         Rf.vn = R(Rt.v[0]); Rf.vt1 = R(Rt.v[1]); Rf.vt2 = R(Rt.v[2]);
+        Lf.T = R(L.T); Rf.T = R(Rt.T);
         if constexpr (M::kFast) {
             Lf.RT = R(A.ig.Rgas) * R(L.T);
             Rf.RT = R(A.ig.Rgas) * R(Rt.T);
         } else {
             Lf.RT = Rf.RT = R(0);
         }
-        const ShockOpt<R> opt{R(A.shear_tol), A.always_hanel != 0};
-        Flux5T<R> Fl = flux_calc<M, FLUX>(Lf, Rf, shock, opt);
+        const FluxOpt<R> opt{R(A.shear_tol), A.always_hanel != 0, A.shock_efm != 0, A.other_roe != 0, R(A.M_inf)};
+        Flux5T<R> Fl = flux_calc<M, FLUX>(Lf, Rf, cast_gas<R>(A.ig), shock, opt);
         F[0] = Fl.mass; F[1] = Fl.mn; F[2] = Fl.mt1; F[3] = Fl.mt2; F[4] = Fl.E;
     } else {
         using MA = typename TPArith<M>::type;
@@ -219,18 +220,7 @@ __device__ __forceinline__ void gen_flux(const TPArgs& A, const GSide<NSP>& L, c
         for (int q = 0; q < 3; ++q) { Lc.v[q] = L.v[q]; Rc.v[q] = Rt.v[q]; }
 #pragma unroll
         for (int q = 0; q < NSP; ++q) { Lc.Y[q] = L.Y[q]; Rc.Y[q] = Rt.Y[q]; }
-        if constexpr (FLUX == FLUX_HLLC) {
-            tp_hllc<MA, NSP>(Lc, Rc, F);
-        } else if constexpr (FLUX == FLUX_SHOCK) {
-            const double sound_speed = 0.5 * (Lc.a + Rc.a);
-            const double shear_y = dv<MA>(fabs(Lc.v[1] - Rc.v[1]), sound_speed);
-            const double shear_z = dv<MA>(fabs(Lc.v[2] - Rc.v[2]), sound_speed);
-            const bool shear_is_small = fmax(shear_y, shear_z) <= A.shear_tol;
-            if (A.always_hanel || (shock && shear_is_small)) tp_hanel<MA, NSP>(Lc, Rc, F);
-            else tp_ausmdv<MA, NSP>(Lc, Rc, F);
-        } else {
-            tp_ausmdv<MA, NSP>(Lc, Rc, F);
-        }
+        tp_flux<MA, NSP, FLUX>(A, *A.gas, Lc, Rc, shock, F);
     }
 }
 
@@ -644,6 +634,7 @@ void launch_gen_stage_m(const TPArgs& A, hipStream_t stream, int flux)
 {
     if (flux == FLUX_HLLC) launch_gen_faces<M, NSP, FLUX_HLLC>(A, stream);
     else if (flux == FLUX_SHOCK) launch_gen_faces<M, NSP, FLUX_SHOCK>(A, stream);
+    else if (flux == FLUX_OTHER) launch_gen_faces<M, NSP, FLUX_OTHER>(A, stream);
     else launch_gen_faces<M, NSP, FLUX_AUSMDV>(A, stream);
     const long nc = (long)A.ni * A.nj * ((A.dims == 3) ? A.nk : 1);
     const dim3 grid((unsigned)((nc + 127) / 128)), b(128);

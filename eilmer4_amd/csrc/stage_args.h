@@ -109,7 +109,11 @@ struct StageArgs {
     const unsigned char* nbrS[6];
     unsigned char* Snew;
     double shear_tol, compression_tol;
-    int always_hanel;
+    // flux calculator options: every face takes the near-shock calculator
+    // (hanel, efm); that calculator is efm; FLUX_OTHER is roe (else
+    // ausm_plus_up); ausm_plus_up's M_inf (globalconfig.d:591)
+    int always_hanel, shock_efm, other_roe;
+    double M_inf;
     // boundaries whose faces the interior code does not handle (one-sided
     // reconstruction, wall or outflow flux): the stage kernel leaves fix_lo[d]
     // and fix_hi[d] layers of cells at the two ends of direction d to the
@@ -223,7 +227,8 @@ struct TPArgs {
     const unsigned char* nbrS[6];
     unsigned char* Snew;
     double shear_tol, compression_tol;
-    int always_hanel;
+    int always_hanel, shock_efm, other_roe;   // flux calculator options, as in StageArgs
+    double M_inf;
     // faces whose flux the face kernels do not compute as the reference does
     // (next to a wall without ghost data: the wall face and the one after it;
     // a simple-outflow-flux face), recomputed by the boundary-face kernel:

@@ -300,7 +300,7 @@ __device__ __forceinline__ R vcomp(const SideT<R>& S)
 template <class M, int FLUX, int PN, int PT1, int PT2>
 __device__ __forceinline__ void side_flux(const SideT<typename M::R>& L, const SideT<typename M::R>& Rs,
                                           const IdealGasT<typename M::R>& g, typename M::R Fg[5],
-                                          ShockOpt<typename M::R> opt = {})
+                                          FluxOpt<typename M::R> opt = {})
 {
     using R = typename M::R;
     FaceStateT<R> Lf, Rf;
@@ -316,6 +316,8 @@ __device__ __forceinline__ void side_flux(const SideT<typename M::R>& L, const S
     Rf.vn = vcomp<PN>(Rs); Rf.vt1 = vcomp<PT1>(Rs); Rf.vt2 = vcomp<PT2>(Rs);
     R TL = Lf.u * g.Cvinv;
     R TR = Rf.u * g.Cvinv;
+    Lf.T = TL;
+    Rf.T = TR;
     if constexpr (M::kFast) {
         Lf.RT = g.Rgas * TL;
         Rf.RT = g.Rgas * TR;
@@ -326,7 +328,7 @@ __device__ __forceinline__ void side_flux(const SideT<typename M::R>& L, const S
         Rf.p = Rf.rho * g.Rgas * TR;
         Lf.RT = Rf.RT = R(0);
     }
-    Flux5T<R> F = flux_calc<M, FLUX>(Lf, Rf, shock, opt);
+    Flux5T<R> F = flux_calc<M, FLUX>(Lf, Rf, g, shock, opt);
     Fg[0] = F.mass;
     Fg[1 + PN] = F.mn;
     Fg[1 + PT1] = F.mt1;
@@ -335,9 +337,9 @@ __device__ __forceinline__ void side_flux(const SideT<typename M::R>& L, const S
 }
 
 template <class R>
-__device__ __forceinline__ ShockOpt<R> shock_opt(KArgs A)
+__device__ __forceinline__ FluxOpt<R> shock_opt(KArgs A)
 {
-    return ShockOpt<R>{R(A->shear_tol), A->always_hanel != 0};
+    return FluxOpt<R>{R(A->shear_tol), A->always_hanel != 0, A->shock_efm != 0, A->other_roe != 0, R(A->M_inf)};
 }
 
 template <class R>
@@ -1029,10 +1031,12 @@ void launch_boundary_cells_mode(const StageArgs& A, hipStream_t stream, int dims
     if (dims == 3) {
         if (flux == FLUX_HLLC) hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_HLLC, false>), grid, block, 0, stream, A);
         else if (flux == FLUX_SHOCK) hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_SHOCK, false>), grid, block, 0, stream, A);
+        else if (flux == FLUX_OTHER) hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_OTHER, false>), grid, block, 0, stream, A);
         else hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_AUSMDV, false>), grid, block, 0, stream, A);
     } else {
         if (flux == FLUX_HLLC) hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_HLLC, false>), grid, block, 0, stream, A);
         else if (flux == FLUX_SHOCK) hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_SHOCK, false>), grid, block, 0, stream, A);
+        else if (flux == FLUX_OTHER) hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_OTHER, false>), grid, block, 0, stream, A);
         else hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_AUSMDV, false>), grid, block, 0, stream, A);
     }
 }
@@ -1047,10 +1051,12 @@ void launch_boundary_cells_batch_mode(const BatchArgs& B, int nwg, hipStream_t s
     if (dims == 3) {
         if (flux == FLUX_HLLC) hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_HLLC, true>), grid, block, 0, stream, B);
         else if (flux == FLUX_SHOCK) hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_SHOCK, true>), grid, block, 0, stream, B);
+        else if (flux == FLUX_OTHER) hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_OTHER, true>), grid, block, 0, stream, B);
         else hipLaunchKernelGGL((boundary_cell_kernel<M, 3, FLUX_AUSMDV, true>), grid, block, 0, stream, B);
     } else {
         if (flux == FLUX_HLLC) hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_HLLC, true>), grid, block, 0, stream, B);
         else if (flux == FLUX_SHOCK) hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_SHOCK, true>), grid, block, 0, stream, B);
+        else if (flux == FLUX_OTHER) hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_OTHER, true>), grid, block, 0, stream, B);
         else hipLaunchKernelGGL((boundary_cell_kernel<M, 2, FLUX_AUSMDV, true>), grid, block, 0, stream, B);
     }
 }
@@ -1059,18 +1065,28 @@ template <class M, int DIMS, int FLUX>
 void launch_stage_df(const StageArgs& A, dim3 grid, hipStream_t stream)
 {
     const dim3 block(TILE_X * TILE_Y);
-    if (A.high_order && A.apply_limiter && A.extrema_clipping)
-        hipLaunchKernelGGL((stage_kernel<M, TILE_X, TILE_Y, DIMS, FLUX, true, false>), grid, block, 0, stream, A);
-    else
-        hipLaunchKernelGGL((stage_kernel<M, TILE_X, TILE_Y, DIMS, FLUX, false, false>), grid, block, 0, stream, A);
+    // the less common calculators (FLUX_OTHER) read the reconstruction options
+    // from the arguments in every case
+    if constexpr (FLUX != FLUX_OTHER) {
+        if (A.high_order && A.apply_limiter && A.extrema_clipping) {
+            hipLaunchKernelGGL((stage_kernel<M, TILE_X, TILE_Y, DIMS, FLUX, true, false>), grid, block, 0, stream, A);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((stage_kernel<M, TILE_X, TILE_Y, DIMS, FLUX, false, false>), grid, block, 0, stream, A);
 }
 
 template <class M, int DIMS, int FLUX>
 void launch_stage_batch_df(const BatchArgs& B, int nwg, bool def, hipStream_t stream)
 {
     const dim3 block(TILE_X * TILE_Y), grid(nwg);
-    if (def) hipLaunchKernelGGL((stage_kernel<M, TILE_X, TILE_Y, DIMS, FLUX, true, true>), grid, block, 0, stream, B);
-    else hipLaunchKernelGGL((stage_kernel<M, TILE_X, TILE_Y, DIMS, FLUX, false, true>), grid, block, 0, stream, B);
+    if constexpr (FLUX != FLUX_OTHER) {
+        if (def) {
+            hipLaunchKernelGGL((stage_kernel<M, TILE_X, TILE_Y, DIMS, FLUX, true, true>), grid, block, 0, stream, B);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((stage_kernel<M, TILE_X, TILE_Y, DIMS, FLUX, false, true>), grid, block, 0, stream, B);
 }
 
 template <class M>
@@ -1079,10 +1095,12 @@ void launch_stage_batch_mode(const BatchArgs& B, int nwg, bool def, hipStream_t 
     if (dims == 3) {
         if (flux == FLUX_HLLC) launch_stage_batch_df<M, 3, FLUX_HLLC>(B, nwg, def, stream);
         else if (flux == FLUX_SHOCK) launch_stage_batch_df<M, 3, FLUX_SHOCK>(B, nwg, def, stream);
+        else if (flux == FLUX_OTHER) launch_stage_batch_df<M, 3, FLUX_OTHER>(B, nwg, def, stream);
         else launch_stage_batch_df<M, 3, FLUX_AUSMDV>(B, nwg, def, stream);
     } else {
         if (flux == FLUX_HLLC) launch_stage_batch_df<M, 2, FLUX_HLLC>(B, nwg, def, stream);
         else if (flux == FLUX_SHOCK) launch_stage_batch_df<M, 2, FLUX_SHOCK>(B, nwg, def, stream);
+        else if (flux == FLUX_OTHER) launch_stage_batch_df<M, 2, FLUX_OTHER>(B, nwg, def, stream);
         else launch_stage_batch_df<M, 2, FLUX_AUSMDV>(B, nwg, def, stream);
     }
 }
@@ -1094,10 +1112,12 @@ void launch_stage_mode(const StageArgs& A, dim3 grid, hipStream_t stream, int di
     if (dims == 3) {
         if (flux == FLUX_HLLC) launch_stage_df<M, 3, FLUX_HLLC>(A, grid, stream);
         else if (flux == FLUX_SHOCK) launch_stage_df<M, 3, FLUX_SHOCK>(A, grid, stream);
+        else if (flux == FLUX_OTHER) launch_stage_df<M, 3, FLUX_OTHER>(A, grid, stream);
         else launch_stage_df<M, 3, FLUX_AUSMDV>(A, grid, stream);
     } else {
         if (flux == FLUX_HLLC) launch_stage_df<M, 2, FLUX_HLLC>(A, grid, stream);
         else if (flux == FLUX_SHOCK) launch_stage_df<M, 2, FLUX_SHOCK>(A, grid, stream);
+        else if (flux == FLUX_OTHER) launch_stage_df<M, 2, FLUX_OTHER>(A, grid, stream);
         else launch_stage_df<M, 2, FLUX_AUSMDV>(A, grid, stream);
     }
 }

@@ -304,6 +304,153 @@ __device__ __forceinline__ void tp_hllc(const TPCell<NSP>& L, const TPCell<NSP>&
     for (int i = 0; i < NSP; ++i) F[5 + i] = F[5 + i] + SK * (coef * K.Y[i] - rK * K.Y[i]);
 }
 
+// gamma = Cp/Cv of a mixture state (gas_model.d:162; therm_perf_gas.d:366-380)
+template <class M, int NSP>
+__device__ __forceinline__ double tp_gamma(const TPGas& g, const double* Y, double T)
+{
+    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    if constexpr (M::kFast) {
+        const double R = tp_Rmix<NSP>(g, Y);
+        double u, cv;
+        tp_mix_u_cv<NSP>(g, Y, R, T, &u, &cv);
+        return dv<MD>(cv + R, cv);
+    } else {
+        double Cp = 0.0, Cv = 0.0;
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) Cp += Y[s] * cea_Cp<M>(g.curve[s], T);
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) Cv += Y[s] * (cea_Cp<M>(g.curve[s], T) - g.R[s]);
+        return dv<MD>(Cp, Cv);
+    }
+}
+
+// Cv of a mixture state: sum Y_i (Cp_i(T) - R_i) (therm_perf_gas.d:366-371)
+template <class M, int NSP>
+__device__ __forceinline__ double tp_cv(const TPGas& g, const double* Y, double T)
+{
+    if constexpr (M::kFast) {
+        const double R = tp_Rmix<NSP>(g, Y);
+        double u, cv;
+        tp_mix_u_cv<NSP>(g, Y, R, T, &u, &cv);
+        return cv;
+    } else {
+        double Cv = 0.0;
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) Cv += Y[s] * (cea_Cp<M>(g.curve[s], T) - g.R[s]);
+        return Cv;
+    }
+}
+
+// efmflx (fluxcalc.d:626-790) with species, which follow the sign of the mass
+// flux; Cv of each side from the gas model at the side's temperature.
+template <class M, int NSP>
+__device__ __forceinline__ void tp_efm(const TPGas& g, const TPCell<NSP>& L, const TPCell<NSP>& Rt, double* F)
+{
+    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    const double dtwspi = 0.282094792;
+    const double rhoL = L.rho, presL = L.p, eL = L.u, tL = L.T;
+    const double hL = eL + dv<MD>(presL, rhoL);
+    const double vnL = L.v[0], vpL = L.v[1], vqL = L.v[2];
+    const double rhoR = Rt.rho, presR = Rt.p, eR = Rt.u, tR = Rt.T;
+    const double hR = eR + dv<MD>(presR, rhoR);
+    const double vnR = Rt.v[0], vpR = Rt.v[1], vqR = Rt.v[2];
+    const double cvL = tp_cv<M, NSP>(g, L.Y, tL);
+    const double RgasL = dv<MD>(presL, rhoL * tL);
+    const double cvR = tp_cv<M, NSP>(g, Rt.Y, tR);
+    const double RgasR = dv<MD>(presR, rhoR * tR);
+    const double rLsqrt = sqrt(rhoL), rRsqrt = sqrt(rhoR);
+    const double alpha = dv<MD>(rLsqrt, rLsqrt + rRsqrt);
+    const double cv = alpha * cvL + (1.0 - alpha) * cvR;
+    const double Rgas = alpha * RgasL + (1.0 - alpha) * RgasR;
+    const double cp = cv + Rgas;
+    const double gam = dv<MD>(cp, cv);
+    const double con = dv<MD>(0.5 * (gam + 1.0), gam - 1.0);
+    const double rtL = Rgas * tL;
+    const double cmpL = sqrt(2.0 * rtL);
+    const double hvsqL = 0.5 * (vnL * vnL + vpL * vpL + vqL * vqL);
+    const double snL = dv<MD>(vnL, 1.0 * cmpL);
+    double exL, efL;
+    exxef<MD>(snL, exL, efL);
+    const double wL = 0.5 * (1.0 + efL);
+    const double dL = exL * dtwspi;
+    const double rtR = dv<MD>(presR, rhoR);
+    const double cmpR = sqrt(2.0 * rtR);
+    const double hvsqR = 0.5 * (vnR * vnR + vpR * vpR + vqR * vqR);
+    const double snR = dv<MD>(vnR, 1.0 * cmpR);
+    double exR, efR;
+    exxef<MD>(snR, exR, efR);
+    const double wR = 0.5 * (1.0 - efR);
+    const double dR = -exR * dtwspi;
+    const double fmsL = (wL * rhoL * vnL) + (dL * cmpL * rhoL);
+    const double fmsR = (wR * rhoR * vnR) + (dR * cmpR * rhoR);
+    F[0] = fmsL + fmsR;
+    F[1] = fmsL * vnL + fmsR * vnR + wL * presL + wR * presR;
+    F[2] = fmsL * vpL + fmsR * vpR;
+    F[3] = fmsL * vqL + fmsR * vqR;
+    F[4] = (wL * rhoL * vnL) * (hvsqL + hL) + (wR * rhoR * vnR) * (hvsqR + hR) +
+           (dL * cmpL * rhoL) * (hvsqL + con * rtL) + (dR * cmpR * rhoR) * (hvsqR + con * rtR);
+    const bool left = F[0] > 0.0;
+#pragma unroll
+    for (int i = 0; i < NSP; ++i) F[5 + i] = F[0] * (left ? L.Y[i] : Rt.Y[i]);
+}
+
+// The face state of a mixture side as the ideal-gas flux functions read it.
+template <int NSP>
+__device__ __forceinline__ FaceStateT<double> tp_face_state(const TPCell<NSP>& c)
+{
+    FaceStateT<double> f;
+    f.rho = c.rho; f.p = c.p; f.u = c.u; f.a = c.a; f.vn = c.v[0]; f.vt1 = c.v[1]; f.vt2 = c.v[2];
+    f.RT = 0.0; f.T = c.T;
+    return f;
+}
+
+// ausm_plus_up (fluxcalc.d:860-1037) and roe (fluxcalc.d:1202-1371) with
+// species upwinded by the mass flux (ausm_plus_up: M_half > 0 picks the left
+// density, ru_half >= 0 the left species; roe: F.mass >= 0); roe's gamma of
+// each side from the gas model.  The gas-dynamic part is the ideal-gas
+// functions' (they read p, rho, u, a and velocities only).
+template <class M, int NSP>
+__device__ __forceinline__ void tp_other_flux(const TPGas& g, bool use_roe, double M_inf, const TPCell<NSP>& L,
+                                              const TPCell<NSP>& Rt, double* F)
+{
+    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    const FaceStateT<double> Lf = tp_face_state<NSP>(L), Rf = tp_face_state<NSP>(Rt);
+    Flux5T<double> Fl;
+    if (use_roe) Fl = roe<MD>(Lf, Rf, tp_gamma<M, NSP>(g, L.Y, L.T), tp_gamma<M, NSP>(g, Rt.Y, Rt.T));
+    else Fl = ausm_plus_up<MD>(Lf, Rf, M_inf);
+    F[0] = Fl.mass; F[1] = Fl.mn; F[2] = Fl.mt1; F[3] = Fl.mt2; F[4] = Fl.E;
+    const bool left = F[0] >= 0.0;
+#pragma unroll
+    for (int i = 0; i < NSP; ++i) F[5 + i] = F[0] * (left ? L.Y[i] : Rt.Y[i]);
+}
+
+// The flux calculator of a mixture face in the face frame (fluxcalc.d:61-89):
+// FLUX_SHOCK: adaptive_hanel_ausmdv / adaptive_efm_ausmdv (fluxcalc.d:802-836),
+// or hanel / efm throughout; FLUX_OTHER: ausm_plus_up or roe.
+template <class MA, int NSP, int FLUX>
+__device__ __forceinline__ void tp_flux(const TPArgs& A, const TPGas& g, const TPCell<NSP>& Lf,
+                                        const TPCell<NSP>& Rf, bool shock, double* F)
+{
+    if constexpr (FLUX == FLUX_HLLC) {
+        tp_hllc<MA, NSP>(Lf, Rf, F);
+    } else if constexpr (FLUX == FLUX_SHOCK) {
+        const double sound_speed = 0.5 * (Lf.a + Rf.a);
+        const double shear_y = dv<MA>(fabs(Lf.v[1] - Rf.v[1]), sound_speed);
+        const double shear_z = dv<MA>(fabs(Lf.v[2] - Rf.v[2]), sound_speed);
+        const bool shear_is_small = fmax(shear_y, shear_z) <= A.shear_tol;
+        if (A.always_hanel || (shock && shear_is_small)) {
+            if (A.shock_efm) tp_efm<MA, NSP>(g, Lf, Rf, F);
+            else tp_hanel<MA, NSP>(Lf, Rf, F);
+        } else {
+            tp_ausmdv<MA, NSP>(Lf, Rf, F);
+        }
+    } else if constexpr (FLUX == FLUX_OTHER) {
+        tp_other_flux<MA, NSP>(g, A.other_roe != 0, A.M_inf, Lf, Rf, F);
+    } else {
+        tp_ausmdv<MA, NSP>(Lf, Rf, F);
+    }
+}
+
 template <class M, int NSP, int FLUX, int DIR>
 __global__ __launch_bounds__(128) void tp_face_kernel(TPArgs A)
 {
@@ -377,20 +524,8 @@ __global__ __launch_bounds__(128) void tp_face_kernel(TPArgs A)
     if (!okL) Lf = tp_cell<NSP>(r[1], pn, pt1, pt2);
     if (!okR) Rf = tp_cell<NSP>(r[2], pn, pt1, pt2);
     double F[5 + NSP];
-    if constexpr (FLUX == FLUX_HLLC) {
-        tp_hllc<MA, NSP>(Lf, Rf, F);
-    } else if constexpr (FLUX == FLUX_SHOCK) {
-        // adaptive_hanel_ausmdv (fluxcalc.d:822-836), or hanel throughout
-        const bool shock = (r[1].s && *r[1].s) || (r[2].s && *r[2].s);
-        const double sound_speed = 0.5 * (Lf.a + Rf.a);
-        const double shear_y = dv<MA>(fabs(Lf.v[1] - Rf.v[1]), sound_speed);
-        const double shear_z = dv<MA>(fabs(Lf.v[2] - Rf.v[2]), sound_speed);
-        const bool shear_is_small = fmax(shear_y, shear_z) <= A.shear_tol;
-        if (A.always_hanel || (shock && shear_is_small)) tp_hanel<MA, NSP>(Lf, Rf, F);
-        else tp_ausmdv<MA, NSP>(Lf, Rf, F);
-    } else {
-        tp_ausmdv<MA, NSP>(Lf, Rf, F);
-    }
+    const bool shock = (FLUX == FLUX_SHOCK) && ((r[1].s && *r[1].s) || (r[2].s && *r[2].s));
+    tp_flux<MA, NSP, FLUX>(A, *A.gas, Lf, Rf, shock, F);
     if (active) {
         double* dst = A.F[DIR];
         const long fq = A.fqs[DIR];
@@ -711,26 +846,6 @@ __global__ __launch_bounds__(128) void tp_shock_detect_kernel(TPArgs A)
     A.Snew[c] = S ? 1 : 0;
 }
 
-// gamma = Cp/Cv of a mixture state (gas_model.d:162; therm_perf_gas.d:366-380)
-template <class M, int NSP>
-__device__ __forceinline__ double tp_gamma(const TPGas& g, const double* Y, double T)
-{
-    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
-    if constexpr (M::kFast) {
-        const double R = tp_Rmix<NSP>(g, Y);
-        double u, cv;
-        tp_mix_u_cv<NSP>(g, Y, R, T, &u, &cv);
-        return dv<MD>(cv + R, cv);
-    } else {
-        double Cp = 0.0, Cv = 0.0;
-#pragma unroll
-        for (int s = 0; s < NSP; ++s) Cp += Y[s] * cea_Cp<M>(g.curve[s], T);
-#pragma unroll
-        for (int s = 0; s < NSP; ++s) Cv += Y[s] * (cea_Cp<M>(g.curve[s], T) - g.R[s]);
-        return dv<MD>(Cp, Cv);
-    }
-}
-
 // The faces of a mixture block whose flux is not the two-sided reconstruction
 // of tp_face_kernel (sfluidblock.d:2189-2345): next to a wall without ghost
 // data (WallBC_WithSlip1) the wall face takes the wall flux of
@@ -879,19 +994,8 @@ __global__ __launch_bounds__(64) void tp_boundary_face_kernel(TPArgs A)
         }
         if (!okL) Lf = tp_cell<NSP>(r[1], pn, pt1, pt2);
         if (!okR) Rf = tp_cell<NSP>(r[2], pn, pt1, pt2);
-        if constexpr (FLUX == FLUX_HLLC) {
-            tp_hllc<MA, NSP>(Lf, Rf, F);
-        } else if constexpr (FLUX == FLUX_SHOCK) {
-            const bool shock = (r[1].s && *r[1].s) || (r[2].s && *r[2].s);
-            const double sound_speed = 0.5 * (Lf.a + Rf.a);
-            const double shear_y = dv<MA>(fabs(Lf.v[1] - Rf.v[1]), sound_speed);
-            const double shear_z = dv<MA>(fabs(Lf.v[2] - Rf.v[2]), sound_speed);
-            const bool shear_is_small = fmax(shear_y, shear_z) <= A.shear_tol;
-            if (A.always_hanel || (shock && shear_is_small)) tp_hanel<MA, NSP>(Lf, Rf, F);
-            else tp_ausmdv<MA, NSP>(Lf, Rf, F);
-        } else {
-            tp_ausmdv<MA, NSP>(Lf, Rf, F);
-        }
+        const bool shock = (FLUX == FLUX_SHOCK) && ((r[1].s && *r[1].s) || (r[2].s && *r[2].s));
+        tp_flux<MA, NSP, FLUX>(A, g, Lf, Rf, shock, F);
     }
     const int nfi = A.ni + (d == 0), nfj = A.nj + (d == 1);
     const long f = ((long)k * nfj + j) * nfi + i;
@@ -994,11 +1098,13 @@ void launch_tp_stage_m(const TPArgs& A, hipStream_t stream, int flux)
 {
     if (flux == FLUX_HLLC) launch_tp_faces<M, FLUX_HLLC>(A, stream);
     else if (flux == FLUX_SHOCK) launch_tp_faces<M, FLUX_SHOCK>(A, stream);
+    else if (flux == FLUX_OTHER) launch_tp_faces<M, FLUX_OTHER>(A, stream);
     else launch_tp_faces<M, FLUX_AUSMDV>(A, stream);
     if (A.fix_off[6] > 0) {
         const dim3 g((unsigned)((A.fix_off[6] + 63) / 64)), b(64);
         if (flux == FLUX_HLLC) hipLaunchKernelGGL((tp_boundary_face_kernel<M, 5, FLUX_HLLC>), g, b, 0, stream, A);
         else if (flux == FLUX_SHOCK) hipLaunchKernelGGL((tp_boundary_face_kernel<M, 5, FLUX_SHOCK>), g, b, 0, stream, A);
+        else if (flux == FLUX_OTHER) hipLaunchKernelGGL((tp_boundary_face_kernel<M, 5, FLUX_OTHER>), g, b, 0, stream, A);
         else hipLaunchKernelGGL((tp_boundary_face_kernel<M, 5, FLUX_AUSMDV>), g, b, 0, stream, A);
     }
     const long nc = (long)A.ni * A.nj * ((A.dims == 3) ? A.nk : 1);

@@ -24,8 +24,10 @@ class FluxCalculator(IntEnum):          # fvcore.d:184-194
 
 
 # calculators the device path implements
-DEVICE_FLUX_CALCULATORS = (FluxCalculator.ausmdv, FluxCalculator.hanel, FluxCalculator.adaptive_hanel_ausmdv,
-                           FluxCalculator.hllc)
+# (hlle is MHD only, fluxcalc.d:1197, and adaptive_hlle_roe takes it near shocks)
+DEVICE_FLUX_CALCULATORS = (FluxCalculator.ausmdv, FluxCalculator.hanel, FluxCalculator.efm,
+                           FluxCalculator.ausm_plus_up, FluxCalculator.adaptive_efm_ausmdv,
+                           FluxCalculator.adaptive_hanel_ausmdv, FluxCalculator.roe, FluxCalculator.hllc)
 
 
 def flux_calculator_name(fcalc):

@@ -23,7 +23,7 @@ EXPORTS = [
     "e4hip_block_upload", "e4hip_block_download", "e4hip_block_ncells", "e4hip_config",
     "e4hip_explicit_increment", "e4hip_determine_dt", "e4hip_profile_enable", "e4hip_profile_read",
     "e4hip_gas_tp_set_guess_model", "e4hip_tp_guess", "e4hip_tp_decode_stats", "e4hip_tp_face_stats",
-    "e4hip_mlp_grad", "e4hip_mlp_train",
+    "e4hip_mlp_grad", "e4hip_mlp_train", "e4hip_block_kchunk",
 ]
 
 
@@ -41,7 +41,8 @@ class StepConfig(C.Structure):
                 ("thermo_interpolator", C.c_int), ("precision_mode", C.c_int),
                 ("max_invalid_cells", C.c_int), ("learned_eos_guess", C.c_int), ("batch_blocks", C.c_int),
                 ("shear_tolerance", C.c_double), ("compression_tolerance", C.c_double),
-                ("stringent_cfl", C.c_int), ("use_graphs", C.c_int)]
+                ("stringent_cfl", C.c_int), ("use_graphs", C.c_int), ("M_inf", C.c_double),
+                ("autotune", C.c_int)]
 
 
 _lib = None
@@ -67,6 +68,7 @@ def load(path=LIB_PATH):
     L.e4hip_block_create_geometry.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_int, _ip]
     L.e4hip_block_destroy.argtypes = [C.c_int]
     L.e4hip_block_set_bc.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
+    L.e4hip_block_kchunk.argtypes = [C.c_int, _ip]
     L.e4hip_block_connect.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
     L.e4hip_block_set_axisymmetric.argtypes = [C.c_int, C.c_double, C.c_double]
     L.e4hip_block_set_source_terms.argtypes = [C.c_int, _dp]

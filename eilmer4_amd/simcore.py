@@ -43,6 +43,8 @@ class GlobalConfig:
     max_step: int = 100
     fixed_time_step: bool = False
     stringent_cfl: bool = False             # globalconfig.d:729
+    M_inf: float = 0.01                     # ausm_plus_up, globalconfig.d:591
+    autotune: bool = True                   # k-chunk heights of the stage kernel timed per block shape
     use_graphs: bool = False                # replay repeated steps from a captured HIP graph
 
     def step_config(self):
@@ -55,7 +57,8 @@ class GlobalConfig:
                                int(self.thermo_interpolator), int(self.precision_mode), int(self.max_invalid_cells),
                                int(bool(self.learned_eos_guess)), int(bool(self.batch_blocks)),
                                float(self.shear_tolerance), float(self.compression_tolerance),
-                               int(bool(self.stringent_cfl)), int(bool(self.use_graphs)))
+                               int(bool(self.stringent_cfl)), int(bool(self.use_graphs)), float(self.M_inf),
+                               int(bool(self.autotune)))
 
 
 class SimState:

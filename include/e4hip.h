@@ -44,10 +44,16 @@ extern "C" {
 /* ---- FluxCalculator (eilmer/fvcore.d:184-194; values kept, hllc appended) */
 #define E4HIP_FLUX_AUSMDV      0
 #define E4HIP_FLUX_HANEL       1   /* fluxcalc.d:532-623 */
+#define E4HIP_FLUX_EFM         2   /* fluxcalc.d:626-820 (efmflx with exxef) */
+#define E4HIP_FLUX_AUSM_PLUS_UP 3  /* fluxcalc.d:860-1037, with E4hipStepConfig.M_inf */
+#define E4HIP_FLUX_ADAPTIVE_EFM_AUSMDV 4  /* EFM at shock points with small shear, AUSMDV
+                                             elsewhere (fluxcalc.d:802-820); "adaptive" */
 #define E4HIP_FLUX_ADAPTIVE_HANEL_AUSMDV 5  /* the reference's default (globalconfig.d:533):
                                               Hanel at shock points with small shear,
                                               AUSMDV elsewhere (fluxcalc.d:822-836) */
-#define E4HIP_FLUX_HLLC        9
+#define E4HIP_FLUX_ROE         8   /* fluxcalc.d:1202-1371 (Roe-Pike, Kermani-Plett fix) */
+#define E4HIP_FLUX_HLLC        9   /* (no counterpart in the reference; Toro 2009 10.4) */
+/* adaptive_hlle_roe (6) and hlle (7, MHD only, fluxcalc.d:1197) are refused. */
 
 /* ---- GasdynamicUpdate (eilmer/fvcore.d:30-41) */
 #define E4HIP_UPDATE_EULER       0
@@ -129,6 +135,12 @@ typedef struct E4hipStepConfig {
                                         launches for many small blocks); any call that changes
                                         blocks, boundaries or the config drops the graph.
                                         Default 0 */
+    double M_inf;                    /* ausm_plus_up's floor on the reference Mach number
+                                        squared: default 0.01 (globalconfig.d:591) */
+    int autotune;                    /* 1: before the first step of a block shape (and of a
+                                        batched block set) under a new arithmetic, time the
+                                        stage kernel at candidate k-chunk heights and keep the
+                                        fastest (results do not depend on it).  Default 1 */
 } E4hipStepConfig;
 
 /* Library set-up on one GPU; creates the library's HIP stream.
@@ -293,6 +305,12 @@ int e4hip_determine_dt(const int* blks, int nblk, double dt_current, double cfl_
  * read, and resets them. */
 int e4hip_profile_enable(int on);
 int e4hip_profile_read(int k, double* total_ms, long* launches);
+
+/* The k-chunk height (planes per workgroup) the stage kernel of a uniform 3D
+ * ideal-gas block launches with on its own under the current configuration:
+ * the autotuner's choice once its shape was tuned, else the default.
+ * No counterpart in the reference (its loops have no tiling). */
+int e4hip_block_kchunk(int blk, int* kchunk);
 
 #ifdef __cplusplus
 }

@@ -32,7 +32,8 @@ enum { NSP_MAX = 8, NSEG_MAX = 4, NQ_MAX = 5 + NSP_MAX };
 enum { GAS_IDEAL = 0, GAS_TP = 1 };
 // FluxCalculator values follow the reference's enum order (eilmer/fvcore.d:184-194);
 // hllc is the new member appended after roe.
-enum { FLUX_AUSMDV = 0, FLUX_HANEL = 1, FLUX_ADAPTIVE_HANEL_AUSMDV = 5, FLUX_HLLC = 9 };
+enum { FLUX_AUSMDV = 0, FLUX_HANEL = 1, FLUX_EFM = 2, FLUX_AUSM_PLUS_UP = 3, FLUX_ADAPTIVE_EFM_AUSMDV = 4,
+       FLUX_ADAPTIVE_HANEL_AUSMDV = 5, FLUX_ROE = 8, FLUX_HLLC = 9 };
 // GasdynamicUpdate (eilmer/fvcore.d:30-41).
 enum { UPD_EULER = 0, UPD_PC = 1, UPD_MIDPOINT = 2, UPD_CLASSIC_RK3 = 3, UPD_TVD_RK3 = 4, UPD_DENMAN_RK3 = 5 };
 // InterpolateOption (eilmer/fvcore.d:155).
@@ -51,6 +52,9 @@ static const double R_universal = 8.31451;     // gas/physical_constants.d:13
 static const double T_MIN = 20.0;              // gas/gas_model.d:45
 static const double T_MAX = 100000.0;          // gas/gas_model.d:46
 static const double epsilon_van_albada = 1.0e-12; // eilmer/onedinterp.d:22
+// Blocks (and state arrays) below this many cells run their loops on one
+// thread: a thread team per loop costs more than such a loop's work.
+static const int OMP_MIN_CELLS = 32768;
 
 struct CEACurve {          // gas/thermo/cea_thermo_curves.d:22-169
     double R;
@@ -413,7 +417,7 @@ int ref_gas_update_many(Gas* g, int which, int n, double* q)
 {
     int bad = 0;
     const int w = 5 + g->nsp;
-    #pragma omp parallel for reduction(+:bad) schedule(static)
+    #pragma omp parallel for reduction(+:bad) schedule(static) if (n >= OMP_MIN_CELLS)
     for (int i = 0; i < n; ++i) {
         GasState Q; unpackQ(g, q + (size_t)i * w, Q);
         int rc = -9;
@@ -516,6 +520,7 @@ struct Config {
     int max_invalid_cells;
     double shear_tolerance;         // globalconfig.d:585
     double compression_tolerance;   // globalconfig.d:600
+    double M_inf = 0.01;            // ausm_plus_up's reference Mach number squared floor, globalconfig.d:591
 };
 
 // onedinterp.d:435-585 together with the EOS mixins of onedinterp.d:1039-1068.
@@ -924,20 +929,299 @@ static void adaptive_hanel_ausmdv(const Gas* g, const Config& cfg, const FlowSta
     else ausmdv(g, Lft, Rght, F);
 }
 
+static double gas_gamma(const Gas* g, const GasState& Q);
+static double gas_Cv(const Gas* g, const GasState& Q);
+
+// fluxcalc.d:792-820: exp(-x^2) and erf(x) by the reference's polynomial
+// approximation (Abramowitz and Stegun 7.1.26), clamped beyond |x| = 5.
+static void exxef(double sn, double& exx, double& ef)
+{
+    const double P = 0.327591100;
+    const double A1 = 0.254829592;
+    const double A2 = -0.284496736;
+    const double A3 = 1.421413741;
+    const double A4 = -1.453152027;
+    const double A5 = 1.061405429;
+    const double LIMIT = 5.0;
+    const double EXLIM = 0.138879e-10;
+    const double EFLIM = 1.0;
+    double ef1;
+    if (fabs(sn) > LIMIT) {
+        exx = EXLIM;
+        ef1 = EFLIM;
+    } else {
+        const double snsq = sn * sn;
+        exx = exp(-snsq);
+        const double y = 1.0 / (1.0 + P * fabs(sn));
+        ef1 = 1.0 - y * (A1 + y * (A2 + y * (A3 + y * (A4 + A5 * y)))) * exx;
+    }
+    ef = copysign(ef1, sn);
+}
+
+// fluxcalc.d:626-790: Macrossan and Pullin's equilibrium flux method.  The
+// gas "constants" come from the two states (Cv of the gas model, R = p/(rho T)),
+// weighted by sqrt(rho); the right state's RT is p/rho where the left one's is
+// R T, as the reference has it.  Species follow the sign of the mass flux.
+static void efm(const Gas* g, const FlowState& Lft, const FlowState& Rght, double* F)
+{
+    const double PHI = 1.0;
+    const double dtwspi = 0.282094792;
+    const double rhoL = Lft.gas.rho, presL = Lft.gas.p, eL = Lft.gas.u;
+    const double hL = eL + presL / rhoL;
+    const double tL = Lft.gas.T;
+    const double vnL = Lft.vel[0], vpL = Lft.vel[1], vqL = Lft.vel[2];
+    const double rhoR = Rght.gas.rho, presR = Rght.gas.p, eR = Rght.gas.u;
+    const double hR = eR + presR / rhoR;
+    const double tR = Rght.gas.T;
+    const double vnR = Rght.vel[0], vpR = Rght.vel[1], vqR = Rght.vel[2];
+    const double cvL = gas_Cv(g, Lft.gas);
+    const double RgasL = presL / (rhoL * tL);
+    const double cvR = gas_Cv(g, Rght.gas);
+    const double RgasR = presR / (rhoR * tR);
+    const double rLsqrt = sqrt(rhoL);
+    const double rRsqrt = sqrt(rhoR);
+    const double alpha = rLsqrt / (rLsqrt + rRsqrt);
+    const double cv = alpha * cvL + (1.0 - alpha) * cvR;
+    const double Rgas = alpha * RgasL + (1.0 - alpha) * RgasR;
+    const double cp = cv + Rgas;
+    const double gam = cp / cv;
+    const double con = 0.5 * (gam + 1.0) / (gam - 1.0);
+    const double rtL = Rgas * tL;
+    const double cmpL = sqrt(2.0 * rtL);
+    const double hvsqL = 0.5 * (vnL * vnL + vpL * vpL + vqL * vqL);
+    const double snL = vnL / (PHI * cmpL);
+    double exL, efL;
+    exxef(snL, exL, efL);
+    const double wL = 0.5 * (1.0 + efL);
+    const double dL = exL * dtwspi;
+    const double rtR = presR / rhoR;
+    const double cmpR = sqrt(2.0 * rtR);
+    const double hvsqR = 0.5 * (vnR * vnR + vpR * vpR + vqR * vqR);
+    const double snR = vnR / (PHI * cmpR);
+    double exR, efR;
+    exxef(snR, exR, efR);
+    const double wR = 0.5 * (1.0 - efR);
+    const double dR = -exR * dtwspi;
+    const double fmsL = (wL * rhoL * vnL) + (dL * cmpL * rhoL);
+    const double fmsR = (wR * rhoR * vnR) + (dR * cmpR * rhoR);
+    F[0] = fmsL + fmsR;
+    F[1] = fmsL * vnL + fmsR * vnR + wL * presL + wR * presR;
+    F[2] = fmsL * vpL + fmsR * vpR;
+    F[3] = fmsL * vqL + fmsR * vqR;
+    F[4] = (wL * rhoL * vnL) * (hvsqL + hL) + (wR * rhoR * vnR) * (hvsqR + hR) +
+           (dL * cmpL * rhoL) * (hvsqL + con * rtL) + (dR * cmpR * rhoR) * (hvsqR + con * rtR);
+    const int nsp = g->nsp;
+    if (nsp > 1) {
+        const FlowState& up = (F[0] > 0.0) ? Lft : Rght;
+        for (int i = 0; i < nsp; ++i) F[5 + i] = F[0] * up.gas.massf[i];
+    }
+}
+
+// fluxcalc.d:802-820: EFM across a shock (either side flagged) where the
+// tangential velocity jump is small, AUSMDV elsewhere.
+static void adaptive_efm_ausmdv(const Gas* g, const Config& cfg, const FlowState& Lft, const FlowState& Rght,
+                                double* F)
+{
+    const double sound_speed = 0.5 * (Lft.gas.a + Rght.gas.a);
+    const double shear_y = fabs(Lft.vel[1] - Rght.vel[1]) / sound_speed;
+    const double shear_z = fabs(Lft.vel[2] - Rght.vel[2]) / sound_speed;
+    const bool shear_is_small = fmax(shear_y, shear_z) <= cfg.shear_tolerance;
+    if ((Lft.S == 1 || Rght.S == 1) && shear_is_small) efm(g, Lft, Rght, F);
+    else ausmdv(g, Lft, Rght, F);
+}
+
+// fluxcalc.d:860-1037: Liou's AUSM+-up (2006) with the interface sound speed
+// the mean of the two sides' and the reference's M_inf floor on M0^2.
+static double aup_M1plus(double M) { return 0.5 * (M + fabs(M)); }
+static double aup_M1minus(double M) { return 0.5 * (M - fabs(M)); }
+static double aup_M2plus(double M) { return 0.25 * (M + 1.0) * (M + 1.0); }
+static double aup_M2minus(double M) { return -0.25 * (M - 1.0) * (M - 1.0); }
+static double aup_M4plus(double M, double beta)
+{
+    if (fabs(M) >= 1.0) return aup_M1plus(M);
+    const double M2p = aup_M2plus(M), M2m = aup_M2minus(M);
+    return M2p * (1.0 - 16.0 * beta * M2m);
+}
+static double aup_M4minus(double M, double beta)
+{
+    if (fabs(M) >= 1.0) return aup_M1minus(M);
+    const double M2p = aup_M2plus(M), M2m = aup_M2minus(M);
+    return M2m * (1.0 + 16.0 * beta * M2p);
+}
+static double aup_P5plus(double M, double alpha)
+{
+    if (fabs(M) >= 1.0) return (1.0 / M) * aup_M1plus(M);
+    const double M2p = aup_M2plus(M), M2m = aup_M2minus(M);
+    return M2p * ((2.0 - M) - 16.0 * alpha * M * M2m);
+}
+static double aup_P5minus(double M, double alpha)
+{
+    if (fabs(M) >= 1.0) return (1.0 / M) * aup_M1minus(M);
+    const double M2p = aup_M2plus(M), M2m = aup_M2minus(M);
+    return M2m * ((-2.0 - M) + 16.0 * alpha * M * M2p);
+}
+
+static void ausm_plus_up(const Gas* g, const FlowState& Lft, const FlowState& Rght, double M_inf, double* F)
+{
+    const double rL = Lft.gas.rho, pL = Lft.gas.p;
+    const double uL = Lft.vel[0], vL = Lft.vel[1], wL = Lft.vel[2];
+    const double eL = Lft.gas.u, aL = Lft.gas.a;
+    const double keL = 0.5 * (uL * uL + vL * vL + wL * wL);
+    const double HL = eL + pL / rL + keL;
+    const double rR = Rght.gas.rho, pR = Rght.gas.p;
+    const double uR = Rght.vel[0], vR = Rght.vel[1], wR = Rght.vel[2];
+    const double eR = Rght.gas.u, aR = Rght.gas.a;
+    const double keR = 0.5 * (uR * uR + vR * vR + wR * wR);
+    const double HR = eR + pR / rR + keR;
+    const double a_half = 0.5 * (aR + aL);
+    const double ML = uL / a_half;
+    const double MR = uR / a_half;
+    const double MbarSq = (uL * uL + uR * uR) / (2.0 * a_half * a_half);
+    const double M0Sq = fmin(1.0, fmax(MbarSq, M_inf));
+    const double fa = sqrt(M0Sq) * (2.0 - sqrt(M0Sq));
+    const double alpha = 0.1875 * (-4.0 + 5 * fa * fa);
+    const double beta = 0.125;
+    const double M4plus_ML = aup_M4plus(ML, beta);
+    const double P5plus_ML = aup_P5plus(ML, alpha);
+    const double M4minus_MR = aup_M4minus(MR, beta);
+    const double P5minus_MR = aup_P5minus(MR, alpha);
+    const double KP = 0.25;
+    const double KU = 0.75;
+    const double SIGMA = 1.0;
+    const double r_half = 0.5 * (rL + rR);
+    const double Mp = -KP / fa * fmax((1.0 - SIGMA * MbarSq), 0.0) * (pR - pL) / (r_half * a_half * a_half);
+    const double Pu = -KU * P5plus_ML * P5minus_MR * (rL + rR) * fa * a_half * (uR - uL);
+    const double M_half = M4plus_ML + M4minus_MR + Mp;
+    double ru_half = a_half * M_half;
+    if (M_half > 0.0) ru_half *= rL;
+    else ru_half *= rR;
+    const double p_half = P5plus_ML * pL + P5minus_MR * pR + Pu;
+    const double ru2_half = (ru_half >= 0.0) ? ru_half * uL : ru_half * uR;
+    const int nsp = g->nsp;
+    F[0] = ru_half;
+    if (ru_half >= 0.0) {
+        F[1] = ru2_half + p_half; F[2] = ru_half * vL; F[3] = ru_half * wL;
+        F[4] = ru_half * HL;
+        if (nsp > 1) for (int i = 0; i < nsp; ++i) F[5 + i] = ru_half * Lft.gas.massf[i];
+    } else {
+        F[1] = ru2_half + p_half; F[2] = ru_half * vR; F[3] = ru_half * wR;
+        F[4] = ru_half * HR;
+        if (nsp > 1) for (int i = 0; i < nsp; ++i) F[5 + i] = ru_half * Rght.gas.massf[i];
+    }
+}
+
+// fluxcalc.d:1202-1371: Roe-Pike (Toro 2009, section 11.3) with the
+// Kermani-Plett entropy fix on the acoustic eigenvalues; gamma of each side
+// from the gas model (Cp/Cv), averaged.  The sums run over all five waves
+// in the reference's order, zero eigenvector entries included.
+static void roe(const Gas* g, const FlowState& Lft, const FlowState& Rght, double* F)
+{
+    const double rL = Lft.gas.rho, pL = Lft.gas.p;
+    const double pLrL = pL / rL;
+    const double uL = Lft.vel[0], vL = Lft.vel[1], wL = Lft.vel[2];
+    const double eL = Lft.gas.u, aL = Lft.gas.a;
+    const double keL = 0.5 * (uL * uL + vL * vL + wL * wL);
+    const double HL = eL + pLrL + keL;
+    const double rR = Rght.gas.rho, pR = Rght.gas.p;
+    const double pRrR = pR / rR;
+    const double uR = Rght.vel[0], vR = Rght.vel[1], wR = Rght.vel[2];
+    const double eR = Rght.gas.u, aR = Rght.gas.a;
+    const double keR = 0.5 * (uR * uR + vR * vR + wR * wR);
+    const double HR = eR + pRrR + keR;
+    const double gL = gas_gamma(g, Lft.gas);
+    const double gR = gas_gamma(g, Rght.gas);
+    const double ghat = (gR + gL) / 2.0;
+    const double rhat = sqrt(rL * rR);
+    const double uhat = (sqrt(rL) * uL + sqrt(rR) * uR) / (sqrt(rL) + sqrt(rR));
+    const double vhat = (sqrt(rL) * vL + sqrt(rR) * vR) / (sqrt(rL) + sqrt(rR));
+    const double what = (sqrt(rL) * wL + sqrt(rR) * wR) / (sqrt(rL) + sqrt(rR));
+    const double Hhat = (sqrt(rL) * HL + sqrt(rR) * HR) / (sqrt(rL) + sqrt(rR));
+    const double ahat2 = (ghat - 1.0) * (Hhat - 0.5 * (uhat * uhat + vhat * vhat + what * what));
+    const double ahat = sqrt(ahat2);
+    double lambda[5] = {uhat - ahat, uhat, uhat, uhat, uhat + ahat};
+    const double lambdaL = uL - aL;
+    const double lambdaR = uR - aR;
+    const double eps = 1.0e-12;
+    const double EPS = 4.0 * fmax(0.0, (lambdaR - lambdaL));
+    if (fabs(lambda[0]) < EPS) lambda[0] = (lambda[0] * lambda[0] + EPS * EPS) / (2.0 * EPS + eps);
+    if (fabs(lambda[4]) < EPS) lambda[4] = (lambda[4] * lambda[4] + EPS * EPS) / (2.0 * EPS + eps);
+    const double K[5][5] = {
+        {1.0, uhat - ahat, vhat, what, Hhat - uhat * ahat},
+        {1.0, uhat, vhat, what, 0.5 * (uhat * uhat + vhat * vhat + what * what)},
+        {0.0, 0.0, 1.0, 0.0, vhat},
+        {0.0, 0.0, 0.0, 1.0, what},
+        {1.0, uhat + ahat, vhat, what, Hhat + uhat * ahat}};
+    double alpha[5];
+    alpha[0] = 1.0 / (2.0 * ahat2) * ((pR - pL) - rhat * ahat * (uR - uL));
+    alpha[1] = (rR - rL) - (pR - pL) / ahat2;
+    alpha[2] = rhat * (vR - vL);
+    alpha[3] = rhat * (wR - wL);
+    alpha[4] = 1.0 / (2.0 * ahat2) * ((pR - pL) + rhat * ahat * (uR - uL));
+    double FL, FR;
+    FL = rL * uL;
+    FR = rR * uR;
+    F[0] = 0.5 * (FL + FR);
+    for (int i = 0; i < 5; ++i) F[0] -= 0.5 * alpha[i] * fabs(lambda[i]) * K[i][0];
+    FL = pL + rL * uL * uL;
+    FR = pR + rR * uR * uR;
+    F[1] = 0.5 * (FL + FR);
+    for (int i = 0; i < 5; ++i) F[1] -= 0.5 * alpha[i] * fabs(lambda[i]) * K[i][1];
+    FL = rL * uL * vL;
+    FR = rR * uR * vR;
+    F[2] = 0.5 * (FL + FR);
+    for (int i = 0; i < 5; ++i) F[2] -= 0.5 * alpha[i] * fabs(lambda[i]) * K[i][2];
+    FL = rL * uL * wL;
+    FR = rR * uR * wR;
+    F[3] = 0.5 * (FL + FR);
+    for (int i = 0; i < 5; ++i) F[3] -= 0.5 * alpha[i] * fabs(lambda[i]) * K[i][3];
+    FL = (rL * eL + rL * (uL * uL + vL * vL + wL * wL) / 2.0 + pL) * uL;
+    FR = (rR * eR + rR * (uR * uR + vR * vR + wR * wR) / 2.0 + pR) * uR;
+    F[4] = 0.5 * (FL + FR);
+    for (int i = 0; i < 5; ++i) F[4] -= 0.5 * alpha[i] * fabs(lambda[i]) * K[i][4];
+    const int nsp = g->nsp;
+    if (nsp > 1) {
+        const FlowState& up = (F[0] >= 0.0) ? Lft : Rght;
+        for (int i = 0; i < nsp; ++i) F[5 + i] = F[0] * up.gas.massf[i];
+    }
+}
+
+// The flux calculator of cfg in the face frame (fluxcalc.d:61-89).
+static void flux_in_face_frame(const Gas* g, const Config& cfg, const FlowState& Lft, const FlowState& Rght,
+                               double* F)
+{
+    switch (cfg.flux_calc) {
+    case FLUX_HLLC: hllc(g, Lft, Rght, F); break;
+    case FLUX_HANEL: hanel(g, Lft, Rght, F); break;
+    case FLUX_EFM: efm(g, Lft, Rght, F); break;
+    case FLUX_AUSM_PLUS_UP: ausm_plus_up(g, Lft, Rght, cfg.M_inf, F); break;
+    case FLUX_ADAPTIVE_EFM_AUSMDV: adaptive_efm_ausmdv(g, cfg, Lft, Rght, F); break;
+    case FLUX_ADAPTIVE_HANEL_AUSMDV: adaptive_hanel_ausmdv(g, cfg, Lft, Rght, F); break;
+    case FLUX_ROE: roe(g, Lft, Rght, F); break;
+    default: ausmdv(g, Lft, Rght, F); break;
+    }
+}
+
 // fluxcalc.d:26-128 on a fixed grid (gvel = 0, omegaz = 0).
 static void compute_interface_flux(const Gas* g, const Config& cfg, FlowState Lft, FlowState Rght,
                                    const Frame& fr, double* F)
 {
     to_local(Lft.vel, fr);
     to_local(Rght.vel, fr);
-    if (cfg.flux_calc == FLUX_HLLC) hllc(g, Lft, Rght, F);
-    else if (cfg.flux_calc == FLUX_HANEL) hanel(g, Lft, Rght, F);
-    else if (cfg.flux_calc == FLUX_ADAPTIVE_HANEL_AUSMDV) adaptive_hanel_ausmdv(g, cfg, Lft, Rght, F);
-    else ausmdv(g, Lft, Rght, F);
+    flux_in_face_frame(g, cfg, Lft, Rght, F);
     // The grid-velocity terms of fluxcalc.d:112-121 add exact zeros on a fixed grid.
     double m[3] = {F[1], F[2], F[3]};
     to_global(m, fr);
     F[1] = m[0]; F[2] = m[1]; F[3] = m[2];
+}
+
+// gas_model.d:159 with ideal_gas.d:160-163, therm_perf_gas.d:366-371: Cv of the
+// state (ideal gas: the constant _Cv; mixture: sum Y_i (Cp_i(T) - R_i))
+static double gas_Cv(const Gas* g, const GasState& Q)
+{
+    if (g->type == GAS_IDEAL) return g->Cv;
+    double Cv[NSP_MAX];
+    for (int i = 0; i < g->nsp; ++i) Cv[i] = eval_Cp(g->curve[i], Q.T) - g->R[i];
+    return mass_average(g, Q, Cv);
 }
 
 // gas_model.d:162: gamma = Cp/Cv of the state (ideal gas: _Cp/_Cv)
@@ -1036,9 +1320,30 @@ int ref_flux(Gas* g, int calc, const double* L, const double* R, double* F)
     };
     unpack(L, l); unpack(R, r);
     l.S = r.S = 0;
-    if (calc == FLUX_HLLC) hllc(g, l, r, F);
-    else if (calc == FLUX_HANEL) hanel(g, l, r, F);
-    else ausmdv(g, l, r, F);
+    Config cfg{};
+    cfg.flux_calc = calc; cfg.shear_tolerance = 0.20; cfg.M_inf = 0.01;
+    flux_in_face_frame(g, cfg, l, r, F);
+    return 0;
+}
+
+// exxef (fluxcalc.d:792-820) for tests.
+void ref_exxef(double sn, double* exx, double* ef) { exxef(sn, *exx, *ef); }
+
+// The same with the sides' shock indicators and the calculators' options.
+int ref_flux_opt(Gas* g, int calc, const double* L, const double* R, int SL, int SR, double shear_tolerance,
+                 double M_inf, double* F)
+{
+    FlowState l, r;
+    auto unpack = [&](const double* s, FlowState& f) {
+        f.gas.rho = s[0]; f.gas.p = s[1]; f.gas.T = s[2]; f.gas.u = s[3]; f.gas.a = s[4];
+        f.vel[0] = s[5]; f.vel[1] = s[6]; f.vel[2] = s[7];
+        for (int i = 0; i < g->nsp; ++i) f.gas.massf[i] = s[8 + i];
+    };
+    unpack(L, l); unpack(R, r);
+    l.S = SL; r.S = SR;
+    Config cfg{};
+    cfg.flux_calc = calc; cfg.shear_tolerance = shear_tolerance; cfg.M_inf = M_inf;
+    flux_in_face_frame(g, cfg, l, r, F);
     return 0;
 }
 
@@ -1104,7 +1409,7 @@ static void par_fill(BigVec<T>& v, size_t n, const T& value)
 {
     v.resize(n);
     T* d = v.data();
-    #pragma omp parallel for schedule(static)
+    #pragma omp parallel for schedule(static) if (n >= 1024 * OMP_MIN_CELLS)
     for (long long i = 0; i < (long long)n; ++i) d[i] = value;
 }
 }   // extern "C++"
@@ -1499,6 +1804,8 @@ void ref_block_set_shock_tolerances(Block* b, double shear_tolerance, double com
     b->cfg.compression_tolerance = compression_tolerance;
 }
 
+void ref_block_set_M_inf(Block* b, double M_inf) { b->cfg.M_inf = M_inf; }
+
 void ref_block_set_config(Block* b, int flux_calc, int update_scheme, int interp_order, int apply_limiter,
                           int extrema_clipping, int thermo_interp, int max_invalid_cells)
 {
@@ -1642,7 +1949,7 @@ int ref_block_set_U(Block* b, const double* U, const double* Tguess)
 {
     const size_t nc = (size_t)b->ni * b->nj * b->nk;
     int bad = 0;
-    #pragma omp parallel for collapse(2) schedule(static) reduction(+:bad)
+    #pragma omp parallel for collapse(2) schedule(static) reduction(+:bad) if (b->ni * b->nj * b->nk >= OMP_MIN_CELLS)
     for (int k = b->kmin; k <= b->kmax; ++k)
         for (int j = b->jmin; j <= b->jmax; ++j) {
             size_t c = ((size_t)(k - b->kmin) * b->nj + (j - b->jmin)) * b->ni;
@@ -1660,7 +1967,7 @@ int ref_block_set_U(Block* b, const double* U, const double* Tguess)
 void ref_block_get_U(const Block* b, int ftl, double* U)
 {
     const size_t nc = (size_t)b->ni * b->nj * b->nk;
-    #pragma omp parallel for collapse(2) schedule(static)
+    #pragma omp parallel for collapse(2) schedule(static) if (b->ni * b->nj * b->nk >= OMP_MIN_CELLS)
     for (int k = b->kmin; k <= b->kmax; ++k)
         for (int j = b->jmin; j <= b->jmax; ++j) {
             size_t c = ((size_t)(k - b->kmin) * b->nj + (j - b->jmin)) * b->ni;
@@ -1684,7 +1991,7 @@ void ref_block_get_S(const Block* b, int* out)
 void ref_block_get_fs(const Block* b, double* out)
 {
     const size_t nc = (size_t)b->ni * b->nj * b->nk;
-    #pragma omp parallel for collapse(2) schedule(static)
+    #pragma omp parallel for collapse(2) schedule(static) if (b->ni * b->nj * b->nk >= OMP_MIN_CELLS)
     for (int k = b->kmin; k <= b->kmax; ++k)
         for (int j = b->jmin; j <= b->jmax; ++j) {
             size_t c = ((size_t)(k - b->kmin) * b->nj + (j - b->jmin)) * b->ni;
@@ -2002,7 +2309,7 @@ static void convective_flux(Block* b)
         const bool bf_p = b->bc[PLUS_FACE[dir]].type == BC_OUTFLOW_SIMPLE_FLUX;
         const int lo = (dir == 0) ? b->imin : (dir == 1) ? b->jmin : b->kmin;
         const int hi = (dir == 0) ? b->imax : (dir == 1) ? b->jmax : b->kmax;
-        #pragma omp parallel for collapse(2) schedule(static)
+        #pragma omp parallel for collapse(2) schedule(static) if (b->ni * b->nj * b->nk >= OMP_MIN_CELLS)
         for (int k = b->kmin; k <= khi; ++k)
             for (int j = b->jmin; j <= jhi; ++j)
                 for (int i = b->imin; i <= ihi; ++i) {
@@ -2217,7 +2524,7 @@ static void block_stage(Block* b, int stage, double dt, int& decode_fail_out, in
     convective_flux(b);
     int decode_fail = 0, invalid = 0;
     {
-        #pragma omp parallel for collapse(2) schedule(static) reduction(+:decode_fail, invalid)
+        #pragma omp parallel for collapse(2) schedule(static) reduction(+:decode_fail, invalid) if (b->ni * b->nj * b->nk >= OMP_MIN_CELLS)
         for (int k = b->kmin; k <= b->kmax; ++k)
             for (int j = b->jmin; j <= b->jmax; ++j)
                 for (int i = b->imin; i <= b->imax; ++i) {
@@ -2248,7 +2555,8 @@ int ref_multi_step(Block** blks, int nblk, double dt, int* nbad)
 {
     const int ns = nstages(blks[0]->cfg.update_scheme);
     // the adaptive flux calculators switch the shock detector on (globalconfig.d:1942-1954)
-    const bool detect = blks[0]->cfg.flux_calc == FLUX_ADAPTIVE_HANEL_AUSMDV;
+    const bool detect = blks[0]->cfg.flux_calc == FLUX_ADAPTIVE_HANEL_AUSMDV ||
+                        blks[0]->cfg.flux_calc == FLUX_ADAPTIVE_EFM_AUSMDV;
     for (int stage = 1; stage <= ns; ++stage) {
         for (int n = 0; n < nblk; ++n) apply_bcs(blks[n]);
         // simcore.d:2352-2358: after the ghost-cell fill, before the fluxes

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libe4ref.so")
 DATA = os.path.join(os.path.dirname(HERE), "eilmer4_amd", "data")
 
 # enum values shared with the product ABI (reference enums, eilmer/fvcore.d)
-AUSMDV, HANEL, ADAPTIVE_HANEL_AUSMDV, HLLC = 0, 1, 5, 9
+AUSMDV, HANEL, EFM, AUSM_PLUS_UP, ADAPTIVE_EFM_AUSMDV, ADAPTIVE_HANEL_AUSMDV, ROE, HLLC = 0, 1, 2, 3, 4, 5, 8, 9
 EULER, PC, MIDPOINT, CLASSIC_RK3, TVD_RK3, DENMAN_RK3 = range(6)
 INTERP_PT, INTERP_RHOU, INTERP_RHOP, INTERP_RHOT = range(4)
 BC_SLIP_WALL, BC_INFLOW, BC_EXTRAPOLATE, BC_PERIODIC, BC_CONNECTED = range(5)
@@ -63,6 +63,9 @@ def lib():
         L.ref_block_free.argtypes = [vp]
         L.ref_block_set_config.argtypes = [vp] + [C.c_int] * 7
         L.ref_block_set_shock_tolerances.argtypes = [vp, C.c_double, C.c_double]
+        L.ref_block_set_M_inf.argtypes = [vp, C.c_double]
+        L.ref_exxef.argtypes = [C.c_double, _dp, _dp]
+        L.ref_flux_opt.argtypes = [vp, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, _dp]
         L.ref_block_get_S.argtypes = [vp, C.POINTER(C.c_int)]
         L.ref_block_set_bc.argtypes = [vp, C.c_int, C.c_int, _dp]
         L.ref_block_set_bc.restype = C.c_int
@@ -207,6 +210,22 @@ def flux(gas, calc, L, R):
     return F
 
 
+def exxef(x):
+    """(exp(-x^2), erf(x)) of the reference's polynomial approximation."""
+    ex, ef = C.c_double(), C.c_double()
+    lib().ref_exxef(float(x), C.byref(ex), C.byref(ef))
+    return ex.value, ef.value
+
+
+def flux_opt(gas, calc, L, R, SL=0, SR=0, shear_tolerance=0.20, M_inf=0.01):
+    """flux() with the sides' shock indicators and the calculators' options."""
+    nq = 5 + (gas.nsp if gas.nsp > 1 else 0)
+    F = np.zeros(nq)
+    lib().ref_flux_opt(gas.h, calc, _p(_f64(L)), _p(_f64(R)), int(SL), int(SR), float(shear_tolerance),
+                       float(M_inf), _p(F))
+    return F
+
+
 class Block:
     """One structured block of the oracle (uniform Cartesian cells)."""
 
@@ -278,6 +297,9 @@ class Block:
 
     def set_shock_tolerances(self, shear_tolerance=0.20, compression_tolerance=-0.30):
         lib().ref_block_set_shock_tolerances(self.h, shear_tolerance, compression_tolerance)
+
+    def set_M_inf(self, M_inf=0.01):
+        lib().ref_block_set_M_inf(self.h, float(M_inf))
 
     def get_S(self):
         out = np.zeros(self.ncells, dtype=np.int32)

@@ -53,9 +53,14 @@ def test_config_defaults_and_device_flux_check():
     assert cfg.interpolation_order == 2 and cfg.apply_limiter and cfg.extrema_clipping
     assert cfg.cfl_value == 0.5 and cfg.cfl_count == 10 and cfg.dt_init == 1e-3 and cfg.dt_max == 1e-3
     assert cfg.gasdynamic_update_scheme == e4.GasdynamicUpdate.pc
-    bad = e4.GlobalConfig(flux_calculator=e4.FluxCalculator.roe)
-    with pytest.raises(e4.FlowSolverException):
-        bad.step_config()
+    assert cfg.M_inf == 0.01 and cfg.shear_tolerance == 0.20     # globalconfig.d:585, 591
+    # hlle is MHD only (fluxcalc.d:1197); adaptive_hlle_roe takes it near shocks
+    for fc in (e4.FluxCalculator.hlle, e4.FluxCalculator.adaptive_hlle_roe):
+        with pytest.raises(e4.FlowSolverException):
+            e4.GlobalConfig(flux_calculator=fc).step_config()
+    for fc in (e4.FluxCalculator.efm, e4.FluxCalculator.ausm_plus_up, e4.FluxCalculator.adaptive_efm_ausmdv,
+               e4.FluxCalculator.roe):
+        assert e4.GlobalConfig(flux_calculator=fc).step_config().flux_calculator == int(fc)
 
 
 def test_library_missing_fails_loudly(tmp_path):

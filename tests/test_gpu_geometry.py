@@ -156,8 +156,16 @@ def test_connected_curvilinear_blocks_exact(gpu, orient_case):
         assert O.multi_step(oblocks, dt)[0] == 0
         sim.state.dt_global = dt
         sim.gasdynamic_explicit_increment_with_fixed_grid()
-    for ob, db in zip(oblocks, dblocks):
-        assert np.array_equal(ob.get_U(), db.download())
+    for n, (ob, db) in enumerate(zip(oblocks, dblocks)):
+        Uo, Ud = ob.get_U(), db.download()
+        if not np.array_equal(Uo, Ud):
+            d = np.abs(Ud - Uo)
+            q, c = np.unravel_index(np.argmax(d / np.abs(Uo).max(axis=1, keepdims=True)), d.shape)
+            ni, nj = db.ni, db.nj
+            print("block %d: rel err %s; largest at q=%d cell (i,j,k)=(%d,%d,%d), %d cells differ"
+                  % (n, rel_err(Ud, Uo), q, c % ni, (c // ni) % nj, c // (ni * nj),
+                     int(np.any(d > 0, axis=0).sum())), flush=True)
+        assert np.array_equal(Uo, Ud)
     for db in dblocks:
         db.destroy()
 
