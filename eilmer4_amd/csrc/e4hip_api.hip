@@ -160,6 +160,45 @@ struct State {
 
 State g_state;
 
+// ---- scratch of e4hip_determine_dt_noalloc, the entry point that allocates
+// nothing: per block its largest CFL signal (bit pattern, atomic max) and its
+// last cell's signal, then the reduced dt_allow and cfl_max.  One array per
+// GPU, held until the process ends; its device address is taken once per
+// device at e4hip_init (host code never passes the array's name to a kernel).
+constexpr int NOALLOC_WORDS = 2 * E4HIP_NOALLOC_MAX_BLOCKS + 2;
+static __device__ __align__(256) unsigned long long scratch[NOALLOC_WORDS];
+constexpr int MAX_DEVICES = 64;
+unsigned long long* scratch_dev[MAX_DEVICES] = {};
+
+__global__ void noalloc_clear_kernel(unsigned long long* s, int n)
+{
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) s[t] = 0ull;
+}
+
+// The blocks' (dt_allow, cfl_max) from their slots, reduced in block order as
+// e4hip_determine_dt does on the host (block_dt, fluidblock.d:776-862,
+// simcore.d:1385-1394), into out[0], out[1].
+__global__ void noalloc_dt_reduce_kernel(const unsigned long long* s, int nblk, double dt_current, double cfl_value,
+                                         int check_cfl, double cfl_allow, double* out)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double dt_allow_all = 1.7976931348623157e308, cfl_max_all = 0.0;
+    for (int n = 0; n < nblk; ++n) {
+        const double smax = __longlong_as_double((long long)s[2 * n]);
+        const double last = __longlong_as_double((long long)s[2 * n + 1]);
+        double dt_allow = cfl_value / smax;
+        double cfl_max = dt_current * smax;
+        if (check_cfl && (cfl_max < 0.0 || cfl_max > cfl_allow)) {
+            cfl_max = 0.5 * cfl_allow;
+            dt_allow = cfl_max / last;
+        }
+        dt_allow_all = fmin(dt_allow_all, dt_allow);
+        cfl_max_all = fmax(cfl_max_all, cfl_max);
+    }
+    out[0] = dt_allow_all;
+    out[1] = cfl_max_all;
+}
+
 bool BlockH::is_tp() const { return g_state.gases[gm].type == GAS_TP; }
 
 void graphs_drop()
@@ -1014,6 +1053,12 @@ int e4hip_init(int device)
 {
     if (g_state.init) return E4HIP_OK;
     HIP_TRY(hipSetDevice(device));
+    if (device < 0 || device >= MAX_DEVICES) return fail(E4HIP_ERR_ARG, "device %d out of range", device);
+    if (!scratch_dev[device]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        scratch_dev[device] = (unsigned long long*)p;
+    }
     HIP_TRY(hipStreamCreateWithFlags(&g_state.stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc(&g_state.smax, sizeof(unsigned long long)));
     HIP_TRY(hipMalloc(&g_state.last_signal, sizeof(double)));
@@ -1986,13 +2031,13 @@ namespace {
 // The CFL-signal kernel of one block on the library stream: the block's
 // largest signal as a bit pattern into *smax_bits (atomic max; the caller
 // clears it first) and the last cell's signal into *last_signal.
-int enqueue_cfl(const BlockH& b, unsigned long long* smax_bits, double* last_signal)
+int enqueue_cfl(const BlockH& b, unsigned long long* smax_bits, double* last_signal, bool profile = true)
 {
     const int stringent = g_state.cfg.stringent_cfl != 0;
     long nwg = (b.ncells + 255) / 256;
     if (nwg > 4096) nwg = 4096;
     hipEvent_t ev = nullptr;
-    prof_begin(&ev);
+    if (profile) prof_begin(&ev);
     if (b.general) {
         e4::TPArgs A = ::tp_args_for(b);
         A.Pin = b.P[0];
@@ -2011,9 +2056,19 @@ int enqueue_cfl(const BlockH& b, unsigned long long* smax_bits, double* last_sig
         C.smax_bits = smax_bits; C.last_signal = last_signal;
         hipLaunchKernelGGL(e4::cfl_kernel<e4::ModeExact>, dim3((unsigned)nwg), dim3(256), 0, g_state.stream, C);
     }
-    prof_end(1, ev);
+    if (profile) prof_end(1, ev);
     HIP_TRY(hipGetLastError());
     return E4HIP_OK;
+}
+
+// cfl_allow of the update scheme (fluidblock.d:798-803)
+double cfl_allow_of_scheme()
+{
+    switch (nstages(g_state.cfg.gasdynamic_update_scheme)) {
+    case 1: return 0.9;
+    case 2: return 1.2;
+    default: return 1.6;
+    }
 }
 
 // A block's dt_allow and cfl_max from its largest signal and its last cell's
@@ -2024,8 +2079,7 @@ int enqueue_cfl(const BlockH& b, unsigned long long* smax_bits, double* last_sig
 void block_dt(unsigned long long smax_bits, double last, double dt_current, double cfl_value, int check_cfl,
               double& dt_allow, double& cfl_max)
 {
-    double cfl_allow = 0.9;   // fluidblock.d:798-803
-    switch (nstages(g_state.cfg.gasdynamic_update_scheme)) { case 1: cfl_allow = 0.9; break; case 2: cfl_allow = 1.2; break; default: cfl_allow = 1.6; }
+    const double cfl_allow = cfl_allow_of_scheme();
     double smax;
     memcpy(&smax, &smax_bits, sizeof(smax));
     dt_allow = cfl_value / smax;
@@ -2065,6 +2119,37 @@ int e4hip_determine_dt(const int* blks, int nblk, double dt_current, double cfl_
     }
     *dt_allow_out = dt_allow_all;
     *cfl_max_out = cfl_max_all;
+    return E4HIP_OK;
+}
+
+int e4hip_determine_dt_noalloc(const int* blks, int nblk, double dt_current, double cfl_value, int check_cfl,
+                               double* dt_allow_out, double* cfl_max_out)
+{
+    if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
+    if (!blks || nblk <= 0 || !dt_allow_out || !cfl_max_out) return fail(E4HIP_ERR_ARG, "bad arguments");
+    if (nblk > E4HIP_NOALLOC_MAX_BLOCKS)
+        return fail(E4HIP_ERR_ARG, "e4hip_determine_dt_noalloc takes at most %d blocks (%d given)",
+                    E4HIP_NOALLOC_MAX_BLOCKS, nblk);
+    unsigned long long* sp = scratch_dev[g_state.device];
+    if (!sp) return fail(E4HIP_ERR_NOT_INIT, "no scratch address for device %d", g_state.device);
+    for (int n = 0; n < nblk; ++n)
+        if (!get_block(blks[n])) return fail(E4HIP_ERR_HANDLE, "unknown block %d", blks[n]);
+    const int nw = 2 * nblk + 2;
+    hipLaunchKernelGGL(noalloc_clear_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, g_state.stream, sp, nw);
+    HIP_TRY(hipGetLastError());
+    for (int n = 0; n < nblk; ++n) {
+        int rc = enqueue_cfl(*get_block(blks[n]), sp + 2 * n, (double*)(sp + 2 * n + 1), false);
+        if (rc) return rc;
+    }
+    double* res = (double*)(sp + 2 * nblk);
+    hipLaunchKernelGGL(noalloc_dt_reduce_kernel, dim3(1), dim3(64), 0, g_state.stream, sp, nblk, dt_current,
+                       cfl_value, check_cfl, cfl_allow_of_scheme(), res);
+    HIP_TRY(hipGetLastError());
+    double out[2];
+    HIP_TRY(hipMemcpyAsync(out, res, sizeof(out), hipMemcpyDeviceToHost, g_state.stream));
+    HIP_TRY(hipStreamSynchronize(g_state.stream));
+    *dt_allow_out = out[0];
+    *cfl_max_out = out[1];
     return E4HIP_OK;
 }
 

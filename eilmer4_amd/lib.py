@@ -23,7 +23,7 @@ EXPORTS = [
     "e4hip_block_upload", "e4hip_block_download", "e4hip_block_ncells", "e4hip_config",
     "e4hip_explicit_increment", "e4hip_determine_dt", "e4hip_profile_enable", "e4hip_profile_read",
     "e4hip_gas_tp_set_guess_model", "e4hip_tp_guess", "e4hip_tp_decode_stats", "e4hip_tp_face_stats",
-    "e4hip_mlp_grad", "e4hip_mlp_train", "e4hip_block_kchunk",
+    "e4hip_mlp_grad", "e4hip_mlp_train", "e4hip_block_kchunk", "e4hip_determine_dt_noalloc",
 ]
 
 
@@ -78,6 +78,7 @@ def load(path=LIB_PATH):
     L.e4hip_config.argtypes = [C.POINTER(StepConfig)]
     L.e4hip_explicit_increment.argtypes = [_ip, C.c_int, C.c_double, _ip]
     L.e4hip_determine_dt.argtypes = [_ip, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
+    L.e4hip_determine_dt_noalloc.argtypes = [_ip, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
     L.e4hip_profile_enable.argtypes = [C.c_int]
     L.e4hip_profile_read.argtypes = [C.c_int, _dp, C.POINTER(C.c_long)]
     _fp = C.POINTER(C.c_float)

@@ -299,6 +299,19 @@ int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_ce
 int e4hip_determine_dt(const int* blks, int nblk, double dt_current, double cfl_value,
                        int check_cfl, double* dt_allow, double* cfl_max);
 
+/* e4hip_determine_dt without device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same inputs, outputs
+ * and results as e4hip_determine_dt (bit for bit); its per-block signals and
+ * the reduction live in a fixed device array of the library (one per GPU, held
+ * until the process ends), and the whole call synchronises once.  At most
+ * E4HIP_NOALLOC_MAX_BLOCKS blocks (E4HIP_ERR_ARG beyond).  Calls that overlap
+ * on one GPU (several host threads, or with each other's work still queued on
+ * another stream) race on that array. */
+#define E4HIP_NOALLOC_MAX_BLOCKS 4096
+int e4hip_determine_dt_noalloc(const int* blks, int nblk, double dt_current, double cfl_value,
+                               int check_cfl, double* dt_allow, double* cfl_max);
+
 /* Per-kernel timing with HIP events on the library stream (off by default).
  * e4hip_profile_read returns, for kernel class k (0 = stage kernels,
  * 1 = CFL kernel), the summed milliseconds and launch count since the last
