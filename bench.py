@@ -137,14 +137,19 @@ def tp_workload(e4, workloads, L, n=128, steps=5, warmup=1):
     return out
 
 
-def c5_workload(e4, workloads, steps=3, warmup=1):
+def c5_workload(e4, workloads, steps=6, warmup=3):
     """The batched multi-block workload (48 blocks: 32 ideal air, 16
     five-species air, random face-matched connections in every orientation
     within each gas class).  Variants, each from the same state at one fixed dt:
       plain                  FP64, reference Newton, one launch per block
-      fp64+batched           FP64, the ideal blocks of a stage in one launch
-      mixed+learned+batched  mixed precision, learned decode guess, batched
-    with each variant's speed-up over plain and its difference from plain."""
+      fp64+batched           FP64, each gas class's blocks of a stage in one
+                             launch per kernel (the ideal blocks' stage kernel
+                             persistent over a tile queue), steps replayed
+                             from captured HIP graphs
+      mixed+learned+batched  mixed precision, learned decode guess, batched,
+                             graphs
+    with each variant's speed-up over plain and its difference from plain.
+    The warm-up steps include the k-chunk tuning and the graph captures."""
     import torch
     from eilmer4_amd import learned_eos
     ig, tg = e4.IdealGas(), e4.ThermallyPerfectGas.from_json()
@@ -169,7 +174,7 @@ def c5_workload(e4, workloads, steps=3, warmup=1):
             e4.connect_blocks(blks[a], fa, blks[b], fb, o)
         sim = e4.Simulation(blks, e4.GlobalConfig(flux_calculator=e4.FluxCalculator.ausmdv,
                                                   precision_mode=e4.PrecisionMode[prec], batch_blocks=batch,
-                                                  learned_eos_guess=learned))
+                                                  use_graphs=batch, learned_eos_guess=learned))
         for b, d in zip(blks, descs):
             b.upload(d["U0"], d["Tguess"])
         if dt_fixed is None:
@@ -383,6 +388,8 @@ def main():
         log("%s/%s: warmup %d steps" % (flux, precision, args.warmup))
         sim.integrate_in_time(target_time=1e30, max_step=args.warmup)
         torch.cuda.synchronize()
+        kc = C.c_int(0)
+        e4.lib.check(L.e4hip_block_kchunk(blk.id, C.byref(kc)))   # the autotuner's choice
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
@@ -402,9 +409,9 @@ def main():
         cfl_ms = tot.value / max(cnt.value, 1)
         L.e4hip_profile_enable(0)
         results[(flux, precision)] = dict(wall=wall, ev_ms=ev_ms, stage_ms=stage_ms, cfl_ms=cfl_ms,
-                                          steps_done=sim.state.step, dt=sim.state.dt_global)
-        log("%s/%s: %d steps in %.3f s (events %.3f ms), stage launch %.3f ms, cfl %.3f ms" %
-            (flux, precision, args.steps, wall, ev_ms, stage_ms, cfl_ms))
+                                          steps_done=sim.state.step, dt=sim.state.dt_global, kchunk=kc.value)
+        log("%s/%s: %d steps in %.3f s (events %.3f ms), stage launch %.3f ms, cfl %.3f ms, kchunk %d" %
+            (flux, precision, args.steps, wall, ev_ms, stage_ms, cfl_ms, kc.value))
         blk.destroy()
 
     # accuracy of the mixed form against the plain FP64 form: same U0, same
@@ -455,7 +462,7 @@ def main():
                    "cells": ncells, "flux_calculator": "ausmdv", "update": "pc", "precision": "fp64"},
         "variants": {"%s/%s" % k: {"value": round(ncells * args.steps / v["wall"] / 1e6, 3),
                                   "ms_per_step": round(v["wall"] * 1e3 / args.steps, 4),
-                                  "stage_launch_ms": round(v["stage_ms"], 4),
+                                  "stage_launch_ms": round(v["stage_ms"], 4), "kchunk": v["kchunk"],
                                   "stage_hbm_frac": round(ALG_BYTES_PER_CELL_STEP / 2.0 * ncells /
                                                           (v["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
                      for k, v in results.items()},
@@ -463,7 +470,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "stage_kernel<%s,32,8,3,%s>" % (head[1], head[0]), "stage_launch_ms": round(r["stage_ms"], 4),
-                     "cfl_launch_ms": round(r["cfl_ms"], 4),
+                     "cfl_launch_ms": round(r["cfl_ms"], 4), "kchunk": r["kchunk"],
                      "alg_bytes_per_launch": alg_bytes_launch},
     }
     if args.tp_n > 0:

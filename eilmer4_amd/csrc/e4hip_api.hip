@@ -134,6 +134,7 @@ struct State {
     std::vector<GasModelH> gases;
     BatchBuffers batch;
     BatchBuffers fill;                         // staging of the batched x_order-1 ghost fill
+    BatchBuffers tpbatch;                      // staging of the batched mixture launches
     int batch_min_blocks = 2;                  // batch the ideal blocks of a stage from this many on
     std::vector<BlockH> blocks;
     E4hipStepConfig cfg{E4HIP_FLUX_AUSMDV, E4HIP_UPDATE_PC, 2, 1, 1, 1, E4HIP_THERMO_RHOU, E4HIP_PRECISION_FP64, 0, 0,
@@ -144,6 +145,7 @@ struct State {
     long prof_n[2] = {0, 0};
     long long decode_evals = 0, decode_fallbacks = 0;
     long long face_evals = 0, face_fallbacks = 0;      // face temperature solves of the thermally perfect path
+    std::vector<int> last_bad;                          // [nblk][3] of the last step: invalid cells + failed decodes
     // graph replay of steps (use_graphs): up to two captured steps (a step
     // swaps U[0] and U[stages], so the starting states of consecutive steps
     // alternate between two keys), and the keys of the last two steps run
@@ -155,6 +157,11 @@ struct State {
     // k-chunk autotuner: chunk height of the stage kernel per block shape and
     // arithmetic (ni, nj, nk, precision, kernel flux, default reconstruction)
     std::vector<std::pair<std::vector<int>, int>> kchunk_tuned;
+    // work queue of persistent batched stage launches (next tile, finished
+    // workgroups; cleared by the last workgroup of each launch) and the GPU's
+    // compute units
+    unsigned int* queue = nullptr;
+    int cus = 0;
     std::string err;
 };
 
@@ -762,7 +769,7 @@ int launch_stage_batch(const std::vector<const BlockH*>& blks, const std::vector
     const int n = (int)blks.size();
     const long chunks = force_chunks > 0 ? force_chunks : batch_chunks(blks);
     std::vector<e4::StageArgs> args(n);
-    std::vector<int> map;
+    std::vector<long> ntile(n);
     for (int e = 0; e < n; ++e) {
         const BlockH& b = *blks[e];
         int kc = 1;
@@ -774,9 +781,19 @@ int launch_stage_batch(const std::vector<const BlockH*>& blks, const std::vector
         dim3 grid;
         int rc = stage_args_for(b, stage, dt, counters[e], kc, args[e], grid);
         if (rc) return rc;
+        ntile[e] = (long)grid.x * grid.y * grid.z;
+    }
+    // the tile list, the largest blocks' tiles first (the reference's
+    // localFluidBlocksBySize order, simcore.d:532-551); a persistent launch
+    // takes them from the queue in this order
+    std::vector<int> order(n);
+    for (int e = 0; e < n; ++e) order[e] = e;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return blks[a]->ncells > blks[b]->ncells; });
+    std::vector<int> map;
+    for (int e : order) {
         args[e].wg0 = (int)map.size();
-        const long nwg = (long)grid.x * grid.y * grid.z;
-        map.insert(map.end(), nwg, e);
+        map.insert(map.end(), ntile[e], e);
     }
     // the boundary-cell workgroups of all blocks, mapped after the stage's
     std::vector<int> bmap;
@@ -791,16 +808,25 @@ int launch_stage_batch(const std::vector<const BlockH*>& blks, const std::vector
     int rc = stage_batch_buffers(g_state.batch, batch_slot(set, stage, blks[0]->dims), args.data(),
                                  sizeof(e4::StageArgs) * n, map.data(), sizeof(int) * map.size(), &ad, &md);
     if (rc) return rc;
-    e4::BatchArgs B;
+    e4::BatchArgs B{};
     B.table = (unsigned long long)(uintptr_t)ad;
     B.wg_entry = (const int*)md;
     const bool def = args[0].high_order && args[0].apply_limiter && args[0].extrema_clipping;
     const int flux = kernel_flux(cfg.flux_calculator);
     hipEvent_t ev = nullptr;
     prof_begin(&ev);
-    const int nwg = (int)nstage_wg;
+    int nwg = (int)nstage_wg;
+    // persistent: as many workgroups as the GPU holds at once (the stage
+    // kernel's occupancy: three workgroups of four waves per compute unit in
+    // the FP64 forms, four in the mixed form), taking tiles from the queue
+    const int resident = g_state.cus * (cfg.precision_mode == E4HIP_PRECISION_MIXED ? 4 : 3);
+    if (g_state.queue && nwg > resident) {
+        B.queue = g_state.queue;
+        B.ntiles = nwg;
+        nwg = resident;
+    }
     const int dims = blks[0]->dims;
-    e4::BatchArgs Bb;
+    e4::BatchArgs Bb{};
     Bb.table = B.table;
     Bb.wg_entry = (const int*)md + nstage_wg;
     const int nbwg = (int)bmap.size();
@@ -943,6 +969,80 @@ int launch_tp_stage(BlockH& b, int stage, double dt, unsigned int* counters)
     return E4HIP_OK;
 }
 
+// Whether a mixture block's stage runs the reference-form (exact) kernels:
+// in fp64_exact, or when its gas cannot use the fast modes' mixture form.
+bool tp_exact_form(const BlockH& b)
+{
+    return g_state.cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT || !tp_mixture_form(g_state.gases[b.gm].tp);
+}
+
+// One stage of several mixture blocks of one dimensionality (uniform cells,
+// the same kernel form and learned-guess use): one launch per face direction
+// over all blocks' faces, each block's boundary-face kernel where it has
+// faces to recompute, one update launch over all blocks' cells.  The blocks'
+// TPArgs go to a device table (staged like the ideal-gas batches), with a
+// workgroup-to-block map per launch.
+int launch_tp_stage_batch(const std::vector<BlockH*>& blks, const std::vector<unsigned int*>& counters, int stage,
+                          double dt, int set)
+{
+    const E4hipStepConfig& cfg = g_state.cfg;
+    const int n = (int)blks.size();
+    const int dims = blks[0]->dims;
+    std::vector<e4::TPArgs> args(n);
+    for (int e = 0; e < n; ++e) {
+        int rc = tp_args_for(*blks[e], stage, dt, counters[e], args[e]);
+        if (rc) return rc;
+    }
+    std::vector<int> map;
+    int wg[4] = {0, 0, 0, 0};
+    size_t off[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 4; ++k) {
+        off[k] = map.size();
+        if (k == 2 && dims != 3) continue;
+        for (int e = 0; e < n; ++e) {
+            const BlockH& b = *blks[e];
+            const long nk = (dims == 3) ? b.nk : 1;
+            long items;
+            if (k == 0) items = (long)(b.ni + 1) * b.nj * nk;
+            else if (k == 1) items = (long)b.ni * (b.nj + 1) * nk;
+            else if (k == 2) items = (long)b.ni * b.nj * (b.nk + 1);
+            else items = (long)b.ni * b.nj * nk;
+            args[e].wg0[k] = (int)(map.size() - off[k]);
+            map.insert(map.end(), (items + 127) / 128, e);
+        }
+        wg[k] = (int)(map.size() - off[k]);
+    }
+    char *ad = nullptr, *md = nullptr;
+    int rc = stage_batch_buffers(g_state.tpbatch, batch_slot(set, stage, dims), args.data(),
+                                 sizeof(e4::TPArgs) * n, map.data(), sizeof(int) * map.size(), &ad, &md);
+    if (rc) return rc;
+    e4::BatchArgs B[4];
+    for (int k = 0; k < 4; ++k) {
+        B[k] = e4::BatchArgs{};
+        B[k].table = (unsigned long long)(uintptr_t)ad;
+        B[k].wg_entry = (const int*)md + off[k];
+    }
+    std::vector<e4::TPArgs> per_block;
+    for (int e = 0; e < n; ++e)
+        if (args[e].fix_off[6] > 0) per_block.push_back(args[e]);
+    const bool guess = args[0].guess != nullptr;
+    const int flux = kernel_flux(cfg.flux_calculator);
+    hipEvent_t ev = nullptr;
+    prof_begin(&ev);
+    if (tp_exact_form(*blks[0]))
+        e4::launch_tp_stage_batch_exact(B, wg, dims, guess, per_block.data(), (int)per_block.size(), g_state.stream,
+                                        flux);
+    else if (cfg.precision_mode == E4HIP_PRECISION_MIXED)
+        e4::launch_tp_stage_batch_mixed(B, wg, dims, guess, per_block.data(), (int)per_block.size(), g_state.stream,
+                                        flux);
+    else
+        e4::launch_tp_stage_batch_fast64(B, wg, dims, guess, per_block.data(), (int)per_block.size(), g_state.stream,
+                                         flux);
+    prof_end(0, ev);
+    HIP_TRY(hipGetLastError());
+    return E4HIP_OK;
+}
+
 // Shock detection of one block for a stage (fluidblock.d:415-431): every
 // block's indicators are swapped first (S[1] keeps the previous detection,
 // which connected blocks' ghost cells carry), then each block detects from the
@@ -1062,6 +1162,13 @@ int e4hip_init(int device)
     HIP_TRY(hipStreamCreateWithFlags(&g_state.stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc(&g_state.smax, sizeof(unsigned long long)));
     HIP_TRY(hipMalloc(&g_state.last_signal, sizeof(double)));
+    HIP_TRY(hipMalloc(&g_state.queue, 2 * sizeof(unsigned int)));
+    HIP_TRY(hipMemset(g_state.queue, 0, 2 * sizeof(unsigned int)));
+    {
+        hipDeviceProp_t prop;
+        HIP_TRY(hipGetDeviceProperties(&prop, device));
+        g_state.cus = prop.multiProcessorCount;
+    }
     g_state.device = device;
     g_state.init = true;
     int rc = ensure_counters(16);
@@ -1083,14 +1190,15 @@ int e4hip_finalize(void)
     if (g_state.counters_host) HIP_TRY(hipHostFree(g_state.counters_host));
     g_state.counters_host = nullptr;
     g_state.counters_raw = g_state.abort = nullptr;
-    for (BatchBuffers* bb : {&g_state.batch, &g_state.fill}) {
+    for (BatchBuffers* bb : {&g_state.batch, &g_state.fill, &g_state.tpbatch}) {
         if (bb->args_dev) { HIP_TRY(hipFree(bb->args_dev)); HIP_TRY(hipHostFree(bb->args_host)); }
         if (bb->map_dev) { HIP_TRY(hipFree(bb->map_dev)); HIP_TRY(hipHostFree(bb->map_host)); }
         *bb = BatchBuffers();
     }
     if (g_state.smax) HIP_TRY(hipFree(g_state.smax));
     if (g_state.last_signal) HIP_TRY(hipFree(g_state.last_signal));
-    g_state.counters = nullptr; g_state.smax = nullptr; g_state.last_signal = nullptr;
+    if (g_state.queue) HIP_TRY(hipFree(g_state.queue));
+    g_state.counters = nullptr; g_state.smax = nullptr; g_state.last_signal = nullptr; g_state.queue = nullptr;
     g_state.counter_slots = 0;
     HIP_TRY(hipStreamDestroy(g_state.stream));
     g_state.stream = nullptr;
@@ -1812,7 +1920,7 @@ static int enqueue_step(const int* blks, int nblk, double dt, int ns, unsigned i
                                      sizeof(e4::StageArgs) * fill_args.size(), fill_map.data(),
                                      sizeof(int) * fill_map.size(), &ad, &md);
             if (rc) return rc;
-            e4::BatchArgs B;
+            e4::BatchArgs B{};
             B.table = (unsigned long long)(uintptr_t)ad;
             B.wg_entry = (const int*)md;
             const int nwg = (int)(fill_map.size() / 2);
@@ -1836,10 +1944,15 @@ static int enqueue_step(const int* blks, int nblk, double dt, int ns, unsigned i
         // are several; thermally perfect blocks: their kernel sequence each
         std::vector<const BlockH*> bl[2];
         std::vector<unsigned int*> bcount[2];
+        std::vector<BlockH*> tl[2];
+        std::vector<unsigned int*> tcount[2];
         for (int n = 0; n < nblk; ++n) {
             BlockH& b = *get_block(blks[n]);
             unsigned int* cnt = counters + e4::COUNTER_SLOT * (n * 3 + (stage - 1));
-            if (b.has_P()) {
+            if (b.has_P() && b.is_tp() && !b.general && batch && g_state.cfg.batch_blocks) {
+                tl[b.dims - 2].push_back(&b);
+                tcount[b.dims - 2].push_back(cnt);
+            } else if (b.has_P()) {
                 rc = launch_tp_stage(b, stage, dt, cnt);
                 if (rc) return rc;
             } else if (batch) {
@@ -1848,6 +1961,34 @@ static int enqueue_step(const int* blks, int nblk, double dt, int ns, unsigned i
             } else {
                 rc = launch_stage(b, stage, dt, cnt);
                 if (rc) return rc;
+            }
+        }
+        // mixture blocks: one launch per kernel over the blocks that share a
+        // kernel form and learned-guess use, the others one by one
+        for (int d = 0; d < 2; ++d) {
+            std::vector<BlockH*> group;
+            std::vector<unsigned int*> gcount;
+            for (size_t e = 0; e < tl[d].size(); ++e) {
+                const BlockH& a = *tl[d][0];
+                const BlockH& b = *tl[d][e];
+                const bool same = tp_exact_form(a) == tp_exact_form(b) &&
+                                  (g_state.gases[a.gm].guess_dev != nullptr) == (g_state.gases[b.gm].guess_dev != nullptr);
+                if (same) {
+                    group.push_back(tl[d][e]);
+                    gcount.push_back(tcount[d][e]);
+                } else {
+                    rc = launch_tp_stage(*tl[d][e], stage, dt, tcount[d][e]);
+                    if (rc) return rc;
+                }
+            }
+            if ((int)group.size() >= g_state.batch_min_blocks) {
+                rc = launch_tp_stage_batch(group, gcount, stage, dt, set);
+                if (rc) return rc;
+            } else {
+                for (size_t e = 0; e < group.size(); ++e) {
+                    rc = launch_tp_stage(*group[e], stage, dt, gcount[e]);
+                    if (rc) return rc;
+                }
             }
         }
         for (int d = 0; d < 2; ++d) {
@@ -1875,6 +2016,8 @@ static int step_outcome(const unsigned int* h, int nblk, int ns, int* n_bad_cell
     const unsigned int* slots = h + ABORT_WORDS;
     int stage_bad[3] = {0, 0, 0};
     bool too_many = false, decode_fail = false;
+    int first_blk = -1, first_stage = -1, first_count = 0;   // the first block over the limit
+    g_state.last_bad.assign(3 * (size_t)nblk, 0);
     *stopped = false;
     for (int s = 0; s < ns && !*stopped; ++s) {
         for (int n = 0; n < nblk; ++n) {
@@ -1885,14 +2028,20 @@ static int step_outcome(const unsigned int* h, int nblk, int ns, int* n_bad_cell
             g_state.face_evals += c[4];
             g_state.face_fallbacks += c[5];
             stage_bad[s] += (int)(inv + dec);
-            if ((int)inv > g_state.cfg.max_invalid_cells) too_many = true;
+            g_state.last_bad[3 * (size_t)n + s] = (int)(inv + dec);
+            if ((int)inv > g_state.cfg.max_invalid_cells) {
+                if (!too_many) { first_blk = n; first_stage = s; first_count = (int)inv; }
+                too_many = true;
+            }
             if (dec) decode_fail = true;
         }
         if (too_many || decode_fail || h[s]) *stopped = true;
     }
     if (n_bad_cells) for (int s = 0; s < 3; ++s) n_bad_cells[s] = stage_bad[s];
     if (decode_fail) return fail(E4HIP_ERR_DECODE, "Bad cell with negative mass.");
-    if (too_many) return fail(E4HIP_ERR_BAD_CELLS, "Too many bad cells; go home.");
+    if (too_many)
+        return fail(E4HIP_ERR_BAD_CELLS, "Too many bad cells; go home. (stage %d: %d bad cells in block[%d])",
+                    first_stage + 1, first_count, first_blk);
     if (*stopped) return fail(E4HIP_ERR_BAD_CELLS, "a stage stopped the step");
     return E4HIP_OK;
 }
@@ -1948,6 +2097,7 @@ static int capture_step(const int* blks, int nblk, double dt, int ns, int set, c
 int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_cells)
 {
     if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
+    g_state.last_bad.clear();
     if (!blks || nblk <= 0) return fail(E4HIP_ERR_ARG, "no blocks");
     if (!(dt > 0.0)) return fail(E4HIP_ERR_ARG, "dt must be positive");
     for (int n = 0; n < nblk; ++n)
@@ -2093,6 +2243,19 @@ void block_dt(unsigned long long smax_bits, double last, double dt_current, doub
 } // namespace
 
 extern "C" {
+
+// Per-block, per-stage bad-cell counts of the last e4hip_explicit_increment
+// (the counts the reference prints per block, simcore.d:2480-2487).
+int e4hip_last_step_bad_cells(int nblk, int* counts)
+{
+    if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
+    if (!counts || nblk <= 0) return fail(E4HIP_ERR_ARG, "counts must hold nblk x 3 ints");
+    if ((size_t)nblk * 3 != g_state.last_bad.size())
+        return fail(E4HIP_ERR_ARG, "the last step had %d blocks, not %d (or did not run)",
+                    (int)(g_state.last_bad.size() / 3), nblk);
+    memcpy(counts, g_state.last_bad.data(), sizeof(int) * g_state.last_bad.size());
+    return E4HIP_OK;
+}
 
 int e4hip_determine_dt(const int* blks, int nblk, double dt_current, double cfl_value, int check_cfl,
                        double* dt_allow_out, double* cfl_max_out)

@@ -641,8 +641,8 @@ void launch_gen_stage_m(const TPArgs& A, hipStream_t stream, int flux)
     if constexpr (NSP == 0) {
         hipLaunchKernelGGL((ig_update_kernel<M>), grid, b, 0, stream, A);
     } else {
-        if (A.guess) hipLaunchKernelGGL((tp_update_kernel<M, NSP, true>), grid, b, 0, stream, A);
-        else hipLaunchKernelGGL((tp_update_kernel<M, NSP, false>), grid, b, 0, stream, A);
+        if (A.guess) hipLaunchKernelGGL((tp_update_kernel<M, NSP, true, false>), grid, b, 0, stream, A);
+        else hipLaunchKernelGGL((tp_update_kernel<M, NSP, false, false>), grid, b, 0, stream, A);
     }
 }
 

@@ -186,7 +186,9 @@ __host__ __device__ inline AxiGeom axi_geom(const double* g, int ni, int nj)
 // table address is an integer so the struct is the same on host and device.
 struct BatchArgs {
     unsigned long long table;         // const StageArgs[nentries] in device memory
-    const int* wg_entry;
+    const int* wg_entry;              // entry of each tile (workgroup) of the launch
+    unsigned int* queue;              // work queue (next tile, finished workgroups), or null
+    int ntiles;                       // tiles of the launch (with a queue)
 };
 
 // Counters of one block and stage: [0] invalid cells, [1] decode failures,
@@ -235,6 +237,10 @@ struct TPArgs {
     // fix_off[6] faces, slab by slab in the order (d, side) = (0,-), (0,+),
     // (1,-), (1,+), (2,-), (2,+); fix_layers[slab] face layers each
     int fix_layers[6];
+    // a batched launch over several mixture blocks: this block's first
+    // workgroup in the launch of the face kernel of direction d (0-2) and of
+    // the update kernel (3)
+    int wg0[4];
     int fix_off[7];
     int noghost;                      // bit f: face f has no ghost-cell data
     // stored ghost cells of face f, [8+nsp][2 * face cells] (layer-major,
@@ -276,6 +282,12 @@ void launch_stage_batch_mixed(const BatchArgs& B, int nwg, bool def, hipStream_t
 // thermally perfect mixture (five species): faces of every direction, then the
 // update; decode of an uploaded state
 void launch_tp_stage_exact(const TPArgs& A, hipStream_t stream, int flux);
+void launch_tp_stage_batch_exact(const BatchArgs* B, const int* wg, int dims, bool guess, const TPArgs* per_block,
+                                 int nper, hipStream_t stream, int flux);
+void launch_tp_stage_batch_fast64(const BatchArgs* B, const int* wg, int dims, bool guess, const TPArgs* per_block,
+                                  int nper, hipStream_t stream, int flux);
+void launch_tp_stage_batch_mixed(const BatchArgs* B, const int* wg, int dims, bool guess, const TPArgs* per_block,
+                                 int nper, hipStream_t stream, int flux);
 void launch_tp_stage_fast64(const TPArgs& A, hipStream_t stream, int flux);
 void launch_tp_stage_mixed(const TPArgs& A, hipStream_t stream, int flux);
 // shock indicators of a stage's stencil state (adaptive_hanel_ausmdv); the

@@ -86,12 +86,27 @@ __device__ __forceinline__ int batch_word(int idx)
     KBatch b = (KBatch)__builtin_amdgcn_kernarg_segment_ptr();
     return __builtin_amdgcn_readfirstlane(b->wg_entry[idx]);
 }
+
+// a batched launch's work queue (null: one tile per workgroup) and tile count
+__device__ __forceinline__ unsigned int* batch_queue()
+{
+    KBatch b = (KBatch)__builtin_amdgcn_kernarg_segment_ptr();
+    return b->queue;
+}
+
+__device__ __forceinline__ int batch_ntiles()
+{
+    KBatch b = (KBatch)__builtin_amdgcn_kernarg_segment_ptr();
+    return b->ntiles;
+}
 #else
 using KArgs = const StageArgs*;
 __device__ __forceinline__ KArgs kargs() { return nullptr; }
 __device__ __forceinline__ KArgs kargs_entry(int) { return nullptr; }
 __device__ __forceinline__ int batch_entry() { return 0; }
 __device__ __forceinline__ int batch_word(int) { return 0; }
+__device__ __forceinline__ unsigned int* batch_queue() { return nullptr; }
+__device__ __forceinline__ int batch_ntiles() { return 0; }
 #endif
 
 // Decode of the cell (i,j,k); positions one or two cells outside the block in
@@ -457,12 +472,14 @@ template <> struct StageOccupancy<ModeMixed> { static constexpr int waves = E4_S
 
 // DEF: the default reconstruction (order 2, limiter, extrema clipping) fixed
 // at compile time; otherwise the three options are read from the arguments.
+// One tile of the stage: with BATCH, tile wg of the launch's tile list (the
+// workgroup's own index, or the one it took from the work queue); without,
+// the workgroup's grid position.
 template <class M, int TX, int TY, int DIMS, int FLUX, bool DEF, bool BATCH>
-__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(StageOccupancy<M>::waves)))
-void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type A)
+__device__ __forceinline__ void stage_tile(int wg)
 {
     // the arguments of this workgroup's block, re-read per phase
-    const int entry = BATCH ? batch_entry() : 0;
+    const int entry = BATCH ? batch_word(wg) : 0;
     auto args = [&]() { return BATCH ? kargs_entry(entry) : kargs(); };
     using R = typename M::R;
     constexpr bool SH = (FLUX == FLUX_SHOCK);
@@ -485,7 +502,7 @@ void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type A
     if (stage_aborted(ap->abort_prev)) return;
     int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
     if constexpr (BATCH) {
-        const int w = (int)blockIdx.x - ap->wg0;
+        const int w = wg - ap->wg0;
         bx = w % ap->tiles_x;
         by = (w / ap->tiles_x) % ap->tiles_y;
         bz = w / (ap->tiles_x * ap->tiles_y);
@@ -702,6 +719,44 @@ void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type A
                 own = qa; qa = qb; qb = q3;
             }
         }
+    }
+}
+
+// The stage kernel.  A batched launch with a work queue (BatchArgs::queue) is
+// persistent: each workgroup takes tiles from the queue (largest blocks'
+// tiles first, as the host orders them) until none is left, and the last
+// workgroup to finish clears the queue for the next launch on the stream;
+// without a queue each workgroup is one tile.
+template <class M, int TX, int TY, int DIMS, int FLUX, bool DEF, bool BATCH>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(StageOccupancy<M>::waves)))
+void stage_kernel(typename std::conditional<BATCH, BatchArgs, StageArgs>::type)
+{
+    if constexpr (BATCH) {
+        unsigned int* q = batch_queue();
+        if (q) {
+            __shared__ int s_tile;
+            const int ntiles = batch_ntiles();
+            for (;;) {
+                if (threadIdx.x == 0) s_tile = (int)atomicAdd(q, 1u);
+                __syncthreads();
+                const int t = s_tile;
+                __syncthreads();
+                if (t >= ntiles) break;
+                stage_tile<M, TX, TY, DIMS, FLUX, DEF, BATCH>(t);
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) {
+                __threadfence();
+                if (atomicAdd(q + 1, 1u) == gridDim.x - 1) {
+                    atomicExch(q, 0u);
+                    atomicExch(q + 1, 0u);
+                }
+            }
+            return;
+        }
+        stage_tile<M, TX, TY, DIMS, FLUX, DEF, BATCH>((int)blockIdx.x);
+    } else {
+        stage_tile<M, TX, TY, DIMS, FLUX, DEF, BATCH>(0);
     }
 }
 

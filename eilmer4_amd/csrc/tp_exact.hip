@@ -9,6 +9,12 @@ void launch_tp_stage_exact(const TPArgs& A, hipStream_t stream, int flux)
     launch_tp_stage_m<ModeExact>(A, stream, flux);
 }
 
+void launch_tp_stage_batch_exact(const BatchArgs* B, const int* wg, int dims, bool guess, const TPArgs* per_block,
+                                 int nper, hipStream_t stream, int flux)
+{
+    launch_tp_stage_batch_m<ModeExact>(B, wg, dims, guess, per_block, nper, stream, flux);
+}
+
 void launch_tp_shock_detect_exact(const TPArgs& A, hipStream_t stream)
 {
     launch_tp_shock_detect_m<ModeExact>(A, stream);

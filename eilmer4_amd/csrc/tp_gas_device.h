@@ -486,18 +486,20 @@ __device__ __forceinline__ double tp_sound_speed(const TPGas& g, const TPState<N
 }
 
 // gas/gas_model.d:296-328 (tolerance 0, assert_error_tolerance 0.1)
-template <class M, int NSP>
-__device__ __forceinline__ int scale_mass_fractions(double* Y)
+// (in FP64, or in FP32 for the mixed form's face reconstruction)
+template <class M, int NSP, class R>
+__device__ __forceinline__ int scale_mass_fractions(R* Y)
 {
-    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
-    double sum = 0.0;
+    using MD = typename std::conditional<std::is_same<R, float>::value, ModeMixed,
+                                         typename std::conditional<M::kFast, ModeFast64, ModeExact>::type>::type;
+    R sum = R(0.0);
 #pragma unroll
     for (int s = 0; s < NSP; ++s) {
-        Y[s] = Y[s] >= 0.0 ? Y[s] : 0.0;
+        Y[s] = Y[s] >= R(0.0) ? Y[s] : R(0.0);
         sum += Y[s];
     }
-    if (fabs(sum - 1.0) > 0.1) return -1;
-    if (fabs(sum - 1.0) > 0.0) {
+    if (ab(sum - R(1.0)) > R(0.1)) return -1;
+    if (ab(sum - R(1.0)) > R(0.0)) {
 #pragma unroll
         for (int s = 0; s < NSP; ++s) Y[s] = dv<MD>(Y[s], sum);
     }

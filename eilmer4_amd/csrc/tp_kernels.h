@@ -28,6 +28,11 @@ namespace e4 {
 // the stored temperature (or the learned guess when one is installed).
 template <class M> struct TPArith { using type = M; static constexpr bool polish = false; };
 template <> struct TPArith<ModeMixed> { using type = ModeFast64; static constexpr bool polish = true; };
+// The arithmetic of the face kernel's reconstruction and flux: that of the
+// mode, except that the mixed form reconstructs and evaluates the flux in FP32
+// (the face temperature solve stays in FP64, TPArith).
+template <class M> struct TPFlux { using type = typename TPArith<M>::type; };
+template <> struct TPFlux<ModeMixed> { using type = ModeMixed; };
 
 // update_thermo_from_rhou of a face state (therm_perf_gas.d:187-191); with
 // polish, Newton from Q.T first (*fallbacks counts the solves it hands on).
@@ -59,12 +64,15 @@ __device__ __forceinline__ int tp_face_eos(const TPGas& g, TPState<NSP>& Q, int*
     }
 }
 
-template <int NSP>
-struct TPCell {
-    double rho, p, T, u, a;
-    double v[3];
-    double Y[NSP];
+// A face-side (or cell) state of a mixture in the face frame, in the
+// arithmetic of the flux path (double, or float in the mixed form).
+template <class R, int NSP>
+struct TPCellT {
+    R rho, p, T, u, a;
+    R v[3];
+    R Y[NSP];
 };
+template <int NSP> using TPCell = TPCellT<double, NSP>;
 
 // Where the values of one stencil position come from: field f (P_* order)
 // is base[f * stride] -- the primitive array of this block or of a connected
@@ -141,64 +149,69 @@ __device__ __forceinline__ double tp_vel(const TPRef& r, int comp)
 }
 
 // The whole state of a stencil position, velocity in the face frame (pn, pt1, pt2).
-template <int NSP>
-__device__ __forceinline__ TPCell<NSP> tp_cell(const TPRef& r, int pn, int pt1, int pt2)
+template <int NSP, class R = double>
+__device__ __forceinline__ TPCellT<R, NSP> tp_cell(const TPRef& r, int pn, int pt1, int pt2)
 {
-    TPCell<NSP> s;
-    s.rho = tp_val(r, P_RHO); s.p = tp_val(r, P_P); s.T = tp_val(r, P_T); s.u = tp_val(r, P_U); s.a = tp_val(r, P_A);
-    s.v[0] = tp_vel(r, pn); s.v[1] = tp_vel(r, pt1); s.v[2] = tp_vel(r, pt2);
+    TPCellT<R, NSP> s;
+    s.rho = R(tp_val(r, P_RHO)); s.p = R(tp_val(r, P_P)); s.T = R(tp_val(r, P_T)); s.u = R(tp_val(r, P_U));
+    s.a = R(tp_val(r, P_A));
+    s.v[0] = R(tp_vel(r, pn)); s.v[1] = R(tp_vel(r, pt1)); s.v[2] = R(tp_vel(r, pt2));
 #pragma unroll
-    for (int sp = 0; sp < NSP; ++sp) s.Y[sp] = tp_val(r, P_Y + sp);
+    for (int sp = 0; sp < NSP; ++sp) s.Y[sp] = R(tp_val(r, P_Y + sp));
     return s;
 }
 
 // fluxcalc.d:376-529 with species; velocities in the face frame (v[0] normal).
+// The flux functions of the mixture take the arithmetic of M: R = double for
+// ModeExact / ModeFast64 and float for ModeMixed's face path.
 template <class M, int NSP>
-__device__ __forceinline__ void tp_ausmdv(const TPCell<NSP>& L, const TPCell<NSP>& Rt, double* F)
+__device__ __forceinline__ void tp_ausmdv(const TPCellT<typename M::R, NSP>& L, const TPCellT<typename M::R, NSP>& Rt,
+                                          typename M::R* F)
 {
-    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
-    double rL = L.rho, pL = L.p, pLrL = dv<MD>(pL, rL), uL = L.v[0], vL = L.v[1], wL = L.v[2];
-    double eL = L.u, aL = L.a;
-    double keL = 0.5 * (uL * uL + vL * vL + wL * wL);
-    double HL = eL + pLrL + keL;
-    double rR = Rt.rho, pR = Rt.p, pRrR = dv<MD>(pR, rR), uR = Rt.v[0], vR = Rt.v[1], wR = Rt.v[2];
-    double eR = Rt.u, aR = Rt.a;
-    double keR = 0.5 * (uR * uR + vR * vR + wR * wR);
-    double HR = eR + pRrR + keR;
-    double alphaL = dv<MD>(2.0 * pLrL, pLrL + pRrR);
-    double alphaR = dv<MD>(2.0 * pRrR, pLrL + pRrR);
-    double am = fmax(aL, aR);
-    double ML = dv<MD>(uL, am);
-    double MR = dv<MD>(uR, am);
-    double pLplus, uLplus;
-    double duL = 0.5 * (uL + fabs(uL));
-    if (fabs(ML) <= 1.0) {
-        pLplus = pL * (ML + 1.0) * (ML + 1.0) * (2.0 - ML) * 0.25;
-        uLplus = alphaL * (dv<MD>((uL + am) * (uL + am), 4.0 * am) - duL) + duL;
+    using R = typename M::R;
+    using MD = typename std::conditional<M::kFast, M, ModeExact>::type;
+    R rL = L.rho, pL = L.p, pLrL = dv<MD>(pL, rL), uL = L.v[0], vL = L.v[1], wL = L.v[2];
+    R eL = L.u, aL = L.a;
+    R keL = R(0.5) * (uL * uL + vL * vL + wL * wL);
+    R HL = eL + pLrL + keL;
+    R rR = Rt.rho, pR = Rt.p, pRrR = dv<MD>(pR, rR), uR = Rt.v[0], vR = Rt.v[1], wR = Rt.v[2];
+    R eR = Rt.u, aR = Rt.a;
+    R keR = R(0.5) * (uR * uR + vR * vR + wR * wR);
+    R HR = eR + pRrR + keR;
+    R alphaL = dv<MD>(R(2.0) * pLrL, pLrL + pRrR);
+    R alphaR = dv<MD>(R(2.0) * pRrR, pLrL + pRrR);
+    R am = mx(aL, aR);
+    R ML = dv<MD>(uL, am);
+    R MR = dv<MD>(uR, am);
+    R pLplus, uLplus;
+    R duL = R(0.5) * (uL + ab(uL));
+    if (ab(ML) <= R(1.0)) {
+        pLplus = pL * (ML + R(1.0)) * (ML + R(1.0)) * (R(2.0) - ML) * R(0.25);
+        uLplus = alphaL * (dv<MD>((uL + am) * (uL + am), R(4.0) * am) - duL) + duL;
     } else {
         pLplus = dv<MD>(pL * duL, uL);
         uLplus = duL;
     }
-    double pRminus, uRminus;
-    double duR = 0.5 * (uR - fabs(uR));
-    if (fabs(MR) <= 1.0) {
-        pRminus = pR * (MR - 1.0) * (MR - 1.0) * (2.0 + MR) * 0.25;
-        uRminus = alphaR * (dv<MD>(-(uR - am) * (uR - am), 4.0 * am) - duR) + duR;
+    R pRminus, uRminus;
+    R duR = R(0.5) * (uR - ab(uR));
+    if (ab(MR) <= R(1.0)) {
+        pRminus = pR * (MR - R(1.0)) * (MR - R(1.0)) * (R(2.0) + MR) * R(0.25);
+        uRminus = alphaR * (dv<MD>(-(uR - am) * (uR - am), R(4.0) * am) - duR) + duR;
     } else {
         pRminus = dv<MD>(pR * duR, uR);
         uRminus = duR;
     }
-    double ru_half = uLplus * rL + uRminus * rR;
-    double p_half = pLplus + pRminus;
-    double dp = pL - pR;
-    dp = dv<MD>(10.0 * fabs(dp), fmin(pL, pR));
-    double s = 0.5 * fmin(1.0, dp);
-    double ru2_AUSMV = uLplus * rL * uL + uRminus * rR * uR;
-    double ru2_AUSMD = 0.5 * (ru_half * (uL + uR) - fabs(ru_half) * (uR - uL));
-    double ru2_half = (0.5 + s) * ru2_AUSMV + (0.5 - s) * ru2_AUSMD;
+    R ru_half = uLplus * rL + uRminus * rR;
+    R p_half = pLplus + pRminus;
+    R dp = pL - pR;
+    dp = dv<MD>(R(10.0) * ab(dp), mn(pL, pR));
+    R s = R(0.5) * mn(R(1.0), dp);
+    R ru2_AUSMV = uLplus * rL * uL + uRminus * rR * uR;
+    R ru2_AUSMD = R(0.5) * (ru_half * (uL + uR) - ab(ru_half) * (uR - uL));
+    R ru2_half = (R(0.5) + s) * ru2_AUSMV + (R(0.5) - s) * ru2_AUSMD;
     F[0] = ru_half;
     F[1] = ru2_half + p_half;
-    if (ru_half >= 0.0) {
+    if (ru_half >= R(0.0)) {
         F[2] = ru_half * vL; F[3] = ru_half * wL; F[4] = ru_half * HL;
 #pragma unroll
         for (int i = 0; i < NSP; ++i) F[5 + i] = ru_half * L.Y[i];
@@ -207,12 +220,12 @@ __device__ __forceinline__ void tp_ausmdv(const TPCell<NSP>& L, const TPCell<NSP
 #pragma unroll
         for (int i = 0; i < NSP; ++i) F[5 + i] = ru_half * Rt.Y[i];
     }
-    bool caseA = ((uL - aL) < 0.0) && ((uR - aR) > 0.0);
-    bool caseB = ((uL + aL) < 0.0) && ((uR + aR) > 0.0);
-    double d_ua = 0.0;
-    if (caseA && !caseB) d_ua = 0.125 * ((uR - aR) - (uL - aL));
-    if (caseB && !caseA) d_ua = 0.125 * ((uR + aR) - (uL + aL));
-    if (d_ua != 0.0) {
+    bool caseA = ((uL - aL) < R(0.0)) && ((uR - aR) > R(0.0));
+    bool caseB = ((uL + aL) < R(0.0)) && ((uR + aR) > R(0.0));
+    R d_ua = R(0.0);
+    if (caseA && !caseB) d_ua = R(0.125) * ((uR - aR) - (uL - aL));
+    if (caseB && !caseA) d_ua = R(0.125) * ((uR + aR) - (uL + aL));
+    if (d_ua != R(0.0)) {
         F[0] -= d_ua * (rR - rL);
         F[1] -= d_ua * (rR * uR - rL * uL);
         F[2] -= d_ua * (rR * vR - rL * vL);
@@ -225,37 +238,40 @@ __device__ __forceinline__ void tp_ausmdv(const TPCell<NSP>& L, const TPCell<NSP
 
 // fluxcalc.d:532-623 with species (Hanel's flux vector splitting).
 template <class M, int NSP>
-__device__ __forceinline__ void tp_hanel(const TPCell<NSP>& L, const TPCell<NSP>& Rt, double* F)
+__device__ __forceinline__ void tp_hanel(const TPCellT<typename M::R, NSP>& L, const TPCellT<typename M::R, NSP>& Rt,
+                                         typename M::R* F)
 {
-    const double rL = L.rho, pL = L.p;
-    const double pLrL = dv<M>(pL, rL);
-    const double uL = L.v[0], vL = L.v[1], wL = L.v[2];
-    const double eL = L.u, aL = L.a;
-    const double keL = 0.5 * (uL * uL + vL * vL + wL * wL);
-    const double HL = eL + pLrL + keL;
-    const double rR = Rt.rho, pR = Rt.p;
-    const double pRrR = dv<M>(pR, rR);
-    const double uR = Rt.v[0], vR = Rt.v[1], wR = Rt.v[2];
-    const double eR = Rt.u, aR = Rt.a;
-    const double keR = 0.5 * (uR * uR + vR * vR + wR * wR);
-    const double HR = eR + pRrR + keR;
-    double pLplus, uLplus;
-    if (fabs(uL) <= aL) {
-        uLplus = dv<M>(1.0, 4.0 * aL) * (uL + aL) * (uL + aL);
-        pLplus = pL * uLplus * (dv<M>(1.0, aL) * (2.0 - dv<M>(uL, aL)));
+    using R = typename M::R;
+    using MD = typename std::conditional<M::kFast, M, ModeExact>::type;
+    const R rL = L.rho, pL = L.p;
+    const R pLrL = dv<MD>(pL, rL);
+    const R uL = L.v[0], vL = L.v[1], wL = L.v[2];
+    const R eL = L.u, aL = L.a;
+    const R keL = R(0.5) * (uL * uL + vL * vL + wL * wL);
+    const R HL = eL + pLrL + keL;
+    const R rR = Rt.rho, pR = Rt.p;
+    const R pRrR = dv<MD>(pR, rR);
+    const R uR = Rt.v[0], vR = Rt.v[1], wR = Rt.v[2];
+    const R eR = Rt.u, aR = Rt.a;
+    const R keR = R(0.5) * (uR * uR + vR * vR + wR * wR);
+    const R HR = eR + pRrR + keR;
+    R pLplus, uLplus;
+    if (ab(uL) <= aL) {
+        uLplus = dv<MD>(R(1.0), R(4.0) * aL) * (uL + aL) * (uL + aL);
+        pLplus = pL * uLplus * (dv<MD>(R(1.0), aL) * (R(2.0) - dv<MD>(uL, aL)));
     } else {
-        uLplus = 0.5 * (uL + fabs(uL));
-        pLplus = pL * uLplus * dv<M>(1.0, uL);
+        uLplus = R(0.5) * (uL + ab(uL));
+        pLplus = pL * uLplus * dv<MD>(R(1.0), uL);
     }
-    double pRminus, uRminus;
-    if (fabs(uR) <= aR) {
-        uRminus = dv<M>(-1.0, 4.0 * aR) * (uR - aR) * (uR - aR);
-        pRminus = pR * uRminus * (dv<M>(1.0, aR) * (-2.0 - dv<M>(uR, aR)));
+    R pRminus, uRminus;
+    if (ab(uR) <= aR) {
+        uRminus = dv<MD>(R(-1.0), R(4.0) * aR) * (uR - aR) * (uR - aR);
+        pRminus = pR * uRminus * (dv<MD>(R(1.0), aR) * (R(-2.0) - dv<MD>(uR, aR)));
     } else {
-        uRminus = 0.5 * (uR - fabs(uR));
-        pRminus = pR * uRminus * dv<M>(1.0, uR);
+        uRminus = R(0.5) * (uR - ab(uR));
+        pRminus = pR * uRminus * dv<MD>(R(1.0), uR);
     }
-    const double p_half = pLplus + pRminus;
+    const R p_half = pLplus + pRminus;
     F[0] = uLplus * rL + uRminus * rR;
     F[1] = uLplus * rL * uL + uRminus * rR * uR + p_half;
     F[2] = uLplus * rL * vL + uRminus * rR * vR;
@@ -267,29 +283,31 @@ __device__ __forceinline__ void tp_hanel(const TPCell<NSP>& L, const TPCell<NSP>
 
 // HLLC with species carried by the contact wave (the oracle's hllc).
 template <class M, int NSP>
-__device__ __forceinline__ void tp_hllc(const TPCell<NSP>& L, const TPCell<NSP>& Rt, double* F)
+__device__ __forceinline__ void tp_hllc(const TPCellT<typename M::R, NSP>& L, const TPCellT<typename M::R, NSP>& Rt,
+                                        typename M::R* F)
 {
-    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
-    double rL = L.rho, pL = L.p, uL = L.v[0], vL = L.v[1], wL = L.v[2], aL = L.a;
-    double EL = rL * (L.u + 0.5 * (uL * uL + vL * vL + wL * wL));
-    double rR = Rt.rho, pR = Rt.p, uR = Rt.v[0], vR = Rt.v[1], wR = Rt.v[2], aR = Rt.a;
-    double ER = rR * (Rt.u + 0.5 * (uR * uR + vR * vR + wR * wR));
-    double SL = fmin(uL - aL, uR - aR);
-    double SR = fmax(uL + aL, uR + aR);
-    double mL = rL * (SL - uL);
-    double mR = rR * (SR - uR);
-    double Sstar = dv<MD>(pR - pL + mL * uL - mR * uR, mL - mR);
-    const bool useL = (0.0 <= SL) || (!(SR <= 0.0) && (0.0 <= Sstar));
-    const TPCell<NSP>& K = useL ? L : Rt;
-    const double rK = K.rho, uK = K.v[0], vK = K.v[1], wK = K.v[2], pK = K.p, EK = useL ? EL : ER;
-    double mf = rK * uK;
+    using R = typename M::R;
+    using MD = typename std::conditional<M::kFast, M, ModeExact>::type;
+    R rL = L.rho, pL = L.p, uL = L.v[0], vL = L.v[1], wL = L.v[2], aL = L.a;
+    R EL = rL * (L.u + R(0.5) * (uL * uL + vL * vL + wL * wL));
+    R rR = Rt.rho, pR = Rt.p, uR = Rt.v[0], vR = Rt.v[1], wR = Rt.v[2], aR = Rt.a;
+    R ER = rR * (Rt.u + R(0.5) * (uR * uR + vR * vR + wR * wR));
+    R SL = mn(uL - aL, uR - aR);
+    R SR = mx(uL + aL, uR + aR);
+    R mL = rL * (SL - uL);
+    R mR = rR * (SR - uR);
+    R Sstar = dv<MD>(pR - pL + mL * uL - mR * uR, mL - mR);
+    const bool useL = (R(0.0) <= SL) || (!(SR <= R(0.0)) && (R(0.0) <= Sstar));
+    const TPCellT<R, NSP>& K = useL ? L : Rt;
+    const R rK = K.rho, uK = K.v[0], vK = K.v[1], wK = K.v[2], pK = K.p, EK = useL ? EL : ER;
+    R mf = rK * uK;
     F[0] = mf; F[1] = mf * uK + pK; F[2] = mf * vK; F[3] = mf * wK; F[4] = uK * (EK + pK);
 #pragma unroll
     for (int i = 0; i < NSP; ++i) F[5 + i] = mf * K.Y[i];
-    if ((0.0 <= SL) || (SR <= 0.0)) return;
-    const double SK = useL ? SL : SR;
-    double coef = dv<MD>(rK * (SK - uK), SK - Sstar);
-    double Us[5];
+    if ((R(0.0) <= SL) || (SR <= R(0.0))) return;
+    const R SK = useL ? SL : SR;
+    R coef = dv<MD>(rK * (SK - uK), SK - Sstar);
+    R Us[5];
     Us[0] = coef;
     Us[1] = coef * Sstar;
     Us[2] = coef * vK;
@@ -341,66 +359,80 @@ __device__ __forceinline__ double tp_cv(const TPGas& g, const double* Y, double 
     }
 }
 
+// Cv (what = 0) or gamma = Cp/Cv (what = 1) of a face-side state in FP64
+// from the gas model at the side's temperature, whatever the flux path's R.
+template <class M, int NSP, class R>
+__device__ __forceinline__ double tp_side_thermo(const TPGas& g, const TPCellT<R, NSP>& c, int what)
+{
+    using MG = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
+    double Y[NSP];
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) Y[s] = double(c.Y[s]);
+    return what ? tp_gamma<MG, NSP>(g, Y, double(c.T)) : tp_cv<MG, NSP>(g, Y, double(c.T));
+}
+
 // efmflx (fluxcalc.d:626-790) with species, which follow the sign of the mass
 // flux; Cv of each side from the gas model at the side's temperature.
 template <class M, int NSP>
-__device__ __forceinline__ void tp_efm(const TPGas& g, const TPCell<NSP>& L, const TPCell<NSP>& Rt, double* F)
+__device__ __forceinline__ void tp_efm(const TPGas& g, const TPCellT<typename M::R, NSP>& L,
+                                       const TPCellT<typename M::R, NSP>& Rt, typename M::R* F)
 {
-    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
-    const double dtwspi = 0.282094792;
-    const double rhoL = L.rho, presL = L.p, eL = L.u, tL = L.T;
-    const double hL = eL + dv<MD>(presL, rhoL);
-    const double vnL = L.v[0], vpL = L.v[1], vqL = L.v[2];
-    const double rhoR = Rt.rho, presR = Rt.p, eR = Rt.u, tR = Rt.T;
-    const double hR = eR + dv<MD>(presR, rhoR);
-    const double vnR = Rt.v[0], vpR = Rt.v[1], vqR = Rt.v[2];
-    const double cvL = tp_cv<M, NSP>(g, L.Y, tL);
-    const double RgasL = dv<MD>(presL, rhoL * tL);
-    const double cvR = tp_cv<M, NSP>(g, Rt.Y, tR);
-    const double RgasR = dv<MD>(presR, rhoR * tR);
-    const double rLsqrt = sqrt(rhoL), rRsqrt = sqrt(rhoR);
-    const double alpha = dv<MD>(rLsqrt, rLsqrt + rRsqrt);
-    const double cv = alpha * cvL + (1.0 - alpha) * cvR;
-    const double Rgas = alpha * RgasL + (1.0 - alpha) * RgasR;
-    const double cp = cv + Rgas;
-    const double gam = dv<MD>(cp, cv);
-    const double con = dv<MD>(0.5 * (gam + 1.0), gam - 1.0);
-    const double rtL = Rgas * tL;
-    const double cmpL = sqrt(2.0 * rtL);
-    const double hvsqL = 0.5 * (vnL * vnL + vpL * vpL + vqL * vqL);
-    const double snL = dv<MD>(vnL, 1.0 * cmpL);
-    double exL, efL;
+    using R = typename M::R;
+    using MD = typename std::conditional<M::kFast, M, ModeExact>::type;
+    const R dtwspi = R(0.282094792);
+    const R rhoL = L.rho, presL = L.p, eL = L.u, tL = L.T;
+    const R hL = eL + dv<MD>(presL, rhoL);
+    const R vnL = L.v[0], vpL = L.v[1], vqL = L.v[2];
+    const R rhoR = Rt.rho, presR = Rt.p, eR = Rt.u, tR = Rt.T;
+    const R hR = eR + dv<MD>(presR, rhoR);
+    const R vnR = Rt.v[0], vpR = Rt.v[1], vqR = Rt.v[2];
+    const R cvL = R(tp_side_thermo<M, NSP>(g, L, 0));
+    const R RgasL = dv<MD>(presL, rhoL * tL);
+    const R cvR = R(tp_side_thermo<M, NSP>(g, Rt, 0));
+    const R RgasR = dv<MD>(presR, rhoR * tR);
+    const R rLsqrt = sq_rt<R>(rhoL), rRsqrt = sq_rt<R>(rhoR);
+    const R alpha = dv<MD>(rLsqrt, rLsqrt + rRsqrt);
+    const R cv = alpha * cvL + (R(1.0) - alpha) * cvR;
+    const R Rgas = alpha * RgasL + (R(1.0) - alpha) * RgasR;
+    const R cp = cv + Rgas;
+    const R gam = dv<MD>(cp, cv);
+    const R con = dv<MD>(R(0.5) * (gam + R(1.0)), gam - R(1.0));
+    const R rtL = Rgas * tL;
+    const R cmpL = sq_rt<R>(R(2.0) * rtL);
+    const R hvsqL = R(0.5) * (vnL * vnL + vpL * vpL + vqL * vqL);
+    const R snL = dv<MD>(vnL, R(1.0) * cmpL);
+    R exL, efL;
     exxef<MD>(snL, exL, efL);
-    const double wL = 0.5 * (1.0 + efL);
-    const double dL = exL * dtwspi;
-    const double rtR = dv<MD>(presR, rhoR);
-    const double cmpR = sqrt(2.0 * rtR);
-    const double hvsqR = 0.5 * (vnR * vnR + vpR * vpR + vqR * vqR);
-    const double snR = dv<MD>(vnR, 1.0 * cmpR);
-    double exR, efR;
+    const R wL = R(0.5) * (R(1.0) + efL);
+    const R dL = exL * dtwspi;
+    const R rtR = dv<MD>(presR, rhoR);
+    const R cmpR = sq_rt<R>(R(2.0) * rtR);
+    const R hvsqR = R(0.5) * (vnR * vnR + vpR * vpR + vqR * vqR);
+    const R snR = dv<MD>(vnR, R(1.0) * cmpR);
+    R exR, efR;
     exxef<MD>(snR, exR, efR);
-    const double wR = 0.5 * (1.0 - efR);
-    const double dR = -exR * dtwspi;
-    const double fmsL = (wL * rhoL * vnL) + (dL * cmpL * rhoL);
-    const double fmsR = (wR * rhoR * vnR) + (dR * cmpR * rhoR);
+    const R wR = R(0.5) * (R(1.0) - efR);
+    const R dR = -exR * dtwspi;
+    const R fmsL = (wL * rhoL * vnL) + (dL * cmpL * rhoL);
+    const R fmsR = (wR * rhoR * vnR) + (dR * cmpR * rhoR);
     F[0] = fmsL + fmsR;
     F[1] = fmsL * vnL + fmsR * vnR + wL * presL + wR * presR;
     F[2] = fmsL * vpL + fmsR * vpR;
     F[3] = fmsL * vqL + fmsR * vqR;
     F[4] = (wL * rhoL * vnL) * (hvsqL + hL) + (wR * rhoR * vnR) * (hvsqR + hR) +
            (dL * cmpL * rhoL) * (hvsqL + con * rtL) + (dR * cmpR * rhoR) * (hvsqR + con * rtR);
-    const bool left = F[0] > 0.0;
+    const bool left = F[0] > R(0.0);
 #pragma unroll
     for (int i = 0; i < NSP; ++i) F[5 + i] = F[0] * (left ? L.Y[i] : Rt.Y[i]);
 }
 
 // The face state of a mixture side as the ideal-gas flux functions read it.
-template <int NSP>
-__device__ __forceinline__ FaceStateT<double> tp_face_state(const TPCell<NSP>& c)
+template <class R, int NSP>
+__device__ __forceinline__ FaceStateT<R> tp_face_state(const TPCellT<R, NSP>& c)
 {
-    FaceStateT<double> f;
+    FaceStateT<R> f;
     f.rho = c.rho; f.p = c.p; f.u = c.u; f.a = c.a; f.vn = c.v[0]; f.vt1 = c.v[1]; f.vt2 = c.v[2];
-    f.RT = 0.0; f.T = c.T;
+    f.RT = R(0.0); f.T = c.T;
     return f;
 }
 
@@ -408,18 +440,22 @@ __device__ __forceinline__ FaceStateT<double> tp_face_state(const TPCell<NSP>& c
 // species upwinded by the mass flux (ausm_plus_up: M_half > 0 picks the left
 // density, ru_half >= 0 the left species; roe: F.mass >= 0); roe's gamma of
 // each side from the gas model.  The gas-dynamic part is the ideal-gas
-// functions' (they read p, rho, u, a and velocities only).
+// functions' (they read p, rho, u, a and velocities only; the RT shortcut of
+// the fast ideal-gas forms is not used for a mixture).
 template <class M, int NSP>
-__device__ __forceinline__ void tp_other_flux(const TPGas& g, bool use_roe, double M_inf, const TPCell<NSP>& L,
-                                              const TPCell<NSP>& Rt, double* F)
+__device__ __forceinline__ void tp_other_flux(const TPGas& g, bool use_roe, double M_inf,
+                                              const TPCellT<typename M::R, NSP>& L,
+                                              const TPCellT<typename M::R, NSP>& Rt, typename M::R* F)
 {
-    using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
-    const FaceStateT<double> Lf = tp_face_state<NSP>(L), Rf = tp_face_state<NSP>(Rt);
-    Flux5T<double> Fl;
-    if (use_roe) Fl = roe<MD>(Lf, Rf, tp_gamma<M, NSP>(g, L.Y, L.T), tp_gamma<M, NSP>(g, Rt.Y, Rt.T));
-    else Fl = ausm_plus_up<MD>(Lf, Rf, M_inf);
+    using R = typename M::R;
+    using MD = typename std::conditional<M::kFast, M, ModeExact>::type;
+    const FaceStateT<R> Lf = tp_face_state<R, NSP>(L), Rf = tp_face_state<R, NSP>(Rt);
+    Flux5T<R> Fl;
+    if (use_roe)
+        Fl = roe<MD>(Lf, Rf, R(tp_side_thermo<M, NSP>(g, L, 1)), R(tp_side_thermo<M, NSP>(g, Rt, 1)));
+    else Fl = ausm_plus_up<MD>(Lf, Rf, R(M_inf));
     F[0] = Fl.mass; F[1] = Fl.mn; F[2] = Fl.mt1; F[3] = Fl.mt2; F[4] = Fl.E;
-    const bool left = F[0] >= 0.0;
+    const bool left = F[0] >= R(0.0);
 #pragma unroll
     for (int i = 0; i < NSP; ++i) F[5 + i] = F[0] * (left ? L.Y[i] : Rt.Y[i]);
 }
@@ -428,16 +464,18 @@ __device__ __forceinline__ void tp_other_flux(const TPGas& g, bool use_roe, doub
 // FLUX_SHOCK: adaptive_hanel_ausmdv / adaptive_efm_ausmdv (fluxcalc.d:802-836),
 // or hanel / efm throughout; FLUX_OTHER: ausm_plus_up or roe.
 template <class MA, int NSP, int FLUX>
-__device__ __forceinline__ void tp_flux(const TPArgs& A, const TPGas& g, const TPCell<NSP>& Lf,
-                                        const TPCell<NSP>& Rf, bool shock, double* F)
+__device__ __forceinline__ void tp_flux(const TPArgs& A, const TPGas& g, const TPCellT<typename MA::R, NSP>& Lf,
+                                        const TPCellT<typename MA::R, NSP>& Rf, bool shock, typename MA::R* F)
 {
+    using R = typename MA::R;
+    using MD = typename std::conditional<MA::kFast, MA, ModeExact>::type;
     if constexpr (FLUX == FLUX_HLLC) {
         tp_hllc<MA, NSP>(Lf, Rf, F);
     } else if constexpr (FLUX == FLUX_SHOCK) {
-        const double sound_speed = 0.5 * (Lf.a + Rf.a);
-        const double shear_y = dv<MA>(fabs(Lf.v[1] - Rf.v[1]), sound_speed);
-        const double shear_z = dv<MA>(fabs(Lf.v[2] - Rf.v[2]), sound_speed);
-        const bool shear_is_small = fmax(shear_y, shear_z) <= A.shear_tol;
+        const R sound_speed = R(0.5) * (Lf.a + Rf.a);
+        const R shear_y = dv<MD>(ab(Lf.v[1] - Rf.v[1]), sound_speed);
+        const R shear_z = dv<MD>(ab(Lf.v[2] - Rf.v[2]), sound_speed);
+        const bool shear_is_small = mx(shear_y, shear_z) <= R(A.shear_tol);
         if (A.always_hanel || (shock && shear_is_small)) {
             if (A.shock_efm) tp_efm<MA, NSP>(g, Lf, Rf, F);
             else tp_hanel<MA, NSP>(Lf, Rf, F);
@@ -451,15 +489,38 @@ __device__ __forceinline__ void tp_flux(const TPArgs& A, const TPGas& g, const T
     }
 }
 
+// The arguments of a batched mixture launch's block entry e: the table of
+// TPArgs in device memory read through a uniform, read-only pointer (the
+// address-space cast is undone by inlining, so the reads stay scalar loads).
+#if defined(__HIP_DEVICE_COMPILE__)
+using TPKArgs = const __attribute__((address_space(4))) TPArgs*;
+using TPKBatch = const __attribute__((address_space(4))) BatchArgs*;
+__device__ __forceinline__ const TPArgs& tp_batch_args(int& entry)
+{
+    TPKBatch b = (TPKBatch)__builtin_amdgcn_kernarg_segment_ptr();
+    entry = __builtin_amdgcn_readfirstlane(b->wg_entry[blockIdx.x]);
+    TPKArgs p = (TPKArgs)(b->table + (unsigned long long)entry * sizeof(TPArgs));
+    return *(const TPArgs*)p;
+}
+#else
+__device__ __forceinline__ const TPArgs& tp_batch_args(int& entry)
+{
+    entry = 0;
+    return *(const TPArgs*)nullptr;
+}
+#endif
+
+// The face kernel's work for face f0 of direction DIR of one block.
 template <class M, int NSP, int FLUX, int DIR>
-__global__ __launch_bounds__(128) void tp_face_kernel(TPArgs A)
+__device__ __forceinline__ void tp_face_body(const TPArgs& A, long f0)
 {
     using MA = typename TPArith<M>::type;
+    using MF = typename TPFlux<M>::type;
+    using R = typename MF::R;
     constexpr bool kPolish = TPArith<M>::polish;
     if (stage_aborted(A.abort_prev)) return;
     const int nfi = A.ni + (DIR == 0), nfj = A.nj + (DIR == 1), nfk = (A.dims == 3) ? A.nk + (DIR == 2) : 1;
     const long nf = (long)nfi * nfj * nfk;
-    const long f0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = f0 < nf;
     const long f = active ? f0 : nf - 1;   // idle lanes compute the last face and store nothing
     const int i = (int)(f % nfi);
@@ -474,33 +535,33 @@ __global__ __launch_bounds__(128) void tp_face_kernel(TPArgs A)
     TPRef r[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) r[m] = tp_ref(A, i + (m - 2) * di, j + (m - 2) * dj, k + (m - 2) * dk);
-    TPCell<NSP> Lf, Rf;
+    TPCellT<R, NSP> Lf, Rf;
     int fe = 0, fb = 0;
     bool okL = false, okR = false;
     if (A.high_order) {
-        const L2R2W& w = A.w[DIR];
+        const L2R2WT<R> w = cast_w<R>(A.w[DIR]);
         const bool lim = A.apply_limiter, clip = A.extrema_clipping;
         const int comp[3] = {pn, pt1, pt2};
 #pragma unroll
         for (int d = 0; d < 3; ++d)
-            interp_l2r2_scalar<MA>(w, lim, clip, tp_vel(r[0], comp[d]), tp_vel(r[1], comp[d]), tp_vel(r[2], comp[d]),
-                                   tp_vel(r[3], comp[d]), Lf.v[d], Rf.v[d]);
+            interp_l2r2_scalar<MF>(w, lim, clip, R(tp_vel(r[0], comp[d])), R(tp_vel(r[1], comp[d])),
+                                   R(tp_vel(r[2], comp[d])), R(tp_vel(r[3], comp[d])), Lf.v[d], Rf.v[d]);
 #pragma unroll
         for (int s = 0; s < NSP; ++s)
-            interp_l2r2_scalar<MA>(w, lim, clip, tp_val(r[0], P_Y + s), tp_val(r[1], P_Y + s), tp_val(r[2], P_Y + s),
-                                   tp_val(r[3], P_Y + s), Lf.Y[s], Rf.Y[s]);
-        if (scale_mass_fractions<MA, NSP>(Lf.Y) != 0) {
+            interp_l2r2_scalar<MF>(w, lim, clip, R(tp_val(r[0], P_Y + s)), R(tp_val(r[1], P_Y + s)),
+                                   R(tp_val(r[2], P_Y + s)), R(tp_val(r[3], P_Y + s)), Lf.Y[s], Rf.Y[s]);
+        if (scale_mass_fractions<MF, NSP>(Lf.Y) != 0) {
 #pragma unroll
-            for (int s = 0; s < NSP; ++s) Lf.Y[s] = tp_val(r[1], P_Y + s);
+            for (int s = 0; s < NSP; ++s) Lf.Y[s] = R(tp_val(r[1], P_Y + s));
         }
-        if (scale_mass_fractions<MA, NSP>(Rf.Y) != 0) {
+        if (scale_mass_fractions<MF, NSP>(Rf.Y) != 0) {
 #pragma unroll
-            for (int s = 0; s < NSP; ++s) Rf.Y[s] = tp_val(r[2], P_Y + s);
+            for (int s = 0; s < NSP; ++s) Rf.Y[s] = R(tp_val(r[2], P_Y + s));
         }
-        interp_l2r2_scalar<MA>(w, lim, clip, tp_val(r[0], P_RHO), tp_val(r[1], P_RHO), tp_val(r[2], P_RHO),
-                               tp_val(r[3], P_RHO), Lf.rho, Rf.rho);
-        interp_l2r2_scalar<MA>(w, lim, clip, tp_val(r[0], P_U), tp_val(r[1], P_U), tp_val(r[2], P_U),
-                               tp_val(r[3], P_U), Lf.u, Rf.u);
+        interp_l2r2_scalar<MF>(w, lim, clip, R(tp_val(r[0], P_RHO)), R(tp_val(r[1], P_RHO)), R(tp_val(r[2], P_RHO)),
+                               R(tp_val(r[3], P_RHO)), Lf.rho, Rf.rho);
+        interp_l2r2_scalar<MF>(w, lim, clip, R(tp_val(r[0], P_U)), R(tp_val(r[1], P_U)), R(tp_val(r[2], P_U)),
+                               R(tp_val(r[3], P_U)), Lf.u, Rf.u);
         // update_thermo_from_rhou on each side (guess: the side's cell T); on
         // failure the side is the cell's state (onedinterp.d:1039-1068); the
         // sound speed is the cell's (sfluidblock.d:2199-2208).  The two sides
@@ -508,24 +569,29 @@ __global__ __launch_bounds__(128) void tp_face_kernel(TPArgs A)
 #pragma nounroll
         for (int sd = 0; sd < 2; ++sd) {
             const TPRef rc = sd ? r[2] : r[1];
+            // the temperature solve in FP64 from the (FP32 in the mixed form)
+            // reconstructed density, energy and mass fractions
             TPState<NSP> Q;
-            Q.rho = sd ? Rf.rho : Lf.rho;
-            Q.u = sd ? Rf.u : Lf.u;
+            Q.rho = double(sd ? Rf.rho : Lf.rho);
+            Q.u = double(sd ? Rf.u : Lf.u);
             Q.T = tp_val(rc, P_T);
 #pragma unroll
-            for (int s = 0; s < NSP; ++s) Q.Y[s] = sd ? Rf.Y[s] : Lf.Y[s];
+            for (int s = 0; s < NSP; ++s) Q.Y[s] = double(sd ? Rf.Y[s] : Lf.Y[s]);
             if (tp_face_eos<MA, NSP, kPolish>(*A.gas, Q, &fe, &fb) == 0) {
-                const double a = tp_val(rc, P_A);
-                if (sd == 0) { Lf.T = Q.T; Lf.u = Q.u; Lf.p = Q.p; Lf.a = a; okL = true; }
-                else { Rf.T = Q.T; Rf.u = Q.u; Rf.p = Q.p; Rf.a = a; okR = true; }
+                const R a = R(tp_val(rc, P_A));
+                if (sd == 0) { Lf.T = R(Q.T); Lf.u = R(Q.u); Lf.p = R(Q.p); Lf.a = a; okL = true; }
+                else { Rf.T = R(Q.T); Rf.u = R(Q.u); Rf.p = R(Q.p); Rf.a = a; okR = true; }
             }
         }
     }
-    if (!okL) Lf = tp_cell<NSP>(r[1], pn, pt1, pt2);
-    if (!okR) Rf = tp_cell<NSP>(r[2], pn, pt1, pt2);
-    double F[5 + NSP];
+    if (!okL) Lf = tp_cell<NSP, R>(r[1], pn, pt1, pt2);
+    if (!okR) Rf = tp_cell<NSP, R>(r[2], pn, pt1, pt2);
+    R Fr[5 + NSP];
     const bool shock = (FLUX == FLUX_SHOCK) && ((r[1].s && *r[1].s) || (r[2].s && *r[2].s));
-    tp_flux<MA, NSP, FLUX>(A, *A.gas, Lf, Rf, shock, F);
+    tp_flux<MF, NSP, FLUX>(A, *A.gas, Lf, Rf, shock, Fr);
+    double F[5 + NSP];
+#pragma unroll
+    for (int q = 0; q < 5 + NSP; ++q) F[q] = double(Fr[q]);
     if (active) {
         double* dst = A.F[DIR];
         const long fq = A.fqs[DIR];
@@ -548,10 +614,24 @@ __global__ __launch_bounds__(128) void tp_face_kernel(TPArgs A)
     }
 }
 
+// One thread per face of direction DIR; BATCH: a launch over several
+// blocks, each workgroup in the block its map entry names.
+template <class M, int NSP, int FLUX, int DIR, bool BATCH>
+__global__ __launch_bounds__(128) void tp_face_kernel(typename std::conditional<BATCH, BatchArgs, TPArgs>::type a)
+{
+    if constexpr (BATCH) {
+        int e;
+        const TPArgs& A = tp_batch_args(e);
+        tp_face_body<M, NSP, FLUX, DIR>(A, (long)((int)blockIdx.x - A.wg0[DIR]) * blockDim.x + threadIdx.x);
+    } else {
+        tp_face_body<M, NSP, FLUX, DIR>(a, (long)blockIdx.x * blockDim.x + threadIdx.x);
+    }
+}
+
 // GUESS: A.guess is installed (the learned temperature guess; its LDS staging
 // and MLP code exist only in that instantiation).
 template <class M, int NSP, bool GUESS>
-__global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
+__device__ __forceinline__ void tp_update_body(const TPArgs& A, long c)
 {
     using MA = typename TPArith<M>::type;
     constexpr bool kPolish = TPArith<M>::polish;
@@ -559,7 +639,6 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
     constexpr int NQ = 5 + NSP;
     if (stage_aborted(A.abort_prev)) return;
     const long ncells = (long)A.ni * A.nj * A.nk;
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = c < ncells;
     const long qs = A.qs;
     bool invalid = false, decode_fail = false, fell_back = false;
@@ -731,6 +810,18 @@ __global__ __launch_bounds__(128) void tp_update_kernel(TPArgs A)
         count_bad_cells(A.counters, A.abort_this, A.max_invalid, nbad, nfail);
         if (fsum_w) atomicAdd(&A.counters[2], (unsigned int)fsum_w);
         if (nfb) atomicAdd(&A.counters[3], nfb);
+    }
+}
+
+template <class M, int NSP, bool GUESS, bool BATCH>
+__global__ __launch_bounds__(128) void tp_update_kernel(typename std::conditional<BATCH, BatchArgs, TPArgs>::type a)
+{
+    if constexpr (BATCH) {
+        int e;
+        const TPArgs& A = tp_batch_args(e);
+        tp_update_body<M, NSP, GUESS>(A, (long)((int)blockIdx.x - A.wg0[3]) * blockDim.x + threadIdx.x);
+    } else {
+        tp_update_body<M, NSP, GUESS>(a, (long)blockIdx.x * blockDim.x + threadIdx.x);
     }
 }
 
@@ -1088,9 +1179,9 @@ void launch_tp_faces(const TPArgs& A, hipStream_t stream)
     const long nf2 = (long)A.ni * A.nj * (A.nk + 1);
     const dim3 b(128);
     auto g = [](long n) { return dim3((unsigned)((n + 127) / 128)); };
-    hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX, 0>), g(nf0), b, 0, stream, A);
-    hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX, 1>), g(nf1), b, 0, stream, A);
-    if (A.dims == 3) hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX, 2>), g(nf2), b, 0, stream, A);
+    hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX, 0, false>), g(nf0), b, 0, stream, A);
+    hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX, 1, false>), g(nf1), b, 0, stream, A);
+    if (A.dims == 3) hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX, 2, false>), g(nf2), b, 0, stream, A);
 }
 
 template <class M>
@@ -1109,8 +1200,43 @@ void launch_tp_stage_m(const TPArgs& A, hipStream_t stream, int flux)
     }
     const long nc = (long)A.ni * A.nj * ((A.dims == 3) ? A.nk : 1);
     const dim3 grid((unsigned)((nc + 127) / 128)), b(128);
-    if (A.guess) hipLaunchKernelGGL((tp_update_kernel<M, 5, true>), grid, b, 0, stream, A);
-    else hipLaunchKernelGGL((tp_update_kernel<M, 5, false>), grid, b, 0, stream, A);
+    if (A.guess) hipLaunchKernelGGL((tp_update_kernel<M, 5, true, false>), grid, b, 0, stream, A);
+    else hipLaunchKernelGGL((tp_update_kernel<M, 5, false, false>), grid, b, 0, stream, A);
+}
+
+// The stage of several mixture blocks of one dimensionality in one launch per
+// kernel: the face kernels of each direction over all blocks' faces, then each
+// block's boundary-face kernel (when it has faces to recompute; `per_block`
+// holds their arguments), then the update over all blocks' cells.  `wg`
+// holds each launch's workgroup count and `B[k]` its map (k = direction,
+// 3 = update).
+template <class M, int FLUX>
+void launch_tp_batch_faces(const BatchArgs* B, const int* wg, int dims, hipStream_t stream)
+{
+    const dim3 b(128);
+    hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX, 0, true>), dim3(wg[0]), b, 0, stream, B[0]);
+    hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX, 1, true>), dim3(wg[1]), b, 0, stream, B[1]);
+    if (dims == 3) hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX, 2, true>), dim3(wg[2]), b, 0, stream, B[2]);
+}
+
+template <class M>
+void launch_tp_stage_batch_m(const BatchArgs* B, const int* wg, int dims, bool guess, const TPArgs* per_block,
+                             int nper, hipStream_t stream, int flux)
+{
+    if (flux == FLUX_HLLC) launch_tp_batch_faces<M, FLUX_HLLC>(B, wg, dims, stream);
+    else if (flux == FLUX_SHOCK) launch_tp_batch_faces<M, FLUX_SHOCK>(B, wg, dims, stream);
+    else if (flux == FLUX_OTHER) launch_tp_batch_faces<M, FLUX_OTHER>(B, wg, dims, stream);
+    else launch_tp_batch_faces<M, FLUX_AUSMDV>(B, wg, dims, stream);
+    for (int n = 0; n < nper; ++n) {
+        const TPArgs& A = per_block[n];
+        const dim3 g((unsigned)((A.fix_off[6] + 63) / 64)), b(64);
+        if (flux == FLUX_HLLC) hipLaunchKernelGGL((tp_boundary_face_kernel<M, 5, FLUX_HLLC>), g, b, 0, stream, A);
+        else if (flux == FLUX_SHOCK) hipLaunchKernelGGL((tp_boundary_face_kernel<M, 5, FLUX_SHOCK>), g, b, 0, stream, A);
+        else if (flux == FLUX_OTHER) hipLaunchKernelGGL((tp_boundary_face_kernel<M, 5, FLUX_OTHER>), g, b, 0, stream, A);
+        else hipLaunchKernelGGL((tp_boundary_face_kernel<M, 5, FLUX_AUSMDV>), g, b, 0, stream, A);
+    }
+    if (guess) hipLaunchKernelGGL((tp_update_kernel<M, 5, true, true>), dim3(wg[3]), dim3(128), 0, stream, B[3]);
+    else hipLaunchKernelGGL((tp_update_kernel<M, 5, false, true>), dim3(wg[3]), dim3(128), 0, stream, B[3]);
 }
 
 template <class M>

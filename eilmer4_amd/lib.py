@@ -24,6 +24,7 @@ EXPORTS = [
     "e4hip_explicit_increment", "e4hip_determine_dt", "e4hip_profile_enable", "e4hip_profile_read",
     "e4hip_gas_tp_set_guess_model", "e4hip_tp_guess", "e4hip_tp_decode_stats", "e4hip_tp_face_stats",
     "e4hip_mlp_grad", "e4hip_mlp_train", "e4hip_block_kchunk", "e4hip_determine_dt_noalloc",
+    "e4hip_last_step_bad_cells",
 ]
 
 
@@ -79,6 +80,7 @@ def load(path=LIB_PATH):
     L.e4hip_explicit_increment.argtypes = [_ip, C.c_int, C.c_double, _ip]
     L.e4hip_determine_dt.argtypes = [_ip, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
     L.e4hip_determine_dt_noalloc.argtypes = [_ip, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
+    L.e4hip_last_step_bad_cells.argtypes = [C.c_int, _ip]
     L.e4hip_profile_enable.argtypes = [C.c_int]
     L.e4hip_profile_read.argtypes = [C.c_int, _dp, C.POINTER(C.c_long)]
     _fp = C.POINTER(C.c_float)

@@ -80,6 +80,7 @@ class Simulation:
         self.apply_config()
         self.state = SimState(self.cfg.dt_init)
         self.last_bad_cells = [0, 0, 0]
+        self.last_bad_cells_per_block = []
 
     def apply_config(self):
         sc = self.cfg.step_config()
@@ -113,9 +114,17 @@ class Simulation:
     def gasdynamic_explicit_increment_with_fixed_grid(self):
         ids, n = self._ids()
         nbad = (C.c_int * 3)()
-        rc = _lib.load().e4hip_explicit_increment(ids, n, self.state.dt_global, nbad)
+        L = _lib.load()
+        rc = L.e4hip_explicit_increment(ids, n, self.state.dt_global, nbad)
         self.last_bad_cells = list(nbad)
+        per = (C.c_int * (3 * n))()
+        if L.e4hip_last_step_bad_cells(n, per) == 0:
+            self.last_bad_cells_per_block = [list(per[3 * b:3 * b + 3]) for b in range(n)]
         if rc == -4:
+            for b, counts in enumerate(self.last_bad_cells_per_block):
+                for s, c in enumerate(counts):
+                    if c > self.cfg.max_invalid_cells:
+                        print("Following stage-%d gasdynamic update: %d bad cells in block[%d]." % (s + 1, c, b))
             raise FlowSolverException("Too many bad cells; go home.")
         if rc == -5:
             raise FlowSolverException("Bad cell with negative mass.")

@@ -288,6 +288,13 @@ int e4hip_config(const E4hipStepConfig* c);
  * gives E4HIP_ERR_DECODE the same way.  Either way U[0] (and a mixture's flow
  * state) is left as it was before the call, and the caller ends the run. */
 int e4hip_explicit_increment(const int* blks, int nblk, double dt, int* n_bad_cells);
+/* The same counts per block: counts[n][s] for the n-th block of the last
+ * e4hip_explicit_increment call (nblk must be that call's) and stage s < 3,
+ * 0 for stages that did not run -- the per-block figures the reference prints
+ * before it throws (simcore.d:2480-2487).  The caller advances the simulation
+ * time (SimState.time = t0 + dt, simcore.d:2951) after a successful call: no
+ * kernel of the path depends on t. */
+int e4hip_last_step_bad_cells(int nblk, int* counts);
 
 /* determine_time_step_size for the listed blocks (fvcell.d:1624-1705;
  * fluidblock.d:776-862): the CFL signal of each cell is the non-stringent one

@@ -149,3 +149,40 @@ def test_stopped_step_keeps_mixture_state(gpu, precision):
     rc, nbad = device_step([db], 0.5 * dt)
     assert rc == 0 and nbad == [0, 0, 0]
     db.destroy()
+
+
+def test_bad_cells_per_block_vs_oracle(gpu):
+    """Three unconnected blocks in one step (a benign one, two cold expansions
+    of different heights): the step stops after stage 1 and the per-block
+    counts (e4hip_last_step_bad_cells) equal each block's own count in the
+    oracle; the later stages report 0, and the Simulation raises the
+    reference's exception."""
+    print("case badcells: per-block counts, 3 blocks", flush=True)
+    gas = e4.IdealGas()
+    cases = [expansion(300.0, 10.0), expansion(1.05, 10.0), expansion(1.05, 10.0, nj=8)]
+    U0s = [workloads.initial_U(gas, f) for _, f, _ in cases]
+    obs = [oracle_block(g, b, U0, flux=O.AUSMDV) for (g, _, b), U0 in zip(cases, U0s)]
+    dt = 2.5 * obs[1].determine_dt(1e-3, 0.5, False)[0]
+    expect = []
+    for ob in obs:
+        rc, nbad = ob.step(dt)
+        expect.append(nbad[0])
+    print("  oracle stage-1 counts per block %s" % expect, flush=True)
+    assert expect[0] == 0 and expect[1] > 0 and expect[2] > expect[1]
+    blocks = [device_block(gas, g, b, U0) for (g, _, b), U0 in zip(cases, U0s)]
+    sim = device_sim(blocks, flux=e4.FluxCalculator.ausmdv, precision="fp64_exact")
+    sim.cfg.fixed_time_step = True
+    sim.state.dt_global = dt
+    with pytest.raises(e4.FlowSolverException):
+        sim.gasdynamic_explicit_increment_with_fixed_grid()
+    got = sim.last_bad_cells_per_block
+    print("  device per block %s" % got, flush=True)
+    assert [c[0] for c in got] == expect
+    assert all(c[1] == 0 and c[2] == 0 for c in got)
+    assert sim.last_bad_cells == [sum(expect), 0, 0]
+    L = e4.load()
+    wrong = (C.c_int * 6)()
+    assert L.e4hip_last_step_bad_cells(2, wrong) == -1      # E4HIP_ERR_ARG: not the last call's block count
+    for b, U0 in zip(blocks, U0s):
+        assert np.array_equal(b.download(), U0)
+        b.destroy()

@@ -171,7 +171,7 @@ def test_boundaries_thermally_perfect(gpu, flux, precision):
     Cp/Cv of the mixture state, and the one-sided faces after them), simple
     outflow flux, and x_order-1 ghosts (stored per face, T by the EOS from the
     previous ghost T).  Bar: 1e-10 (the logarithm differs between the device
-    and host libraries); mixed: density 1e-8."""
+    and host libraries); mixed (FP32 reconstruction and flux): density 1e-5."""
     from test_gpu_tp import setup_pair
     bcs = {e4.Face.west: (BCType.slip_wall_1, None), e4.Face.east: (BCType.outflow_simple_flux, None),
            e4.Face.north: (BCType.extrapolate_1, None), e4.Face.south: (BCType.slip_wall_1, None),
@@ -187,7 +187,7 @@ def test_boundaries_thermally_perfect(gpu, flux, precision):
     print("  U rel err per quantity", err, flush=True)
     assert np.all(np.isfinite(err))
     if precision == "mixed":
-        assert err[0] <= 1.0e-8
+        assert err[0] <= 1.0e-5
     else:
         assert err.max() <= 1.0e-10
     db.destroy()
