@@ -114,8 +114,11 @@ struct StepGraph {
     unsigned long long last_use = 0;
 };
 
-// Words before the block-stage counter slots: the abort flag of each stage.
-constexpr int ABORT_WORDS = 8;
+// Words before the block-stage counter slots: the abort flag of stage s at
+// [32 (s-1)], each in a 128-byte line of its own, so the per-wave reads of
+// the previous stage's flag never meet the atomics on the counters.
+constexpr int ABORT_WORDS = 96;
+constexpr int ABORT_STRIDE = 32;
 // pointers per block in a step's snapshot (U[4], d[2], P[4], S[2])
 constexpr int PTRS_PER_BLOCK = 12;
 
@@ -124,7 +127,7 @@ struct State {
     int device = 0;
     hipStream_t stream = nullptr;
     unsigned int* counters_raw = nullptr;      // ABORT_WORDS abort flags, then the counter slots
-    unsigned int* abort = nullptr;             // = counters_raw: abort flag of stage s at [s-1]
+    unsigned int* abort = nullptr;             // = counters_raw: abort flag of stage s at [ABORT_STRIDE (s-1)]
     unsigned int* counters = nullptr;          // per-stage counters, e4::COUNTER_SLOT uints per block-stage slot
     unsigned int* counters_host = nullptr;     // pinned host copy the step reads the flags and counters back into
     unsigned int* step_abort = nullptr;        // abort flags of the step being enqueued
@@ -430,8 +433,8 @@ template <class Args>
 void abort_fields(int stage, Args& A)
 {
     unsigned int* ab = g_state.step_abort;
-    A.abort_prev = (ab && stage > 1) ? ab + (stage - 2) : nullptr;
-    A.abort_this = ab ? ab + (stage - 1) : nullptr;
+    A.abort_prev = (ab && stage > 1) ? ab + ABORT_STRIDE * (stage - 2) : nullptr;
+    A.abort_this = ab ? ab + ABORT_STRIDE * (stage - 1) : nullptr;
     A.max_invalid = (unsigned int)g_state.cfg.max_invalid_cells;
 }
 
@@ -2035,7 +2038,7 @@ static int step_outcome(const unsigned int* h, int nblk, int ns, int* n_bad_cell
             }
             if (dec) decode_fail = true;
         }
-        if (too_many || decode_fail || h[s]) *stopped = true;
+        if (too_many || decode_fail || h[ABORT_STRIDE * s]) *stopped = true;
     }
     if (n_bad_cells) for (int s = 0; s < 3; ++s) n_bad_cells[s] = stage_bad[s];
     if (decode_fail) return fail(E4HIP_ERR_DECODE, "Bad cell with negative mass.");

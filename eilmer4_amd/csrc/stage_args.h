@@ -144,13 +144,15 @@ struct StageArgs {
     unsigned int max_invalid;
 };
 
-// Whether the stage before this one stopped the step: the word is read with
-// an agent-scope load (the stage before wrote it with device atomics), once
-// per wave.
+// Whether the stage before this one stopped the step, read once per wave.
+// Only the kernels of that stage write the word, and they have finished when
+// a kernel of this stage starts (same stream), so a plain load sees it; the
+// word sits alone in its cache line, away from the counters this stage's
+// waves add to, so the load never queues behind their atomics.
 __device__ __forceinline__ bool stage_aborted(const unsigned int* abort_prev)
 {
     if (!abort_prev) return false;
-    const unsigned int v = __hip_atomic_load(abort_prev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int v = *abort_prev;
     return __builtin_amdgcn_readfirstlane(v) != 0u;
 }
 
@@ -195,8 +197,10 @@ struct BatchArgs {
 // [2] decode energy evaluations, [3] decodes whose polish (from the learned
 // guess or the stored temperature) fell back to the reference solve,
 // [4] face energy evaluations, [5] face-state solves whose polish fell back
-// (thermally perfect path).
-constexpr int COUNTER_SLOT = 8;
+// (thermally perfect path).  A slot is 64 bytes: the slots that one stage's
+// launches add to (the same stage of different blocks, three slots apart)
+// never share a 128-byte cache line.
+constexpr int COUNTER_SLOT = 16;
 
 enum { P_RHO = 0, P_VX = 1, P_VY = 2, P_VZ = 3, P_U = 4, P_T = 5, P_P = 6, P_A = 7, P_Y = 8 };
 
