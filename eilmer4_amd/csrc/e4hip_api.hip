@@ -63,6 +63,7 @@ struct BlockH {
     double* P[4] = {nullptr, nullptr, nullptr, nullptr};
     double* F[3] = {nullptr, nullptr, nullptr};
     long fqs[3] = {0, 0, 0};
+    unsigned int* retry = nullptr;                // faces the mixed form hands on (e4::TPArgs::retry)
     double bc_fs_tp[6][8 + e4::TP_NSP_MAX] = {};
     // stored ghost cells per face (x_order-1 extrapolation of a mixture, static
     // profiles): [8+nsp][2 * face cells]; gS the profile's shock indicators
@@ -937,6 +938,10 @@ int tp_args_for(const BlockH& b, int stage, double dt, unsigned int* counters, e
     for (int f = 0; f < 6; ++f) { A.gx[f] = b.gx[f]; A.gS[f] = b.gS[f]; }
     A.axi = b.axi;
     A.src = b.src;
+    const bool lean = b.is_tp() && !b.general && cfg.precision_mode == E4HIP_PRECISION_MIXED &&
+                      tp_mixture_form(g_state.gases[b.gm].tp);
+    A.retry = lean ? b.retry : nullptr;
+    if (lean && !b.retry) return fail(E4HIP_ERR_ARG, "mixture block without its retry list");
     return shock_fields(b, A);
 }
 
@@ -1033,13 +1038,13 @@ int launch_tp_stage_batch(const std::vector<BlockH*>& blks, const std::vector<un
     hipEvent_t ev = nullptr;
     prof_begin(&ev);
     if (tp_exact_form(*blks[0]))
-        e4::launch_tp_stage_batch_exact(B, wg, dims, guess, per_block.data(), (int)per_block.size(), g_state.stream,
+        e4::launch_tp_stage_batch_exact(B, wg, dims, guess, per_block.data(), (int)per_block.size(), n, g_state.stream,
                                         flux);
     else if (cfg.precision_mode == E4HIP_PRECISION_MIXED)
-        e4::launch_tp_stage_batch_mixed(B, wg, dims, guess, per_block.data(), (int)per_block.size(), g_state.stream,
+        e4::launch_tp_stage_batch_mixed(B, wg, dims, guess, per_block.data(), (int)per_block.size(), n, g_state.stream,
                                         flux);
     else
-        e4::launch_tp_stage_batch_fast64(B, wg, dims, guess, per_block.data(), (int)per_block.size(), g_state.stream,
+        e4::launch_tp_stage_batch_fast64(B, wg, dims, guess, per_block.data(), (int)per_block.size(), n, g_state.stream,
                                          flux);
     prof_end(0, ev);
     HIP_TRY(hipGetLastError());
@@ -1092,6 +1097,11 @@ int alloc_block_buffers(BlockH& b)
             b.fqs[dd] = (nf[dd] + 63) / 64 * 64;
             if (!b.F[dd]) HIP_TRY(hipMalloc(&b.F[dd], sizeof(double) * b.fqs[dd] * b.nq));
         }
+        if (b.is_tp() && !b.general && !b.retry) {
+            const size_t n = 4 + (size_t)nf[0] + nf[1] + ((b.dims == 3) ? nf[2] : 0);
+            HIP_TRY(hipMalloc(&b.retry, sizeof(unsigned int) * n));
+            HIP_TRY(hipMemset(b.retry, 0, sizeof(unsigned int) * 4));
+        }
     }
     if (!b.bc_dev) HIP_TRY(hipMalloc(&b.bc_dev, sizeof(e4::BCTable)));
     for (int t = 0; t < 2; ++t) {
@@ -1110,6 +1120,7 @@ int free_block_buffers(BlockH& b)
     for (auto& p : b.d) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     for (auto& p : b.P) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     for (auto& p : b.F) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
+    if (b.retry) { HIP_TRY(hipFree(b.retry)); b.retry = nullptr; }
     for (auto& p : b.S) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     for (auto& p : b.gx) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
     for (auto& p : b.gS) if (p) { HIP_TRY(hipFree(p)); p = nullptr; }
@@ -1271,7 +1282,10 @@ int e4hip_gas_tp(int nsp, const double* M, int nseg, const double* Tbreaks, cons
         cv.R = g.tp.R[s];
         cv.nseg = nseg;
         for (int q = 0; q <= nseg; ++q) cv.Tb[q] = Tbreaks[s * (nseg + 1) + q];
-        for (int q = 0; q < nseg - 1; ++q) cv.Tbl[q] = Tblends[s * (nseg - 1) + q];
+        for (int q = 0; q < nseg - 1; ++q) {
+            cv.Tbl[q] = Tblends[s * (nseg - 1) + q];
+            cv.rTbl[q] = (cv.Tbl[q] > 0.0) ? 1.0 / cv.Tbl[q] : 0.0;
+        }
         for (int q = 0; q < nseg; ++q)
             for (int j = 0; j < 9; ++j) cv.c[q][j] = coeffs[(s * nseg + q) * 9 + j];
     }

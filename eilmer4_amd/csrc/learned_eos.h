@@ -1,7 +1,7 @@
 // learned_eos.h -- the learned initial guess of the thermally perfect
 // temperature solve: a 7 -> 64 -> 64 -> 64 -> 1 SiLU network with BF16 weights,
 // evaluated for the 64 cells of a wave with v_mfma_f32_16x16x32_bf16 (FP32
-// accumulation), followed by an FP64 Newton polish judged by its energy
+// accumulation), followed by an FP64 Newton polish accepted by its energy
 // residual.
 //
 // Operand layout of v_mfma_f32_16x16x32_bf16 (wave64): lane l supplies
@@ -90,20 +90,25 @@ __device__ __forceinline__ float mlp_guess_wave(const GuessModel& G, const doubl
 
 // FP64 polish from a temperature guess (the learned guess, or the stored or
 // the face's cell temperature): two iterations without bracketing, then the
-// judge: the energy residual at the result, divided by the derivative there,
-// estimates the remaining temperature error, and the polish is accepted when
-// that estimate is at most 1e-11 T.  Q.u is left at the energy of the accepted
-// temperature.  Returns 0 when accepted, -1 to fall back to the reference's
+// acceptance test: the energy residual at the result, divided by the
+// derivative there, estimates the remaining temperature error, and the polish
+// is accepted when that estimate is at most 1e-11 T.  Q.u is left at the
+// energy of the accepted temperature (at the target energy after an early
+// acceptance, below).  Returns 0 when accepted, -1 to fall back to the reference's
 // bracketed solve.
 //
 // Fast modes (which require curves with shared breakpoints): Halley steps
 // (cubic convergence) on the exact u, du/dT and d2u/dT2 of the blended
 // curves, the species sums of every segment formed once per solve.  From a face's cell
 // temperature (typically tens to a few hundred kelvin from the face root) two
-// Halley steps reach the judge's bound; Newton would need three.  ModeExact:
+// Halley steps reach the acceptance bound; Newton would need three.  A step s
+// with |s| <= 1e-4 T that stays on one smooth piece of the curve is accepted
+// without evaluating its result: the cubic convergence leaves the new
+// temperature within about |s|^3 / T^2 <= 1e-12 T of the root, inside the
+// bound the residual test accepts.  ModeExact:
 // Newton steps with the reference's dzdT, sum_s Y_s (Cp_s - R_s), which inside a blend is not the
 // derivative of the blended energy (convergence there is only linear, hence a
-// judge on the residual rather than on the size of the last step).
+// test on the residual rather than on the size of the last step).
 template <class M, int NSP>
 __device__ __forceinline__ int tp_polish_x(const TPGas& g, const TPMix& X, double Rmix, TPState<NSP>& Q, double e_tgt,
                                            double T0, int* fevals)
@@ -125,7 +130,15 @@ __device__ __forceinline__ int tp_polish_x(const TPGas& g, const TPMix& X, doubl
                 Q.u = u;
                 return 0;
             }
-            T -= dv<MD>(2.0 * f * du, 2.0 * du * du - f * d2u);
+            const double s = dv<MD>(2.0 * f * du, 2.0 * du * du - f * d2u);
+            const double Tn = T - s;
+            if (fabs(s) <= 1.0e-4 * Tn && Tn >= 20.0 && Tn <= 100000.0 &&
+                tp_curve_piece(g.curve[0], Tn) == tp_curve_piece(g.curve[0], T)) {
+                Q.T = Tn;
+                Q.u = e_tgt;
+                return 0;
+            }
+            T = Tn;
         }
     } else {
         // two Newton steps, then the energy at the result (one evaluation site)

@@ -264,6 +264,12 @@ struct TPArgs {
     const double* fgeo[3];
     const double* nbrGeo[6];
     IdealGas ig;                      // ideal-gas constants (ideal-gas blocks built from their grid)
+    // mixed form: the faces whose short temperature solve did not settle,
+    // handed by the face kernels to the retry kernel -- per direction d a
+    // count at [d] and the face indices from [4 + first face slot of d]
+    // (slots of direction 0, then 1, then 2); the update kernel clears the
+    // counts.  Null in the other forms.
+    unsigned int* retry;
 };
 
 // Launchers, one per arithmetic mode; each lives in its own translation unit
@@ -287,11 +293,11 @@ void launch_stage_batch_mixed(const BatchArgs& B, int nwg, bool def, hipStream_t
 // update; decode of an uploaded state
 void launch_tp_stage_exact(const TPArgs& A, hipStream_t stream, int flux);
 void launch_tp_stage_batch_exact(const BatchArgs* B, const int* wg, int dims, bool guess, const TPArgs* per_block,
-                                 int nper, hipStream_t stream, int flux);
+                                 int nper, int nblk, hipStream_t stream, int flux);
 void launch_tp_stage_batch_fast64(const BatchArgs* B, const int* wg, int dims, bool guess, const TPArgs* per_block,
-                                  int nper, hipStream_t stream, int flux);
+                                  int nper, int nblk, hipStream_t stream, int flux);
 void launch_tp_stage_batch_mixed(const BatchArgs* B, const int* wg, int dims, bool guess, const TPArgs* per_block,
-                                 int nper, hipStream_t stream, int flux);
+                                 int nper, int nblk, hipStream_t stream, int flux);
 void launch_tp_stage_fast64(const TPArgs& A, hipStream_t stream, int flux);
 void launch_tp_stage_mixed(const TPArgs& A, hipStream_t stream, int flux);
 // shock indicators of a stage's stencil state (adaptive_hanel_ausmdv); the

@@ -10,9 +10,9 @@ void launch_tp_stage_exact(const TPArgs& A, hipStream_t stream, int flux)
 }
 
 void launch_tp_stage_batch_exact(const BatchArgs* B, const int* wg, int dims, bool guess, const TPArgs* per_block,
-                                 int nper, hipStream_t stream, int flux)
+                                 int nper, int nblk, hipStream_t stream, int flux)
 {
-    launch_tp_stage_batch_m<ModeExact>(B, wg, dims, guess, per_block, nper, stream, flux);
+    launch_tp_stage_batch_m<ModeExact>(B, wg, dims, guess, per_block, nper, nblk, stream, flux);
 }
 
 void launch_tp_shock_detect_exact(const TPArgs& A, hipStream_t stream)

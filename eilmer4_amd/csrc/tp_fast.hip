@@ -16,15 +16,15 @@ void launch_tp_stage_mixed(const TPArgs& A, hipStream_t stream, int flux)
 }
 
 void launch_tp_stage_batch_fast64(const BatchArgs* B, const int* wg, int dims, bool guess, const TPArgs* per_block,
-                                  int nper, hipStream_t stream, int flux)
+                                  int nper, int nblk, hipStream_t stream, int flux)
 {
-    launch_tp_stage_batch_m<ModeFast64>(B, wg, dims, guess, per_block, nper, stream, flux);
+    launch_tp_stage_batch_m<ModeFast64>(B, wg, dims, guess, per_block, nper, nblk, stream, flux);
 }
 
 void launch_tp_stage_batch_mixed(const BatchArgs* B, const int* wg, int dims, bool guess, const TPArgs* per_block,
-                                 int nper, hipStream_t stream, int flux)
+                                 int nper, int nblk, hipStream_t stream, int flux)
 {
-    launch_tp_stage_batch_m<ModeMixed>(B, wg, dims, guess, per_block, nper, stream, flux);
+    launch_tp_stage_batch_m<ModeMixed>(B, wg, dims, guess, per_block, nper, nblk, stream, flux);
 }
 
 void launch_tp_shock_detect_fast64(const TPArgs& A, hipStream_t stream)

@@ -24,8 +24,40 @@ struct CeaCurve {
     int nseg;
     double Tb[TP_NSEG_MAX + 1];
     double Tbl[TP_NSEG_MAX];
+    double rTbl[TP_NSEG_MAX];                     // 1 / Tbl (fast modes' blend weights)
     double c[TP_NSEG_MAX][9];
 };
+
+// ln x of a positive, normal, finite x for the fast modes: x = m 2^e with m
+// in [1/sqrt 2, sqrt 2), ln m = 2 atanh z with z = (m - 1) / (m + 1), so
+// |z| < 0.1716, and 2 atanh z = 2 z (1 + z^2/3 + z^4/5 + ... + z^18/19), the
+// first dropped term below 2.2e-17 relative; e ln 2 added in two parts.
+// Within a few ulp, in about a quarter of the instructions of the correctly
+// rounded log (the energy curve evaluates one per temperature).
+__device__ __forceinline__ double log_fast(double x)
+{
+    int e;
+    double m = frexp(x, &e);
+    if (m < 0.70710678118654752440) {
+        m += m;
+        e -= 1;
+    }
+    const double z = dv_seed(m - 1.0, m + 1.0, rcp_seed(m + 1.0));
+    const double w = z * z;
+    double p = 1.0 / 19.0;
+    p = fma(p, w, 1.0 / 17.0);
+    p = fma(p, w, 1.0 / 15.0);
+    p = fma(p, w, 1.0 / 13.0);
+    p = fma(p, w, 1.0 / 11.0);
+    p = fma(p, w, 1.0 / 9.0);
+    p = fma(p, w, 1.0 / 7.0);
+    p = fma(p, w, 1.0 / 5.0);
+    p = fma(p, w, 1.0 / 3.0);
+    p = fma(p, w, 1.0);
+    const double de = (double)e;
+    // ln 2 = 6.93147180559945286227e-01 + 2.31904681384629955842e-17
+    return fma(de, 6.93147180559945286227e-01, fma(de, 2.31904681384629955842e-17, 2.0 * z * p));
+}
 
 struct TPGas {
     int nsp;
@@ -138,7 +170,7 @@ __device__ __forceinline__ void tp_mix_coeffs(const TPGas& g, const double* Y, d
     } else {
         for (int i = 1; i < nb - 1; ++i) {
             const double lo = cv.Tb[i] - 0.5 * cv.Tbl[i - 1], hi = cv.Tb[i] + 0.5 * cv.Tbl[i - 1];
-            if (T >= lo && T <= hi) { s0 = i - 1; s1 = i; wB = dv_seed(1.0, cv.Tbl[i - 1], rcp_seed(cv.Tbl[i - 1])) * (T - lo); break; }
+            if (T >= lo && T <= hi) { s0 = i - 1; s1 = i; wB = cv.rTbl[i - 1] * (T - lo); break; }
             if (T > hi && T < (cv.Tb[i + 1] - 0.5 * cv.Tbl[i])) { s0 = i; break; }
         }
         if (s0 < 0) s0 = 0;
@@ -182,7 +214,7 @@ __device__ __forceinline__ void tp_mix_u_cv(const TPGas& g, const double* Y, dou
     double Mk[8];
     tp_mix_coeffs<NSP>(g, Y, Tc, Mk);
     const double r = dv_seed(1.0, Tc, rcp_seed(Tc));
-    const double lnT = log(Tc);
+    const double lnT = log_fast(Tc);
     const double T2 = Tc * Tc, T3 = T2 * Tc, T4 = T2 * T2;
     double h = tp_mix_h(Mk, Tc, r, lnT);
     const double cp = Mk[0] * r * r + Mk[1] * r + Mk[2] + Mk[3] * Tc + Mk[4] * T2 + Mk[5] * T3 + Mk[6] * T4;
@@ -217,15 +249,26 @@ __device__ __forceinline__ void tp_mix_all(const TPGas& g, const double* Y, TPMi
         }
 }
 
+// A value that is the same in every lane of the wave (here: a field of the
+// gas model, whose address is uniform), pinned to a scalar register: without
+// this the compiler may turn a per-lane choice between two such fields into
+// a per-lane load, a memory round trip inside every curve evaluation.
+__device__ __forceinline__ double uniform_f64(double x)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const unsigned int lo = (unsigned int)__builtin_amdgcn_readfirstlane((int)(unsigned int)b);
+    const unsigned int hi = (unsigned int)__builtin_amdgcn_readfirstlane((int)(unsigned int)(b >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // Weights of the three segments at Tc and their T derivatives: one segment,
 // or the linear blend of two inside a blend range (cea_thermo_curves.d:136-169,
 // the same comparisons).
 __device__ __forceinline__ void tp_seg_weights(const CeaCurve& cv, double Tc, double w[3], double dw[3])
 {
-    const double lo1 = cv.Tb[1] - 0.5 * cv.Tbl[0], hi1 = cv.Tb[1] + 0.5 * cv.Tbl[0];
-    const double lo2 = cv.Tb[2] - 0.5 * cv.Tbl[1], hi2 = cv.Tb[2] + 0.5 * cv.Tbl[1];
-    const double r1 = dv_seed(1.0, cv.Tbl[0], rcp_seed(cv.Tbl[0]));
-    const double r2 = dv_seed(1.0, cv.Tbl[1], rcp_seed(cv.Tbl[1]));
+    const double lo1 = uniform_f64(cv.Tb[1] - 0.5 * cv.Tbl[0]), hi1 = uniform_f64(cv.Tb[1] + 0.5 * cv.Tbl[0]);
+    const double lo2 = uniform_f64(cv.Tb[2] - 0.5 * cv.Tbl[1]), hi2 = uniform_f64(cv.Tb[2] + 0.5 * cv.Tbl[1]);
+    const double r1 = uniform_f64(cv.rTbl[0]), r2 = uniform_f64(cv.rTbl[1]);
     const bool in1 = Tc >= lo1 && Tc <= hi1, in2 = Tc >= lo2 && Tc <= hi2;
     const double wb = in1 ? r1 * (Tc - lo1) : r2 * (Tc - lo2);
     const double rb = in1 ? r1 : r2;
@@ -235,6 +278,25 @@ __device__ __forceinline__ void tp_seg_weights(const CeaCurve& cv, double Tc, do
     dw[0] = in1 ? -rb : 0.0;
     dw[1] = in1 ? rb : (in2 ? -rb : 0.0);
     dw[2] = in2 ? rb : 0.0;
+}
+
+// Which smooth piece of the fast modes' curve T lies on: below the curve
+// range (0), segment 0 (1), the first blend range (2), segment 1 (3), the
+// second blend range (4), segment 2 (5), above the range (6) -- the same
+// comparisons as tp_seg_weights.  On one piece the energy is one smooth
+// expression of T (a segment's polynomial and log terms, their linear blend,
+// or the constant-Cp extension).
+__device__ __forceinline__ int tp_curve_piece(const CeaCurve& cv, double T)
+{
+    if (T < uniform_f64(cv.Tb[0])) return 0;
+    if (T > uniform_f64(cv.Tb[TP_FAST_NSEG])) return 6;
+    const double lo1 = uniform_f64(cv.Tb[1] - 0.5 * cv.Tbl[0]), hi1 = uniform_f64(cv.Tb[1] + 0.5 * cv.Tbl[0]);
+    const double lo2 = uniform_f64(cv.Tb[2] - 0.5 * cv.Tbl[1]), hi2 = uniform_f64(cv.Tb[2] + 0.5 * cv.Tbl[1]);
+    if (T < lo1) return 1;
+    if (T <= hi1) return 2;
+    if (T < lo2) return 3;
+    if (T <= hi2) return 4;
+    return 5;
 }
 
 // u, du/dT and d2u/dT2 of the mixture at T from its segment sums.  SLOPE:
@@ -247,7 +309,7 @@ template <bool SLOPE>
 __device__ __forceinline__ void tp_mix_eval(const TPMix& X, const CeaCurve& c0, double Rmix, double T, double* u,
                                             double* du, double* d2u)
 {
-    const double Tlo = c0.Tb[0], Thi = c0.Tb[TP_FAST_NSEG];
+    const double Tlo = uniform_f64(c0.Tb[0]), Thi = uniform_f64(c0.Tb[TP_FAST_NSEG]);
     const double Tc = fmin(fmax(T, Tlo), Thi);
     double w[3], dw[3];
     tp_seg_weights(c0, Tc, w, dw);
@@ -255,7 +317,7 @@ __device__ __forceinline__ void tp_mix_eval(const TPMix& X, const CeaCurve& c0, 
 #pragma unroll
     for (int k = 0; k < 8; ++k) Mk[k] = w[0] * X.M[0][k] + w[1] * X.M[1][k] + w[2] * X.M[2][k];
     const double r = dv_seed(1.0, Tc, rcp_seed(Tc));
-    const double lnT = log(Tc);
+    const double lnT = log_fast(Tc);
     const double T2 = Tc * Tc, T3 = T2 * Tc, T4 = T2 * T2;
     double h = tp_mix_h(Mk, Tc, r, lnT);
     const double cp = Mk[0] * r * r + Mk[1] * r + Mk[2] + Mk[3] * Tc + Mk[4] * T2 + Mk[5] * T3 + Mk[6] * T4;

@@ -64,6 +64,53 @@ __device__ __forceinline__ int tp_face_eos(const TPGas& g, TPState<NSP>& Q, int*
     }
 }
 
+// Mixed form, the temperatures of the two states of a face: Halley steps on
+// the mixture curve from each side's cell temperature, at most three
+// evaluations per side.  A step s with |s| <= 1e-3 T that stays on the
+// smooth piece of the curve it started from ends a side's solve: Halley's
+// cubic convergence then leaves the new temperature within about
+// |s|^3 / T^2 <= 1e-9 T of the root, far inside the FP32 rounding of the
+// flux the face state feeds; a step that lands on another piece is followed
+// by one from there.  The two sides' evaluations sit side by side in one
+// loop, so each one's chain of dependent FP64 operations fills the other's
+// latency.  okL/okR tell which sides settled (TL, TR then hold their
+// temperatures); a face with a side that did not goes to the retry kernel,
+// whose solve is the polish and bracketed Newton of tp_face_eos.
+template <int NSP>
+__device__ __forceinline__ void tp_face_halley_pair(const TPGas& g, const TPMix& XL, const TPMix& XR, double RL,
+                                                    double RR, double eL, double eR, double& TL, double& TR,
+                                                    bool& okL, bool& okR, int* fevals)
+{
+    const CeaCurve& c0 = g.curve[0];
+    bool liveL = TL >= 20.0 && TL <= 100000.0, liveR = TR >= 20.0 && TR <= 100000.0;
+    okL = false;
+    okR = false;
+    double tL = TL, tR = TR;
+    auto step = [&](double t, double u, double du, double d2u, double e, double& out, bool& live, bool& ok) {
+        if (!live) return t;
+        (*fevals)++;
+        const double f = u - e;
+        const double s = dv<ModeFast64>(2.0 * f * du, 2.0 * du * du - f * d2u);
+        const double tn = t - s;
+        if (tp_curve_piece(c0, tn) == tp_curve_piece(c0, t) && fabs(s) <= 1.0e-3 * tn && tn >= 20.0 &&
+            tn <= 100000.0) {
+            out = tn;
+            ok = true;
+            live = false;
+        }
+        return tn;
+    };
+#pragma nounroll
+    for (int it = 0; it < 3; ++it) {
+        if (!(liveL || liveR)) break;
+        double uL, duL, d2uL, uR, duR, d2uR;
+        tp_mix_eval<true>(XL, c0, RL, tL, &uL, &duL, &d2uL);
+        tp_mix_eval<true>(XR, c0, RR, tR, &uR, &duR, &d2uR);
+        tL = step(tL, uL, duL, d2uL, eL, TL, liveL, okL);
+        tR = step(tR, uR, duR, d2uR, eR, TR, liveR, okR);
+    }
+}
+
 // A face-side (or cell) state of a mixture in the face frame, in the
 // arithmetic of the flux path (double, or float in the mixed form).
 template <class R, int NSP>
@@ -502,16 +549,40 @@ __device__ __forceinline__ const TPArgs& tp_batch_args(int& entry)
     TPKArgs p = (TPKArgs)(b->table + (unsigned long long)entry * sizeof(TPArgs));
     return *(const TPArgs*)p;
 }
+// Table entry e of a batched launch (a uniform index).
+__device__ __forceinline__ const TPArgs& tp_table_args(const BatchArgs& b, int e)
+{
+    TPKArgs p = (TPKArgs)(b.table + (unsigned long long)__builtin_amdgcn_readfirstlane(e) * sizeof(TPArgs));
+    return *(const TPArgs*)p;
+}
 #else
 __device__ __forceinline__ const TPArgs& tp_batch_args(int& entry)
 {
     entry = 0;
     return *(const TPArgs*)nullptr;
 }
+__device__ __forceinline__ const TPArgs& tp_table_args(const BatchArgs&, int)
+{
+    return *(const TPArgs*)nullptr;
+}
 #endif
 
-// The face kernel's work for face f0 of direction DIR of one block.
-template <class M, int NSP, int FLUX, int DIR>
+// First face slot of direction DIR in a block's retry list.
+template <int DIR>
+__device__ __forceinline__ long tp_retry_first(const TPArgs& A)
+{
+    const long nk = (A.dims == 3) ? A.nk : 1;
+    const long nf0 = (long)(A.ni + 1) * A.nj * nk, nf1 = (long)A.ni * (A.nj + 1) * nk;
+    return 4 + ((DIR >= 1) ? nf0 : 0) + ((DIR == 2) ? nf1 : 0);
+}
+
+// The face kernel's work for face f0 of direction DIR of one block.  LEAN
+// (the mixed form's face kernels): the face states' temperatures come from
+// tp_face_halley alone, and a face where it does not settle is appended to
+// the block's retry list instead of being finished here, which keeps the
+// full solve -- and the registers it needs -- out of the kernel that runs
+// every face.
+template <class M, int NSP, int FLUX, int DIR, bool LEAN = false>
 __device__ __forceinline__ void tp_face_body(const TPArgs& A, long f0)
 {
     using MA = typename TPArith<M>::type;
@@ -531,77 +602,123 @@ __device__ __forceinline__ void tp_face_body(const TPArgs& A, long f0)
     const bool d3 = (A.dims == 3);
     const int pn = DIR, pt1 = (DIR == 0) ? 1 : (DIR == 1) ? (d3 ? 2 : 0) : 0;
     const int pt2 = (DIR == 0) ? 2 : (DIR == 1) ? (d3 ? 0 : 2) : 1;
-    // the stencil's four positions; values are loaded one scalar at a time
+    // the stencil's four positions
     TPRef r[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) r[m] = tp_ref(A, i + (m - 2) * di, j + (m - 2) * dj, k + (m - 2) * dk);
     TPCellT<R, NSP> Lf, Rf;
     int fe = 0, fb = 0;
-    bool okL = false, okR = false;
+    bool okL = false, okR = false, defer = false;
     if (A.high_order) {
         const L2R2WT<R> w = cast_w<R>(A.w[DIR]);
         const bool lim = A.apply_limiter, clip = A.extrema_clipping;
         const int comp[3] = {pn, pt1, pt2};
+        // every value the reconstruction and the two solves read, loaded in
+        // straight-line code before any is used: one memory round trip for
+        // the face instead of one per reconstructed scalar
+        constexpr int NV = 5 + NSP;   // velocity (face frame), mass fractions, rho, u
+        double v[4][NV];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) v[m][d] = tp_vel(r[m], comp[d]);
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) v[m][3 + s] = tp_val(r[m], P_Y + s);
+            v[m][3 + NSP] = tp_val(r[m], P_RHO);
+            v[m][4 + NSP] = tp_val(r[m], P_U);
+        }
+        const double TL = tp_val(r[1], P_T), TR = tp_val(r[2], P_T);
+        const double aL = tp_val(r[1], P_A), aR = tp_val(r[2], P_A);
 #pragma unroll
         for (int d = 0; d < 3; ++d)
-            interp_l2r2_scalar<MF>(w, lim, clip, R(tp_vel(r[0], comp[d])), R(tp_vel(r[1], comp[d])),
-                                   R(tp_vel(r[2], comp[d])), R(tp_vel(r[3], comp[d])), Lf.v[d], Rf.v[d]);
+            interp_l2r2_scalar<MF>(w, lim, clip, R(v[0][d]), R(v[1][d]), R(v[2][d]), R(v[3][d]), Lf.v[d], Rf.v[d]);
 #pragma unroll
         for (int s = 0; s < NSP; ++s)
-            interp_l2r2_scalar<MF>(w, lim, clip, R(tp_val(r[0], P_Y + s)), R(tp_val(r[1], P_Y + s)),
-                                   R(tp_val(r[2], P_Y + s)), R(tp_val(r[3], P_Y + s)), Lf.Y[s], Rf.Y[s]);
+            interp_l2r2_scalar<MF>(w, lim, clip, R(v[0][3 + s]), R(v[1][3 + s]), R(v[2][3 + s]), R(v[3][3 + s]),
+                                   Lf.Y[s], Rf.Y[s]);
         if (scale_mass_fractions<MF, NSP>(Lf.Y) != 0) {
 #pragma unroll
-            for (int s = 0; s < NSP; ++s) Lf.Y[s] = R(tp_val(r[1], P_Y + s));
+            for (int s = 0; s < NSP; ++s) Lf.Y[s] = R(v[1][3 + s]);
         }
         if (scale_mass_fractions<MF, NSP>(Rf.Y) != 0) {
 #pragma unroll
-            for (int s = 0; s < NSP; ++s) Rf.Y[s] = R(tp_val(r[2], P_Y + s));
+            for (int s = 0; s < NSP; ++s) Rf.Y[s] = R(v[2][3 + s]);
         }
-        interp_l2r2_scalar<MF>(w, lim, clip, R(tp_val(r[0], P_RHO)), R(tp_val(r[1], P_RHO)), R(tp_val(r[2], P_RHO)),
-                               R(tp_val(r[3], P_RHO)), Lf.rho, Rf.rho);
-        interp_l2r2_scalar<MF>(w, lim, clip, R(tp_val(r[0], P_U)), R(tp_val(r[1], P_U)), R(tp_val(r[2], P_U)),
-                               R(tp_val(r[3], P_U)), Lf.u, Rf.u);
+        interp_l2r2_scalar<MF>(w, lim, clip, R(v[0][3 + NSP]), R(v[1][3 + NSP]), R(v[2][3 + NSP]), R(v[3][3 + NSP]),
+                               Lf.rho, Rf.rho);
+        interp_l2r2_scalar<MF>(w, lim, clip, R(v[0][4 + NSP]), R(v[1][4 + NSP]), R(v[2][4 + NSP]), R(v[3][4 + NSP]),
+                               Lf.u, Rf.u);
         // update_thermo_from_rhou on each side (guess: the side's cell T); on
         // failure the side is the cell's state (onedinterp.d:1039-1068); the
         // sound speed is the cell's (sfluidblock.d:2199-2208).  The two sides
-        // share one copy of the solver code (a loop that is not unrolled).
+        // share one copy of the solver code (a loop that is not unrolled);
+        // the lean form solves both sides together.
+        if constexpr (LEAN) {
+            double YL[NSP], YR[NSP];
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) { YL[s] = double(Lf.Y[s]); YR[s] = double(Rf.Y[s]); }
+            TPMix XL, XR;
+            tp_mix_all<NSP>(*A.gas, YL, XL);
+            tp_mix_all<NSP>(*A.gas, YR, XR);
+            const double RmL = tp_Rmix<NSP>(*A.gas, YL), RmR = tp_Rmix<NSP>(*A.gas, YR);
+            double TfL = TL, TfR = TR;
+            tp_face_halley_pair<NSP>(*A.gas, XL, XR, RmL, RmR, double(Lf.u), double(Rf.u), TfL, TfR, okL, okR, &fe);
+            if (okL) { Lf.T = R(TfL); Lf.p = R(double(Lf.rho) * RmL * TfL); Lf.a = R(aL); }
+            if (okR) { Rf.T = R(TfR); Rf.p = R(double(Rf.rho) * RmR * TfR); Rf.a = R(aR); }
+            defer = !(okL && okR);
+        }
 #pragma nounroll
-        for (int sd = 0; sd < 2; ++sd) {
-            const TPRef rc = sd ? r[2] : r[1];
+        for (int sd = 0; sd < 2 && !LEAN; ++sd) {
             // the temperature solve in FP64 from the (FP32 in the mixed form)
             // reconstructed density, energy and mass fractions
             TPState<NSP> Q;
             Q.rho = double(sd ? Rf.rho : Lf.rho);
             Q.u = double(sd ? Rf.u : Lf.u);
-            Q.T = tp_val(rc, P_T);
+            Q.T = sd ? TR : TL;
 #pragma unroll
             for (int s = 0; s < NSP; ++s) Q.Y[s] = double(sd ? Rf.Y[s] : Lf.Y[s]);
             if (tp_face_eos<MA, NSP, kPolish>(*A.gas, Q, &fe, &fb) == 0) {
-                const R a = R(tp_val(rc, P_A));
+                const R a = R(sd ? aR : aL);
                 if (sd == 0) { Lf.T = R(Q.T); Lf.u = R(Q.u); Lf.p = R(Q.p); Lf.a = a; okL = true; }
                 else { Rf.T = R(Q.T); Rf.u = R(Q.u); Rf.p = R(Q.p); Rf.a = a; okR = true; }
             }
         }
     }
-    if (!okL) Lf = tp_cell<NSP, R>(r[1], pn, pt1, pt2);
-    if (!okR) Rf = tp_cell<NSP, R>(r[2], pn, pt1, pt2);
-    R Fr[5 + NSP];
-    const bool shock = (FLUX == FLUX_SHOCK) && ((r[1].s && *r[1].s) || (r[2].s && *r[2].s));
-    tp_flux<MF, NSP, FLUX>(A, *A.gas, Lf, Rf, shock, Fr);
-    double F[5 + NSP];
-#pragma unroll
-    for (int q = 0; q < 5 + NSP; ++q) F[q] = double(Fr[q]);
-    if (active) {
-        double* dst = A.F[DIR];
-        const long fq = A.fqs[DIR];
-        dst[f] = F[0];
-        dst[(1 + pn) * fq + f] = F[1];
-        dst[(1 + pt1) * fq + f] = F[2];
-        dst[(1 + pt2) * fq + f] = F[3];
-        dst[4 * fq + f] = F[4];
-#pragma unroll
-        for (int s = 0; s < NSP; ++s) dst[(5 + s) * fq + f] = F[5 + s];
+    if constexpr (LEAN) {
+        // hand the face on: one atomic per wave for the faces it defers
+        const unsigned long long m = __ballot(defer && active);
+        if (m) {
+            const int lead = __ffsll(m) - 1;
+            unsigned int base = 0;
+            if (__lane_id() == lead) base = atomicAdd(&A.retry[DIR], (unsigned int)__popcll(m));
+            base = __shfl(base, lead);
+            if (defer && active) {
+                const unsigned int below = (unsigned int)__popcll(m & ((1ull << __lane_id()) - 1ull));
+                A.retry[tp_retry_first<DIR>(A) + base + below] = (unsigned int)f;
+            }
+            fb = (defer && active) ? 1 : 0;
+        }
+    }
+    if (!LEAN || !defer) {
+        if (!okL) Lf = tp_cell<NSP, R>(r[1], pn, pt1, pt2);
+        if (!okR) Rf = tp_cell<NSP, R>(r[2], pn, pt1, pt2);
+        R Fr[5 + NSP];
+        const bool shock = (FLUX == FLUX_SHOCK) && ((r[1].s && *r[1].s) || (r[2].s && *r[2].s));
+        tp_flux<MF, NSP, FLUX>(A, *A.gas, Lf, Rf, shock, Fr);
+        double F[5 + NSP];
+    #pragma unroll
+        for (int q = 0; q < 5 + NSP; ++q) F[q] = double(Fr[q]);
+        if (active) {
+            double* dst = A.F[DIR];
+            const long fq = A.fqs[DIR];
+            dst[f] = F[0];
+            dst[(1 + pn) * fq + f] = F[1];
+            dst[(1 + pt1) * fq + f] = F[2];
+            dst[(1 + pt2) * fq + f] = F[3];
+            dst[4 * fq + f] = F[4];
+    #pragma unroll
+            for (int s = 0; s < NSP; ++s) dst[(5 + s) * fq + f] = F[5 + s];
+        }
     }
     if (!active) { fe = 0; fb = 0; }
     for (int off = 32; off > 0; off >>= 1) {
@@ -616,15 +733,60 @@ __device__ __forceinline__ void tp_face_body(const TPArgs& A, long f0)
 
 // One thread per face of direction DIR; BATCH: a launch over several
 // blocks, each workgroup in the block its map entry names.
+#ifndef E4_TP_LEAN_WAVES
+#define E4_TP_LEAN_WAVES 2
+#endif
+template <class M> struct TPFaceOccupancy { static constexpr int waves = 2; };
+template <> struct TPFaceOccupancy<ModeMixed> { static constexpr int waves = E4_TP_LEAN_WAVES; };
+
 template <class M, int NSP, int FLUX, int DIR, bool BATCH>
-__global__ __launch_bounds__(128) void tp_face_kernel(typename std::conditional<BATCH, BatchArgs, TPArgs>::type a)
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(TPFaceOccupancy<M>::waves)))
+void tp_face_kernel(typename std::conditional<BATCH, BatchArgs, TPArgs>::type a)
 {
+    constexpr bool LEAN = std::is_same<M, ModeMixed>::value;
     if constexpr (BATCH) {
         int e;
         const TPArgs& A = tp_batch_args(e);
-        tp_face_body<M, NSP, FLUX, DIR>(A, (long)((int)blockIdx.x - A.wg0[DIR]) * blockDim.x + threadIdx.x);
+        tp_face_body<M, NSP, FLUX, DIR, LEAN>(A, (long)((int)blockIdx.x - A.wg0[DIR]) * blockDim.x + threadIdx.x);
     } else {
-        tp_face_body<M, NSP, FLUX, DIR>(a, (long)blockIdx.x * blockDim.x + threadIdx.x);
+        tp_face_body<M, NSP, FLUX, DIR, LEAN>(a, (long)blockIdx.x * blockDim.x + threadIdx.x);
+    }
+}
+
+// The faces of direction DIR that the mixed form's face kernels handed on,
+// with the full solve; `part` of `nparts` workgroups of this block take
+// every nparts-th group of 128 list entries (the same trip count for every
+// lane of a group, idle lanes computing nothing they store).
+template <int NSP, int FLUX, int DIR>
+__device__ __forceinline__ void tp_retry_dir(const TPArgs& A, int part, int nparts)
+{
+    const long nk = (A.dims == 3) ? A.nk : 1;
+    const long nf = (long)(A.ni + (DIR == 0)) * (A.nj + (DIR == 1)) * (nk + (DIR == 2));
+    const long n = (long)A.retry[DIR];
+    const unsigned int* list = A.retry + tp_retry_first<DIR>(A);
+    for (long g = (long)part * blockDim.x; g < n; g += (long)nparts * blockDim.x) {
+        const long idx = g + threadIdx.x;
+        const long f = (idx < n) ? (long)list[idx] : nf;
+        tp_face_body<ModeMixed, NSP, FLUX, DIR, false>(A, f);
+    }
+}
+
+// The retry kernel of a mixture block (BATCH: of several, `nparts`
+// workgroups per block entry of the table).
+template <int NSP, int FLUX, bool BATCH>
+__global__ __launch_bounds__(128) void tp_retry_kernel(typename std::conditional<BATCH, BatchArgs, TPArgs>::type a,
+                                                       int nparts)
+{
+    auto run = [&](const TPArgs& A, int part) {
+        if (stage_aborted(A.abort_prev) || !A.retry) return;
+        tp_retry_dir<NSP, FLUX, 0>(A, part, nparts);
+        tp_retry_dir<NSP, FLUX, 1>(A, part, nparts);
+        if (A.dims == 3) tp_retry_dir<NSP, FLUX, 2>(A, part, nparts);
+    };
+    if constexpr (BATCH) {
+        run(tp_table_args(a, (int)blockIdx.x / nparts), (int)blockIdx.x % nparts);
+    } else {
+        run(a, (int)blockIdx.x);
     }
 }
 
@@ -638,6 +800,12 @@ __device__ __forceinline__ void tp_update_body(const TPArgs& A, long c)
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
     constexpr int NQ = 5 + NSP;
     if (stage_aborted(A.abort_prev)) return;
+    // the retry kernel of this stage has run: clear the list for the next stage
+    if (A.retry && c == 0) {
+        A.retry[0] = 0u;
+        A.retry[1] = 0u;
+        A.retry[2] = 0u;
+    }
     const long ncells = (long)A.ni * A.nj * A.nk;
     const bool active = c < ncells;
     const long qs = A.qs;
@@ -1184,6 +1352,31 @@ void launch_tp_faces(const TPArgs& A, hipStream_t stream)
     if (A.dims == 3) hipLaunchKernelGGL((tp_face_kernel<M, 5, FLUX, 2, false>), g(nf2), b, 0, stream, A);
 }
 
+// The retry kernel after the mixed form's face kernels (it must finish the
+// handed-on faces before the boundary-face kernel overwrites its faces):
+// `parts` workgroups per block.
+template <int FLUX, bool BATCH>
+void launch_tp_retry_f(const typename std::conditional<BATCH, BatchArgs, TPArgs>::type& a, int nblk, int parts,
+                       hipStream_t stream)
+{
+    hipLaunchKernelGGL((tp_retry_kernel<5, FLUX, BATCH>), dim3((unsigned)(nblk * parts)), dim3(128), 0, stream, a,
+                       parts);
+}
+
+template <bool BATCH>
+void launch_tp_retry(const typename std::conditional<BATCH, BatchArgs, TPArgs>::type& a, int nblk, int parts,
+                     hipStream_t stream, int flux)
+{
+    if (flux == FLUX_HLLC) launch_tp_retry_f<FLUX_HLLC, BATCH>(a, nblk, parts, stream);
+    else if (flux == FLUX_SHOCK) launch_tp_retry_f<FLUX_SHOCK, BATCH>(a, nblk, parts, stream);
+    else if (flux == FLUX_OTHER) launch_tp_retry_f<FLUX_OTHER, BATCH>(a, nblk, parts, stream);
+    else launch_tp_retry_f<FLUX_AUSMDV, BATCH>(a, nblk, parts, stream);
+}
+
+// workgroups of the retry kernel per block
+constexpr int TP_RETRY_PARTS = 512;
+constexpr int TP_RETRY_PARTS_BATCH = 64;
+
 template <class M>
 void launch_tp_stage_m(const TPArgs& A, hipStream_t stream, int flux)
 {
@@ -1191,6 +1384,7 @@ void launch_tp_stage_m(const TPArgs& A, hipStream_t stream, int flux)
     else if (flux == FLUX_SHOCK) launch_tp_faces<M, FLUX_SHOCK>(A, stream);
     else if (flux == FLUX_OTHER) launch_tp_faces<M, FLUX_OTHER>(A, stream);
     else launch_tp_faces<M, FLUX_AUSMDV>(A, stream);
+    if constexpr (std::is_same<M, ModeMixed>::value) launch_tp_retry<false>(A, 1, TP_RETRY_PARTS, stream, flux);
     if (A.fix_off[6] > 0) {
         const dim3 g((unsigned)((A.fix_off[6] + 63) / 64)), b(64);
         if (flux == FLUX_HLLC) hipLaunchKernelGGL((tp_boundary_face_kernel<M, 5, FLUX_HLLC>), g, b, 0, stream, A);
@@ -1221,12 +1415,14 @@ void launch_tp_batch_faces(const BatchArgs* B, const int* wg, int dims, hipStrea
 
 template <class M>
 void launch_tp_stage_batch_m(const BatchArgs* B, const int* wg, int dims, bool guess, const TPArgs* per_block,
-                             int nper, hipStream_t stream, int flux)
+                             int nper, int nblk, hipStream_t stream, int flux)
 {
     if (flux == FLUX_HLLC) launch_tp_batch_faces<M, FLUX_HLLC>(B, wg, dims, stream);
     else if (flux == FLUX_SHOCK) launch_tp_batch_faces<M, FLUX_SHOCK>(B, wg, dims, stream);
     else if (flux == FLUX_OTHER) launch_tp_batch_faces<M, FLUX_OTHER>(B, wg, dims, stream);
     else launch_tp_batch_faces<M, FLUX_AUSMDV>(B, wg, dims, stream);
+    if constexpr (std::is_same<M, ModeMixed>::value)
+        launch_tp_retry<true>(B[0], nblk, TP_RETRY_PARTS_BATCH, stream, flux);
     for (int n = 0; n < nper; ++n) {
         const TPArgs& A = per_block[n];
         const dim3 g((unsigned)((A.fix_off[6] + 63) / 64)), b(64);

@@ -169,9 +169,9 @@ int e4hip_gas_tp(int nsp, const double* M, int nseg, const double* Tbreaks, cons
  * counterpart in the reference; it replaces the starting point of
  * therm_perf_gas_mix_eos.d:70-153 in the decode).  A 7 -> 64 -> 64 -> 64 -> 1
  * SiLU network on (rho, u, Y0..Y4), evaluated in BF16 MFMA with FP32
- * accumulation, then two FP64 Newton steps judged by the energy residual at
- * the result (accepted when |residual / (dz/dT)| <= 1e-11 T); a cell that fails
- * the judge falls back to the reference's bracketed solve from its stored
+ * accumulation, then two FP64 Newton steps accepted by the energy residual at
+ * the result (when |residual / (dz/dT)| <= 1e-11 T); a cell that fails the
+ * test falls back to the reference's bracketed solve from its stored
  * temperature.  params: E4HIP_MLP_NPARAM floats in the training layout
  * (W1[64][8], b1[64], W2[64][64], b2, W3[64][64], b3, w4[64], b4); features are
  * normalised as (x - in_mean) * in_std, and T = T_mean + T_std * output. */

@@ -116,9 +116,12 @@ def _oracle_decode_T(og, ob_like, U, Tguess):
     return b.get_fs()[2]
 
 
-@pytest.mark.parametrize("learned", [False, True])
-def test_tp128_one_step(gpu, learned):
-    print("case tp128 one step fp64 learned=%s" % learned, flush=True)
+@pytest.mark.parametrize("precision,learned", [("fp64", False), ("fp64", True), ("mixed", False), ("mixed", True)])
+def test_tp128_one_step(gpu, precision, learned):
+    """fp64: U within 1e-9 and T within 1e-10 of the oracle; mixed (FP32
+    reconstruction and flux, short face temperature solves, FP64 decode):
+    U and T within 1e-5."""
+    print("case tp128 one step %s learned=%s" % (precision, learned), flush=True)
     gas = e4.ThermallyPerfectGas.from_json()
     if learned:
         learned_eos.GuessModel.load().install(gas)
@@ -140,7 +143,7 @@ def test_tp128_one_step(gpu, learned):
     for face, (kind, fs) in bcs.items():
         db.set_bc(face, kind, fs)
     sim = e4.Simulation([db], e4.GlobalConfig(flux_calculator=e4.FluxCalculator.ausmdv, learned_eos_guess=learned,
-                                              precision_mode=e4.PrecisionMode.fp64))
+                                              precision_mode=e4.PrecisionMode[precision]))
     db.upload(U0, Tg)
     sim.state.dt_global = dt
     learned_eos.decode_stats(reset=True)
@@ -152,8 +155,12 @@ def test_tp128_one_step(gpu, learned):
     errT = float(np.max(np.abs(fs[2] - fso[2]) / fso[2]))
     print("  U rel err per quantity %s; T rel err %.3e; decode energy evals per cell-stage %.3f, fallbacks %d"
           % (err, errT, ev / (2.0 * n ** 3), fb), flush=True)
-    assert err.max() <= 1.0e-9
-    assert errT <= 1.0e-10
+    if precision == "mixed":
+        assert err.max() <= 1.0e-5
+        assert errT <= 1.0e-5
+    else:
+        assert err.max() <= 1.0e-9
+        assert errT <= 1.0e-10
 
 
 def _c5_blocks(descs, conns, ig, tg, og_ideal, og_tp):
