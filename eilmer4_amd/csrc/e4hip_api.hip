@@ -612,30 +612,44 @@ int autotune_blocks(const int* blks, int nblk, double dt)
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
+        // each candidate: the scheme's stages in order (the later stages read
+        // more arrays than the first), one warm-up pass and three timed ones;
+        // two rounds over the candidates, each keeping its faster round
+        const int ns = nstages(g_state.cfg.gasdynamic_update_scheme);
+        std::vector<float> cand_ms(cand.size(), 1e30f);
+        for (int round = 0; round < 2; ++round) {
+            for (size_t ci = 0; ci < cand.size(); ++ci) {
+                const int kc = cand[ci];
+                e4::StageArgs A[3];
+                dim3 grid;
+                for (int st = 1; st <= ns; ++st) {
+                    int rc = stage_args_for(b, st, dt, g_state.counters, kc, A[st - 1], grid);
+                    if (rc) return rc;
+                }
+                auto launch = [&]() {
+                    for (int st = 0; st < ns; ++st) {
+                        switch (g_state.cfg.precision_mode) {
+                        case E4HIP_PRECISION_FP64_EXACT: e4::launch_stage_exact(A[st], grid, g_state.stream, 3, flux); break;
+                        case E4HIP_PRECISION_MIXED: e4::launch_stage_mixed(A[st], grid, g_state.stream, 3, flux); break;
+                        default: e4::launch_stage_fast64(A[st], grid, g_state.stream, 3, flux); break;
+                        }
+                    }
+                };
+                launch();
+                HIP_TRY(hipEventRecord(e0, g_state.stream));
+                for (int r = 0; r < 3; ++r) launch();
+                HIP_TRY(hipEventRecord(e1, g_state.stream));
+                HIP_TRY(hipEventSynchronize(e1));
+                HIP_TRY(hipGetLastError());
+                float ms = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                cand_ms[ci] = std::min(cand_ms[ci], ms);
+            }
+        }
         int best = cand[0];
         float best_ms = 1e30f;
-        for (int kc : cand) {
-            e4::StageArgs A;
-            dim3 grid;
-            int rc = stage_args_for(b, 1, dt, g_state.counters, kc, A, grid);
-            if (rc) return rc;
-            auto launch = [&]() {
-                switch (g_state.cfg.precision_mode) {
-                case E4HIP_PRECISION_FP64_EXACT: e4::launch_stage_exact(A, grid, g_state.stream, 3, flux); break;
-                case E4HIP_PRECISION_MIXED: e4::launch_stage_mixed(A, grid, g_state.stream, 3, flux); break;
-                default: e4::launch_stage_fast64(A, grid, g_state.stream, 3, flux); break;
-                }
-            };
-            launch();
-            HIP_TRY(hipEventRecord(e0, g_state.stream));
-            for (int r = 0; r < 3; ++r) launch();
-            HIP_TRY(hipEventRecord(e1, g_state.stream));
-            HIP_TRY(hipEventSynchronize(e1));
-            HIP_TRY(hipGetLastError());
-            float ms = 0.0f;
-            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best_ms) { best_ms = ms; best = kc; }
-        }
+        for (size_t ci = 0; ci < cand.size(); ++ci)
+            if (cand_ms[ci] < best_ms) { best_ms = cand_ms[ci]; best = cand[ci]; }
         HIP_TRY(hipEventDestroy(e0));
         HIP_TRY(hipEventDestroy(e1));
         g_state.kchunk_tuned.push_back({key, best});
@@ -662,23 +676,39 @@ int autotune_blocks(const int* blks, int nblk, double dt)
                 hipEvent_t e0, e1;
                 HIP_TRY(hipEventCreate(&e0));
                 HIP_TRY(hipEventCreate(&e1));
+                // as for one block: the scheme's stages in order, two rounds
+                const int ns = nstages(g_state.cfg.gasdynamic_update_scheme);
+                const long cand[] = {1L, 2L, 3L, 4L, 6L, 8L, 12L, 16L, 32L};
+                float cand_ms[sizeof(cand) / sizeof(cand[0])];
+                for (float& m : cand_ms) m = 1e30f;
+                for (int round = 0; round < 2; ++round) {
+                    for (size_t ci = 0; ci < sizeof(cand) / sizeof(cand[0]); ++ci) {
+                        auto pass = [&]() {
+                            for (int st = 1; st <= ns; ++st) {
+                                int rc = launch_stage_batch(bl, cnt, st, dt, 0, cand[ci]);
+                                if (rc) return rc;
+                            }
+                            return (int)E4HIP_OK;
+                        };
+                        int rc = pass();
+                        if (rc) return rc;
+                        HIP_TRY(hipEventRecord(e0, g_state.stream));
+                        for (int r = 0; r < 3; ++r) {
+                            rc = pass();
+                            if (rc) return rc;
+                        }
+                        HIP_TRY(hipEventRecord(e1, g_state.stream));
+                        HIP_TRY(hipEventSynchronize(e1));
+                        HIP_TRY(hipGetLastError());
+                        float ms = 0.0f;
+                        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                        cand_ms[ci] = std::min(cand_ms[ci], ms);
+                    }
+                }
                 long best = 1;
                 float best_ms = 1e30f;
-                for (long chunks : {1L, 2L, 3L, 4L, 6L, 8L, 12L, 16L, 32L}) {
-                    int rc = launch_stage_batch(bl, cnt, 1, dt, 0, chunks);
-                    if (rc) return rc;
-                    HIP_TRY(hipEventRecord(e0, g_state.stream));
-                    for (int r = 0; r < 3; ++r) {
-                        rc = launch_stage_batch(bl, cnt, 1, dt, 0, chunks);
-                        if (rc) return rc;
-                    }
-                    HIP_TRY(hipEventRecord(e1, g_state.stream));
-                    HIP_TRY(hipEventSynchronize(e1));
-                    HIP_TRY(hipGetLastError());
-                    float ms = 0.0f;
-                    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-                    if (ms < best_ms) { best_ms = ms; best = chunks; }
-                }
+                for (size_t ci = 0; ci < sizeof(cand) / sizeof(cand[0]); ++ci)
+                    if (cand_ms[ci] < best_ms) { best_ms = cand_ms[ci]; best = cand[ci]; }
                 HIP_TRY(hipEventDestroy(e0));
                 HIP_TRY(hipEventDestroy(e1));
                 g_state.kchunk_tuned.push_back({key, (int)best});

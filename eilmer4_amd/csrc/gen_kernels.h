@@ -467,18 +467,24 @@ __global__ __launch_bounds__(128) void ig_update_kernel(TPArgs A)
         gen_integral<5>(A, c, i, j, k, integ);
         const double vol_inv = 1.0 / A.geo[c];
         double Un[5];
+        // every value the update reads, loaded before the first store
+        double uold[5], dd0[5], dd1[5], srcq[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            uold[q] = A.Uold[c + q * qs];
+            dd0[q] = (A.stage >= 2) ? A.d0[c + q * qs] : 0.0;
+            dd1[q] = (A.stage >= 3) ? A.d1[c + q * qs] : 0.0;
+            srcq[q] = A.src ? A.src[c + q * qs] : 0.0;   // user-defined source terms
+        }
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
             double Qs = 0.0;
-            if (A.src) Qs += A.src[c + q * qs];   // user-defined source terms
+            if (A.src) Qs += srcq[q];
             const double d = vol_inv * integ[q] + Qs;
-            const double dd0 = (A.stage >= 2) ? A.d0[c + q * qs] : 0.0;
-            const double dd1 = (A.stage >= 3) ? A.d1[c + q * qs] : 0.0;
             if (A.dout) A.dout[c + q * qs] = d;
-            const double uold = A.Uold[c + q * qs];
-            if (A.stage == 1) Un[q] = uold + A.c1 * d;
-            else if (A.stage == 2) Un[q] = uold + A.dt * (A.g1 * dd0 + A.g2 * d);
-            else Un[q] = uold + A.dt * (A.g1 * dd0 + A.g2 * dd1 + A.g3 * d);
+            if (A.stage == 1) Un[q] = uold[q] + A.c1 * d;
+            else if (A.stage == 2) Un[q] = uold[q] + A.dt * (A.g1 * dd0[q] + A.g2 * d);
+            else Un[q] = uold[q] + A.dt * (A.g1 * dd0[q] + A.g2 * dd1[q] + A.g3 * d);
         }
         double rho, v[3], u, T, p, a;
         if (!ig_decode_cell<M>(A.ig, Un, rho, v, u, T, p, a)) {

@@ -399,15 +399,24 @@ __device__ __forceinline__ void finish_cell(KArgs ap, long c, const double integ
     const long qs = opaque_qs(ap->qs);
     const double* __restrict__ src = ap->src;
     double Un[5];
+    // every value the update reads, loaded before the first store (the
+    // compiler cannot move a load of one array across a store to another, so
+    // interleaved they would cost a memory round trip per quantity)
+    double uold[5], dd0[5], dd1[5], srcq[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        uold[q] = ap->Uold[c + q * qs];
+        dd0[q] = (ap->stage >= 2) ? ap->d0[c + q * qs] : 0.0;
+        dd1[q] = (ap->stage >= 3) ? ap->d1[c + q * qs] : 0.0;
+        srcq[q] = src ? src[c + q * qs] : 0.0;
+    }
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
         double Q = (q == 2) ? qy : 0.0;
-        if (src) Q += src[c + q * qs];
+        if (src) Q += srcq[q];
         double d = vol_inv * integ[q] + Q;
-        double dd0 = (ap->stage >= 2) ? ap->d0[c + q * qs] : 0.0;
-        double dd1 = (ap->stage >= 3) ? ap->d1[c + q * qs] : 0.0;
         if (ap->dout) ap->dout[c + q * qs] = d;
-        Un[q] = stage_update(ap, ap->Uold[c + q * qs], dd0, dd1, d);
+        Un[q] = stage_update(ap, uold[q], dd0[q], dd1[q], d);
     }
     bool decode_fail = false, invalid = false;
     double rho = Un[0];
