@@ -145,10 +145,9 @@ struct State {
                         0, 0.20, -0.30, 0, 0, 0.01, 1};
     bool profiling = false;
     std::vector<ProfEvent> prof_pending;
-    double prof_ms[2] = {0, 0};
-    long prof_n[2] = {0, 0};
-    long long decode_evals = 0, decode_fallbacks = 0;
-    long long face_evals = 0, face_fallbacks = 0;      // face temperature solves of the thermally perfect path
+    double prof_ms[3] = {0, 0, 0};
+    long prof_n[3] = {0, 0, 0};
+    unsigned long long* stats = nullptr;               // e4::add_stat's copies (device)
     std::vector<int> last_bad;                          // [nblk][3] of the last step: invalid cells + failed decodes
     // graph replay of steps (use_graphs): up to two captured steps (a step
     // swaps U[0] and U[stages], so the starting states of consecutive steps
@@ -331,6 +330,27 @@ int ensure_counters(int slots)
     g_state.abort = g_state.counters_raw;
     g_state.counters = g_state.counters_raw + ABORT_WORDS;
     g_state.counter_slots = slots;
+    return E4HIP_OK;
+}
+
+// The solver statistics: the sums of counters ka and kb over e4::add_stat's
+// copies (after the stream drains); reset clears both in every copy.
+int read_stats(int ka, int kb, long long* a, long long* b, int reset)
+{
+    if (!g_state.init) return fail(E4HIP_ERR_NOT_INIT, "e4hip_init not called");
+    const int n = e4::STAT_LANES * e4::STAT_STRIDE;
+    std::vector<unsigned long long> h(n);
+    HIP_TRY(hipStreamSynchronize(g_state.stream));
+    HIP_TRY(hipMemcpy(h.data(), g_state.stats, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost));
+    unsigned long long sa = 0, sb = 0;
+    for (int l = 0; l < e4::STAT_LANES; ++l) {
+        sa += h[l * e4::STAT_STRIDE + ka];
+        sb += h[l * e4::STAT_STRIDE + kb];
+        if (reset) { h[l * e4::STAT_STRIDE + ka] = 0; h[l * e4::STAT_STRIDE + kb] = 0; }
+    }
+    if (a) *a = (long long)sa;
+    if (b) *b = (long long)sb;
+    if (reset) HIP_TRY(hipMemcpy(g_state.stats, h.data(), sizeof(unsigned long long) * n, hipMemcpyHostToDevice));
     return E4HIP_OK;
 }
 
@@ -971,6 +991,7 @@ int tp_args_for(const BlockH& b, int stage, double dt, unsigned int* counters, e
     const bool lean = b.is_tp() && !b.general && cfg.precision_mode == E4HIP_PRECISION_MIXED &&
                       tp_mixture_form(g_state.gases[b.gm].tp);
     A.retry = lean ? b.retry : nullptr;
+    A.stats = g_state.stats;
     if (lean && !b.retry) return fail(E4HIP_ERR_ARG, "mixture block without its retry list");
     return shock_fields(b, A);
 }
@@ -1208,6 +1229,8 @@ int e4hip_init(int device)
     HIP_TRY(hipMalloc(&g_state.last_signal, sizeof(double)));
     HIP_TRY(hipMalloc(&g_state.queue, 2 * sizeof(unsigned int)));
     HIP_TRY(hipMemset(g_state.queue, 0, 2 * sizeof(unsigned int)));
+    HIP_TRY(hipMalloc(&g_state.stats, sizeof(unsigned long long) * e4::STAT_LANES * e4::STAT_STRIDE));
+    HIP_TRY(hipMemset(g_state.stats, 0, sizeof(unsigned long long) * e4::STAT_LANES * e4::STAT_STRIDE));
     {
         hipDeviceProp_t prop;
         HIP_TRY(hipGetDeviceProperties(&prop, device));
@@ -1242,6 +1265,7 @@ int e4hip_finalize(void)
     if (g_state.smax) HIP_TRY(hipFree(g_state.smax));
     if (g_state.last_signal) HIP_TRY(hipFree(g_state.last_signal));
     if (g_state.queue) HIP_TRY(hipFree(g_state.queue));
+    if (g_state.stats) HIP_TRY(hipFree(g_state.stats));
     g_state.counters = nullptr; g_state.smax = nullptr; g_state.last_signal = nullptr; g_state.queue = nullptr;
     g_state.counter_slots = 0;
     HIP_TRY(hipStreamDestroy(g_state.stream));
@@ -1403,18 +1427,12 @@ int e4hip_tp_guess(int gm, int n, const double* x, double* T)
 
 int e4hip_tp_decode_stats(long long* energy_evals, long long* fallbacks, int reset)
 {
-    if (energy_evals) *energy_evals = g_state.decode_evals;
-    if (fallbacks) *fallbacks = g_state.decode_fallbacks;
-    if (reset) { g_state.decode_evals = 0; g_state.decode_fallbacks = 0; }
-    return E4HIP_OK;
+    return read_stats(e4::STAT_DECODE_EVALS, e4::STAT_DECODE_FALLBACKS, energy_evals, fallbacks, reset);
 }
 
 int e4hip_tp_face_stats(long long* energy_evals, long long* fallbacks, int reset)
 {
-    if (energy_evals) *energy_evals = g_state.face_evals;
-    if (fallbacks) *fallbacks = g_state.face_fallbacks;
-    if (reset) { g_state.face_evals = 0; g_state.face_fallbacks = 0; }
-    return E4HIP_OK;
+    return read_stats(e4::STAT_FACE_EVALS, e4::STAT_FACE_FALLBACKS, energy_evals, fallbacks, reset);
 }
 
 } // extern "C"
@@ -1741,6 +1759,7 @@ static e4::TPArgs tp_args_for(const BlockH& b)
     for (int d = 0; d < 3; ++d) A.fqs[d] = b.fqs[d];
     A.gas = g_state.gases[b.gm].tp_dev;
     A.counters = g_state.counters;
+    A.stats = g_state.stats;
     return A;
 }
 
@@ -1765,17 +1784,18 @@ int e4hip_block_upload(int blk, const double* U, const double* Tguess)
         e4::TPArgs A = tp_args_for(*b);
         A.Pout = b->P[0];
         A.guess = g_state.cfg.learned_eos_guess ? g_state.gases[b->gm].guess_dev : nullptr;
+        hipEvent_t ev = nullptr;
+        prof_begin(&ev);
         if (g_state.cfg.precision_mode == E4HIP_PRECISION_FP64_EXACT || !tp_mixture_form(g_state.gases[b->gm].tp))
             e4::launch_tp_decode_exact(A, b->U[0], Tg, g_state.stream);
         else e4::launch_tp_decode_fast64(A, b->U[0], Tg, g_state.stream);
+        prof_end(2, ev);
         HIP_TRY(hipGetLastError());
         unsigned int cnt[e4::COUNTER_SLOT] = {};
         HIP_TRY(hipMemcpyAsync(cnt, g_state.counters, sizeof(cnt), hipMemcpyDeviceToHost, g_state.stream));
         HIP_TRY(hipStreamSynchronize(g_state.stream));
         HIP_TRY(hipFree(Tg));
         unsigned int nbad = cnt[0];
-        g_state.decode_evals += cnt[2];
-        g_state.decode_fallbacks += cnt[3];
         if (nbad) return fail(E4HIP_ERR_DECODE, "%u cells of block %d do not decode to a valid state", nbad, blk);
         return E4HIP_OK;
     }
@@ -2070,10 +2090,6 @@ static int step_outcome(const unsigned int* h, int nblk, int ns, int* n_bad_cell
         for (int n = 0; n < nblk; ++n) {
             const unsigned int* c = &slots[CS * (n * 3 + s)];
             const unsigned int inv = c[0], dec = c[1];
-            g_state.decode_evals += c[2];
-            g_state.decode_fallbacks += c[3];
-            g_state.face_evals += c[4];
-            g_state.face_fallbacks += c[5];
             stage_bad[s] += (int)(inv + dec);
             g_state.last_bad[3 * (size_t)n + s] = (int)(inv + dec);
             if ((int)inv > g_state.cfg.max_invalid_cells) {
@@ -2381,7 +2397,7 @@ int e4hip_block_kchunk(int blk, int* kchunk)
 
 int e4hip_profile_read(int k, double* total_ms, long* launches)
 {
-    if (k < 0 || k > 1 || !total_ms || !launches) return fail(E4HIP_ERR_ARG, "bad arguments");
+    if (k < 0 || k > 2 || !total_ms || !launches) return fail(E4HIP_ERR_ARG, "bad arguments");
     if (!g_state.prof_pending.empty()) {
         HIP_TRY(hipStreamSynchronize(g_state.stream));
         for (auto& pe : g_state.prof_pending) {

@@ -363,8 +363,8 @@ __global__ __launch_bounds__(128) void gen_face_kernel(TPArgs A)
             fb += __shfl_xor(fb, off);
         }
         if (__lane_id() == 0) {
-            if (fe) atomicAdd(&A.counters[4], (unsigned int)fe);
-            if (fb) atomicAdd(&A.counters[5], (unsigned int)fb);
+            add_stat(A.stats, STAT_FACE_EVALS, (unsigned int)fe);
+            add_stat(A.stats, STAT_FACE_FALLBACKS, (unsigned int)fb);
         }
     }
 }

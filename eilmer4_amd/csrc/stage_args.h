@@ -199,8 +199,24 @@ struct BatchArgs {
 // [4] face energy evaluations, [5] face-state solves whose polish fell back
 // (thermally perfect path).  A slot is 64 bytes: the slots that one stage's
 // launches add to (the same stage of different blocks, three slots apart)
-// never share a 128-byte cache line.
+// never share a 128-byte cache line.  Words 2-5 are unused: the solver
+// statistics go to the spread array below.
 constexpr int COUNTER_SLOT = 16;
+
+// The thermally perfect path's solver statistics, summed over the run:
+// energy evaluations and fall-backs of the decodes and of the face-state
+// solves.  Kept in STAT_LANES copies 128 bytes apart, each wave adding to the
+// copy its workgroup index selects, so that the one atomic per wave of every
+// kernel of a launch does not queue on one cache line (on a single line those
+// atomics set the pace of the whole mixture path); the host adds the copies.
+constexpr int STAT_LANES = 64;
+constexpr int STAT_STRIDE = 16;       // unsigned long longs: 128 bytes
+enum { STAT_DECODE_EVALS = 0, STAT_DECODE_FALLBACKS = 1, STAT_FACE_EVALS = 2, STAT_FACE_FALLBACKS = 3 };
+
+__device__ __forceinline__ void add_stat(unsigned long long* stats, int k, unsigned int v)
+{
+    if (stats && v) atomicAdd(&stats[(blockIdx.x % STAT_LANES) * STAT_STRIDE + k], (unsigned long long)v);
+}
 
 enum { P_RHO = 0, P_VX = 1, P_VY = 2, P_VZ = 3, P_U = 4, P_T = 5, P_P = 6, P_A = 7, P_Y = 8 };
 
@@ -270,6 +286,7 @@ struct TPArgs {
     // (slots of direction 0, then 1, then 2); the update kernel clears the
     // counts.  Null in the other forms.
     unsigned int* retry;
+    unsigned long long* stats;        // solver statistics (STAT_LANES copies, see add_stat)
 };
 
 // Launchers, one per arithmetic mode; each lives in its own translation unit

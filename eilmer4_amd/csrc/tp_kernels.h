@@ -726,8 +726,8 @@ __device__ __forceinline__ void tp_face_body(const TPArgs& A, long f0)
         fb += __shfl_xor(fb, off);
     }
     if (__lane_id() == 0) {
-        if (fe) atomicAdd(&A.counters[4], (unsigned int)fe);
-        if (fb) atomicAdd(&A.counters[5], (unsigned int)fb);
+        add_stat(A.stats, STAT_FACE_EVALS, (unsigned int)fe);
+        add_stat(A.stats, STAT_FACE_FALLBACKS, (unsigned int)fb);
     }
 }
 
@@ -976,8 +976,8 @@ __device__ __forceinline__ void tp_update_body(const TPArgs& A, long c)
     for (int off = 32; off > 0; off >>= 1) fsum_w += __shfl_xor(fsum_w, off);
     if (__lane_id() == 0) {
         count_bad_cells(A.counters, A.abort_this, A.max_invalid, nbad, nfail);
-        if (fsum_w) atomicAdd(&A.counters[2], (unsigned int)fsum_w);
-        if (nfb) atomicAdd(&A.counters[3], nfb);
+        add_stat(A.stats, STAT_DECODE_EVALS, (unsigned int)fsum_w);
+        add_stat(A.stats, STAT_DECODE_FALLBACKS, nfb);
     }
 }
 
@@ -1062,8 +1062,8 @@ __global__ __launch_bounds__(128) void tp_decode_kernel(TPArgs A, const double* 
     for (int off = 32; off > 0; off >>= 1) fsum_w += __shfl_xor(fsum_w, off);
     if (__lane_id() == 0) {
         if (nb) atomicAdd(&A.counters[0], nb);
-        if (fsum_w) atomicAdd(&A.counters[2], (unsigned int)fsum_w);
-        if (nfb) atomicAdd(&A.counters[3], nfb);
+        add_stat(A.stats, STAT_DECODE_EVALS, (unsigned int)fsum_w);
+        add_stat(A.stats, STAT_DECODE_FALLBACKS, nfb);
     }
 }
 

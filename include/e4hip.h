@@ -321,8 +321,8 @@ int e4hip_determine_dt_noalloc(const int* blks, int nblk, double dt_current, dou
 
 /* Per-kernel timing with HIP events on the library stream (off by default).
  * e4hip_profile_read returns, for kernel class k (0 = stage kernels,
- * 1 = CFL kernel), the summed milliseconds and launch count since the last
- * read, and resets them. */
+ * 1 = CFL kernel, 2 = the decode of an uploaded mixture state), the summed
+ * milliseconds and launch count since the last read, and resets them. */
 int e4hip_profile_enable(int on);
 int e4hip_profile_read(int k, double* total_ms, long* launches);
 
