@@ -180,8 +180,9 @@ def c5_workload(e4, workloads, steps=6, warmup=3):
                              launch per kernel (the ideal blocks' stage kernel
                              persistent over a tile queue), steps replayed
                              from captured HIP graphs
-      mixed+learned+batched  mixed precision, learned decode guess, batched,
-                             graphs
+      mixed+batched          mixed precision (FP32 reconstruction and flux,
+                             short face solves, FP64 decode), batched, graphs
+      mixed+learned+batched  as mixed+batched with the learned decode guess
     with each variant's speed-up over plain and its difference from plain.
     The warm-up steps include the k-chunk tuning and the graph captures."""
     import torch
@@ -194,7 +195,7 @@ def c5_workload(e4, workloads, steps=6, warmup=3):
     log("C5 workload: %d blocks, %.2f M cells, generated in %.1f s" % (len(descs), ncells / 1e6,
                                                                      time.perf_counter() - t0))
     variants = [("plain", "fp64", False, False), ("fp64+batched", "fp64", True, False),
-                ("mixed+learned+batched", "mixed", True, True)]
+                ("mixed+batched", "mixed", True, False), ("mixed+learned+batched", "mixed", True, True)]
     out, finals, dt_fixed = {}, {}, None
     for name, prec, batch, learned in variants:
         blks = []
