@@ -351,10 +351,13 @@ __global__ __launch_bounds__(128) void gen_face_kernel(TPArgs A)
         F[1] = m[0]; F[2] = m[1]; F[3] = m[2];
     }
     if (active) {
-        double* dst = A.F[DIR];
+        // mixture blocks keep the flux in the element type the mixture update
+        // reads (FP32 in the mixed form)
+        using FS = typename std::conditional<(NSP > 0), TPFluxStore<M>, double>::type;
+        FS* dst = reinterpret_cast<FS*>(A.F[DIR]);
         const long fq = A.fqs[DIR];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) dst[q * fq + f] = F[q];
+        for (int q = 0; q < NQ; ++q) dst[q * fq + f] = FS(F[q]);
     }
     if constexpr (NSP > 0) {
         if (!active) { fe = 0; fb = 0; }

@@ -34,6 +34,12 @@ template <> struct TPArith<ModeMixed> { using type = ModeFast64; static constexp
 template <class M> struct TPFlux { using type = typename TPArith<M>::type; };
 template <> struct TPFlux<ModeMixed> { using type = ModeMixed; };
 
+// Element type of a mixture block's face-flux arrays: the mixed form computes
+// its fluxes in FP32 and keeps them as floats (packed in the first half of each
+// array), which halves the flux traffic between the face and update kernels.
+template <class M>
+using TPFluxStore = typename std::conditional<std::is_same<M, ModeMixed>::value, float, double>::type;
+
 // update_thermo_from_rhou of a face state (therm_perf_gas.d:187-191); with
 // polish, Newton from Q.T first (*fallbacks counts the solves it hands on).
 template <class M, int NSP, bool POLISH>
@@ -705,11 +711,12 @@ __device__ __forceinline__ void tp_face_body(const TPArgs& A, long f0)
         R Fr[5 + NSP];
         const bool shock = (FLUX == FLUX_SHOCK) && ((r[1].s && *r[1].s) || (r[2].s && *r[2].s));
         tp_flux<MF, NSP, FLUX>(A, *A.gas, Lf, Rf, shock, Fr);
-        double F[5 + NSP];
+        using FS = TPFluxStore<M>;
+        FS F[5 + NSP];
     #pragma unroll
-        for (int q = 0; q < 5 + NSP; ++q) F[q] = double(Fr[q]);
+        for (int q = 0; q < 5 + NSP; ++q) F[q] = FS(Fr[q]);
         if (active) {
-            double* dst = A.F[DIR];
+            FS* dst = reinterpret_cast<FS*>(A.F[DIR]);
             const long fq = A.fqs[DIR];
             dst[f] = F[0];
             dst[(1 + pn) * fq + f] = F[1];
@@ -824,12 +831,14 @@ __device__ __forceinline__ void tp_update_body(const TPArgs& A, long c)
         double integ[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) integ[q] = 0.0;
-        auto add_face = [&](const double* Fsrc, long fq, long fidx, double outsign, double Aface) {
+        using FS = TPFluxStore<M>;
+        auto add_face = [&](const double* Fraw, long fq, long fidx, double outsign, double Aface) {
+            const FS* Fsrc = reinterpret_cast<const FS*>(Fraw);
             double area = outsign * Aface;
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 if (q == 3 && A.dims != 3) continue;
-                integ[q] -= Fsrc[q * fq + fidx] * area;
+                integ[q] -= double(Fsrc[q * fq + fidx]) * area;
             }
         };
         double vol_inv = A.vol_inv, qy = 0.0;
@@ -1258,15 +1267,16 @@ __global__ __launch_bounds__(64) void tp_boundary_face_kernel(TPArgs A)
     }
     const int nfi = A.ni + (d == 0), nfj = A.nj + (d == 1);
     const long f = ((long)k * nfj + j) * nfi + i;
-    double* dst = A.F[d];
+    using FS = TPFluxStore<M>;
+    FS* dst = reinterpret_cast<FS*>(A.F[d]);
     const long fq = A.fqs[d];
-    dst[f] = F[0];
-    dst[(1 + pn) * fq + f] = F[1];
-    dst[(1 + pt1) * fq + f] = F[2];
-    dst[(1 + pt2) * fq + f] = F[3];
-    dst[4 * fq + f] = F[4];
+    dst[f] = FS(F[0]);
+    dst[(1 + pn) * fq + f] = FS(F[1]);
+    dst[(1 + pt1) * fq + f] = FS(F[2]);
+    dst[(1 + pt2) * fq + f] = FS(F[3]);
+    dst[4 * fq + f] = FS(F[4]);
 #pragma unroll
-    for (int s = 0; s < NSP; ++s) dst[(5 + s) * fq + f] = F[5 + s];
+    for (int s = 0; s < NSP; ++s) dst[(5 + s) * fq + f] = FS(F[5 + s]);
 }
 
 // x_order-1 ghost cells of one face of a mixture block (extrapolate_copy.d:
