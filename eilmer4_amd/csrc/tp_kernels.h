@@ -990,8 +990,11 @@ __device__ __forceinline__ void tp_update_body(const TPArgs& A, long c)
     }
 }
 
+#ifndef E4_TP_UPDATE_WAVES
+#define E4_TP_UPDATE_WAVES 3
+#endif
 template <class M, int NSP, bool GUESS, bool BATCH>
-__global__ __launch_bounds__(128) void tp_update_kernel(typename std::conditional<BATCH, BatchArgs, TPArgs>::type a)
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(E4_TP_UPDATE_WAVES))) void tp_update_kernel(typename std::conditional<BATCH, BatchArgs, TPArgs>::type a)
 {
     if constexpr (BATCH) {
         int e;
