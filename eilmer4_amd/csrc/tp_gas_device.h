@@ -340,6 +340,55 @@ __device__ __forceinline__ void tp_mix_eval(const TPMix& X, const CeaCurve& c0, 
     *d2u = d2h;
 }
 
+// tp_mix_eval<true> from the mass fractions instead of tp_mix_all's sums:
+// each evaluation forms the species sums of the segments that some lane of
+// the wave weights (a wave-uniform test), so that a solve keeps NSP mass
+// fractions in registers instead of all the segments' sums.  Per lane the
+// arithmetic is that of tp_mix_eval<true>: a segment no lane weights would add
+// w * M = 0.
+template <int NSP>
+__device__ __forceinline__ void tp_mix_eval_y(const TPGas& g, const double* Y, double Rmix, double T, double* u,
+                                              double* du, double* d2u)
+{
+    const CeaCurve& c0 = g.curve[0];
+    const double Tlo = uniform_f64(c0.Tb[0]), Thi = uniform_f64(c0.Tb[TP_FAST_NSEG]);
+    const double Tc = fmin(fmax(T, Tlo), Thi);
+    double w[3], dw[3];
+    tp_seg_weights(c0, Tc, w, dw);
+    double Mk[8], dM[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) { Mk[k] = 0.0; dM[k] = 0.0; }
+#pragma unroll
+    for (int q = 0; q < TP_FAST_NSEG; ++q) {
+        if (__ballot(w[q] != 0.0 || dw[q] != 0.0) == 0) continue;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            double m = 0.0;
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) m += Y[s] * g.RC[q][k][s];
+            Mk[k] += w[q] * m;
+            dM[k] += dw[q] * m;
+        }
+    }
+    const double r = dv_seed(1.0, Tc, rcp_seed(Tc));
+    const double lnT = log_fast(Tc);
+    const double T2 = Tc * Tc, T3 = T2 * Tc, T4 = T2 * T2;
+    double h = tp_mix_h(Mk, Tc, r, lnT);
+    const double cp = Mk[0] * r * r + Mk[1] * r + Mk[2] + Mk[3] * Tc + Mk[4] * T2 + Mk[5] * T3 + Mk[6] * T4;
+    double dh = cp;
+    double d2h = -2.0 * Mk[0] * r * r * r - Mk[1] * r * r + Mk[3] + 2.0 * Mk[4] * Tc + 3.0 * Mk[5] * T2 +
+                 4.0 * Mk[6] * T3;
+    if (dw[0] != 0.0 || dw[1] != 0.0 || dw[2] != 0.0) {
+        dh += tp_mix_h(dM, Tc, r, lnT);
+        d2h += 2.0 * (dM[0] * r * r + dM[1] * r + dM[2] + dM[3] * Tc + dM[4] * T2 + dM[5] * T3 + dM[6] * T4);
+    }
+    if (T < Tlo) { h -= cp * (Tlo - T); dh = cp; d2h = 0.0; }
+    else if (T > Thi) { h += cp * (T - Thi); dh = cp; d2h = 0.0; }
+    *u = h - Rmix * T;
+    *du = dh - Rmix;
+    *d2u = d2h;
+}
+
 // therm_perf_gas_mix_eos.d:63-69 (mass_average, gas_model.d:330-338)
 template <class M, int NSP>
 __device__ __forceinline__ double tp_energy(const TPGas& g, const double* Y, double T)

@@ -83,7 +83,7 @@ __device__ __forceinline__ int tp_face_eos(const TPGas& g, TPState<NSP>& Q, int*
 // temperatures); a face with a side that did not goes to the retry kernel,
 // whose solve is the polish and bracketed Newton of tp_face_eos.
 template <int NSP>
-__device__ __forceinline__ void tp_face_halley_pair(const TPGas& g, const TPMix& XL, const TPMix& XR, double RL,
+__device__ __forceinline__ void tp_face_halley_pair(const TPGas& g, const double* YL, const double* YR, double RL,
                                                     double RR, double eL, double eR, double& TL, double& TR,
                                                     bool& okL, bool& okR, int* fevals)
 {
@@ -110,8 +110,8 @@ __device__ __forceinline__ void tp_face_halley_pair(const TPGas& g, const TPMix&
     for (int it = 0; it < 3; ++it) {
         if (!(liveL || liveR)) break;
         double uL, duL, d2uL, uR, duR, d2uR;
-        tp_mix_eval<true>(XL, c0, RL, tL, &uL, &duL, &d2uL);
-        tp_mix_eval<true>(XR, c0, RR, tR, &uR, &duR, &d2uR);
+        tp_mix_eval_y<NSP>(g, YL, RL, tL, &uL, &duL, &d2uL);
+        tp_mix_eval_y<NSP>(g, YR, RR, tR, &uR, &duR, &d2uR);
         tL = step(tL, uL, duL, d2uL, eL, TL, liveL, okL);
         tR = step(tR, uR, duR, d2uR, eR, TR, liveR, okR);
     }
@@ -663,12 +663,9 @@ __device__ __forceinline__ void tp_face_body(const TPArgs& A, long f0)
             double YL[NSP], YR[NSP];
 #pragma unroll
             for (int s = 0; s < NSP; ++s) { YL[s] = double(Lf.Y[s]); YR[s] = double(Rf.Y[s]); }
-            TPMix XL, XR;
-            tp_mix_all<NSP>(*A.gas, YL, XL);
-            tp_mix_all<NSP>(*A.gas, YR, XR);
             const double RmL = tp_Rmix<NSP>(*A.gas, YL), RmR = tp_Rmix<NSP>(*A.gas, YR);
             double TfL = TL, TfR = TR;
-            tp_face_halley_pair<NSP>(*A.gas, XL, XR, RmL, RmR, double(Lf.u), double(Rf.u), TfL, TfR, okL, okR, &fe);
+            tp_face_halley_pair<NSP>(*A.gas, YL, YR, RmL, RmR, double(Lf.u), double(Rf.u), TfL, TfR, okL, okR, &fe);
             if (okL) { Lf.T = R(TfL); Lf.p = R(double(Lf.rho) * RmL * TfL); Lf.a = R(aL); }
             if (okR) { Rf.T = R(TfR); Rf.p = R(double(Rf.rho) * RmR * TfR); Rf.a = R(aR); }
             defer = !(okL && okR);
@@ -741,7 +738,7 @@ __device__ __forceinline__ void tp_face_body(const TPArgs& A, long f0)
 // One thread per face of direction DIR; BATCH: a launch over several
 // blocks, each workgroup in the block its map entry names.
 #ifndef E4_TP_LEAN_WAVES
-#define E4_TP_LEAN_WAVES 2
+#define E4_TP_LEAN_WAVES 3
 #endif
 template <class M> struct TPFaceOccupancy { static constexpr int waves = 2; };
 template <> struct TPFaceOccupancy<ModeMixed> { static constexpr int waves = E4_TP_LEAN_WAVES; };
