@@ -109,8 +109,8 @@ __device__ __forceinline__ float mlp_guess_wave(const GuessModel& G, const doubl
 // Newton steps with the reference's dzdT, sum_s Y_s (Cp_s - R_s), which inside a blend is not the
 // derivative of the blended energy (convergence there is only linear, hence a
 // test on the residual rather than on the size of the last step).
-template <class M, int NSP>
-__device__ __forceinline__ int tp_polish_x(const TPGas& g, const TPMix& X, double Rmix, TPState<NSP>& Q, double e_tgt,
+template <class M, int NSP, class MX>
+__device__ __forceinline__ int tp_polish_x(const TPGas& g, const MX& X, double Rmix, TPState<NSP>& Q, double e_tgt,
                                            double T0, int* fevals)
 {
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;

@@ -340,13 +340,13 @@ __device__ __forceinline__ void tp_mix_eval(const TPMix& X, const CeaCurve& c0, 
     *d2u = d2h;
 }
 
-// tp_mix_eval<true> from the mass fractions instead of tp_mix_all's sums:
-// each evaluation forms the species sums of the segments that some lane of
-// the wave weights (a wave-uniform test), so that a solve keeps NSP mass
+// tp_mix_eval from the mass fractions instead of tp_mix_all's sums: each
+// evaluation forms the species sums of the segments that some lane of the
+// wave weights (a wave-uniform test), so that a solve keeps NSP mass
 // fractions in registers instead of all the segments' sums.  Per lane the
-// arithmetic is that of tp_mix_eval<true>: a segment no lane weights would add
+// arithmetic is that of tp_mix_eval: a segment no lane weights would add
 // w * M = 0.
-template <int NSP>
+template <bool SLOPE, int NSP>
 __device__ __forceinline__ void tp_mix_eval_y(const TPGas& g, const double* Y, double Rmix, double T, double* u,
                                               double* du, double* d2u)
 {
@@ -360,14 +360,14 @@ __device__ __forceinline__ void tp_mix_eval_y(const TPGas& g, const double* Y, d
     for (int k = 0; k < 8; ++k) { Mk[k] = 0.0; dM[k] = 0.0; }
 #pragma unroll
     for (int q = 0; q < TP_FAST_NSEG; ++q) {
-        if (__ballot(w[q] != 0.0 || dw[q] != 0.0) == 0) continue;
+        if (__ballot(w[q] != 0.0 || (SLOPE && dw[q] != 0.0)) == 0) continue;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             double m = 0.0;
 #pragma unroll
             for (int s = 0; s < NSP; ++s) m += Y[s] * g.RC[q][k][s];
             Mk[k] += w[q] * m;
-            dM[k] += dw[q] * m;
+            if constexpr (SLOPE) dM[k] += dw[q] * m;
         }
     }
     const double r = dv_seed(1.0, Tc, rcp_seed(Tc));
@@ -375,18 +375,47 @@ __device__ __forceinline__ void tp_mix_eval_y(const TPGas& g, const double* Y, d
     const double T2 = Tc * Tc, T3 = T2 * Tc, T4 = T2 * T2;
     double h = tp_mix_h(Mk, Tc, r, lnT);
     const double cp = Mk[0] * r * r + Mk[1] * r + Mk[2] + Mk[3] * Tc + Mk[4] * T2 + Mk[5] * T3 + Mk[6] * T4;
-    double dh = cp;
-    double d2h = -2.0 * Mk[0] * r * r * r - Mk[1] * r * r + Mk[3] + 2.0 * Mk[4] * Tc + 3.0 * Mk[5] * T2 +
-                 4.0 * Mk[6] * T3;
-    if (dw[0] != 0.0 || dw[1] != 0.0 || dw[2] != 0.0) {
-        dh += tp_mix_h(dM, Tc, r, lnT);
-        d2h += 2.0 * (dM[0] * r * r + dM[1] * r + dM[2] + dM[3] * Tc + dM[4] * T2 + dM[5] * T3 + dM[6] * T4);
+    double dh = cp, d2h = 0.0;
+    if constexpr (SLOPE) {
+        d2h = -2.0 * Mk[0] * r * r * r - Mk[1] * r * r + Mk[3] + 2.0 * Mk[4] * Tc + 3.0 * Mk[5] * T2 +
+              4.0 * Mk[6] * T3;
+        if (dw[0] != 0.0 || dw[1] != 0.0 || dw[2] != 0.0) {
+            dh += tp_mix_h(dM, Tc, r, lnT);
+            d2h += 2.0 * (dM[0] * r * r + dM[1] * r + dM[2] + dM[3] * Tc + dM[4] * T2 + dM[5] * T3 + dM[6] * T4);
+        }
     }
     if (T < Tlo) { h -= cp * (Tlo - T); dh = cp; d2h = 0.0; }
     else if (T > Thi) { h += cp * (T - Thi); dh = cp; d2h = 0.0; }
     *u = h - Rmix * T;
     *du = dh - Rmix;
     *d2u = d2h;
+}
+
+// The composition as the fast solves see it through tp_mix_eval: TPMix, the
+// segment sums formed once per solve, or TPMixY, the mass fractions with the
+// sums formed at each evaluation (tp_mix_eval_y: fewer registers held through
+// the solve, more arithmetic per evaluation).
+template <int NSP>
+struct TPMixY {
+    const TPGas* g;
+    double Y[NSP];
+};
+
+template <int NSP>
+__device__ __forceinline__ TPMixY<NSP> tp_mix_y(const TPGas& g, const double* Y)
+{
+    TPMixY<NSP> X;
+    X.g = &g;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) X.Y[s] = Y[s];
+    return X;
+}
+
+template <bool SLOPE, int NSP>
+__device__ __forceinline__ void tp_mix_eval(const TPMixY<NSP>& X, const CeaCurve&, double Rmix, double T, double* u,
+                                            double* du, double* d2u)
+{
+    tp_mix_eval_y<SLOPE, NSP>(*X.g, X.Y, Rmix, T, u, du, d2u);
 }
 
 // therm_perf_gas_mix_eos.d:63-69 (mass_average, gas_model.d:330-338)
@@ -449,8 +478,8 @@ __device__ __forceinline__ void tp_u_dzdT(const TPGas& g, const double* Y, doubl
 // All energy evaluations (the two bracket ends, then the iterates from x0)
 // go through one call site in one loop, which keeps the inlined code to one
 // copy of the energy evaluation.
-template <class M, int NSP>
-__device__ __forceinline__ int tp_newton(const TPGas& g, const TPMix& X, double Rmix, TPState<NSP>& Q, double e_tgt,
+template <class M, int NSP, class MX>
+__device__ __forceinline__ int tp_newton(const TPGas& g, const MX& X, double Rmix, TPState<NSP>& Q, double e_tgt,
                                          double x0, double xMin, double xMax, double tol, double* root, int* fevals)
 {
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
@@ -506,8 +535,8 @@ __device__ __forceinline__ int tp_newton(const TPGas& g, const TPMix& X, double 
 // therm_perf_gas_mix_eos.d:70-153: guess = Q.T, bracket [max(T-500, 20), +1000],
 // then [20, 1e5]; tolerance 1e-6 K.  X and Rmix: the composition's segment
 // sums and gas constant (fast modes; unused by ModeExact).
-template <class M, int NSP>
-__device__ __forceinline__ int tp_update_temperature_x(const TPGas& g, const TPMix& X, double Rmix, TPState<NSP>& Q,
+template <class M, int NSP, class MX>
+__device__ __forceinline__ int tp_update_temperature_x(const TPGas& g, const MX& X, double Rmix, TPState<NSP>& Q,
                                                        int* fevals)
 {
     const double Tsave = Q.T;
@@ -547,8 +576,8 @@ __device__ __forceinline__ int tp_update_temperature(const TPGas& g, TPState<NSP
 
 // Frozen sound speed a = sqrt(Cp/Cv R T) (therm_perf_gas.d:345-385) from the
 // composition's segment sums (fast modes).
-template <class M>
-__device__ __forceinline__ double tp_sound_speed_x(const TPGas& g, const TPMix& X, double Rmix, double T)
+template <class M, class MX>
+__device__ __forceinline__ double tp_sound_speed_x(const TPGas& g, const MX& X, double Rmix, double T)
 {
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
     double u, cv, d2u;

@@ -46,9 +46,10 @@ template <class M, int NSP, bool POLISH>
 __device__ __forceinline__ int tp_face_eos(const TPGas& g, TPState<NSP>& Q, int* fevals, int* fallbacks)
 {
     if constexpr (M::kFast) {
-        // one set of segment sums serves the polish and the bracketed solve
-        TPMix X;
-        tp_mix_all<NSP>(g, Q.Y, X);
+        // the composition for the polish and the bracketed solve, its
+        // segment sums formed at each evaluation: fewer registers in the face
+        // kernels (three waves per SIMD)
+        const TPMixY<NSP> X = tp_mix_y<NSP>(g, Q.Y);
         const double Rmix = tp_Rmix<NSP>(g, Q.Y);
         int rc = -1;
         if constexpr (POLISH) {
@@ -110,8 +111,8 @@ __device__ __forceinline__ void tp_face_halley_pair(const TPGas& g, const double
     for (int it = 0; it < 3; ++it) {
         if (!(liveL || liveR)) break;
         double uL, duL, d2uL, uR, duR, d2uR;
-        tp_mix_eval_y<NSP>(g, YL, RL, tL, &uL, &duL, &d2uL);
-        tp_mix_eval_y<NSP>(g, YR, RR, tR, &uR, &duR, &d2uR);
+        tp_mix_eval_y<true, NSP>(g, YL, RL, tL, &uL, &duL, &d2uL);
+        tp_mix_eval_y<true, NSP>(g, YR, RR, tR, &uR, &duR, &d2uR);
         tL = step(tL, uL, duL, d2uL, eL, TL, liveL, okL);
         tR = step(tR, uR, duR, d2uR, eR, TR, liveR, okR);
     }
@@ -740,7 +741,7 @@ __device__ __forceinline__ void tp_face_body(const TPArgs& A, long f0)
 #ifndef E4_TP_LEAN_WAVES
 #define E4_TP_LEAN_WAVES 3
 #endif
-template <class M> struct TPFaceOccupancy { static constexpr int waves = 2; };
+template <class M> struct TPFaceOccupancy { static constexpr int waves = 3; };
 template <> struct TPFaceOccupancy<ModeMixed> { static constexpr int waves = E4_TP_LEAN_WAVES; };
 
 template <class M, int NSP, int FLUX, int DIR, bool BATCH>
@@ -917,7 +918,9 @@ __device__ __forceinline__ void tp_update_body(const TPArgs& A, long c)
         int rc = -1;
         if constexpr (MA::kFast) {
             // one set of segment sums for the polish, the bracketed solve and
-            // the sound speed
+            // the sound speed (sums formed per evaluation, TPMixY, measured
+            // slower here: 326 -> 357 us for the mixed form, 366 -> 534 us for
+            // the plain one at 128^3, although they allow four waves per SIMD)
             TPMix X;
             tp_mix_all<NSP>(*A.gas, Q.Y, X);
             const double Rmix = tp_Rmix<NSP>(*A.gas, Q.Y);
