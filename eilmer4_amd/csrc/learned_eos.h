@@ -10,7 +10,10 @@
 //   C (16 x 16):           col l%16, rows 4*(l/16) .. 4*(l/16)+3
 // A wave's 64 cells are four 16-row tiles; hidden layers of 64 units are four
 // 16-column tiles and two 32-wide k-chunks.  Activations pass between layers
-// through LDS as BF16 rows (one row per cell).
+// through LDS as BF16 rows (one row per cell), in one buffer per wave that each
+// layer overwrites in place: a layer loads all its A operands into registers
+// before it stores any output (a wave's LDS operations run in program order),
+// which halves the LDS of the kernels that evaluate the network.
 #pragma once
 #include <hip/hip_runtime.h>
 #include "tp_gas_device.h"
@@ -23,7 +26,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int GM_NH = 64;
 constexpr int GM_K1 = 32;            // input features padded to one k-chunk
 constexpr int GM_RS = 72;            // LDS row stride (bf16) of the activation buffers
-constexpr int GM_LDS_PER_WAVE = 2 * 64 * GM_RS;   // two activation buffers, bf16 elements
+constexpr int GM_LDS_PER_WAVE = 64 * GM_RS;       // one activation buffer, bf16 elements
 
 struct GuessModel {
     float in_mean[8], in_std[8];     // feature normalisation (rho, u, Y0..Y4)
@@ -36,12 +39,20 @@ struct GuessModel {
 
 __device__ __forceinline__ float gm_silu(float x) { return x * __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
 
-// One dense layer of the wave: out = silu(in . W^T + b) for 64 rows.
+// One dense layer of the wave, in place: buf = silu(buf . W^T + b) for 64 rows
+// (K input columns, 64 output columns).  buf is not restrict-qualified: the
+// stores must stay after the loads.
 template <int K>
-__device__ __forceinline__ void gm_layer(const __bf16* __restrict__ in, __bf16* __restrict__ out,
-                                         const __bf16* __restrict__ W, const float* __restrict__ b, int lane)
+__device__ __forceinline__ void gm_layer(__bf16* buf, const __bf16* __restrict__ W, const float* __restrict__ b,
+                                         int lane)
 {
     const int r16 = lane & 15, kq = lane >> 4;
+    bf16x8 a[4][K / 32];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int kc = 0; kc < K / 32; ++kc)
+            a[rt][kc] = *reinterpret_cast<const bf16x8*>(buf + (rt * 16 + r16) * GM_RS + kc * 32 + 8 * kq);
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt) {
 #pragma unroll
@@ -49,14 +60,13 @@ __device__ __forceinline__ void gm_layer(const __bf16* __restrict__ in, __bf16* 
             f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int kc = 0; kc < K / 32; ++kc) {
-                bf16x8 a = *reinterpret_cast<const bf16x8*>(in + (rt * 16 + r16) * GM_RS + kc * 32 + 8 * kq);
                 bf16x8 bw = *reinterpret_cast<const bf16x8*>(W + (nt * 16 + r16) * K + kc * 32 + 8 * kq);
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rt][kc], bw, acc, 0, 0, 0);
             }
             const int col = nt * 16 + r16;
             const float bias = b[col];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) out[(rt * 16 + 4 * kq + q) * GM_RS + col] = (__bf16)gm_silu(acc[q] + bias);
+            for (int q = 0; q < 4; ++q) buf[(rt * 16 + 4 * kq + q) * GM_RS + col] = (__bf16)gm_silu(acc[q] + bias);
         }
     }
 }
@@ -67,23 +77,22 @@ __device__ __forceinline__ void gm_layer(const __bf16* __restrict__ in, __bf16* 
 __device__ __forceinline__ float mlp_guess_wave(const GuessModel& G, const double* x, __bf16* lds)
 {
     const int lane = __lane_id();
-    __bf16* bufA = lds;
-    __bf16* bufB = lds + 64 * GM_RS;
+    __bf16* buf = lds;
 #pragma unroll
     for (int f = 0; f < GM_K1; ++f) {
         float z = (f < 7) ? ((float)x[f] - G.in_mean[f]) * G.in_std[f] : 0.0f;
-        bufA[lane * GM_RS + f] = (__bf16)z;
+        buf[lane * GM_RS + f] = (__bf16)z;
     }
     __syncthreads();
-    gm_layer<GM_K1>(bufA, bufB, G.W1, G.b1, lane);
+    gm_layer<GM_K1>(buf, G.W1, G.b1, lane);
     __syncthreads();
-    gm_layer<GM_NH>(bufB, bufA, G.W2, G.b2, lane);
+    gm_layer<GM_NH>(buf, G.W2, G.b2, lane);
     __syncthreads();
-    gm_layer<GM_NH>(bufA, bufB, G.W3, G.b3, lane);
+    gm_layer<GM_NH>(buf, G.W3, G.b3, lane);
     __syncthreads();
     float y = G.b4;
 #pragma unroll 8
-    for (int n = 0; n < GM_NH; ++n) y += G.w4[n] * (float)bufB[lane * GM_RS + n];
+    for (int n = 0; n < GM_NH; ++n) y += G.w4[n] * (float)buf[lane * GM_RS + n];
     __syncthreads();
     return G.T_mean + G.T_std * y;
 }
