@@ -49,7 +49,7 @@ void launch_tp_decode_fast64(const TPArgs& A, const double* U, const double* Tgu
 __global__ __launch_bounds__(128) void mlp_guess_kernel(const GuessModel* G, const double* __restrict__ x, int n,
                                                         double* __restrict__ T)
 {
-    __shared__ __bf16 lds[2 * GM_LDS_PER_WAVE];
+    __shared__ __attribute__((aligned(16))) __bf16 lds[2 * GM_LDS_PER_WAVE];
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double xi[7] = {1.0, 1.0e6, 0.2, 0.2, 0.2, 0.2, 0.2};
     if (c < n)

@@ -906,7 +906,7 @@ __device__ __forceinline__ void tp_update_body(const TPArgs& A, long c)
     // learned initial guess for the whole wave (all lanes take part)
     double Tguess = 0.0;
     if constexpr (GUESS) {
-        __shared__ __bf16 gm_lds[2 * GM_LDS_PER_WAVE];
+        __shared__ __attribute__((aligned(16))) __bf16 gm_lds[2 * GM_LDS_PER_WAVE];
         double x[7] = {have_state ? Q.rho : 1.0, have_state ? Q.u : 1.0e6, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int s = 0; s < 5 && s < NSP; ++s) x[2 + s] = have_state ? Q.Y[s] : 0.2;
@@ -1012,7 +1012,7 @@ __global__ __launch_bounds__(128) void tp_decode_kernel(TPArgs A, const double* 
                                                         const double* __restrict__ Tguess)
 {
     using MD = typename std::conditional<M::kFast, ModeFast64, ModeExact>::type;
-    __shared__ __bf16 gm_lds[2 * GM_LDS_PER_WAVE];
+    __shared__ __attribute__((aligned(16))) __bf16 gm_lds[2 * GM_LDS_PER_WAVE];
     const long ncells = (long)A.ni * A.nj * A.nk;
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long qs = A.qs;
