@@ -209,6 +209,22 @@ constexpr int COUNTER_SLOT = 16;
 // copy its workgroup index selects, so that the one atomic per wave of every
 // kernel of a launch does not queue on one cache line (on a single line those
 // atomics set the pace of the whole mixture path); the host adds the copies.
+// Workgroups go to the GPU's eight XCDs (each with its own L2) round-robin in
+// dispatch order.  xcd_contiguous maps dispatch index L of n to a logical
+// index such that each XCD receives a contiguous run of logical workgroups:
+// neighbouring tiles, which read each other's halo cells, then share one L2.
+// A bijection on [0, n): the last n % 8 workgroups keep their index.
+#ifndef E4_XCD_REMAP
+#define E4_XCD_REMAP 1
+#endif
+constexpr int GPU_XCDS = 8;
+__device__ __forceinline__ int xcd_contiguous(int L, int n)
+{
+    if (!E4_XCD_REMAP) return L;
+    const int per = n / GPU_XCDS;
+    return (L < per * GPU_XCDS) ? (L % GPU_XCDS) * per + L / GPU_XCDS : L;
+}
+
 constexpr int STAT_LANES = 64;
 constexpr int STAT_STRIDE = 16;       // unsigned long longs: 128 bytes
 enum { STAT_DECODE_EVALS = 0, STAT_DECODE_FALLBACKS = 1, STAT_FACE_EVALS = 2, STAT_FACE_FALLBACKS = 3 };
