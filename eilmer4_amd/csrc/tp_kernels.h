@@ -878,12 +878,17 @@ __device__ __forceinline__ void tp_update_body(const TPArgs& A, long c)
             double d = vol_inv * integ[q] + Qs;
             double dd0 = (A.stage >= 2) ? A.d0[c + q * qs] : 0.0;
             double dd1 = (A.stage >= 3) ? A.d1[c + q * qs] : 0.0;
-            if (A.dout) A.dout[c + q * qs] = d;
+            // dU/dt and U are stored now (the low-T fallback below re-stores
+            // what it changes), as 64-bit integers: a store of a double might
+            // alias the gas model's tables read after it, which would force
+            // vector loads of those uniform tables; an integer store cannot
+            // (type-based alias analysis), so they stay scalar loads
+            if (A.dout) reinterpret_cast<unsigned long long*>(A.dout)[c + q * qs] = __builtin_bit_cast(unsigned long long, d);
             double uold = A.Uold[c + q * qs];
             if (A.stage == 1) Un[q] = uold + A.c1 * d;
             else if (A.stage == 2) Un[q] = uold + A.dt * (A.g1 * dd0 + A.g2 * d);
             else Un[q] = uold + A.dt * (A.g1 * dd0 + A.g2 * dd1 + A.g3 * d);
-            A.Unew[c + q * qs] = Un[q];   // stored now; the low-T fallback below re-stores what it changes
+            reinterpret_cast<unsigned long long*>(A.Unew)[c + q * qs] = __builtin_bit_cast(unsigned long long, Un[q]);
         }
         // decode_conserved (fvcell.d:563-732): density, velocity, u, mass fractions
         rho = Un[0];
@@ -954,12 +959,14 @@ __device__ __forceinline__ void tp_update_body(const TPArgs& A, long c)
             // ignore_low_T_thermo_update_failure: T = 200 K and re-encode U
             Q.T = 200.0;
             tp_update_thermo_from_rhoT<MA, NSP>(*A.gas, Q);
-            double* Un_out = A.Unew + c;
-            Un_out[qs] = rho * v[0]; Un_out[2 * qs] = rho * v[1]; Un_out[3 * qs] = rho * v[2];
+            // (integer stores, as above)
+            unsigned long long* Un_out = reinterpret_cast<unsigned long long*>(A.Unew) + c;
+            auto bits = [](double x) { return __builtin_bit_cast(unsigned long long, x); };
+            Un_out[qs] = bits(rho * v[0]); Un_out[2 * qs] = bits(rho * v[1]); Un_out[3 * qs] = bits(rho * v[2]);
             double ke2 = 0.5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-            Un_out[4 * qs] = rho * (Q.u + ke2);
+            Un_out[4 * qs] = bits(rho * (Q.u + ke2));
 #pragma unroll
-            for (int s = 0; s < NSP; ++s) Un_out[(5 + s) * qs] = rho * Q.Y[s];
+            for (int s = 0; s < NSP; ++s) Un_out[(5 + s) * qs] = bits(rho * Q.Y[s]);
             Q.a = tp_sound_speed<MA, NSP>(*A.gas, Q);
         }
         // check_data (flowstate.d:345-388)
