@@ -922,12 +922,10 @@ __device__ __forceinline__ void tp_update_body(const TPArgs& A, long c)
         const double Tprev = A.Pin[P_T * qs + c];   // the previous temperature: the bracketed solve's guess
         int rc = -1;
         if constexpr (MA::kFast) {
-            // one set of segment sums for the polish, the bracketed solve and
-            // the sound speed (sums formed per evaluation, TPMixY, measured
-            // slower here: 326 -> 357 us for the mixed form, 366 -> 534 us for
-            // the plain one at 128^3, although they allow four waves per SIMD)
-            TPMix X;
-            tp_mix_all<NSP>(*A.gas, Q.Y, X);
+            // the composition for the polish, the bracketed solve and the sound
+            // speed, its segment sums formed at each evaluation from scalar
+            // loads of the gas tables: 111-115 VGPRs, four waves per SIMD
+            const TPMixY<NSP> X = tp_mix_y<NSP>(*A.gas, Q.Y);
             const double Rmix = tp_Rmix<NSP>(*A.gas, Q.Y);
             if constexpr (GUESS || kPolish) {
                 rc = tp_polish_x<MA, NSP>(*A.gas, X, Rmix, Q, e_tgt, GUESS ? Tguess : Tprev, &fe);
