@@ -173,13 +173,13 @@ __device__ __forceinline__ int tp_polish_x(const TPGas& g, const MX& X, double R
 template <class M, int NSP>
 __device__ __forceinline__ int tp_polish(const TPGas& g, TPState<NSP>& Q, double e_tgt, double T0, int* fevals)
 {
-    TPMix X;
-    double Rmix = 0.0;
     if constexpr (M::kFast) {
-        tp_mix_all<NSP>(g, Q.Y, X);
-        Rmix = tp_Rmix<NSP>(g, Q.Y);
+        const TPMixY<NSP> X = tp_mix_y<NSP>(g, Q.Y);
+        return tp_polish_x<M, NSP>(g, X, tp_Rmix<NSP>(g, Q.Y), Q, e_tgt, T0, fevals);
+    } else {
+        TPMix X;   // unused by the reference form
+        return tp_polish_x<M, NSP>(g, X, 0.0, Q, e_tgt, T0, fevals);
     }
-    return tp_polish_x<M, NSP>(g, X, Rmix, Q, e_tgt, T0, fevals);
 }
 
 } // namespace e4

@@ -565,13 +565,15 @@ __device__ __forceinline__ int tp_update_temperature_x(const TPGas& g, const MX&
 template <class M, int NSP>
 __device__ __forceinline__ int tp_update_temperature(const TPGas& g, TPState<NSP>& Q, int* fevals)
 {
-    TPMix X;
-    double Rmix = 0.0;
     if constexpr (M::kFast) {
-        tp_mix_all<NSP>(g, Q.Y, X);
-        Rmix = tp_Rmix<NSP>(g, Q.Y);
+        // the composition with its segment sums formed at each evaluation
+        // (fewer registers than holding all segments' sums through the solve)
+        const TPMixY<NSP> X = tp_mix_y<NSP>(g, Q.Y);
+        return tp_update_temperature_x<M, NSP>(g, X, tp_Rmix<NSP>(g, Q.Y), Q, fevals);
+    } else {
+        TPMix X;   // unused by the reference form
+        return tp_update_temperature_x<M, NSP>(g, X, 0.0, Q, fevals);
     }
-    return tp_update_temperature_x<M, NSP>(g, X, Rmix, Q, fevals);
 }
 
 // Frozen sound speed a = sqrt(Cp/Cv R T) (therm_perf_gas.d:345-385) from the
